@@ -1,0 +1,222 @@
+/*
+ * rsgpu: MI355X (gfx950) sparse Krylov path for the rs reservoir simulator.
+ *
+ * C ABI only: plain pointers, sizes and opaque handles.  Every entry point
+ * returns an int status (RSGPU_OK = 0); reasons 1-3 map to MTJ's
+ * IterativeSolverNotConvergedException.Reason, 10-14 to the runtime
+ * exceptions the MTJ classes throw, negative values to HIP/runtime errors
+ * whose text rsgpu_last_error() returns.
+ *
+ * Pointer conventions: arguments named *_host are host memory (a Java array
+ * pinned by GetPrimitiveArrayCritical, a numpy buffer ...); arguments named
+ * *_dev are device memory on the context's device.  Device-pointer calls
+ * are enqueued on the context's stream; host-pointer calls block until
+ * their result is back in host memory.
+ *
+ * Threading: one handle = one MTJ object; calls on one context must not
+ * overlap (the reference classes are single-threaded too).  Different
+ * contexts may be used from different threads.
+ *
+ * Each declaration cites the reference interface it replaces (paths
+ * relative to the reference root, mtj = mtj/src/no/uib/cipr/matrix).
+ */
+#ifndef RSGPU_H
+#define RSGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+enum rsgpu_status {
+    RSGPU_OK = 0,
+    RSGPU_BREAKDOWN = 1,        /* NotConvergedException.Reason.Breakdown  */
+    RSGPU_DIVERGENCE = 2,       /* Reason.Divergence (incl. NaN residual)  */
+    RSGPU_ITERATIONS = 3,       /* Reason.Iterations                       */
+    RSGPU_ZERO_PIVOT = 10,      /* ILU.java:105-107 RuntimeException        */
+    RSGPU_MISSING_DIAGONAL = 11,/* ILU.java:134-136 RuntimeException        */
+    RSGPU_NOT_IN_PATTERN = 12,  /* CompRowMatrix.java:435 IndexOutOfBounds  */
+    RSGPU_BAD_ARGUMENT = 13,    /* IllegalArgumentException                 */
+    RSGPU_SINGULAR = 14,        /* MatrixSingularException (GMRES dtrtrs)   */
+    RSGPU_HIP_ERROR = -1,
+    RSGPU_OUT_OF_MEMORY = -2,
+    RSGPU_NOT_BUILT = -3
+};
+
+typedef struct rsgpu_ctx rsgpu_ctx;
+typedef struct rsgpu_csr rsgpu_csr;
+typedef struct rsgpu_prec rsgpu_prec;
+typedef struct rsgpu_solver rsgpu_solver;
+
+/* ---- context ------------------------------------------------------------ */
+/* No reference counterpart (the JVM is the context); owns the HIP stream. */
+int rsgpu_ctx_create(int device, rsgpu_ctx **out);
+int rsgpu_ctx_destroy(rsgpu_ctx *ctx);
+/* Use a caller-owned hipStream_t (e.g. torch's current stream); NULL
+ * restores the context's own stream. */
+int rsgpu_ctx_set_stream(rsgpu_ctx *ctx, void *hip_stream);
+int rsgpu_ctx_synchronize(rsgpu_ctx *ctx);
+const char *rsgpu_last_error(rsgpu_ctx *ctx);
+int rsgpu_version(void);
+
+/* ---- CompRowMatrix (mtj/sparse/CompRowMatrix.java) ----------------------- */
+/* CompRowMatrix(int numRows, int numColumns, int[][] nz), :160-190.  rowptr
+ * (n+1) and colind (nnz) describe the pattern; rows need not be sorted (they
+ * are sorted as Arrays.sort does, :188); a column outside [0, ncols) gives
+ * RSGPU_BAD_ARGUMENT.  Values start at zero. */
+int rsgpu_csr_create(rsgpu_ctx *ctx, int n, int ncols, int nnz,
+                     const int32_t *rowptr_host, const int32_t *colind_host,
+                     rsgpu_csr **out);
+int rsgpu_csr_destroy(rsgpu_csr *A);
+/* getRowPointers/getColumnIndices (:261-273): the sorted pattern. */
+int rsgpu_csr_get_pattern(const rsgpu_csr *A, int32_t *rowptr_host,
+                          int32_t *colind_host);
+/* Replaces the data[] array (getData(), :275-277) wholesale. */
+int rsgpu_csr_set_values(rsgpu_csr *A, const double *data_host);
+int rsgpu_csr_set_values_dev(rsgpu_csr *A, const double *data_dev);
+int rsgpu_csr_get_values(const rsgpu_csr *A, double *data_host);
+double *rsgpu_csr_values_dev(rsgpu_csr *A);
+/* CompRowMatrix.zero() (:450-453). */
+int rsgpu_csr_zero(rsgpu_csr *A);
+/* CompRowMatrix.add(row, col, value) for a batch of triples, applied in
+ * array order (:405-437); a triple outside the pattern makes the call
+ * return RSGPU_NOT_IN_PATTERN with nothing applied. */
+int rsgpu_csr_add_batch(rsgpu_csr *A, int count, const int32_t *rows_host,
+                        const int32_t *cols_host, const double *vals_host);
+/* Matrices.zeroRows(A, diagonal, rows...) (mtj/Matrices.java:631-652). */
+int rsgpu_csr_zero_rows(rsgpu_csr *A, double diagonal,
+                        const int32_t *rows_host, int nrows);
+/* mult(x, y) (:301-333): y = A x, with y[i] = 0 when the row sum is 0. */
+int rsgpu_csr_mult(rsgpu_csr *A, const double *x_dev, double *y_dev);
+/* multAdd(alpha, x, y) (:336-353): y += alpha * A x. */
+int rsgpu_csr_multadd(rsgpu_csr *A, double alpha, const double *x_dev,
+                      double *y_dev);
+/* transMult(x, y) (:356-372) and transMultAdd(alpha, x, y) (:375-394):
+ * computed as a gather over a transposed copy (deterministic, no atomics). */
+int rsgpu_csr_transmult(rsgpu_csr *A, const double *x_dev, double *y_dev);
+int rsgpu_csr_transmultadd(rsgpu_csr *A, double alpha, const double *x_dev,
+                           double *y_dev);
+/* Single-precision copy of the values (no reference counterpart; used by
+ * the mixed-precision form) and y = A x in FP32. */
+int rsgpu_csr_sync_f32(rsgpu_csr *A);
+int rsgpu_csr_mult_f32(rsgpu_csr *A, const float *x_dev, float *y_dev);
+/* Host-array conveniences (DenseVector.getData() round trip). */
+int rsgpu_csr_mult_host(rsgpu_csr *A, const double *x_host, double *y_host);
+int rsgpu_csr_transmult_host(rsgpu_csr *A, const double *x_host,
+                             double *y_host);
+
+/* ---- Preconditioner (mtj/sparse/Preconditioner.java:42-63) --------------- */
+enum rsgpu_prec_kind {
+    RSGPU_PREC_IDENTITY = 0,   /* AbstractIterativeSolver.IdentityPreconditioner */
+    RSGPU_PREC_ILU0 = 1,       /* mtj/sparse/ILU.java                            */
+    RSGPU_PREC_BLOCK_ILU0 = 2, /* distributed/BlockDiagonalPreconditioner + ILU  */
+    RSGPU_PREC_DIAGONAL = 3    /* mtj/sparse/DiagonalPreconditioner.java         */
+};
+enum rsgpu_precision { RSGPU_FP64 = 0, RSGPU_FP32 = 1 };
+
+/* nblocks: contiguous equal row ranges (DistRowMatrix ownership,
+ * distributed/DistRowMatrix.java:56-76) for BLOCK_ILU0, ignored otherwise.
+ * precision: arithmetic of the factors and sweeps. */
+int rsgpu_prec_create(rsgpu_ctx *ctx, int kind, int nblocks, int precision,
+                      rsgpu_prec **out);
+int rsgpu_prec_destroy(rsgpu_prec *M);
+/* setMatrix(A) (ILU.java:80-84): copy values, factor.  The pattern analysis
+ * (level schedule) is done on the first call and reused while A's pattern
+ * is the same object.  bad_row_out (may be NULL) receives the reference's
+ * row/entry number for ZERO_PIVOT / MISSING_DIAGONAL. */
+int rsgpu_prec_set_matrix(rsgpu_prec *M, rsgpu_csr *A, int *bad_row_out);
+/* apply(b, x) (ILU.java:64-70) and transApply(b, x) (:72-78). FP64 vectors;
+ * an FP32 preconditioner converts on the way in and out. */
+int rsgpu_prec_apply(rsgpu_prec *M, const double *b_dev, double *x_dev);
+int rsgpu_prec_trans_apply(rsgpu_prec *M, const double *b_dev, double *x_dev);
+int rsgpu_prec_apply_f32(rsgpu_prec *M, const float *b_dev, float *x_dev);
+int rsgpu_prec_apply_host(rsgpu_prec *M, const double *b_host,
+                          double *x_host);
+int rsgpu_prec_trans_apply_host(rsgpu_prec *M, const double *b_host,
+                                double *x_host);
+/* Copy the factored LU values (CSR order of the preconditioner's pattern:
+ * A's pattern for ILU0, its block-diagonal part for BLOCK_ILU0). */
+int rsgpu_prec_get_factors(const rsgpu_prec *M, int32_t *rowptr_host,
+                           int32_t *colind_host, double *lu_host, int *nnz);
+int rsgpu_prec_info(const rsgpu_prec *M, int *nnz, int *nlevels_lower,
+                    int *nlevels_upper);
+
+/* ---- IterativeSolver (mtj/sparse/IterativeSolver.java:43-67) ------------- */
+enum rsgpu_solver_kind {
+    RSGPU_BICGSTAB = 0,  /* mtj/sparse/BiCGstab.java */
+    RSGPU_GMRES = 1,     /* mtj/sparse/GMRES.java    */
+    RSGPU_IR = 2         /* mtj/sparse/IR.java       */
+};
+
+/* DefaultIterationMonitor(maxIter, rtol, atol, dtol)
+ * (DefaultIterationMonitor.java:66-90, :133-156). */
+typedef struct {
+    int kind;
+    int restart;          /* GMRES(m); GMRES.java:84-86 default 30 */
+    int max_iter;
+    double rtol, atol, dtol;
+    /* Mixed precision (no reference counterpart): when mixed != 0 the solve
+     * is an FP64 iterative refinement (IR.java:57-70 form) on the true
+     * residual b - A x, whose correction is an FP32 solve of the given kind
+     * with the FP32 preconditioner, stopped at inner_rtol or inner_max_iter.
+     */
+    int mixed;
+    int inner_max_iter;
+    double inner_rtol;
+    int check_every;      /* host polls the device monitor every k steps */
+} rsgpu_solver_opts;
+
+typedef struct {
+    int iterations;        /* IterationMonitor.iterations()              */
+    double residual;       /* IterationMonitor.residual()                */
+    int status;            /* rsgpu_status                               */
+    int hist_len;          /* residuals reported (OutputIterationReporter)*/
+    int inner_iterations;  /* mixed: total inner FP32 iterations         */
+    int outer_iterations;  /* mixed: FP64 refinement steps               */
+    double true_residual;  /* ||b - A x||_2 in FP64 at exit              */
+    double b_norm;         /* ||b||_2                                    */
+} rsgpu_stats;
+
+void rsgpu_solver_opts_default(rsgpu_solver_opts *opts);
+/* BiCGstab(template)/GMRES(template, restart)/IR(template): n = size. */
+int rsgpu_solver_create(rsgpu_ctx *ctx, const rsgpu_solver_opts *opts, int n,
+                        rsgpu_solver **out);
+int rsgpu_solver_destroy(rsgpu_solver *S);
+/* solve(A, b, x) (BiCGstab.java:65-116, GMRES.java:127-176, IR.java:57-70):
+ * x is the initial guess and the result.  M may be NULL (identity). */
+int rsgpu_solver_solve(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M,
+                       const double *b_dev, double *x_dev, rsgpu_stats *st);
+int rsgpu_solver_solve_host(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M,
+                            const double *b_host, double *x_host,
+                            rsgpu_stats *st);
+/* Residuals the monitor saw, in order (IterationReporter.monitor replay). */
+int rsgpu_solver_history(const rsgpu_solver *S, double *out_host, int cap);
+
+/* ---- MPFA-O assembly (rs geometry/flux/O_MPFA.java, LinearSolver.java) --- */
+/* Cartesian grid, dim 2 or 3 (nz ignored for dim 2), K6_host per cell
+ * (xx,yy,zz,xy,xz,yz).  Computes O_MPFA.calculateTransmissibilities
+ * (O_MPFA.java:54-112) on the device, one interaction region per corner
+ * point, and returns the per-connection lists in the order rs stores them:
+ * md_ptr_host[nconn+1], md_elem_host / md_t_host (caller allocates nconn*18
+ * in 3-D, nconn*6 in 2-D).  check_fail_host counts Connection.check
+ * failures (Connection.java:80-107). */
+int rsgpu_mpfa_o(rsgpu_ctx *ctx, int dim, int nx, int ny, int nz, double dx,
+                 double dy, double dz, const double *K6_host,
+                 double continuity, int32_t *md_ptr_host,
+                 int32_t *md_elem_host, double *md_t_host,
+                 int *check_fail_host);
+/* MPFA-O + LinearSolver.determineSparsityPattern (LinearSolver.java:111-168)
+ * + PotentialCalculator.discretizeFlux (PotentialCalculator.java:319-333)
+ * fully on the device: creates A (pattern with the diagonal, values
+ * assembled in connection order then MD order). */
+int rsgpu_mpfa_o_assemble(rsgpu_ctx *ctx, int dim, int nx, int ny, int nz,
+                          double dx, double dy, double dz,
+                          const double *K6_host, double continuity,
+                          rsgpu_csr **A_out, int *check_fail_host);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RSGPU_H */

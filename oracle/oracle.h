@@ -1,0 +1,177 @@
+/*
+ * CPU oracle: a scalar C restatement of the MTJ sparse Krylov path and of the
+ * rs MPFA-O transmissibility / CSR assembly path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the shipped library (ressim_amd/)
+ * links, loads or calls this code.  It is the checker that every GPU result
+ * is compared against, and the timed "MTJ-equivalent C restatement" CPU
+ * baseline.  It is compiled with -O2 -ffp-contract=off so that every sum is
+ * evaluated in the reference's left-to-right statement order.
+ *
+ * Reference files restated (paths relative to the reference root):
+ *   mtj/src/no/uib/cipr/matrix/sparse/{CompRowMatrix,BiCGstab,GMRES,ILU,
+ *     UnitLowerCompRowMatrix,UpperCompRowMatrix,DefaultIterationMonitor,
+ *     AbstractIterationMonitor,IR,Arrays}.java
+ *   mtj/src/no/uib/cipr/matrix/{DenseVector,GivensRotation,Matrices,
+ *     AbstractTriangDenseMatrix,DenseMatrix}.java
+ *   mtj/src/no/uib/cipr/matrix/distributed/BlockDiagonalPreconditioner.java
+ *   rs/src/no/uib/cipr/rs/geometry/flux/{O_MPFA,MPFA}.java
+ *   rs/src/no/uib/cipr/rs/geometry/Connection.java
+ *   rs/src/no/uib/cipr/rs/meshgen/structured/{CartesianTopology,
+ *     CartesianTopology2D,CartesianTopology3D,NaturalIJKOrdering,HexCell,
+ *     QuadCell,BilinearMap,TrilinearMap,StructuredGeometry2D,
+ *     StructuredGeometry3D}.java
+ *   rs/src/no/uib/cipr/rs/numerics/{LinearSolver,PotentialCalculator}.java
+ *
+ * Dense LAPACK/BLAS calls of the reference (dgesv, dtrtrs, dgemm) are
+ * restated from the published reference-BLAS / LAPACK 3.0 algorithms
+ * (dgetf2 + dgetrs, column-oriented dtrsm, j-l-i dgemm).  Which LAPACK the
+ * reference ran is not fixed by its sources, so bit-level agreement of MPFA
+ * values with a JVM run is unpinned; the MPFA functions are pinned by the
+ * invariants listed in tests/test_oracle_mpfa.py instead.
+ */
+#ifndef RS_ORACLE_H
+#define RS_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes: identical to include/rsgpu.h. */
+enum {
+    OR_OK = 0,
+    OR_BREAKDOWN = 1,
+    OR_DIVERGENCE = 2,
+    OR_ITERATIONS = 3,
+    OR_ZERO_PIVOT = 10,
+    OR_MISSING_DIAGONAL = 11,
+    OR_NOT_IN_PATTERN = 12,
+    OR_BAD_ARGUMENT = 13,
+    OR_SINGULAR = 14
+};
+
+/* ---- CompRowMatrix ------------------------------------------------------ */
+int or_csr_construct(int n, int ncols, const int *rowlen, const int *cols,
+                     int *rowptr, int *colind);
+int or_binary_search(const int *index, int key, int begin, int end);
+int or_csr_add(const int *rowptr, const int *colind, double *data, int row,
+               int col, double value);
+int or_csr_set(const int *rowptr, const int *colind, double *data, int row,
+               int col, double value);
+int or_zero_rows(int n, const int *rowptr, const int *colind, double *data,
+                 double diagonal, const int *rows, int nrows);
+void or_csr_mult(int n, const int *rowptr, const int *colind,
+                 const double *data, const double *x, double *y);
+void or_csr_multadd(int n, const int *rowptr, const int *colind,
+                    const double *data, double alpha, const double *x,
+                    double *y);
+void or_csr_transmult(int n, int ncols, const int *rowptr, const int *colind,
+                      const double *data, const double *x, double *y);
+void or_csr_transmultadd(int n, int ncols, const int *rowptr,
+                         const int *colind, const double *data, double alpha,
+                         const double *x, double *y);
+void or_csr_mult_f32(int n, const int *rowptr, const int *colind,
+                     const float *data, const float *x, float *y);
+
+/* ---- DenseVector BLAS-1 -------------------------------------------------- */
+double or_dot(int n, const double *x, const double *y);
+double or_norm2(int n, const double *x);
+
+/* ---- ILU(0) -------------------------------------------------------------- */
+int or_ilu_factor(int n, const int *rowptr, const int *colind, double *lu,
+                  int *diagind, int *bad_row);
+int or_ilu_factor_f32(int n, const int *rowptr, const int *colind, float *lu,
+                      int *diagind, int *bad_row);
+void or_ilu_apply(int n, const int *rowptr, const int *colind,
+                  const double *lu, const int *diagind, const double *b,
+                  double *x, double *y);
+void or_ilu_apply_f32(int n, const int *rowptr, const int *colind,
+                      const float *lu, const int *diagind, const float *b,
+                      float *x, float *y);
+void or_ilu_transapply(int n, const int *rowptr, const int *colind,
+                       const double *lu, const int *diagind, const double *b,
+                       double *x, double *y);
+int or_block_filter_count(int n, const int *rowptr, const int *colind,
+                          int nblocks, const int *bstart);
+int or_block_filter(int n, const int *rowptr, const int *colind,
+                    const double *data, int nblocks, const int *bstart,
+                    int *browptr, int *bcolind, double *bdata, int *src_index);
+
+/* ---- Krylov solvers ------------------------------------------------------ */
+typedef struct {
+    int max_iter;
+    double rtol, atol, dtol;
+    int restart;        /* GMRES(m) */
+    /* preconditioner: 0 identity, 1 ILU(0), 2 block-Jacobi ILU(0) */
+    int prec_kind;
+    int nblocks;        /* for block-Jacobi, contiguous equal row ranges */
+} or_opts;
+
+typedef struct {
+    int iterations;
+    double residual;    /* last monitored residual */
+    int status;
+    int bad_row;
+    int hist_len;
+} or_stats;
+
+int or_bicgstab(int n, const int *rowptr, const int *colind,
+                const double *data, const double *b, double *x,
+                const or_opts *opts, or_stats *stats, double *hist,
+                int hist_cap);
+int or_gmres(int n, const int *rowptr, const int *colind, const double *data,
+             const double *b, double *x, const or_opts *opts, or_stats *stats,
+             double *hist, int hist_cap);
+int or_ir(int n, const int *rowptr, const int *colind, const double *data,
+          const double *b, double *x, const or_opts *opts, or_stats *stats,
+          double *hist, int hist_cap);
+int or_block_starts(int n, int nblocks, int *bstart);
+
+/* ---- Dense LU (LAPACK 3.0 dgetf2/dgetrs restatement) --------------------- */
+int or_dgesv(int n, int nrhs, double *a, int lda, int *ipiv, double *b,
+             int ldb);
+
+/* ---- Cartesian mesh + MPFA-O -------------------------------------------- */
+/* dim 2: nz ignored (0).  K6 holds per cell (xx,yy,zz,xy,xz,yz).
+ * Outputs per connection c: md_ptr[c]..md_ptr[c+1] in md_elem / md_t;
+ * caller allocates md_ptr[nconn+1] and nconn*18 (3D) / nconn*6 (2D) slots.
+ * here/there: element index on either side of each connection. */
+int or_mesh_counts(int dim, int nx, int ny, int nz, int *ncells, int *nconn,
+                   int *npoints);
+int or_mesh_connections(int dim, int nx, int ny, int nz, int *here,
+                        int *there);
+int or_mpfa_o(int dim, int nx, int ny, int nz, double dx, double dy,
+              double dz, const double *K6, double continuity, int *md_ptr,
+              int *md_elem, double *md_t, int *check_fail);
+int or_mpfa_corner(int dim, int nx, int ny, int nz, double dx, double dy,
+                   double dz, const double *K6, double continuity, int point,
+                   int *nc, int *ne, double *A, double *B, double *C,
+                   double *D, double *T, int *elems);
+int or_element_centers(int dim, int nx, int ny, int nz, double dx, double dy,
+                       double dz, double *centers);
+int or_interface_geometry(int dim, int nx, int ny, int nz, double dx,
+                          double dy, double dz, double *normal_area,
+                          double *center);
+
+/* rs LinearSolver.determineSparsityPattern (with_diag=1) and
+ * PotentialCalculator.determineSparsityPattern (with_diag=0). */
+int or_pattern_count(int n, int nconn, const int *here, const int *there,
+                     const int *md_ptr, const int *md_elem, int with_diag,
+                     int *rowlen);
+int or_pattern(int n, int nconn, const int *here, const int *there,
+               const int *md_ptr, const int *md_elem, int with_diag,
+               int *rowptr, int *colind);
+/* PotentialCalculator.discretizeFlux: A[i,k] += t, A[j,k] -= t. */
+int or_assemble_flux(int n, int nconn, const int *here, const int *there,
+                     const int *md_ptr, const int *md_elem, const double *md_t,
+                     const int *rowptr, const int *colind, double *data);
+/* PotentialCalculator.calculateFlux: F_c = sum_k t_k p_k. */
+void or_connection_flux(int nconn, const int *md_ptr, const int *md_elem,
+                        const double *md_t, const double *p, double *flux);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

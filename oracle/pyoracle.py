@@ -1,0 +1,368 @@
+"""ctypes bindings of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke``
+and by the ``cpu_baseline`` leg of ``bench.py`` -- never by the shipped
+package.  The C sources restate MTJ and rs (see oracle/oracle.h for the
+reference file list and the statement-order contract).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+OK, BREAKDOWN, DIVERGENCE, ITERATIONS = 0, 1, 2, 3
+ZERO_PIVOT, MISSING_DIAGONAL, NOT_IN_PATTERN, BAD_ARGUMENT, SINGULAR = 10, 11, 12, 13, 14
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+
+
+class Opts(C.Structure):
+    _fields_ = [("max_iter", C.c_int), ("rtol", C.c_double), ("atol", C.c_double),
+                ("dtol", C.c_double), ("restart", C.c_int), ("prec_kind", C.c_int),
+                ("nblocks", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("residual", C.c_double), ("status", C.c_int),
+                ("bad_row", C.c_int), ("hist_len", C.c_int)]
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.or_csr_construct.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _i32p, _i32p]
+        L.or_csr_add.argtypes = [_i32p, _i32p, _f64p, C.c_int, C.c_int, C.c_double]
+        L.or_zero_rows.argtypes = [C.c_int, _i32p, _i32p, _f64p, C.c_double, _i32p, C.c_int]
+        L.or_csr_mult.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p]
+        L.or_csr_mult_f32.argtypes = [C.c_int, _i32p, _i32p, _f32p, _f32p, _f32p]
+        L.or_csr_multadd.argtypes = [C.c_int, _i32p, _i32p, _f64p, C.c_double, _f64p, _f64p]
+        L.or_csr_transmult.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p]
+        L.or_csr_transmultadd.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _f64p, C.c_double,
+                                          _f64p, _f64p]
+        L.or_dot.argtypes = [C.c_int, _f64p, _f64p]
+        L.or_dot.restype = C.c_double
+        L.or_norm2.argtypes = [C.c_int, _f64p]
+        L.or_norm2.restype = C.c_double
+        L.or_ilu_factor.argtypes = [C.c_int, _i32p, _i32p, _f64p, _i32p, C.POINTER(C.c_int)]
+        L.or_ilu_factor_f32.argtypes = [C.c_int, _i32p, _i32p, _f32p, _i32p, C.POINTER(C.c_int)]
+        L.or_ilu_apply.argtypes = [C.c_int, _i32p, _i32p, _f64p, _i32p, _f64p, _f64p, _f64p]
+        L.or_ilu_apply_f32.argtypes = [C.c_int, _i32p, _i32p, _f32p, _i32p, _f32p, _f32p, _f32p]
+        L.or_ilu_transapply.argtypes = [C.c_int, _i32p, _i32p, _f64p, _i32p, _f64p, _f64p,
+                                        _f64p]
+        L.or_block_filter_count.argtypes = [C.c_int, _i32p, _i32p, C.c_int, _i32p]
+        L.or_block_filter.argtypes = [C.c_int, _i32p, _i32p, _f64p, C.c_int, _i32p, _i32p,
+                                      _i32p, _f64p, _i32p]
+        L.or_block_starts.argtypes = [C.c_int, C.c_int, _i32p]
+        for f in (L.or_bicgstab, L.or_gmres, L.or_ir):
+            f.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.POINTER(Opts),
+                          C.POINTER(Stats), _f64p, C.c_int]
+        L.or_dgesv.argtypes = [C.c_int, C.c_int, _f64p, C.c_int, _i32p, _f64p, C.c_int]
+        L.or_mesh_counts.argtypes = [C.c_int] * 4 + [C.POINTER(C.c_int)] * 3
+        L.or_mesh_connections.argtypes = [C.c_int] * 4 + [_i32p, _i32p]
+        L.or_mpfa_o.argtypes = [C.c_int] * 4 + [C.c_double] * 3 + [_f64p, C.c_double, _i32p,
+                                                                    _i32p, _f64p,
+                                                                    C.POINTER(C.c_int)]
+        L.or_mpfa_corner.argtypes = ([C.c_int] * 4 + [C.c_double] * 3 +
+                                     [_f64p, C.c_double, C.c_int, C.POINTER(C.c_int),
+                                      C.POINTER(C.c_int)] + [_f64p] * 5 + [_i32p])
+        L.or_element_centers.argtypes = [C.c_int] * 4 + [C.c_double] * 3 + [_f64p]
+        L.or_interface_geometry.argtypes = [C.c_int] * 4 + [C.c_double] * 3 + [_f64p, _f64p]
+        L.or_pattern_count.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _i32p, _i32p, C.c_int,
+                                       _i32p]
+        L.or_pattern.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _i32p, _i32p, C.c_int, _i32p,
+                                 _i32p]
+        L.or_assemble_flux.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _i32p, _i32p, _f64p,
+                                       _i32p, _i32p, _f64p]
+        L.or_connection_flux.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p]
+        _lib = L
+    return _lib
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ---------------------------------------------------------------- CSR
+def csr_construct(n, ncols, rows):
+    """CompRowMatrix(int,int,int[][]): rows = list of column lists."""
+    rowlen = _i32([len(r) for r in rows])
+    cols = _i32(np.concatenate([np.asarray(r, dtype=np.int32) for r in rows])
+                if rows and sum(len(r) for r in rows) else np.zeros(0, np.int32))
+    rowptr = np.zeros(n + 1, np.int32)
+    colind = np.zeros(max(int(rowlen.sum()), 1), np.int32)
+    st = lib().or_csr_construct(n, ncols, rowlen, cols if cols.size else np.zeros(1, np.int32),
+                                rowptr, colind)
+    return st, rowptr, colind[: int(rowlen.sum())]
+
+
+def csr_add(rowptr, colind, data, row, col, value):
+    return lib().or_csr_add(rowptr, colind, data, row, col, value)
+
+
+def zero_rows(rowptr, colind, data, diagonal, rows):
+    rows = _i32(rows)
+    return lib().or_zero_rows(len(rowptr) - 1, rowptr, colind, data, diagonal,
+                              rows if rows.size else np.zeros(1, np.int32), rows.size)
+
+
+def mult(rowptr, colind, data, x):
+    n = len(rowptr) - 1
+    y = np.empty(n)
+    lib().or_csr_mult(n, _i32(rowptr), _i32(colind), _f64(data), _f64(x), y)
+    return y
+
+
+def mult_f32(rowptr, colind, data, x):
+    n = len(rowptr) - 1
+    y = np.empty(n, np.float32)
+    lib().or_csr_mult_f32(n, _i32(rowptr), _i32(colind),
+                          np.ascontiguousarray(data, np.float32),
+                          np.ascontiguousarray(x, np.float32), y)
+    return y
+
+
+def multadd(rowptr, colind, data, alpha, x, y):
+    y = _f64(y).copy()
+    lib().or_csr_multadd(len(rowptr) - 1, _i32(rowptr), _i32(colind), _f64(data), alpha,
+                         _f64(x), y)
+    return y
+
+
+def transmult(rowptr, colind, data, x, ncols=None):
+    n = len(rowptr) - 1
+    ncols = n if ncols is None else ncols
+    y = np.empty(ncols)
+    lib().or_csr_transmult(n, ncols, _i32(rowptr), _i32(colind), _f64(data), _f64(x), y)
+    return y
+
+
+def transmultadd(rowptr, colind, data, alpha, x, y):
+    n = len(rowptr) - 1
+    y = _f64(y).copy()
+    lib().or_csr_transmultadd(n, y.size, _i32(rowptr), _i32(colind), _f64(data), alpha,
+                              _f64(x), y)
+    return y
+
+
+def dot(x, y):
+    return lib().or_dot(len(x), _f64(x), _f64(y))
+
+
+def norm2(x):
+    return lib().or_norm2(len(x), _f64(x))
+
+
+# ---------------------------------------------------------------- ILU
+def ilu_factor(rowptr, colind, data):
+    n = len(rowptr) - 1
+    lu = _f64(data).copy()
+    diag = np.zeros(max(n, 1), np.int32)
+    bad = C.c_int(-1)
+    st = lib().or_ilu_factor(n, _i32(rowptr), _i32(colind), lu, diag, C.byref(bad))
+    return st, lu, diag[:n], bad.value
+
+
+def ilu_factor_f32(rowptr, colind, data):
+    n = len(rowptr) - 1
+    lu = np.ascontiguousarray(data, np.float32).copy()
+    diag = np.zeros(max(n, 1), np.int32)
+    bad = C.c_int(-1)
+    st = lib().or_ilu_factor_f32(n, _i32(rowptr), _i32(colind), lu, diag, C.byref(bad))
+    return st, lu, diag[:n], bad.value
+
+
+def ilu_apply(rowptr, colind, lu, diag, b):
+    n = len(rowptr) - 1
+    x = np.empty(n)
+    y = np.empty(n)
+    lib().or_ilu_apply(n, _i32(rowptr), _i32(colind), _f64(lu), _i32(diag), _f64(b), x, y)
+    return x
+
+
+def ilu_apply_f32(rowptr, colind, lu, diag, b):
+    n = len(rowptr) - 1
+    x = np.empty(n, np.float32)
+    y = np.empty(n, np.float32)
+    lib().or_ilu_apply_f32(n, _i32(rowptr), _i32(colind), np.ascontiguousarray(lu, np.float32),
+                           _i32(diag), np.ascontiguousarray(b, np.float32), x, y)
+    return x
+
+
+def ilu_transapply(rowptr, colind, lu, diag, b):
+    n = len(rowptr) - 1
+    x = np.empty(n)
+    y = np.empty(n)
+    lib().or_ilu_transapply(n, _i32(rowptr), _i32(colind), _f64(lu), _i32(diag), _f64(b), x, y)
+    return x
+
+
+def block_starts(n, nblocks):
+    bs = np.zeros(nblocks + 1, np.int32)
+    st = lib().or_block_starts(n, nblocks, bs)
+    if st:
+        raise ValueError("bad block count")
+    return bs
+
+
+def block_filter(rowptr, colind, data, nblocks):
+    n = len(rowptr) - 1
+    bs = block_starts(n, nblocks)
+    cnt = lib().or_block_filter_count(n, _i32(rowptr), _i32(colind), nblocks, bs)
+    browptr = np.zeros(n + 1, np.int32)
+    bcol = np.zeros(max(cnt, 1), np.int32)
+    bdat = np.zeros(max(cnt, 1))
+    src = np.zeros(max(cnt, 1), np.int32)
+    lib().or_block_filter(n, _i32(rowptr), _i32(colind), _f64(data), nblocks, bs, browptr,
+                          bcol, bdat, src)
+    return browptr, bcol[:cnt], bdat[:cnt], src[:cnt]
+
+
+# ---------------------------------------------------------------- solvers
+PREC = {"none": 0, "identity": 0, "ilu": 1, "block_ilu": 2}
+
+
+def solve(kind, rowptr, colind, data, b, x0=None, *, max_iter=100000, rtol=1e-5, atol=1e-50,
+          dtol=1e5, restart=30, prec="ilu", nblocks=1, hist_cap=None):
+    """Run or_bicgstab / or_gmres / or_ir.  Returns (x, Stats, history)."""
+    n = len(rowptr) - 1
+    x = np.zeros(n) if x0 is None else _f64(x0).copy()
+    o = Opts(max_iter, rtol, atol, dtol, restart, PREC[prec], nblocks)
+    s = Stats()
+    cap = (max_iter + 2) * 2 if hist_cap is None else hist_cap
+    cap = max(1, min(cap, 10_000_000))
+    hist = np.zeros(cap)
+    fn = {"bicgstab": lib().or_bicgstab, "gmres": lib().or_gmres, "ir": lib().or_ir}[kind]
+    fn(n, _i32(rowptr), _i32(colind), _f64(data), _f64(b), x, C.byref(o), C.byref(s), hist, cap)
+    return x, s, hist[: min(s.hist_len, cap)]
+
+
+def dgesv(a, b):
+    a = np.asfortranarray(a, dtype=np.float64)
+    b = np.asfortranarray(b, dtype=np.float64)
+    n = a.shape[0]
+    nrhs = 1 if b.ndim == 1 else b.shape[1]
+    af = np.array(a.T, dtype=np.float64, order="C").reshape(-1)  # column-major copy
+    bf = np.array(b.reshape(n, nrhs).T, dtype=np.float64, order="C").reshape(-1)
+    piv = np.zeros(n, np.int32)
+    st = lib().or_dgesv(n, nrhs, af, n, piv, bf, n)
+    return st, bf.reshape(nrhs, n).T
+
+
+# ---------------------------------------------------------------- mesh / MPFA
+def mesh_counts(dim, nx, ny, nz=0):
+    a, b, c = C.c_int(), C.c_int(), C.c_int()
+    st = lib().or_mesh_counts(dim, nx, ny, nz, C.byref(a), C.byref(b), C.byref(c))
+    if st:
+        raise ValueError("bad mesh")
+    return a.value, b.value, c.value
+
+
+def mesh_connections(dim, nx, ny, nz=0):
+    _, nconn, _ = mesh_counts(dim, nx, ny, nz)
+    here = np.zeros(max(nconn, 1), np.int32)
+    there = np.zeros(max(nconn, 1), np.int32)
+    lib().or_mesh_connections(dim, nx, ny, nz, here, there)
+    return here[:nconn], there[:nconn]
+
+
+def mpfa_o(dim, nx, ny, nz, dx, dy, dz, K6, continuity=1.0):
+    """O_MPFA transmissibilities: (md_ptr, md_elem, md_t, n_check_failures)."""
+    _, nconn, _ = mesh_counts(dim, nx, ny, nz)
+    cap = 18 if dim == 3 else 6
+    md_ptr = np.zeros(nconn + 1, np.int32)
+    md_elem = np.zeros(max(nconn * cap, 1), np.int32)
+    md_t = np.zeros(max(nconn * cap, 1))
+    fails = C.c_int(0)
+    st = lib().or_mpfa_o(dim, nx, ny, nz, dx, dy, dz, _f64(K6).reshape(-1), continuity,
+                         md_ptr, md_elem, md_t, C.byref(fails))
+    if st:
+        raise RuntimeError(f"or_mpfa_o failed with status {st}")
+    m = int(md_ptr[-1])
+    return md_ptr, md_elem[:m].copy(), md_t[:m].copy(), fails.value
+
+
+def mpfa_corner(dim, nx, ny, nz, dx, dy, dz, K6, point, continuity=1.0):
+    A = np.zeros(144)
+    B = np.zeros(96)
+    Cm = np.zeros(144)
+    D = np.zeros(96)
+    T = np.zeros(96)
+    el = np.zeros(8, np.int32)
+    nc, ne = C.c_int(), C.c_int()
+    st = lib().or_mpfa_corner(dim, nx, ny, nz, dx, dy, dz, _f64(K6).reshape(-1), continuity,
+                              point, C.byref(nc), C.byref(ne), A, B, Cm, D, T, el)
+    if st:
+        raise RuntimeError(f"or_mpfa_corner failed with status {st}")
+    c, e = nc.value, ne.value
+
+    def cm(a, r, k):
+        return a[: r * k].reshape(k, r).T.copy()
+    return dict(A=cm(A, c, c), B=cm(B, c, e), C=cm(Cm, c, c), D=cm(D, c, e), T=cm(T, c, e),
+                elements=el[:e].copy())
+
+
+def element_centers(dim, nx, ny, nz, dx, dy, dz):
+    n, _, _ = mesh_counts(dim, nx, ny, nz)
+    out = np.zeros(3 * n)
+    lib().or_element_centers(dim, nx, ny, nz, dx, dy, dz, out)
+    return out.reshape(n, 3)
+
+
+def interface_geometry(dim, nx, ny, nz, dx, dy, dz):
+    _, nconn, _ = mesh_counts(dim, nx, ny, nz)
+    na = np.zeros(4 * max(nconn, 1))
+    cc = np.zeros(3 * max(nconn, 1))
+    lib().or_interface_geometry(dim, nx, ny, nz, dx, dy, dz, na, cc)
+    return na[: 4 * nconn].reshape(nconn, 4), cc[: 3 * nconn].reshape(nconn, 3)
+
+
+def pattern(n, here, there, md_ptr, md_elem, with_diag=True):
+    nconn = len(here)
+    rowlen = np.zeros(n, np.int32)
+    args = (n, nconn, _i32(here), _i32(there), _i32(md_ptr), _i32(md_elem), int(with_diag))
+    st = lib().or_pattern_count(*args, rowlen)
+    if st:
+        raise RuntimeError("pattern overflow")
+    rowptr = np.zeros(n + 1, np.int32)
+    colind = np.zeros(max(int(rowlen.sum()), 1), np.int32)
+    lib().or_pattern(*args, rowptr, colind)
+    return rowptr, colind[: int(rowlen.sum())]
+
+
+def assemble_flux(n, here, there, md_ptr, md_elem, md_t, rowptr, colind):
+    data = np.zeros(len(colind))
+    st = lib().or_assemble_flux(n, len(here), _i32(here), _i32(there), _i32(md_ptr),
+                                _i32(md_elem), _f64(md_t), _i32(rowptr), _i32(colind), data)
+    if st:
+        raise RuntimeError(f"assembly failed with status {st}")
+    return data
+
+
+def connection_flux(md_ptr, md_elem, md_t, p):
+    nconn = len(md_ptr) - 1
+    f = np.zeros(max(nconn, 1))
+    lib().or_connection_flux(nconn, _i32(md_ptr), _i32(md_elem), _f64(md_t), _f64(p), f)
+    return f[:nconn]

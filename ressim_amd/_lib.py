@@ -1,0 +1,121 @@
+"""ctypes binding of librsgpu.so (the C ABI declared in include/rsgpu.h).
+
+The library is loaded from this package directory only.  There is no CPU
+fallback: if the HIP library is missing or cannot be loaded, every entry
+point raises ``RuntimeError``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librsgpu.so")
+
+OK = 0
+BREAKDOWN, DIVERGENCE, ITERATIONS = 1, 2, 3
+ZERO_PIVOT, MISSING_DIAGONAL, NOT_IN_PATTERN, BAD_ARGUMENT, SINGULAR = 10, 11, 12, 13, 14
+HIP_ERROR, OUT_OF_MEMORY, NOT_BUILT = -1, -2, -3
+
+PREC_IDENTITY, PREC_ILU0, PREC_BLOCK_ILU0, PREC_DIAGONAL = 0, 1, 2, 3
+FP64, FP32 = 0, 1
+BICGSTAB, GMRES, IR = 0, 1, 2
+
+
+class SolverOpts(C.Structure):
+    _fields_ = [("kind", C.c_int), ("restart", C.c_int), ("max_iter", C.c_int),
+                ("rtol", C.c_double), ("atol", C.c_double), ("dtol", C.c_double),
+                ("mixed", C.c_int), ("inner_max_iter", C.c_int),
+                ("inner_rtol", C.c_double), ("check_every", C.c_int)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("residual", C.c_double), ("status", C.c_int),
+                ("hist_len", C.c_int), ("inner_iterations", C.c_int),
+                ("outer_iterations", C.c_int), ("true_residual", C.c_double),
+                ("b_norm", C.c_double)]
+
+
+_vp = C.c_void_p
+_ip = C.c_void_p      # int32 pointers passed as addresses
+_dp = C.c_void_p
+_h = C.c_void_p
+_hp = C.POINTER(C.c_void_p)
+
+# name: (restype, argtypes)
+SIGNATURES = {
+    "rsgpu_ctx_create": (C.c_int, [C.c_int, _hp]),
+    "rsgpu_ctx_destroy": (C.c_int, [_h]),
+    "rsgpu_ctx_set_stream": (C.c_int, [_h, _vp]),
+    "rsgpu_ctx_synchronize": (C.c_int, [_h]),
+    "rsgpu_last_error": (C.c_char_p, [_h]),
+    "rsgpu_version": (C.c_int, []),
+    "rsgpu_csr_create": (C.c_int, [_h, C.c_int, C.c_int, C.c_int, _ip, _ip, _hp]),
+    "rsgpu_csr_destroy": (C.c_int, [_h]),
+    "rsgpu_csr_get_pattern": (C.c_int, [_h, _ip, _ip]),
+    "rsgpu_csr_set_values": (C.c_int, [_h, _dp]),
+    "rsgpu_csr_set_values_dev": (C.c_int, [_h, _dp]),
+    "rsgpu_csr_get_values": (C.c_int, [_h, _dp]),
+    "rsgpu_csr_values_dev": (C.c_void_p, [_h]),
+    "rsgpu_csr_zero": (C.c_int, [_h]),
+    "rsgpu_csr_add_batch": (C.c_int, [_h, C.c_int, _ip, _ip, _dp]),
+    "rsgpu_csr_zero_rows": (C.c_int, [_h, C.c_double, _ip, C.c_int]),
+    "rsgpu_csr_mult": (C.c_int, [_h, _dp, _dp]),
+    "rsgpu_csr_multadd": (C.c_int, [_h, C.c_double, _dp, _dp]),
+    "rsgpu_csr_transmult": (C.c_int, [_h, _dp, _dp]),
+    "rsgpu_csr_transmultadd": (C.c_int, [_h, C.c_double, _dp, _dp]),
+    "rsgpu_csr_sync_f32": (C.c_int, [_h]),
+    "rsgpu_csr_mult_f32": (C.c_int, [_h, _vp, _vp]),
+    "rsgpu_csr_mult_host": (C.c_int, [_h, _dp, _dp]),
+    "rsgpu_csr_transmult_host": (C.c_int, [_h, _dp, _dp]),
+    "rsgpu_prec_create": (C.c_int, [_h, C.c_int, C.c_int, C.c_int, _hp]),
+    "rsgpu_prec_destroy": (C.c_int, [_h]),
+    "rsgpu_prec_set_matrix": (C.c_int, [_h, _h, C.POINTER(C.c_int)]),
+    "rsgpu_prec_apply": (C.c_int, [_h, _dp, _dp]),
+    "rsgpu_prec_trans_apply": (C.c_int, [_h, _dp, _dp]),
+    "rsgpu_prec_apply_f32": (C.c_int, [_h, _vp, _vp]),
+    "rsgpu_prec_apply_host": (C.c_int, [_h, _dp, _dp]),
+    "rsgpu_prec_trans_apply_host": (C.c_int, [_h, _dp, _dp]),
+    "rsgpu_prec_get_factors": (C.c_int, [_h, _ip, _ip, _dp, C.POINTER(C.c_int)]),
+    "rsgpu_prec_info": (C.c_int, [_h, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                  C.POINTER(C.c_int)]),
+    "rsgpu_solver_opts_default": (None, [C.POINTER(SolverOpts)]),
+    "rsgpu_solver_create": (C.c_int, [_h, C.POINTER(SolverOpts), C.c_int, _hp]),
+    "rsgpu_solver_destroy": (C.c_int, [_h]),
+    "rsgpu_solver_solve": (C.c_int, [_h, _h, _h, _dp, _dp, C.POINTER(Stats)]),
+    "rsgpu_solver_solve_host": (C.c_int, [_h, _h, _h, _dp, _dp, C.POINTER(Stats)]),
+    "rsgpu_solver_history": (C.c_int, [_h, _dp, C.c_int]),
+    "rsgpu_mpfa_o": (C.c_int, [_h, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                               C.c_double, C.c_double, _dp, C.c_double, _ip, _ip, _dp,
+                               C.POINTER(C.c_int)]),
+    "rsgpu_mpfa_o_assemble": (C.c_int, [_h, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                        C.c_double, C.c_double, _dp, C.c_double, _hp,
+                                        C.POINTER(C.c_int)]),
+}
+
+_lib = None
+
+
+def load(path: str | None = None):
+    """Load librsgpu.so; raises RuntimeError when it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"librsgpu.so not built ({p}); run python -m ressim_amd.build")
+    L = C.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def declared_symbols(header: str) -> list[str]:
+    """Function names declared in a C header (for the export check)."""
+    import re
+    txt = open(header).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(rsgpu_[a-z0-9_]+)\s*\(", txt)))

@@ -1,0 +1,583 @@
+// C-ABI entry points: context, CompRowMatrix, Preconditioner.
+// Each function maps C++ exceptions (RsError) to a status code and keeps the
+// message for rsgpu_last_error().
+#include <algorithm>
+#include <atomic>
+
+#include "rsgpu_impl.hpp"
+
+static std::atomic<uint64_t> g_pattern_id{1};
+uint64_t next_pattern_id() { return g_pattern_id++; }
+
+static int fail(rsgpu_ctx *ctx, const RsError &e)
+{
+    if (ctx)
+        ctx->err = e.msg;
+    return e.code;
+}
+
+#define RS_TRY(ctx) try {
+#define RS_CATCH(ctx)                                                         \
+    }                                                                         \
+    catch (const RsError &e) { return fail(ctx, e); }                         \
+    catch (const std::bad_alloc &) {                                          \
+        return fail(ctx, RsError{RSGPU_OUT_OF_MEMORY, "host out of memory"}); \
+    }
+
+extern "C" int rsgpu_version(void) { return 1; }
+
+// ---------------------------------------------------------------- context
+extern "C" int rsgpu_ctx_create(int device, rsgpu_ctx **out)
+{
+    if (!out)
+        return RSGPU_BAD_ARGUMENT;
+    rsgpu_ctx *c = new rsgpu_ctx();
+    c->device = device;
+    RS_TRY(c)
+    RS_HIP(hipSetDevice(device));
+    RS_HIP(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
+    c->stream = c->own;
+    *out = c;
+    return RSGPU_OK;
+    RS_CATCH(c)
+}
+
+extern "C" int rsgpu_ctx_destroy(rsgpu_ctx *c)
+{
+    if (!c)
+        return RSGPU_OK;
+    if (c->own) {
+        (void)hipStreamSynchronize(c->own);
+        (void)hipStreamDestroy(c->own);
+    }
+    delete c;
+    return RSGPU_OK;
+}
+
+extern "C" int rsgpu_ctx_set_stream(rsgpu_ctx *c, void *stream)
+{
+    if (!c)
+        return RSGPU_BAD_ARGUMENT;
+    c->stream = stream ? (hipStream_t)stream : c->own;
+    return RSGPU_OK;
+}
+
+extern "C" int rsgpu_ctx_synchronize(rsgpu_ctx *c)
+{
+    if (!c)
+        return RSGPU_BAD_ARGUMENT;
+    RS_TRY(c)
+    RS_HIP(hipStreamSynchronize(c->stream));
+    return RSGPU_OK;
+    RS_CATCH(c)
+}
+
+extern "C" const char *rsgpu_last_error(rsgpu_ctx *c)
+{
+    return c ? c->err.c_str() : "no context";
+}
+
+// ---------------------------------------------------------------- CSR
+extern "C" int rsgpu_csr_create(rsgpu_ctx *ctx, int n, int ncols, int nnz,
+                                const int32_t *rowptr, const int32_t *colind,
+                                rsgpu_csr **out)
+{
+    if (!ctx || !out || n < 0 || ncols < 0 || nnz < 0 || !rowptr ||
+        (nnz > 0 && !colind))
+        return RSGPU_BAD_ARGUMENT;
+    rsgpu_csr *A = new rsgpu_csr();
+    RS_TRY(ctx)
+    RS_HIP(hipSetDevice(ctx->device));
+    A->ctx = ctx;
+    A->n = n;
+    A->ncols = ncols;
+    A->nnz = nnz;
+    if (rowptr[0] != 0 || rowptr[n] != nnz)
+        throw RsError{RSGPU_BAD_ARGUMENT, "rowptr must start at 0 and end at nnz"};
+    A->h_rowptr.assign(rowptr, rowptr + n + 1);
+    A->h_colind.assign(colind, colind + nnz);
+    for (int i = 0; i < n; ++i) {
+        if (rowptr[i + 1] < rowptr[i])
+            throw RsError{RSGPU_BAD_ARGUMENT, "rowptr must be non-decreasing"};
+        for (int j = rowptr[i]; j < rowptr[i + 1]; ++j)
+            if (colind[j] < 0 || colind[j] >= ncols)
+                throw RsError{RSGPU_BAD_ARGUMENT,
+                              "nz[" + std::to_string(i) + "][" +
+                                  std::to_string(j - rowptr[i]) + "]=" +
+                                  std::to_string(colind[j]) +
+                                  ", which is not a valid column index"};
+        // Arrays.sort of each row (CompRowMatrix.java:188)
+        std::sort(A->h_colind.begin() + rowptr[i],
+                  A->h_colind.begin() + rowptr[i + 1]);
+    }
+    hipStream_t s = ctx->stream;
+    A->rowptr.upload(A->h_rowptr, s);
+    A->colind.upload(A->h_colind, s);
+    A->val.alloc(std::max(nnz, 1));
+    A->val.zero(s);
+    spmv_build_plan(A->h_rowptr, n, A->plan, s);
+    A->pattern_id = next_pattern_id();
+    RS_HIP(hipStreamSynchronize(s));
+    *out = A;
+    return RSGPU_OK;
+    RS_CATCH(ctx)
+}
+
+extern "C" int rsgpu_csr_destroy(rsgpu_csr *A)
+{
+    if (A) {
+        if (A->ctx)
+            (void)hipStreamSynchronize(A->ctx->stream);
+        delete A;
+    }
+    return RSGPU_OK;
+}
+
+extern "C" int rsgpu_csr_get_pattern(const rsgpu_csr *A, int32_t *rowptr,
+                                     int32_t *colind)
+{
+    if (!A)
+        return RSGPU_BAD_ARGUMENT;
+    if (rowptr)
+        std::copy(A->h_rowptr.begin(), A->h_rowptr.end(), rowptr);
+    if (colind)
+        std::copy(A->h_colind.begin(), A->h_colind.end(), colind);
+    return RSGPU_OK;
+}
+
+extern "C" int rsgpu_csr_set_values(rsgpu_csr *A, const double *data)
+{
+    if (!A || (!data && A->nnz))
+        return RSGPU_BAD_ARGUMENT;
+    RS_TRY(A->ctx)
+    if (A->nnz) {
+        RS_HIP(hipMemcpyAsync(A->val.p, data, sizeof(double) * A->nnz,
+                              hipMemcpyHostToDevice, A->ctx->stream));
+        RS_HIP(hipStreamSynchronize(A->ctx->stream));
+    }
+    return RSGPU_OK;
+    RS_CATCH(A->ctx)
+}
+
+extern "C" int rsgpu_csr_set_values_dev(rsgpu_csr *A, const double *data)
+{
+    if (!A || (!data && A->nnz))
+        return RSGPU_BAD_ARGUMENT;
+    RS_TRY(A->ctx)
+    if (A->nnz)
+        RS_HIP(hipMemcpyAsync(A->val.p, data, sizeof(double) * A->nnz,
+                              hipMemcpyDeviceToDevice, A->ctx->stream));
+    return RSGPU_OK;
+    RS_CATCH(A->ctx)
+}
+
+extern "C" int rsgpu_csr_get_values(const rsgpu_csr *A, double *data)
+{
+    if (!A || (!data && A->nnz))
+        return RSGPU_BAD_ARGUMENT;
+    RS_TRY(A->ctx)
+    if (A->nnz) {
+        RS_HIP(hipMemcpyAsync(data, A->val.p, sizeof(double) * A->nnz,
+                              hipMemcpyDeviceToHost, A->ctx->stream));
+        RS_HIP(hipStreamSynchronize(A->ctx->stream));
+    }
+    return RSGPU_OK;
+    RS_CATCH(A->ctx)
+}
+
+extern "C" double *rsgpu_csr_values_dev(rsgpu_csr *A) { return A ? A->val.p : nullptr; }
+
+extern "C" int rsgpu_csr_zero(rsgpu_csr *A)
+{
+    if (!A)
+        return RSGPU_BAD_ARGUMENT;
+    RS_TRY(A->ctx)
+    A->val.zero(A->ctx->stream);
+    return RSGPU_OK;
+    RS_CATCH(A->ctx)
+}
+
+// CompRowMatrix.add for a batch, in array order.  Index lookup on the host
+// pattern (binary search as getIndex), values accumulated on the device by
+// one ordered pass per row so that repeated (row,col) pairs add in order.
+__global__ void k_add_ordered(const int *__restrict__ rp, const int *__restrict__ pos,
+                              const double *__restrict__ v, double *val, int nrows_touched)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrows_touched)
+        return;
+    for (int q = rp[t]; q < rp[t + 1]; ++q)
+        val[pos[q]] += v[q];
+}
+
+extern "C" int rsgpu_csr_add_batch(rsgpu_csr *A, int count, const int32_t *rows,
+                                   const int32_t *cols, const double *vals)
+{
+    if (!A || count < 0 || (count && (!rows || !cols || !vals)))
+        return RSGPU_BAD_ARGUMENT;
+    RS_TRY(A->ctx)
+    if (count == 0)
+        return RSGPU_OK;
+    std::vector<int> pos(count);
+    for (int q = 0; q < count; ++q) {
+        const int r = rows[q], c = cols[q];
+        if (r < 0 || r >= A->n || c < 0 || c >= A->ncols)
+            throw RsError{RSGPU_BAD_ARGUMENT, "index out of range"};
+        auto b = A->h_colind.begin() + A->h_rowptr[r],
+             e = A->h_colind.begin() + A->h_rowptr[r + 1];
+        auto it = std::lower_bound(b, e, c);
+        if (it == e || *it != c)
+            throw RsError{RSGPU_NOT_IN_PATTERN,
+                          "Entry (" + std::to_string(r + 1) + ", " +
+                              std::to_string(c + 1) +
+                              ") is not in the matrix structure"};
+        pos[q] = (int)(it - A->h_colind.begin());
+    }
+    // group by target position keeping array order: entries for the same
+    // position are applied sequentially by one thread
+    std::vector<int> order(count);
+    for (int q = 0; q < count; ++q)
+        order[q] = q;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return pos[a] < pos[b]; });
+    std::vector<int> grp_ptr{0}, gpos(count);
+    std::vector<double> gval(count);
+    for (int q = 0; q < count; ++q) {
+        gpos[q] = pos[order[q]];
+        gval[q] = vals[order[q]];
+        if (q > 0 && gpos[q] != gpos[q - 1])
+            grp_ptr.push_back(q);
+    }
+    grp_ptr.push_back(count);
+    hipStream_t s = A->ctx->stream;
+    DevBuf<int> dgp, dpos;
+    DevBuf<double> dv;
+    dgp.upload(grp_ptr, s);
+    dpos.upload(gpos, s);
+    dv.upload(gval, s);
+    const int ng = (int)grp_ptr.size() - 1;
+    hipLaunchKernelGGL(k_add_ordered, dim3((ng + 255) / 256), dim3(256), 0, s,
+                       dgp.p, dpos.p, dv.p, A->val.p, ng);
+    RS_CHECK_LAUNCH();
+    RS_HIP(hipStreamSynchronize(s));
+    return RSGPU_OK;
+    RS_CATCH(A->ctx)
+}
+
+extern "C" int rsgpu_csr_zero_rows(rsgpu_csr *A, double diagonal,
+                                   const int32_t *rows, int nrows)
+{
+    if (!A || nrows < 0 || (nrows && !rows))
+        return RSGPU_BAD_ARGUMENT;
+    RS_TRY(A->ctx)
+    for (int q = 0; q < nrows; ++q)
+        if (rows[q] < 0 || rows[q] >= A->n)
+            throw RsError{RSGPU_BAD_ARGUMENT, "row index out of range"};
+    if (diagonal != 0)
+        for (int q = 0; q < nrows; ++q) {
+            const int r = rows[q];
+            if (r >= A->ncols ||
+                !std::binary_search(A->h_colind.begin() + A->h_rowptr[r],
+                                    A->h_colind.begin() + A->h_rowptr[r + 1], r))
+                throw RsError{RSGPU_NOT_IN_PATTERN,
+                              "Entry (" + std::to_string(r + 1) + ", " +
+                                  std::to_string(r + 1) +
+                                  ") is not in the matrix structure"};
+        }
+    hipStream_t s = A->ctx->stream;
+    DevBuf<int> d;
+    d.upload(rows, nrows, s);
+    csr_zero_rows_dev(A, diagonal, d.p, nrows);
+    RS_HIP(hipStreamSynchronize(s));
+    return RSGPU_OK;
+    RS_CATCH(A->ctx)
+}
+
+extern "C" int rsgpu_csr_mult(rsgpu_csr *A, const double *x, double *y)
+{
+    if (!A)
+        return RSGPU_BAD_ARGUMENT;
+    RS_TRY(A->ctx)
+    spmv_launch<double>(A->ctx->stream, A, A->val.p, x, y, 0, 0.0, nullptr);
+    return RSGPU_OK;
+    RS_CATCH(A->ctx)
+}
+
+extern "C" int rsgpu_csr_multadd(rsgpu_csr *A, double alpha, const double *x,
+                                 double *y)
+{
+    if (!A)
+        return RSGPU_BAD_ARGUMENT;
+    RS_TRY(A->ctx)
+    spmv_launch<double>(A->ctx->stream, A, A->val.p, x, y, 1, alpha, nullptr);
+    return RSGPU_OK;
+    RS_CATCH(A->ctx)
+}
+
+extern "C" int rsgpu_csr_transmult(rsgpu_csr *A, const double *x, double *y)
+{
+    if (!A)
+        return RSGPU_BAD_ARGUMENT;
+    RS_TRY(A->ctx)
+    csr_transmult_launch(A, 1.0, x, y, false);
+    return RSGPU_OK;
+    RS_CATCH(A->ctx)
+}
+
+extern "C" int rsgpu_csr_transmultadd(rsgpu_csr *A, double alpha, const double *x,
+                                      double *y)
+{
+    if (!A)
+        return RSGPU_BAD_ARGUMENT;
+    RS_TRY(A->ctx)
+    csr_transmult_launch(A, alpha, x, y, true);
+    return RSGPU_OK;
+    RS_CATCH(A->ctx)
+}
+
+extern "C" int rsgpu_csr_sync_f32(rsgpu_csr *A)
+{
+    if (!A)
+        return RSGPU_BAD_ARGUMENT;
+    RS_TRY(A->ctx)
+    A->val32.alloc(std::max(A->nnz, 1));
+    convert_f64_to_f32(A->ctx->stream, A->val.p, A->val32.p, A->nnz);
+    return RSGPU_OK;
+    RS_CATCH(A->ctx)
+}
+
+extern "C" int rsgpu_csr_mult_f32(rsgpu_csr *A, const float *x, float *y)
+{
+    if (!A || !A->val32.p)
+        return RSGPU_BAD_ARGUMENT;
+    RS_TRY(A->ctx)
+    spmv_launch<float>(A->ctx->stream, A, A->val32.p, x, y, 0, 0.0, nullptr);
+    return RSGPU_OK;
+    RS_CATCH(A->ctx)
+}
+
+template <class F>
+static int host_roundtrip(rsgpu_ctx *ctx, int nin, const double *in_host,
+                          int nout, double *out_host, F f)
+{
+    hipStream_t s = ctx->stream;
+    DevBuf<double> a, b;
+    a.upload(in_host, nin, s);
+    b.alloc(std::max(nout, 1));
+    f(a.p, b.p);
+    if (nout)
+        RS_HIP(hipMemcpyAsync(out_host, b.p, sizeof(double) * nout,
+                              hipMemcpyDeviceToHost, s));
+    RS_HIP(hipStreamSynchronize(s));
+    return RSGPU_OK;
+}
+
+extern "C" int rsgpu_csr_mult_host(rsgpu_csr *A, const double *x, double *y)
+{
+    if (!A || !x || !y)
+        return RSGPU_BAD_ARGUMENT;
+    RS_TRY(A->ctx)
+    return host_roundtrip(A->ctx, A->ncols, x, A->n, y, [&](double *a, double *b) {
+        spmv_launch<double>(A->ctx->stream, A, A->val.p, a, b, 0, 0.0, nullptr);
+    });
+    RS_CATCH(A->ctx)
+}
+
+extern "C" int rsgpu_csr_transmult_host(rsgpu_csr *A, const double *x, double *y)
+{
+    if (!A || !x || !y)
+        return RSGPU_BAD_ARGUMENT;
+    RS_TRY(A->ctx)
+    return host_roundtrip(A->ctx, A->n, x, A->ncols, y, [&](double *a, double *b) {
+        csr_transmult_launch(A, 1.0, a, b, false);
+    });
+    RS_CATCH(A->ctx)
+}
+
+// ---------------------------------------------------------------- preconditioner
+extern "C" int rsgpu_prec_create(rsgpu_ctx *ctx, int kind, int nblocks,
+                                 int precision, rsgpu_prec **out)
+{
+    if (!ctx || !out)
+        return RSGPU_BAD_ARGUMENT;
+    if (kind != RSGPU_PREC_IDENTITY && kind != RSGPU_PREC_ILU0 &&
+        kind != RSGPU_PREC_BLOCK_ILU0) {
+        ctx->err = "preconditioner kind not available";
+        return RSGPU_BAD_ARGUMENT;
+    }
+    if (precision != RSGPU_FP64 && precision != RSGPU_FP32)
+        return RSGPU_BAD_ARGUMENT;
+    if (kind == RSGPU_PREC_BLOCK_ILU0 && nblocks < 1)
+        return RSGPU_BAD_ARGUMENT;
+    rsgpu_prec *M = new rsgpu_prec();
+    M->ctx = ctx;
+    M->kind = kind;
+    M->nblocks = kind == RSGPU_PREC_BLOCK_ILU0 ? nblocks : 1;
+    M->precision = precision;
+    M->ready = (kind == RSGPU_PREC_IDENTITY);
+    *out = M;
+    return RSGPU_OK;
+}
+
+extern "C" int rsgpu_prec_destroy(rsgpu_prec *M)
+{
+    if (M) {
+        if (M->ctx)
+            (void)hipStreamSynchronize(M->ctx->stream);
+        delete M;
+    }
+    return RSGPU_OK;
+}
+
+extern "C" int rsgpu_prec_set_matrix(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
+{
+    if (!M || !A)
+        return RSGPU_BAD_ARGUMENT;
+    if (bad_row)
+        *bad_row = -1;
+    RS_TRY(M->ctx)
+    RS_HIP(hipSetDevice(M->ctx->device));
+    if (M->kind == RSGPU_PREC_IDENTITY) {
+        M->n = A->n;
+        M->pattern_id = A->pattern_id;
+        M->ready = true;
+        return RSGPU_OK;
+    }
+    if (!M->ready || M->pattern_id != A->pattern_id) {
+        try {
+            prec_setup(M, A);
+        } catch (const RsError &e) {
+            if (e.code == RSGPU_MISSING_DIAGONAL && bad_row) {
+                // message carries "row k+1"
+                auto p = e.msg.rfind(' ');
+                *bad_row = std::stoi(e.msg.substr(p + 1)) - 1;
+            }
+            M->ready = false;
+            throw;
+        }
+    }
+    int st = prec_factor(M, A, bad_row);
+    if (st == RSGPU_ZERO_PIVOT)
+        M->ctx->err = "Zero pivot encountered on row " +
+                      std::to_string((bad_row ? *bad_row : 0) + 1) +
+                      " during ILU process";
+    return st;
+    RS_CATCH(M->ctx)
+}
+
+extern "C" int rsgpu_prec_apply(rsgpu_prec *M, const double *b, double *x)
+{
+    if (!M || !M->ready)
+        return RSGPU_BAD_ARGUMENT;
+    RS_TRY(M->ctx)
+    prec_apply(M, b, x, nullptr);
+    return RSGPU_OK;
+    RS_CATCH(M->ctx)
+}
+
+extern "C" int rsgpu_prec_apply_f32(rsgpu_prec *M, const float *b, float *x)
+{
+    if (!M || !M->ready)
+        return RSGPU_BAD_ARGUMENT;
+    RS_TRY(M->ctx)
+    prec_apply(M, b, x, nullptr);
+    return RSGPU_OK;
+    RS_CATCH(M->ctx)
+}
+
+extern "C" int rsgpu_prec_trans_apply(rsgpu_prec *M, const double *b, double *x)
+{
+    if (!M || !M->ready)
+        return RSGPU_BAD_ARGUMENT;
+    RS_TRY(M->ctx)
+    prec_trans_apply(M, b, x);
+    return RSGPU_OK;
+    RS_CATCH(M->ctx)
+}
+
+extern "C" int rsgpu_prec_apply_host(rsgpu_prec *M, const double *b, double *x)
+{
+    if (!M || !M->ready || !b || !x)
+        return RSGPU_BAD_ARGUMENT;
+    RS_TRY(M->ctx)
+    return host_roundtrip(M->ctx, M->n, b, M->n, x, [&](double *a, double *o) {
+        prec_apply(M, a, o, nullptr);
+    });
+    RS_CATCH(M->ctx)
+}
+
+extern "C" int rsgpu_prec_trans_apply_host(rsgpu_prec *M, const double *b,
+                                           double *x)
+{
+    if (!M || !M->ready || !b || !x)
+        return RSGPU_BAD_ARGUMENT;
+    RS_TRY(M->ctx)
+    return host_roundtrip(M->ctx, M->n, b, M->n, x, [&](double *a, double *o) {
+        prec_trans_apply(M, a, o);
+    });
+    RS_CATCH(M->ctx)
+}
+
+extern "C" int rsgpu_prec_get_factors(const rsgpu_prec *M, int32_t *rowptr,
+                                      int32_t *colind, double *lu, int *nnz)
+{
+    if (!M || !M->ready || M->kind == RSGPU_PREC_IDENTITY)
+        return RSGPU_BAD_ARGUMENT;
+    RS_TRY(M->ctx)
+    if (nnz)
+        *nnz = M->nnz;
+    if (rowptr)
+        std::copy(M->h_rowptr.begin(), M->h_rowptr.end(), rowptr);
+    if (colind)
+        std::copy(M->h_colind.begin(), M->h_colind.end(), colind);
+    if (lu && M->nnz) {
+        if (M->precision == RSGPU_FP64) {
+            RS_HIP(hipMemcpy(lu, M->lu64.p, sizeof(double) * M->nnz,
+                             hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> t(M->nnz);
+            RS_HIP(hipMemcpy(t.data(), M->lu32.p, sizeof(float) * M->nnz,
+                             hipMemcpyDeviceToHost));
+            for (int i = 0; i < M->nnz; ++i)
+                lu[i] = t[i];
+        }
+    }
+    return RSGPU_OK;
+    RS_CATCH(M->ctx)
+}
+
+extern "C" int rsgpu_prec_info(const rsgpu_prec *M, int *nnz, int *nl, int *nu)
+{
+    if (!M)
+        return RSGPU_BAD_ARGUMENT;
+    if (nnz)
+        *nnz = M->nnz;
+    if (nl)
+        *nl = M->L.max_levels;
+    if (nu)
+        *nu = M->U.max_levels;
+    return RSGPU_OK;
+}
+
+// host-array solve: upload b and x, solve, download x
+struct rsgpu_solver;
+extern "C" int rsgpu_solver_solve_host(rsgpu_solver *S, rsgpu_csr *A,
+                                       rsgpu_prec *M, const double *b,
+                                       double *x, rsgpu_stats *st)
+{
+    if (!S || !A || !b || !x)
+        return RSGPU_BAD_ARGUMENT;
+    rsgpu_ctx *ctx = A->ctx;
+    RS_TRY(ctx)
+    hipStream_t s = ctx->stream;
+    DevBuf<double> db, dx;
+    db.upload(b, A->n, s);
+    dx.upload(x, A->n, s);
+    int rc = rsgpu_solver_solve(S, A, M, db.p, dx.p, st);
+    if (A->n)
+        RS_HIP(hipMemcpyAsync(x, dx.p, sizeof(double) * A->n,
+                              hipMemcpyDeviceToHost, s));
+    RS_HIP(hipStreamSynchronize(s));
+    return rc;
+    RS_CATCH(ctx)
+}

@@ -1,0 +1,1027 @@
+// Krylov solvers on gfx950 with the monitor and all scalars on the device.
+//
+// Reference: BiCGstab.solve (mtj/src/no/uib/cipr/matrix/sparse/
+// BiCGstab.java:65-116), GMRES.solve (GMRES.java:127-176) with
+// GivensRotation (mtj/src/no/uib/cipr/matrix/GivensRotation.java:42-93) and
+// the dtrtrs back-solve (AbstractTriangDenseMatrix.java:227-256), IR.solve
+// (IR.java:57-70), DefaultIterationMonitor (DefaultIterationMonitor.java:
+// 133-156).
+//
+// Design.  The host enqueues the fixed kernel sequence of one iteration (or
+// one GMRES inner step) without waiting; every kernel starts by reading the
+// device flags (done / cycle_end) and returns at once when the solve has
+// stopped.  Scalar decisions (alpha, beta, omega, Givens rotations,
+// convergence, breakdown) are taken by the finishing workgroup of the
+// reduction that produces their operands, so no kernel launch exists only
+// to do scalar work.  The host reads the flags one step behind (an event
+// per step), so the GPU queue never drains while the host decides.
+// Each vector update keeps the reference's per-element operation order
+// (including DenseVector.add/set's alpha == 0 shortcuts); dot products and
+// norms are deterministic tree sums.
+#include <algorithm>
+#include <cmath>
+
+#include "spmv_core.hpp"
+
+#define RS_VBS 256
+
+struct rsgpu_solver {
+    rsgpu_ctx *ctx = nullptr;
+    rsgpu_solver_opts o{};
+    int n = 0;
+    DevState *st = nullptr;       // device
+    DevState *st_in = nullptr;    // inner solve (mixed)
+    DevBuf<double> hist, hist_in;
+    DevBuf<double> partials;
+    DevBuf<unsigned> ticket;
+    DevBuf<double> gmres_small;   // H, s, gc, gs for outer and inner
+    // FP64 work vectors
+    DevBuf<double> w64;
+    // FP32 work vectors (inner solve)
+    DevBuf<float> w32;
+    DevBuf<double> xr;            // mixed: residual / correction in FP64
+    int last_hist = 0;
+    std::vector<double> h_hist;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int *flags_host = nullptr;    // pinned [2][4]
+};
+
+static int grid_for(int n)
+{
+    int g = (n + RS_VBS - 1) / RS_VBS;
+    return std::max(1, std::min(g, 2048));
+}
+
+__device__ __forceinline__ bool stopped(const DevState *st, bool inner_step)
+{
+    return st->done || (inner_step && st->cycle_end);
+}
+
+// =========================================================== fused SpMV
+// y = A x (mult rule), plus up to two partial dot products over the rows,
+// finished by Fin.
+template <typename T>
+struct EpiMultDot {
+    T *y;
+    const T *z1;    // nullptr -> no first dot
+    const T *z2;    // nullptr -> second dot uses y itself
+    int nd;
+    double a1 = 0, a2 = 0;
+    __device__ T init(int) const { return T(0); }
+    __device__ void row(int r, T dot)
+    {
+        const T v = (dot != T(0)) ? dot : T(0);
+        y[r] = v;
+        if (nd >= 1)
+            a1 += (double)(z1[r] * v);
+        if (nd >= 2)
+            a2 += (double)(z2 ? z2[r] * v : v * v);
+    }
+};
+
+// residual r = b + (-1)*A x, optional copy, norm^2 partial
+template <typename T>
+struct EpiResidNorm {
+    T *y;
+    T *copy;
+    const T *b;
+    float *y32;     // optional FP32 shadow of the residual (mixed form)
+    double a1 = 0, a2 = 0;
+    __device__ T init(int) const { return T(0); }
+    __device__ void row(int r, T dot)
+    {
+        const T v = b[r] + T(-1) * dot;
+        y[r] = v;
+        if (copy)
+            copy[r] = v;
+        if (y32)
+            y32[r] = (float)v;
+        a1 += (double)(v * v);
+    }
+};
+
+enum {
+    FIN_NONE = 0,
+    FIN_ALPHA,       // BiCGstab: alpha = rho1 / (rtilde . v)
+    FIN_OMEGA,       // BiCGstab: omega = (t . s) / (t . t)
+    FIN_INIT_NORM,   // monitor(||r||) at iteration 0 (BiCGstab, IR)
+    FIN_GM_NORMR,    // GMRES: normr = ||r||; outer monitor
+};
+
+__device__ void finish_scalar(DevState *st, int fin, double t1, double t2)
+{
+    switch (fin) {
+    case FIN_ALPHA:
+        st->alpha = st->rho1 / t1;
+        break;
+    case FIN_OMEGA:
+        st->ts = t1;
+        st->tt = t2;
+        st->omega = t1 / t2;
+        break;
+    case FIN_INIT_NORM:
+        if (dev_monitor(st, sqrt(t1)))
+            st->done = 1;
+        break;
+    case FIN_GM_NORMR:
+        st->normr = sqrt(t1);
+        if (dev_monitor(st, st->normr))
+            st->done = 1;
+        break;
+    default:
+        break;
+    }
+}
+
+template <typename T, class Epi, int NR>
+__global__ void __launch_bounds__(RS_SPMV_BS)
+    k_spmv_red(const int *__restrict__ wg_rows, const int *__restrict__ rowptr,
+               const int *__restrict__ colind, const T *__restrict__ val,
+               const T *__restrict__ x, Epi epi, DevState *st, RedSlot slot,
+               int fin, int inner_step)
+{
+    if (stopped(st, inner_step))
+        return;
+    spmv_core<T, RS_SPMV_BS, RS_SPMV_NNZ>(wg_rows, rowptr, colind, val, x, epi);
+    double v[NR], tot[NR];
+    v[0] = epi.a1;
+    if (NR > 1)
+        v[NR - 1] = epi.a2;
+    if (block_reduce_finish<RS_SPMV_BS, NR>(slot, v, tot) && threadIdx.x == 0)
+        finish_scalar(st, fin, tot[0], NR > 1 ? tot[NR - 1] : 0.0);
+}
+
+// =========================================================== BiCGstab kernels
+// rho1 = rtilde . r; breakdown checks; beta
+template <typename T>
+__global__ void __launch_bounds__(RS_VBS)
+    k_bicg_rho(int n, const T *__restrict__ rt, const T *__restrict__ r,
+               DevState *st, RedSlot slot)
+{
+    if (st->done)
+        return;
+    double a = 0;
+    for (int i = blockIdx.x * RS_VBS + threadIdx.x; i < n; i += gridDim.x * RS_VBS)
+        a += (double)(rt[i] * r[i]);
+    double v[1] = {a}, tot[1];
+    if (block_reduce_finish<RS_VBS, 1>(slot, v, tot) && threadIdx.x == 0) {
+        st->rho1 = tot[0];
+        if (st->rho1 == 0 || st->omega == 0) {
+            st->status = RSGPU_BREAKDOWN;
+            st->done = 1;
+            return;
+        }
+        if (!st->first)
+            st->beta = (st->rho1 / st->rho2) * (st->alpha / st->omega);
+    }
+}
+
+// p = r (first) or temp = (-omega) v + p; p = r + beta*temp
+template <typename T>
+__global__ void __launch_bounds__(RS_VBS)
+    k_bicg_p(int n, const T *__restrict__ r, const T *__restrict__ v,
+             T *__restrict__ p, const DevState *st)
+{
+    if (st->done)
+        return;
+    const int i = blockIdx.x * RS_VBS + threadIdx.x;
+    if (i >= n)
+        return;
+    if (st->first) {
+        p[i] = r[i];
+        return;
+    }
+    const T mo = (T)(-st->omega), beta = (T)st->beta;
+    T temp = (mo == T(0)) ? T(0) : mo * v[i];
+    temp += p[i];
+    T pi = r[i];
+    if (beta != T(0))
+        pi += beta * temp;
+    p[i] = pi;
+}
+
+// s = r + (-alpha) v; monitor(||s||)
+template <typename T>
+__global__ void __launch_bounds__(RS_VBS)
+    k_bicg_s(int n, const T *__restrict__ r, const T *__restrict__ v,
+             T *__restrict__ s, DevState *st, RedSlot slot)
+{
+    if (st->done)
+        return;
+    const T ma = (T)(-st->alpha);
+    double a = 0;
+    for (int i = blockIdx.x * RS_VBS + threadIdx.x; i < n; i += gridDim.x * RS_VBS) {
+        T si = r[i];
+        if (ma != T(0))
+            si += ma * v[i];
+        s[i] = si;
+        a += (double)(si * si);
+    }
+    double vv[1] = {a}, tot[1];
+    if (block_reduce_finish<RS_VBS, 1>(slot, vv, tot) && threadIdx.x == 0) {
+        if (dev_monitor(st, sqrt(tot[0]))) {
+            if (st->conv)
+                st->half = 1;     // return x.add(alpha, phat)
+            else
+                st->done = 1;
+        }
+    }
+}
+
+// early return after s converged: x += alpha * phat
+template <typename T>
+__global__ void __launch_bounds__(RS_VBS)
+    k_bicg_half(int n, const T *__restrict__ phat, T *__restrict__ x,
+                DevState *st, RedSlot slot)
+{
+    if (st->done || !st->half)
+        return;
+    const T al = (T)st->alpha;
+    for (int i = blockIdx.x * RS_VBS + threadIdx.x; i < n; i += gridDim.x * RS_VBS)
+        if (al != T(0))
+            x[i] += al * phat[i];
+    double v[1] = {0.0}, tot[1];
+    if (block_reduce_finish<RS_VBS, 1>(slot, v, tot) && threadIdx.x == 0)
+        st->done = 1;
+}
+
+// x += alpha phat; x += omega shat; r = s + (-omega) t; monitor(||r||)
+template <typename T>
+__global__ void __launch_bounds__(RS_VBS)
+    k_bicg_update(int n, const T *__restrict__ phat, const T *__restrict__ shat,
+                  const T *__restrict__ s, const T *__restrict__ t,
+                  T *__restrict__ x, T *__restrict__ r, DevState *st,
+                  RedSlot slot)
+{
+    if (st->done)
+        return;
+    const T al = (T)st->alpha, om = (T)st->omega, mo = (T)(-st->omega);
+    double a = 0;
+    for (int i = blockIdx.x * RS_VBS + threadIdx.x; i < n; i += gridDim.x * RS_VBS) {
+        T xi = x[i];
+        if (al != T(0))
+            xi += al * phat[i];
+        if (om != T(0))
+            xi += om * shat[i];
+        x[i] = xi;
+        T ri = s[i];
+        if (mo != T(0))
+            ri += mo * t[i];
+        r[i] = ri;
+        a += (double)(ri * ri);
+    }
+    double v[1] = {a}, tot[1];
+    if (block_reduce_finish<RS_VBS, 1>(slot, v, tot) && threadIdx.x == 0) {
+        st->rho2 = st->rho1;
+        st->first = 0;
+        st->iter++;
+        if (dev_monitor(st, sqrt(tot[0])))
+            st->done = 1;
+    }
+}
+
+// =========================================================== GMRES kernels
+// v0 = (1/normr) r; s = [normr, 0...]; inner check on |s_0|
+template <typename T>
+__global__ void __launch_bounds__(RS_VBS)
+    k_gm_start(int n, const T *__restrict__ r, T *__restrict__ v0, DevState *st,
+               RedSlot slot)
+{
+    if (st->done)
+        return;
+    const T sc = (T)(1 / st->normr);
+    for (int i = blockIdx.x * RS_VBS + threadIdx.x; i < n; i += gridDim.x * RS_VBS)
+        v0[i] = (sc == T(0)) ? T(0) : sc * r[i];
+    double v[1] = {0.0}, tot[1];
+    if (block_reduce_finish<RS_VBS, 1>(slot, v, tot) && threadIdx.x == 0) {
+        for (int k = 0; k <= st->m; ++k)
+            st->s[k] = 0;
+        st->s[0] = st->normr;
+        st->gi = 0;
+        st->cycle_end = 0;
+        if (st->m <= 0 || dev_monitor(st, fabs(st->s[0]))) {
+            st->cycle_end = 1;
+            if (st->status)
+                st->done = 1;
+        }
+    }
+}
+
+// MGS step k of column i: w -= H(k-1,i) v_{k-1} (k > 0), H(k,i) = w . v_k
+template <typename T>
+__global__ void __launch_bounds__(RS_VBS)
+    k_gm_mgs(int n, int i, int k, T *__restrict__ w, const T *__restrict__ vprev,
+             const T *__restrict__ vk, DevState *st, RedSlot slot)
+{
+    if (stopped(st, true))
+        return;
+    const int ld = st->m + 1;
+    const T h = k > 0 ? (T)(-st->H[(k - 1) + i * ld]) : T(0);
+    double a = 0;
+    for (int j = blockIdx.x * RS_VBS + threadIdx.x; j < n; j += gridDim.x * RS_VBS) {
+        T wj = w[j];
+        if (k > 0 && h != T(0)) {
+            wj += h * vprev[j];
+            w[j] = wj;
+        }
+        a += (double)(wj * vk[j]);
+    }
+    double v[1] = {a}, tot[1];
+    if (block_reduce_finish<RS_VBS, 1>(slot, v, tot) && threadIdx.x == 0)
+        st->H[k + i * ld] = tot[0];
+}
+
+// last MGS update w -= H(i,i) v_i, H(i+1,i) = ||w||, Givens, monitor
+template <typename T>
+__global__ void __launch_bounds__(RS_VBS)
+    k_gm_last(int n, int i, T *__restrict__ w, const T *__restrict__ vi,
+              DevState *st, RedSlot slot)
+{
+    if (stopped(st, true))
+        return;
+    const int ld = st->m + 1;
+    const T h = (T)(-st->H[i + i * ld]);
+    double a = 0;
+    for (int j = blockIdx.x * RS_VBS + threadIdx.x; j < n; j += gridDim.x * RS_VBS) {
+        T wj = w[j];
+        if (h != T(0)) {
+            wj += h * vi[j];
+            w[j] = wj;
+        }
+        a += (double)(wj * wj);
+    }
+    double v[1] = {a}, tot[1];
+    if (block_reduce_finish<RS_VBS, 1>(slot, v, tot) && threadIdx.x == 0) {
+        double *Hc = st->H + i * ld;
+        Hc[i + 1] = sqrt(tot[0]);
+        st->hsub = Hc[i + 1];
+        // previous rotations on column i
+        for (int q = 0; q < i; ++q) {
+            const double c = st->gc[q], s = st->gs[q];
+            const double temp = c * Hc[q] + s * Hc[q + 1];
+            Hc[q + 1] = -s * Hc[q] + c * Hc[q + 1];
+            Hc[q] = temp;
+        }
+        // new rotation (GivensRotation(x, y))
+        const double x = Hc[i], y = Hc[i + 1];
+        const double roe = fabs(x) > fabs(y) ? x : y;
+        const double scale = fabs(x) + fabs(y);
+        double c, s;
+        if (scale != 0) {
+            const double xs = x / scale, ys = y / scale;
+            double rr = scale * sqrt(xs * xs + ys * ys);
+            if (roe < 0)
+                rr *= -1;
+            c = x / rr;
+            s = y / rr;
+        } else {
+            c = 1;
+            s = 0;
+        }
+        st->gc[i] = c;
+        st->gs[i] = s;
+        {
+            const double temp = c * Hc[i] + s * Hc[i + 1];
+            Hc[i + 1] = -s * Hc[i] + c * Hc[i + 1];
+            Hc[i] = temp;
+        }
+        {
+            double *sv = st->s;
+            const double temp = c * sv[i] + s * sv[i + 1];
+            sv[i + 1] = -s * sv[i] + c * sv[i + 1];
+            sv[i] = temp;
+        }
+        st->iter++;
+        st->gi = i + 1;
+        if (!(i + 1 < st->m) || dev_monitor(st, fabs(st->s[i + 1]))) {
+            st->cycle_end = 1;
+            if (st->status)
+                st->done = 1;
+        }
+    }
+}
+
+// v_{i+1} = (1/H(i+1,i)) w
+template <typename T>
+__global__ void __launch_bounds__(RS_VBS)
+    k_gm_scale(int n, const T *__restrict__ w, T *__restrict__ vn,
+               const DevState *st)
+{
+    if (stopped(st, true))
+        return;
+    const int j = blockIdx.x * RS_VBS + threadIdx.x;
+    if (j >= n)
+        return;
+    const T sc = (T)(1. / st->hsub);
+    vn[j] = (sc == T(0)) ? T(0) : sc * w[j];
+}
+
+// dtrtrs on the leading gi x gi triangle of H (singular -> status 14)
+__global__ void k_gm_backsolve(DevState *st)
+{
+    if (st->done || threadIdx.x != 0)
+        return;
+    const int ld = st->m + 1, m = st->gi;
+    const double *H = st->H;
+    double *s = st->s;
+    for (int k = 0; k < m; ++k)
+        if (H[k + k * ld] == 0) {
+            st->status = RSGPU_SINGULAR;
+            st->done = 1;
+            return;
+        }
+    for (int k = m - 1; k >= 0; --k)
+        if (s[k] != 0) {
+            s[k] = s[k] / H[k + k * ld];
+            for (int i = 0; i < k; ++i)
+                s[i] = s[i] - s[k] * H[i + k * ld];
+        }
+}
+
+// x += s_j v_j, j = 0..gi-1, in that order per element
+template <typename T>
+__global__ void __launch_bounds__(RS_VBS)
+    k_gm_xupdate(int n, const T *__restrict__ V, size_t ldv, T *__restrict__ x,
+                 const DevState *st)
+{
+    if (st->done)
+        return;
+    const int j = blockIdx.x * RS_VBS + threadIdx.x;
+    if (j >= n)
+        return;
+    const int m = st->gi;
+    T xj = x[j];
+    for (int q = 0; q < m; ++q) {
+        const T sq = (T)st->s[q];
+        if (sq != T(0))
+            xj += sq * V[(size_t)q * ldv + j];
+    }
+    x[j] = xj;
+}
+
+// ||r|| at the end of a cycle: iter.next(), cycle reset, outer monitor
+template <typename T>
+__global__ void __launch_bounds__(RS_VBS)
+    k_gm_outer(int n, const T *__restrict__ r, DevState *st, RedSlot slot,
+               int first)
+{
+    if (st->done)
+        return;
+    double a = 0;
+    for (int i = blockIdx.x * RS_VBS + threadIdx.x; i < n; i += gridDim.x * RS_VBS)
+        a += (double)(r[i] * r[i]);
+    double v[1] = {a}, tot[1];
+    if (block_reduce_finish<RS_VBS, 1>(slot, v, tot) && threadIdx.x == 0) {
+        st->normr = sqrt(tot[0]);
+        if (!first)
+            st->iter++;
+        st->cycle_end = 0;
+        if (dev_monitor(st, st->normr))
+            st->done = 1;
+    }
+}
+
+// IR: x += z (FP64 x, correction in T)
+template <typename T>
+__global__ void __launch_bounds__(RS_VBS)
+    k_ir_add(int n, const T *__restrict__ z, double *__restrict__ x,
+             DevState *st)
+{
+    if (st->done)
+        return;
+    const int i = blockIdx.x * RS_VBS + threadIdx.x;
+    if (i < n)
+        x[i] += (double)z[i];
+}
+
+__global__ void k_ir_next(DevState *st)
+{
+    if (threadIdx.x == 0 && !st->done)
+        st->iter++;
+}
+
+// Cycle-end sequence: the preconditioner apply of the outer residual must
+// run although the inner loop has set cycle_end.
+__global__ void k_clear_cycle(DevState *st)
+{
+    if (threadIdx.x == 0 && !st->done)
+        st->cycle_end = 0;
+}
+
+template <typename T>
+__global__ void k_fill(T *__restrict__ a, size_t n, T v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        a[i] = v;
+}
+
+// =========================================================== host drivers
+namespace {
+
+struct Vecs {
+    int n;
+};
+
+template <typename T>
+const T *csr_values(rsgpu_csr *A);
+template <>
+const double *csr_values<double>(rsgpu_csr *A) { return A->val.p; }
+template <>
+const float *csr_values<float>(rsgpu_csr *A) { return A->val32.p; }
+
+void state_init(rsgpu_solver *S, DevState *dst, DevBuf<double> &hist,
+                int max_iter, double rtol, double atol, double dtol, int m,
+                double *small)
+{
+    DevState h;
+    std::memset(&h, 0, sizeof(h));
+    h.rtol = rtol;
+    h.atol = atol;
+    h.dtol = dtol;
+    h.max_iter = max_iter;
+    h.first = 1;
+    h.rho1 = h.rho2 = h.alpha = h.beta = h.omega = 1;
+    h.hist = hist.p;
+    h.hist_cap = (int)hist.n;
+    h.m = m;
+    if (small) {
+        h.H = small;
+        h.s = small + (size_t)(m + 1) * m;
+        h.gc = h.s + (m + 1);
+        h.gs = h.gc + (m + 1);
+    }
+    RS_HIP(hipMemcpyAsync(dst, &h, sizeof(h), hipMemcpyHostToDevice,
+                          S->ctx->stream));
+}
+
+RedSlot slot_of(rsgpu_solver *S) { return RedSlot{S->partials.p, S->ticket.p}; }
+
+// host view of the flags, read one step behind
+struct Poller {
+    rsgpu_solver *S;
+    DevState *st;
+    int k = 0;
+    bool pending = false;
+    void post()
+    {
+        hipStream_t s = S->ctx->stream;
+        int *dst = S->flags_host + 4 * (k & 1);
+        RS_HIP(hipMemcpyAsync(dst, &st->done, sizeof(int) * 1,
+                              hipMemcpyDeviceToHost, s));
+        RS_HIP(hipMemcpyAsync(dst + 1, &st->cycle_end, sizeof(int),
+                              hipMemcpyDeviceToHost, s));
+        RS_HIP(hipEventRecord(S->ev[k & 1], s));
+        k++;
+    }
+    // wait for the previous post; returns {done, cycle_end}
+    void wait_prev(int &done, int &cend)
+    {
+        done = 0;
+        cend = 0;
+        if (k < 2)
+            return;
+        const int q = (k - 2) & 1;
+        RS_HIP(hipEventSynchronize(S->ev[q]));
+        done = S->flags_host[4 * q];
+        cend = S->flags_host[4 * q + 1];
+    }
+    void wait_last(int &done, int &cend)
+    {
+        const int q = (k - 1) & 1;
+        RS_HIP(hipEventSynchronize(S->ev[q]));
+        done = S->flags_host[4 * q];
+        cend = S->flags_host[4 * q + 1];
+    }
+};
+
+template <typename T>
+void launch_spmv_red(rsgpu_solver *S, rsgpu_csr *A, const T *x, EpiMultDot<T> epi,
+                     DevState *st, int fin, int inner)
+{
+    hipStream_t s = S->ctx->stream;
+    dim3 g(A->plan.nwg), b(RS_SPMV_BS);
+    if (epi.nd <= 1)
+        hipLaunchKernelGGL((k_spmv_red<T, EpiMultDot<T>, 1>), g, b, 0, s,
+                           A->plan.wg_rows.p, A->rowptr.p, A->colind.p,
+                           csr_values<T>(A), x, epi, st, slot_of(S), fin, inner);
+    else
+        hipLaunchKernelGGL((k_spmv_red<T, EpiMultDot<T>, 2>), g, b, 0, s,
+                           A->plan.wg_rows.p, A->rowptr.p, A->colind.p,
+                           csr_values<T>(A), x, epi, st, slot_of(S), fin, inner);
+    RS_CHECK_LAUNCH();
+}
+
+template <typename T>
+void launch_resid(rsgpu_solver *S, rsgpu_csr *A, const T *b, const T *x, T *r,
+                  T *copy, float *r32, DevState *st, int fin)
+{
+    hipStream_t s = S->ctx->stream;
+    EpiResidNorm<T> epi{r, copy, b, r32};
+    hipLaunchKernelGGL((k_spmv_red<T, EpiResidNorm<T>, 1>), dim3(A->plan.nwg),
+                       dim3(RS_SPMV_BS), 0, s, A->plan.wg_rows.p, A->rowptr.p,
+                       A->colind.p, csr_values<T>(A), x, epi, st, slot_of(S),
+                       fin, 0);
+    RS_CHECK_LAUNCH();
+}
+
+// ------------------------------------------------------------ BiCGstab
+// work: p s phat shat t v r rtilde (8 n)
+template <typename T>
+void run_bicgstab(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const T *b,
+                  T *x, T *work, DevState *st)
+{
+    hipStream_t s = S->ctx->stream;
+    const int n = A->n;
+    T *p = work, *sv = work + n, *phat = work + 2 * (size_t)n,
+      *shat = work + 3 * (size_t)n, *t = work + 4 * (size_t)n,
+      *v = work + 5 * (size_t)n, *r = work + 6 * (size_t)n,
+      *rt = work + 7 * (size_t)n;
+    const int G = grid_for(n), E = (n + RS_VBS - 1) / RS_VBS;
+    RedSlot sl = slot_of(S);
+    // r = b - A x; rtilde = r; monitor(||r||) at iter 0
+    launch_resid<T>(S, A, b, x, r, rt, nullptr, st, FIN_INIT_NORM);
+    Poller P{S, st};
+    int done = 0, cend = 0;
+    for (;;) {
+        hipLaunchKernelGGL(k_bicg_rho<T>, dim3(G), dim3(RS_VBS), 0, s, n, rt, r,
+                           st, sl);
+        hipLaunchKernelGGL(k_bicg_p<T>, dim3(E), dim3(RS_VBS), 0, s, n, r, v, p,
+                           st);
+        prec_apply(M, p, phat, st);
+        launch_spmv_red<T>(S, A, phat, EpiMultDot<T>{v, rt, nullptr, 1}, st,
+                           FIN_ALPHA, 0);
+        hipLaunchKernelGGL(k_bicg_s<T>, dim3(G), dim3(RS_VBS), 0, s, n, r, v, sv,
+                           st, sl);
+        hipLaunchKernelGGL(k_bicg_half<T>, dim3(G), dim3(RS_VBS), 0, s, n, phat,
+                           x, st, sl);
+        prec_apply(M, sv, shat, st);
+        launch_spmv_red<T>(S, A, shat, EpiMultDot<T>{t, sv, nullptr, 2}, st,
+                           FIN_OMEGA, 0);
+        hipLaunchKernelGGL(k_bicg_update<T>, dim3(G), dim3(RS_VBS), 0, s, n,
+                           phat, shat, sv, t, x, r, st, sl);
+        RS_CHECK_LAUNCH();
+        P.post();
+        P.wait_prev(done, cend);
+        if (done)
+            break;
+    }
+    P.wait_last(done, cend);
+}
+
+// ------------------------------------------------------------ GMRES(m)
+// work: w u r + (m+1) basis vectors
+template <typename T>
+void run_gmres(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const T *b, T *x,
+               T *work, DevState *st, int m)
+{
+    hipStream_t s = S->ctx->stream;
+    const int n = A->n;
+    T *w = work, *u = work + n, *r = work + 2 * (size_t)n,
+      *V = work + 3 * (size_t)n;
+    const int G = grid_for(n), E = (n + RS_VBS - 1) / RS_VBS;
+    RedSlot sl = slot_of(S);
+    // u = b - A x; r = M u; normr; outer monitor at iter 0
+    launch_resid<T>(S, A, b, x, u, nullptr, nullptr, st, FIN_NONE);
+    prec_apply(M, u, r, st);
+    hipLaunchKernelGGL(k_gm_outer<T>, dim3(G), dim3(RS_VBS), 0, s, n, r, st, sl,
+                       1);
+    Poller P{S, st};
+    int done = 0, cend = 0;
+    const int every = std::max(1, S->o.check_every);
+    for (;;) {
+        hipLaunchKernelGGL(k_gm_start<T>, dim3(G), dim3(RS_VBS), 0, s, n, r, V,
+                           st, sl);
+        for (int i = 0; i < m; ++i) {
+            T *vi = V + (size_t)i * n;
+            launch_spmv_red<T>(S, A, vi, EpiMultDot<T>{u, nullptr, nullptr, 0},
+                               st, FIN_NONE, 1);
+            prec_apply(M, u, w, st);
+            for (int k = 0; k <= i; ++k)
+                hipLaunchKernelGGL(k_gm_mgs<T>, dim3(G), dim3(RS_VBS), 0, s, n, i,
+                                   k, w, k > 0 ? V + (size_t)(k - 1) * n : V,
+                                   V + (size_t)k * n, st, sl);
+            hipLaunchKernelGGL(k_gm_last<T>, dim3(G), dim3(RS_VBS), 0, s, n, i, w,
+                               vi, st, sl);
+            hipLaunchKernelGGL(k_gm_scale<T>, dim3(E), dim3(RS_VBS), 0, s, n, w,
+                               V + (size_t)(i + 1) * n, st);
+            RS_CHECK_LAUNCH();
+            if ((i + 1) % every == 0 && i + 1 < m) {
+                P.post();
+                P.wait_prev(done, cend);
+                if (done || cend)
+                    break;
+            }
+        }
+        hipLaunchKernelGGL(k_gm_backsolve, dim3(1), dim3(64), 0, s, st);
+        hipLaunchKernelGGL(k_gm_xupdate<T>, dim3(E), dim3(RS_VBS), 0, s, n, V,
+                           (size_t)n, x, st);
+        launch_resid<T>(S, A, b, x, u, nullptr, nullptr, st, FIN_NONE);
+        // r = M u belongs to the outer loop: clear the inner-loop flag first
+        hipLaunchKernelGGL(k_clear_cycle, dim3(1), dim3(64), 0, s, st);
+        prec_apply(M, u, r, st);
+        hipLaunchKernelGGL(k_gm_outer<T>, dim3(G), dim3(RS_VBS), 0, s, n, r, st,
+                           sl, 0);
+        RS_CHECK_LAUNCH();
+        P.post();
+        P.wait_last(done, cend);
+        if (done)
+            break;
+    }
+}
+
+} // namespace
+
+// =========================================================== public API
+static int solver_catch(rsgpu_ctx *ctx, const RsError &e)
+{
+    if (ctx)
+        ctx->err = e.msg;
+    return e.code;
+}
+
+extern "C" void rsgpu_solver_opts_default(rsgpu_solver_opts *o)
+{
+    std::memset(o, 0, sizeof(*o));
+    o->kind = RSGPU_BICGSTAB;
+    o->restart = 30;
+    o->max_iter = 100000;
+    o->rtol = 1e-5;
+    o->atol = 1e-50;
+    o->dtol = 1e5;
+    o->mixed = 0;
+    o->inner_max_iter = 200;
+    o->inner_rtol = 1e-4;
+    o->check_every = 4;
+}
+
+extern "C" int rsgpu_solver_create(rsgpu_ctx *ctx, const rsgpu_solver_opts *opts,
+                                   int n, rsgpu_solver **out)
+{
+    if (!ctx || !opts || !out || n < 0)
+        return RSGPU_BAD_ARGUMENT;
+    if (opts->kind < 0 || opts->kind > 2)
+        return RSGPU_BAD_ARGUMENT;
+    if (opts->kind == RSGPU_GMRES && opts->restart <= 0) {
+        ctx->err = "restart must be a positive integer";
+        return RSGPU_BAD_ARGUMENT;
+    }
+    rsgpu_solver *S = new rsgpu_solver();
+    try {
+        RS_HIP(hipSetDevice(ctx->device));
+        S->ctx = ctx;
+        S->o = *opts;
+        S->n = n;
+        RS_HIP(hipMalloc((void **)&S->st, sizeof(DevState)));
+        RS_HIP(hipMalloc((void **)&S->st_in, sizeof(DevState)));
+        const int maxwg = std::max({grid_for(n), (n + 7) / 8, 4096});
+        S->partials.alloc((size_t)maxwg * 2);
+        S->ticket.alloc(4);
+        S->ticket.zero(ctx->stream);
+        const int m = std::max(1, opts->restart);
+        S->gmres_small.alloc((size_t)2 * ((m + 1) * m + 3 * (m + 1)));
+        size_t hcap = std::min<size_t>((size_t)opts->max_iter * 2 + 8, 1u << 22);
+        S->hist.alloc(hcap);
+        S->hist_in.alloc(std::min<size_t>((size_t)opts->inner_max_iter * 2 + 8,
+                                          1u << 20));
+        const int inner_kind = opts->mixed ? opts->kind : opts->kind;
+        size_t nvec64 = 0, nvec32 = 0;
+        if (!opts->mixed) {
+            nvec64 = opts->kind == RSGPU_BICGSTAB ? 8
+                     : opts->kind == RSGPU_GMRES  ? 3 + (size_t)m + 1
+                                                  : 2;
+        } else {
+            nvec64 = 2;   // r, scratch
+            nvec32 = 2 + (inner_kind == RSGPU_GMRES ? 3 + (size_t)m + 1 : 8);
+        }
+        S->w64.alloc(std::max<size_t>(1, nvec64 * (size_t)n));
+        if (nvec32)
+            S->w32.alloc(std::max<size_t>(1, nvec32 * (size_t)n));
+        RS_HIP(hipEventCreateWithFlags(&S->ev[0], hipEventDisableTiming));
+        RS_HIP(hipEventCreateWithFlags(&S->ev[1], hipEventDisableTiming));
+        RS_HIP(hipHostMalloc((void **)&S->flags_host, sizeof(int) * 8,
+                             hipHostMallocDefault));
+        RS_HIP(hipStreamSynchronize(ctx->stream));
+    } catch (const RsError &e) {
+        rsgpu_solver_destroy(S);
+        return solver_catch(ctx, e);
+    }
+    *out = S;
+    return RSGPU_OK;
+}
+
+extern "C" int rsgpu_solver_destroy(rsgpu_solver *S)
+{
+    if (!S)
+        return RSGPU_OK;
+    if (S->st)
+        (void)hipFree(S->st);
+    if (S->st_in)
+        (void)hipFree(S->st_in);
+    for (auto &e : S->ev)
+        if (e)
+            (void)hipEventDestroy(e);
+    if (S->flags_host)
+        (void)hipHostFree(S->flags_host);
+    delete S;
+    return RSGPU_OK;
+}
+
+namespace {
+
+double dev_norm2(rsgpu_solver *S, const double *v, int n);
+
+template <typename T>
+void run_kind(rsgpu_solver *S, int kind, rsgpu_csr *A, rsgpu_prec *M, const T *b,
+              T *x, T *work, DevState *st, int m)
+{
+    if (kind == RSGPU_BICGSTAB)
+        run_bicgstab<T>(S, A, M, b, x, work, st);
+    else if (kind == RSGPU_GMRES)
+        run_gmres<T>(S, A, M, b, x, work, st, m);
+    else
+        throw RsError{RSGPU_BAD_ARGUMENT, "unsupported inner solver"};
+}
+
+// IR with M = preconditioner (IR.java:57-70), FP64
+void run_ir(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const double *b,
+            double *x, DevState *st)
+{
+    hipStream_t s = S->ctx->stream;
+    const int n = A->n;
+    double *r = S->w64.p, *z = S->w64.p + n;
+    const int E = (n + RS_VBS - 1) / RS_VBS;
+    Poller P{S, st};
+    int done = 0, cend = 0;
+    launch_resid<double>(S, A, b, x, r, nullptr, nullptr, st, FIN_INIT_NORM);
+    for (;;) {
+        prec_apply(M, r, z, st);
+        hipLaunchKernelGGL(k_ir_add<double>, dim3(E), dim3(RS_VBS), 0, s, n, z, x,
+                           st);
+        hipLaunchKernelGGL(k_ir_next, dim3(1), dim3(64), 0, s, st);
+        launch_resid<double>(S, A, b, x, r, nullptr, nullptr, st, FIN_INIT_NORM);
+        RS_CHECK_LAUNCH();
+        P.post();
+        P.wait_prev(done, cend);
+        if (done)
+            break;
+    }
+    P.wait_last(done, cend);
+}
+
+// Mixed: FP64 refinement on the true residual around an FP32 Krylov solve
+void run_mixed(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const double *b,
+               double *x, DevState *st, rsgpu_stats *stats)
+{
+    hipStream_t s = S->ctx->stream;
+    const int n = A->n;
+    const int m = std::max(1, S->o.restart);
+    double *r = S->w64.p;
+    float *r32 = S->w32.p, *z32 = S->w32.p + n, *work32 = S->w32.p + 2 * (size_t)n;
+    const int E = (n + RS_VBS - 1) / RS_VBS;
+    if (!A->val32.p)
+        throw RsError{RSGPU_BAD_ARGUMENT,
+                      "mixed solve needs rsgpu_csr_sync_f32 after the last value change"};
+    if (M && M->kind != RSGPU_PREC_IDENTITY && M->precision != RSGPU_FP32)
+        throw RsError{RSGPU_BAD_ARGUMENT, "mixed solve needs an FP32 preconditioner"};
+    int inner_total = 0, outer = 0;
+    DevState host;
+    double *small_in = S->gmres_small.p + ((size_t)(m + 1) * m + 3 * (m + 1));
+    launch_resid<double>(S, A, b, x, r, nullptr, r32, st, FIN_INIT_NORM);
+    for (;;) {
+        RS_HIP(hipMemcpyAsync(&host, st, sizeof(host), hipMemcpyDeviceToHost, s));
+        RS_HIP(hipStreamSynchronize(s));
+        if (host.done)
+            break;
+        // inner FP32 solve: z32 = A^-1 r32 from z32 = 0
+        state_init(S, S->st_in, S->hist_in, S->o.inner_max_iter, S->o.inner_rtol,
+                   1e-50, 1e5, m, small_in);
+        hipLaunchKernelGGL(k_fill<float>, dim3(E), dim3(RS_VBS), 0, s, z32,
+                           (size_t)n, 0.0f);
+        run_kind<float>(S, S->o.kind, A, M, r32, z32, work32, S->st_in, m);
+        DevState hin;
+        RS_HIP(hipMemcpyAsync(&hin, S->st_in, sizeof(hin), hipMemcpyDeviceToHost,
+                              s));
+        RS_HIP(hipStreamSynchronize(s));
+        inner_total += hin.iter;
+        // a breakdown of the inner solve still yields a usable correction
+        // unless it produced non-finite values; the outer monitor decides.
+        hipLaunchKernelGGL(k_ir_add<float>, dim3(E), dim3(RS_VBS), 0, s, n, z32, x,
+                           st);
+        hipLaunchKernelGGL(k_ir_next, dim3(1), dim3(64), 0, s, st);
+        launch_resid<double>(S, A, b, x, r, nullptr, r32, st, FIN_INIT_NORM);
+        outer++;
+    }
+    if (stats) {
+        stats->inner_iterations = inner_total;
+        stats->outer_iterations = outer;
+    }
+}
+
+} // namespace
+
+__global__ void k_norm2_plain(int n, const double *__restrict__ v, double *out,
+                              RedSlot slot)
+{
+    double a = 0;
+    for (int i = blockIdx.x * RS_VBS + threadIdx.x; i < n; i += gridDim.x * RS_VBS)
+        a += v[i] * v[i];
+    double vv[1] = {a}, tot[1];
+    if (block_reduce_finish<RS_VBS, 1>(slot, vv, tot) && threadIdx.x == 0)
+        *out = sqrt(tot[0]);
+}
+
+namespace {
+double dev_norm2(rsgpu_solver *S, const double *v, int n)
+{
+    hipStream_t s = S->ctx->stream;
+    double *out = S->partials.p + S->partials.n - 1;
+    // partials are sized for the largest grid; the last slot is free
+    RedSlot sl{S->partials.p, S->ticket.p + 1};
+    hipLaunchKernelGGL(k_norm2_plain, dim3(grid_for(n)), dim3(RS_VBS), 0, s, n, v,
+                       out, sl);
+    RS_CHECK_LAUNCH();
+    double h = 0;
+    RS_HIP(hipMemcpyAsync(&h, out, sizeof(h), hipMemcpyDeviceToHost, s));
+    RS_HIP(hipStreamSynchronize(s));
+    return h;
+}
+} // namespace
+
+extern "C" int rsgpu_solver_solve(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M,
+                                  const double *b_dev, double *x_dev,
+                                  rsgpu_stats *stats)
+{
+    if (!S || !A)
+        return RSGPU_BAD_ARGUMENT;
+    rsgpu_ctx *ctx = S->ctx;
+    if (A->n != A->ncols) {
+        ctx->err = "!A.isSquare()";
+        return RSGPU_BAD_ARGUMENT;
+    }
+    if (A->n != S->n) {
+        ctx->err = "b.size() != A.numRows()";
+        return RSGPU_BAD_ARGUMENT;
+    }
+    rsgpu_prec ident;
+    ident.ctx = ctx;
+    ident.kind = RSGPU_PREC_IDENTITY;
+    ident.n = A->n;
+    ident.ready = true;
+    if (!M)
+        M = &ident;
+    if (M->kind != RSGPU_PREC_IDENTITY && (!M->ready || M->pattern_id != A->pattern_id)) {
+        ctx->err = "Preconditioner.setMatrix must be called before use";
+        return RSGPU_BAD_ARGUMENT;
+    }
+    try {
+        RS_HIP(hipSetDevice(ctx->device));
+        const int m = std::max(1, S->o.restart);
+        const size_t need = (size_t)2 * std::max(A->plan.nwg, grid_for(A->n)) + 2;
+        if (S->partials.n < need + 4096)
+            S->partials.alloc(need + 4096);
+        state_init(S, S->st, S->hist, S->o.max_iter, S->o.rtol, S->o.atol,
+                   S->o.dtol, m, S->gmres_small.p);
+        if (stats)
+            std::memset(stats, 0, sizeof(*stats));
+        if (S->o.mixed) {
+            run_mixed(S, A, M, b_dev, x_dev, S->st, stats);
+        } else if (S->o.kind == RSGPU_IR) {
+            run_ir(S, A, M, b_dev, x_dev, S->st);
+        } else {
+            run_kind<double>(S, S->o.kind, A, M, b_dev, x_dev, S->w64.p, S->st, m);
+        }
+        DevState h;
+        RS_HIP(hipMemcpyAsync(&h, S->st, sizeof(h), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        RS_HIP(hipStreamSynchronize(ctx->stream));
+        S->last_hist = std::min(h.hist_len, h.hist_cap);
+        if (stats) {
+            stats->iterations = h.iter;
+            stats->residual = h.residual;
+            stats->status = h.status;
+            stats->hist_len = h.hist_len;
+            // true residual ||b - A x|| in FP64
+            double *r = S->w64.p;
+            spmv_launch<double>(ctx->stream, A, A->val.p, x_dev, r, 2, -1.0, b_dev);
+            stats->true_residual = dev_norm2(S, r, A->n);
+            stats->b_norm = dev_norm2(S, b_dev, A->n);
+        }
+        return h.status;
+    } catch (const RsError &e) {
+        return solver_catch(ctx, e);
+    }
+}
+
+extern "C" int rsgpu_solver_history(const rsgpu_solver *S, double *out, int cap)
+{
+    if (!S || !out)
+        return RSGPU_BAD_ARGUMENT;
+    const int k = std::min(cap, S->last_hist);
+    if (k > 0) {
+        if (hipMemcpy(out, S->hist.p, sizeof(double) * k, hipMemcpyDeviceToHost) !=
+            hipSuccess)
+            return RSGPU_HIP_ERROR;
+    }
+    return k;
+}

@@ -1,0 +1,327 @@
+// Internal definitions shared by the rsgpu HIP translation units.
+// Target: gfx950 (CDNA4), wave64.  All arithmetic that must agree bit for
+// bit with the reference's sequential Java loops is compiled with
+// -ffp-contract=off and keeps the reference's per-row summation order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rsgpu.h"
+
+#define RS_WAVE 64
+
+// ---------------------------------------------------------------- errors
+struct RsError {
+    int code;
+    std::string msg;
+};
+
+#define RS_HIP(call)                                                          \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess)                                                 \
+            throw RsError{e_ == hipErrorOutOfMemory ? RSGPU_OUT_OF_MEMORY     \
+                                                    : RSGPU_HIP_ERROR,        \
+                          std::string(#call) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+#define RS_CHECK_LAUNCH() RS_HIP(hipGetLastError())
+
+// ---------------------------------------------------------------- device buffer
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release()
+    {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count)
+    {
+        if (count == n && p)
+            return;
+        release();
+        if (count) {
+            RS_HIP(hipMalloc((void **)&p, sizeof(T) * count));
+            n = count;
+        }
+    }
+    void upload(const T *h, size_t count, hipStream_t s)
+    {
+        alloc(count);
+        if (count)
+            RS_HIP(hipMemcpyAsync(p, h, sizeof(T) * count,
+                                  hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T> &v, hipStream_t s)
+    {
+        upload(v.data(), v.size(), s);
+    }
+    void zero(hipStream_t s)
+    {
+        if (n)
+            RS_HIP(hipMemsetAsync(p, 0, sizeof(T) * n, s));
+    }
+};
+
+// ---------------------------------------------------------------- handles
+struct rsgpu_ctx {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // pinned host staging for small device->host reads
+    void *pinned = nullptr;
+};
+
+struct SpmvPlan {
+    // contiguous row ranges per workgroup; long rows get their own group
+    DevBuf<int> wg_rows;
+    int nwg = 0;
+};
+
+struct rsgpu_csr {
+    rsgpu_ctx *ctx = nullptr;
+    int n = 0, ncols = 0, nnz = 0;
+    uint64_t pattern_id = 0;
+    std::vector<int> h_rowptr, h_colind;
+    DevBuf<int> rowptr, colind;
+    DevBuf<double> val;
+    DevBuf<float> val32;
+    SpmvPlan plan;
+    // transposed copy (lazy): rows = columns of A, values gathered via tperm
+    bool has_t = false;
+    DevBuf<int> trowptr, tcolind, tperm;
+    SpmvPlan tplan;
+    DevBuf<double> tval;
+};
+
+// Level-scheduled sweep data of one triangular factor (see ilu.hip)
+struct SweepData {
+    int nlev_total = 0;      // sum over blocks
+    int max_levels = 0;      // per block
+    DevBuf<int> blk_lev;     // [nblocks+1] level ranges per block
+    DevBuf<int> lev_meta;    // per level: count, width, ell offset, row offset
+    DevBuf<int> rows;        // local row per (level,t)
+    DevBuf<int> ell_col;     // local column per slot (nb = zero slot)
+    DevBuf<int> ell_src;     // source position in LU, -1 padding
+    size_t ell_size = 0;
+    int max_count = 0, max_width = 0;
+};
+
+struct rsgpu_prec {
+    rsgpu_ctx *ctx = nullptr;
+    int kind = 0, nblocks = 1, precision = RSGPU_FP64;
+    int n = 0;
+    uint64_t pattern_id = 0;
+    bool ready = false;
+    // block structure
+    std::vector<int> h_bstart;
+    DevBuf<int> bstart;
+    int max_block_rows = 0;
+    // factor pattern (A's pattern or its block-diagonal part)
+    int nnz = 0;
+    std::vector<int> h_rowptr, h_colind;
+    DevBuf<int> rowptr, colind, diag, src;   // src: position in A (block filter)
+    bool src_identity = true;
+    DevBuf<int> fac_rows;       // rows in factor level order, per block
+    DevBuf<int> fac_lev;        // level row ranges
+    DevBuf<int> fac_blk_lev;    // per block level ranges
+    int fac_levels_max = 0;
+    DevBuf<double> lu64;
+    DevBuf<float> lu32;
+    DevBuf<int> status;          // [2]: code, bad row
+    SweepData L, U, Lt, Ut;      // forward/backward and transposed sweeps
+    bool has_t = false;
+    DevBuf<double> ell64_L, ell64_U, ell64_Lt, ell64_Ut, dg64;
+    DevBuf<float> ell32_L, ell32_U, dg32;
+    DevBuf<double> scratch64;    // global fallback vectors for large blocks
+    DevBuf<float> scratch32;
+    DevBuf<double> tmp64;        // FP32 apply on FP64 vectors
+    DevBuf<float> tmpa32, tmpb32;
+    int lds_rows = 0;            // rows that fit the LDS vector
+};
+
+// ---------------------------------------------------------------- device state
+enum {
+    ST_FLAG_DONE = 1,
+};
+
+// Scalars of an iterative solve kept on the device so that iterations can
+// be enqueued without host round trips.  Layout shared by all solvers.
+struct DevState {
+    // monitor (DefaultIterationMonitor)
+    int iter;
+    int status;
+    int done;
+    int conv;
+    int hist_len;
+    int hist_cap;
+    int max_iter;
+    int half;           // BiCGstab: s converged, x += alpha*phat pending
+    int first;
+    int cycle_end;      // GMRES: inner loop finished
+    int gi;             // GMRES: inner index i
+    int pad0;
+    double initR, rtol, atol, dtol, residual;
+    double *hist;
+    // BiCGstab
+    double rho1, rho2, alpha, beta, omega;
+    double ts, tt;
+    // GMRES(m): Hessenberg (m+1) x m column-major, rotated rhs s, Givens
+    double normr;
+    double hsub;        // unrotated H(i+1,i) for the basis scaling
+    double *H, *s, *gc, *gs;
+    int m;
+    int pad1;
+};
+
+// ---------------------------------------------------------------- reductions
+// Deterministic two-level reduction: fixed per-thread strides, wave64 shuffle
+// trees, per-workgroup partials, and a finishing workgroup chosen by an
+// arrival ticket that sums the partials in a fixed order and resets the
+// ticket for the next launch.
+struct RedSlot {
+    double *partials;   // [max_wg * 2]
+    unsigned *ticket;
+};
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+template <int BS>
+__device__ __forceinline__ double block_sum(double v, double *sh)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (BS > 64) {
+        if (lane == 0)
+            sh[w] = v;
+        __syncthreads();
+        v = 0;
+        if (threadIdx.x < 64) {
+            v = threadIdx.x < BS / 64 ? sh[threadIdx.x] : 0.0;
+            v = wave_sum(v);
+        }
+        __syncthreads();
+    }
+    return v;   // valid in thread 0
+}
+
+// Publish this group's partial(s); returns true in every thread of the
+// finishing group, whose thread 0 then holds the totals in tot[].
+template <int BS, int NR>
+__device__ bool block_reduce_finish(RedSlot slot, const double (&v)[NR],
+                                    double (&tot)[NR])
+{
+    __shared__ double sh[BS / 64 + 1];
+    __shared__ int last;
+    double r[NR];
+#pragma unroll
+    for (int q = 0; q < NR; ++q)
+        r[q] = block_sum<BS>(v[q], sh);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NR; ++q)
+            slot.partials[(size_t)q * gridDim.x + blockIdx.x] = r[q];
+        __threadfence();
+        unsigned t = atomicAdd(slot.ticket, 1u);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last)
+        return false;
+    __threadfence();
+#pragma unroll
+    for (int q = 0; q < NR; ++q) {
+        double acc = 0;
+        for (int i = threadIdx.x; i < (int)gridDim.x; i += BS)
+            acc += slot.partials[(size_t)q * gridDim.x + i];
+        tot[q] = block_sum<BS>(acc, sh);
+    }
+    if (threadIdx.x == 0)
+        *slot.ticket = 0u;
+    return true;
+}
+
+// java.lang.Math.max (NaN wins)
+__device__ __forceinline__ double java_max(double a, double b)
+{
+    if (a != a)
+        return a;
+    if (b != b)
+        return b;
+    return a > b ? a : b;
+}
+
+// AbstractIterationMonitor.converged + DefaultIterationMonitor.convergedI.
+// Returns true when the solve must stop (converged or a failure status).
+__device__ __forceinline__ bool dev_monitor(DevState *st, double r)
+{
+    if (st->hist && st->hist_len < st->hist_cap)
+        st->hist[st->hist_len] = r;
+    st->hist_len++;
+    st->residual = r;
+    if (st->iter == 0)
+        st->initR = r;
+    if (r < java_max(st->rtol * st->initR, st->atol)) {
+        st->conv = 1;
+        return true;
+    }
+    if (r > st->dtol * st->initR) {
+        st->status = RSGPU_DIVERGENCE;
+        return true;
+    }
+    if (st->iter >= st->max_iter) {
+        st->status = RSGPU_ITERATIONS;
+        return true;
+    }
+    if (r != r) {
+        st->status = RSGPU_DIVERGENCE;
+        return true;
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------- host API
+// spmv.hip
+void spmv_build_plan(const std::vector<int> &rowptr, int n, SpmvPlan &plan,
+                     hipStream_t s);
+template <typename T>
+void spmv_launch(hipStream_t s, const rsgpu_csr *A, const T *val, const T *x,
+                 T *y, int mode, double alpha, const T *b);
+void csr_build_transpose(rsgpu_csr *A);
+void csr_transmult_launch(rsgpu_csr *A, double alpha, const double *x,
+                          double *y, bool add);
+void csr_launch_gather(hipStream_t s, const int *perm, const double *src,
+                       double *dst, int n);
+void csr_zero_rows_dev(rsgpu_csr *A, double diagonal, const int *rows_dev,
+                       int nrows);
+void convert_f64_to_f32(hipStream_t s, const double *a, float *b, size_t n);
+void convert_f32_to_f64(hipStream_t s, const float *a, double *b, size_t n);
+
+// ilu.hip
+void prec_setup(rsgpu_prec *M, rsgpu_csr *A);
+int prec_factor(rsgpu_prec *M, rsgpu_csr *A, int *bad_row);
+void prec_apply(rsgpu_prec *M, const double *b, double *x, const DevState *st);
+void prec_apply(rsgpu_prec *M, const float *b, float *x, const DevState *st);
+void prec_trans_apply(rsgpu_prec *M, const double *b, double *x);
+
+uint64_t next_pattern_id();

@@ -1,0 +1,221 @@
+// CSR SpMV entry kernels, transposed copy, zeroRows, conversions.
+#include <algorithm>
+#include <numeric>
+
+#include "spmv_core.hpp"
+
+// Greedy contiguous row partition: a group takes rows until the next one
+// would overflow the LDS stage or the lane count; a longer row is alone.
+void spmv_build_plan(const std::vector<int> &rowptr, int n, SpmvPlan &plan,
+                     hipStream_t s)
+{
+    std::vector<int> starts;
+    starts.reserve(n / 16 + 2);
+    int r = 0;
+    while (r < n) {
+        starts.push_back(r);
+        const int j0 = rowptr[r];
+        int e = r + 1;
+        if (rowptr[r + 1] - j0 <= RS_SPMV_NNZ)
+            while (e < n && e - r < RS_SPMV_BS &&
+                   rowptr[e + 1] - j0 <= RS_SPMV_NNZ)
+                ++e;
+        r = e;
+    }
+    starts.push_back(n);
+    plan.nwg = (int)starts.size() - 1;
+    plan.wg_rows.upload(starts, s);
+}
+
+template <typename T, class Epi>
+__global__ void __launch_bounds__(RS_SPMV_BS)
+    k_spmv(const int *__restrict__ wg_rows, const int *__restrict__ rowptr,
+           const int *__restrict__ colind, const T *__restrict__ val,
+           const T *__restrict__ x, Epi epi)
+{
+    spmv_core<T, RS_SPMV_BS, RS_SPMV_NNZ>(wg_rows, rowptr, colind, val, x,
+                                          epi);
+}
+
+// mode: 0 mult, 1 multAdd(alpha), 2 residual b + alpha*Ax, 3 plain sum,
+// 4 transMultAdd(alpha) accumulator form
+template <typename T>
+void spmv_launch(hipStream_t s, const rsgpu_csr *A, const T *val, const T *x,
+                 T *y, int mode, double alpha, const T *b)
+{
+    if (A->n == 0)
+        return;
+    dim3 g(A->plan.nwg), bl(RS_SPMV_BS);
+    const int *wr = A->plan.wg_rows.p, *rp = A->rowptr.p, *ci = A->colind.p;
+    switch (mode) {
+    case 0:
+        hipLaunchKernelGGL((k_spmv<T, EpiMult<T>>), g, bl, 0, s, wr, rp, ci,
+                           val, x, EpiMult<T>{y});
+        break;
+    case 1:
+        hipLaunchKernelGGL((k_spmv<T, EpiMultAdd<T>>), g, bl, 0, s, wr, rp, ci,
+                           val, x, EpiMultAdd<T>{y, (T)alpha});
+        break;
+    case 2:
+        hipLaunchKernelGGL((k_spmv<T, EpiResid<T>>), g, bl, 0, s, wr, rp, ci,
+                           val, x, EpiResid<T>{y, b, (T)alpha});
+        break;
+    default:
+        hipLaunchKernelGGL((k_spmv<T, EpiPlain<T>>), g, bl, 0, s, wr, rp, ci,
+                           val, x, EpiPlain<T>{y});
+        break;
+    }
+    RS_CHECK_LAUNCH();
+}
+
+template void spmv_launch<double>(hipStream_t, const rsgpu_csr *,
+                                  const double *, const double *, double *,
+                                  int, double, const double *);
+template void spmv_launch<float>(hipStream_t, const rsgpu_csr *,
+                                 const float *, const float *, float *, int,
+                                 double, const float *);
+
+// ---------------------------------------------------------------- transpose
+__global__ void k_gather(const int *__restrict__ perm,
+                         const double *__restrict__ src,
+                         double *__restrict__ dst, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        dst[i] = src[perm[i]];
+}
+
+void csr_launch_gather(hipStream_t s, const int *perm, const double *src,
+                       double *dst, int n)
+{
+    if (n <= 0)
+        return;
+    hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, s, perm,
+                       src, dst, n);
+    RS_CHECK_LAUNCH();
+}
+
+// Transposed pattern: row c of A' lists the rows i with a(i,c) != 0 in
+// ascending i, the order in which transMult's row-by-row scatter
+// (CompRowMatrix.java:356-372) adds into y[c].
+void csr_build_transpose(rsgpu_csr *A)
+{
+    if (A->has_t)
+        return;
+    hipStream_t s = A->ctx->stream;
+    const int n = A->n, m = A->ncols, nnz = A->nnz;
+    std::vector<int> trp(m + 1, 0), tci(nnz), tpm(nnz);
+    for (int j = 0; j < nnz; ++j)
+        trp[A->h_colind[j] + 1]++;
+    for (int c = 0; c < m; ++c)
+        trp[c + 1] += trp[c];
+    std::vector<int> fill(trp.begin(), trp.end() - 1);
+    for (int i = 0; i < n; ++i)
+        for (int j = A->h_rowptr[i]; j < A->h_rowptr[i + 1]; ++j) {
+            int c = A->h_colind[j];
+            int p = fill[c]++;
+            tci[p] = i;
+            tpm[p] = j;
+        }
+    A->trowptr.upload(trp, s);
+    A->tcolind.upload(tci, s);
+    A->tperm.upload(tpm, s);
+    A->tval.alloc(std::max(nnz, 1));
+    spmv_build_plan(trp, m, A->tplan, s);
+    RS_HIP(hipStreamSynchronize(s));
+    A->has_t = true;
+}
+
+template <class Epi>
+__global__ void __launch_bounds__(RS_SPMV_BS)
+    k_spmv_t(const int *__restrict__ wg_rows, const int *__restrict__ rowptr,
+             const int *__restrict__ colind, const double *__restrict__ val,
+             const double *__restrict__ x, Epi epi)
+{
+    spmv_core<double, RS_SPMV_BS, RS_SPMV_NNZ>(wg_rows, rowptr, colind, val, x,
+                                               epi);
+}
+
+void csr_transmult_launch(rsgpu_csr *A, double alpha, const double *x,
+                          double *y, bool add)
+{
+    csr_build_transpose(A);
+    hipStream_t s = A->ctx->stream;
+    if (A->ncols == 0)
+        return;
+    csr_launch_gather(s, A->tperm.p, A->val.p, A->tval.p, A->nnz);
+    dim3 g(A->tplan.nwg), bl(RS_SPMV_BS);
+    if (!add)
+        hipLaunchKernelGGL((k_spmv_t<EpiPlain<double>>), g, bl, 0, s,
+                           A->tplan.wg_rows.p, A->trowptr.p, A->tcolind.p,
+                           A->tval.p, x, EpiPlain<double>{y});
+    else
+        hipLaunchKernelGGL((k_spmv_t<EpiTransAdd<double>>), g, bl, 0, s,
+                           A->tplan.wg_rows.p, A->trowptr.p, A->tcolind.p,
+                           A->tval.p, x,
+                           EpiTransAdd<double>{y, alpha, 1. / alpha});
+    RS_CHECK_LAUNCH();
+}
+
+// ---------------------------------------------------------------- zeroRows
+__global__ void k_zero_rows(const int *__restrict__ rowptr,
+                            const int *__restrict__ colind, double *val,
+                            const int *__restrict__ rows, int nrows,
+                            double diagonal)
+{
+    // one wave per listed row
+    const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (w >= nrows)
+        return;
+    const int r = rows[w];
+    for (int j = rowptr[r] + lane; j < rowptr[r + 1]; j += 64)
+        val[j] = (colind[j] == r) ? diagonal : 0.0;
+}
+
+void csr_zero_rows_dev(rsgpu_csr *A, double diagonal, const int *rows_dev,
+                       int nrows)
+{
+    if (nrows <= 0)
+        return;
+    const int threads = nrows * 64;
+    hipLaunchKernelGGL(k_zero_rows, dim3((threads + 255) / 256), dim3(256), 0,
+                       A->ctx->stream, A->rowptr.p, A->colind.p, A->val.p,
+                       rows_dev, nrows, diagonal);
+    RS_CHECK_LAUNCH();
+}
+
+// ---------------------------------------------------------------- conversions
+__global__ void k_f64_to_f32(const double *__restrict__ a,
+                             float *__restrict__ b, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        b[i] = (float)a[i];
+}
+
+__global__ void k_f32_to_f64(const float *__restrict__ a,
+                             double *__restrict__ b, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        b[i] = (double)a[i];
+}
+
+void convert_f64_to_f32(hipStream_t s, const double *a, float *b, size_t n)
+{
+    if (!n)
+        return;
+    hipLaunchKernelGGL(k_f64_to_f32, dim3((unsigned)((n + 255) / 256)),
+                       dim3(256), 0, s, a, b, n);
+    RS_CHECK_LAUNCH();
+}
+
+void convert_f32_to_f64(hipStream_t s, const float *a, double *b, size_t n)
+{
+    if (!n)
+        return;
+    hipLaunchKernelGGL(k_f32_to_f64, dim3((unsigned)((n + 255) / 256)),
+                       dim3(256), 0, s, a, b, n);
+    RS_CHECK_LAUNCH();
+}

@@ -1,0 +1,107 @@
+// CSR SpMV core shared by the plain SpMV entry points and the fused Krylov
+// kernels.
+//
+// Layout: each workgroup owns a contiguous row range whose nonzeros fit the
+// LDS stage (plan built on the host, spmv.hip).  Phase 1 streams values,
+// column indices and the gathered x coalesced (one nonzero per lane, NNZ_WG
+// per group) and stores the products val[j]*x[col[j]] in LDS.  Phase 2 gives
+// one row per lane and sums that row's products left to right, which is the
+// statement order of CompRowMatrix.mult/multAdd
+// (mtj/src/no/uib/cipr/matrix/sparse/CompRowMatrix.java:301-353); with
+// -ffp-contract=off the per-row result is bit-identical to the sequential
+// loop.  A row longer than the stage is summed chunk by chunk by one lane,
+// again in order.
+#pragma once
+
+#include "rsgpu_impl.hpp"
+
+#define RS_SPMV_BS 256
+#define RS_SPMV_NNZ 2048
+
+template <typename T, int BS, int NNZ_WG, class Epi>
+__device__ __forceinline__ void spmv_core(const int *__restrict__ wg_rows,
+                                          const int *__restrict__ rowptr,
+                                          const int *__restrict__ colind,
+                                          const T *__restrict__ val,
+                                          const T *__restrict__ x, Epi &epi)
+{
+    __shared__ T prod[NNZ_WG];
+    const int r0 = wg_rows[blockIdx.x], r1 = wg_rows[blockIdx.x + 1];
+    const int j0 = rowptr[r0], j1 = rowptr[r1];
+    if (j1 - j0 <= NNZ_WG) {
+#pragma unroll
+        for (int q = 0; q < NNZ_WG / BS; ++q) {
+            const int j = j0 + (int)threadIdx.x + q * BS;
+            if (j < j1)
+                prod[j - j0] = val[j] * x[colind[j]];
+        }
+        __syncthreads();
+        for (int r = r0 + (int)threadIdx.x; r < r1; r += BS) {
+            const int a = rowptr[r] - j0, e = rowptr[r + 1] - j0;
+            T dot = epi.init(r);
+            for (int j = a; j < e; ++j)
+                dot += prod[j];
+            epi.row(r, dot);
+        }
+    } else {
+        // a single row (the plan isolates rows longer than the stage)
+        T dot = epi.init(r0);
+        for (int c = j0; c < j1; c += NNZ_WG) {
+            const int ce = c + NNZ_WG < j1 ? c + NNZ_WG : j1;
+            for (int j = c + (int)threadIdx.x; j < ce; j += BS)
+                prod[j - c] = val[j] * x[colind[j]];
+            __syncthreads();
+            if (threadIdx.x == 0)
+                for (int j = 0; j < ce - c; ++j)
+                    dot += prod[j];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0)
+            epi.row(r0, dot);
+    }
+}
+
+// y = A x with CompRowMatrix.mult's rule: y[i] = dot only if dot != 0.
+template <typename T>
+struct EpiMult {
+    T *y;
+    __device__ T init(int) const { return T(0); }
+    __device__ void row(int r, T dot) { y[r] = (dot != T(0)) ? dot : T(0); }
+};
+
+// y += alpha * (A x)   (multAdd)
+template <typename T>
+struct EpiMultAdd {
+    T *y;
+    T alpha;
+    __device__ T init(int) const { return T(0); }
+    __device__ void row(int r, T dot) { y[r] += alpha * dot; }
+};
+
+// y = b + alpha*(A x)  (multAdd on y.set(b): the residual form)
+template <typename T>
+struct EpiResid {
+    T *y;
+    const T *b;
+    T alpha;
+    __device__ T init(int) const { return T(0); }
+    __device__ void row(int r, T dot) { y[r] = b[r] + alpha * dot; }
+};
+
+// y = sum (no zero rule; used for the transposed gather)
+template <typename T>
+struct EpiPlain {
+    T *y;
+    __device__ T init(int) const { return T(0); }
+    __device__ void row(int r, T dot) { y[r] = dot; }
+};
+
+// transMultAdd: y = alpha * (y * (1/alpha) + A'x), the accumulator starts
+// at y*(1/alpha) (CompRowMatrix.java:375-394).
+template <typename T>
+struct EpiTransAdd {
+    T *y;
+    T alpha, inv;
+    __device__ T init(int r) const { return y[r] * inv; }
+    __device__ void row(int r, T dot) { y[r] = dot * alpha; }
+};

@@ -1,0 +1,588 @@
+"""Host-side mirror of MTJ's sparse solver interfaces over the rsgpu C ABI.
+
+Names, argument meaning and error behaviour follow
+mtj/src/no/uib/cipr/matrix/sparse/{IterativeSolver,Preconditioner,
+CompRowMatrix,ILU,BiCGstab,GMRES,IR,DefaultIterationMonitor,
+IterativeSolverNotConvergedException}.java and
+mtj/src/no/uib/cipr/matrix/Matrices.java (zeroRows), so code written against
+MTJ reads the same:
+
+    A = CompRowMatrix(n, n, nz)            # int[][] nz, rows sorted as MTJ does
+    A.set_data(values)                     # or A.add(i, j, v) batches
+    M = ILU(A)                             # Preconditioner
+    solver = BiCGstab(b)                   # template vector
+    solver.setPreconditioner(M)
+    solver.setIterationMonitor(DefaultIterationMonitor(5000, 1e-10, 1e-50, 1e5))
+    M.setMatrix(A)
+    solver.solve(A, b, x)                  # x: initial guess and result
+
+Vectors are numpy float64 arrays (host; copied through the *_host entry
+points, as a JNI binding would pass Java double[]) or torch CUDA float64
+tensors (device; passed by pointer).  Every computation runs in librsgpu.so
+on the GPU; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import enum
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib as L
+
+
+class NotConvergedReason(enum.Enum):
+    """NotConvergedException.Reason (mtj/.../NotConvergedException.java:38-48)."""
+    Iterations = L.ITERATIONS
+    Divergence = L.DIVERGENCE
+    Breakdown = L.BREAKDOWN
+
+
+class IterativeSolverNotConvergedException(Exception):
+    """mtj/.../sparse/IterativeSolverNotConvergedException.java:54-89."""
+
+    def __init__(self, reason: NotConvergedReason, residual: float, iterations: int,
+                 message: str = ""):
+        super().__init__(message or f"{reason.name} (residual {residual:g}, "
+                                    f"{iterations} iterations)")
+        self.reason = reason
+        self.residual = residual
+        self.iterations = iterations
+
+    def getReason(self):
+        return self.reason
+
+    def getResidual(self):
+        return self.residual
+
+    def getIterations(self):
+        return self.iterations
+
+
+class MatrixSingularException(RuntimeError):
+    """mtj/.../MatrixSingularException.java (GMRES back-solve)."""
+
+
+class RsgpuError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"rsgpu status {code}: {msg}")
+        self.code = code
+
+
+def _raise(code: int, ctx_handle, what: str = ""):
+    if code == L.OK:
+        return
+    msg = L.load().rsgpu_last_error(ctx_handle).decode(errors="replace") if ctx_handle else ""
+    msg = msg or what
+    if code == L.NOT_IN_PATTERN:
+        raise IndexError(msg)
+    if code in (L.ZERO_PIVOT, L.MISSING_DIAGONAL):
+        raise RuntimeError(msg)
+    if code == L.BAD_ARGUMENT:
+        raise ValueError(msg)
+    if code == L.SINGULAR:
+        raise MatrixSingularException(msg)
+    raise RsgpuError(code, msg)
+
+
+# ------------------------------------------------------------------ context
+class Context:
+    """One HIP device + stream.  ``Context.default()`` is shared."""
+    _default = {}
+
+    def __init__(self, device: int = 0):
+        lib = L.load()
+        h = C.c_void_p()
+        _raise(lib.rsgpu_ctx_create(device, C.byref(h)), None, "ctx_create failed")
+        self.h = h
+        self.device = device
+
+    @classmethod
+    def default(cls, device: int = 0) -> "Context":
+        if device not in cls._default:
+            cls._default[device] = Context(device)
+        return cls._default[device]
+
+    def synchronize(self):
+        _raise(L.load().rsgpu_ctx_synchronize(self.h), self.h)
+
+    def set_stream(self, stream_handle: Optional[int]):
+        _raise(L.load().rsgpu_ctx_set_stream(self.h, stream_handle), self.h)
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                L.load().rsgpu_ctx_destroy(self.h)
+        except Exception:
+            pass
+
+
+def _is_device(v) -> bool:
+    return hasattr(v, "is_cuda") and v.is_cuda
+
+
+def _dptr(v):
+    return C.c_void_p(v.data_ptr())
+
+
+def _hptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _torch_sync(*vs):
+    for v in vs:
+        if _is_device(v):
+            import torch
+            torch.cuda.synchronize(v.device)
+            return
+
+
+# ------------------------------------------------------------------ CompRowMatrix
+class CompRowMatrix:
+    """mtj/.../sparse/CompRowMatrix.java on the GPU.
+
+    ``CompRowMatrix(n, m, nz)`` with nz a list of per-row column lists
+    (:160-190), or ``CompRowMatrix.from_csr(n, m, rowptr, colind)``.
+    """
+
+    def __init__(self, numRows: int, numColumns: int, nz: Sequence[Sequence[int]],
+                 ctx: Optional[Context] = None):
+        if len(nz) != numRows:
+            raise ValueError("nz.length != numRows")
+        rowptr = np.zeros(numRows + 1, np.int32)
+        rowptr[1:] = np.cumsum([len(r) for r in nz])
+        colind = (np.concatenate([np.asarray(r, np.int32) for r in nz])
+                  if rowptr[-1] else np.zeros(0, np.int32))
+        self._create(numRows, numColumns, rowptr, colind, ctx)
+
+    @classmethod
+    def from_csr(cls, numRows, numColumns, rowptr, colind, data=None,
+                 ctx: Optional[Context] = None) -> "CompRowMatrix":
+        self = cls.__new__(cls)
+        self._create(numRows, numColumns, np.ascontiguousarray(rowptr, np.int32),
+                     np.ascontiguousarray(colind, np.int32), ctx)
+        if data is not None:
+            self.set_data(data)
+        return self
+
+    @classmethod
+    def _wrap(cls, handle, n, m, nnz, ctx) -> "CompRowMatrix":
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.h = handle
+        self.n, self.m, self.nnz = n, m, nnz
+        return self
+
+    def _create(self, n, m, rowptr, colind, ctx):
+        self.ctx = ctx or Context.default()
+        lib = L.load()
+        h = C.c_void_p()
+        rowptr = np.ascontiguousarray(rowptr, np.int32)
+        colind = np.ascontiguousarray(colind, np.int32)
+        nnz = int(rowptr[-1]) if len(rowptr) else 0
+        _raise(lib.rsgpu_csr_create(self.ctx.h, n, m, nnz, _hptr(rowptr),
+                                    _hptr(colind) if nnz else None, C.byref(h)),
+               self.ctx.h)
+        self.h = h
+        self.n, self.m, self.nnz = n, m, nnz
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                L.load().rsgpu_csr_destroy(self.h)
+        except Exception:
+            pass
+
+    def numRows(self):
+        return self.n
+
+    def numColumns(self):
+        return self.m
+
+    def isSquare(self):
+        return self.n == self.m
+
+    def getRowPointers(self) -> np.ndarray:
+        rp = np.zeros(self.n + 1, np.int32)
+        L.load().rsgpu_csr_get_pattern(self.h, _hptr(rp), None)
+        return rp
+
+    def getColumnIndices(self) -> np.ndarray:
+        ci = np.zeros(max(self.nnz, 1), np.int32)
+        L.load().rsgpu_csr_get_pattern(self.h, None, _hptr(ci))
+        return ci[: self.nnz]
+
+    def getData(self) -> np.ndarray:
+        d = np.zeros(max(self.nnz, 1))
+        _raise(L.load().rsgpu_csr_get_values(self.h, _hptr(d)), self.ctx.h)
+        return d[: self.nnz]
+
+    def set_data(self, data):
+        """Replace the values (getData() array) wholesale."""
+        if _is_device(data):
+            _torch_sync(data)
+            _raise(L.load().rsgpu_csr_set_values_dev(self.h, _dptr(data)), self.ctx.h)
+            self.ctx.synchronize()
+            return
+        d = np.ascontiguousarray(data, np.float64)
+        if d.size != self.nnz:
+            raise ValueError("data length != nnz")
+        _raise(L.load().rsgpu_csr_set_values(self.h, _hptr(d)), self.ctx.h)
+
+    def zero(self):
+        _raise(L.load().rsgpu_csr_zero(self.h), self.ctx.h)
+        return self
+
+    def add(self, row, column, value):
+        """add(row, column, value); arrays apply a batch in array order."""
+        r = np.atleast_1d(np.asarray(row, np.int32))
+        c = np.atleast_1d(np.asarray(column, np.int32))
+        v = np.atleast_1d(np.asarray(value, np.float64))
+        _raise(L.load().rsgpu_csr_add_batch(self.h, r.size, _hptr(r), _hptr(c), _hptr(v)),
+               self.ctx.h)
+
+    def sync_f32(self):
+        _raise(L.load().rsgpu_csr_sync_f32(self.h), self.ctx.h)
+
+    def mult(self, x, y):
+        """y = A x (CompRowMatrix.java:301-333)."""
+        lib = L.load()
+        if _is_device(x):
+            _torch_sync(x, y)
+            _raise(lib.rsgpu_csr_mult(self.h, _dptr(x), _dptr(y)), self.ctx.h)
+            self.ctx.synchronize()
+            return y
+        xx = np.ascontiguousarray(x, np.float64)
+        _raise(lib.rsgpu_csr_mult_host(self.h, _hptr(xx), _hptr(y)), self.ctx.h)
+        return y
+
+    def multAdd(self, alpha, x, y):
+        """y += alpha A x (device tensors)."""
+        _torch_sync(x, y)
+        _raise(L.load().rsgpu_csr_multadd(self.h, alpha, _dptr(x), _dptr(y)), self.ctx.h)
+        self.ctx.synchronize()
+        return y
+
+    def transMult(self, x, y):
+        lib = L.load()
+        if _is_device(x):
+            _torch_sync(x, y)
+            _raise(lib.rsgpu_csr_transmult(self.h, _dptr(x), _dptr(y)), self.ctx.h)
+            self.ctx.synchronize()
+            return y
+        xx = np.ascontiguousarray(x, np.float64)
+        _raise(lib.rsgpu_csr_transmult_host(self.h, _hptr(xx), _hptr(y)), self.ctx.h)
+        return y
+
+    def transMultAdd(self, alpha, x, y):
+        _torch_sync(x, y)
+        _raise(L.load().rsgpu_csr_transmultadd(self.h, alpha, _dptr(x), _dptr(y)),
+               self.ctx.h)
+        self.ctx.synchronize()
+        return y
+
+
+class Matrices:
+    @staticmethod
+    def zeroRows(A: CompRowMatrix, diagonal: float, *rows):
+        """mtj/.../Matrices.java:631-652."""
+        if len(rows) == 1 and not np.isscalar(rows[0]):
+            rows = rows[0]
+        r = np.ascontiguousarray(np.asarray(rows, np.int32).reshape(-1))
+        _raise(L.load().rsgpu_csr_zero_rows(A.h, diagonal, _hptr(r) if r.size else None,
+                                            r.size), A.ctx.h)
+
+
+# ------------------------------------------------------------------ preconditioners
+class Preconditioner:
+    """mtj/.../sparse/Preconditioner.java:42-63."""
+    kind = L.PREC_IDENTITY
+
+    def __init__(self, nblocks: int = 1, precision: int = L.FP64,
+                 ctx: Optional[Context] = None):
+        self.ctx = ctx or Context.default()
+        h = C.c_void_p()
+        _raise(L.load().rsgpu_prec_create(self.ctx.h, self.kind, nblocks, precision,
+                                          C.byref(h)), self.ctx.h)
+        self.h = h
+        self.precision = precision
+        self.n = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                L.load().rsgpu_prec_destroy(self.h)
+        except Exception:
+            pass
+
+    def setMatrix(self, A: CompRowMatrix):
+        bad = C.c_int(-1)
+        code = L.load().rsgpu_prec_set_matrix(self.h, A.h, C.byref(bad))
+        self.bad_row = bad.value
+        _raise(code, self.ctx.h)
+        self.n = A.n
+
+    def apply(self, b, x):
+        lib = L.load()
+        if _is_device(b):
+            _torch_sync(b, x)
+            _raise(lib.rsgpu_prec_apply(self.h, _dptr(b), _dptr(x)), self.ctx.h)
+            self.ctx.synchronize()
+            return x
+        bb = np.ascontiguousarray(b, np.float64)
+        _raise(lib.rsgpu_prec_apply_host(self.h, _hptr(bb), _hptr(x)), self.ctx.h)
+        return x
+
+    def transApply(self, b, x):
+        lib = L.load()
+        if _is_device(b):
+            _torch_sync(b, x)
+            _raise(lib.rsgpu_prec_trans_apply(self.h, _dptr(b), _dptr(x)), self.ctx.h)
+            self.ctx.synchronize()
+            return x
+        bb = np.ascontiguousarray(b, np.float64)
+        _raise(lib.rsgpu_prec_trans_apply_host(self.h, _hptr(bb), _hptr(x)), self.ctx.h)
+        return x
+
+    def factors(self):
+        """(rowptr, colind, lu) of the factored matrix (ILU's internal LU)."""
+        nnz = C.c_int(0)
+        lib = L.load()
+        _raise(lib.rsgpu_prec_get_factors(self.h, None, None, None, C.byref(nnz)), self.ctx.h)
+        rp = np.zeros(self.n + 1, np.int32)
+        ci = np.zeros(max(nnz.value, 1), np.int32)
+        lu = np.zeros(max(nnz.value, 1))
+        _raise(lib.rsgpu_prec_get_factors(self.h, _hptr(rp), _hptr(ci), _hptr(lu),
+                                          C.byref(nnz)), self.ctx.h)
+        return rp, ci[: nnz.value], lu[: nnz.value]
+
+    def info(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        L.load().rsgpu_prec_info(self.h, C.byref(a), C.byref(b), C.byref(c))
+        return dict(nnz=a.value, levels_lower=b.value, levels_upper=c.value)
+
+
+class IdentityPreconditioner(Preconditioner):
+    kind = L.PREC_IDENTITY
+
+
+class ILU(Preconditioner):
+    """ILU(0) (mtj/.../sparse/ILU.java).  ``ILU(A)`` accepts the matrix the
+    reference passes as the factor storage; only its pattern matters here."""
+    kind = L.PREC_ILU0
+
+    def __init__(self, A: Optional[CompRowMatrix] = None, precision: int = L.FP64,
+                 ctx: Optional[Context] = None):
+        if A is not None and not A.isSquare():
+            raise ValueError("ILU only applies to square matrices")
+        super().__init__(1, precision, ctx or (A.ctx if A is not None else None))
+
+
+class BlockILU(Preconditioner):
+    """Block-Jacobi ILU(0): BlockDiagonalPreconditioner(new ILU(...)) over
+    ``nblocks`` contiguous row ranges (distributed/BlockDiagonalPreconditioner
+    .java:46-75 with DistRowMatrix ownership)."""
+    kind = L.PREC_BLOCK_ILU0
+
+    def __init__(self, nblocks: int, precision: int = L.FP64,
+                 ctx: Optional[Context] = None):
+        super().__init__(nblocks, precision, ctx)
+
+
+# ------------------------------------------------------------------ monitor and solvers
+class DefaultIterationMonitor:
+    """mtj/.../sparse/DefaultIterationMonitor.java (defaults :85-90)."""
+
+    def __init__(self, maxIter: int = 100000, rtol: float = 1e-5, atol: float = 1e-50,
+                 dtol: float = 1e5):
+        self.maxIter, self.rtol, self.atol, self.dtol = maxIter, rtol, atol, dtol
+        self._iterations = 0
+        self._residual = 0.0
+        self.history = np.zeros(0)
+
+    def setMaxIterations(self, v):
+        self.maxIter = v
+
+    def setRelativeTolerance(self, v):
+        self.rtol = v
+
+    def setAbsoluteTolerance(self, v):
+        self.atol = v
+
+    def setDivergenceTolerance(self, v):
+        self.dtol = v
+
+    def iterations(self):
+        return self._iterations
+
+    def residual(self):
+        return self._residual
+
+
+class AbstractIterativeSolver:
+    kind = L.BICGSTAB
+
+    def __init__(self, template, restart: int = 30):
+        self.n = len(template) if not _is_device(template) else template.numel()
+        self.M: Preconditioner = None
+        self.iter = DefaultIterationMonitor()
+        self.restart = restart
+        self.mixed = False
+        self.inner_rtol = 1e-4
+        self.inner_max_iter = 200
+        self.check_every = 4
+        self._h = None
+        self._key = None
+        self.last_stats = None
+
+    def setPreconditioner(self, M: Preconditioner):
+        self.M = M
+
+    def getPreconditioner(self):
+        return self.M
+
+    def setIterationMonitor(self, it: DefaultIterationMonitor):
+        self.iter = it
+
+    def getIterationMonitor(self):
+        return self.iter
+
+    def _handle(self, ctx: Context):
+        key = (self.kind, self.restart, self.iter.maxIter, self.iter.rtol, self.iter.atol,
+               self.iter.dtol, self.mixed, self.inner_rtol, self.inner_max_iter,
+               self.check_every, id(ctx))
+        if self._h is not None and key == self._key:
+            return self._h
+        lib = L.load()
+        if self._h is not None:
+            lib.rsgpu_solver_destroy(self._h)
+            self._h = None
+        o = L.SolverOpts()
+        lib.rsgpu_solver_opts_default(C.byref(o))
+        o.kind, o.restart, o.max_iter = self.kind, self.restart, self.iter.maxIter
+        o.rtol, o.atol, o.dtol = self.iter.rtol, self.iter.atol, self.iter.dtol
+        o.mixed = int(self.mixed)
+        o.inner_rtol, o.inner_max_iter = self.inner_rtol, self.inner_max_iter
+        o.check_every = self.check_every
+        h = C.c_void_p()
+        _raise(lib.rsgpu_solver_create(ctx.h, C.byref(o), self.n, C.byref(h)), ctx.h)
+        self._h, self._key = h, key
+        return h
+
+    def __del__(self):
+        try:
+            if self._h is not None:
+                L.load().rsgpu_solver_destroy(self._h)
+        except Exception:
+            pass
+
+    def solve(self, A: CompRowMatrix, b, x):
+        """solve(A, b, x): x is the initial guess and the result."""
+        if not A.isSquare():
+            raise ValueError("!A.isSquare()")
+        if len(b) != A.n:
+            raise ValueError("b.size() != A.numRows()")
+        if len(x) != len(b):
+            raise ValueError("b.size() != x.size()")
+        lib = L.load()
+        h = self._handle(A.ctx)
+        st = L.Stats()
+        Mh = self.M.h if self.M is not None else None
+        if _is_device(b):
+            _torch_sync(b, x)
+            code = lib.rsgpu_solver_solve(h, A.h, Mh, _dptr(b), _dptr(x), C.byref(st))
+        else:
+            if not (isinstance(x, np.ndarray) and x.dtype == np.float64
+                    and x.flags.c_contiguous):
+                raise TypeError("x must be a contiguous float64 numpy array")
+            bb = np.ascontiguousarray(b, np.float64)
+            code = lib.rsgpu_solver_solve_host(h, A.h, Mh, _hptr(bb), _hptr(x), C.byref(st))
+        self.last_stats = st
+        self.iter._iterations = st.iterations
+        self.iter._residual = st.residual
+        hl = max(0, min(st.hist_len, 1 << 22))
+        hist = np.zeros(max(hl, 1))
+        k = lib.rsgpu_solver_history(h, _hptr(hist), hl)
+        self.iter.history = hist[: max(k, 0)]
+        if code in (L.BREAKDOWN, L.DIVERGENCE, L.ITERATIONS):
+            raise IterativeSolverNotConvergedException(
+                NotConvergedReason(code), st.residual, st.iterations)
+        _raise(code, A.ctx.h)
+        return x
+
+
+class BiCGstab(AbstractIterativeSolver):
+    """mtj/.../sparse/BiCGstab.java."""
+    kind = L.BICGSTAB
+
+
+class GMRES(AbstractIterativeSolver):
+    """mtj/.../sparse/GMRES.java; restart 30 by default (:84-86)."""
+    kind = L.GMRES
+
+    def __init__(self, template, restart: int = 30):
+        if restart <= 0:
+            raise ValueError("restart must be a positive integer")
+        super().__init__(template, restart)
+
+    def setRestart(self, restart: int):
+        if restart <= 0:
+            raise ValueError("restart must be a positive integer")
+        self.restart = restart
+
+
+class IR(AbstractIterativeSolver):
+    """mtj/.../sparse/IR.java."""
+    kind = L.IR
+
+
+class MixedPrecisionSolver(AbstractIterativeSolver):
+    """FP64 iterative refinement on the true residual around an FP32 inner
+    solve of the given kind (BiCGstab or GMRES) with an FP32 preconditioner.
+    No reference counterpart; the outer loop has IR.java's form."""
+
+    def __init__(self, template, inner: str = "gmres", restart: int = 30,
+                 inner_rtol: float = 1e-4, inner_max_iter: int = 200):
+        super().__init__(template, restart)
+        self.kind = {"gmres": L.GMRES, "bicgstab": L.BICGSTAB}[inner]
+        self.mixed = True
+        self.inner_rtol = inner_rtol
+        self.inner_max_iter = inner_max_iter
+
+
+# ------------------------------------------------------------------ MPFA-O
+def mpfa_o(dim, nx, ny, nz, dx, dy, dz, K6, continuity=1.0, ctx: Optional[Context] = None):
+    """Device O_MPFA transmissibilities -> (md_ptr, md_elem, md_t, check_failures)."""
+    ctx = ctx or Context.default()
+    if dim == 3:
+        nconn = 3 * nx * ny * nz - nx * ny - nx * nz - ny * nz
+        cap = 18
+    else:
+        nconn = 2 * nx * ny - nx - ny
+        cap = 6
+    md_ptr = np.zeros(nconn + 1, np.int32)
+    md_elem = np.zeros(max(nconn * cap, 1), np.int32)
+    md_t = np.zeros(max(nconn * cap, 1))
+    fails = C.c_int(0)
+    K = np.ascontiguousarray(K6, np.float64).reshape(-1)
+    _raise(L.load().rsgpu_mpfa_o(ctx.h, dim, nx, ny, nz, dx, dy, dz, _hptr(K), continuity,
+                                 _hptr(md_ptr), _hptr(md_elem), _hptr(md_t),
+                                 C.byref(fails)), ctx.h)
+    m = int(md_ptr[-1])
+    return md_ptr, md_elem[:m].copy(), md_t[:m].copy(), fails.value
+
+
+def mpfa_o_assemble(dim, nx, ny, nz, dx, dy, dz, K6, continuity=1.0,
+                    ctx: Optional[Context] = None):
+    """Device MPFA-O + LinearSolver pattern + flux assembly -> CompRowMatrix."""
+    ctx = ctx or Context.default()
+    h = C.c_void_p()
+    fails = C.c_int(0)
+    K = np.ascontiguousarray(K6, np.float64).reshape(-1)
+    _raise(L.load().rsgpu_mpfa_o_assemble(ctx.h, dim, nx, ny, nz, dx, dy, dz, _hptr(K),
+                                          continuity, C.byref(h), C.byref(fails)), ctx.h)
+    n = nx * ny * (nz if dim == 3 else 1)
+    A = CompRowMatrix._wrap(h, n, n, 0, ctx)
+    A.nnz = int(A.getRowPointers()[-1])
+    return A, fails.value

@@ -1,0 +1,159 @@
+"""Seeded synthetic inputs for the five benchmark/parity workloads.
+
+Random numbers come from a counter-based SplitMix64 stream (Steele, Lea and
+Flood, "Fast splittable pseudorandom number generators", OOPSLA 2014) turned
+into standard normals with the Box-Muller transform.  Everything here is
+input generation on the host; the physics and the solve run on the GPU.
+
+Permeability tensors are stored per cell as (xx, yy, zz, xy, xz, yz), the
+component order of rs ``Tensor3D(xx,yy,zz,xy,xz,yz)``
+(rs/src/no/uib/cipr/rs/geometry/Tensor3D.java:37-60).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64(seed: int, stream: int, count: int, offset: int = 0) -> np.ndarray:
+    """Counter-based SplitMix64: output i = mix(base + (offset+i+1)*golden)."""
+    with np.errstate(over="ignore"):
+        base = np.uint64((seed * 0x100000001B3 + stream * 0xC2B2AE3D27D4EB4F) % (1 << 64))
+        ctr = np.arange(offset + 1, offset + count + 1, dtype=np.uint64)
+        z = base + ctr * _GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform01(seed: int, stream: int, count: int, offset: int = 0) -> np.ndarray:
+    """Doubles in [0,1) from the top 53 bits."""
+    z = splitmix64(seed, stream, count, offset)
+    return (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def normal(seed: int, stream: int, count: int) -> np.ndarray:
+    """Standard normals by Box-Muller (cosine branch), two uniforms each."""
+    u = uniform01(seed, stream, 2 * count)
+    u1, u2 = u[0::2], u[1::2]
+    return np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * np.pi * u2)
+
+
+def full_tensor(kxx, kyy, kzz, off=0.1):
+    """K6 with off-diagonals off*sqrt(k_ii*k_jj) (SPD for off < 0.5)."""
+    K = np.empty((kxx.size, 6))
+    K[:, 0], K[:, 1], K[:, 2] = kxx, kyy, kzz
+    K[:, 3] = off * np.sqrt(kxx * kyy)
+    K[:, 4] = off * np.sqrt(kxx * kzz)
+    K[:, 5] = off * np.sqrt(kyy * kzz)
+    return K
+
+
+@dataclass
+class PressureProblem:
+    """Single-phase incompressible pressure system on a Cartesian grid."""
+    name: str
+    dim: int
+    nx: int
+    ny: int
+    nz: int
+    dx: float
+    dy: float
+    dz: float
+    K6: np.ndarray
+    sources: dict          # cell -> rate (right-hand side entry)
+    pinned: list = field(default_factory=list)   # cells locked by zeroRows
+    seed: int = 0
+
+    @property
+    def n(self) -> int:
+        return self.nx * self.ny * (self.nz if self.dim == 3 else 1)
+
+    @property
+    def centre(self) -> int:
+        k = (self.nz // 2) if self.dim == 3 else 0
+        return (self.nx // 2) + (self.ny // 2) * self.nx + k * self.nx * self.ny
+
+    def rhs(self) -> np.ndarray:
+        """Sources, with locked cells set to zero as rs LinearSolver.solve
+        does after Matrices.zeroRows (LinearSolver.java:256-260)."""
+        b = np.zeros(self.n)
+        for c, q in self.sources.items():
+            b[c] = q
+        for p in self.pinned:
+            b[p] = 0.0
+        return b
+
+
+def _pm1(n: int, dim: int, nx: int, ny: int, nz: int):
+    """+1 at the first cell, -1 at the opposite corner; the pure-Neumann
+    system is made nonsingular by locking the centre cell (p = 0)."""
+    k = nz // 2 if dim == 3 else 0
+    centre = nx // 2 + (ny // 2) * nx + k * nx * ny
+    return {0: 1.0, n - 1: -1.0}, [centre]
+
+
+def config1(seed: int = 1, nx: int = 64, ny: int = 64) -> PressureProblem:
+    """2-D 64x64, dx = dy = 1/64, k_xx, k_yy = exp(g) independent (sigma 1),
+    k_xy = 0.1 sqrt(k_xx k_yy); +1 / -1 in opposite corners, centre locked."""
+    n = nx * ny
+    g = normal(seed, 0, 2 * n)
+    kxx, kyy = np.exp(g[:n]), np.exp(g[n:])
+    K = full_tensor(kxx, kyy, np.ones(n))
+    K[:, 4] = 0.0
+    K[:, 5] = 0.0
+    src, pin = _pm1(n, 2, nx, ny, 0)
+    return PressureProblem("C1-2d-%dx%d" % (nx, ny), 2, nx, ny, 0, 1.0 / nx, 1.0 / ny, -1.0,
+                           K, sources=src, pinned=pin, seed=seed)
+
+
+def config2(seed: int = 2, nx: int = 128, ny: int = 128, nz: int = 64,
+            sigma: float = 2.0, kv_ratio: float = 0.1) -> PressureProblem:
+    """3-D 128x128x64, k_h = exp(sigma g), k_v = 0.1 k_h, off-diagonals
+    0.1 sqrt(k_ii k_jj); unit cells (dz = -1: depth grows with k);
+    +1 / -1 in opposite corners, centre cell locked."""
+    n = nx * ny * nz
+    kh = np.exp(sigma * normal(seed, 0, n))
+    K = full_tensor(kh, kh, kv_ratio * kh)
+    src, pin = _pm1(n, 3, nx, ny, nz)
+    return PressureProblem("C2-3d-%dx%dx%d" % (nx, ny, nz), 3, nx, ny, nz, 1.0, 1.0, -1.0, K,
+                           sources=src, pinned=pin, seed=seed)
+
+
+def config5(seed: int = 5, nx: int = 64, ny: int = 64, nz: int = 32,
+            sigma: float = 1.0) -> PressureProblem:
+    """3-D 64x64x32 with scalar log-permeability (sigma 1) times a fixed
+    full-tensor shape; injector in the centre column top, producers in the
+    four corner columns at the bottom (pinned)."""
+    n = nx * ny * nz
+    logk = sigma * normal(seed, 0, n)
+    k = np.exp(logk)
+    K = full_tensor(k, k, k)
+    layer = nx * ny * (nz - 1)
+    prods = [layer + 0, layer + nx - 1, layer + nx * (ny - 1), layer + nx * ny - 1]
+    inj = (nx // 2) + (ny // 2) * nx
+    return PressureProblem("C5-3d-%dx%dx%d" % (nx, ny, nz), 3, nx, ny, nz, 1.0, 1.0, -1.0, K,
+                           sources={inj: 1.0}, pinned=prods, seed=seed)
+
+
+def small3d(seed: int = 7, nx: int = 8, ny: int = 7, nz: int = 5, sigma: float = 1.0):
+    """A small 3-D problem for parity tests."""
+    n = nx * ny * nz
+    kh = np.exp(sigma * normal(seed, 0, n))
+    K = full_tensor(kh, kh, 0.3 * kh)
+    src, pin = _pm1(n, 3, nx, ny, nz)
+    return PressureProblem("small3d", 3, nx, ny, nz, 1.0, 1.0, -1.0, K, sources=src,
+                           pinned=pin, seed=seed)
+
+
+def stencil_nnz(dim: int, nx: int, ny: int, nz: int = 1) -> int:
+    """Nonzeros of the full-tensor MPFA-O pattern: prod(3 n_d - 2)."""
+    if dim == 2:
+        return (3 * nx - 2) * (3 * ny - 2)
+    return (3 * nx - 2) * (3 * ny - 2) * (3 * nz - 2)

@@ -1,0 +1,168 @@
+"""GPU parity: BiCGstab / GMRES(m) / IR / mixed-precision solves through the
+C ABI against the CPU oracle (FP64).  Dot products are tree sums on the GPU
+and sequential in the oracle, so iterates agree to round-off: the bar is the
+north star's relative residual 1e-10 and relative solution error 1e-8."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from ressim_amd import problems as P
+from tests.helpers import oracle_system, scipy_csr
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c1(oracle):
+    return oracle_system(oracle, P.config1())
+
+
+@pytest.fixture(scope="module")
+def s3(oracle):
+    return oracle_system(oracle, P.small3d(nx=14, ny=12, nz=9))
+
+
+def _run(gpu, s, solver, M, x0=None):
+    rp, ci, d, b = s["rowptr"], s["colind"], s["data"], s["b"]
+    n = len(rp) - 1
+    A = gpu.CompRowMatrix.from_csr(n, n, rp, ci, d)
+    if M is not None:
+        M.setMatrix(A)
+        solver.setPreconditioner(M)
+    x = np.zeros(n) if x0 is None else x0.copy()
+    solver.solve(A, b, x)
+    return x, A
+
+
+def _check(s, x, xref, res_tol=1e-10, err_tol=1e-8):
+    A = scipy_csr(s["rowptr"], s["colind"], s["data"])
+    b = s["b"]
+    assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < res_tol
+    assert np.linalg.norm(x - xref) / np.linalg.norm(xref) < err_tol
+
+
+@pytest.mark.parametrize("name", ["c1", "s3"])
+def test_bicgstab_ilu(gpu, oracle, request, name):
+    s = request.getfixturevalue(name)
+    xo, so, ho = oracle.solve("bicgstab", s["rowptr"], s["colind"], s["data"], s["b"],
+                              rtol=1e-12, prec="ilu")
+    assert so.status == 0
+    solver = gpu.BiCGstab(s["b"])
+    solver.setIterationMonitor(gpu.DefaultIterationMonitor(100000, 1e-12, 1e-50, 1e5))
+    x, _ = _run(gpu, s, solver, gpu.ILU())
+    _check(s, x, xo)
+    it = solver.getIterationMonitor()
+    assert abs(it.iterations() - so.iterations) <= max(2, so.iterations // 10)
+    # the reporter history starts with the same initial residuals
+    np.testing.assert_allclose(it.history[:2], ho[:2], rtol=1e-12)
+
+
+@pytest.mark.parametrize("prec,nb", [("ilu", 1), ("block_ilu", 8)])
+def test_gmres(gpu, oracle, c1, prec, nb):
+    s = c1
+    xo, so, ho = oracle.solve("gmres", s["rowptr"], s["colind"], s["data"], s["b"],
+                              rtol=1e-12, prec=prec, nblocks=nb)
+    assert so.status == 0
+    solver = gpu.GMRES(s["b"], 30)
+    solver.setIterationMonitor(gpu.DefaultIterationMonitor(100000, 1e-12, 1e-50, 1e5))
+    M = gpu.ILU() if prec == "ilu" else gpu.BlockILU(nb)
+    x, _ = _run(gpu, s, solver, M)
+    _check(s, x, xo)
+    it = solver.getIterationMonitor()
+    assert abs(it.iterations() - so.iterations) <= max(3, so.iterations // 10)
+    np.testing.assert_allclose(it.history[:3], ho[:3], rtol=1e-10)
+
+
+def test_ir_diag_dominant(gpu, oracle):
+    rng = np.random.default_rng(11)
+    n = 400
+    A = sp.random(n, n, density=0.02, random_state=2, format="csr") + 8 * sp.eye(n)
+    A = A.tocsr()
+    A.sort_indices()
+    b = rng.standard_normal(n)
+    s = dict(rowptr=A.indptr.astype(np.int32), colind=A.indices.astype(np.int32),
+             data=A.data, b=b)
+    xo, so, _ = oracle.solve("ir", s["rowptr"], s["colind"], s["data"], b, rtol=1e-13,
+                             prec="ilu")
+    assert so.status == 0
+    solver = gpu.IR(b)
+    solver.setIterationMonitor(gpu.DefaultIterationMonitor(1000, 1e-13, 1e-50, 1e5))
+    x, _ = _run(gpu, s, solver, gpu.ILU())
+    _check(s, x, xo, 1e-12, 1e-10)
+    assert solver.getIterationMonitor().iterations() == so.iterations
+
+
+def test_mixed_precision(gpu, oracle, s3):
+    s = s3
+    xo, so, _ = oracle.solve("gmres", s["rowptr"], s["colind"], s["data"], s["b"],
+                             rtol=1e-13, prec="ilu")
+    from ressim_amd import _lib
+    rp, ci, d, b = s["rowptr"], s["colind"], s["data"], s["b"]
+    n = len(rp) - 1
+    A = gpu.CompRowMatrix.from_csr(n, n, rp, ci, d)
+    A.sync_f32()
+    M = gpu.BlockILU(4, precision=_lib.FP32)
+    M.setMatrix(A)
+    solver = gpu.MixedPrecisionSolver(b, inner="gmres", inner_rtol=1e-4)
+    solver.setPreconditioner(M)
+    solver.setIterationMonitor(gpu.DefaultIterationMonitor(100, 1e-10, 1e-50, 1e5))
+    x = np.zeros(n)
+    solver.solve(A, b, x)
+    st = solver.last_stats
+    assert st.outer_iterations >= 2 and st.inner_iterations > 0
+    assert st.true_residual / st.b_norm < 1e-10
+    _check(s, x, xo, 1e-10, 1e-8)
+
+
+def test_failure_reasons(gpu, c1):
+    s = c1
+    solver = gpu.BiCGstab(s["b"])
+    solver.setIterationMonitor(gpu.DefaultIterationMonitor(1, 1e-12, 1e-50, 1e5))
+    with pytest.raises(gpu.IterativeSolverNotConvergedException) as e:
+        _run(gpu, s, solver, gpu.ILU())
+    assert e.value.getReason() == gpu.NotConvergedReason.Iterations
+    solver = gpu.GMRES(s["b"])
+    solver.setIterationMonitor(gpu.DefaultIterationMonitor(1000, 1e-12, 1e-50, 1e-30))
+    with pytest.raises(gpu.IterativeSolverNotConvergedException) as e:
+        _run(gpu, s, solver, gpu.ILU())
+    assert e.value.getReason() == gpu.NotConvergedReason.Divergence
+    bn = dict(s)
+    bn["b"] = s["b"].copy()
+    bn["b"][5] = np.nan
+    solver = gpu.BiCGstab(s["b"])
+    with pytest.raises(gpu.IterativeSolverNotConvergedException) as e:
+        _run(gpu, bn, solver, gpu.ILU())
+    assert e.value.getReason() == gpu.NotConvergedReason.Divergence
+    # exact breakdown (see tests/test_oracle_sparse.py)
+    A = sp.csr_matrix(np.array([[-2.0, 0.0], [-1.0, 1.0]]))
+    A.sort_indices()
+    t = dict(rowptr=A.indptr.astype(np.int32), colind=A.indices.astype(np.int32),
+             data=A.data, b=np.array([1.0, 1.0]))
+    solver = gpu.BiCGstab(t["b"])
+    with pytest.raises(gpu.IterativeSolverNotConvergedException) as e:
+        _run(gpu, t, solver, None)
+    assert e.value.getReason() == gpu.NotConvergedReason.Breakdown
+    assert e.value.getIterations() == 1
+
+
+def test_solve_device_vectors_repeat(gpu, oracle, c1):
+    """rs calls setMatrix + solve on every Newton step with new values on the
+    same pattern; device vectors are used by pointer."""
+    import torch
+    s = c1
+    rp, ci, d, b = s["rowptr"], s["colind"], s["data"], s["b"]
+    n = len(rp) - 1
+    A = gpu.CompRowMatrix.from_csr(n, n, rp, ci, d)
+    M = gpu.ILU()
+    solver = gpu.BiCGstab(b)
+    solver.setPreconditioner(M)
+    solver.setIterationMonitor(gpu.DefaultIterationMonitor(10000, 1e-12, 1e-50, 1e5))
+    for scale in (1.0, 2.0, 0.5):
+        A.set_data(d * scale)
+        M.setMatrix(A)
+        bd = torch.tensor(b, device="cuda")
+        xd = torch.zeros(n, dtype=torch.float64, device="cuda")
+        solver.solve(A, bd, xd)
+        xo, so, _ = oracle.solve("bicgstab", rp, ci, d * scale, b, rtol=1e-12, prec="ilu")
+        x = xd.cpu().numpy()
+        assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-8

@@ -1,0 +1,180 @@
+"""Pins of the CPU oracle's mesh / MPFA-O / assembly restatement.
+
+Each check is independent of the oracle's own formulas: the reference's
+legacy fixture, closed forms of Cartesian geometry, exact reproduction of
+linear pressure fields (MPFA.testUniformFlow, MPFA.java:246-283), the
+two-point (harmonic-mean) limit for diagonal K, the zero-sum invariant of
+Connection.check (Connection.java:80-107) and the 9/27-point pattern size.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from ressim_amd import problems as P
+from tests.helpers import oracle_system, scipy_csr
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_legacy_fixture_mpfa_quad(oracle):
+    """rs/data/test/mpfa_quad: 10x10 unit-K, dx = dy = 0.1."""
+    d = json.load(open(os.path.join(GOLD, "mpfa_quad.json")))
+    n = 100
+    K = np.zeros((n, 6))
+    K[:, 0] = K[:, 1] = K[:, 2] = 1.0
+    md_ptr, md_el, md_t, fails = oracle.mpfa_o(2, 10, 10, 0, 0.1, 0.1, -1.0, K)
+    assert fails == 0
+    here, there = oracle.mesh_connections(2, 10, 10)
+    mine = {}
+    for c in range(len(here)):
+        pair = (int(here[c]), int(there[c]))
+        mine[pair] = dict(zip(md_el[md_ptr[c]:md_ptr[c + 1]].tolist(),
+                              md_t[md_ptr[c]:md_ptr[c + 1]].tolist()))
+    assert len(d["connections"]) == len(mine) == 180
+    for _, els, ts in d["connections"]:
+        ref = dict(zip(els, ts))
+        pos = [e for e, t in ref.items() if t > 0.5]
+        neg = [e for e, t in ref.items() if t < -0.5]
+        assert len(pos) == 1 and len(neg) == 1
+        got = mine[(pos[0], neg[0])]
+        for e in set(ref) | set(got):
+            assert abs(ref.get(e, 0.0) - got.get(e, 0.0)) < 1e-12
+
+
+def test_geometry_closed_forms(oracle):
+    nx, ny, nz, dx, dy, dz = 4, 3, 2, 0.5, 0.25, -2.0
+    c = oracle.element_centers(3, nx, ny, nz, dx, dy, dz)
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    e = (i + j * nx + k * nx * ny).reshape(-1)
+    np.testing.assert_allclose(c[e, 0], ((i + 0.5) * dx).reshape(-1), rtol=0, atol=1e-15)
+    np.testing.assert_allclose(c[e, 1], ((j + 0.5) * dy).reshape(-1), rtol=0, atol=1e-15)
+    np.testing.assert_allclose(c[e, 2], ((k + 0.5) * dz).reshape(-1), rtol=0, atol=1e-15)
+    na, cc = oracle.interface_geometry(3, nx, ny, nz, dx, dy, dz)
+    here, there = oracle.mesh_connections(3, nx, ny, nz)
+    # connection numbering: z-faces, then y, then x (CartesianTopology3D)
+    nzf, nyf = nx * ny * (nz - 1), nx * (ny - 1) * nz
+    assert np.all(there[:nzf] - here[:nzf] == nx * ny)
+    assert np.all(there[nzf:nzf + nyf] - here[nzf:nzf + nyf] == nx)
+    assert np.all(there[nzf + nyf:] - here[nzf + nyf:] == 1)
+    # outward normal of the here side, area
+    np.testing.assert_allclose(na[:nzf], np.tile([0, 0, -1, dx * dy], (nzf, 1)), atol=1e-15)
+    np.testing.assert_allclose(na[nzf:nzf + nyf], np.tile([0, 1, 0, dx * abs(dz)], (nyf, 1)),
+                               atol=1e-15)
+    np.testing.assert_allclose(na[nzf + nyf:], np.tile([1, 0, 0, dy * abs(dz)],
+                                                       (len(here) - nzf - nyf, 1)), atol=1e-15)
+
+
+@pytest.mark.parametrize("dim,shape", [(2, (7, 5, 0)), (3, (5, 4, 3))])
+def test_uniform_flow_full_tensor(oracle, dim, shape):
+    """MPFA-O is exact for linear potentials with constant K, including the
+    full-tensor case (flux = -(K grad p).n A) on every connection."""
+    nx, ny, nz = shape
+    n = nx * ny * (nz if dim == 3 else 1)
+    Kc = np.array([2.0, 1.5, 0.7, 0.3, 0.1 if dim == 3 else 0, 0.2 if dim == 3 else 0])
+    K = np.tile(Kc, (n, 1))
+    dx, dy, dz = 0.3, 0.45, -0.6
+    md_ptr, md_el, md_t, fails = oracle.mpfa_o(dim, nx, ny, nz, dx, dy, dz, K)
+    assert fails == 0
+    c = oracle.element_centers(dim, nx, ny, nz, dx, dy, dz)
+    g = np.array([0.31, -0.77, 0.53 if dim == 3 else 0.0])
+    F = oracle.connection_flux(md_ptr, md_el, md_t, c @ g)
+    na, _ = oracle.interface_geometry(dim, nx, ny, nz, dx, dy, dz)
+    Km = np.array([[Kc[0], Kc[3], Kc[4]], [Kc[3], Kc[1], Kc[5]], [Kc[4], Kc[5], Kc[2]]])
+    true = -(na[:, :3] @ (Km @ g)) * na[:, 3]
+    # as MPFA.testUniformFlow, skip connections whose face touches the
+    # boundary (no-flow half faces there); check all others exactly
+    here, there = oracle.mesh_connections(dim, nx, ny, nz)
+    inner = np.array([not _touches_boundary(dim, nx, ny, nz, h, t)
+                      for h, t in zip(here, there)])
+    assert inner.sum() > 0
+    np.testing.assert_allclose(F[inner], true[inner], rtol=1e-12, atol=1e-12)
+
+
+def _touches_boundary(dim, nx, ny, nz, h, t):
+    """Face between cells h < t has a corner on the domain boundary."""
+    nzz = nz if dim == 3 else 1
+    i, j, k = h % nx, (h // nx) % ny, h // (nx * ny)
+    d = t - h
+    tang = []
+    if d == 1:
+        tang = [(j, ny)] + ([(k, nzz)] if dim == 3 else [])
+    elif d == nx:
+        tang = [(i, nx)] + ([(k, nzz)] if dim == 3 else [])
+    else:
+        tang = [(i, nx), (j, ny)]
+    return any(a == 0 or a + 1 == m for a, m in tang)
+
+
+def test_tpfa_limit_diagonal_K(oracle):
+    """Diagonal heterogeneous K on a Cartesian grid: MPFA-O reduces to the
+    harmonic-mean two-point flux; off-stencil entries vanish (to round-off)."""
+    rng = np.random.default_rng(3)
+    nx, ny, nz = 5, 4, 3
+    n = nx * ny * nz
+    K = np.zeros((n, 6))
+    K[:, :3] = np.exp(rng.standard_normal((n, 3)))
+    dx, dy, dz = 1.0, 2.0, -0.5
+    md_ptr, md_el, md_t, fails = oracle.mpfa_o(3, nx, ny, nz, dx, dy, dz, K)
+    assert fails == 0
+    here, there = oracle.mesh_connections(3, nx, ny, nz)
+    for c in range(len(here)):
+        h, t = here[c], there[c]
+        ax = {nx * ny: 2, nx: 1, 1: 0}[t - h]
+        d = [dx, dy, abs(dz)][ax]
+        area = [dy * abs(dz), dx * abs(dz), dx * dy][ax]
+        T = area / (d / 2 / K[h, ax] + d / 2 / K[t, ax])
+        lst = dict(zip(md_el[md_ptr[c]:md_ptr[c + 1]], md_t[md_ptr[c]:md_ptr[c + 1]]))
+        assert abs(lst[h] - T) < 1e-12 * T and abs(lst[t] + T) < 1e-12 * T
+        for e, v in lst.items():
+            if e not in (h, t):
+                assert abs(v) < 1e-12 * T
+
+
+def test_pattern_size_and_symmetry(oracle):
+    for pr in (P.config1(nx=9, ny=7), P.small3d()):
+        s = oracle_system(oracle, pr)
+        assert s["fails"] == 0
+        nnz = P.stencil_nnz(pr.dim, pr.nx, pr.ny, pr.nz)
+        assert len(s["colind"]) == nnz
+        A = scipy_csr(s["rowptr"], s["colind"], np.ones(nnz))
+        assert (A != A.T).nnz == 0
+        # rows sorted
+        for r in range(pr.n):
+            row = s["colind"][s["rowptr"][r]:s["rowptr"][r + 1]]
+            assert np.all(np.diff(row) > 0)
+        # the flux part has zero row sums before locking (conservation)
+        Au = scipy_csr(s["rowptr"], s["colind"], s["data_unpinned"])
+        assert np.abs(Au.T @ np.ones(pr.n)).max() < 1e-9
+
+
+def test_corner_local_system(oracle):
+    """Interior corner of a unit-K uniform 2-D grid: the O-method's T row of
+    each connection is the two-point +-1 (times dy/dx) with zero cross terms."""
+    nx, ny = 4, 4
+    K = np.zeros((16, 6))
+    K[:, 0] = K[:, 1] = K[:, 2] = 1.0
+    p = 2 + 2 * (nx + 1)          # interior point (2,2)
+    r = oracle.mpfa_corner(2, nx, ny, 0, 1.0, 1.0, -1.0, K, p)
+    assert r["A"].shape == (4, 4) and r["B"].shape == (4, 4)
+    T = r["T"]
+    # each row: +1/2 and -1/2 (half faces) at the two cells of the connection
+    for row in T:
+        nz = np.abs(row) > 1e-12
+        assert nz.sum() == 2
+        np.testing.assert_allclose(sorted(row[nz]), [-0.5, 0.5], atol=1e-14)
+    np.testing.assert_allclose(r["T"], r["C"] @ np.linalg.solve(r["A"], r["B"]) - r["D"],
+                               atol=1e-14)
+
+
+def test_dgesv_matches_numpy(oracle):
+    rng = np.random.default_rng(0)
+    for n in (1, 3, 4, 12):
+        a = rng.standard_normal((n, n)) + n * np.eye(n)
+        b = rng.standard_normal((n, 3))
+        st, x = oracle.dgesv(a, b)
+        assert st == 0
+        np.testing.assert_allclose(x, np.linalg.solve(a, b), rtol=1e-12, atol=1e-12)
+    st, _ = oracle.dgesv(np.zeros((3, 3)), np.ones((3, 1)))
+    assert st == oracle.SINGULAR
