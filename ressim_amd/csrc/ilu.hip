@@ -14,12 +14,20 @@
 // levels with workgroup barriers only; blocks never synchronise with each
 // other.  Within a level one lane owns one row and performs exactly the
 // reference's per-row operation sequence, so factors and sweeps are
-// bit-identical to the sequential loops.  The block's solution vector lives
-// in LDS (blocks up to 8192 FP64 / 16384 FP32 rows), otherwise in a global
-// scratch region.  Sweep operands are stored per level in an ELL layout
-// (slot-major, lanes contiguous) so that the L/U value and column streams
-// are coalesced; padding slots multiply 0 by a zero slot and leave the sums
-// unchanged.
+// bit-identical to the sequential loops.
+//
+// Sweep layout.  The block's vector lives in LDS (blocks up to 8191 FP64 /
+// 16383 FP32 rows; larger blocks use a global scratch region) and is
+// initialised with the block's right-hand side, so both sweeps run in place.
+// The factor values of each level are stored lane-major: row t of a level
+// owns W consecutive (value, local column) slots, W the sweep's largest row
+// padded to a multiple of 4, so a lane fetches its operands with a few
+// 16-byte loads.  Padding slots are identity updates (value -0.0 for the
+// sum form, +0.0 for the subtract form, column = the zero slot), which keeps
+// results bit-identical including signed zeros.  The U sweeps' diagonals
+// are stored in level order.  With one wave per block, the apply kernel
+// keeps four levels of operands in flight (depth-4 register pipeline); the
+// barrier between levels only orders LDS traffic, so the loads cross it.
 #include <algorithm>
 
 #include "rsgpu_impl.hpp"
@@ -27,28 +35,39 @@
 #define RS_LDS_BYTES 65536
 
 struct LevelMeta {
-    int count, width, off, roff;
+    int count, off, roff, pad;
 };
 
 // ---------------------------------------------------------------- host setup
-// Level schedule of one triangular part inside [b0,b1): forward (lower) or
-// backward (upper); transposed parts are described by explicit row lists.
 struct HostSweep {
+    int W = 4;
     std::vector<int> blk_lev;          // nblocks+1
     std::vector<LevelMeta> meta;
-    std::vector<int> rows, col, src;   // rows local; col local (nb = zero)
+    std::vector<int> rows, row_glob, col, src;
+    int max_count = 0, max_width = 0;
 };
 
-// entries[r] lists (column, LU position) of the part, in operation order
+static int pad_width(int w)
+{
+    w = std::max(w, 1);
+    return (w + 3) & ~3;
+}
+
+// Levels of one triangular part.  ent[r] lists (column, LU position) in
+// operation order.  forward: dependencies on smaller rows; else larger.
 static void build_sweep(int nblocks, const std::vector<int> &bstart,
                         const std::vector<std::vector<std::pair<int, int>>> &ent,
-                        bool forward, HostSweep &H)
+                        bool forward, int W, HostSweep &H)
 {
+    H.W = W;
     H.blk_lev.assign(nblocks + 1, 0);
     H.meta.clear();
     H.rows.clear();
+    H.row_glob.clear();
     H.col.clear();
     H.src.clear();
+    H.max_count = 0;
+    H.max_width = 0;
     const int n = bstart[nblocks];
     std::vector<int> level(n, 0);
     for (int b = 0; b < nblocks; ++b) {
@@ -79,47 +98,59 @@ static void build_sweep(int nblocks, const std::vector<int> &bstart,
         for (int l = 0; l < nlev; ++l) {
             LevelMeta m;
             m.count = (int)byl[l].size();
-            int w = 0;
-            for (int r : byl[l])
-                w = std::max(w, (int)ent[r].size());
-            m.width = w;
             m.off = (int)H.col.size();
             m.roff = (int)H.rows.size();
-            for (int r : byl[l])
+            m.pad = 0;
+            H.max_count = std::max(H.max_count, m.count);
+            for (int r : byl[l]) {
                 H.rows.push_back(r - b0);
-            H.col.resize(H.col.size() + (size_t)w * m.count, nb);
-            H.src.resize(H.src.size() + (size_t)w * m.count, -1);
+                H.row_glob.push_back(r);
+            }
+            H.col.resize(H.col.size() + (size_t)W * m.count, nb);
+            H.src.resize(H.src.size() + (size_t)W * m.count, -1);
             for (int t = 0; t < m.count; ++t) {
                 const auto &E = ent[byl[l][t]];
+                H.max_width = std::max(H.max_width, (int)E.size());
                 for (int s = 0; s < (int)E.size(); ++s) {
-                    H.col[m.off + (size_t)s * m.count + t] = E[s].first - b0;
-                    H.src[m.off + (size_t)s * m.count + t] = E[s].second;
+                    H.col[m.off + (size_t)t * W + s] = E[s].first - b0;
+                    H.src[m.off + (size_t)t * W + s] = E[s].second;
                 }
             }
             H.meta.push_back(m);
         }
     }
     H.blk_lev[nblocks] = (int)H.meta.size();
+    // trailing dummy row of slots: the arrays are never empty
+    H.col.resize(H.col.size() + W, 0);
+    H.src.resize(H.src.size() + W, -1);
+    H.row_glob.push_back(0);
+}
+
+static int max_entries(const std::vector<std::vector<std::pair<int, int>>> &ent)
+{
+    size_t w = 0;
+    for (auto &e : ent)
+        w = std::max(w, e.size());
+    return (int)w;
 }
 
 static void upload_sweep(const HostSweep &H, SweepData &D, hipStream_t s)
 {
+    D.W = H.W;
     D.blk_lev.upload(H.blk_lev, s);
     D.lev_meta.upload((const int *)H.meta.data(), H.meta.size() * 4, s);
     D.rows.upload(H.rows, s);
+    D.row_glob.upload(H.row_glob, s);
     D.ell_col.upload(H.col, s);
     D.ell_src.upload(H.src, s);
     D.ell_size = H.col.size();
+    D.nrows_lev = (int)H.row_glob.size();
     D.nlev_total = (int)H.meta.size();
     D.max_levels = 0;
-    D.max_count = 0;
-    D.max_width = 0;
     for (size_t b = 0; b + 1 < H.blk_lev.size(); ++b)
         D.max_levels = std::max(D.max_levels, H.blk_lev[b + 1] - H.blk_lev[b]);
-    for (auto &m : H.meta) {
-        D.max_count = std::max(D.max_count, m.count);
-        D.max_width = std::max(D.max_width, m.width);
-    }
+    D.max_count = H.max_count;
+    D.max_width = H.max_width;
 }
 
 // MTJ's factor loop reads colind[rowptr[k+1]] when row `index` has a column
@@ -134,8 +165,7 @@ static bool ilu_quirk_active(int n, const std::vector<int> &rp,
     for (int i = 0; i < n && sym; ++i)
         for (int j = rp[i]; j < rp[i + 1]; ++j) {
             int c = ci[j];
-            if (!std::binary_search(ci.begin() + rp[c], ci.begin() + rp[c + 1],
-                                    i)) {
+            if (!std::binary_search(ci.begin() + rp[c], ci.begin() + rp[c + 1], i)) {
                 sym = false;
                 break;
             }
@@ -211,8 +241,7 @@ void prec_setup(rsgpu_prec *M, rsgpu_csr *A)
         auto it = std::lower_bound(ci.begin() + rp[k], ci.begin() + rp[k + 1], k);
         if (it == ci.begin() + rp[k + 1] || *it != k)
             throw RsError{RSGPU_MISSING_DIAGONAL,
-                          "Missing diagonal entry on row " + std::to_string(k + 1),
-                          };
+                          "Missing diagonal entry on row " + std::to_string(k + 1)};
         dg[k] = (int)(it - ci.begin());
     }
     if (ilu_quirk_active(n, rp, ci, dg))
@@ -233,25 +262,23 @@ void prec_setup(rsgpu_prec *M, rsgpu_csr *A)
         for (int j = dg[r] + 1; j < rp[r + 1]; ++j)
             eU[r].push_back({ci[j], j});
     }
+    const int W = pad_width(std::max(max_entries(eL), max_entries(eU)));
     HostSweep H;
-    build_sweep(nb, M->h_bstart, eL, true, H);
+    build_sweep(nb, M->h_bstart, eL, true, W, H);
     upload_sweep(H, M->L, s);
-    build_sweep(nb, M->h_bstart, eU, false, H);
+    build_sweep(nb, M->h_bstart, eU, false, W, H);
     upload_sweep(H, M->U, s);
-    // diagonal gather positions reuse `diag`
     M->status.alloc(2);
-    const size_t esz = std::max<size_t>(1, M->L.ell_size);
-    const size_t usz = std::max<size_t>(1, M->U.ell_size);
     if (M->precision == RSGPU_FP64) {
         M->lu64.alloc(std::max(M->nnz, 1));
-        M->ell64_L.alloc(esz);
-        M->ell64_U.alloc(usz);
-        M->dg64.alloc(std::max(n, 1));
+        M->ell64_L.alloc(M->L.ell_size);
+        M->ell64_U.alloc(M->U.ell_size);
+        M->dgU64.alloc(M->U.nrows_lev);
     } else {
         M->lu32.alloc(std::max(M->nnz, 1));
-        M->ell32_L.alloc(esz);
-        M->ell32_U.alloc(usz);
-        M->dg32.alloc(std::max(n, 1));
+        M->ell32_L.alloc(M->L.ell_size);
+        M->ell32_U.alloc(M->U.ell_size);
+        M->dgU32.alloc(M->U.nrows_lev);
     }
     const int elem = M->precision == RSGPU_FP64 ? 8 : 4;
     M->lds_rows = RS_LDS_BYTES / elem - 1;
@@ -276,36 +303,34 @@ static void prec_setup_transpose(rsgpu_prec *M)
     const auto &rp = M->h_rowptr, &ci = M->h_colind;
     std::vector<int> dg(n);
     for (int k = 0; k < n; ++k)
-        dg[k] = (int)(std::lower_bound(ci.begin() + rp[k],
-                                       ci.begin() + rp[k + 1], k) -
+        dg[k] = (int)(std::lower_bound(ci.begin() + rp[k], ci.begin() + rp[k + 1], k) -
                       ci.begin());
     std::vector<std::vector<std::pair<int, int>>> eUt(n), eLt(n);
-    for (int i = 0; i < n; ++i) {            // ascending source rows
+    for (int i = 0; i < n; ++i)
         for (int j = dg[i] + 1; j < rp[i + 1]; ++j)
             eUt[ci[j]].push_back({i, j});
-    }
-    for (int i = n - 1; i >= 0; --i) {       // descending source rows
+    for (int i = n - 1; i >= 0; --i)
         for (int j = rp[i]; j < dg[i]; ++j)
             eLt[ci[j]].push_back({i, j});
-    }
+    const int W = pad_width(std::max(max_entries(eUt), max_entries(eLt)));
     HostSweep H;
-    build_sweep(M->nblocks, M->h_bstart, eUt, true, H);
+    build_sweep(M->nblocks, M->h_bstart, eUt, true, W, H);
     upload_sweep(H, M->Ut, s);
-    build_sweep(M->nblocks, M->h_bstart, eLt, false, H);
+    build_sweep(M->nblocks, M->h_bstart, eLt, false, W, H);
     upload_sweep(H, M->Lt, s);
-    M->ell64_Ut.alloc(std::max<size_t>(1, M->Ut.ell_size));
-    M->ell64_Lt.alloc(std::max<size_t>(1, M->Lt.ell_size));
+    M->ell64_Ut.alloc(M->Ut.ell_size);
+    M->ell64_Lt.alloc(M->Lt.ell_size);
+    M->dgUt64.alloc(M->Ut.nrows_lev);
     if (!M->scratch64.p && M->max_block_rows > RS_LDS_BYTES / 8 - 1)
         M->scratch64.alloc((size_t)n + M->nblocks);
     RS_HIP(hipStreamSynchronize(s));
     M->has_t = true;
 }
 
-// ---------------------------------------------------------------- kernels
+// ---------------------------------------------------------------- factor
 template <typename T>
-__global__ void k_load_values(const double *__restrict__ a,
-                              const int *__restrict__ src, T *__restrict__ lu,
-                              int nnz)
+__global__ void k_load_values(const double *__restrict__ a, const int *__restrict__ src,
+                              T *__restrict__ lu, int nnz)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nnz)
@@ -317,10 +342,9 @@ __global__ void k_load_values(const double *__restrict__ a,
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_ilu_factor(const int *__restrict__ bstart, const int *__restrict__ blk_lev,
-                 const LevelMeta *__restrict__ meta,
-                 const int *__restrict__ rows, const int *__restrict__ rowptr,
-                 const int *__restrict__ colind, const int *__restrict__ diag,
-                 T *lu, int *status)
+                 const LevelMeta *__restrict__ meta, const int *__restrict__ rows,
+                 const int *__restrict__ rowptr, const int *__restrict__ colind,
+                 const int *__restrict__ diag, T *lu, int *status)
 {
     const int b = blockIdx.x;
     const int b0 = bstart[b];
@@ -353,31 +377,32 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// Padding slots hold the identity of the sweep's update: -0.0 for the
+// sum form (s + (-0)*0 == s for every s, signed zeros included) and +0.0
+// for the subtract form (a - (+0)*0 == a).
 template <typename T>
-__global__ void k_gather_ell(const int *__restrict__ src,
-                             const T *__restrict__ lu, T *__restrict__ ell,
-                             size_t n)
+__global__ void k_gather_ell(const int *__restrict__ src, const T *__restrict__ lu,
+                             T *__restrict__ ell, size_t n, T pad)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         int s = src[i];
-        ell[i] = s >= 0 ? lu[s] : T(0);
+        ell[i] = s >= 0 ? lu[s] : pad;
     }
 }
 
+// level-ordered diagonal: dgl[p] = LU(r, r) for the p-th level row r
 template <typename T>
-__global__ void k_gather_diag(const int *__restrict__ diag,
-                              const T *__restrict__ lu, T *__restrict__ dg,
-                              int n)
+__global__ void k_gather_rowdiag(const int *__restrict__ row_glob,
+                                 const int *__restrict__ diag, const T *__restrict__ lu,
+                                 T *__restrict__ dgl, int nrows)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n)
-        dg[i] = lu[diag[i]];
+    if (i < nrows)
+        dgl[i] = lu[diag[row_glob[i]]];
 }
 
-// LDS-only barrier: all cross-lane traffic inside the level loop goes
-// through the LDS vector, so global loads (next level's operands) may stay
-// in flight across it.
+// ---------------------------------------------------------------- sweeps
 __device__ __forceinline__ void lds_barrier()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
@@ -387,58 +412,55 @@ __device__ __forceinline__ void lds_barrier()
 
 enum { FORM_SUM = 0, FORM_SUBSEQ = 1 };
 
-// One triangular sweep over this block's levels.  FORM_SUM: x_r = rhs - sum
-// (sum left to right), FORM_SUBSEQ: acc = rhs; acc -= v*x in slot order.
-// DIV divides by the diagonal.  RHS_GLOBAL reads rhs from b, else in place.
-template <typename T, int FORM, bool DIV, bool RHS_GLOBAL, bool LDS>
-__device__ __forceinline__ void sweep_block(
-    int lb, int le, const LevelMeta *__restrict__ meta,
-    const int *__restrict__ rows, const int *__restrict__ ecol,
-    const T *__restrict__ eval, const T *__restrict__ dg,
-    const T *__restrict__ b, T *xv)
+// one row update: FORM_SUM x_r = (x_r - sum_s v_s x_c) [/ d];
+// FORM_SUBSEQ x_r = (((x_r - v_0 x_c0) - v_1 x_c1) ...) [/ d]
+template <typename T, int FORM, bool DIV>
+__device__ __forceinline__ T row_update(T rhs, const T *v, const int *c, int W, const T *xv,
+                                        T d)
+{
+    T acc;
+    if (FORM == FORM_SUM) {
+        T sum = 0;
+        for (int s = 0; s < W; ++s)
+            sum += v[s] * xv[c[s]];
+        acc = rhs - sum;
+    } else {
+        acc = rhs;
+        for (int s = 0; s < W; ++s)
+            acc -= v[s] * xv[c[s]];
+    }
+    if (DIV)
+        acc = acc / d;
+    return acc;
+}
+
+// Generic in-place sweep (any W, any level width): LDS or global vector.
+template <typename T, int FORM, bool DIV, bool LDS>
+__device__ void sweep_generic(int lb, int le, int W, const LevelMeta *__restrict__ meta,
+                              const int *__restrict__ rows, const int *__restrict__ ecol,
+                              const T *__restrict__ eval, const T *__restrict__ dgl, T *xv)
 {
     for (int l = lb; l < le; ++l) {
         const LevelMeta m = meta[l];
         for (int t = threadIdx.x; t < m.count; t += blockDim.x) {
             const int r = rows[m.roff + t];
-            const T rhs = RHS_GLOBAL ? b[r] : xv[r];
-            T acc;
-            if (FORM == FORM_SUM) {
-                T sum = 0;
-                for (int s = 0; s < m.width; ++s) {
-                    const size_t q = (size_t)m.off + (size_t)s * m.count + t;
-                    sum += eval[q] * xv[ecol[q]];
-                }
-                acc = rhs - sum;
-            } else {
-                acc = rhs;
-                for (int s = 0; s < m.width; ++s) {
-                    const size_t q = (size_t)m.off + (size_t)s * m.count + t;
-                    acc -= eval[q] * xv[ecol[q]];
-                }
-            }
-            if (DIV)
-                acc = acc / dg[r];
-            xv[r] = acc;
+            const size_t q = (size_t)m.off + (size_t)t * W;
+            const T d = DIV ? dgl[m.roff + t] : T(1);
+            xv[r] = row_update<T, FORM, DIV>(xv[r], eval + q, ecol + q, W, xv, d);
         }
-        if (LDS)
-            lds_barrier();
-        else
-            __syncthreads();
+        __syncthreads();
     }
 }
 
-// apply (TRANS=false): L forward (sum form) then U backward (sum form, /d);
-// transApply (TRANS=true): U' forward (subtract, /d) then L' backward.
 template <typename T, bool LDS, bool TRANS>
 __global__ void __launch_bounds__(256)
-    k_ilu_apply(const int *__restrict__ bstart,
+    k_ilu_apply(const int *__restrict__ bstart, int W,
                 const int *__restrict__ blkA, const LevelMeta *__restrict__ metaA,
                 const int *__restrict__ rowsA, const int *__restrict__ colA,
-                const T *__restrict__ valA,
+                const T *__restrict__ valA, const T *__restrict__ dglA,
                 const int *__restrict__ blkB, const LevelMeta *__restrict__ metaB,
                 const int *__restrict__ rowsB, const int *__restrict__ colB,
-                const T *__restrict__ valB, const T *__restrict__ dg,
+                const T *__restrict__ valB, const T *__restrict__ dglB,
                 const T *__restrict__ b, T *__restrict__ x, T *scratch,
                 const int *skip1, const int *skip2)
 {
@@ -448,29 +470,176 @@ __global__ void __launch_bounds__(256)
     const int blk = blockIdx.x;
     const int b0 = bstart[blk], b1 = bstart[blk + 1], nb = b1 - b0;
     T *xv = LDS ? (T *)rs_smem : scratch + b0 + blk;
+    for (int r = threadIdx.x; r < nb; r += blockDim.x)
+        xv[r] = b[b0 + r];
     if (threadIdx.x == 0)
-        xv[nb] = T(0);   // zero slot for padding entries
-    if (LDS)
-        lds_barrier();
-    else
-        __syncthreads();
-    const T *bb = b + b0;
-    const T *dgb = dg + b0;
+        xv[nb] = T(0);
+    __syncthreads();
     if (!TRANS) {
-        sweep_block<T, FORM_SUM, false, true, LDS>(blkA[blk], blkA[blk + 1],
-                                                   metaA, rowsA, colA, valA,
-                                                   dgb, bb, xv);
-        sweep_block<T, FORM_SUM, true, false, LDS>(blkB[blk], blkB[blk + 1],
-                                                   metaB, rowsB, colB, valB,
-                                                   dgb, bb, xv);
+        sweep_generic<T, FORM_SUM, false, LDS>(blkA[blk], blkA[blk + 1], W, metaA, rowsA, colA,
+                                               valA, dglA, xv);
+        sweep_generic<T, FORM_SUM, true, LDS>(blkB[blk], blkB[blk + 1], W, metaB, rowsB, colB,
+                                              valB, dglB, xv);
     } else {
-        sweep_block<T, FORM_SUBSEQ, true, true, LDS>(blkA[blk], blkA[blk + 1],
-                                                     metaA, rowsA, colA, valA,
-                                                     dgb, bb, xv);
-        sweep_block<T, FORM_SUBSEQ, false, false, LDS>(blkB[blk], blkB[blk + 1],
-                                                       metaB, rowsB, colB,
-                                                       valB, dgb, bb, xv);
+        sweep_generic<T, FORM_SUBSEQ, true, LDS>(blkA[blk], blkA[blk + 1], W, metaA, rowsA,
+                                                 colA, valA, dglA, xv);
+        sweep_generic<T, FORM_SUBSEQ, false, LDS>(blkB[blk], blkB[blk + 1], W, metaB, rowsB,
+                                                  colB, valB, dglB, xv);
     }
+    for (int r = threadIdx.x; r < nb; r += blockDim.x)
+        x[b0 + r] = xv[r];
+}
+
+// ---------------------------------------------------------------- pipelined
+template <typename T, int W>
+struct alignas(16) VRow {
+    T v[W];
+};
+template <int W>
+struct alignas(16) CRow {
+    int c[W];
+};
+
+template <typename T, int W>
+struct LevelRegs {
+    int r;
+    T d;
+    VRow<T, W> v;
+    CRow<W> c;
+};
+
+// All loads of a level are independent of each other and of the vector:
+// row index, level-ordered diagonal, and the lane's W values / columns.
+// Inactive lanes read the level's last row and are ignored.
+template <typename T, int W, bool DIV>
+__device__ __forceinline__ void level_load(const LevelMeta &m, const int *__restrict__ rows,
+                                           const int *__restrict__ ecol,
+                                           const T *__restrict__ eval,
+                                           const T *__restrict__ dgl, LevelRegs<T, W> &R)
+{
+    const int t = threadIdx.x;
+    const bool act = t < m.count;
+    const int tt = act ? t : m.count - 1;
+    const int r = rows[m.roff + tt];
+    R.r = act ? r : -1;
+    if (DIV)
+        R.d = dgl[m.roff + tt];
+    const size_t q = (size_t)m.off + (size_t)tt * W;
+    R.v = *reinterpret_cast<const VRow<T, W> *>(eval + q);
+    R.c = *reinterpret_cast<const CRow<W> *>(ecol + q);
+}
+
+template <typename T, int W, int FORM, bool DIV>
+__device__ __forceinline__ void level_compute(const LevelRegs<T, W> &R, T *xv)
+{
+    if (R.r < 0)
+        return;
+    T xs[W];
+#pragma unroll
+    for (int s = 0; s < W; ++s)
+        xs[s] = xv[R.c.c[s]];
+    const T rhs = xv[R.r];
+    T acc;
+    if (FORM == FORM_SUM) {
+        T sum = 0;
+#pragma unroll
+        for (int s = 0; s < W; ++s)
+            sum += R.v.v[s] * xs[s];
+        acc = rhs - sum;
+    } else {
+        acc = rhs;
+#pragma unroll
+        for (int s = 0; s < W; ++s)
+            acc -= R.v.v[s] * xs[s];
+    }
+    if (DIV)
+        acc = acc / R.d;
+    xv[R.r] = acc;
+}
+
+// Four register sets in rotation: while level l computes, levels l+1..l+3
+// are loading.  No register copies between the sets.
+template <typename T, int W, int FORM, bool DIV>
+__device__ __forceinline__ void sweep_pf(int nl, const LevelMeta *sm,
+                                         const int *__restrict__ rows,
+                                         const int *__restrict__ ecol,
+                                         const T *__restrict__ eval,
+                                         const T *__restrict__ dgl, T *xv)
+{
+    if (nl <= 0)
+        return;
+    LevelRegs<T, W> r0, r1, r2, r3;
+    level_load<T, W, DIV>(sm[0], rows, ecol, eval, dgl, r0);
+    if (1 < nl)
+        level_load<T, W, DIV>(sm[1], rows, ecol, eval, dgl, r1);
+    if (2 < nl)
+        level_load<T, W, DIV>(sm[2], rows, ecol, eval, dgl, r2);
+    for (int l = 0; l < nl; l += 4) {
+        if (l + 3 < nl)
+            level_load<T, W, DIV>(sm[l + 3], rows, ecol, eval, dgl, r3);
+        level_compute<T, W, FORM, DIV>(r0, xv);
+        lds_barrier();
+        if (l + 1 >= nl)
+            break;
+        if (l + 4 < nl)
+            level_load<T, W, DIV>(sm[l + 4], rows, ecol, eval, dgl, r0);
+        level_compute<T, W, FORM, DIV>(r1, xv);
+        lds_barrier();
+        if (l + 2 >= nl)
+            break;
+        if (l + 5 < nl)
+            level_load<T, W, DIV>(sm[l + 5], rows, ecol, eval, dgl, r1);
+        level_compute<T, W, FORM, DIV>(r2, xv);
+        lds_barrier();
+        if (l + 3 >= nl)
+            break;
+        if (l + 6 < nl)
+            level_load<T, W, DIV>(sm[l + 6], rows, ecol, eval, dgl, r2);
+        level_compute<T, W, FORM, DIV>(r3, xv);
+        lds_barrier();
+    }
+}
+
+// One wave per block (every level has at most 64 rows); vector and both
+// sweeps' level metadata in LDS.
+template <typename T, int W, bool TRANS>
+__global__ void __launch_bounds__(64)
+    k_ilu_apply_pf(const int *__restrict__ bstart,
+                   const int *__restrict__ blkA, const LevelMeta *__restrict__ metaA,
+                   const int *__restrict__ rowsA, const int *__restrict__ colA,
+                   const T *__restrict__ valA, const T *__restrict__ dglA,
+                   const int *__restrict__ blkB, const LevelMeta *__restrict__ metaB,
+                   const int *__restrict__ rowsB, const int *__restrict__ colB,
+                   const T *__restrict__ valB, const T *__restrict__ dglB,
+                   const T *__restrict__ b, T *__restrict__ x, int xv_bytes,
+                   const int *skip1, const int *skip2)
+{
+    if ((skip1 && *skip1) || (skip2 && *skip2))
+        return;
+    extern __shared__ unsigned char rs_smem[];
+    const int blk = blockIdx.x;
+    const int b0 = bstart[blk], b1 = bstart[blk + 1], nb = b1 - b0;
+    T *xv = (T *)rs_smem;
+    LevelMeta *sm = (LevelMeta *)(rs_smem + xv_bytes);
+    const int la0 = blkA[blk], nla = blkA[blk + 1] - la0;
+    const int lb0 = blkB[blk], nlb = blkB[blk + 1] - lb0;
+    for (int q = threadIdx.x; q < nla; q += blockDim.x)
+        sm[q] = metaA[la0 + q];
+    for (int q = threadIdx.x; q < nlb; q += blockDim.x)
+        sm[nla + q] = metaB[lb0 + q];
+    for (int r = threadIdx.x; r < nb; r += blockDim.x)
+        xv[r] = b[b0 + r];
+    if (threadIdx.x == 0)
+        xv[nb] = T(0);
+    __syncthreads();
+    if (!TRANS) {
+        sweep_pf<T, W, FORM_SUM, false>(nla, sm, rowsA, colA, valA, dglA, xv);
+        sweep_pf<T, W, FORM_SUM, true>(nlb, sm + nla, rowsB, colB, valB, dglB, xv);
+    } else {
+        sweep_pf<T, W, FORM_SUBSEQ, true>(nla, sm, rowsA, colA, valA, dglA, xv);
+        sweep_pf<T, W, FORM_SUBSEQ, false>(nlb, sm + nla, rowsB, colB, valB, dglB, xv);
+    }
+    __syncthreads();
     for (int r = threadIdx.x; r < nb; r += blockDim.x)
         x[b0 + r] = xv[r];
 }
@@ -482,61 +651,60 @@ static void launch_factor(rsgpu_prec *M, T *lu, const double *aval)
     hipStream_t s = M->ctx->stream;
     const int nnz = M->nnz;
     if (nnz > 0) {
-        hipLaunchKernelGGL(k_load_values<T>, dim3((nnz + 255) / 256), dim3(256),
-                           0, s, aval, M->src_identity ? nullptr : M->src.p,
-                           lu, nnz);
+        hipLaunchKernelGGL(k_load_values<T>, dim3((nnz + 255) / 256), dim3(256), 0, s, aval,
+                           M->src_identity ? nullptr : M->src.p, lu, nnz);
         RS_CHECK_LAUNCH();
     }
     int init[2] = {0, 0x7fffffff};
-    RS_HIP(hipMemcpyAsync(M->status.p, init, sizeof(init),
-                          hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_ilu_factor<T>, dim3(M->nblocks),
-                       dim3(M->L.max_count > 64 ? 256 : 64), 0, s,
-                       M->bstart.p, M->L.blk_lev.p,
-                       (const LevelMeta *)M->L.lev_meta.p, M->L.rows.p,
-                       M->rowptr.p, M->colind.p, M->diag.p, lu, M->status.p);
+    RS_HIP(hipMemcpyAsync(M->status.p, init, sizeof(init), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_ilu_factor<T>, dim3(M->nblocks), dim3(M->L.max_count > 64 ? 256 : 64),
+                       0, s, M->bstart.p, M->L.blk_lev.p, (const LevelMeta *)M->L.lev_meta.p,
+                       M->L.rows.p, M->rowptr.p, M->colind.p, M->diag.p, lu, M->status.p);
     RS_CHECK_LAUNCH();
 }
 
 template <typename T>
-static void launch_ell(hipStream_t s, const SweepData &D, const T *lu, T *ell)
+static void launch_ell(hipStream_t s, const SweepData &D, const T *lu, T *ell, bool sum_form)
 {
     if (D.ell_size == 0)
         return;
-    hipLaunchKernelGGL(k_gather_ell<T>, dim3((unsigned)((D.ell_size + 255) / 256)),
-                       dim3(256), 0, s, D.ell_src.p, lu, ell, D.ell_size);
+    hipLaunchKernelGGL(k_gather_ell<T>, dim3((unsigned)((D.ell_size + 255) / 256)), dim3(256),
+                       0, s, D.ell_src.p, lu, ell, D.ell_size, sum_form ? T(-0.0) : T(0.0));
+    RS_CHECK_LAUNCH();
+}
+
+template <typename T>
+static void launch_rowdiag(hipStream_t s, const SweepData &D, const int *diag, const T *lu,
+                           T *dgl)
+{
+    if (D.nrows_lev <= 0)
+        return;
+    hipLaunchKernelGGL(k_gather_rowdiag<T>, dim3((D.nrows_lev + 255) / 256), dim3(256), 0, s,
+                       D.row_glob.p, diag, lu, dgl, D.nrows_lev);
     RS_CHECK_LAUNCH();
 }
 
 int prec_factor(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
 {
     hipStream_t s = M->ctx->stream;
-    const int n = M->n;
     int st[2];
     if (M->precision == RSGPU_FP64) {
         launch_factor<double>(M, M->lu64.p, A->val.p);
-        launch_ell<double>(s, M->L, M->lu64.p, M->ell64_L.p);
-        launch_ell<double>(s, M->U, M->lu64.p, M->ell64_U.p);
-        if (n > 0)
-            hipLaunchKernelGGL(k_gather_diag<double>, dim3((n + 255) / 256),
-                               dim3(256), 0, s, M->diag.p, M->lu64.p,
-                               M->dg64.p, n);
+        launch_ell<double>(s, M->L, M->lu64.p, M->ell64_L.p, true);
+        launch_ell<double>(s, M->U, M->lu64.p, M->ell64_U.p, true);
+        launch_rowdiag<double>(s, M->U, M->diag.p, M->lu64.p, M->dgU64.p);
         if (M->has_t) {
-            launch_ell<double>(s, M->Ut, M->lu64.p, M->ell64_Ut.p);
-            launch_ell<double>(s, M->Lt, M->lu64.p, M->ell64_Lt.p);
+            launch_ell<double>(s, M->Ut, M->lu64.p, M->ell64_Ut.p, false);
+            launch_ell<double>(s, M->Lt, M->lu64.p, M->ell64_Lt.p, false);
+            launch_rowdiag<double>(s, M->Ut, M->diag.p, M->lu64.p, M->dgUt64.p);
         }
     } else {
         launch_factor<float>(M, M->lu32.p, A->val.p);
-        launch_ell<float>(s, M->L, M->lu32.p, M->ell32_L.p);
-        launch_ell<float>(s, M->U, M->lu32.p, M->ell32_U.p);
-        if (n > 0)
-            hipLaunchKernelGGL(k_gather_diag<float>, dim3((n + 255) / 256),
-                               dim3(256), 0, s, M->diag.p, M->lu32.p,
-                               M->dg32.p, n);
+        launch_ell<float>(s, M->L, M->lu32.p, M->ell32_L.p, true);
+        launch_ell<float>(s, M->U, M->lu32.p, M->ell32_U.p, true);
+        launch_rowdiag<float>(s, M->U, M->diag.p, M->lu32.p, M->dgU32.p);
     }
-    RS_CHECK_LAUNCH();
-    RS_HIP(hipMemcpyAsync(st, M->status.p, sizeof(st), hipMemcpyDeviceToHost,
-                          s));
+    RS_HIP(hipMemcpyAsync(st, M->status.p, sizeof(st), hipMemcpyDeviceToHost, s));
     RS_HIP(hipStreamSynchronize(s));
     if (st[0] != 0) {
         if (bad_row)
@@ -546,29 +714,58 @@ int prec_factor(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
     return RSGPU_OK;
 }
 
+template <typename T, bool TRANS, int W>
+static void launch_pf(rsgpu_prec *M, const SweepData &A_, const T *va, const T *da,
+                      const SweepData &B_, const T *vb, const T *db, const T *b, T *x,
+                      size_t xv_bytes, size_t shm, const int *sk1, const int *sk2)
+{
+    hipLaunchKernelGGL((k_ilu_apply_pf<T, W, TRANS>), dim3(M->nblocks), dim3(64), shm,
+                       M->ctx->stream, M->bstart.p, A_.blk_lev.p,
+                       (const LevelMeta *)A_.lev_meta.p, A_.rows.p, A_.ell_col.p, va, da,
+                       B_.blk_lev.p, (const LevelMeta *)B_.lev_meta.p, B_.rows.p,
+                       B_.ell_col.p, vb, db, b, x, (int)xv_bytes, sk1, sk2);
+}
+
 template <typename T, bool TRANS>
-static void launch_apply(rsgpu_prec *M, const SweepData &A_, const T *va,
-                         const SweepData &B_, const T *vb, const T *dg,
-                         const T *b, T *x, T *scratch, const int *sk1,
-                         const int *sk2)
+static void launch_apply(rsgpu_prec *M, const SweepData &A_, const T *va, const T *da,
+                         const SweepData &B_, const T *vb, const T *db, const T *b, T *x,
+                         T *scratch, const int *sk1, const int *sk2)
 {
     hipStream_t s = M->ctx->stream;
     if (M->n == 0)
         return;
-    const int bs = std::max(A_.max_count, B_.max_count) > 64 ? 256 : 64;
+    const int maxc = std::max(A_.max_count, B_.max_count);
     const bool lds = M->max_block_rows <= (int)(RS_LDS_BYTES / sizeof(T)) - 1;
+    const size_t xv_bytes = ((size_t)(M->max_block_rows + 1) * sizeof(T) + 15) & ~(size_t)15;
+    const size_t pf_shm =
+        xv_bytes + (size_t)(A_.max_levels + B_.max_levels) * sizeof(LevelMeta);
+    const int W = A_.W;
+    if (lds && maxc <= 64 && A_.W == B_.W && W <= 16 && pf_shm <= RS_LDS_BYTES) {
+        if (W <= 4)
+            launch_pf<T, TRANS, 4>(M, A_, va, da, B_, vb, db, b, x, xv_bytes, pf_shm, sk1, sk2);
+        else if (W <= 8)
+            launch_pf<T, TRANS, 8>(M, A_, va, da, B_, vb, db, b, x, xv_bytes, pf_shm, sk1, sk2);
+        else if (W <= 12)
+            launch_pf<T, TRANS, 12>(M, A_, va, da, B_, vb, db, b, x, xv_bytes, pf_shm, sk1,
+                                    sk2);
+        else
+            launch_pf<T, TRANS, 16>(M, A_, va, da, B_, vb, db, b, x, xv_bytes, pf_shm, sk1,
+                                    sk2);
+        RS_CHECK_LAUNCH();
+        return;
+    }
+    const int bs = maxc > 64 ? 256 : 64;
     const size_t shm = lds ? (size_t)(M->max_block_rows + 1) * sizeof(T) : 0;
     auto kern = lds ? k_ilu_apply<T, true, TRANS> : k_ilu_apply<T, false, TRANS>;
-    hipLaunchKernelGGL(kern, dim3(M->nblocks), dim3(bs), shm, s, M->bstart.p,
-                       A_.blk_lev.p, (const LevelMeta *)A_.lev_meta.p,
-                       A_.rows.p, A_.ell_col.p, va, B_.blk_lev.p,
-                       (const LevelMeta *)B_.lev_meta.p, B_.rows.p,
-                       B_.ell_col.p, vb, dg, b, x, scratch, sk1, sk2);
+    hipLaunchKernelGGL(kern, dim3(M->nblocks), dim3(bs), shm, s, M->bstart.p, A_.W,
+                       A_.blk_lev.p, (const LevelMeta *)A_.lev_meta.p, A_.rows.p,
+                       A_.ell_col.p, va, da, B_.blk_lev.p, (const LevelMeta *)B_.lev_meta.p,
+                       B_.rows.p, B_.ell_col.p, vb, db, b, x, scratch, sk1, sk2);
     RS_CHECK_LAUNCH();
 }
 
-__global__ void k_copy_skip(const double *__restrict__ a, double *__restrict__ b,
-                            int n, const int *sk1, const int *sk2)
+__global__ void k_copy_skip(const double *__restrict__ a, double *__restrict__ b, int n,
+                            const int *sk1, const int *sk2)
 {
     if ((sk1 && *sk1) || (sk2 && *sk2))
         return;
@@ -577,8 +774,8 @@ __global__ void k_copy_skip(const double *__restrict__ a, double *__restrict__ b
         b[i] = a[i];
 }
 
-__global__ void k_copy_skip32(const float *__restrict__ a, float *__restrict__ b,
-                              int n, const int *sk1, const int *sk2)
+__global__ void k_copy_skip32(const float *__restrict__ a, float *__restrict__ b, int n,
+                              const int *sk1, const int *sk2)
 {
     if ((sk1 && *sk1) || (sk2 && *sk2))
         return;
@@ -588,52 +785,48 @@ __global__ void k_copy_skip32(const float *__restrict__ a, float *__restrict__ b
 }
 
 // apply with optional device skip flags (done / cycle_end of a solver)
-void prec_apply(rsgpu_prec *M, const double *b, double *x,
-                const DevState *st)
+void prec_apply(rsgpu_prec *M, const double *b, double *x, const DevState *st)
 {
     const int *sk1 = st ? &st->done : nullptr;
     const int *sk2 = st ? &st->cycle_end : nullptr;
     hipStream_t s = M->ctx->stream;
     if (M->kind == RSGPU_PREC_IDENTITY) {
         if (M->n)
-            hipLaunchKernelGGL(k_copy_skip, dim3((M->n + 255) / 256), dim3(256),
-                               0, s, b, x, M->n, sk1, sk2);
+            hipLaunchKernelGGL(k_copy_skip, dim3((M->n + 255) / 256), dim3(256), 0, s, b, x,
+                               M->n, sk1, sk2);
         RS_CHECK_LAUNCH();
         return;
     }
     if (M->precision == RSGPU_FP64) {
-        launch_apply<double, false>(M, M->L, M->ell64_L.p, M->U, M->ell64_U.p,
-                                    M->dg64.p, b, x, M->scratch64.p, sk1, sk2);
+        launch_apply<double, false>(M, M->L, M->ell64_L.p, nullptr, M->U, M->ell64_U.p,
+                                    M->dgU64.p, b, x, M->scratch64.p, sk1, sk2);
         return;
     }
     // FP32 factors on FP64 vectors
     M->tmpa32.alloc(M->n);
     M->tmpb32.alloc(M->n);
     convert_f64_to_f32(s, b, M->tmpa32.p, M->n);
-    launch_apply<float, false>(M, M->L, M->ell32_L.p, M->U, M->ell32_U.p,
-                               M->dg32.p, M->tmpa32.p, M->tmpb32.p,
-                               M->scratch32.p, sk1, sk2);
+    launch_apply<float, false>(M, M->L, M->ell32_L.p, nullptr, M->U, M->ell32_U.p, M->dgU32.p,
+                               M->tmpa32.p, M->tmpb32.p, M->scratch32.p, sk1, sk2);
     convert_f32_to_f64(s, M->tmpb32.p, x, M->n);
 }
 
-void prec_apply(rsgpu_prec *M, const float *b, float *x,
-                const DevState *st)
+void prec_apply(rsgpu_prec *M, const float *b, float *x, const DevState *st)
 {
     const int *sk1 = st ? &st->done : nullptr;
     const int *sk2 = st ? &st->cycle_end : nullptr;
     hipStream_t s = M->ctx->stream;
     if (M->kind == RSGPU_PREC_IDENTITY) {
         if (M->n)
-            hipLaunchKernelGGL(k_copy_skip32, dim3((M->n + 255) / 256),
-                               dim3(256), 0, s, b, x, M->n, sk1, sk2);
+            hipLaunchKernelGGL(k_copy_skip32, dim3((M->n + 255) / 256), dim3(256), 0, s, b, x,
+                               M->n, sk1, sk2);
         RS_CHECK_LAUNCH();
         return;
     }
     if (M->precision != RSGPU_FP32)
-        throw RsError{RSGPU_BAD_ARGUMENT,
-                      "FP32 apply needs an FP32 preconditioner"};
-    launch_apply<float, false>(M, M->L, M->ell32_L.p, M->U, M->ell32_U.p,
-                               M->dg32.p, b, x, M->scratch32.p, sk1, sk2);
+        throw RsError{RSGPU_BAD_ARGUMENT, "FP32 apply needs an FP32 preconditioner"};
+    launch_apply<float, false>(M, M->L, M->ell32_L.p, nullptr, M->U, M->ell32_U.p, M->dgU32.p,
+                               b, x, M->scratch32.p, sk1, sk2);
 }
 
 void prec_trans_apply(rsgpu_prec *M, const double *b, double *x)
@@ -647,10 +840,10 @@ void prec_trans_apply(rsgpu_prec *M, const double *b, double *x)
         throw RsError{RSGPU_BAD_ARGUMENT, "transApply needs an FP64 preconditioner"};
     if (!M->has_t) {
         prec_setup_transpose(M);
-        launch_ell<double>(s, M->Ut, M->lu64.p, M->ell64_Ut.p);
-        launch_ell<double>(s, M->Lt, M->lu64.p, M->ell64_Lt.p);
+        launch_ell<double>(s, M->Ut, M->lu64.p, M->ell64_Ut.p, false);
+        launch_ell<double>(s, M->Lt, M->lu64.p, M->ell64_Lt.p, false);
+        launch_rowdiag<double>(s, M->Ut, M->diag.p, M->lu64.p, M->dgUt64.p);
     }
-    launch_apply<double, true>(M, M->Ut, M->ell64_Ut.p, M->Lt, M->ell64_Lt.p,
-                               M->dg64.p, b, x, M->scratch64.p, nullptr,
-                               nullptr);
+    launch_apply<double, true>(M, M->Ut, M->ell64_Ut.p, M->dgUt64.p, M->Lt, M->ell64_Lt.p,
+                               nullptr, b, x, M->scratch64.p, nullptr, nullptr);
 }

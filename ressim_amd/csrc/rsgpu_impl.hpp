@@ -110,14 +110,17 @@ struct rsgpu_csr {
 
 // Level-scheduled sweep data of one triangular factor (see ilu.hip)
 struct SweepData {
+    int W = 4;               // padded row width (entries per lane)
     int nlev_total = 0;      // sum over blocks
     int max_levels = 0;      // per block
     DevBuf<int> blk_lev;     // [nblocks+1] level ranges per block
-    DevBuf<int> lev_meta;    // per level: count, width, ell offset, row offset
+    DevBuf<int> lev_meta;    // per level: count, ell offset, row offset, pad
     DevBuf<int> rows;        // local row per (level,t)
+    DevBuf<int> row_glob;    // global row per (level,t) (diagonal gather)
     DevBuf<int> ell_col;     // local column per slot (nb = zero slot)
     DevBuf<int> ell_src;     // source position in LU, -1 padding
     size_t ell_size = 0;
+    int nrows_lev = 0;
     int max_count = 0, max_width = 0;
 };
 
@@ -145,8 +148,8 @@ struct rsgpu_prec {
     DevBuf<int> status;          // [2]: code, bad row
     SweepData L, U, Lt, Ut;      // forward/backward and transposed sweeps
     bool has_t = false;
-    DevBuf<double> ell64_L, ell64_U, ell64_Lt, ell64_Ut, dg64;
-    DevBuf<float> ell32_L, ell32_U, dg32;
+    DevBuf<double> ell64_L, ell64_U, ell64_Lt, ell64_Ut, dgU64, dgUt64;
+    DevBuf<float> ell32_L, ell32_U, dgU32;
     DevBuf<double> scratch64;    // global fallback vectors for large blocks
     DevBuf<float> scratch32;
     DevBuf<double> tmp64;        // FP32 apply on FP64 vectors
@@ -227,6 +230,16 @@ __device__ __forceinline__ double block_sum(double v, double *sh)
 
 // Publish this group's partial(s); returns true in every thread of the
 // finishing group, whose thread 0 then holds the totals in tot[].
+//
+// The partials are written with agent-scope atomic stores, which the L2
+// writes through to the device coherence point, and the store is waited for
+// (vmcnt(0)) before the arrival ticket is taken.  No group therefore needs a
+// device-wide release fence (on a multi-XCD device that writes back the
+// whole L2 and costs tens of microseconds when every group does it).  Only
+// the finishing group performs an acquire (L2 invalidate) before it reads
+// the partials with agent-scope atomic loads.
+__device__ __forceinline__ void wait_vm_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 template <int BS, int NR>
 __device__ bool block_reduce_finish(RedSlot slot, const double (&v)[NR],
                                     double (&tot)[NR])
@@ -240,25 +253,40 @@ __device__ bool block_reduce_finish(RedSlot slot, const double (&v)[NR],
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int q = 0; q < NR; ++q)
-            slot.partials[(size_t)q * gridDim.x + blockIdx.x] = r[q];
-        __threadfence();
-        unsigned t = atomicAdd(slot.ticket, 1u);
+            __hip_atomic_store(&slot.partials[(size_t)q * gridDim.x + blockIdx.x], r[q],
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        wait_vm_stores();
+        const unsigned t = __hip_atomic_fetch_add(slot.ticket, 1u, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT);
         last = (t == gridDim.x - 1);
     }
     __syncthreads();
     if (!last)
         return false;
-    __threadfence();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 #pragma unroll
     for (int q = 0; q < NR; ++q) {
         double acc = 0;
         for (int i = threadIdx.x; i < (int)gridDim.x; i += BS)
-            acc += slot.partials[(size_t)q * gridDim.x + i];
+            acc += __hip_atomic_load(&slot.partials[(size_t)q * gridDim.x + i],
+                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         tot[q] = block_sum<BS>(acc, sh);
     }
     if (threadIdx.x == 0)
-        *slot.ticket = 0u;
+        __hip_atomic_store(slot.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return true;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  This bijection gives XCD x a contiguous run of logical groups, so
+// neighbouring row ranges (which gather overlapping parts of x) share one
+// L2 instead of all eight fetching the same lines.
+#define RS_NUM_XCD 8
+__device__ __forceinline__ int xcd_remap(int bid, int nwg)
+{
+    const int x = bid % RS_NUM_XCD, k = bid / RS_NUM_XCD;
+    const int per = nwg / RS_NUM_XCD, rem = nwg % RS_NUM_XCD;
+    return x < rem ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
 }
 
 // java.lang.Math.max (NaN wins)

@@ -26,14 +26,16 @@ __device__ __forceinline__ void spmv_core(const int *__restrict__ wg_rows,
                                           const T *__restrict__ x, Epi &epi)
 {
     __shared__ T prod[NNZ_WG];
-    const int r0 = wg_rows[blockIdx.x], r1 = wg_rows[blockIdx.x + 1];
+    const int g = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int r0 = wg_rows[g], r1 = wg_rows[g + 1];
     const int j0 = rowptr[r0], j1 = rowptr[r1];
     if (j1 - j0 <= NNZ_WG) {
 #pragma unroll
         for (int q = 0; q < NNZ_WG / BS; ++q) {
             const int j = j0 + (int)threadIdx.x + q * BS;
             if (j < j1)
-                prod[j - j0] = val[j] * x[colind[j]];
+                prod[j - j0] = __builtin_nontemporal_load(&val[j]) *
+                               x[__builtin_nontemporal_load(&colind[j])];
         }
         __syncthreads();
         for (int r = r0 + (int)threadIdx.x; r < r1; r += BS) {

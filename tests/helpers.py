@@ -14,10 +14,15 @@ def oracle_system(po, pr: P.PressureProblem, with_diag=True):
     here, there = po.mesh_connections(pr.dim, pr.nx, pr.ny, pr.nz)
     rp, ci = po.pattern(pr.n, here, there, md_ptr, md_el, with_diag)
     A = po.assemble_flux(pr.n, here, there, md_ptr, md_el, md_t, rp, ci)
+    flux_only = A.copy()
+    if pr.accumulation:
+        for i in range(pr.n):
+            assert po.csr_add(rp, ci, A, i, i, pr.accumulation) == 0
     A_unpinned = A.copy()
     st = po.zero_rows(rp, ci, A, 1.0, pr.pinned)
     assert st == 0
-    return dict(rowptr=rp, colind=ci, data=A, data_unpinned=A_unpinned, b=pr.rhs(),
+    return dict(rowptr=rp, colind=ci, data=A, data_unpinned=A_unpinned, flux=flux_only,
+                b=pr.rhs(),
                 md_ptr=md_ptr, md_elem=md_el, md_t=md_t, here=here, there=there,
                 fails=fails)
 

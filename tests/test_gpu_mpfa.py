@@ -31,6 +31,10 @@ def test_assembly_bitexact(gpu, oracle, pr):
     assert fails == 0
     np.testing.assert_array_equal(A.getRowPointers(), s["rowptr"])
     np.testing.assert_array_equal(A.getColumnIndices(), s["colind"])
+    np.testing.assert_array_equal(A.getData(), s["flux"])
+    if pr.accumulation:
+        idx = np.arange(pr.n, dtype=np.int32)
+        A.add(idx, idx, np.full(pr.n, pr.accumulation))
     np.testing.assert_array_equal(A.getData(), s["data_unpinned"])
     assert A.nnz == P.stencil_nnz(pr.dim, pr.nx, pr.ny, pr.nz)
     gpu.Matrices.zeroRows(A, 1.0, pr.pinned)

@@ -145,7 +145,7 @@ def test_pattern_size_and_symmetry(oracle):
             row = s["colind"][s["rowptr"][r]:s["rowptr"][r + 1]]
             assert np.all(np.diff(row) > 0)
         # the flux part has zero row sums before locking (conservation)
-        Au = scipy_csr(s["rowptr"], s["colind"], s["data_unpinned"])
+        Au = scipy_csr(s["rowptr"], s["colind"], s["flux"])
         assert np.abs(Au.T @ np.ones(pr.n)).max() < 1e-9
 
 
