@@ -510,18 +510,22 @@ struct LevelRegs {
 
 // All loads of a level are independent of each other and of the vector:
 // row index, level-ordered diagonal, and the lane's W values / columns.
-// Inactive lanes read the level's last row and are ignored.
+// Inactive lanes redo the level's last row and write it to the junk slot
+// nb + 1, so neither loads nor updates sit behind branches (branches make
+// the compiler's wait counting fall back to waiting for every load).
 template <typename T, int W, bool DIV>
 __device__ __forceinline__ void level_load(const LevelMeta &m, const int *__restrict__ rows,
                                            const int *__restrict__ ecol,
                                            const T *__restrict__ eval,
-                                           const T *__restrict__ dgl, LevelRegs<T, W> &R)
+                                           const T *__restrict__ dgl, int junk,
+                                           LevelRegs<T, W> &R)
 {
     const int t = threadIdx.x;
     const bool act = t < m.count;
     const int tt = act ? t : m.count - 1;
     const int r = rows[m.roff + tt];
-    R.r = act ? r : -1;
+    R.r = act ? r : -1 - r;   // inactive: -1 - (row to read)
+    (void)junk;
     if (DIV)
         R.d = dgl[m.roff + tt];
     const size_t q = (size_t)m.off + (size_t)tt * W;
@@ -530,15 +534,16 @@ __device__ __forceinline__ void level_load(const LevelMeta &m, const int *__rest
 }
 
 template <typename T, int W, int FORM, bool DIV>
-__device__ __forceinline__ void level_compute(const LevelRegs<T, W> &R, T *xv)
+__device__ __forceinline__ void level_compute(const LevelRegs<T, W> &R, T *xv, int junk)
 {
-    if (R.r < 0)
-        return;
+    const bool act = R.r >= 0;
+    const int rr = act ? R.r : -1 - R.r;
+    const int wr = act ? R.r : junk;
     T xs[W];
 #pragma unroll
     for (int s = 0; s < W; ++s)
         xs[s] = xv[R.c.c[s]];
-    const T rhs = xv[R.r];
+    const T rhs = xv[rr];
     T acc;
     if (FORM == FORM_SUM) {
         T sum = 0;
@@ -554,50 +559,49 @@ __device__ __forceinline__ void level_compute(const LevelRegs<T, W> &R, T *xv)
     }
     if (DIV)
         acc = acc / R.d;
-    xv[R.r] = acc;
+    xv[wr] = acc;
 }
 
 // Four register sets in rotation: while level l computes, levels l+1..l+3
-// are loading.  No register copies between the sets.
+// are loading.  The prefetch index is clamped instead of branched on (the
+// tail reloads the last level), so every load and wait is unconditional.
 template <typename T, int W, int FORM, bool DIV>
 __device__ __forceinline__ void sweep_pf(int nl, const LevelMeta *sm,
                                          const int *__restrict__ rows,
                                          const int *__restrict__ ecol,
                                          const T *__restrict__ eval,
-                                         const T *__restrict__ dgl, T *xv)
+                                         const T *__restrict__ dgl, T *xv, int junk)
 {
     if (nl <= 0)
         return;
+    const int last = nl - 1;
     LevelRegs<T, W> r0, r1, r2, r3;
-    level_load<T, W, DIV>(sm[0], rows, ecol, eval, dgl, r0);
-    if (1 < nl)
-        level_load<T, W, DIV>(sm[1], rows, ecol, eval, dgl, r1);
-    if (2 < nl)
-        level_load<T, W, DIV>(sm[2], rows, ecol, eval, dgl, r2);
+    level_load<T, W, DIV>(sm[0], rows, ecol, eval, dgl, junk, r0);
+    level_load<T, W, DIV>(sm[min(1, last)], rows, ecol, eval, dgl, junk, r1);
+    level_load<T, W, DIV>(sm[min(2, last)], rows, ecol, eval, dgl, junk, r2);
+// the scheduling barriers keep each prefetch issued before the compute of
+// the current level (the scheduler would otherwise sink the loads past the
+// LDS barrier and batch them, exposing a full memory latency per batch)
+#define RS_PF_STAGE(LOADREG, LOADLEV, COMPREG)                                  \
+    level_load<T, W, DIV>(sm[min((LOADLEV), last)], rows, ecol, eval, dgl, junk, \
+                          LOADREG);                                             \
+    asm volatile("" ::: "memory");                                              \
+    level_compute<T, W, FORM, DIV>(COMPREG, xv, junk);                          \
+    lds_barrier();                                                              \
+    __builtin_amdgcn_sched_barrier(0);
     for (int l = 0; l < nl; l += 4) {
-        if (l + 3 < nl)
-            level_load<T, W, DIV>(sm[l + 3], rows, ecol, eval, dgl, r3);
-        level_compute<T, W, FORM, DIV>(r0, xv);
-        lds_barrier();
+        RS_PF_STAGE(r3, l + 3, r0)
         if (l + 1 >= nl)
             break;
-        if (l + 4 < nl)
-            level_load<T, W, DIV>(sm[l + 4], rows, ecol, eval, dgl, r0);
-        level_compute<T, W, FORM, DIV>(r1, xv);
-        lds_barrier();
+        RS_PF_STAGE(r0, l + 4, r1)
         if (l + 2 >= nl)
             break;
-        if (l + 5 < nl)
-            level_load<T, W, DIV>(sm[l + 5], rows, ecol, eval, dgl, r1);
-        level_compute<T, W, FORM, DIV>(r2, xv);
-        lds_barrier();
+        RS_PF_STAGE(r1, l + 5, r2)
         if (l + 3 >= nl)
             break;
-        if (l + 6 < nl)
-            level_load<T, W, DIV>(sm[l + 6], rows, ecol, eval, dgl, r2);
-        level_compute<T, W, FORM, DIV>(r3, xv);
-        lds_barrier();
+        RS_PF_STAGE(r2, l + 6, r3)
     }
+#undef RS_PF_STAGE
 }
 
 // One wave per block (every level has at most 64 rows); vector and both
@@ -632,12 +636,13 @@ __global__ void __launch_bounds__(64)
     if (threadIdx.x == 0)
         xv[nb] = T(0);
     __syncthreads();
+    const int junk = nb + 1;
     if (!TRANS) {
-        sweep_pf<T, W, FORM_SUM, false>(nla, sm, rowsA, colA, valA, dglA, xv);
-        sweep_pf<T, W, FORM_SUM, true>(nlb, sm + nla, rowsB, colB, valB, dglB, xv);
+        sweep_pf<T, W, FORM_SUM, false>(nla, sm, rowsA, colA, valA, dglA, xv, junk);
+        sweep_pf<T, W, FORM_SUM, true>(nlb, sm + nla, rowsB, colB, valB, dglB, xv, junk);
     } else {
-        sweep_pf<T, W, FORM_SUBSEQ, true>(nla, sm, rowsA, colA, valA, dglA, xv);
-        sweep_pf<T, W, FORM_SUBSEQ, false>(nlb, sm + nla, rowsB, colB, valB, dglB, xv);
+        sweep_pf<T, W, FORM_SUBSEQ, true>(nla, sm, rowsA, colA, valA, dglA, xv, junk);
+        sweep_pf<T, W, FORM_SUBSEQ, false>(nlb, sm + nla, rowsB, colB, valB, dglB, xv, junk);
     }
     __syncthreads();
     for (int r = threadIdx.x; r < nb; r += blockDim.x)
@@ -735,8 +740,8 @@ static void launch_apply(rsgpu_prec *M, const SweepData &A_, const T *va, const 
     if (M->n == 0)
         return;
     const int maxc = std::max(A_.max_count, B_.max_count);
-    const bool lds = M->max_block_rows <= (int)(RS_LDS_BYTES / sizeof(T)) - 1;
-    const size_t xv_bytes = ((size_t)(M->max_block_rows + 1) * sizeof(T) + 15) & ~(size_t)15;
+    const bool lds = M->max_block_rows <= (int)(RS_LDS_BYTES / sizeof(T)) - 2;
+    const size_t xv_bytes = ((size_t)(M->max_block_rows + 2) * sizeof(T) + 15) & ~(size_t)15;
     const size_t pf_shm =
         xv_bytes + (size_t)(A_.max_levels + B_.max_levels) * sizeof(LevelMeta);
     const int W = A_.W;

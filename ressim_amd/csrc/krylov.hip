@@ -29,6 +29,7 @@ struct rsgpu_solver {
     rsgpu_ctx *ctx = nullptr;
     rsgpu_solver_opts o{};
     int n = 0;
+    size_t ldv = 0;               // internal vector stride (multiple of 4)
     DevState *st = nullptr;       // device
     DevState *st_in = nullptr;    // inner solve (mixed)
     DevBuf<double> hist, hist_in;
@@ -46,10 +47,11 @@ struct rsgpu_solver {
     int *flags_host = nullptr;    // pinned [2][4]
 };
 
+// vector kernels: at least 4 elements per lane, at most 1024 groups
 static int grid_for(int n)
 {
-    int g = (n + RS_VBS - 1) / RS_VBS;
-    return std::max(1, std::min(g, 2048));
+    int g = (n + 4 * RS_VBS - 1) / (4 * RS_VBS);
+    return std::max(1, std::min(g, 1024));
 }
 
 __device__ __forceinline__ bool stopped(const DevState *st, bool inner_step)
@@ -150,6 +152,36 @@ __global__ void __launch_bounds__(RS_SPMV_BS)
     if (block_reduce_finish<RS_SPMV_BS, NR>(slot, v, tot) && threadIdx.x == 0)
         finish_scalar(st, fin, tot[0], NR > 1 ? tot[NR - 1] : 0.0);
 }
+
+// y = A x without a reduction, skipped when the solve has stopped
+template <typename T>
+__global__ void __launch_bounds__(RS_SPMV_BS)
+    k_spmv_skip(const int *__restrict__ wg_rows, const int *__restrict__ rowptr,
+                const int *__restrict__ colind, const T *__restrict__ val,
+                const T *__restrict__ x, T *__restrict__ y, const DevState *st,
+                int inner_step)
+{
+    if (stopped(st, inner_step))
+        return;
+    EpiMult<T> epi{y};
+    spmv_core<T, RS_SPMV_BS, RS_SPMV_NNZ>(wg_rows, rowptr, colind, val, x, epi);
+}
+
+// 16-byte vector of T for the solver's internal vectors (leading dimension
+// padded to a multiple of 4 elements, so every internal vector is 16-byte
+// aligned); element-wise arithmetic is unchanged by the vector access.
+template <typename T>
+struct V16;
+template <>
+struct V16<double> {
+    using type = double2;
+    static constexpr int N = 2;
+};
+template <>
+struct V16<float> {
+    using type = float4;
+    static constexpr int N = 4;
+};
 
 // =========================================================== BiCGstab kernels
 // rho1 = rtilde . r; breakdown checks; beta
@@ -315,12 +347,34 @@ __global__ void __launch_bounds__(RS_VBS)
 {
     if (stopped(st, true))
         return;
+    using VT = typename V16<T>::type;
+    constexpr int N = V16<T>::N;
     const int ld = st->m + 1;
     const T h = k > 0 ? (T)(-st->H[(k - 1) + i * ld]) : T(0);
+    const bool upd = k > 0 && h != T(0);
     double a = 0;
-    for (int j = blockIdx.x * RS_VBS + threadIdx.x; j < n; j += gridDim.x * RS_VBS) {
+    const int nv = n / N;
+    const int stride = gridDim.x * RS_VBS;
+    for (int j = blockIdx.x * RS_VBS + threadIdx.x; j < nv; j += stride) {
+        VT wv = reinterpret_cast<const VT *>(w)[j];
+        const VT kv = reinterpret_cast<const VT *>(vk)[j];
+        T *we = reinterpret_cast<T *>(&wv);
+        const T *ke = reinterpret_cast<const T *>(&kv);
+        if (upd) {
+            const VT pv = reinterpret_cast<const VT *>(vprev)[j];
+            const T *pe = reinterpret_cast<const T *>(&pv);
+#pragma unroll
+            for (int c = 0; c < N; ++c)
+                we[c] += h * pe[c];
+            reinterpret_cast<VT *>(w)[j] = wv;
+        }
+#pragma unroll
+        for (int c = 0; c < N; ++c)
+            a += (double)(we[c] * ke[c]);
+    }
+    for (int j = nv * N + blockIdx.x * RS_VBS + threadIdx.x; j < n; j += stride) {
         T wj = w[j];
-        if (k > 0 && h != T(0)) {
+        if (upd) {
             wj += h * vprev[j];
             w[j] = wj;
         }
@@ -339,12 +393,32 @@ __global__ void __launch_bounds__(RS_VBS)
 {
     if (stopped(st, true))
         return;
+    using VT = typename V16<T>::type;
+    constexpr int N = V16<T>::N;
     const int ld = st->m + 1;
     const T h = (T)(-st->H[i + i * ld]);
+    const bool upd = h != T(0);
     double a = 0;
-    for (int j = blockIdx.x * RS_VBS + threadIdx.x; j < n; j += gridDim.x * RS_VBS) {
+    const int nv = n / N;
+    const int stride = gridDim.x * RS_VBS;
+    for (int j = blockIdx.x * RS_VBS + threadIdx.x; j < nv; j += stride) {
+        VT wv = reinterpret_cast<const VT *>(w)[j];
+        T *we = reinterpret_cast<T *>(&wv);
+        if (upd) {
+            const VT iv = reinterpret_cast<const VT *>(vi)[j];
+            const T *ie = reinterpret_cast<const T *>(&iv);
+#pragma unroll
+            for (int c = 0; c < N; ++c)
+                we[c] += h * ie[c];
+            reinterpret_cast<VT *>(w)[j] = wv;
+        }
+#pragma unroll
+        for (int c = 0; c < N; ++c)
+            a += (double)(we[c] * we[c]);
+    }
+    for (int j = nv * N + blockIdx.x * RS_VBS + threadIdx.x; j < n; j += stride) {
         T wj = w[j];
-        if (h != T(0)) {
+        if (upd) {
             wj += h * vi[j];
             w[j] = wj;
         }
@@ -633,10 +707,9 @@ void run_bicgstab(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const T *b,
 {
     hipStream_t s = S->ctx->stream;
     const int n = A->n;
-    T *p = work, *sv = work + n, *phat = work + 2 * (size_t)n,
-      *shat = work + 3 * (size_t)n, *t = work + 4 * (size_t)n,
-      *v = work + 5 * (size_t)n, *r = work + 6 * (size_t)n,
-      *rt = work + 7 * (size_t)n;
+    const size_t L = S->ldv;
+    T *p = work, *sv = work + L, *phat = work + 2 * L, *shat = work + 3 * L,
+      *t = work + 4 * L, *v = work + 5 * L, *r = work + 6 * L, *rt = work + 7 * L;
     const int G = grid_for(n), E = (n + RS_VBS - 1) / RS_VBS;
     RedSlot sl = slot_of(S);
     // r = b - A x; rtilde = r; monitor(||r||) at iter 0
@@ -677,8 +750,8 @@ void run_gmres(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const T *b, T *x,
 {
     hipStream_t s = S->ctx->stream;
     const int n = A->n;
-    T *w = work, *u = work + n, *r = work + 2 * (size_t)n,
-      *V = work + 3 * (size_t)n;
+    const size_t L = S->ldv;
+    T *w = work, *u = work + L, *r = work + 2 * L, *V = work + 3 * L;
     const int G = grid_for(n), E = (n + RS_VBS - 1) / RS_VBS;
     RedSlot sl = slot_of(S);
     // u = b - A x; r = M u; normr; outer monitor at iter 0
@@ -693,18 +766,19 @@ void run_gmres(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const T *b, T *x,
         hipLaunchKernelGGL(k_gm_start<T>, dim3(G), dim3(RS_VBS), 0, s, n, r, V,
                            st, sl);
         for (int i = 0; i < m; ++i) {
-            T *vi = V + (size_t)i * n;
-            launch_spmv_red<T>(S, A, vi, EpiMultDot<T>{u, nullptr, nullptr, 0},
-                               st, FIN_NONE, 1);
+            T *vi = V + (size_t)i * L;
+            hipLaunchKernelGGL(k_spmv_skip<T>, dim3(A->plan.nwg), dim3(RS_SPMV_BS), 0, s,
+                               A->plan.wg_rows.p, A->rowptr.p, A->colind.p,
+                               csr_values<T>(A), vi, u, st, 1);
             prec_apply(M, u, w, st);
             for (int k = 0; k <= i; ++k)
                 hipLaunchKernelGGL(k_gm_mgs<T>, dim3(G), dim3(RS_VBS), 0, s, n, i,
-                                   k, w, k > 0 ? V + (size_t)(k - 1) * n : V,
-                                   V + (size_t)k * n, st, sl);
+                                   k, w, k > 0 ? V + (size_t)(k - 1) * L : V,
+                                   V + (size_t)k * L, st, sl);
             hipLaunchKernelGGL(k_gm_last<T>, dim3(G), dim3(RS_VBS), 0, s, n, i, w,
                                vi, st, sl);
             hipLaunchKernelGGL(k_gm_scale<T>, dim3(E), dim3(RS_VBS), 0, s, n, w,
-                               V + (size_t)(i + 1) * n, st);
+                               V + (size_t)(i + 1) * L, st);
             RS_CHECK_LAUNCH();
             if ((i + 1) % every == 0 && i + 1 < m) {
                 P.post();
@@ -714,8 +788,7 @@ void run_gmres(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const T *b, T *x,
             }
         }
         hipLaunchKernelGGL(k_gm_backsolve, dim3(1), dim3(64), 0, s, st);
-        hipLaunchKernelGGL(k_gm_xupdate<T>, dim3(E), dim3(RS_VBS), 0, s, n, V,
-                           (size_t)n, x, st);
+        hipLaunchKernelGGL(k_gm_xupdate<T>, dim3(E), dim3(RS_VBS), 0, s, n, V, L, x, st);
         launch_resid<T>(S, A, b, x, u, nullptr, nullptr, st, FIN_NONE);
         // r = M u belongs to the outer loop: clear the inner-loop flag first
         hipLaunchKernelGGL(k_clear_cycle, dim3(1), dim3(64), 0, s, st);
@@ -776,7 +849,7 @@ extern "C" int rsgpu_solver_create(rsgpu_ctx *ctx, const rsgpu_solver_opts *opts
         RS_HIP(hipMalloc((void **)&S->st_in, sizeof(DevState)));
         const int maxwg = std::max({grid_for(n), (n + 7) / 8, 4096});
         S->partials.alloc((size_t)maxwg * 2);
-        S->ticket.alloc(4);
+        S->ticket.alloc(2 * RS_TICKET_WORDS);
         S->ticket.zero(ctx->stream);
         const int m = std::max(1, opts->restart);
         S->gmres_small.alloc((size_t)2 * ((m + 1) * m + 3 * (m + 1)));
@@ -794,9 +867,10 @@ extern "C" int rsgpu_solver_create(rsgpu_ctx *ctx, const rsgpu_solver_opts *opts
             nvec64 = 2;   // r, scratch
             nvec32 = 2 + (inner_kind == RSGPU_GMRES ? 3 + (size_t)m + 1 : 8);
         }
-        S->w64.alloc(std::max<size_t>(1, nvec64 * (size_t)n));
+        S->ldv = ((size_t)n + 3) & ~(size_t)3;
+        S->w64.alloc(std::max<size_t>(1, nvec64 * S->ldv));
         if (nvec32)
-            S->w32.alloc(std::max<size_t>(1, nvec32 * (size_t)n));
+            S->w32.alloc(std::max<size_t>(1, nvec32 * S->ldv));
         RS_HIP(hipEventCreateWithFlags(&S->ev[0], hipEventDisableTiming));
         RS_HIP(hipEventCreateWithFlags(&S->ev[1], hipEventDisableTiming));
         RS_HIP(hipHostMalloc((void **)&S->flags_host, sizeof(int) * 8,
@@ -849,7 +923,7 @@ void run_ir(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const double *b,
 {
     hipStream_t s = S->ctx->stream;
     const int n = A->n;
-    double *r = S->w64.p, *z = S->w64.p + n;
+    double *r = S->w64.p, *z = S->w64.p + S->ldv;
     const int E = (n + RS_VBS - 1) / RS_VBS;
     Poller P{S, st};
     int done = 0, cend = 0;
@@ -877,7 +951,7 @@ void run_mixed(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const double *b,
     const int n = A->n;
     const int m = std::max(1, S->o.restart);
     double *r = S->w64.p;
-    float *r32 = S->w32.p, *z32 = S->w32.p + n, *work32 = S->w32.p + 2 * (size_t)n;
+    float *r32 = S->w32.p, *z32 = S->w32.p + S->ldv, *work32 = S->w32.p + 2 * S->ldv;
     const int E = (n + RS_VBS - 1) / RS_VBS;
     if (!A->val32.p)
         throw RsError{RSGPU_BAD_ARGUMENT,
@@ -937,7 +1011,7 @@ double dev_norm2(rsgpu_solver *S, const double *v, int n)
     hipStream_t s = S->ctx->stream;
     double *out = S->partials.p + S->partials.n - 1;
     // partials are sized for the largest grid; the last slot is free
-    RedSlot sl{S->partials.p, S->ticket.p + 1};
+    RedSlot sl{S->partials.p, S->ticket.p + RS_TICKET_WORDS};
     hipLaunchKernelGGL(k_norm2_plain, dim3(grid_for(n)), dim3(RS_VBS), 0, s, n, v,
                        out, sl);
     RS_CHECK_LAUNCH();

@@ -233,11 +233,19 @@ __device__ __forceinline__ double block_sum(double v, double *sh)
 //
 // The partials are written with agent-scope atomic stores, which the L2
 // writes through to the device coherence point, and the store is waited for
-// (vmcnt(0)) before the arrival ticket is taken.  No group therefore needs a
+// (vmcnt(0)) before an arrival ticket is taken.  No group therefore needs a
 // device-wide release fence (on a multi-XCD device that writes back the
-// whole L2 and costs tens of microseconds when every group does it).  Only
-// the finishing group performs an acquire (L2 invalidate) before it reads
-// the partials with agent-scope atomic loads.
+// whole L2).  Arrivals are counted in two levels -- RS_NGROUP counters on
+// separate 128-byte lines, then one global counter taken by the last
+// arrival of each group -- so no single address serialises thousands of
+// atomics.  Only the finishing group performs an acquire (L2 invalidate)
+// before it reads the partials with agent-scope atomic loads.  Every
+// counter is reset by the arrival that completes it, ready for the next
+// launch.
+#define RS_NGROUP 64
+#define RS_TPAD 32
+#define RS_TICKET_WORDS ((RS_NGROUP + 1) * RS_TPAD)
+
 __device__ __forceinline__ void wait_vm_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 template <int BS, int NR>
@@ -256,9 +264,25 @@ __device__ bool block_reduce_finish(RedSlot slot, const double (&v)[NR],
             __hip_atomic_store(&slot.partials[(size_t)q * gridDim.x + blockIdx.x], r[q],
                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         wait_vm_stores();
-        const unsigned t = __hip_atomic_fetch_add(slot.ticket, 1u, __ATOMIC_RELAXED,
-                                                  __HIP_MEMORY_SCOPE_AGENT);
-        last = (t == gridDim.x - 1);
+        const int G = (int)gridDim.x;
+        const int ng = G < RS_NGROUP ? G : RS_NGROUP;
+        const int g = (int)blockIdx.x % ng;
+        const int gsize = (G - g + ng - 1) / ng;
+        unsigned *gt = slot.ticket + (1 + g) * RS_TPAD;
+        int fin = 0;
+        const unsigned t =
+            __hip_atomic_fetch_add(gt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((int)t == gsize - 1) {
+            __hip_atomic_store(gt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned u = __hip_atomic_fetch_add(slot.ticket, 1u, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT);
+            if ((int)u == ng - 1) {
+                __hip_atomic_store(slot.ticket, 0u, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+                fin = 1;
+            }
+        }
+        last = fin;
     }
     __syncthreads();
     if (!last)
@@ -272,8 +296,6 @@ __device__ bool block_reduce_finish(RedSlot slot, const double (&v)[NR],
                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         tot[q] = block_sum<BS>(acc, sh);
     }
-    if (threadIdx.x == 0)
-        __hip_atomic_store(slot.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return true;
 }
 
