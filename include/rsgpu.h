@@ -112,7 +112,8 @@ enum rsgpu_prec_kind {
     RSGPU_PREC_IDENTITY = 0,   /* AbstractIterativeSolver.IdentityPreconditioner */
     RSGPU_PREC_ILU0 = 1,       /* mtj/sparse/ILU.java                            */
     RSGPU_PREC_BLOCK_ILU0 = 2, /* distributed/BlockDiagonalPreconditioner + ILU  */
-    RSGPU_PREC_DIAGONAL = 3    /* mtj/sparse/DiagonalPreconditioner.java         */
+    RSGPU_PREC_DIAGONAL = 3,   /* mtj/sparse/DiagonalPreconditioner.java         */
+    RSGPU_PREC_AMG = 4         /* mtj/sparse/AMG.java (smoothed aggregation)     */
 };
 enum rsgpu_precision { RSGPU_FP64 = 0, RSGPU_FP32 = 1 };
 
@@ -142,6 +143,41 @@ int rsgpu_prec_get_factors(const rsgpu_prec *M, int32_t *rowptr_host,
                            int32_t *colind_host, double *lu_host, int *nnz);
 int rsgpu_prec_info(const rsgpu_prec *M, int *nnz, int *nlevels_lower,
                     int *nlevels_upper);
+
+/* AMG parameters: the constructor arguments of AMG(omegaPreF, omegaPreR,
+ * omegaPostF, omegaPostR, nu1, nu2, gamma, min, omega)
+ * (mtj/sparse/AMG.java:140-156, defaults :204-206) plus three of this
+ * library's: block_rows (SSOR smoother blocks of that many contiguous rows;
+ * couplings leaving a block use the vector from the start of the SSOR
+ * application; 0 = one block per level), ordering (0: rows in natural order
+ * as SSOR.java; 1: rows by greedy colour, multicolour SSOR, whose sweep is
+ * one parallel step per colour), zero_guess (0: the cycle starts from x's
+ * old content as AMG.apply does, AMG.java:208-216; 1: from zero, a fixed
+ * linear preconditioner), max_coarse (largest dense coarsest level
+ * accepted).  block_rows = 0 with ordering = 0 is SSOR.java exactly. */
+typedef struct {
+    double omega_pre_f, omega_pre_r, omega_post_f, omega_post_r;
+    int nu1, nu2, gamma, min;
+    double omega;
+    int reverse;          /* 1 SSOR, 0 SOR (AMG.java:148,186) */
+    int block_rows;
+    int zero_guess;
+    int max_coarse;
+    int ordering;
+} rsgpu_amg_params;
+void rsgpu_amg_params_default(rsgpu_amg_params *p);
+/* Takes effect at the next rsgpu_prec_set_matrix (AMG.setMatrix,
+ * AMG.java:228-294, which rebuilds the hierarchy every call). */
+int rsgpu_prec_set_amg_params(rsgpu_prec *M, const rsgpu_amg_params *p);
+/* Hierarchy introspection (tests): level sizes / nonzeros; returns the
+ * level count m (the last level is solved by a dense LU). */
+int rsgpu_amg_levels(const rsgpu_prec *M, int *sizes, int *nnzs, int cap);
+/* Level k's matrix (CSR), and for k < m-1 its aggregate of every row and
+ * the smoothed prolongator P (CSR, n_k x n_{k+1}); pointers may be NULL. */
+int rsgpu_amg_get_level(const rsgpu_prec *M, int k, int32_t *rowptr,
+                        int32_t *colind, double *data, int32_t *aggregate,
+                        int32_t *p_rowptr, int32_t *p_colind, double *p_data,
+                        int *p_nnz);
 
 /* ---- IterativeSolver (mtj/sparse/IterativeSolver.java:43-67) ------------- */
 enum rsgpu_solver_kind {

@@ -411,11 +411,14 @@ typedef struct {
     int kind, n;
     int *rowptr, *colind, *diagind;
     double *lu, *y;
+    or_amg *amg;
 } prec_t;
 
 static void prec_free(prec_t *p)
 {
-    if (p->kind != 0) {
+    if (p->kind == 3) {
+        or_amg_free(p->amg);
+    } else if (p->kind != 0) {
         free(p->rowptr);
         free(p->colind);
         free(p->diagind);
@@ -434,6 +437,8 @@ static int prec_setup(prec_t *p, int n, const int *rowptr, const int *colind,
     p->y = (double *)malloc(sizeof(double) * n);
     if (p->kind == 0)
         return OR_OK;
+    if (p->kind == 3)
+        return or_amg_setup(n, rowptr, colind, data, &o->amg, &p->amg, bad_row);
     int nnz;
     p->rowptr = (int *)malloc(sizeof(int) * (n + 1));
     if (p->kind == 1) {
@@ -466,6 +471,11 @@ static void prec_apply(prec_t *p, const double *b, double *x)
 {
     if (p->kind == 0) {
         v_copy(p->n, x, b);
+        return;
+    }
+    if (p->kind == 3) {
+        /* AMG.apply starts the cycle from x's old content (AMG.java:208-216) */
+        or_amg_apply(p->amg, b, x, 0);
         return;
     }
     or_ilu_apply(p->n, p->rowptr, p->colind, p->lu, p->diagind, b, x, p->y);

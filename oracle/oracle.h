@@ -11,7 +11,8 @@
  * Reference files restated (paths relative to the reference root):
  *   mtj/src/no/uib/cipr/matrix/sparse/{CompRowMatrix,BiCGstab,GMRES,ILU,
  *     UnitLowerCompRowMatrix,UpperCompRowMatrix,DefaultIterationMonitor,
- *     AbstractIterationMonitor,IR,Arrays}.java
+ *     AbstractIterationMonitor,IR,Arrays,AMG,SSOR,CompColMatrix}.java,
+ *   mtj/src/no/uib/cipr/matrix/DenseLU.java (amg.c)
  *   mtj/src/no/uib/cipr/matrix/{DenseVector,GivensRotation,Matrices,
  *     AbstractTriangDenseMatrix,DenseMatrix}.java
  *   mtj/src/no/uib/cipr/matrix/distributed/BlockDiagonalPreconditioner.java
@@ -99,14 +100,48 @@ int or_block_filter(int n, const int *rowptr, const int *colind,
                     const double *data, int nblocks, const int *bstart,
                     int *browptr, int *bcolind, double *bdata, int *src_index);
 
+/* ---- Smoothed-aggregation AMG (amg.c) ------------------------------------ */
+typedef struct {
+    double omega_pre_f, omega_pre_r, omega_post_f, omega_post_r;
+    int nu1, nu2, gamma, min;
+    double omega;       /* Jacobi damping of the tentative prolongator */
+    int reverse;        /* SSOR (1) or SOR (0) smoothing */
+    int mode;           /* 0 sequential aggregation as AMG.java, 1 parallel */
+    int block_rows;     /* smoother blocks per level (0: one block) */
+    int max_coarse;     /* largest dense coarsest level accepted */
+    int zero_guess;     /* 1: each cycle starts from zero instead of x */
+    int ordering;       /* SSOR row order: 0 natural, 1 greedy colour */
+} or_amg_params;
+typedef struct or_amg or_amg;
+void or_amg_params_default(or_amg_params *p);
+int or_amg_setup(int n, const int *rowptr, const int *colind,
+                 const double *data, const or_amg_params *p, or_amg **out,
+                 int *bad_row);
+void or_amg_apply(or_amg *A, const double *b, double *x, int transpose);
+void or_amg_free(or_amg *A);
+int or_amg_levels(const or_amg *A, int *sizes, int *nnzs, int cap);
+int or_amg_get_level(const or_amg *A, int k, int *rowptr, int *colind,
+                     double *data, int *pt, int *Prp, int *Pci, double *Pv);
+int or_amg_p_nnz(const or_amg *A, int k);
+int or_amg_aggregate(int n, const int *rowptr, const int *colind,
+                     const double *data, double eps, int mode, int *pt,
+                     int *nagg, int *nroots);
+void or_hashset_order(const int *cols, int cnt, int *out);
+uint64_t or_amg_priority(int i);
+int or_ssor_apply(int n, const int *rowptr, const int *colind,
+                  const double *data, int block_rows, int ordering,
+                  double omegaF, double omegaR, int reverse, const double *b,
+                  double *x, int *order_out);
+
 /* ---- Krylov solvers ------------------------------------------------------ */
 typedef struct {
     int max_iter;
     double rtol, atol, dtol;
     int restart;        /* GMRES(m) */
-    /* preconditioner: 0 identity, 1 ILU(0), 2 block-Jacobi ILU(0) */
+    /* preconditioner: 0 identity, 1 ILU(0), 2 block-Jacobi ILU(0), 3 AMG */
     int prec_kind;
     int nblocks;        /* for block-Jacobi, contiguous equal row ranges */
+    or_amg_params amg;  /* for AMG */
 } or_opts;
 
 typedef struct {

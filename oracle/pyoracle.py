@@ -24,10 +24,30 @@ _f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _f32p = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 
 
+class AmgParams(C.Structure):
+    _fields_ = [("omega_pre_f", C.c_double), ("omega_pre_r", C.c_double),
+                ("omega_post_f", C.c_double), ("omega_post_r", C.c_double),
+                ("nu1", C.c_int), ("nu2", C.c_int), ("gamma", C.c_int), ("min", C.c_int),
+                ("omega", C.c_double), ("reverse", C.c_int), ("mode", C.c_int),
+                ("block_rows", C.c_int), ("max_coarse", C.c_int), ("zero_guess", C.c_int),
+                ("ordering", C.c_int)]
+
+
+def amg_params(**kw):
+    """AMG() defaults (AMG.java:204-206) overridden by keyword."""
+    p = AmgParams()
+    lib().or_amg_params_default(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise TypeError(f"unknown AMG parameter {k}")
+        setattr(p, k, v)
+    return p
+
+
 class Opts(C.Structure):
     _fields_ = [("max_iter", C.c_int), ("rtol", C.c_double), ("atol", C.c_double),
                 ("dtol", C.c_double), ("restart", C.c_int), ("prec_kind", C.c_int),
-                ("nblocks", C.c_int)]
+                ("nblocks", C.c_int), ("amg", AmgParams)]
 
 
 class Stats(C.Structure):
@@ -76,6 +96,24 @@ def lib():
             f.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.POINTER(Opts),
                           C.POINTER(Stats), _f64p, C.c_int]
         L.or_dgesv.argtypes = [C.c_int, C.c_int, _f64p, C.c_int, _i32p, _f64p, C.c_int]
+        L.or_amg_params_default.argtypes = [C.c_void_p]
+        L.or_amg_setup.argtypes = [C.c_int, _i32p, _i32p, _f64p, C.c_void_p, C.c_void_p,
+                                   C.c_void_p]
+        L.or_amg_apply.argtypes = [C.c_void_p, _f64p, _f64p, C.c_int]
+        L.or_amg_apply.restype = None
+        L.or_amg_free.argtypes = [C.c_void_p]
+        L.or_amg_free.restype = None
+        L.or_amg_levels.argtypes = [C.c_void_p, _i32p, _i32p, C.c_int]
+        L.or_amg_get_level.argtypes = [C.c_void_p, C.c_int] + [C.c_void_p] * 7
+        L.or_amg_p_nnz.argtypes = [C.c_void_p, C.c_int]
+        L.or_amg_aggregate.argtypes = [C.c_int, _i32p, _i32p, _f64p, C.c_double, C.c_int,
+                                       _i32p, C.c_void_p, C.c_void_p]
+        L.or_hashset_order.argtypes = [_i32p, C.c_int, _i32p]
+        L.or_hashset_order.restype = None
+        L.or_amg_priority.argtypes = [C.c_int]
+        L.or_amg_priority.restype = C.c_uint64
+        L.or_ssor_apply.argtypes = [C.c_int, _i32p, _i32p, _f64p, C.c_int, C.c_int,
+                                    C.c_double, C.c_double, C.c_int, _f64p, _f64p, _i32p]
         L.or_mesh_counts.argtypes = [C.c_int] * 4 + [C.POINTER(C.c_int)] * 3
         L.or_mesh_connections.argtypes = [C.c_int] * 4 + [_i32p, _i32p]
         L.or_mpfa_o.argtypes = [C.c_int] * 4 + [C.c_double] * 3 + [_f64p, C.c_double, _i32p,
@@ -241,15 +279,18 @@ def block_filter(rowptr, colind, data, nblocks):
 
 
 # ---------------------------------------------------------------- solvers
-PREC = {"none": 0, "identity": 0, "ilu": 1, "block_ilu": 2}
+PREC = {"none": 0, "identity": 0, "ilu": 1, "block_ilu": 2, "amg": 3}
 
 
 def solve(kind, rowptr, colind, data, b, x0=None, *, max_iter=100000, rtol=1e-5, atol=1e-50,
-          dtol=1e5, restart=30, prec="ilu", nblocks=1, hist_cap=None):
-    """Run or_bicgstab / or_gmres / or_ir.  Returns (x, Stats, history)."""
+          dtol=1e5, restart=30, prec="ilu", nblocks=1, hist_cap=None, amg=None):
+    """Run or_bicgstab / or_gmres / or_ir.  Returns (x, Stats, history).
+    amg: AmgParams (or dict of overrides) when prec == "amg"."""
     n = len(rowptr) - 1
     x = np.zeros(n) if x0 is None else _f64(x0).copy()
-    o = Opts(max_iter, rtol, atol, dtol, restart, PREC[prec], nblocks)
+    if amg is None or isinstance(amg, dict):
+        amg = amg_params(**(amg or {}))
+    o = Opts(max_iter, rtol, atol, dtol, restart, PREC[prec], nblocks, amg)
     s = Stats()
     cap = (max_iter + 2) * 2 if hist_cap is None else hist_cap
     cap = max(1, min(cap, 10_000_000))
@@ -257,6 +298,89 @@ def solve(kind, rowptr, colind, data, b, x0=None, *, max_iter=100000, rtol=1e-5,
     fn = {"bicgstab": lib().or_bicgstab, "gmres": lib().or_gmres, "ir": lib().or_ir}[kind]
     fn(n, _i32(rowptr), _i32(colind), _f64(data), _f64(b), x, C.byref(o), C.byref(s), hist, cap)
     return x, s, hist[: min(s.hist_len, cap)]
+
+
+class AMG:
+    """Oracle AMG hierarchy (or_amg_*): setMatrix on construction."""
+
+    def __init__(self, rowptr, colind, data, **params):
+        self.p = amg_params(**params)
+        self.n = len(rowptr) - 1
+        h = C.c_void_p()
+        bad = C.c_int(-1)
+        st = lib().or_amg_setup(self.n, _i32(rowptr), _i32(colind), _f64(data),
+                                C.byref(self.p), C.byref(h), C.byref(bad))
+        if st:
+            raise RuntimeError(f"or_amg_setup status {st} (row {bad.value})")
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_amg_free(self.h)
+            self.h = None
+
+    def levels(self):
+        sz = np.zeros(64, np.int32)
+        nz = np.zeros(64, np.int32)
+        m = lib().or_amg_levels(self.h, sz, nz, 64)
+        return [(int(sz[k]), int(nz[k])) for k in range(m)]
+
+    def level(self, k):
+        """(rowptr, colind, data, pt, P rowptr, P colind, P data); the last
+        four are None on the coarsest level."""
+        lv = self.levels()
+        n, nnz = lv[k]
+        rp = np.zeros(n + 1, np.int32)
+        ci = np.zeros(max(nnz, 1), np.int32)
+        v = np.zeros(max(nnz, 1))
+        if k < len(lv) - 1:
+            pn = lib().or_amg_p_nnz(self.h, k)
+            pt = np.zeros(max(n, 1), np.int32)
+            prp = np.zeros(n + 1, np.int32)
+            pci = np.zeros(max(pn, 1), np.int32)
+            pv = np.zeros(max(pn, 1))
+            lib().or_amg_get_level(self.h, k, *[a.ctypes.data for a in
+                                                  (rp, ci, v, pt, prp, pci, pv)])
+            return rp, ci[:nnz], v[:nnz], pt[:n], prp, pci[:pn], pv[:pn]
+        lib().or_amg_get_level(self.h, k, rp.ctypes.data, ci.ctypes.data, v.ctypes.data,
+                               None, None, None, None)
+        return rp, ci[:nnz], v[:nnz], None, None, None, None
+
+    def apply(self, b, x, transpose=False):
+        x = _f64(x).copy()
+        lib().or_amg_apply(self.h, _f64(b), x, int(bool(transpose)))
+        return x
+
+
+def amg_aggregate(rowptr, colind, data, eps, mode=0):
+    n = len(rowptr) - 1
+    pt = np.zeros(max(n, 1), np.int32)
+    nagg = C.c_int()
+    nroots = C.c_int()
+    st = lib().or_amg_aggregate(n, _i32(rowptr), _i32(colind), _f64(data), eps, mode, pt,
+                                C.byref(nagg), C.byref(nroots))
+    if st:
+        raise RuntimeError(f"or_amg_aggregate status {st}")
+    return pt[:n], nagg.value, nroots.value
+
+
+def hashset_order(cols):
+    cols = _i32(cols)
+    out = np.zeros(max(len(cols), 1), np.int32)
+    lib().or_hashset_order(cols, len(cols), out)
+    return out[: len(cols)]
+
+
+def ssor_apply(rowptr, colind, data, b, x, *, omega_f=1.0, omega_r=1.0, reverse=True,
+               block_rows=0, ordering=0, return_order=False):
+    n = len(rowptr) - 1
+    x = _f64(x).copy()
+    order = np.zeros(max(n, 1), np.int32)
+    st = lib().or_ssor_apply(n, _i32(rowptr), _i32(colind), _f64(data), block_rows, ordering,
+                             omega_f, omega_r, int(reverse), _f64(b), x, order)
+    if st:
+        raise RuntimeError(f"or_ssor_apply status {st}")
+    return (x, order[:n]) if return_order else x
 
 
 def dgesv(a, b):

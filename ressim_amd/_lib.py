@@ -17,7 +17,7 @@ BREAKDOWN, DIVERGENCE, ITERATIONS = 1, 2, 3
 ZERO_PIVOT, MISSING_DIAGONAL, NOT_IN_PATTERN, BAD_ARGUMENT, SINGULAR = 10, 11, 12, 13, 14
 HIP_ERROR, OUT_OF_MEMORY, NOT_BUILT = -1, -2, -3
 
-PREC_IDENTITY, PREC_ILU0, PREC_BLOCK_ILU0, PREC_DIAGONAL = 0, 1, 2, 3
+PREC_IDENTITY, PREC_ILU0, PREC_BLOCK_ILU0, PREC_DIAGONAL, PREC_AMG = 0, 1, 2, 3, 4
 FP64, FP32 = 0, 1
 BICGSTAB, GMRES, IR = 0, 1, 2
 
@@ -27,6 +27,14 @@ class SolverOpts(C.Structure):
                 ("rtol", C.c_double), ("atol", C.c_double), ("dtol", C.c_double),
                 ("mixed", C.c_int), ("inner_max_iter", C.c_int),
                 ("inner_rtol", C.c_double), ("check_every", C.c_int)]
+
+
+class AmgParams(C.Structure):
+    _fields_ = [("omega_pre_f", C.c_double), ("omega_pre_r", C.c_double),
+                ("omega_post_f", C.c_double), ("omega_post_r", C.c_double),
+                ("nu1", C.c_int), ("nu2", C.c_int), ("gamma", C.c_int), ("min", C.c_int),
+                ("omega", C.c_double), ("reverse", C.c_int), ("block_rows", C.c_int),
+                ("zero_guess", C.c_int), ("max_coarse", C.c_int), ("ordering", C.c_int)]
 
 
 class Stats(C.Structure):
@@ -79,6 +87,11 @@ SIGNATURES = {
     "rsgpu_prec_get_factors": (C.c_int, [_h, _ip, _ip, _dp, C.POINTER(C.c_int)]),
     "rsgpu_prec_info": (C.c_int, [_h, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                   C.POINTER(C.c_int)]),
+    "rsgpu_amg_params_default": (None, [C.POINTER(AmgParams)]),
+    "rsgpu_prec_set_amg_params": (C.c_int, [_h, C.POINTER(AmgParams)]),
+    "rsgpu_amg_levels": (C.c_int, [_h, _ip, _ip, C.c_int]),
+    "rsgpu_amg_get_level": (C.c_int, [_h, C.c_int, _ip, _ip, _dp, _ip, _ip, _ip, _dp,
+                                      C.POINTER(C.c_int)]),
     "rsgpu_solver_opts_default": (None, [C.POINTER(SolverOpts)]),
     "rsgpu_solver_create": (C.c_int, [_h, C.POINTER(SolverOpts), C.c_int, _hp]),
     "rsgpu_solver_destroy": (C.c_int, [_h]),

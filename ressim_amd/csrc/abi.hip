@@ -356,13 +356,17 @@ extern "C" int rsgpu_csr_mult_f32(rsgpu_csr *A, const float *x, float *y)
     RS_CATCH(A->ctx)
 }
 
+// inout: the output vector's content is an input too (Preconditioner.apply
+// of SSOR/AMG starts from x's old content)
 template <class F>
 static int host_roundtrip(rsgpu_ctx *ctx, int nin, const double *in_host,
-                          int nout, double *out_host, F f)
+                          int nout, double *out_host, F f, bool inout = false)
 {
     hipStream_t s = ctx->stream;
     DevBuf<double> a, b;
     a.upload(in_host, nin, s);
+    if (inout)
+        b.upload(out_host, nout, s);
     b.alloc(std::max(nout, 1));
     f(a.p, b.p);
     if (nout)
@@ -401,12 +405,16 @@ extern "C" int rsgpu_prec_create(rsgpu_ctx *ctx, int kind, int nblocks,
     if (!ctx || !out)
         return RSGPU_BAD_ARGUMENT;
     if (kind != RSGPU_PREC_IDENTITY && kind != RSGPU_PREC_ILU0 &&
-        kind != RSGPU_PREC_BLOCK_ILU0) {
+        kind != RSGPU_PREC_BLOCK_ILU0 && kind != RSGPU_PREC_AMG) {
         ctx->err = "preconditioner kind not available";
         return RSGPU_BAD_ARGUMENT;
     }
     if (precision != RSGPU_FP64 && precision != RSGPU_FP32)
         return RSGPU_BAD_ARGUMENT;
+    if (kind == RSGPU_PREC_AMG && precision != RSGPU_FP64) {
+        ctx->err = "AMG runs in FP64";
+        return RSGPU_BAD_ARGUMENT;
+    }
     if (kind == RSGPU_PREC_BLOCK_ILU0 && nblocks < 1)
         return RSGPU_BAD_ARGUMENT;
     rsgpu_prec *M = new rsgpu_prec();
@@ -415,8 +423,48 @@ extern "C" int rsgpu_prec_create(rsgpu_ctx *ctx, int kind, int nblocks,
     M->nblocks = kind == RSGPU_PREC_BLOCK_ILU0 ? nblocks : 1;
     M->precision = precision;
     M->ready = (kind == RSGPU_PREC_IDENTITY);
+    rsgpu_amg_params_default(&M->amg_params);
     *out = M;
     return RSGPU_OK;
+}
+
+extern "C" int rsgpu_prec_set_amg_params(rsgpu_prec *M, const rsgpu_amg_params *p)
+{
+    if (!M || !p || M->kind != RSGPU_PREC_AMG)
+        return RSGPU_BAD_ARGUMENT;
+    // SSOR.setOmega range (SSOR.java:122-130)
+    if (p->omega_pre_f < 0 || p->omega_pre_f > 2 || p->omega_pre_r < 0 || p->omega_pre_r > 2 ||
+        p->omega_post_f < 0 || p->omega_post_f > 2 || p->omega_post_r < 0 ||
+        p->omega_post_r > 2 || p->nu1 < 0 || p->nu2 < 0 || p->gamma < 1 || p->min < 0 ||
+        p->block_rows < 0 || p->max_coarse < 1 || p->ordering < 0 || p->ordering > 1) {
+        M->ctx->err = "AMG parameter out of range";
+        return RSGPU_BAD_ARGUMENT;
+    }
+    M->amg_params = *p;
+    return RSGPU_OK;
+}
+
+extern "C" int rsgpu_amg_levels(const rsgpu_prec *M, int *sizes, int *nnzs, int cap)
+{
+    if (!M || M->kind != RSGPU_PREC_AMG || !M->ready)
+        return 0;
+    try {
+        return amg_levels(M, sizes, nnzs, cap);
+    } catch (...) {
+        return 0;
+    }
+}
+
+extern "C" int rsgpu_amg_get_level(const rsgpu_prec *M, int k, int32_t *rowptr,
+                                   int32_t *colind, double *data, int32_t *pt, int32_t *Prp,
+                                   int32_t *Pci, double *Pv, int *pnnz)
+{
+    if (!M || M->kind != RSGPU_PREC_AMG || !M->ready)
+        return RSGPU_BAD_ARGUMENT;
+    RS_TRY(M->ctx)
+    RS_HIP(hipStreamSynchronize(M->ctx->stream));
+    return amg_get_level(M, k, rowptr, colind, data, pt, Prp, Pci, Pv, pnnz);
+    RS_CATCH(M->ctx)
 }
 
 extern "C" int rsgpu_prec_destroy(rsgpu_prec *M)
@@ -442,6 +490,15 @@ extern "C" int rsgpu_prec_set_matrix(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
         M->pattern_id = A->pattern_id;
         M->ready = true;
         return RSGPU_OK;
+    }
+    if (M->kind == RSGPU_PREC_AMG) {
+        M->ready = false;
+        M->n = A->n;
+        M->pattern_id = A->pattern_id;
+        const int st = amg_setup(M, A, bad_row);
+        M->ready = st == RSGPU_OK;
+        M->nnz = A->nnz;
+        return st;
     }
     if (!M->ready || M->pattern_id != A->pattern_id) {
         try {
@@ -500,9 +557,9 @@ extern "C" int rsgpu_prec_apply_host(rsgpu_prec *M, const double *b, double *x)
     if (!M || !M->ready || !b || !x)
         return RSGPU_BAD_ARGUMENT;
     RS_TRY(M->ctx)
-    return host_roundtrip(M->ctx, M->n, b, M->n, x, [&](double *a, double *o) {
-        prec_apply(M, a, o, nullptr);
-    });
+    return host_roundtrip(
+        M->ctx, M->n, b, M->n, x, [&](double *a, double *o) { prec_apply(M, a, o, nullptr); },
+        true);
     RS_CATCH(M->ctx)
 }
 
@@ -512,9 +569,9 @@ extern "C" int rsgpu_prec_trans_apply_host(rsgpu_prec *M, const double *b,
     if (!M || !M->ready || !b || !x)
         return RSGPU_BAD_ARGUMENT;
     RS_TRY(M->ctx)
-    return host_roundtrip(M->ctx, M->n, b, M->n, x, [&](double *a, double *o) {
-        prec_trans_apply(M, a, o);
-    });
+    return host_roundtrip(
+        M->ctx, M->n, b, M->n, x, [&](double *a, double *o) { prec_trans_apply(M, a, o); },
+        true);
     RS_CATCH(M->ctx)
 }
 
@@ -552,6 +609,14 @@ extern "C" int rsgpu_prec_info(const rsgpu_prec *M, int *nnz, int *nl, int *nu)
         return RSGPU_BAD_ARGUMENT;
     if (nnz)
         *nnz = M->nnz;
+    if (M->kind == RSGPU_PREC_AMG) {
+        const int m = rsgpu_amg_levels(M, nullptr, nullptr, 0);
+        if (nl)
+            *nl = m;
+        if (nu)
+            *nu = m;
+        return RSGPU_OK;
+    }
     if (nl)
         *nl = M->L.max_levels;
     if (nu)

@@ -1,0 +1,1598 @@
+// Smoothed-aggregation AMG preconditioner on gfx950.
+//
+// Reference: mtj/src/no/uib/cipr/matrix/sparse/AMG.java (setMatrix :228-294,
+// cycle :302-342, Aggregator :376-642, Interpolator :648-928), SSOR.java
+// (apply :143-194), CompColMatrix.multAdd/transMult (:277-318), DenseLU
+// (dgetrf/dgetrs with one right-hand side).
+//
+// The whole setup runs on the device, level by level:
+//   strength N(i) = {j : |a_ij| >= eps sqrt(a_ii a_jj)}, eps = 0.08 * 0.5^k;
+//   aggregation in three phases.  Phase 1 (disjoint strongly coupled
+//   neighbourhoods) and phase 3 (remnants) are greedy loops in row order in
+//   the reference; here they are prioritised rounds: a candidate wins when
+//   its hashed priority is the largest on every node of its neighbourhood,
+//   so winners are disjoint and the result is maximal, as the greedy loop's.
+//   Phase 2 (each free node joins the aggregate it overlaps most, the first
+//   to reach the largest overlap in HashSet iteration order) is independent
+//   per node and follows the reference exactly;
+//   the smoothed prolongator P = (I - omega D^-1 A_F) P_tent row by row with
+//   the reference's per-entry operation order;
+//   P' by a stable radix sort (rows stay ascending in every column);
+//   the Galerkin operator as (P' (A P)) with one wavefront per output row
+//   and an LDS hash accumulator.  The wavefront walks the left factor's row
+//   in order and spreads the matching right row over its lanes, so every
+//   entry receives its terms in the order of createGalerkinSlow
+//   (AMG.java:883-919) and only nonzero results are stored;
+//   the coarsest level is factored by a dense LU (dgetf2 order) in one
+//   workgroup.
+//
+// The smoother is SSOR (forward omega_F, reverse omega_R).  Rows are visited
+// in natural order (SSOR.java) or by greedy colour (multicolour SSOR, the
+// default: rows of one colour are independent); either way the rows form
+// levels whose members do not couple, and a coupling reads this sweep's
+// value when its row was visited earlier, the previous value otherwise.
+// Optionally the rows are split into contiguous blocks (block-Jacobi:
+// couplings that leave a block read the vector from the start of the
+// application).  A block that fits LDS is swept by one workgroup with its
+// vector in LDS; a single block larger than that is swept in place in
+// global memory with one launch per level (colour).
+//
+// AMG.apply starts the cycle from x's old content (AMG.java:208-216); the
+// zero_guess option starts from zero instead, which makes the
+// preconditioner a fixed linear operator.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <memory>
+
+#include "spmv_core.hpp"
+
+#define RS_AMG_LDS_ROWS 8190
+
+struct Skip {
+    const int *a, *b;
+    __device__ __forceinline__ bool on() const { return (a && *a) || (b && *b); }
+};
+
+struct AmgLevelMeta {
+    int count, off, roff, pad;
+};
+
+// one level of the hierarchy
+struct AmgLevel {
+    int n = 0, nnz = 0, c = 0;
+    // matrix (level 0 borrows the pattern of the caller's matrix)
+    DevBuf<int> rp_own, ci_own;
+    const int *rp = nullptr, *ci = nullptr;
+    DevBuf<double> v;
+    DevBuf<int> diag;
+    std::vector<int> h_rp, h_ci;
+    SpmvPlan plan;
+    // aggregation / transfer
+    DevBuf<int> pt;
+    int pnnz = 0;
+    DevBuf<int> Prp, Pci, Trp, Tci;
+    DevBuf<double> Pv, Tv;
+    SpmvPlan Pplan, Tplan;
+    // smoother
+    int nb = 1, W = 4, max_block_rows = 0, nlev_max = 0;
+    DevBuf<int> bstart, blk_lev, meta, rows, row_glob, ell_col, ell_src;
+    DevBuf<double> ell_val, dgl;
+    size_t ell_size = 0;
+    int nrows_lev = 0;
+    uint64_t sched_pattern = 0;
+    int nb_rows_param = -1, ordering_param = -1;
+    bool global_sweep = false;          // one block, swept level by level
+    std::vector<AmgLevelMeta> h_meta;   // host copy (launch sizes)
+    // vectors
+    DevBuf<double> u, ualt, f, r;
+};
+
+struct AmgData {
+    rsgpu_amg_params p;
+    std::vector<std::unique_ptr<AmgLevel>> L;   // m levels; the last is dense
+    int m = 0;
+    int nc = 0;
+    DevBuf<double> lu;
+    DevBuf<int> ipiv, flag;
+    DevBuf<char> tmp_bytes;    // hipcub temporary storage
+    uint64_t pattern0 = 0;
+};
+
+void amg_destroy(AmgData *d) { delete d; }
+
+extern "C" void rsgpu_amg_params_default(rsgpu_amg_params *p)
+{
+    if (!p)
+        return;
+    // AMG() (AMG.java:204-206)
+    p->omega_pre_f = 1;
+    p->omega_pre_r = 1.85;
+    p->omega_post_f = 1.85;
+    p->omega_post_r = 1;
+    p->nu1 = 1;
+    p->nu2 = 1;
+    p->gamma = 1;
+    p->min = 40;
+    p->omega = 2. / 3;
+    p->reverse = 1;
+    p->block_rows = 0;
+    p->zero_guess = 0;
+    p->max_coarse = 2048;
+    p->ordering = 1;
+}
+
+// ---------------------------------------------------------------- helpers
+template <typename T>
+static void swap_buf(DevBuf<T> &a, DevBuf<T> &b)
+{
+    std::swap(a.p, b.p);
+    std::swap(a.n, b.n);
+}
+
+static void scan_exclusive(AmgData *D, hipStream_t s, const int *in, int *out, int n)
+{
+    size_t bytes = 0;
+    RS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, s));
+    if (D->tmp_bytes.n < bytes)
+        D->tmp_bytes.alloc(bytes);
+    RS_HIP(hipcub::DeviceScan::ExclusiveSum(D->tmp_bytes.p, bytes, in, out, n, s));
+}
+
+static int read_int(hipStream_t s, const int *p)
+{
+    int v = 0;
+    RS_HIP(hipMemcpyAsync(&v, p, sizeof(int), hipMemcpyDeviceToHost, s));
+    RS_HIP(hipStreamSynchronize(s));
+    return v;
+}
+
+static void set_int(hipStream_t s, int *p, int v)
+{
+    RS_HIP(hipMemcpyAsync(p, &v, sizeof(int), hipMemcpyHostToDevice, s));
+}
+
+static inline int gridn(long long n, int bs = 256)
+{
+    return (int)std::max<long long>(1, (n + bs - 1) / bs);
+}
+
+__device__ __forceinline__ unsigned long long amg_prio(int i)
+{
+    // MurmurHash3 32-bit finaliser of the index, index in the low word
+    unsigned h = (unsigned)i;
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return ((unsigned long long)h << 32) | (unsigned)i;
+}
+
+// Java 8 HashSet<Integer> iteration key: (bucket, value)
+__device__ __forceinline__ unsigned long long hs_key(int v, int cap)
+{
+    unsigned h = (unsigned)v;
+    return ((unsigned long long)((h ^ (h >> 16)) & (unsigned)(cap - 1)) << 32) | h;
+}
+
+__device__ __forceinline__ int hs_cap(int size)
+{
+    int cap = 16;
+    while (size > (cap * 3) / 4)
+        cap <<= 1;
+    return cap;
+}
+
+__device__ __forceinline__ int bsearch_dev(const int *a, int key, int lo, int hi)
+{
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        int v = a[mid];
+        if (v == key)
+            return mid;
+        if (v < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return -1;
+}
+
+// ---------------------------------------------------------------- strength
+__global__ void k_amg_diag(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                           int *__restrict__ diag, int *bad)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    int d = bsearch_dev(ci, i, rp[i], rp[i + 1]);
+    diag[i] = d;
+    if (d < 0)
+        atomicMin(bad, i + 1);
+}
+
+// N(i) count (or fill) and the initial R set (AMG.java:482-532)
+__global__ void k_amg_strength(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                               const double *__restrict__ v, const int *__restrict__ diag,
+                               double eps, int *__restrict__ cnt, const int *__restrict__ Np,
+                               int *__restrict__ Ni, char *__restrict__ R)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const double aii = v[diag[i]];
+    int q = Np ? Np[i] : 0, c = 0;
+    char h = 0;
+    for (int j = rp[i]; j < rp[i + 1]; ++j) {
+        const int col = ci[j];
+        const double aij = v[j];
+        if (col != i && aij != 0)
+            h = 1;
+        const double ajj = v[diag[col]];
+        if (fabs(aij) >= eps * sqrt(aii * ajj)) {
+            if (Ni)
+                Ni[q + c] = col;
+            ++c;
+        }
+    }
+    if (cnt)
+        cnt[i] = c;
+    if (R)
+        R[i] = h;
+}
+
+// ---------------------------------------------------------------- phase 1
+__global__ void k_p1_cand(int n, const int *__restrict__ Np, const int *__restrict__ Ni,
+                          const char *__restrict__ R, char *__restrict__ cand,
+                          unsigned long long *__restrict__ m, int *any)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool ok = false;
+    if (i < n) {
+        ok = R[i] != 0;
+        for (int q = Np[i]; q < Np[i + 1] && ok; ++q)
+            ok = R[Ni[q]] != 0;
+        cand[i] = ok;
+        m[i] = 0;
+    }
+    if (__ballot(ok) && (threadIdx.x & 63) == 0)
+        atomicOr(any, 1);
+}
+
+__global__ void k_p1_max(int n, const int *__restrict__ Np, const int *__restrict__ Ni,
+                         const char *__restrict__ cand, unsigned long long *m)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !cand[i])
+        return;
+    const unsigned long long p = amg_prio(i);
+    for (int q = Np[i]; q < Np[i + 1]; ++q)
+        atomicMax(&m[Ni[q]], p);
+}
+
+__global__ void k_p1_sel(int n, const int *__restrict__ Np, const int *__restrict__ Ni,
+                         const char *__restrict__ cand, const unsigned long long *__restrict__ m,
+                         char *R, int *root_of)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !cand[i])
+        return;
+    const unsigned long long p = amg_prio(i);
+    for (int q = Np[i]; q < Np[i + 1]; ++q)
+        if (m[Ni[q]] != p)
+            return;
+    for (int q = Np[i]; q < Np[i + 1]; ++q) {
+        R[Ni[q]] = 0;
+        root_of[Ni[q]] = i;
+    }
+}
+
+__global__ void k_root_flag(int n, const int *__restrict__ root_of, int *__restrict__ flag)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        flag[i] = root_of[i] == i;
+}
+
+__global__ void k_owner(int n, const int *__restrict__ root_of, const int *__restrict__ id,
+                        int *__restrict__ owner, int *__restrict__ pt)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n)
+        return;
+    int r = root_of[j];
+    int o = r >= 0 ? id[r] : -1;
+    owner[j] = o;
+    pt[j] = o;
+}
+
+// ---------------------------------------------------------------- phase 2
+// AMG.java:584-611 for one free node: the aggregate whose overlap count
+// first reaches the maximum while N(i) is iterated in HashSet order.
+__global__ void k_enlarge(int n, const int *__restrict__ Np, const int *__restrict__ Ni,
+                          const int *__restrict__ owner, char *R, int *pt)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !R[i])
+        return;
+    const int *N = Ni + Np[i];
+    const int len = Np[i + 1] - Np[i];
+    const int cap = hs_cap(len);
+    int maxv = 0;
+    for (int a = 0; a < len; ++a) {
+        const int k = owner[N[a]];
+        if (k < 0)
+            continue;
+        int c = 0;
+        for (int b = 0; b < len; ++b)
+            c += owner[N[b]] == k;
+        maxv = c > maxv ? c : maxv;
+    }
+    if (maxv == 0)
+        return;
+    int best = 0x7fffffff, win = -1;
+    for (int a = 0; a < len; ++a) {
+        const int k = owner[N[a]];
+        if (k < 0)
+            continue;
+        const unsigned long long ka = hs_key(N[a], cap);
+        int c = 0, occ = 0, rank = 0;
+        for (int b = 0; b < len; ++b) {
+            const bool lt = hs_key(N[b], cap) < ka;
+            const bool same = owner[N[b]] == k;
+            c += same;
+            occ += same && lt;
+            rank += lt;
+        }
+        if (c == maxv && occ == maxv - 1 && rank < best) {
+            best = rank;
+            win = k;
+        }
+    }
+    pt[i] = win;
+    R[i] = 0;
+}
+
+// ---------------------------------------------------------------- phase 3
+__global__ void k_p3_cand(int n, const int *__restrict__ Np, const int *__restrict__ Ni,
+                          const char *__restrict__ R, char *__restrict__ cand,
+                          unsigned long long *__restrict__ m, int *any)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool ok = false;
+    if (i < n) {
+        if (R[i])
+            for (int q = Np[i]; q < Np[i + 1] && !ok; ++q)
+                ok = R[Ni[q]] != 0;
+        cand[i] = ok;
+        m[i] = 0;
+    }
+    if (__ballot(ok) && (threadIdx.x & 63) == 0)
+        atomicOr(any, 1);
+}
+
+__global__ void k_p3_max(int n, const int *__restrict__ Np, const int *__restrict__ Ni,
+                         const char *__restrict__ R, const char *__restrict__ cand,
+                         unsigned long long *m)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !cand[i])
+        return;
+    const unsigned long long p = amg_prio(i);
+    for (int q = Np[i]; q < Np[i + 1]; ++q)
+        if (R[Ni[q]])
+            atomicMax(&m[Ni[q]], p);
+}
+
+__global__ void k_p3_sel(int n, const int *__restrict__ Np, const int *__restrict__ Ni,
+                         const char *__restrict__ R, const char *__restrict__ cand,
+                         const unsigned long long *__restrict__ m, int *__restrict__ sel)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    int s = 0;
+    if (cand[i]) {
+        const unsigned long long p = amg_prio(i);
+        s = 1;
+        for (int q = Np[i]; q < Np[i + 1]; ++q)
+            if (R[Ni[q]] && m[Ni[q]] != p) {
+                s = 0;
+                break;
+            }
+    }
+    sel[i] = s;
+}
+
+__global__ void k_p3_assign(int n, const int *__restrict__ Np, const int *__restrict__ Ni,
+                            const char *__restrict__ R, const int *__restrict__ sel,
+                            const int *__restrict__ id, int base, int *pt)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !sel[i])
+        return;
+    const int a = base + id[i];
+    for (int q = Np[i]; q < Np[i + 1]; ++q)
+        if (R[Ni[q]])
+            pt[Ni[q]] = a;
+}
+
+__global__ void k_p3_clear(int n, const int *__restrict__ pt, char *R)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n && R[j] && pt[j] >= 0)
+        R[j] = 0;
+}
+
+// ---------------------------------------------------------------- prolongator
+// Row i of P = (I - omega D^-1 A_F) P_tent (AMG.java:830-873): aggregates in
+// ascending order; each entry accumulates its couplings in column order.
+__device__ __forceinline__ bool amg_weak(double a, int col, const int *Ni, int n0, int n1)
+{
+    return a != 0 && bsearch_dev(Ni, col, n0, n1) < 0;
+}
+
+__global__ void k_prolong(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                          const double *__restrict__ v, const int *__restrict__ diag,
+                          const int *__restrict__ Np, const int *__restrict__ Ni,
+                          const int *__restrict__ pt, double omega, int *__restrict__ cnt,
+                          const int *__restrict__ Prp, int *__restrict__ Pci,
+                          double *__restrict__ Pv)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const int own = pt[i];
+    if (own == -1) {
+        if (cnt)
+            cnt[i] = 0;
+        return;
+    }
+    const int r0 = rp[i], r1 = rp[i + 1], n0 = Np[i], n1 = Np[i + 1];
+    double weak = 0;
+    for (int j = r0; j < r1; ++j) {
+        const int col = ci[j];
+        if (pt[col] == -1)
+            continue;
+        if (amg_weak(v[j], col, Ni, n0, n1))
+            weak += v[j];
+    }
+    const double scale = -omega / v[diag[i]];
+    int prev = -1, q = 0;
+    const int base = Prp ? Prp[i] : 0;
+    for (;;) {
+        int next = 0x7fffffff;
+        for (int j = r0; j < r1; ++j) {
+            const int k = pt[ci[j]];
+            if (k <= prev || k >= next)
+                continue;
+            if (amg_weak(v[j], ci[j], Ni, n0, n1))
+                continue;
+            next = k;
+        }
+        if (own > prev && own < next)
+            next = own;
+        if (next == 0x7fffffff)
+            break;
+        double acc = 0;
+        for (int j = r0; j < r1; ++j) {
+            if (pt[ci[j]] != next)
+                continue;
+            if (amg_weak(v[j], ci[j], Ni, n0, n1))
+                continue;
+            acc += v[j];
+        }
+        if (next == own)
+            acc -= weak;
+        acc *= scale;
+        if (next == own)
+            acc += 1;
+        if (acc != 0) {
+            if (Pci) {
+                Pci[base + q] = next;
+                Pv[base + q] = acc;
+            }
+            ++q;
+        }
+        prev = next;
+    }
+    if (cnt)
+        cnt[i] = q;
+}
+
+__global__ void k_entry_rows(int n, const int *__restrict__ rp, int *__restrict__ erow)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    for (int e = rp[i]; e < rp[i + 1]; ++e)
+        erow[e] = i;
+}
+
+__global__ void k_iota(int n, int *a)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        a[i] = i;
+}
+
+__global__ void k_col_count(int nnz, const int *__restrict__ ci, int *cnt)
+{
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nnz)
+        atomicAdd(&cnt[ci[e]], 1);
+}
+
+__global__ void k_transpose_fill(int nnz, const int *__restrict__ perm,
+                                 const int *__restrict__ erow, const double *__restrict__ v,
+                                 int *__restrict__ tci, double *__restrict__ tv)
+{
+    int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < nnz) {
+        const int e = perm[p];
+        tci[p] = erow[e];
+        tv[p] = v[e];
+    }
+}
+
+// ---------------------------------------------------------------- SpGEMM
+// Upper bound of a product row's length: sum of the right rows it touches.
+__global__ void k_spgemm_ub(int nrows, const int *__restrict__ Xrp, const int *__restrict__ Xci,
+                            const int *__restrict__ Yrp, int *ubmax)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int u = 0;
+    if (i < nrows)
+        for (int e = Xrp[i]; e < Xrp[i + 1]; ++e) {
+            const int j = Xci[e];
+            u += Yrp[j + 1] - Yrp[j];
+        }
+    for (int o = 32; o > 0; o >>= 1)
+        u = max(u, __shfl_xor(u, o, 64));
+    if ((threadIdx.x & 63) == 0)
+        atomicMax(ubmax, u);
+}
+
+__device__ __forceinline__ unsigned hslot(int k, int tab)
+{
+    return ((unsigned)k * 2654435761u) & (unsigned)(tab - 1);
+}
+
+// C = X Y, one wavefront per row of X.  The wave walks X's row entries in
+// order; lane t handles the t-th entry of the matching Y row (distinct
+// columns), so each output entry receives its products in X-row order.
+// count pass: cnt[row] = nonzero entries; fill pass: sorted entries at Crp.
+template <int TAB>
+__global__ void __launch_bounds__(64)
+    k_spgemm(int nrows, const int *__restrict__ Xrp, const int *__restrict__ Xci,
+             const double *__restrict__ Xv, const int *__restrict__ Yrp,
+             const int *__restrict__ Yci, const double *__restrict__ Yv, int *__restrict__ cnt,
+             const int *__restrict__ Crp, int *__restrict__ Cci, double *__restrict__ Cv,
+             int *err)
+{
+    __shared__ int keys[TAB];
+    __shared__ double vals[TAB];
+    __shared__ int ck[TAB];
+    __shared__ double cv[TAB];
+    const int row = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (row >= nrows)
+        return;
+    for (int s = lane; s < TAB; s += 64) {
+        keys[s] = -1;
+        vals[s] = 0;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __syncthreads();
+    bool overflow = false;
+    for (int e = Xrp[row]; e < Xrp[row + 1]; ++e) {
+        const int j = Xci[e];
+        const double xv = Xv[e];
+        for (int t = Yrp[j] + lane; t < Yrp[j + 1]; t += 64) {
+            const int k = Yci[t];
+            unsigned s = hslot(k, TAB);
+            int probes = 0;
+            for (;;) {
+                const int old = atomicCAS(&keys[s], -1, k);
+                if (old == -1 || old == k)
+                    break;
+                s = (s + 1) & (TAB - 1);
+                if (++probes >= TAB) {
+                    overflow = true;
+                    break;
+                }
+            }
+            if (!overflow)
+                vals[s] += xv * Yv[t];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        __syncthreads();
+    }
+    if (__ballot(overflow)) {
+        if (lane == 0)
+            atomicOr(err, 1);
+        return;
+    }
+    // compact nonzero entries
+    int mine = 0;
+    for (int s = lane; s < TAB; s += 64)
+        mine += keys[s] != -1 && vals[s] != 0;
+    int incl = mine;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o, 64);
+        if (lane >= o)
+            incl += y;
+    }
+    const int total = __shfl(incl, 63, 64);
+    if (cnt) {
+        if (lane == 0)
+            cnt[row] = total;
+        return;
+    }
+    int w = incl - mine;
+    for (int s = lane; s < TAB; s += 64)
+        if (keys[s] != -1 && vals[s] != 0) {
+            ck[w] = keys[s];
+            cv[w] = vals[s];
+            ++w;
+        }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __syncthreads();
+    const int base = Crp[row];
+    for (int a = lane; a < total; a += 64) {
+        const int k = ck[a];
+        int rank = 0;
+        for (int b = 0; b < total; ++b)
+            rank += ck[b] < k;
+        Cci[base + rank] = k;
+        Cv[base + rank] = cv[a];
+    }
+}
+
+// ---------------------------------------------------------------- dense coarsest
+__global__ void k_dense_fill(int nc, const int *__restrict__ rp, const int *__restrict__ ci,
+                             const double *__restrict__ v, double *__restrict__ a)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nc)
+        return;
+    for (int c = 0; c < nc; ++c)
+        a[i + (size_t)c * nc] = 0;
+    for (int j = rp[i]; j < rp[i + 1]; ++j)
+        a[i + (size_t)ci[j] * nc] = v[j];
+}
+
+// LAPACK dgetf2 (right-looking, partial pivoting) in one workgroup
+__global__ void __launch_bounds__(256) k_dense_getf2(int n, double *a, int *ipiv, int *info)
+{
+    __shared__ int sjp;
+    __shared__ int sinfo;
+    if (threadIdx.x == 0)
+        sinfo = 0;
+    for (int j = 0; j < n; ++j) {
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int jp = j;
+            double amax = fabs(a[j + (size_t)j * n]);
+            for (int i = j + 1; i < n; ++i)
+                if (fabs(a[i + (size_t)j * n]) > amax) {
+                    amax = fabs(a[i + (size_t)j * n]);
+                    jp = i;
+                }
+            ipiv[j] = jp;
+            sjp = jp;
+            if (a[jp + (size_t)j * n] == 0 && sinfo == 0)
+                sinfo = j + 1;
+        }
+        __syncthreads();
+        const int jp = sjp;
+        const bool nz = a[jp + (size_t)j * n] != 0;
+        __syncthreads();
+        if (nz) {
+            if (jp != j)
+                for (int c = threadIdx.x; c < n; c += blockDim.x) {
+                    double t = a[j + (size_t)c * n];
+                    a[j + (size_t)c * n] = a[jp + (size_t)c * n];
+                    a[jp + (size_t)c * n] = t;
+                }
+            __syncthreads();
+            if (j < n - 1) {
+                const double r = 1.0 / a[j + (size_t)j * n];
+                for (int i = j + 1 + threadIdx.x; i < n; i += blockDim.x)
+                    a[i + (size_t)j * n] *= r;
+            }
+        }
+        __syncthreads();
+        if (j < n - 1) {
+            const int w = n - 1 - j;
+            for (int q = threadIdx.x; q < w * w; q += blockDim.x) {
+                const int i = j + 1 + q % w, c = j + 1 + q / w;
+                const double yj = a[j + (size_t)c * n];
+                if (yj != 0) {
+                    const double temp = -1.0 * yj;
+                    a[i + (size_t)c * n] = a[i + (size_t)c * n] + a[i + (size_t)j * n] * temp;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        *info = sinfo;
+}
+
+// u = f, then dgetrs ('N' or 'T') with one right-hand side
+__global__ void __launch_bounds__(256)
+    k_dense_getrs(int n, const double *__restrict__ a, const int *__restrict__ ipiv,
+                  const double *__restrict__ f, double *b, int trans, Skip sk)
+{
+    if (sk.on())
+        return;
+    __shared__ int flag;
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        b[i] = f[i];
+    __syncthreads();
+    if (!trans) {
+        if (threadIdx.x == 0)
+            for (int j = 0; j < n; ++j)
+                if (ipiv[j] != j) {
+                    double t = b[j];
+                    b[j] = b[ipiv[j]];
+                    b[ipiv[j]] = t;
+                }
+        __syncthreads();
+        for (int k = 0; k < n; ++k) {
+            const double bk = b[k];
+            __syncthreads();
+            if (bk != 0)
+                for (int i = k + 1 + threadIdx.x; i < n; i += blockDim.x)
+                    b[i] = b[i] - bk * a[i + (size_t)k * n];
+            __syncthreads();
+        }
+        for (int k = n - 1; k >= 0; --k) {
+            if (threadIdx.x == 0) {
+                flag = b[k] != 0;
+                if (flag)
+                    b[k] = b[k] / a[k + (size_t)k * n];
+            }
+            __syncthreads();
+            if (flag) {
+                const double bk = b[k];
+                for (int i = threadIdx.x; i < k; i += blockDim.x)
+                    b[i] = b[i] - bk * a[i + (size_t)k * n];
+            }
+            __syncthreads();
+        }
+    } else if (threadIdx.x == 0) {
+        for (int i = 0; i < n; ++i) {
+            double t = b[i];
+            for (int k = 0; k < i; ++k)
+                t -= a[k + (size_t)i * n] * b[k];
+            b[i] = t / a[i + (size_t)i * n];
+        }
+        for (int i = n - 1; i >= 0; --i) {
+            double t = b[i];
+            for (int k = i + 1; k < n; ++k)
+                t -= a[k + (size_t)i * n] * b[k];
+            b[i] = t;
+        }
+        for (int j = n - 1; j >= 0; --j)
+            if (ipiv[j] != j) {
+                double t = b[j];
+                b[j] = b[ipiv[j]];
+                b[ipiv[j]] = t;
+            }
+    }
+}
+
+// dgetrs 'N' for n <= 64 in one wavefront: lane i holds b[i]; the row swaps,
+// the unit-lower and the upper column sweeps run in dgetrs's order with the
+// pivot values broadcast across lanes.
+__global__ void __launch_bounds__(64)
+    k_dense_getrs64(int n, const double *__restrict__ a, const int *__restrict__ ipiv,
+                    const double *__restrict__ f, double *b, Skip sk)
+{
+    if (sk.on())
+        return;
+    const int lane = threadIdx.x;
+    double v = lane < n ? f[lane] : 0.0;
+    for (int j = 0; j < n; ++j) {
+        const int p = ipiv[j];
+        if (p != j) {
+            const double vj = __shfl(v, j, 64), vp = __shfl(v, p, 64);
+            if (lane == j)
+                v = vp;
+            if (lane == p)
+                v = vj;
+        }
+    }
+    for (int k = 0; k < n; ++k) {
+        const double bk = __shfl(v, k, 64);
+        if (bk != 0 && lane > k && lane < n)
+            v = v - bk * a[lane + (size_t)k * n];
+    }
+    for (int k = n - 1; k >= 0; --k) {
+        double bk = __shfl(v, k, 64);
+        if (bk != 0) {
+            bk = bk / a[k + (size_t)k * n];
+            if (lane == k)
+                v = bk;
+            if (lane < k)
+                v = v - bk * a[lane + (size_t)k * n];
+        }
+    }
+    if (lane < n)
+        b[lane] = v;
+}
+
+// ---------------------------------------------------------------- smoother
+__global__ void k_amg_gather(const int *__restrict__ src, const double *__restrict__ v,
+                             double *__restrict__ ell, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const int s = src[i];
+        ell[i] = s >= 0 ? v[s] : -0.0;
+    }
+}
+
+__global__ void k_amg_rowdiag(const int *__restrict__ row_glob, const int *__restrict__ diag,
+                              const double *__restrict__ v, double *__restrict__ dgl, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        dgl[i] = v[diag[row_glob[i]]];
+}
+
+// One SSOR application (SSOR.java:143-194) per block.  ell_col >= 0 is a
+// local row of the block (LDS), < 0 encodes the global column -1-c, read
+// from xold (the vector at the start of the application).
+__device__ __forceinline__ void amg_ssor_sweep(int lb, int le, int step, int W,
+                                               const AmgLevelMeta *__restrict__ meta,
+                                               const int *__restrict__ rows,
+                                               const int *__restrict__ ecol,
+                                               const double *__restrict__ eval,
+                                               const double *__restrict__ dgl,
+                                               const double *__restrict__ f, int b0,
+                                               const double *__restrict__ xold, double omega,
+                                               double *xv)
+{
+    for (int l = lb; l != le; l += step) {
+        const AmgLevelMeta m = meta[l];
+        for (int t = threadIdx.x; t < m.count; t += blockDim.x) {
+            const int r = rows[m.roff + t];
+            const size_t q = (size_t)m.off + (size_t)t * W;
+            double sigma = 0;
+            for (int s = 0; s < W; ++s) {
+                const int c = ecol[q + s];
+                const double xc = c >= 0 ? xv[c] : xold[-1 - c];
+                sigma += eval[q + s] * xc;
+            }
+            sigma = (f[b0 + r] - sigma) / dgl[m.roff + t];
+            const double xr = xv[r];
+            xv[r] = xr + omega * (sigma - xr);
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(256)
+    k_amg_ssor(const int *__restrict__ bstart, const int *__restrict__ blk_lev,
+               const AmgLevelMeta *__restrict__ meta, const int *__restrict__ rows,
+               const int *__restrict__ ecol, const double *__restrict__ eval,
+               const double *__restrict__ dgl, int W, const double *__restrict__ f,
+               const double *__restrict__ xold, double *__restrict__ xnew, double omega_f,
+               double omega_r, int reverse, Skip sk)
+{
+    if (sk.on())
+        return;
+    __shared__ double xv[RS_AMG_LDS_ROWS + 1];
+    const int blk = blockIdx.x;
+    const int b0 = bstart[blk], nb = bstart[blk + 1] - b0;
+    for (int r = threadIdx.x; r < nb; r += blockDim.x)
+        xv[r] = xold[b0 + r];
+    if (threadIdx.x == 0)
+        xv[nb] = 0;
+    __syncthreads();
+    const int l0 = blk_lev[blk], l1 = blk_lev[blk + 1];
+    amg_ssor_sweep(l0, l1, 1, W, meta, rows, ecol, eval, dgl, f, b0, xold, omega_f, xv);
+    if (reverse && l1 > l0)
+        amg_ssor_sweep(l1 - 1, l0 - 1, -1, W, meta, rows, ecol, eval, dgl, f, b0, xold,
+                       omega_r, xv);
+    for (int r = threadIdx.x; r < nb; r += blockDim.x)
+        xnew[b0 + r] = xv[r];
+}
+
+// One level (colour) of an in-place SSOR sweep over a whole matrix: the
+// level's rows do not couple, so every row reads the current vector.
+__global__ void __launch_bounds__(256)
+    k_amg_ssor_level(AmgLevelMeta m, const int *__restrict__ rows,
+                     const int *__restrict__ ecol, const double *__restrict__ eval,
+                     const double *__restrict__ dgl, int W, const double *__restrict__ f,
+                     double *x, double omega, Skip sk)
+{
+    if (sk.on())
+        return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m.count)
+        return;
+    const int r = rows[m.roff + t];
+    const size_t q = (size_t)m.off + (size_t)t * W;
+    double sigma = 0;
+    for (int s = 0; s < W; ++s)
+        sigma += eval[q + s] * x[ecol[q + s]];
+    sigma = (f[r] - sigma) / dgl[m.roff + t];
+    const double xr = x[r];
+    x[r] = xr + omega * (sigma - xr);
+}
+
+// ---------------------------------------------------------------- cycle kernels
+template <class Epi>
+__global__ void __launch_bounds__(RS_SPMV_BS)
+    k_amg_spmv(const int *__restrict__ wg_rows, const int *__restrict__ rowptr,
+               const int *__restrict__ colind, const double *__restrict__ val,
+               const double *__restrict__ x, Epi epi, Skip sk)
+{
+    if (sk.on())
+        return;
+    spmv_core<double, RS_SPMV_BS, RS_SPMV_NNZ>(wg_rows, rowptr, colind, val, x, epi);
+}
+
+// prolongation u += P uc with CompColMatrix.multAdd's per-row order
+struct EpiSeqAdd {
+    double *y;
+    __device__ double init(int r) const { return y[r]; }
+    __device__ void row(int r, double dot) { y[r] = dot; }
+};
+
+__global__ void k_amg_copy(const double *__restrict__ a, double *__restrict__ b, int n, Skip sk)
+{
+    if (sk.on())
+        return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        b[i] = a ? a[i] : 0.0;
+}
+
+// ---------------------------------------------------------------- host: level setup
+static void plan_from_host(const std::vector<int> &rp, int n, SpmvPlan &plan, hipStream_t s)
+{
+    spmv_build_plan(rp, n, plan, s);
+}
+
+static void download(hipStream_t s, std::vector<int> &h, const int *d, size_t n)
+{
+    h.resize(n);
+    if (n)
+        RS_HIP(hipMemcpyAsync(h.data(), d, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+}
+
+// Symmetrised in-block level schedule and full-row ELL for SSOR
+// Greedy colouring in row order of the symmetrised in-block graph: a row
+// takes the smallest colour none of its earlier in-block neighbours has.
+static void greedy_colour(int n, const std::vector<int> &rp, const std::vector<int> &ci,
+                          const std::vector<int> &blk, std::vector<int> &colour)
+{
+    const int nnz = rp[n];
+    std::vector<int> tp(n + 1, 0), ti(std::max(nnz, 1));
+    for (int j = 0; j < nnz; ++j)
+        tp[ci[j] + 1]++;
+    for (int i = 0; i < n; ++i)
+        tp[i + 1] += tp[i];
+    std::vector<int> pos(tp.begin(), tp.end() - 1);
+    for (int i = 0; i < n; ++i)
+        for (int j = rp[i]; j < rp[i + 1]; ++j)
+            ti[pos[ci[j]]++] = i;
+    std::vector<int> mark(n + 2, -1);
+    colour.assign(n, 0);
+    for (int i = 0; i < n; ++i) {
+        for (int j = rp[i]; j < rp[i + 1]; ++j) {
+            const int c = ci[j];
+            if (c < i && blk[c] == blk[i])
+                mark[colour[c]] = i;
+        }
+        for (int j = tp[i]; j < tp[i + 1]; ++j) {
+            const int c = ti[j];
+            if (c < i && blk[c] == blk[i])
+                mark[colour[c]] = i;
+        }
+        int col = 0;
+        while (mark[col] == i)
+            ++col;
+        colour[i] = col;
+    }
+}
+
+static void build_ssor_schedule(AmgLevel &L, int block_rows, int ordering, hipStream_t s)
+{
+    const int n = L.n;
+    const auto &rp = L.h_rp, &ci = L.h_ci;
+    int nb = 1;
+    if (block_rows > 0 && n > 0)
+        nb = (n + block_rows - 1) / block_rows;
+    std::vector<int> bstart(nb + 1);
+    for (int b = 0; b <= nb; ++b)
+        bstart[b] = (int)(((long long)n * b) / nb);
+    int maxrows = 0, W = 1;
+    for (int b = 0; b < nb; ++b)
+        maxrows = std::max(maxrows, bstart[b + 1] - bstart[b]);
+    const bool global = maxrows > RS_AMG_LDS_ROWS;
+    if (global && nb > 1)
+        throw RsError{RSGPU_BAD_ARGUMENT, "AMG smoother block larger than the LDS vector"};
+    for (int i = 0; i < n; ++i)
+        W = std::max(W, rp[i + 1] - rp[i] - 1);
+    W = (W + 3) & ~3;
+    std::vector<int> level(n, 0), pushed(n, 0), blk(n, 0), colour;
+    for (int b = 0; b < nb; ++b)
+        for (int r = bstart[b]; r < bstart[b + 1]; ++r)
+            blk[r] = b;
+    if (ordering == 1)
+        greedy_colour(n, rp, ci, blk, colour);
+    std::vector<int> blk_lev(nb + 1), rows, row_glob, col, src;
+    std::vector<AmgLevelMeta> meta;
+    int nlev_max = 0;
+    for (int b = 0; b < nb; ++b) {
+        const int b0 = bstart[b], b1 = bstart[b + 1];
+        int maxl = -1;
+        if (ordering == 1) {
+            for (int r = b0; r < b1; ++r) {
+                level[r] = colour[r];
+                maxl = std::max(maxl, level[r]);
+            }
+        } else {
+            for (int r = b0; r < b1; ++r) {
+                int l = pushed[r];
+                for (int j = rp[r]; j < rp[r + 1]; ++j) {
+                    const int c = ci[j];
+                    if (c >= b0 && c < r)
+                        l = std::max(l, level[c] + 1);
+                }
+                level[r] = l;
+                maxl = std::max(maxl, l);
+                for (int j = rp[r]; j < rp[r + 1]; ++j) {
+                    const int c = ci[j];
+                    if (c > r && c < b1)
+                        pushed[c] = std::max(pushed[c], l + 1);
+                }
+            }
+        }
+        const int nlev = maxl + 1;
+        nlev_max = std::max(nlev_max, nlev);
+        std::vector<std::vector<int>> byl(std::max(nlev, 0));
+        for (int r = b0; r < b1; ++r)
+            byl[level[r]].push_back(r);
+        blk_lev[b] = (int)meta.size();
+        for (int l = 0; l < nlev; ++l) {
+            AmgLevelMeta m;
+            m.count = (int)byl[l].size();
+            m.off = (int)col.size();
+            m.roff = (int)rows.size();
+            m.pad = 0;
+            col.resize(col.size() + (size_t)W * m.count, b1 - b0);
+            src.resize(src.size() + (size_t)W * m.count, -1);
+            for (int t = 0; t < m.count; ++t) {
+                const int r = byl[l][t];
+                rows.push_back(r - b0);
+                row_glob.push_back(r);
+                int q = 0;
+                for (int j = rp[r]; j < rp[r + 1]; ++j) {
+                    const int c = ci[j];
+                    if (c == r)
+                        continue;
+                    const size_t o = (size_t)m.off + (size_t)t * W + q;
+                    col[o] = (c >= b0 && c < b1) ? c - b0 : -1 - c;
+                    src[o] = j;
+                    ++q;
+                }
+            }
+            meta.push_back(m);
+        }
+    }
+    blk_lev[nb] = (int)meta.size();
+    col.resize(col.size() + W, 0);
+    src.resize(src.size() + W, -1);
+    row_glob.push_back(0);
+    rows.push_back(0);
+    L.nb = nb;
+    L.W = W;
+    L.max_block_rows = maxrows;
+    L.nlev_max = nlev_max;
+    L.global_sweep = global;
+    L.h_meta = meta;
+    L.bstart.upload(bstart, s);
+    L.blk_lev.upload(blk_lev, s);
+    L.meta.upload((const int *)meta.data(), meta.size() * 4, s);
+    L.rows.upload(rows, s);
+    L.row_glob.upload(row_glob, s);
+    L.ell_col.upload(col, s);
+    L.ell_src.upload(src, s);
+    L.ell_size = col.size();
+    L.nrows_lev = (int)row_glob.size();
+    L.ell_val.alloc(L.ell_size);
+    L.dgl.alloc(L.nrows_lev);
+}
+
+static void ssor_values(AmgLevel &L, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_amg_gather, dim3(gridn((long long)L.ell_size)), dim3(256), 0, s,
+                       L.ell_src.p, L.v.p, L.ell_val.p, L.ell_size);
+    hipLaunchKernelGGL(k_amg_rowdiag, dim3(gridn(L.nrows_lev)), dim3(256), 0, s,
+                       L.row_glob.p, L.diag.p, L.v.p, L.dgl.p, L.nrows_lev);
+    RS_CHECK_LAUNCH();
+}
+
+template <int TAB>
+static void spgemm_launch(hipStream_t s, int nrows, const int *Xrp, const int *Xci,
+                          const double *Xv, const int *Yrp, const int *Yci, const double *Yv,
+                          int *cnt, const int *Crp, int *Cci, double *Cv, int *err)
+{
+    hipLaunchKernelGGL(k_spgemm<TAB>, dim3(nrows), dim3(64), 0, s, nrows, Xrp, Xci, Xv, Yrp,
+                       Yci, Yv, cnt, Crp, Cci, Cv, err);
+    RS_CHECK_LAUNCH();
+}
+
+// C = X Y (CSR, sorted rows, nonzeros only); returns nnz
+static int spgemm(AmgData *D, hipStream_t s, int nrows, const int *Xrp, const int *Xci,
+                  const double *Xv, const int *Yrp, const int *Yci, const double *Yv,
+                  int ycols, DevBuf<int> &Crp, DevBuf<int> &Cci, DevBuf<double> &Cv)
+{
+    Crp.alloc((size_t)nrows + 1);
+    if (nrows == 0) {
+        set_int(s, Crp.p, 0);
+        Cci.alloc(1);
+        Cv.alloc(1);
+        return 0;
+    }
+    DevBuf<int> cnt, flag;
+    cnt.alloc((size_t)nrows + 1);
+    flag.alloc(2);
+    RS_HIP(hipMemsetAsync(flag.p, 0, 2 * sizeof(int), s));
+    hipLaunchKernelGGL(k_spgemm_ub, dim3(gridn(nrows)), dim3(256), 0, s, nrows, Xrp, Xci, Yrp,
+                       flag.p);
+    RS_CHECK_LAUNCH();
+    int ub = std::min(read_int(s, flag.p), ycols);
+    int tab = 64;
+    while (tab < 2 * ub && tab < 2048)
+        tab <<= 1;
+    RS_HIP(hipMemsetAsync(flag.p, 0, 2 * sizeof(int), s));
+    RS_HIP(hipMemsetAsync(cnt.p + nrows, 0, sizeof(int), s));
+    auto go = [&](int *c, const int *rp, int *ci, double *cv) {
+        switch (tab) {
+        case 64:
+            spgemm_launch<64>(s, nrows, Xrp, Xci, Xv, Yrp, Yci, Yv, c, rp, ci, cv, flag.p);
+            break;
+        case 128:
+            spgemm_launch<128>(s, nrows, Xrp, Xci, Xv, Yrp, Yci, Yv, c, rp, ci, cv, flag.p);
+            break;
+        case 256:
+            spgemm_launch<256>(s, nrows, Xrp, Xci, Xv, Yrp, Yci, Yv, c, rp, ci, cv, flag.p);
+            break;
+        case 512:
+            spgemm_launch<512>(s, nrows, Xrp, Xci, Xv, Yrp, Yci, Yv, c, rp, ci, cv, flag.p);
+            break;
+        case 1024:
+            spgemm_launch<1024>(s, nrows, Xrp, Xci, Xv, Yrp, Yci, Yv, c, rp, ci, cv, flag.p);
+            break;
+        default:
+            spgemm_launch<2048>(s, nrows, Xrp, Xci, Xv, Yrp, Yci, Yv, c, rp, ci, cv, flag.p);
+            break;
+        }
+    };
+    go(cnt.p, nullptr, nullptr, nullptr);
+    if (read_int(s, flag.p))
+        throw RsError{RSGPU_BAD_ARGUMENT, "AMG Galerkin product row exceeds 2048 entries"};
+    scan_exclusive(D, s, cnt.p, Crp.p, nrows + 1);
+    const int nnz = read_int(s, Crp.p + nrows);
+    Cci.alloc(std::max(nnz, 1));
+    Cv.alloc(std::max(nnz, 1));
+    go(nullptr, Crp.p, Cci.p, Cv.p);
+    RS_HIP(hipStreamSynchronize(s));
+    return nnz;
+}
+
+// aggregation of level L into L.pt; returns the aggregate count
+static int amg_aggregate(AmgData *D, AmgLevel &L, int k, hipStream_t s)
+{
+    const int n = L.n;
+    const int G = gridn(n);
+    DevBuf<int> cnt, Np, Ni, root_of, owner, id, sel, flag;
+    DevBuf<char> R, cand;
+    DevBuf<unsigned long long> m;
+    cnt.alloc((size_t)n + 1);
+    Np.alloc((size_t)n + 1);
+    R.alloc(n);
+    cand.alloc(n);
+    m.alloc(n);
+    root_of.alloc(n);
+    owner.alloc(n);
+    id.alloc((size_t)n + 1);
+    sel.alloc((size_t)n + 1);
+    flag.alloc(1);
+    L.pt.alloc(n);
+    const double eps = 0.08 * std::pow(0.5, k);
+    hipLaunchKernelGGL(k_amg_strength, dim3(G), dim3(256), 0, s, n, L.rp, L.ci, L.v.p, L.diag.p,
+                       eps, cnt.p, (const int *)nullptr, (int *)nullptr, R.p);
+    RS_HIP(hipMemsetAsync(cnt.p + n, 0, sizeof(int), s));
+    scan_exclusive(D, s, cnt.p, Np.p, n + 1);
+    const int nn = read_int(s, Np.p + n);
+    Ni.alloc(std::max(nn, 1));
+    hipLaunchKernelGGL(k_amg_strength, dim3(G), dim3(256), 0, s, n, L.rp, L.ci, L.v.p, L.diag.p,
+                       eps, (int *)nullptr, (const int *)Np.p, Ni.p, (char *)nullptr);
+    RS_HIP(hipMemsetAsync(root_of.p, 0xff, sizeof(int) * n, s));
+    // Every round retires at least the candidate of largest priority; a
+    // neighbourhood can only be empty with a NaN diagonal, which the round
+    // limit turns into an error.
+    const int max_rounds = 10000;
+    int rounds = 0;
+    // phase 1
+    for (;;) {
+        if (++rounds > max_rounds)
+            throw RsError{RSGPU_BAD_ARGUMENT, "AMG aggregation did not terminate (NaN entries?)"};
+        set_int(s, flag.p, 0);
+        hipLaunchKernelGGL(k_p1_cand, dim3(G), dim3(256), 0, s, n, Np.p, Ni.p, R.p, cand.p, m.p,
+                           flag.p);
+        RS_CHECK_LAUNCH();
+        if (!read_int(s, flag.p))
+            break;
+        hipLaunchKernelGGL(k_p1_max, dim3(G), dim3(256), 0, s, n, Np.p, Ni.p, cand.p, m.p);
+        hipLaunchKernelGGL(k_p1_sel, dim3(G), dim3(256), 0, s, n, Np.p, Ni.p, cand.p, m.p, R.p,
+                           root_of.p);
+        RS_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL(k_root_flag, dim3(G), dim3(256), 0, s, n, root_of.p, sel.p);
+    RS_HIP(hipMemsetAsync(sel.p + n, 0, sizeof(int), s));
+    scan_exclusive(D, s, sel.p, id.p, n + 1);
+    int c = read_int(s, id.p + n);
+    hipLaunchKernelGGL(k_owner, dim3(G), dim3(256), 0, s, n, root_of.p, id.p, owner.p, L.pt.p);
+    // phase 2
+    hipLaunchKernelGGL(k_enlarge, dim3(G), dim3(256), 0, s, n, Np.p, Ni.p, owner.p, R.p,
+                       L.pt.p);
+    RS_CHECK_LAUNCH();
+    // phase 3
+    rounds = 0;
+    for (;;) {
+        if (++rounds > max_rounds)
+            throw RsError{RSGPU_BAD_ARGUMENT, "AMG aggregation did not terminate (NaN entries?)"};
+        set_int(s, flag.p, 0);
+        hipLaunchKernelGGL(k_p3_cand, dim3(G), dim3(256), 0, s, n, Np.p, Ni.p, R.p, cand.p, m.p,
+                           flag.p);
+        RS_CHECK_LAUNCH();
+        if (!read_int(s, flag.p))
+            break;
+        hipLaunchKernelGGL(k_p3_max, dim3(G), dim3(256), 0, s, n, Np.p, Ni.p, R.p, cand.p, m.p);
+        hipLaunchKernelGGL(k_p3_sel, dim3(G), dim3(256), 0, s, n, Np.p, Ni.p, R.p, cand.p, m.p,
+                           sel.p);
+        RS_HIP(hipMemsetAsync(sel.p + n, 0, sizeof(int), s));
+        scan_exclusive(D, s, sel.p, id.p, n + 1);
+        hipLaunchKernelGGL(k_p3_assign, dim3(G), dim3(256), 0, s, n, Np.p, Ni.p, R.p, sel.p,
+                           id.p, c, L.pt.p);
+        hipLaunchKernelGGL(k_p3_clear, dim3(G), dim3(256), 0, s, n, L.pt.p, R.p);
+        RS_CHECK_LAUNCH();
+        c += read_int(s, id.p + n);
+    }
+    if (c == 0)
+        return 0;
+    // prolongator (count, scan, fill)
+    L.Prp.alloc((size_t)n + 1);
+    hipLaunchKernelGGL(k_prolong, dim3(G), dim3(256), 0, s, n, L.rp, L.ci, L.v.p, L.diag.p,
+                       Np.p, Ni.p, L.pt.p, D->p.omega, cnt.p, (const int *)nullptr,
+                       (int *)nullptr, (double *)nullptr);
+    RS_HIP(hipMemsetAsync(cnt.p + n, 0, sizeof(int), s));
+    scan_exclusive(D, s, cnt.p, L.Prp.p, n + 1);
+    L.pnnz = read_int(s, L.Prp.p + n);
+    L.Pci.alloc(std::max(L.pnnz, 1));
+    L.Pv.alloc(std::max(L.pnnz, 1));
+    hipLaunchKernelGGL(k_prolong, dim3(G), dim3(256), 0, s, n, L.rp, L.ci, L.v.p, L.diag.p,
+                       Np.p, Ni.p, L.pt.p, D->p.omega, (int *)nullptr, (const int *)L.Prp.p,
+                       L.Pci.p, L.Pv.p);
+    RS_CHECK_LAUNCH();
+    return c;
+}
+
+// P' as CSR (c rows), rows ascending within each column (stable sort)
+static void amg_transpose(AmgData *D, AmgLevel &L, hipStream_t s)
+{
+    const int n = L.n, c = L.c, nnz = L.pnnz;
+    L.Trp.alloc((size_t)c + 1);
+    L.Tci.alloc(std::max(nnz, 1));
+    L.Tv.alloc(std::max(nnz, 1));
+    DevBuf<int> erow, keys_out, vals_in, vals_out, cnt;
+    erow.alloc(std::max(nnz, 1));
+    keys_out.alloc(std::max(nnz, 1));
+    vals_in.alloc(std::max(nnz, 1));
+    vals_out.alloc(std::max(nnz, 1));
+    cnt.alloc((size_t)c + 1);
+    hipLaunchKernelGGL(k_entry_rows, dim3(gridn(n)), dim3(256), 0, s, n, L.Prp.p, erow.p);
+    hipLaunchKernelGGL(k_iota, dim3(gridn(nnz)), dim3(256), 0, s, nnz, vals_in.p);
+    RS_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int) * ((size_t)c + 1), s));
+    hipLaunchKernelGGL(k_col_count, dim3(gridn(nnz)), dim3(256), 0, s, nnz, L.Pci.p, cnt.p);
+    RS_CHECK_LAUNCH();
+    scan_exclusive(D, s, cnt.p, L.Trp.p, c + 1);
+    int bits = 1;
+    while ((1 << bits) <= c && bits < 31)
+        ++bits;
+    size_t bytes = 0;
+    RS_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, L.Pci.p, keys_out.p, vals_in.p,
+                                              vals_out.p, nnz, 0, bits, s));
+    if (D->tmp_bytes.n < bytes)
+        D->tmp_bytes.alloc(bytes);
+    RS_HIP(hipcub::DeviceRadixSort::SortPairs(D->tmp_bytes.p, bytes, L.Pci.p, keys_out.p,
+                                              vals_in.p, vals_out.p, nnz, 0, bits, s));
+    hipLaunchKernelGGL(k_transpose_fill, dim3(gridn(nnz)), dim3(256), 0, s, nnz, vals_out.p,
+                       erow.p, L.Pv.p, L.Tci.p, L.Tv.p);
+    RS_CHECK_LAUNCH();
+    RS_HIP(hipStreamSynchronize(s));
+}
+
+// u and ualt carry one extra zero entry: the padding column of an in-place
+// (global) sweep
+static void level_vectors(AmgLevel &L, hipStream_t s)
+{
+    L.u.alloc((size_t)L.n + 1);
+    L.ualt.alloc((size_t)L.n + 1);
+    L.u.zero(s);
+    L.ualt.zero(s);
+    L.f.alloc(std::max(L.n, 1));
+    L.r.alloc(std::max(L.n, 1));
+}
+
+// AMG.setMatrix (AMG.java:228-294).  Level 0 holds its own copy of A
+// (new CompRowMatrix(A), AMG.java:232); its pattern, SpMV plan and smoother
+// schedule are kept while A's pattern stays the same.
+int amg_setup(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
+{
+    hipStream_t s = M->ctx->stream;
+    if (A->n != A->ncols)
+        throw RsError{RSGPU_BAD_ARGUMENT, "AMG only applies to square matrices"};
+    if (!M->amg)
+        M->amg = new AmgData();
+    AmgData *D = M->amg;
+    D->p = M->amg_params;
+    const rsgpu_amg_params &p = D->p;
+    const bool same = !D->L.empty() && D->pattern0 == A->pattern_id;
+    if (!same) {
+        D->L.clear();
+        D->L.push_back(std::make_unique<AmgLevel>());
+        AmgLevel &L0 = *D->L[0];
+        L0.n = A->n;
+        L0.nnz = A->nnz;
+        L0.rp_own.alloc((size_t)A->n + 1);
+        L0.ci_own.alloc(std::max(A->nnz, 1));
+        RS_HIP(hipMemcpyAsync(L0.rp_own.p, A->rowptr.p, sizeof(int) * ((size_t)A->n + 1),
+                              hipMemcpyDeviceToDevice, s));
+        if (A->nnz)
+            RS_HIP(hipMemcpyAsync(L0.ci_own.p, A->colind.p, sizeof(int) * A->nnz,
+                                  hipMemcpyDeviceToDevice, s));
+        L0.rp = L0.rp_own.p;
+        L0.ci = L0.ci_own.p;
+        L0.h_rp = A->h_rowptr;
+        L0.h_ci = A->h_colind;
+        plan_from_host(L0.h_rp, L0.n, L0.plan, s);
+        D->pattern0 = A->pattern_id;
+    } else {
+        D->L.resize(1);
+    }
+    {
+        AmgLevel &L0 = *D->L[0];
+        L0.v.alloc(std::max(A->nnz, 1));
+        if (A->nnz)
+            RS_HIP(hipMemcpyAsync(L0.v.p, A->val.p, sizeof(double) * A->nnz,
+                                  hipMemcpyDeviceToDevice, s));
+    }
+    DevBuf<int> bad;
+    bad.alloc(1);
+    for (int k = 0; D->L[k]->n > p.min; ++k) {
+        AmgLevel &L = *D->L[k];
+        const int n = L.n;
+        L.diag.alloc(n);
+        set_int(s, bad.p, 0x7fffffff);
+        hipLaunchKernelGGL(k_amg_diag, dim3(gridn(n)), dim3(256), 0, s, n, L.rp, L.ci, L.diag.p,
+                           bad.p);
+        RS_CHECK_LAUNCH();
+        const int b = read_int(s, bad.p);
+        if (b != 0x7fffffff) {
+            if (bad_row)
+                *bad_row = b;
+            M->ctx->err = "Matrix is missing a diagonal entry on row " + std::to_string(b);
+            D->L.clear();
+            return RSGPU_MISSING_DIAGONAL;
+        }
+        const int c = amg_aggregate(D, L, k, s);
+        if (c == 0)
+            break;
+        L.c = c;
+        amg_transpose(D, L, s);
+        // AP = A P (n x c), Ac = P' AP (c x c)
+        DevBuf<int> APrp, APci;
+        DevBuf<double> APv;
+        spgemm(D, s, n, L.rp, L.ci, L.v.p, L.Prp.p, L.Pci.p, L.Pv.p, c, APrp, APci, APv);
+        D->L.push_back(std::make_unique<AmgLevel>());
+        AmgLevel &Lc = *D->L[k + 1];
+        Lc.n = c;
+        Lc.nnz = spgemm(D, s, c, L.Trp.p, L.Tci.p, L.Tv.p, APrp.p, APci.p, APv.p, c,
+                        Lc.rp_own, Lc.ci_own, Lc.v);
+        Lc.rp = Lc.rp_own.p;
+        Lc.ci = Lc.ci_own.p;
+    }
+    D->m = (int)D->L.size();
+    // smoothers, transfer plans and vectors
+    for (int k = 0; k < D->m; ++k) {
+        AmgLevel &L = *D->L[k];
+        level_vectors(L, s);
+        if (k == D->m - 1)
+            break;
+        if (k > 0) {
+            download(s, L.h_rp, L.rp, (size_t)L.n + 1);
+            RS_HIP(hipStreamSynchronize(s));
+            download(s, L.h_ci, L.ci, (size_t)L.h_rp[L.n]);
+        }
+        std::vector<int> prp, trp;
+        download(s, prp, L.Prp.p, (size_t)L.n + 1);
+        download(s, trp, L.Trp.p, (size_t)L.c + 1);
+        RS_HIP(hipStreamSynchronize(s));
+        if (k > 0)
+            plan_from_host(L.h_rp, L.n, L.plan, s);
+        plan_from_host(prp, L.n, L.Pplan, s);
+        plan_from_host(trp, L.c, L.Tplan, s);
+        if (!(k == 0 && L.sched_pattern == A->pattern_id && L.nb_rows_param == p.block_rows &&
+              L.ordering_param == p.ordering)) {
+            build_ssor_schedule(L, p.block_rows, p.ordering, s);
+            L.sched_pattern = k == 0 ? A->pattern_id : 0;
+            L.nb_rows_param = p.block_rows;
+            L.ordering_param = p.ordering;
+        }
+        ssor_values(L, s);
+    }
+    // coarsest level: dense LU (AMG.java:268-271)
+    AmgLevel &Lc = *D->L[D->m - 1];
+    D->nc = Lc.n;
+    if (D->nc > p.max_coarse)
+        throw RsError{RSGPU_BAD_ARGUMENT,
+                      "AMG coarsest level has " + std::to_string(D->nc) +
+                          " rows, above max_coarse"};
+    D->lu.alloc((size_t)std::max(D->nc, 1) * std::max(D->nc, 1));
+    D->ipiv.alloc(std::max(D->nc, 1));
+    D->flag.alloc(1);
+    if (D->nc > 0) {
+        const int *crp = Lc.rp;
+        const int *cci = Lc.ci;
+        hipLaunchKernelGGL(k_dense_fill, dim3(gridn(D->nc)), dim3(256), 0, s, D->nc, crp, cci,
+                           Lc.v.p, D->lu.p);
+        hipLaunchKernelGGL(k_dense_getf2, dim3(1), dim3(256), 0, s, D->nc, D->lu.p, D->ipiv.p,
+                           D->flag.p);
+        RS_CHECK_LAUNCH();
+        if (read_int(s, D->flag.p)) {
+            M->ctx->err = "AMG coarsest matrix is singular";
+            return RSGPU_SINGULAR;
+        }
+    }
+    RS_HIP(hipStreamSynchronize(s));
+    return RSGPU_OK;
+}
+
+// ---------------------------------------------------------------- cycle
+static void amg_cycle(AmgData *D, int k, const double *f, bool trans, hipStream_t s, Skip sk)
+{
+    AmgLevel &L = *D->L[k];
+    const rsgpu_amg_params &p = D->p;
+    if (k == D->m - 1) {
+        if (D->nc > 0 && D->nc <= 64 && !trans)
+            hipLaunchKernelGGL(k_dense_getrs64, dim3(1), dim3(64), 0, s, D->nc, D->lu.p,
+                               D->ipiv.p, f, L.u.p, sk);
+        else if (D->nc > 0)
+            hipLaunchKernelGGL(k_dense_getrs, dim3(1), dim3(256), 0, s, D->nc, D->lu.p,
+                               D->ipiv.p, f, L.u.p, trans ? 1 : 0, sk);
+        RS_CHECK_LAUNCH();
+        return;
+    }
+    auto smooth = [&](double wf, double wr) {
+        if (L.global_sweep) {
+            const int nl = (int)L.h_meta.size();
+            for (int l = 0; l < nl; ++l)
+                hipLaunchKernelGGL(k_amg_ssor_level, dim3(gridn(L.h_meta[l].count)), dim3(256),
+                                   0, s, L.h_meta[l], L.rows.p, L.ell_col.p, L.ell_val.p,
+                                   L.dgl.p, L.W, f, L.u.p, wf, sk);
+            if (p.reverse)
+                for (int l = nl - 1; l >= 0; --l)
+                    hipLaunchKernelGGL(k_amg_ssor_level, dim3(gridn(L.h_meta[l].count)),
+                                       dim3(256), 0, s, L.h_meta[l], L.rows.p, L.ell_col.p,
+                                       L.ell_val.p, L.dgl.p, L.W, f, L.u.p, wr, sk);
+            RS_CHECK_LAUNCH();
+            return;
+        }
+        hipLaunchKernelGGL(k_amg_ssor, dim3(L.nb), dim3(256), 0, s, L.bstart.p, L.blk_lev.p,
+                           (const AmgLevelMeta *)L.meta.p, L.rows.p, L.ell_col.p, L.ell_val.p,
+                           L.dgl.p, L.W, f, L.u.p, L.ualt.p, wf, wr, p.reverse, sk);
+        RS_CHECK_LAUNCH();
+        swap_buf(L.u, L.ualt);
+    };
+    // SSOR.transApply is apply (the matrix is assumed symmetric)
+    for (int i = 0; i < p.nu1; ++i)
+        smooth(p.omega_pre_f, p.omega_pre_r);
+    AmgLevel &C = *D->L[k + 1];
+    hipLaunchKernelGGL(k_amg_copy, dim3(gridn(C.n)), dim3(256), 0, s, (const double *)nullptr,
+                       C.u.p, C.n, sk);
+    // r = f - A u (r.set(f); A.multAdd(-1, u, r))
+    hipLaunchKernelGGL((k_amg_spmv<EpiResid<double>>), dim3(L.plan.nwg), dim3(RS_SPMV_BS), 0,
+                       s, L.plan.wg_rows.p, L.rp, L.ci, L.v.p, L.u.p,
+                       EpiResid<double>{L.r.p, f, -1.0}, sk);
+    // f_c = I' r (CompColMatrix.transMult)
+    hipLaunchKernelGGL((k_amg_spmv<EpiPlain<double>>), dim3(L.Tplan.nwg), dim3(RS_SPMV_BS), 0,
+                       s, L.Tplan.wg_rows.p, L.Trp.p, L.Tci.p, L.Tv.p, L.r.p,
+                       EpiPlain<double>{C.f.p}, sk);
+    RS_CHECK_LAUNCH();
+    for (int g = 0; g < p.gamma; ++g)
+        amg_cycle(D, k + 1, C.f.p, trans, s, sk);
+    // u += I u_c (CompColMatrix.multAdd)
+    hipLaunchKernelGGL((k_amg_spmv<EpiSeqAdd>), dim3(L.Pplan.nwg), dim3(RS_SPMV_BS), 0, s,
+                       L.Pplan.wg_rows.p, L.Prp.p, L.Pci.p, L.Pv.p, C.u.p, EpiSeqAdd{L.u.p},
+                       sk);
+    RS_CHECK_LAUNCH();
+    for (int i = 0; i < p.nu2; ++i)
+        smooth(p.omega_post_f, p.omega_post_r);
+}
+
+// AMG.apply / transApply (AMG.java:208-226)
+void amg_apply(rsgpu_prec *M, const double *b, double *x, const DevState *st, bool trans)
+{
+    AmgData *D = M->amg;
+    hipStream_t s = M->ctx->stream;
+    Skip sk{st ? &st->done : nullptr, st ? &st->cycle_end : nullptr};
+    AmgLevel &L0 = *D->L[0];
+    const int n = L0.n;
+    if (n == 0)
+        return;
+    hipLaunchKernelGGL(k_amg_copy, dim3(gridn(n)), dim3(256), 0, s,
+                       D->p.zero_guess ? (const double *)nullptr : (const double *)x, L0.u.p, n,
+                       sk);
+    RS_CHECK_LAUNCH();
+    amg_cycle(D, 0, b, trans, s, sk);
+    hipLaunchKernelGGL(k_amg_copy, dim3(gridn(n)), dim3(256), 0, s, (const double *)L0.u.p, x,
+                       n, sk);
+    RS_CHECK_LAUNCH();
+}
+
+int amg_levels(const rsgpu_prec *M, int *sizes, int *nnzs, int cap)
+{
+    const AmgData *D = M->amg;
+    if (!D)
+        return 0;
+    for (int k = 0; k < D->m && k < cap; ++k) {
+        if (sizes)
+            sizes[k] = D->L[k]->n;
+        if (nnzs)
+            nnzs[k] = D->L[k]->nnz;
+    }
+    return D->m;
+}
+
+int amg_get_level(const rsgpu_prec *M, int k, int32_t *rowptr, int32_t *colind, double *data,
+                  int32_t *pt, int32_t *Prp, int32_t *Pci, double *Pv, int *pnnz)
+{
+    const AmgData *D = M->amg;
+    if (!D || k < 0 || k >= D->m)
+        return RSGPU_BAD_ARGUMENT;
+    const AmgLevel &L = *D->L[k];
+    const int n = L.n, nnz = L.nnz;
+    if (rowptr)
+        RS_HIP(hipMemcpy(rowptr, L.rp, sizeof(int) * ((size_t)n + 1), hipMemcpyDeviceToHost));
+    if (colind && nnz)
+        RS_HIP(hipMemcpy(colind, L.ci, sizeof(int) * nnz, hipMemcpyDeviceToHost));
+    if (data && nnz)
+        RS_HIP(hipMemcpy(data, L.v.p, sizeof(double) * nnz, hipMemcpyDeviceToHost));
+    if (pnnz)
+        *pnnz = k < D->m - 1 ? L.pnnz : 0;
+    if (k < D->m - 1) {
+        if (pt && n)
+            RS_HIP(hipMemcpy(pt, L.pt.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+        if (Prp)
+            RS_HIP(hipMemcpy(Prp, L.Prp.p, sizeof(int) * ((size_t)n + 1),
+                             hipMemcpyDeviceToHost));
+        if (Pci && L.pnnz)
+            RS_HIP(hipMemcpy(Pci, L.Pci.p, sizeof(int) * L.pnnz, hipMemcpyDeviceToHost));
+        if (Pv && L.pnnz)
+            RS_HIP(hipMemcpy(Pv, L.Pv.p, sizeof(double) * L.pnnz, hipMemcpyDeviceToHost));
+    }
+    return RSGPU_OK;
+}

@@ -792,6 +792,10 @@ __global__ void k_copy_skip32(const float *__restrict__ a, float *__restrict__ b
 // apply with optional device skip flags (done / cycle_end of a solver)
 void prec_apply(rsgpu_prec *M, const double *b, double *x, const DevState *st)
 {
+    if (M->kind == RSGPU_PREC_AMG) {
+        amg_apply(M, b, x, st, false);
+        return;
+    }
     const int *sk1 = st ? &st->done : nullptr;
     const int *sk2 = st ? &st->cycle_end : nullptr;
     hipStream_t s = M->ctx->stream;
@@ -828,7 +832,7 @@ void prec_apply(rsgpu_prec *M, const float *b, float *x, const DevState *st)
         RS_CHECK_LAUNCH();
         return;
     }
-    if (M->precision != RSGPU_FP32)
+    if (M->precision != RSGPU_FP32 || M->kind == RSGPU_PREC_AMG)
         throw RsError{RSGPU_BAD_ARGUMENT, "FP32 apply needs an FP32 preconditioner"};
     launch_apply<float, false>(M, M->L, M->ell32_L.p, nullptr, M->U, M->ell32_U.p, M->dgU32.p,
                                b, x, M->scratch32.p, sk1, sk2);
@@ -836,6 +840,10 @@ void prec_apply(rsgpu_prec *M, const float *b, float *x, const DevState *st)
 
 void prec_trans_apply(rsgpu_prec *M, const double *b, double *x)
 {
+    if (M->kind == RSGPU_PREC_AMG) {
+        amg_apply(M, b, x, nullptr, true);
+        return;
+    }
     hipStream_t s = M->ctx->stream;
     if (M->kind == RSGPU_PREC_IDENTITY) {
         prec_apply(M, b, x, (const DevState *)nullptr);

@@ -868,9 +868,15 @@ extern "C" int rsgpu_solver_create(rsgpu_ctx *ctx, const rsgpu_solver_opts *opts
             nvec32 = 2 + (inner_kind == RSGPU_GMRES ? 3 + (size_t)m + 1 : 8);
         }
         S->ldv = ((size_t)n + 3) & ~(size_t)3;
+        // the solver's vectors start as zero vectors, as the reference's
+        // template-built DenseVectors do (a preconditioner such as AMG reads
+        // its output vector's old content)
         S->w64.alloc(std::max<size_t>(1, nvec64 * S->ldv));
-        if (nvec32)
+        S->w64.zero(ctx->stream);
+        if (nvec32) {
             S->w32.alloc(std::max<size_t>(1, nvec32 * S->ldv));
+            S->w32.zero(ctx->stream);
+        }
         RS_HIP(hipEventCreateWithFlags(&S->ev[0], hipEventDisableTiming));
         RS_HIP(hipEventCreateWithFlags(&S->ev[1], hipEventDisableTiming));
         RS_HIP(hipHostMalloc((void **)&S->flags_host, sizeof(int) * 8,

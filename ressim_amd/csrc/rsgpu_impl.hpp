@@ -124,6 +124,9 @@ struct SweepData {
     int max_count = 0, max_width = 0;
 };
 
+struct AmgData;
+void amg_destroy(AmgData *d);
+
 struct rsgpu_prec {
     rsgpu_ctx *ctx = nullptr;
     int kind = 0, nblocks = 1, precision = RSGPU_FP64;
@@ -155,6 +158,10 @@ struct rsgpu_prec {
     DevBuf<double> tmp64;        // FP32 apply on FP64 vectors
     DevBuf<float> tmpa32, tmpb32;
     int lds_rows = 0;            // rows that fit the LDS vector
+    // AMG hierarchy (amg.hip)
+    rsgpu_amg_params amg_params{};
+    AmgData *amg = nullptr;
+    ~rsgpu_prec() { amg_destroy(amg); }
 };
 
 // ---------------------------------------------------------------- device state
@@ -373,5 +380,13 @@ int prec_factor(rsgpu_prec *M, rsgpu_csr *A, int *bad_row);
 void prec_apply(rsgpu_prec *M, const double *b, double *x, const DevState *st);
 void prec_apply(rsgpu_prec *M, const float *b, float *x, const DevState *st);
 void prec_trans_apply(rsgpu_prec *M, const double *b, double *x);
+
+// amg.hip
+int amg_setup(rsgpu_prec *M, rsgpu_csr *A, int *bad_row);
+void amg_apply(rsgpu_prec *M, const double *b, double *x, const DevState *st, bool trans);
+int amg_levels(const rsgpu_prec *M, int *sizes, int *nnzs, int cap);
+int amg_get_level(const rsgpu_prec *M, int k, int32_t *rowptr, int32_t *colind,
+                  double *data, int32_t *pt, int32_t *Prp, int32_t *Pci, double *Pv,
+                  int *pnnz);
 
 uint64_t next_pattern_id();

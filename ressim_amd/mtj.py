@@ -390,6 +390,67 @@ class BlockILU(Preconditioner):
         super().__init__(nblocks, precision, ctx)
 
 
+class AMG(Preconditioner):
+    """Smoothed-aggregation AMG (mtj/.../sparse/AMG.java).  The positional
+    arguments are AMG(omegaPreF, omegaPreR, omegaPostF, omegaPostR, nu1, nu2,
+    gamma, min, omega) (AMG.java:140-156); ``AMG()`` uses the defaults of
+    AMG.java:204-206.  Keywords: block_rows (SSOR smoother blocks, 0 = one
+    block = SSOR.java), zero_guess (start each cycle from zero instead of x's
+    old content), max_coarse.  Aggregation runs as prioritised parallel
+    rounds on the device (phases 1 and 3); see the header of amg.hip."""
+    kind = L.PREC_AMG
+    _ORDER = ("omega_pre_f", "omega_pre_r", "omega_post_f", "omega_post_r", "nu1", "nu2",
+              "gamma", "min", "omega")
+
+    def __init__(self, *args, ctx: Optional[Context] = None, **kw):
+        super().__init__(1, L.FP64, ctx)
+        p = L.AmgParams()
+        L.load().rsgpu_amg_params_default(C.byref(p))
+        if len(args) == 7:
+            # AMG(omegaPre, omegaPost, nu1, nu2, gamma, min, omega): SOR smoothing
+            # (AMG.java:179-194)
+            args = (args[0], args[0], args[1], args[1]) + tuple(args[2:])
+            kw.setdefault("reverse", 0)
+        elif args and len(args) != 9:
+            raise TypeError("AMG takes 0, 7 or 9 positional arguments")
+        for name, v in zip(self._ORDER, args):
+            setattr(p, name, v)
+        for name, v in kw.items():
+            if not hasattr(p, name):
+                raise TypeError(f"unknown AMG parameter {name}")
+            setattr(p, name, v)
+        self.params = p
+        _raise(L.load().rsgpu_prec_set_amg_params(self.h, C.byref(p)), self.ctx.h)
+
+    def levels(self):
+        sz = np.zeros(64, np.int32)
+        nz = np.zeros(64, np.int32)
+        m = L.load().rsgpu_amg_levels(self.h, _hptr(sz), _hptr(nz), 64)
+        return [(int(sz[k]), int(nz[k])) for k in range(m)]
+
+    def level(self, k):
+        """(rowptr, colind, data, aggregate, P rowptr, P colind, P data) of
+        level k; the last four are None on the coarsest level."""
+        lv = self.levels()
+        n, nnz = lv[k]
+        lib = L.load()
+        pn = C.c_int(0)
+        rp = np.zeros(n + 1, np.int32)
+        ci = np.zeros(max(nnz, 1), np.int32)
+        v = np.zeros(max(nnz, 1))
+        _raise(lib.rsgpu_amg_get_level(self.h, k, _hptr(rp), _hptr(ci), _hptr(v), None, None,
+                                       None, None, C.byref(pn)), self.ctx.h)
+        if k == len(lv) - 1:
+            return rp, ci[:nnz], v[:nnz], None, None, None, None
+        pt = np.zeros(max(n, 1), np.int32)
+        prp = np.zeros(n + 1, np.int32)
+        pci = np.zeros(max(pn.value, 1), np.int32)
+        pv = np.zeros(max(pn.value, 1))
+        _raise(lib.rsgpu_amg_get_level(self.h, k, None, None, None, _hptr(pt), _hptr(prp),
+                                       _hptr(pci), _hptr(pv), C.byref(pn)), self.ctx.h)
+        return rp, ci[:nnz], v[:nnz], pt[:n], prp, pci[: pn.value], pv[: pn.value]
+
+
 # ------------------------------------------------------------------ monitor and solvers
 class DefaultIterationMonitor:
     """mtj/.../sparse/DefaultIterationMonitor.java (defaults :85-90)."""
