@@ -70,7 +70,7 @@ def main():
     ap.add_argument("--inner-rtol", type=float, default=1e-3)
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-iters", type=int, default=62)
+    ap.add_argument("--cpu-iters", type=int, default=300)
     ap.add_argument("--probe", choices=["spmv", "ilu", "solve"], default=None,
                     help="profiling probe: run only the named kernel(s) on the workload "
                          "(plus a 512 MiB device copy as the traffic reference) and exit")

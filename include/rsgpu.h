@@ -49,6 +49,7 @@ typedef struct rsgpu_ctx rsgpu_ctx;
 typedef struct rsgpu_csr rsgpu_csr;
 typedef struct rsgpu_prec rsgpu_prec;
 typedef struct rsgpu_solver rsgpu_solver;
+typedef struct rsgpu_flux rsgpu_flux;
 
 /* ---- context ------------------------------------------------------------ */
 /* No reference counterpart (the JVM is the context); owns the HIP stream. */
@@ -101,6 +102,9 @@ int rsgpu_csr_transmultadd(rsgpu_csr *A, double alpha, const double *x_dev,
 /* Single-precision copy of the values (no reference counterpart; used by
  * the mixed-precision form) and y = A x in FP32. */
 int rsgpu_csr_sync_f32(rsgpu_csr *A);
+/* A new matrix holding A' (rows ascending per column, as transMult's
+ * scatter visits them, CompRowMatrix.java:356-372); values copied now. */
+int rsgpu_csr_transpose(const rsgpu_csr *A, rsgpu_csr **At);
 int rsgpu_csr_mult_f32(rsgpu_csr *A, const float *x_dev, float *y_dev);
 /* Host-array conveniences (DenseVector.getData() round trip). */
 int rsgpu_csr_mult_host(rsgpu_csr *A, const double *x_host, double *y_host);
@@ -202,6 +206,11 @@ typedef struct {
     int inner_max_iter;
     double inner_rtol;
     int check_every;      /* host polls the device monitor every k steps */
+    /* Transposed solve (no reference counterpart; the adjoint uses it): A
+     * is the transpose made by rsgpu_csr_transpose and M the
+     * preconditioner set on the original matrix, applied through
+     * transApply (ILU.java:72-78), i.e. (LU)^-T. */
+    int prec_transpose;
 } rsgpu_solver_opts;
 
 typedef struct {
@@ -251,6 +260,44 @@ int rsgpu_mpfa_o_assemble(rsgpu_ctx *ctx, int dim, int nx, int ny, int nz,
                           double dx, double dy, double dz,
                           const double *K6_host, double continuity,
                           rsgpu_csr **A_out, int *check_fail_host);
+
+/* ---- Fluxes (rs numerics/PotentialCalculator.java) ----------------------- */
+/* A device flux operator from per-connection MD lists (as rsgpu_mpfa_o
+ * returns them, MD order kept) and each connection's here/there cells. */
+int rsgpu_flux_create(rsgpu_ctx *ctx, int nconn, int ncells,
+                      const int32_t *md_ptr_host, const int32_t *md_elem_host,
+                      const double *md_t_host, const int32_t *here_host,
+                      const int32_t *there_host, rsgpu_flux **out);
+/* MPFA-O computed on the device straight into a flux operator. */
+int rsgpu_mpfa_o_flux(rsgpu_ctx *ctx, int dim, int nx, int ny, int nz,
+                      double dx, double dy, double dz, const double *K6_host,
+                      double continuity, rsgpu_flux **out,
+                      int *check_fail_host);
+int rsgpu_flux_destroy(rsgpu_flux *F);
+int rsgpu_flux_info(const rsgpu_flux *F, int *nconn, int *ncells,
+                    int *nentries);
+/* calculateFlux(mesh, potential) (PotentialCalculator.java:265-280):
+ * flux[c] = sum over c.MD in MD order of t_k * p[k]. */
+int rsgpu_flux_apply(rsgpu_flux *F, const double *p_dev, double *flux_dev);
+/* Net outflow of the listed cells (well rates of pinned cells, Source.java
+ * FIXED semantics): sum over each cell's connections in ascending order of
+ * +flux (cell is "here") or -flux ("there"). */
+int rsgpu_flux_rates(rsgpu_flux *F, const double *flux_dev, int ncells_sel,
+                     const int32_t *cells_host, double *rates_host);
+
+/* ---- Adjoint (no reference counterpart: rs has no gradients) ------------ */
+/* tbar[q] = (w[here(c)] - w[there(c)]) * p[md_elem[q]]: gradient of
+ * sum_i w_i * (net outflow of cell i) with respect to the MD
+ * transmissibilities at fixed potential p (layout of F's MD lists). */
+int rsgpu_flux_vjp(rsgpu_flux *F, const double *p_dev, const double *w_dev,
+                   double *tbar_dev);
+/* Gradient through O_MPFA.calculateTransmissibilities: for K_e =
+ * exp(m_e) K0_e, mbar[e] = sum_q tbar[q] d t_q / d m_e (F supplies the MD
+ * lists of the same grid, as rsgpu_mpfa_o_flux built them). */
+int rsgpu_mpfa_o_vjp(rsgpu_ctx *ctx, int dim, int nx, int ny, int nz,
+                     double dx, double dy, double dz, const double *K6_host,
+                     double continuity, const rsgpu_flux *F,
+                     const double *tbar_dev, double *mbar_dev);
 
 #ifdef __cplusplus
 }

@@ -583,11 +583,14 @@ static v3 continuity_point(const mesh_t *m, const rintf_t *I, v3 corner,
               (center.z - corner.z) * continuity + corner.z);
 }
 
-/* calculateT (O_MPFA.java:193-239) for one corner; matrices column-major
- * with leading dimension nc (A, C: nc x nc; B, D, T: nc x ne). */
-static int corner_T(const mesh_t *m, const double *K6, double continuity,
-                    int pi, int pj, int pk, region_t *R, double *A, double *B,
-                    double *C, double *D, double *T)
+/* Local matrices of one corner (assembleElementContribution,
+ * O_MPFA.java:422-457).  When Ae..De are given, each element's own
+ * contribution is also kept (Ae + le*144 etc.): every entry is linear in
+ * that element's tensor, which the adjoint uses. */
+static int corner_local(const mesh_t *m, const double *K6, double continuity,
+                        int pi, int pj, int pk, region_t *R, double *A,
+                        double *B, double *C, double *D, double *Ae,
+                        double *Be, double *Ce, double *De)
 {
     build_region(m, pi, pj, pk, R);
     const int nc = R->nc, ne = R->ne;
@@ -595,6 +598,12 @@ static int corner_T(const mesh_t *m, const double *K6, double continuity,
     memset(C, 0, sizeof(double) * nc * nc);
     memset(B, 0, sizeof(double) * nc * ne);
     memset(D, 0, sizeof(double) * nc * ne);
+    if (Ae) {
+        memset(Ae, 0, sizeof(double) * 8 * 144);
+        memset(Ce, 0, sizeof(double) * 8 * 144);
+        memset(Be, 0, sizeof(double) * 8 * 96);
+        memset(De, 0, sizeof(double) * 8 * 96);
+    }
     v3 corner = point_at(m, pi, pj, pk);
     for (int le = 0; le < ne; ++le) {
         const rel_t *E = &R->el[le];
@@ -645,15 +654,39 @@ static int corner_T(const mesh_t *m, const double *K6, double continuity,
             B[is + ei * nc] += -f;
             if (I->is_here)
                 D[is + ei * nc] += -f;
+            if (Ae) {
+                Be[le * 96 + is + ei * nc] += -f;
+                if (I->is_here)
+                    De[le * 96 + is + ei * nc] += -f;
+            }
             for (int jj = 1; jj < nI + 1; ++jj) {
                 f = vdot(Kn[q], grad[jj]);
                 int js = R->intf[R->eintf[le][jj - 1]].lidx;
                 A[is + js * nc] += f;
                 if (I->is_here)
                     C[is + js * nc] += f;
+                if (Ae) {
+                    Ae[le * 144 + is + js * nc] += f;
+                    if (I->is_here)
+                        Ce[le * 144 + is + js * nc] += f;
+                }
             }
         }
     }
+    return OR_OK;
+}
+
+/* calculateT (O_MPFA.java:193-239) for one corner; matrices column-major
+ * with leading dimension nc (A, C: nc x nc; B, D, T: nc x ne). */
+static int corner_T(const mesh_t *m, const double *K6, double continuity,
+                    int pi, int pj, int pk, region_t *R, double *A, double *B,
+                    double *C, double *D, double *T)
+{
+    int st = corner_local(m, K6, continuity, pi, pj, pk, R, A, B, C, D, NULL,
+                          NULL, NULL, NULL);
+    if (st)
+        return st;
+    const int nc = R->nc, ne = R->ne;
     /* MPFA.calculateTmatrix: AiB = A\B (dgesv on copies), T = C*AiB - D via
      * dgemm(alpha=1, beta=1) on -D */
     double Acp[144], AiB[96];
@@ -955,4 +988,153 @@ void or_connection_flux(int nconn, const int *md_ptr, const int *md_elem,
             f += md_t[q] * p[md_elem[q]];
         flux[c] = f;
     }
+}
+
+/* Net outflow of the listed cells: sum over each cell's connections in
+ * ascending connection order of +flux (here side) or -flux (there side). */
+void or_cell_rates(int nconn, const int *here, const int *there,
+                   const double *flux, int nsel, const int *cells,
+                   double *rates)
+{
+    for (int t = 0; t < nsel; ++t) {
+        double q = 0;
+        for (int c = 0; c < nconn; ++c) {
+            if (here[c] == cells[t])
+                q += flux[c];
+            if (there[c] == cells[t])
+                q += -flux[c];
+        }
+        rates[t] = q;
+    }
+}
+
+/* ---- Adjoint of the transmissibility map ------------------------------- */
+/* Given tbar (one value per MD entry, the gradient of an objective with
+ * respect to the stored transmissibilities), returns for every cell e
+ * mbar[e] = d objective / d m_e where the cell's tensor is K_e = exp(m_e)
+ * K0_e.  Per corner T = C A^-1 B - D (MPFA.calculateTmatrix) and every
+ * entry of A, B, C, D is linear in one element's tensor, so with
+ * Tbar the corner's share of tbar:
+ *   Bbar = A^-T C' Tbar,  Abar = -Bbar (A^-1 B)',  Cbar = Tbar (A^-1 B)',
+ *   Dbar = -Tbar,  mbar_e += <Abar,A_e> + <Bbar,B_e> + <Cbar,C_e> + <Dbar,D_e>,
+ * the inner products taken over element e's contributions in assembly
+ * order (each contribution of an element lands on its own entry).  This is
+ * not a function of the reference (rs has no gradients). */
+int or_mpfa_o_vjp(int dim, int nx, int ny, int nz, double dx, double dy,
+                  double dz, const double *K6, double continuity,
+                  const int *md_ptr, const int *md_elem, const double *tbar,
+                  double *mbar)
+{
+    mesh_t m;
+    int st = mesh_init(&m, dim, nx, ny, nz, dx, dy, dz);
+    if (st)
+        return st;
+    for (int e = 0; e < m.ncells; ++e)
+        mbar[e] = 0;
+    double A[144], B[96], C[144], D[96];
+    double *Ae = (double *)malloc(sizeof(double) * 8 * 144);
+    double *Ce = (double *)malloc(sizeof(double) * 8 * 144);
+    double *Be = (double *)malloc(sizeof(double) * 8 * 96);
+    double *De = (double *)malloc(sizeof(double) * 8 * 96);
+    region_t R;
+    for (int p = 0; p < m.npoints && !st; ++p) {
+        int pi, pj, pk;
+        point_ijk(&m, p, &pi, &pj, &pk);
+        st = corner_local(&m, K6, continuity, pi, pj, pk, &R, A, B, C, D, Ae, Be,
+                          Ce, De);
+        if (st)
+            break;
+        const int nc = R.nc, ne = R.ne;
+        double Tb[96], AiB[96], W[96], Bb[96], Ab[144], Cb[144], LU[144];
+        int piv[12];
+        memset(Tb, 0, sizeof(Tb));
+        for (int a = 0; a < R.nif; ++a) {
+            const rintf_t *I = &R.intf[a];
+            if (I->boundary || !I->is_here)
+                continue;
+            const int c = I->conn, i = I->lidx;
+            for (int le = 0; le < ne; ++le)
+                for (int q = md_ptr[c]; q < md_ptr[c + 1]; ++q)
+                    if (md_elem[q] == R.el[le].e) {
+                        Tb[i + le * nc] = tbar[q];
+                        break;
+                    }
+        }
+        /* LU of A (dgetf2) and AiB = A^-1 B */
+        memcpy(LU, A, sizeof(double) * nc * nc);
+        memcpy(AiB, B, sizeof(double) * nc * ne);
+        if (or_dgesv(nc, ne, LU, nc, piv, AiB, nc)) {
+            st = OR_SINGULAR;
+            break;
+        }
+        /* W = C' Tbar */
+        for (int j = 0; j < ne; ++j)
+            for (int l = 0; l < nc; ++l) {
+                double s = 0;
+                for (int i = 0; i < nc; ++i)
+                    s += C[i + l * nc] * Tb[i + j * nc];
+                W[l + j * nc] = s;
+            }
+        /* Bbar = A^-T W: U' y = W, L' z = y, then undo the row swaps */
+        memcpy(Bb, W, sizeof(double) * nc * ne);
+        for (int j = 0; j < ne; ++j) {
+            double *b = Bb + j * nc;
+            for (int i = 0; i < nc; ++i) {
+                double t = b[i];
+                for (int k = 0; k < i; ++k)
+                    t -= LU[k + i * nc] * b[k];
+                b[i] = t / LU[i + i * nc];
+            }
+            for (int i = nc - 1; i >= 0; --i) {
+                double t = b[i];
+                for (int k = i + 1; k < nc; ++k)
+                    t -= LU[k + i * nc] * b[k];
+                b[i] = t;
+            }
+            for (int i = nc - 1; i >= 0; --i)
+                if (piv[i] != i) {
+                    double t = b[i];
+                    b[i] = b[piv[i]];
+                    b[piv[i]] = t;
+                }
+        }
+        /* Abar = -Bbar AiB', Cbar = Tbar AiB' */
+        for (int l = 0; l < nc; ++l)
+            for (int i = 0; i < nc; ++i) {
+                double sa = 0, sc = 0;
+                for (int j = 0; j < ne; ++j) {
+                    sa += Bb[i + j * nc] * AiB[l + j * nc];
+                    sc += Tb[i + j * nc] * AiB[l + j * nc];
+                }
+                Ab[i + l * nc] = -sa;
+                Cb[i + l * nc] = sc;
+            }
+        /* element terms in assembly order (the contribution walk of
+         * corner_local: interface q of the element, then its basis jj) */
+        for (int le = 0; le < ne; ++le) {
+            double g = 0;
+            const int nI = R.neintf[le];
+            for (int q = 0; q < nI; ++q) {
+                const rintf_t *I = &R.intf[R.eintf[le][q]];
+                const int is = I->lidx;
+                const double bq = Be[le * 96 + is + le * nc];
+                g += Bb[is + le * nc] * bq;
+                if (I->is_here)
+                    g += (-Tb[is + le * nc]) * De[le * 96 + is + le * nc];
+                for (int jj = 1; jj < nI + 1; ++jj) {
+                    const int js = R.intf[R.eintf[le][jj - 1]].lidx;
+                    g += Ab[is + js * nc] * Ae[le * 144 + is + js * nc];
+                    if (I->is_here)
+                        g += Cb[is + js * nc] * Ce[le * 144 + is + js * nc];
+                }
+            }
+            mbar[R.el[le].e] += g;
+        }
+    }
+    free(Ae);
+    free(Ce);
+    free(Be);
+    free(De);
+    mesh_free(&m);
+    return st;
 }

@@ -205,6 +205,13 @@ int or_assemble_flux(int n, int nconn, const int *here, const int *there,
 /* PotentialCalculator.calculateFlux: F_c = sum_k t_k p_k. */
 void or_connection_flux(int nconn, const int *md_ptr, const int *md_elem,
                         const double *md_t, const double *p, double *flux);
+int or_mpfa_o_vjp(int dim, int nx, int ny, int nz, double dx, double dy,
+                  double dz, const double *K6, double continuity,
+                  const int *md_ptr, const int *md_elem, const double *tbar,
+                  double *mbar);
+void or_cell_rates(int nconn, const int *here, const int *there,
+                   const double *flux, int nsel, const int *cells,
+                   double *rates);
 
 #ifdef __cplusplus
 }

@@ -131,6 +131,10 @@ def lib():
         L.or_assemble_flux.argtypes = [C.c_int, C.c_int, _i32p, _i32p, _i32p, _i32p, _f64p,
                                        _i32p, _i32p, _f64p]
         L.or_connection_flux.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p]
+        L.or_cell_rates.argtypes = [C.c_int, _i32p, _i32p, _f64p, C.c_int, _i32p, _f64p]
+        L.or_mpfa_o_vjp.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                    C.c_double, _f64p, C.c_double, _i32p, _i32p, _f64p, _f64p]
+        L.or_cell_rates.restype = None
         _lib = L
     return _lib
 
@@ -483,6 +487,25 @@ def assemble_flux(n, here, there, md_ptr, md_elem, md_t, rowptr, colind):
     if st:
         raise RuntimeError(f"assembly failed with status {st}")
     return data
+
+
+def mpfa_o_vjp(dim, nx, ny, nz, dx, dy, dz, K6, md_ptr, md_elem, tbar, continuity=1.0):
+    """d(tbar . t)/d m for K_e = exp(m_e) K0_e, per cell."""
+    n = nx * ny * (nz if dim == 3 else 1)
+    out = np.zeros(max(n, 1))
+    st = lib().or_mpfa_o_vjp(dim, nx, ny, nz if dim == 3 else 0, dx, dy, dz,
+                             _f64(np.ascontiguousarray(K6).reshape(-1)), continuity,
+                             _i32(md_ptr), _i32(md_elem), _f64(tbar), out)
+    if st:
+        raise RuntimeError(f"or_mpfa_o_vjp status {st}")
+    return out[:n]
+
+
+def cell_rates(here, there, flux, cells):
+    cells = _i32(cells)
+    r = np.zeros(max(len(cells), 1))
+    lib().or_cell_rates(len(here), _i32(here), _i32(there), _f64(flux), len(cells), cells, r)
+    return r[: len(cells)]
 
 
 def connection_flux(md_ptr, md_elem, md_t, p):

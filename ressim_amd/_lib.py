@@ -26,7 +26,8 @@ class SolverOpts(C.Structure):
     _fields_ = [("kind", C.c_int), ("restart", C.c_int), ("max_iter", C.c_int),
                 ("rtol", C.c_double), ("atol", C.c_double), ("dtol", C.c_double),
                 ("mixed", C.c_int), ("inner_max_iter", C.c_int),
-                ("inner_rtol", C.c_double), ("check_every", C.c_int)]
+                ("inner_rtol", C.c_double), ("check_every", C.c_int),
+                ("prec_transpose", C.c_int)]
 
 
 class AmgParams(C.Structure):
@@ -87,6 +88,7 @@ SIGNATURES = {
     "rsgpu_prec_get_factors": (C.c_int, [_h, _ip, _ip, _dp, C.POINTER(C.c_int)]),
     "rsgpu_prec_info": (C.c_int, [_h, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                   C.POINTER(C.c_int)]),
+    "rsgpu_csr_transpose": (C.c_int, [_h, _hp]),
     "rsgpu_amg_params_default": (None, [C.POINTER(AmgParams)]),
     "rsgpu_prec_set_amg_params": (C.c_int, [_h, C.POINTER(AmgParams)]),
     "rsgpu_amg_levels": (C.c_int, [_h, _ip, _ip, C.c_int]),
@@ -104,6 +106,18 @@ SIGNATURES = {
     "rsgpu_mpfa_o_assemble": (C.c_int, [_h, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                         C.c_double, C.c_double, _dp, C.c_double, _hp,
                                         C.POINTER(C.c_int)]),
+    "rsgpu_flux_create": (C.c_int, [_h, C.c_int, C.c_int, _ip, _ip, _dp, _ip, _ip, _hp]),
+    "rsgpu_mpfa_o_flux": (C.c_int, [_h, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                    C.c_double, C.c_double, _dp, C.c_double, _hp,
+                                    C.POINTER(C.c_int)]),
+    "rsgpu_flux_destroy": (C.c_int, [_h]),
+    "rsgpu_flux_info": (C.c_int, [_h, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                  C.POINTER(C.c_int)]),
+    "rsgpu_flux_apply": (C.c_int, [_h, _dp, _dp]),
+    "rsgpu_flux_rates": (C.c_int, [_h, _dp, C.c_int, _ip, _dp]),
+    "rsgpu_flux_vjp": (C.c_int, [_h, _dp, _dp, _dp]),
+    "rsgpu_mpfa_o_vjp": (C.c_int, [_h, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                   C.c_double, C.c_double, _dp, C.c_double, _h, _dp, _dp]),
 }
 
 _lib = None

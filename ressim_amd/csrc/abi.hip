@@ -123,6 +123,56 @@ extern "C" int rsgpu_csr_create(rsgpu_ctx *ctx, int n, int ncols, int nnz,
     RS_CATCH(ctx)
 }
 
+extern "C" int rsgpu_csr_transpose(const rsgpu_csr *A, rsgpu_csr **out)
+{
+    if (!A || !out)
+        return RSGPU_BAD_ARGUMENT;
+    rsgpu_csr *T = new rsgpu_csr();
+    RS_TRY(A->ctx)
+    RS_HIP(hipSetDevice(A->ctx->device));
+    hipStream_t s = A->ctx->stream;
+    const int n = A->n, m = A->ncols, nnz = A->nnz;
+    std::vector<int> trp(m + 1, 0), tci(std::max(nnz, 0)), tpm(std::max(nnz, 1), 0);
+    for (int j = 0; j < nnz; ++j)
+        trp[A->h_colind[j] + 1]++;
+    for (int c = 0; c < m; ++c)
+        trp[c + 1] += trp[c];
+    std::vector<int> fill(trp.begin(), trp.end() - 1);
+    for (int i = 0; i < n; ++i)
+        for (int j = A->h_rowptr[i]; j < A->h_rowptr[i + 1]; ++j) {
+            const int p = fill[A->h_colind[j]]++;
+            tci[p] = i;
+            tpm[p] = j;
+        }
+    T->ctx = A->ctx;
+    T->n = m;
+    T->ncols = n;
+    T->nnz = nnz;
+    T->h_rowptr = trp;
+    T->h_colind = tci;
+    T->rowptr.upload(trp, s);
+    T->colind.upload(tci, s);
+    T->val.alloc(std::max(nnz, 1));
+    DevBuf<int> perm;
+    perm.upload(tpm, s);
+    csr_launch_gather(s, perm.p, A->val.p, T->val.p, nnz);
+    spmv_build_plan(T->h_rowptr, m, T->plan, s);
+    T->pattern_id = next_pattern_id();
+    T->transpose_of = A->pattern_id;
+    RS_HIP(hipStreamSynchronize(s));
+    *out = T;
+    return RSGPU_OK;
+    }
+    catch (const RsError &e) {
+        delete T;
+        return fail(A->ctx, e);
+    }
+    catch (const std::bad_alloc &) {
+        delete T;
+        return fail(A->ctx, RsError{RSGPU_OUT_OF_MEMORY, "host out of memory"});
+    }
+}
+
 extern "C" int rsgpu_csr_destroy(rsgpu_csr *A)
 {
     if (A) {

@@ -838,17 +838,19 @@ void prec_apply(rsgpu_prec *M, const float *b, float *x, const DevState *st)
                                b, x, M->scratch32.p, sk1, sk2);
 }
 
-void prec_trans_apply(rsgpu_prec *M, const double *b, double *x)
+void prec_trans_apply(rsgpu_prec *M, const double *b, double *x, const DevState *st)
 {
     if (M->kind == RSGPU_PREC_AMG) {
-        amg_apply(M, b, x, nullptr, true);
+        amg_apply(M, b, x, st, true);
         return;
     }
     hipStream_t s = M->ctx->stream;
     if (M->kind == RSGPU_PREC_IDENTITY) {
-        prec_apply(M, b, x, (const DevState *)nullptr);
+        prec_apply(M, b, x, st);
         return;
     }
+    const int *sk1 = st ? &st->done : nullptr;
+    const int *sk2 = st ? &st->cycle_end : nullptr;
     if (M->precision != RSGPU_FP64)
         throw RsError{RSGPU_BAD_ARGUMENT, "transApply needs an FP64 preconditioner"};
     if (!M->has_t) {
@@ -858,5 +860,5 @@ void prec_trans_apply(rsgpu_prec *M, const double *b, double *x)
         launch_rowdiag<double>(s, M->Ut, M->diag.p, M->lu64.p, M->dgUt64.p);
     }
     launch_apply<double, true>(M, M->Ut, M->ell64_Ut.p, M->dgUt64.p, M->Lt, M->ell64_Lt.p,
-                               nullptr, b, x, M->scratch64.p, nullptr, nullptr);
+                               nullptr, b, x, M->scratch64.p, sk1, sk2);
 }

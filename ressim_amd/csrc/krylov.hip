@@ -700,6 +700,26 @@ void launch_resid(rsgpu_solver *S, rsgpu_csr *A, const T *b, const T *x, T *r,
 }
 
 // ------------------------------------------------------------ BiCGstab
+// M.apply, or M.transApply when the solve is the transposed one
+// (rsgpu_solver_opts.prec_transpose: M is the preconditioner of A, the
+// matrix is A')
+static void papply(rsgpu_solver *S, rsgpu_prec *M, const double *b, double *x,
+                   const DevState *st)
+{
+    if (S->o.prec_transpose)
+        prec_trans_apply(M, b, x, st);
+    else
+        prec_apply(M, b, x, st);
+}
+
+static void papply(rsgpu_solver *S, rsgpu_prec *M, const float *b, float *x,
+                   const DevState *st)
+{
+    if (S->o.prec_transpose)
+        throw RsError{RSGPU_BAD_ARGUMENT, "the transposed solve runs in FP64"};
+    prec_apply(M, b, x, st);
+}
+
 // work: p s phat shat t v r rtilde (8 n)
 template <typename T>
 void run_bicgstab(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const T *b,
@@ -721,14 +741,14 @@ void run_bicgstab(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const T *b,
                            st, sl);
         hipLaunchKernelGGL(k_bicg_p<T>, dim3(E), dim3(RS_VBS), 0, s, n, r, v, p,
                            st);
-        prec_apply(M, p, phat, st);
+        papply(S, M, p, phat, st);
         launch_spmv_red<T>(S, A, phat, EpiMultDot<T>{v, rt, nullptr, 1}, st,
                            FIN_ALPHA, 0);
         hipLaunchKernelGGL(k_bicg_s<T>, dim3(G), dim3(RS_VBS), 0, s, n, r, v, sv,
                            st, sl);
         hipLaunchKernelGGL(k_bicg_half<T>, dim3(G), dim3(RS_VBS), 0, s, n, phat,
                            x, st, sl);
-        prec_apply(M, sv, shat, st);
+        papply(S, M, sv, shat, st);
         launch_spmv_red<T>(S, A, shat, EpiMultDot<T>{t, sv, nullptr, 2}, st,
                            FIN_OMEGA, 0);
         hipLaunchKernelGGL(k_bicg_update<T>, dim3(G), dim3(RS_VBS), 0, s, n,
@@ -756,7 +776,7 @@ void run_gmres(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const T *b, T *x,
     RedSlot sl = slot_of(S);
     // u = b - A x; r = M u; normr; outer monitor at iter 0
     launch_resid<T>(S, A, b, x, u, nullptr, nullptr, st, FIN_NONE);
-    prec_apply(M, u, r, st);
+    papply(S, M, u, r, st);
     hipLaunchKernelGGL(k_gm_outer<T>, dim3(G), dim3(RS_VBS), 0, s, n, r, st, sl,
                        1);
     Poller P{S, st};
@@ -770,7 +790,7 @@ void run_gmres(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const T *b, T *x,
             hipLaunchKernelGGL(k_spmv_skip<T>, dim3(A->plan.nwg), dim3(RS_SPMV_BS), 0, s,
                                A->plan.wg_rows.p, A->rowptr.p, A->colind.p,
                                csr_values<T>(A), vi, u, st, 1);
-            prec_apply(M, u, w, st);
+            papply(S, M, u, w, st);
             for (int k = 0; k <= i; ++k)
                 hipLaunchKernelGGL(k_gm_mgs<T>, dim3(G), dim3(RS_VBS), 0, s, n, i,
                                    k, w, k > 0 ? V + (size_t)(k - 1) * L : V,
@@ -792,7 +812,7 @@ void run_gmres(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const T *b, T *x,
         launch_resid<T>(S, A, b, x, u, nullptr, nullptr, st, FIN_NONE);
         // r = M u belongs to the outer loop: clear the inner-loop flag first
         hipLaunchKernelGGL(k_clear_cycle, dim3(1), dim3(64), 0, s, st);
-        prec_apply(M, u, r, st);
+        papply(S, M, u, r, st);
         hipLaunchKernelGGL(k_gm_outer<T>, dim3(G), dim3(RS_VBS), 0, s, n, r, st,
                            sl, 0);
         RS_CHECK_LAUNCH();
@@ -826,6 +846,7 @@ extern "C" void rsgpu_solver_opts_default(rsgpu_solver_opts *o)
     o->inner_max_iter = 200;
     o->inner_rtol = 1e-4;
     o->check_every = 4;
+    o->prec_transpose = 0;
 }
 
 extern "C" int rsgpu_solver_create(rsgpu_ctx *ctx, const rsgpu_solver_opts *opts,
@@ -935,7 +956,7 @@ void run_ir(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const double *b,
     int done = 0, cend = 0;
     launch_resid<double>(S, A, b, x, r, nullptr, nullptr, st, FIN_INIT_NORM);
     for (;;) {
-        prec_apply(M, r, z, st);
+        papply(S, M, r, z, st);
         hipLaunchKernelGGL(k_ir_add<double>, dim3(E), dim3(RS_VBS), 0, s, n, z, x,
                            st);
         hipLaunchKernelGGL(k_ir_next, dim3(1), dim3(64), 0, s, st);
@@ -1050,8 +1071,16 @@ extern "C" int rsgpu_solver_solve(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M,
     ident.ready = true;
     if (!M)
         M = &ident;
-    if (M->kind != RSGPU_PREC_IDENTITY && (!M->ready || M->pattern_id != A->pattern_id)) {
-        ctx->err = "Preconditioner.setMatrix must be called before use";
+    const uint64_t want = S->o.prec_transpose ? A->transpose_of : A->pattern_id;
+    if (M->kind != RSGPU_PREC_IDENTITY && (!M->ready || M->pattern_id != want)) {
+        ctx->err = S->o.prec_transpose
+                       ? "transposed solve: the preconditioner must be set on the matrix "
+                         "whose transpose is solved"
+                       : "Preconditioner.setMatrix must be called before use";
+        return RSGPU_BAD_ARGUMENT;
+    }
+    if (S->o.prec_transpose && S->o.mixed) {
+        ctx->err = "the transposed solve runs in FP64";
         return RSGPU_BAD_ARGUMENT;
     }
     try {

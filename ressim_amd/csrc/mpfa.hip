@@ -312,6 +312,70 @@ __device__ int dgesv(int n, int nrhs, double *a, int lda, double *b, int ldb)
     return 0;
 }
 
+// dgesv keeping the pivots (the adjoint solves with A')
+__device__ int dgesv_piv(int n, int nrhs, double *a, int lda, double *b, int ldb, int *ipiv)
+{
+    int info = 0;
+    for (int j = 0; j < n; ++j) {
+        int jp = j;
+        double amax = fabs(a[j + j * lda]);
+        for (int i = j + 1; i < n; ++i) {
+            const double v = fabs(a[i + j * lda]);
+            if (v > amax) {
+                amax = v;
+                jp = i;
+            }
+        }
+        ipiv[j] = jp;
+        if (a[jp + j * lda] != 0) {
+            if (jp != j)
+                for (int c = 0; c < n; ++c) {
+                    const double t = a[j + c * lda];
+                    a[j + c * lda] = a[jp + c * lda];
+                    a[jp + c * lda] = t;
+                }
+            if (j < n - 1) {
+                const double r = 1.0 / a[j + j * lda];
+                for (int i = j + 1; i < n; ++i)
+                    a[i + j * lda] *= r;
+            }
+        } else if (info == 0) {
+            info = j + 1;
+        }
+        if (j < n - 1)
+            for (int c = j + 1; c < n; ++c) {
+                const double yj = a[j + c * lda];
+                if (yj != 0) {
+                    const double temp = -1.0 * yj;
+                    for (int i = j + 1; i < n; ++i)
+                        a[i + c * lda] = a[i + c * lda] + a[i + j * lda] * temp;
+                }
+            }
+    }
+    if (info)
+        return 1;
+    for (int j = 0; j < n; ++j)
+        if (ipiv[j] != j)
+            for (int c = 0; c < nrhs; ++c) {
+                const double t = b[j + c * ldb];
+                b[j + c * ldb] = b[ipiv[j] + c * ldb];
+                b[ipiv[j] + c * ldb] = t;
+            }
+    for (int c = 0; c < nrhs; ++c)
+        for (int k = 0; k < n; ++k)
+            if (b[k + c * ldb] != 0)
+                for (int i = k + 1; i < n; ++i)
+                    b[i + c * ldb] = b[i + c * ldb] - b[k + c * ldb] * a[i + k * lda];
+    for (int c = 0; c < nrhs; ++c)
+        for (int k = n - 1; k >= 0; --k)
+            if (b[k + c * ldb] != 0) {
+                b[k + c * ldb] = b[k + c * ldb] / a[k + k * lda];
+                for (int i = 0; i < k; ++i)
+                    b[i + c * ldb] = b[i + c * ldb] - b[k + c * ldb] * a[i + k * lda];
+            }
+    return 0;
+}
+
 struct RIntf {
     signed char ei, ej, ek, f;   // owning element offsets relative to corner - 1
     signed char elocal, boundary, is_here, lidx;
@@ -519,6 +583,268 @@ __global__ void __launch_bounds__(64)
         }
         st_n[(size_t)c * MPFA_SLOTS + slot] = (signed char)ne;
     }
+}
+
+__global__ void __launch_bounds__(64)
+    k_corner_vjp(Grid g, const double *__restrict__ K6, double continuity,
+                 const int *__restrict__ md_ptr, const int *__restrict__ md_elem,
+                 const double *__restrict__ tbar, double *__restrict__ st_m,
+                 int *__restrict__ err)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.npoints)
+        return;
+    const int npx = g.nx + 1, npy = g.ny + 1;
+    const int pi = p % npx, pj = (p / npx) % npy, pk = g.dim == 3 ? p / (npx * npy) : 0;
+    // ---- region: elements i-major, interfaces in orientation order
+    int ne = 0, nif = 0;
+    int eidx[8], ei_[8], ej_[8], ek_[8];
+    RIntf I[24];
+    int eintf[8][3], neintf[8];
+    const int kmin = g.dim == 3 ? pk - 1 : 0, kmax = g.dim == 3 ? pk : 0;
+    for (int i = pi - 1; i <= pi; ++i)
+        for (int j = pj - 1; j <= pj; ++j)
+            for (int k = kmin; k <= kmax; ++k) {
+                if (!elem_valid(g, i, j, k))
+                    continue;
+                const int le = ne++;
+                eidx[le] = elem_index(g, i, j, k);
+                ei_[le] = i;
+                ej_[le] = j;
+                ek_[le] = k;
+                neintf[le] = 0;
+                for (int f = 0; f < g.nf; ++f) {
+                    bool on;
+                    if (g.dim == 3) {
+                        switch (f) {
+                        case 0: on = (pk == k); break;
+                        case 1: on = (pk == k + 1); break;
+                        case 2: on = (pj == j); break;
+                        case 3: on = (pj == j + 1); break;
+                        case 4: on = (pi == i); break;
+                        default: on = (pi == i + 1); break;
+                        }
+                    } else {
+                        switch (f) {
+                        case 0: on = (pj == j); break;
+                        case 1: on = (pj == j + 1); break;
+                        case 2: on = (pi == i); break;
+                        default: on = (pi == i + 1); break;
+                        }
+                    }
+                    if (!on)
+                        continue;
+                    RIntf &R = I[nif];
+                    R.ei = (signed char)(i - pi + 1);
+                    R.ej = (signed char)(j - pj + 1);
+                    R.ek = (signed char)(k - pk + 1);
+                    R.f = (signed char)f;
+                    R.elocal = (signed char)le;
+                    int ih = 0;
+                    R.conn = face_connection(g, i, j, k, f, ih);
+                    R.boundary = R.conn < 0;
+                    R.is_here = R.boundary ? 1 : (signed char)ih;
+                    eintf[le][neintf[le]++] = nif;
+                    nif++;
+                }
+            }
+    int nc = 0;
+    for (int a = 0; a < nif; ++a) {
+        int found = -1;
+        if (!I[a].boundary)
+            for (int b = 0; b < a; ++b)
+                if (!I[b].boundary && I[b].conn == I[a].conn) {
+                    found = I[b].lidx;
+                    break;
+                }
+        I[a].lidx = (signed char)(found >= 0 ? found : nc++);
+    }
+    // ---- local matrices (column-major, ld = nc)
+    double A[144], B[96], C[144], D[96];
+    for (int q = 0; q < nc * nc; ++q) {
+        A[q] = 0;
+        C[q] = 0;
+    }
+    for (int q = 0; q < nc * ne; ++q) {
+        B[q] = 0;
+        D[q] = 0;
+    }
+    const v3 corner = point_at(g, pi, pj, pk);
+    double fst[8][3][4];   // each element's contributions f(q, jj), jj = 0 for B/D
+    for (int le = 0; le < ne; ++le) {
+        const int nI = neintf[le], nb = nI + 1;
+        double Am[16], Psi[16];
+        const v3 elp = elem_center(g, ei_[le], ej_[le], ek_[le]);
+        for (int q = 0; q < 16; ++q) {
+            Am[q] = 0;
+            Psi[q] = 0;
+        }
+        Am[0] = 1;
+        Am[0 + 1 * nb] = elp.x;
+        Am[0 + 2 * nb] = elp.y;
+        if (g.dim == 3)
+            Am[0 + 3 * nb] = elp.z;
+        for (int q = 0; q < nI; ++q) {
+            const RIntf &R = I[eintf[le][q]];
+            v3 cc, nn;
+            double ar;
+            face_geom(g, pi - 1 + R.ei, pj - 1 + R.ej, pk - 1 + R.ek, R.f, cc, nn, ar);
+            v3 cp = cc;
+            if (continuity != 1)
+                cp = mk((cc.x - corner.x) * continuity + corner.x,
+                        (cc.y - corner.y) * continuity + corner.y,
+                        (cc.z - corner.z) * continuity + corner.z);
+            Am[q + 1] = 1;
+            Am[q + 1 + 1 * nb] = cp.x;
+            Am[q + 1 + 2 * nb] = cp.y;
+            if (g.dim == 3)
+                Am[q + 1 + 3 * nb] = cp.z;
+        }
+        for (int q = 0; q < nb; ++q)
+            Psi[q + q * nb] = 1;
+        if (dgesv(nb, nb, Am, nb, Psi, nb)) {
+            atomicMax(err, 1);
+            return;
+        }
+        v3 grad[4];
+        for (int q = 0; q < nb; ++q)
+            grad[q] = mk(Psi[1 + q * nb], Psi[2 + q * nb], g.dim == 3 ? Psi[3 + q * nb] : 0);
+        v3 Kn[3];
+        for (int q = 0; q < nI; ++q) {
+            const RIntf &R = I[eintf[le][q]];
+            Kn[q] = tensor_mult(K6 + 6 * (size_t)eidx[le],
+                                normal_area(g, pi - 1 + R.ei, pj - 1 + R.ej, pk - 1 + R.ek,
+                                            R.f, pi, pj, pk));
+        }
+        for (int q = 0; q < nI; ++q) {
+            const RIntf &R = I[eintf[le][q]];
+            double f = vdot(Kn[q], grad[0]);
+            const int is = R.lidx, ei = R.elocal;
+            B[is + ei * nc] += -f;
+            if (R.is_here)
+                D[is + ei * nc] += -f;
+            fst[le][q][0] = f;
+            for (int jj = 1; jj < nI + 1; ++jj) {
+                f = vdot(Kn[q], grad[jj]);
+                const int js = I[eintf[le][jj - 1]].lidx;
+                A[is + js * nc] += f;
+                if (R.is_here)
+                    C[is + js * nc] += f;
+                fst[le][q][jj] = f;
+            }
+        }
+    }
+    // Tbar: this corner's rows of the connection lists' gradients
+    double Tb[96];
+    for (int q = 0; q < nc * ne; ++q)
+        Tb[q] = 0;
+    for (int a = 0; a < nif; ++a) {
+        const RIntf &R = I[a];
+        if (R.boundary || !R.is_here)
+            continue;
+        const int c = R.conn, i = R.lidx;
+        for (int le = 0; le < ne; ++le)
+            for (int q = md_ptr[c]; q < md_ptr[c + 1]; ++q)
+                if (md_elem[q] == eidx[le]) {
+                    Tb[i + le * nc] = tbar[q];
+                    break;
+                }
+    }
+    // LU of A with pivots and AiB = A^-1 B (dgetf2 + dgetrs, as dgesv)
+    int piv[12];
+    if (dgesv_piv(nc, ne, A, nc, B, nc, piv)) {
+        atomicMax(err, 2);
+        return;
+    }
+    // W = C' Tbar, then Bbar = A^-T W (in W)
+    double W[96];
+    for (int j = 0; j < ne; ++j)
+        for (int l = 0; l < nc; ++l) {
+            double sacc = 0;
+            for (int i = 0; i < nc; ++i)
+                sacc += C[i + l * nc] * Tb[i + j * nc];
+            W[l + j * nc] = sacc;
+        }
+    for (int j = 0; j < ne; ++j) {
+        double *b = W + j * nc;
+        for (int i = 0; i < nc; ++i) {
+            double t = b[i];
+            for (int k = 0; k < i; ++k)
+                t -= A[k + i * nc] * b[k];
+            b[i] = t / A[i + i * nc];
+        }
+        for (int i = nc - 1; i >= 0; --i) {
+            double t = b[i];
+            for (int k = i + 1; k < nc; ++k)
+                t -= A[k + i * nc] * b[k];
+            b[i] = t;
+        }
+        for (int i = nc - 1; i >= 0; --i)
+            if (piv[i] != i) {
+                const double t = b[i];
+                b[i] = b[piv[i]];
+                b[piv[i]] = t;
+            }
+    }
+    // element terms in assembly order; Abar(i,l) = -sum_j Bbar(i,j) AiB(l,j),
+    // Cbar(i,l) = sum_j Tbar(i,j) AiB(l,j)
+    for (int le = 0; le < ne; ++le) {
+        double gsum = 0;
+        const int nI = neintf[le];
+        for (int q = 0; q < nI; ++q) {
+            const RIntf &R = I[eintf[le][q]];
+            const int is = R.lidx;
+            const double bq = -fst[le][q][0];
+            gsum += W[is + le * nc] * bq;
+            if (R.is_here)
+                gsum += (-Tb[is + le * nc]) * bq;
+            for (int jj = 1; jj < nI + 1; ++jj) {
+                const int js = I[eintf[le][jj - 1]].lidx;
+                double sa = 0, sc = 0;
+                for (int j = 0; j < ne; ++j) {
+                    sa += W[is + j * nc] * B[js + j * nc];
+                    sc += Tb[is + j * nc] * B[js + j * nc];
+                }
+                gsum += (-sa) * fst[le][q][jj];
+                if (R.is_here)
+                    gsum += sc * fst[le][q][jj];
+            }
+        }
+        st_m[(size_t)p * 8 + le] = gsum;
+    }
+}
+
+// mbar of a cell: its corners' terms in ascending point order
+__global__ void k_vjp_gather(Grid g, const double *__restrict__ st_m, double *__restrict__ mbar)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= g.ncells)
+        return;
+    const int i = e % g.nx, j = (e / g.nx) % g.ny, k = g.dim == 3 ? e / (g.nx * g.ny) : 0;
+    const int npx = g.nx + 1, npy = g.ny + 1;
+    double acc = 0;
+    const int kt = g.dim == 3 ? 1 : 0;
+    for (int pk = k; pk <= k + kt; ++pk)
+        for (int pj = j; pj <= j + 1; ++pj)
+            for (int pi = i; pi <= i + 1; ++pi) {
+                // local index of e among the corner's elements (i-major)
+                int le = 0, me = -1;
+                const int kmin = g.dim == 3 ? pk - 1 : 0, kmax = g.dim == 3 ? pk : 0;
+                for (int a = pi - 1; a <= pi && me < 0; ++a)
+                    for (int b = pj - 1; b <= pj && me < 0; ++b)
+                        for (int c = kmin; c <= kmax; ++c) {
+                            if (!elem_valid(g, a, b, c))
+                                continue;
+                            if (a == i && b == j && c == k) {
+                                me = le;
+                                break;
+                            }
+                            ++le;
+                        }
+                const int p = pi + pj * npx + (g.dim == 3 ? pk * npx * npy : 0);
+                acc += st_m[(size_t)p * 8 + me];
+            }
+    mbar[e] = acc;
 }
 
 // storeTransmissibilities replayed per connection over its corners in point
@@ -833,6 +1159,110 @@ extern "C" int rsgpu_mpfa_o(rsgpu_ctx *ctx, int dim, int nx, int ny, int nz, dou
         }
         if (check_fail)
             *check_fail = R.fails;
+        return RSGPU_OK;
+    } catch (const RsError &e) {
+        return fail(ctx, e);
+    }
+}
+
+rsgpu_flux *flux_from_device(rsgpu_ctx *ctx, int nconn, int ncells, const std::vector<int> &mp,
+                             DevBuf<int> &md_elem, DevBuf<double> &md_t,
+                             const std::vector<int> &here, const std::vector<int> &there);
+
+extern "C" int rsgpu_mpfa_o_flux(rsgpu_ctx *ctx, int dim, int nx, int ny, int nz, double dx,
+                                 double dy, double dz, const double *K6, double continuity,
+                                 rsgpu_flux **F_out, int *check_fail)
+{
+    if (!ctx || !K6 || !F_out)
+        return RSGPU_BAD_ARGUMENT;
+    try {
+        RS_HIP(hipSetDevice(ctx->device));
+        MpfaResult R;
+        mpfa_run(ctx, dim, nx, ny, nz, dx, dy, dz, K6, continuity, R);
+        std::vector<int> here(R.nconn), there(R.nconn);
+        if (R.nconn) {
+            RS_HIP(hipMemcpy(here.data(), R.d_here.p, sizeof(int) * R.nconn,
+                             hipMemcpyDeviceToHost));
+            RS_HIP(hipMemcpy(there.data(), R.d_there.p, sizeof(int) * R.nconn,
+                             hipMemcpyDeviceToHost));
+        }
+        *F_out = flux_from_device(ctx, R.nconn, R.ncells, R.md_ptr, R.d_md_elem, R.d_md_t, here,
+                                  there);
+        if (check_fail)
+            *check_fail = R.fails;
+        return RSGPU_OK;
+    } catch (const RsError &e) {
+        return fail(ctx, e);
+    }
+}
+
+// adjoint of the transmissibility map (no reference counterpart)
+int flux_md_lists(const rsgpu_flux *F, const int **md_ptr, const int **md_elem, int *nconn,
+                  int *ncells);
+
+extern "C" int rsgpu_mpfa_o_vjp(rsgpu_ctx *ctx, int dim, int nx, int ny, int nz, double dx,
+                                double dy, double dz, const double *K6, double continuity,
+                                const rsgpu_flux *F, const double *tbar_dev, double *mbar_dev)
+{
+    if (!ctx || !K6 || !F || !tbar_dev || !mbar_dev)
+        return RSGPU_BAD_ARGUMENT;
+    try {
+        RS_HIP(hipSetDevice(ctx->device));
+        if (dim != 2 && dim != 3)
+            throw RsError{RSGPU_BAD_ARGUMENT, "dim must be 2 or 3"};
+        if (nx < 1 || ny < 1 || (dim == 3 && nz < 1))
+            throw RsError{RSGPU_BAD_ARGUMENT, "grid sizes must be positive"};
+        if (!(continuity > 0 && continuity <= 1))
+            throw RsError{RSGPU_BAD_ARGUMENT, "continuity must be in (0,1]"};
+        hipStream_t s = ctx->stream;
+        Grid g{};
+        g.dim = dim;
+        g.nx = nx;
+        g.ny = ny;
+        g.nz = dim == 3 ? nz : 1;
+        g.nf = dim == 3 ? 6 : 4;
+        g.ncells = nx * ny * g.nz;
+        g.nconn = dim == 3 ? 3 * nx * ny * nz - nx * ny - nx * nz - ny * nz
+                           : 2 * nx * ny - nx - ny;
+        g.npoints = dim == 3 ? (nx + 1) * (ny + 1) * (nz + 1) : (nx + 1) * (ny + 1);
+        const int *mdp, *mde;
+        int fnc, fcells;
+        flux_md_lists(F, &mdp, &mde, &fnc, &fcells);
+        if (fnc != g.nconn || fcells != g.ncells)
+            throw RsError{RSGPU_BAD_ARGUMENT, "flux operator does not match the grid"};
+        std::vector<double> xs(nx + 1), ys(ny + 1), zs(g.nz + 1);
+        double o = 0;
+        xs[0] = 0;
+        for (int i = 0; i < nx; ++i) xs[i + 1] = (o += dx);
+        o = 0;
+        ys[0] = 0;
+        for (int i = 0; i < ny; ++i) ys[i + 1] = (o += dy);
+        o = 0;
+        zs[0] = 0;
+        if (dim == 3)
+            for (int i = 0; i < nz; ++i) zs[i + 1] = (o += dz);
+        DevBuf<double> dxs, dys, dzs, K, st_m;
+        DevBuf<int> err;
+        dxs.upload(xs, s);
+        dys.upload(ys, s);
+        dzs.upload(zs, s);
+        g.xs = dxs.p;
+        g.ys = dys.p;
+        g.zs = dzs.p;
+        K.upload(K6, (size_t)g.ncells * 6, s);
+        st_m.alloc((size_t)g.npoints * 8);
+        err.alloc(1);
+        err.zero(s);
+        hipLaunchKernelGGL(k_corner_vjp, dim3((g.npoints + 63) / 64), dim3(64), 0, s, g, K.p,
+                           continuity, mdp, mde, tbar_dev, st_m.p, err.p);
+        hipLaunchKernelGGL(k_vjp_gather, dim3((g.ncells + 255) / 256), dim3(256), 0, s, g,
+                           st_m.p, mbar_dev);
+        RS_CHECK_LAUNCH();
+        int herr = 0;
+        RS_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        RS_HIP(hipStreamSynchronize(s));
+        if (herr)
+            throw RsError{RSGPU_SINGULAR, "singular MPFA-O local system"};
         return RSGPU_OK;
     } catch (const RsError &e) {
         return fail(ctx, e);

@@ -106,6 +106,8 @@ struct rsgpu_csr {
     DevBuf<int> trowptr, tcolind, tperm;
     SpmvPlan tplan;
     DevBuf<double> tval;
+    // pattern id of the matrix this one is the transpose of (rsgpu_csr_transpose)
+    uint64_t transpose_of = 0;
 };
 
 // Level-scheduled sweep data of one triangular factor (see ilu.hip)
@@ -379,7 +381,8 @@ void prec_setup(rsgpu_prec *M, rsgpu_csr *A);
 int prec_factor(rsgpu_prec *M, rsgpu_csr *A, int *bad_row);
 void prec_apply(rsgpu_prec *M, const double *b, double *x, const DevState *st);
 void prec_apply(rsgpu_prec *M, const float *b, float *x, const DevState *st);
-void prec_trans_apply(rsgpu_prec *M, const double *b, double *x);
+void prec_trans_apply(rsgpu_prec *M, const double *b, double *x,
+                      const DevState *st = nullptr);
 
 // amg.hip
 int amg_setup(rsgpu_prec *M, rsgpu_csr *A, int *bad_row);

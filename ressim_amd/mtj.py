@@ -242,6 +242,12 @@ class CompRowMatrix:
         _raise(L.load().rsgpu_csr_add_batch(self.h, r.size, _hptr(r), _hptr(c), _hptr(v)),
                self.ctx.h)
 
+    def transpose(self) -> "CompRowMatrix":
+        """A new matrix holding A' (values copied now)."""
+        h = C.c_void_p()
+        _raise(L.load().rsgpu_csr_transpose(self.h, C.byref(h)), self.ctx.h)
+        return CompRowMatrix._wrap(h, self.m, self.n, self.nnz, self.ctx)
+
     def sync_f32(self):
         _raise(L.load().rsgpu_csr_sync_f32(self.h), self.ctx.h)
 
@@ -493,6 +499,9 @@ class AbstractIterativeSolver:
         self.inner_rtol = 1e-4
         self.inner_max_iter = 200
         self.check_every = 4
+        # transposed solve: A' x = b with the preconditioner of A applied by
+        # transApply (the adjoint's solve)
+        self.prec_transpose = False
         self._h = None
         self._key = None
         self.last_stats = None
@@ -512,7 +521,7 @@ class AbstractIterativeSolver:
     def _handle(self, ctx: Context):
         key = (self.kind, self.restart, self.iter.maxIter, self.iter.rtol, self.iter.atol,
                self.iter.dtol, self.mixed, self.inner_rtol, self.inner_max_iter,
-               self.check_every, id(ctx))
+               self.check_every, self.prec_transpose, id(ctx))
         if self._h is not None and key == self._key:
             return self._h
         lib = L.load()
@@ -526,6 +535,7 @@ class AbstractIterativeSolver:
         o.mixed = int(self.mixed)
         o.inner_rtol, o.inner_max_iter = self.inner_rtol, self.inner_max_iter
         o.check_every = self.check_every
+        o.prec_transpose = int(self.prec_transpose)
         h = C.c_void_p()
         _raise(lib.rsgpu_solver_create(ctx.h, C.byref(o), self.n, C.byref(h)), ctx.h)
         self._h, self._key = h, key

@@ -78,3 +78,43 @@ def tridiag(n):
 def pentadiag(n):
     """IncompleteFactorizationTest.testPentaDiagonal (4 / -1 on +-1, +-2)."""
     return band_csr(n, 4.0, (1, 2))
+
+
+def oracle_well_mismatch(po, pr, m, q_obs, need_grad=True):
+    """CPU FP64 forward + adjoint of J(m) = sum_w (q_w - q_obs_w)^2 on the
+    oracle (K_e = exp(m_e) K0_e with K0 = the problem's tensor shape
+    K6 / K6_xx, wells = pr.pinned): (J, q, dJ/dm)."""
+    n = pr.n
+    K0 = pr.K6 / pr.K6[:, :1]
+    K6 = np.exp(m)[:, None] * K0
+    md_ptr, md_el, md_t, fails = po.mpfa_o(pr.dim, pr.nx, pr.ny, pr.nz, pr.dx, pr.dy, pr.dz, K6)
+    here, there = po.mesh_connections(pr.dim, pr.nx, pr.ny, pr.nz)
+    rp, ci = po.pattern(n, here, there, md_ptr, md_el)
+    Af = po.assemble_flux(n, here, there, md_ptr, md_el, md_t, rp, ci)
+    A = Af.copy()
+    wells = np.asarray(pr.pinned, np.int32)
+    po.zero_rows(rp, ci, A, 1.0, wells)
+    b = pr.rhs()
+    p, st, _ = po.solve("bicgstab", rp, ci, A, b, rtol=1e-13, atol=1e-300, prec="ilu")
+    assert st.status == 0
+    flux = po.connection_flux(md_ptr, md_el, md_t, p)
+    q = po.cell_rates(here, there, flux, wells)
+    r = q - q_obs
+    J = float(r @ r)
+    if not need_grad:
+        return J, q, None
+    rho = np.zeros(n)
+    rho[wells] = 2 * r
+    g = scipy_csr(rp, ci, Af).T @ rho
+    At = scipy_csr(rp, ci, A).T.tocsr()
+    At.sort_indices()
+    lam, st2, _ = po.solve("bicgstab", At.indptr.astype(np.int32), At.indices.astype(np.int32),
+                           At.data.copy(), g, rtol=1e-13, atol=1e-300, prec="ilu")
+    assert st2.status == 0
+    lam[wells] = 0
+    w = rho - lam
+    conn = np.repeat(np.arange(len(here)), np.diff(md_ptr))
+    tbar = (w[here] - w[there])[conn] * p[md_el]
+    mbar = po.mpfa_o_vjp(pr.dim, pr.nx, pr.ny, pr.nz, pr.dx, pr.dy, pr.dz, K6, md_ptr, md_el,
+                         tbar)
+    return J, q, mbar

@@ -1,0 +1,104 @@
+"""GPU adjoint of the well-rate mismatch (config-5 objective, reduced grid):
+the transmissibility-map VJP against the oracle bit for bit, the full
+gradient against the oracle's CPU FP64 adjoint and against central finite
+differences of the GPU forward model (step 1e-6 in log-permeability)."""
+import numpy as np
+import pytest
+
+from ressim_amd import problems as P
+from tests.helpers import oracle_system, oracle_well_mismatch
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(nx=10, ny=9, nz=6):
+    pr = P.config5(nx=nx, ny=ny, nz=nz)
+    m = np.log(pr.K6[:, 0])
+    K0 = pr.K6 / pr.K6[:, :1]
+    obs = P.config5(seed=6, nx=nx, ny=ny, nz=nz)
+    return pr, m, K0, obs
+
+
+@pytest.mark.parametrize("pr", [P.config1(nx=12, ny=9), P.config5(nx=8, ny=7, nz=5)],
+                         ids=lambda p: p.name)
+def test_mpfa_vjp_bitexact(gpu, oracle, pr):
+    import torch
+    from ressim_amd import _lib as L
+    from ressim_amd.mtj import _dptr, _hptr, Context
+    from ressim_amd.potential import FluxOperator
+    s = oracle_system(oracle, pr)
+    F, fails = FluxOperator.mpfa_o(pr.dim, pr.nx, pr.ny, pr.nz, pr.dx, pr.dy, pr.dz, pr.K6)
+    tbar = np.random.default_rng(4).standard_normal(len(s["md_t"]))
+    ref = oracle.mpfa_o_vjp(pr.dim, pr.nx, pr.ny, pr.nz, pr.dx, pr.dy, pr.dz, pr.K6,
+                            s["md_ptr"], s["md_elem"], tbar)
+    td = torch.tensor(tbar, device="cuda")
+    out = torch.empty(pr.n, dtype=torch.float64, device="cuda")
+    K6 = np.ascontiguousarray(pr.K6)
+    ctx = Context.default()
+    assert L.load().rsgpu_mpfa_o_vjp(ctx.h, pr.dim, pr.nx, pr.ny, pr.nz, pr.dx, pr.dy, pr.dz,
+                                     _hptr(K6), 1.0, F.h, _dptr(td), _dptr(out)) == 0
+    ctx.synchronize()
+    np.testing.assert_array_equal(out.cpu().numpy(), ref)
+    # flux VJP: tbar = (w[here] - w[there]) p[k]
+    p = torch.tensor(np.random.default_rng(5).standard_normal(pr.n), device="cuda")
+    w = torch.tensor(np.random.default_rng(6).standard_normal(pr.n), device="cuda")
+    tb = F.vjp(p, w).cpu().numpy()
+    conn = np.repeat(np.arange(len(s["here"])), np.diff(s["md_ptr"]))
+    pn, wn = p.cpu().numpy(), w.cpu().numpy()
+    np.testing.assert_array_equal(tb, (wn[s["here"]] - wn[s["there"]])[conn] * pn[s["md_elem"]])
+
+
+def test_gradient_vs_cpu_adjoint_and_fd(gpu, oracle):
+    from ressim_amd.adjoint import WellRateMismatch
+    pr, m, K0, obs = _problem()
+    q_obs = oracle_well_mismatch(oracle, obs, np.log(obs.K6[:, 0]), np.zeros(4),
+                                 need_grad=False)[1]
+    W = WellRateMismatch(pr.dim, pr.nx, pr.ny, pr.nz, pr.dx, pr.dy, pr.dz, K0,
+                         pr.sources, pr.pinned, q_obs=q_obs)
+    J = W.forward(m)
+    g = W.backward()
+    Jo, qo, go = oracle_well_mismatch(oracle, pr, m, q_obs)
+    print("J", J, Jo, "iterations", W.last_iterations, flush=True)
+    assert abs(J - Jo) <= 1e-10 * abs(Jo)
+    np.testing.assert_allclose(W.q, qo, rtol=1e-10, atol=1e-14)
+    # producers take all the injected fluid
+    assert abs(W.q.sum() + 1.0) < 1e-10
+    assert np.linalg.norm(g - go) <= 1e-8 * np.linalg.norm(go)
+    # central differences on random cells, step 1e-6
+    rng = np.random.default_rng(12)
+    cells = rng.choice(pr.n, 8, replace=False)
+    h = 1e-6
+    for e in cells:
+        mp, mm = m.copy(), m.copy()
+        mp[e] += h
+        mm[e] -= h
+        fd = (W.forward(mp) - W.forward(mm)) / (2 * h)
+        assert abs(fd - g[e]) <= 1e-5 * max(abs(g[e]), 1e-3 * np.abs(g).max()), (e, fd, g[e])
+
+
+def test_transposed_solve(gpu, oracle):
+    """A' x = b through the transposed CSR and ILU(A).transApply."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    import torch
+    s = oracle_system(oracle, P.small3d(nx=9, ny=8, nz=6))
+    n = len(s["rowptr"]) - 1
+    A = gpu.CompRowMatrix.from_csr(n, n, s["rowptr"], s["colind"], s["data"])
+    M = gpu.ILU(A)
+    M.setMatrix(A)
+    At = A.transpose()
+    b = np.random.default_rng(1).standard_normal(n)
+    solver = gpu.BiCGstab(b)
+    solver.setIterationMonitor(gpu.DefaultIterationMonitor(5000, 1e-12, 1e-50, 1e5))
+    solver.setPreconditioner(M)
+    solver.prec_transpose = True
+    x = np.zeros(n)
+    solver.solve(At, b, x)
+    As = sp.csr_matrix((s["data"], s["colind"], s["rowptr"]), shape=(n, n))
+    xe = spl.spsolve(As.T.tocsc(), b)
+    assert np.linalg.norm(x - xe) / np.linalg.norm(xe) < 1e-9
+    # the preconditioner must belong to the matrix whose transpose is solved
+    solver2 = gpu.BiCGstab(b)
+    solver2.setPreconditioner(M)
+    with pytest.raises(ValueError):
+        solver2.solve(At, b, np.zeros(n))

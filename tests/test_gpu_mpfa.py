@@ -49,3 +49,26 @@ def test_continuity_parameter(gpu, oracle):
         np.testing.assert_array_equal(got[2], ref[2])
     with pytest.raises(ValueError):
         gpu.mpfa_o(3, 2, 2, 2, 1, 1, -1, pr.K6[:8], 0.0)
+
+
+@pytest.mark.parametrize("pr", CASES, ids=lambda p: p.name)
+@pytest.mark.parametrize("source", ["lists", "device"])
+def test_flux_and_rates_bitexact(gpu, oracle, pr, source):
+    """PotentialCalculator.calculateFlux and cell net outflows on the device
+    against the oracle, bit for bit."""
+    from ressim_amd.potential import FluxOperator
+    s = oracle_system(oracle, pr)
+    here, there = s["here"], s["there"]
+    if source == "lists":
+        F = FluxOperator.from_lists(pr.n, s["md_ptr"], s["md_elem"], s["md_t"], here, there)
+    else:
+        F, fails = FluxOperator.mpfa_o(pr.dim, pr.nx, pr.ny, pr.nz, pr.dx, pr.dy, pr.dz,
+                                       pr.K6)
+        assert fails == 0
+    assert (F.nconn, F.ncells) == (len(here), pr.n)
+    p = np.random.default_rng(9).standard_normal(pr.n)
+    f_ref = oracle.connection_flux(s["md_ptr"], s["md_elem"], s["md_t"], p)
+    f = F.calculateFlux(p)
+    np.testing.assert_array_equal(f, f_ref)
+    cells = np.array([0, pr.n // 2, pr.n - 1] + list(pr.pinned), np.int32)
+    np.testing.assert_array_equal(F.rates(f, cells), oracle.cell_rates(here, there, f_ref, cells))
