@@ -71,6 +71,9 @@ def main():
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=300)
+    ap.add_argument("--adjoint", action="store_true",
+                    help="measure the adjoint gradient workload (64x64x32 well-rate mismatch) "
+                         "instead of the pressure solve and print its own JSON line")
     ap.add_argument("--probe", choices=["spmv", "ilu", "solve"], default=None,
                     help="profiling probe: run only the named kernel(s) on the workload "
                          "(plus a 512 MiB device copy as the traffic reference) and exit")
@@ -83,6 +86,8 @@ def main():
     import torch
     from ressim_amd import _lib, mtj, problems as P
 
+    if a.adjoint:
+        return bench_adjoint(a)
     lib = _lib.load()
     dev = torch.device("cuda:0")
     torch.cuda.set_device(0)
@@ -254,6 +259,76 @@ def main():
                      "algorithmic_bytes": b64},
         "cpu_baseline": cpu,
     }
+    print(json.dumps(out), flush=True)
+
+
+def bench_adjoint(a):
+    """Gradient of J(m) = sum_w (q_w - q_obs_w)^2 with respect to all
+    131,072 log-permeabilities (64x64x32, 4 producers locked, 1 injector;
+    q_obs from a second seeded field): forward and adjoint times, central
+    differences (step 1e-6) on 32 random cells whose gradient exceeds 1e-3
+    of the largest, and the oracle's CPU FP64 adjoint."""
+    import torch
+    from ressim_amd import problems as P
+    from ressim_amd.adjoint import WellRateMismatch
+    nx, ny, nz = 64, 64, 32
+    pr = P.config5(nx=nx, ny=ny, nz=nz)
+    obs = P.config5(seed=6, nx=nx, ny=ny, nz=nz)
+    m = np.log(pr.K6[:, 0])
+    W0 = WellRateMismatch(3, nx, ny, nz, 1.0, 1.0, -1.0, obs.K6 / obs.K6[:, :1], obs.sources,
+                          obs.pinned)
+    q_obs = W0.rates(np.log(obs.K6[:, 0]))
+    W = WellRateMismatch(3, nx, ny, nz, 1.0, 1.0, -1.0, pr.K6 / pr.K6[:, :1], pr.sources,
+                         pr.pinned, q_obs=q_obs)
+    W.forward(m)
+    W.backward()
+    tf, tb = [], []
+    for _ in range(max(1, a.steps)):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        J = W.forward(m)
+        torch.cuda.synchronize()
+        tf.append(time.perf_counter() - t)
+        t = time.perf_counter()
+        g = W.backward()
+        torch.cuda.synchronize()
+        tb.append(time.perf_counter() - t)
+    rng = np.random.default_rng(2024)
+    big = np.flatnonzero(np.abs(g) >= 1e-3 * np.abs(g).max())
+    cells = rng.choice(big, min(32, big.size), replace=False)
+    errs = {}
+    for h in (1e-6, 1e-4):
+        e = []
+        for c in cells:
+            mp, mm = m.copy(), m.copy()
+            mp[c] += h
+            mm[c] -= h
+            fd = (W.forward(mp) - W.forward(mm)) / (2 * h)
+            e.append(abs(fd - g[c]) / abs(fd))
+        errs[h] = e
+    log(f"adjoint: J={J:.6e} forward {np.median(tf) * 1e3:.1f} ms backward "
+        f"{np.median(tb) * 1e3:.1f} ms iterations {W.last_iterations}")
+    cpu = None
+    if not a.no_cpu:
+        from oracle import pyoracle as po
+        from tests.helpers import oracle_well_mismatch
+        t = time.perf_counter()
+        Jo, qo, go = oracle_well_mismatch(po, pr, m, q_obs)
+        cpu = {"seconds": time.perf_counter() - t, "cores": 1, "kind": "port",
+               "rel_J_diff": abs(J - Jo) / abs(Jo),
+               "rel_grad_diff": float(np.linalg.norm(g - go) / np.linalg.norm(go))}
+    out = {"metric": "adjoint gradient of well-rate mismatch w.r.t. log-permeability",
+           "value": 1.0 / (np.median(tf) + np.median(tb)), "unit": "gradients/s",
+           "n_gpus": 1, "steps": a.steps, "higher_is_better": True, "dtype": "f64",
+           "data": "synthetic (seeded log-normal permeability, two fields)",
+           "config": {"workload": f"MPFA-O 3-D {nx}x{ny}x{nz}, 4 producers + 1 injector, "
+                                  f"BiCGstab + block ILU(0) x{W.nblocks} to 1e-13 forward and "
+                                  f"adjoint", "n": pr.n},
+           "forward_ms": float(np.median(tf) * 1e3), "backward_ms": float(np.median(tb) * 1e3),
+           "fd_cells": int(cells.size),
+           "fd_rel_err_h1e-6": {"max": max(errs[1e-6]), "median": float(np.median(errs[1e-6]))},
+           "fd_rel_err_h1e-4": {"max": max(errs[1e-4]), "median": float(np.median(errs[1e-4]))},
+           "cpu_adjoint": cpu}
     print(json.dumps(out), flush=True)
 
 

@@ -117,7 +117,8 @@ enum rsgpu_prec_kind {
     RSGPU_PREC_ILU0 = 1,       /* mtj/sparse/ILU.java                            */
     RSGPU_PREC_BLOCK_ILU0 = 2, /* distributed/BlockDiagonalPreconditioner + ILU  */
     RSGPU_PREC_DIAGONAL = 3,   /* mtj/sparse/DiagonalPreconditioner.java         */
-    RSGPU_PREC_AMG = 4         /* mtj/sparse/AMG.java (smoothed aggregation)     */
+    RSGPU_PREC_AMG = 4,        /* mtj/sparse/AMG.java (smoothed aggregation)     */
+    RSGPU_PREC_SSOR = 5        /* mtj/sparse/SSOR.java                           */
 };
 enum rsgpu_precision { RSGPU_FP64 = 0, RSGPU_FP32 = 1 };
 
@@ -173,6 +174,12 @@ void rsgpu_amg_params_default(rsgpu_amg_params *p);
 /* Takes effect at the next rsgpu_prec_set_matrix (AMG.setMatrix,
  * AMG.java:228-294, which rebuilds the hierarchy every call). */
 int rsgpu_prec_set_amg_params(rsgpu_prec *M, const rsgpu_amg_params *p);
+/* SSOR(F, reverse, omegaF, omegaR) (SSOR.java:80-130): apply(b, x) starts
+ * from x's old content (SSOR.java:143-194); transApply = apply.  block_rows
+ * and ordering as for the AMG smoother (0, 0 = SSOR.java exactly).
+ * Takes effect at the next set_matrix. */
+int rsgpu_prec_set_ssor(rsgpu_prec *M, double omega_f, double omega_r,
+                        int reverse, int block_rows, int ordering);
 /* Hierarchy introspection (tests): level sizes / nonzeros; returns the
  * level count m (the last level is solved by a dense LU). */
 int rsgpu_amg_levels(const rsgpu_prec *M, int *sizes, int *nnzs, int cap);
@@ -187,7 +194,8 @@ int rsgpu_amg_get_level(const rsgpu_prec *M, int k, int32_t *rowptr,
 enum rsgpu_solver_kind {
     RSGPU_BICGSTAB = 0,  /* mtj/sparse/BiCGstab.java */
     RSGPU_GMRES = 1,     /* mtj/sparse/GMRES.java    */
-    RSGPU_IR = 2         /* mtj/sparse/IR.java       */
+    RSGPU_IR = 2,        /* mtj/sparse/IR.java       */
+    RSGPU_CG = 3         /* mtj/sparse/CG.java       */
 };
 
 /* DefaultIterationMonitor(maxIter, rtol, atol, dtol)

@@ -412,12 +412,21 @@ typedef struct {
     int *rowptr, *colind, *diagind;
     double *lu, *y;
     or_amg *amg;
+    /* SSOR: the matrix and parameters; diagonal: 1 / A(i,i) */
+    const int *arp, *aci;
+    const double *av;
+    or_amg_params sp;
+    double *invdiag;
 } prec_t;
 
 static void prec_free(prec_t *p)
 {
     if (p->kind == 3) {
         or_amg_free(p->amg);
+    } else if (p->kind == 4) {
+        free(p->invdiag);
+    } else if (p->kind == 5) {
+        /* nothing owned */
     } else if (p->kind != 0) {
         free(p->rowptr);
         free(p->colind);
@@ -439,6 +448,34 @@ static int prec_setup(prec_t *p, int n, const int *rowptr, const int *colind,
         return OR_OK;
     if (p->kind == 3)
         return or_amg_setup(n, rowptr, colind, data, &o->amg, &p->amg, bad_row);
+    if (p->kind == 4) {
+        /* DiagonalPreconditioner.setMatrix (DiagonalPreconditioner.java:67-78) */
+        p->invdiag = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+        for (int i = 0; i < n; ++i) {
+            int d = or_binary_search(colind, i, rowptr[i], rowptr[i + 1]);
+            double v = d >= 0 ? data[d] : 0.0;
+            if (v == 0) {
+                if (bad_row)
+                    *bad_row = i;
+                return OR_ZERO_PIVOT;
+            }
+            p->invdiag[i] = 1 / v;
+        }
+        return OR_OK;
+    }
+    if (p->kind == 5) {
+        for (int i = 0; i < n; ++i)
+            if (or_binary_search(colind, i, rowptr[i], rowptr[i + 1]) < 0) {
+                if (bad_row)
+                    *bad_row = i + 1;
+                return OR_MISSING_DIAGONAL;
+            }
+        p->arp = rowptr;
+        p->aci = colind;
+        p->av = data;
+        p->sp = o->amg;
+        return OR_OK;
+    }
     int nnz;
     p->rowptr = (int *)malloc(sizeof(int) * (n + 1));
     if (p->kind == 1) {
@@ -476,6 +513,17 @@ static void prec_apply(prec_t *p, const double *b, double *x)
     if (p->kind == 3) {
         /* AMG.apply starts the cycle from x's old content (AMG.java:208-216) */
         or_amg_apply(p->amg, b, x, 0);
+        return;
+    }
+    if (p->kind == 4) {
+        for (int i = 0; i < p->n; ++i)
+            x[i] = b[i] * p->invdiag[i];
+        return;
+    }
+    if (p->kind == 5) {
+        /* SSOR.apply starts from x's old content (SSOR.java:143-194) */
+        or_ssor_apply(p->n, p->arp, p->aci, p->av, p->sp.block_rows, p->sp.ordering,
+                      p->sp.omega_pre_f, p->sp.omega_pre_r, p->sp.reverse, b, x, NULL);
         return;
     }
     or_ilu_apply(p->n, p->rowptr, p->colind, p->lu, p->diagind, b, x, p->y);
@@ -748,6 +796,57 @@ int or_gmres(int n, const int *rowptr, const int *colind, const double *data,
     free(s);
     free(H);
     free(rot);
+    prec_free(&M);
+    fill_stats(stats, &it, st);
+    return st;
+}
+
+/* ---- CG (CG.java:58-92) --------------------------------------------------- */
+int or_cg(int n, const int *rowptr, const int *colind, const double *data,
+          const double *b, double *x, const or_opts *opts, or_stats *stats,
+          double *hist, int hist_cap)
+{
+    prec_t M;
+    int bad = -1;
+    int st = prec_setup(&M, n, rowptr, colind, data, opts, &bad);
+    monitor_t it;
+    mon_init(&it, opts, hist, hist_cap);
+    if (st) {
+        prec_free(&M);
+        fill_stats(stats, &it, st);
+        if (stats)
+            stats->bad_row = bad;
+        return st;
+    }
+    double *w = (double *)calloc((size_t)n * 4 + 1, sizeof(double));
+    double *r = w, *z = w + n, *p = w + 2 * (size_t)n, *q = w + 3 * (size_t)n;
+    double alpha = 0, beta = 0, rho = 0, rho_1 = 0;
+    int conv = 0;
+    v_copy(n, r, b);
+    or_csr_multadd(n, rowptr, colind, data, -1, x, r);
+    for (it.iter = 0;; it.iter++) {
+        st = mon_converged(&it, or_norm2(n, r), &conv);
+        if (st || conv)
+            break;
+        prec_apply(&M, r, z);
+        rho = or_dot(n, r, z);
+        if (it.iter == 0) {
+            v_copy(n, p, z);
+        } else {
+            beta = rho / rho_1;
+            /* p.scale(beta).add(z) */
+            for (int i = 0; i < n; ++i) {
+                double pi = beta == 0 ? 0.0 : (beta == 1 ? p[i] : beta * p[i]);
+                p[i] = pi + z[i];
+            }
+        }
+        or_csr_mult(n, rowptr, colind, data, p, q);
+        alpha = rho / or_dot(n, p, q);
+        v_add_scaled(n, x, alpha, p);
+        v_add_scaled(n, r, -alpha, q);
+        rho_1 = rho;
+    }
+    free(w);
     prec_free(&M);
     fill_stats(stats, &it, st);
     return st;

@@ -138,7 +138,9 @@ typedef struct {
     int max_iter;
     double rtol, atol, dtol;
     int restart;        /* GMRES(m) */
-    /* preconditioner: 0 identity, 1 ILU(0), 2 block-Jacobi ILU(0), 3 AMG */
+    /* preconditioner: 0 identity, 1 ILU(0), 2 block-Jacobi ILU(0), 3 AMG,
+     * 4 diagonal, 5 SSOR (parameters in amg: omega_pre_f/r, reverse,
+     * block_rows, ordering) */
     int prec_kind;
     int nblocks;        /* for block-Jacobi, contiguous equal row ranges */
     or_amg_params amg;  /* for AMG */
@@ -159,6 +161,9 @@ int or_bicgstab(int n, const int *rowptr, const int *colind,
 int or_gmres(int n, const int *rowptr, const int *colind, const double *data,
              const double *b, double *x, const or_opts *opts, or_stats *stats,
              double *hist, int hist_cap);
+int or_cg(int n, const int *rowptr, const int *colind, const double *data,
+          const double *b, double *x, const or_opts *opts, or_stats *stats,
+          double *hist, int hist_cap);
 int or_ir(int n, const int *rowptr, const int *colind, const double *data,
           const double *b, double *x, const or_opts *opts, or_stats *stats,
           double *hist, int hist_cap);

@@ -92,7 +92,7 @@ def lib():
         L.or_block_filter.argtypes = [C.c_int, _i32p, _i32p, _f64p, C.c_int, _i32p, _i32p,
                                       _i32p, _f64p, _i32p]
         L.or_block_starts.argtypes = [C.c_int, C.c_int, _i32p]
-        for f in (L.or_bicgstab, L.or_gmres, L.or_ir):
+        for f in (L.or_bicgstab, L.or_gmres, L.or_ir, L.or_cg):
             f.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.POINTER(Opts),
                           C.POINTER(Stats), _f64p, C.c_int]
         L.or_dgesv.argtypes = [C.c_int, C.c_int, _f64p, C.c_int, _i32p, _f64p, C.c_int]
@@ -283,7 +283,8 @@ def block_filter(rowptr, colind, data, nblocks):
 
 
 # ---------------------------------------------------------------- solvers
-PREC = {"none": 0, "identity": 0, "ilu": 1, "block_ilu": 2, "amg": 3}
+PREC = {"none": 0, "identity": 0, "ilu": 1, "block_ilu": 2, "amg": 3, "diagonal": 4,
+        "ssor": 5}
 
 
 def solve(kind, rowptr, colind, data, b, x0=None, *, max_iter=100000, rtol=1e-5, atol=1e-50,
@@ -299,7 +300,8 @@ def solve(kind, rowptr, colind, data, b, x0=None, *, max_iter=100000, rtol=1e-5,
     cap = (max_iter + 2) * 2 if hist_cap is None else hist_cap
     cap = max(1, min(cap, 10_000_000))
     hist = np.zeros(cap)
-    fn = {"bicgstab": lib().or_bicgstab, "gmres": lib().or_gmres, "ir": lib().or_ir}[kind]
+    fn = {"bicgstab": lib().or_bicgstab, "gmres": lib().or_gmres, "ir": lib().or_ir,
+          "cg": lib().or_cg}[kind]
     fn(n, _i32(rowptr), _i32(colind), _f64(data), _f64(b), x, C.byref(o), C.byref(s), hist, cap)
     return x, s, hist[: min(s.hist_len, cap)]
 

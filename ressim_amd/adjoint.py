@@ -34,7 +34,8 @@ from .potential import FluxOperator
 
 class WellRateMismatch:
     def __init__(self, dim, nx, ny, nz, dx, dy, dz, K0, sources: dict, wells: Sequence[int],
-                 q_obs=None, rtol: float = 1e-13, ctx: Optional[Context] = None):
+                 q_obs=None, rtol: float = 1e-13, nblocks: int = 64,
+                 ctx: Optional[Context] = None):
         self.dim, self.nx, self.ny, self.nz = dim, nx, ny, nz
         self.dx, self.dy, self.dz = dx, dy, dz
         self.K0 = np.ascontiguousarray(K0, np.float64)
@@ -43,6 +44,9 @@ class WellRateMismatch:
         self.wells = np.ascontiguousarray(wells, np.int32)
         self.q_obs = None if q_obs is None else np.asarray(q_obs, np.float64)
         self.rtol = rtol
+        # block-Jacobi ILU(0) (nblocks = 1: MTJ's ILU; one workgroup sweeps
+        # the whole matrix, so large grids want blocks)
+        self.nblocks = max(1, min(nblocks, self.n))
         self.ctx = ctx or Context.default()
         self.last_iterations = (0, 0)
 
@@ -76,7 +80,7 @@ class WellRateMismatch:
         A, fails2 = mtj.mpfa_o_assemble(*g, ctx=self.ctx)
         At_flux = A.transpose()
         mtj.Matrices.zeroRows(A, 1.0, self.wells)
-        M = mtj.ILU(A)
+        M = mtj.ILU(A) if self.nblocks == 1 else mtj.BlockILU(self.nblocks, ctx=self.ctx)
         M.setMatrix(A)
         p, it = self._solve(A, M, self._rhs())
         flux = F.calculateFlux(p)

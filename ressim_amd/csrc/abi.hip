@@ -455,14 +455,15 @@ extern "C" int rsgpu_prec_create(rsgpu_ctx *ctx, int kind, int nblocks,
     if (!ctx || !out)
         return RSGPU_BAD_ARGUMENT;
     if (kind != RSGPU_PREC_IDENTITY && kind != RSGPU_PREC_ILU0 &&
-        kind != RSGPU_PREC_BLOCK_ILU0 && kind != RSGPU_PREC_AMG) {
+        kind != RSGPU_PREC_BLOCK_ILU0 && kind != RSGPU_PREC_AMG &&
+        kind != RSGPU_PREC_DIAGONAL && kind != RSGPU_PREC_SSOR) {
         ctx->err = "preconditioner kind not available";
         return RSGPU_BAD_ARGUMENT;
     }
     if (precision != RSGPU_FP64 && precision != RSGPU_FP32)
         return RSGPU_BAD_ARGUMENT;
-    if (kind == RSGPU_PREC_AMG && precision != RSGPU_FP64) {
-        ctx->err = "AMG runs in FP64";
+    if ((kind == RSGPU_PREC_AMG || kind == RSGPU_PREC_SSOR) && precision != RSGPU_FP64) {
+        ctx->err = "AMG and SSOR run in FP64";
         return RSGPU_BAD_ARGUMENT;
     }
     if (kind == RSGPU_PREC_BLOCK_ILU0 && nblocks < 1)
@@ -474,7 +475,35 @@ extern "C" int rsgpu_prec_create(rsgpu_ctx *ctx, int kind, int nblocks,
     M->precision = precision;
     M->ready = (kind == RSGPU_PREC_IDENTITY);
     rsgpu_amg_params_default(&M->amg_params);
+    if (kind == RSGPU_PREC_SSOR) {
+        // SSOR(F) defaults: reverse sweep, omega 1 (SSOR.java:103-105)
+        M->amg_params.omega_pre_f = 1;
+        M->amg_params.omega_pre_r = 1;
+        M->amg_params.reverse = 1;
+        M->amg_params.block_rows = 0;
+        M->amg_params.ordering = 0;
+    }
     *out = M;
+    return RSGPU_OK;
+}
+
+extern "C" int rsgpu_prec_set_ssor(rsgpu_prec *M, double omega_f, double omega_r, int reverse,
+                                   int block_rows, int ordering)
+{
+    if (!M || M->kind != RSGPU_PREC_SSOR)
+        return RSGPU_BAD_ARGUMENT;
+    // SSOR.setOmega (SSOR.java:122-130)
+    if (omega_f < 0 || omega_f > 2 || omega_r < 0 || omega_r > 2 || block_rows < 0 ||
+        ordering < 0 || ordering > 1) {
+        M->ctx->err = "omegaF and omegaR must be between 0 and 2";
+        return RSGPU_BAD_ARGUMENT;
+    }
+    M->amg_params.omega_pre_f = omega_f;
+    M->amg_params.omega_pre_r = omega_r;
+    M->amg_params.reverse = reverse ? 1 : 0;
+    M->amg_params.block_rows = block_rows;
+    M->amg_params.ordering = ordering;
+    M->ready = false;
     return RSGPU_OK;
 }
 
@@ -541,11 +570,14 @@ extern "C" int rsgpu_prec_set_matrix(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
         M->ready = true;
         return RSGPU_OK;
     }
-    if (M->kind == RSGPU_PREC_AMG) {
+    if (M->kind == RSGPU_PREC_AMG || M->kind == RSGPU_PREC_SSOR ||
+        M->kind == RSGPU_PREC_DIAGONAL) {
         M->ready = false;
         M->n = A->n;
         M->pattern_id = A->pattern_id;
-        const int st = amg_setup(M, A, bad_row);
+        const int st = M->kind == RSGPU_PREC_AMG    ? amg_setup(M, A, bad_row)
+                       : M->kind == RSGPU_PREC_SSOR ? ssor_setup(M, A, bad_row)
+                                                    : diag_setup(M, A, bad_row);
         M->ready = st == RSGPU_OK;
         M->nnz = A->nnz;
         return st;

@@ -1596,3 +1596,104 @@ int amg_get_level(const rsgpu_prec *M, int k, int32_t *rowptr, int32_t *colind, 
     }
     return RSGPU_OK;
 }
+
+// ---------------------------------------------------------------- SSOR preconditioner
+// SSOR.setMatrix (SSOR.java:132-141): copy of the matrix, diagonal indices
+// (missing diagonal -> error), and the sweep schedule (reused while the
+// pattern is the same).  The parameters live in amg_params (omega_pre_f,
+// omega_pre_r, reverse, block_rows, ordering).
+int ssor_setup(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
+{
+    hipStream_t s = M->ctx->stream;
+    if (A->n != A->ncols)
+        throw RsError{RSGPU_BAD_ARGUMENT, "SSOR only applies to square matrices"};
+    if (!M->amg)
+        M->amg = new AmgData();
+    AmgData *D = M->amg;
+    D->p = M->amg_params;
+    const rsgpu_amg_params &p = D->p;
+    const bool same = !D->L.empty() && D->pattern0 == A->pattern_id;
+    if (!same) {
+        D->L.clear();
+        D->L.push_back(std::make_unique<AmgLevel>());
+        AmgLevel &L0 = *D->L[0];
+        L0.n = A->n;
+        L0.nnz = A->nnz;
+        L0.rp = A->rowptr.p;       // pattern borrowed; values copied below
+        L0.ci = A->colind.p;
+        L0.h_rp = A->h_rowptr;
+        L0.h_ci = A->h_colind;
+        D->pattern0 = A->pattern_id;
+    }
+    AmgLevel &L = *D->L[0];
+    L.rp = A->rowptr.p;
+    L.ci = A->colind.p;
+    L.v.alloc(std::max(A->nnz, 1));
+    if (A->nnz)
+        RS_HIP(hipMemcpyAsync(L.v.p, A->val.p, sizeof(double) * A->nnz,
+                              hipMemcpyDeviceToDevice, s));
+    L.diag.alloc(std::max(L.n, 1));
+    DevBuf<int> bad;
+    bad.alloc(1);
+    set_int(s, bad.p, 0x7fffffff);
+    if (L.n)
+        hipLaunchKernelGGL(k_amg_diag, dim3(gridn(L.n)), dim3(256), 0, s, L.n, L.rp, L.ci,
+                           L.diag.p, bad.p);
+    RS_CHECK_LAUNCH();
+    const int b = read_int(s, bad.p);
+    if (b != 0x7fffffff) {
+        if (bad_row)
+            *bad_row = b;
+        M->ctx->err = "Missing diagonal on row " + std::to_string(b);
+        D->L.clear();
+        return RSGPU_MISSING_DIAGONAL;
+    }
+    if (!(L.sched_pattern == A->pattern_id && L.nb_rows_param == p.block_rows &&
+          L.ordering_param == p.ordering)) {
+        build_ssor_schedule(L, p.block_rows, p.ordering, s);
+        L.sched_pattern = A->pattern_id;
+        L.nb_rows_param = p.block_rows;
+        L.ordering_param = p.ordering;
+        level_vectors(L, s);
+    }
+    ssor_values(L, s);
+    D->m = 1;
+    RS_HIP(hipStreamSynchronize(s));
+    return RSGPU_OK;
+}
+
+// SSOR.apply(b, x): one application starting from x's content
+void ssor_apply(rsgpu_prec *M, const double *b, double *x, const DevState *st)
+{
+    AmgData *D = M->amg;
+    hipStream_t s = M->ctx->stream;
+    Skip sk{st ? &st->done : nullptr, st ? &st->cycle_end : nullptr};
+    AmgLevel &L = *D->L[0];
+    const int n = L.n;
+    if (n == 0)
+        return;
+    const rsgpu_amg_params &p = D->p;
+    hipLaunchKernelGGL(k_amg_copy, dim3(gridn(n)), dim3(256), 0, s, (const double *)x, L.u.p, n,
+                       sk);
+    if (L.global_sweep) {
+        const int nl = (int)L.h_meta.size();
+        for (int l = 0; l < nl; ++l)
+            hipLaunchKernelGGL(k_amg_ssor_level, dim3(gridn(L.h_meta[l].count)), dim3(256), 0, s,
+                               L.h_meta[l], L.rows.p, L.ell_col.p, L.ell_val.p, L.dgl.p, L.W, b,
+                               L.u.p, p.omega_pre_f, sk);
+        if (p.reverse)
+            for (int l = nl - 1; l >= 0; --l)
+                hipLaunchKernelGGL(k_amg_ssor_level, dim3(gridn(L.h_meta[l].count)), dim3(256),
+                                   0, s, L.h_meta[l], L.rows.p, L.ell_col.p, L.ell_val.p,
+                                   L.dgl.p, L.W, b, L.u.p, p.omega_pre_r, sk);
+    } else {
+        hipLaunchKernelGGL(k_amg_ssor, dim3(L.nb), dim3(256), 0, s, L.bstart.p, L.blk_lev.p,
+                           (const AmgLevelMeta *)L.meta.p, L.rows.p, L.ell_col.p, L.ell_val.p,
+                           L.dgl.p, L.W, b, L.u.p, L.ualt.p, p.omega_pre_f, p.omega_pre_r,
+                           p.reverse, sk);
+        swap_buf(L.u, L.ualt);
+    }
+    hipLaunchKernelGGL(k_amg_copy, dim3(gridn(n)), dim3(256), 0, s, (const double *)L.u.p, x, n,
+                       sk);
+    RS_CHECK_LAUNCH();
+}

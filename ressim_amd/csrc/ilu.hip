@@ -790,10 +790,96 @@ __global__ void k_copy_skip32(const float *__restrict__ a, float *__restrict__ b
 }
 
 // apply with optional device skip flags (done / cycle_end of a solver)
+// ---------------------------------------------------------------- diagonal
+// DiagonalPreconditioner.setMatrix (DiagonalPreconditioner.java:67-78):
+// invdiag = 1 / A(i,i); an absent or zero diagonal is an error.
+__global__ void k_invdiag(int n, const int *__restrict__ rp, const int *__restrict__ ci,
+                          const double *__restrict__ v, double *__restrict__ inv,
+                          float *__restrict__ inv32, int *bad)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    int lo = rp[i], hi = rp[i + 1];
+    double d = 0;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const int c = ci[mid];
+        if (c == i) {
+            d = v[mid];
+            break;
+        }
+        if (c < i)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    if (d == 0) {
+        atomicMin(bad, i + 1);
+        return;
+    }
+    const double r = 1 / d;
+    inv[i] = r;
+    if (inv32)
+        inv32[i] = (float)r;
+}
+
+template <typename T>
+__global__ void k_diag_apply(int n, const T *__restrict__ inv, const T *__restrict__ b,
+                             T *__restrict__ x, const int *sk1, const int *sk2)
+{
+    if ((sk1 && *sk1) || (sk2 && *sk2))
+        return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        x[i] = b[i] * inv[i];
+}
+
+int diag_setup(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
+{
+    hipStream_t s = M->ctx->stream;
+    if (A->n != A->ncols)
+        throw RsError{RSGPU_BAD_ARGUMENT, "Matrix size differs from preconditioner size"};
+    const int n = A->n;
+    M->n = n;
+    M->invdiag.alloc(std::max(n, 1));
+    if (M->precision == RSGPU_FP32)
+        M->invdiag32.alloc(std::max(n, 1));
+    DevBuf<int> bad;
+    int h = 0x7fffffff;
+    bad.upload(&h, 1, s);
+    if (n)
+        hipLaunchKernelGGL(k_invdiag, dim3((n + 255) / 256), dim3(256), 0, s, n, A->rowptr.p,
+                           A->colind.p, A->val.p, M->invdiag.p,
+                           M->precision == RSGPU_FP32 ? M->invdiag32.p : nullptr, bad.p);
+    RS_CHECK_LAUNCH();
+    RS_HIP(hipMemcpyAsync(&h, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    RS_HIP(hipStreamSynchronize(s));
+    if (h != 0x7fffffff) {
+        if (bad_row)
+            *bad_row = h - 1;
+        M->ctx->err = "Zero diagonal on row " + std::to_string(h);
+        return RSGPU_ZERO_PIVOT;
+    }
+    return RSGPU_OK;
+}
+
 void prec_apply(rsgpu_prec *M, const double *b, double *x, const DevState *st)
 {
     if (M->kind == RSGPU_PREC_AMG) {
         amg_apply(M, b, x, st, false);
+        return;
+    }
+    if (M->kind == RSGPU_PREC_SSOR) {
+        ssor_apply(M, b, x, st);
+        return;
+    }
+    if (M->kind == RSGPU_PREC_DIAGONAL) {
+        if (M->n)
+            hipLaunchKernelGGL(k_diag_apply<double>, dim3((M->n + 255) / 256), dim3(256), 0,
+                               M->ctx->stream, M->n, M->invdiag.p, b, x,
+                               st ? &st->done : nullptr, st ? &st->cycle_end : nullptr);
+        RS_CHECK_LAUNCH();
         return;
     }
     const int *sk1 = st ? &st->done : nullptr;
@@ -832,8 +918,16 @@ void prec_apply(rsgpu_prec *M, const float *b, float *x, const DevState *st)
         RS_CHECK_LAUNCH();
         return;
     }
-    if (M->precision != RSGPU_FP32 || M->kind == RSGPU_PREC_AMG)
-        throw RsError{RSGPU_BAD_ARGUMENT, "FP32 apply needs an FP32 preconditioner"};
+    if (M->kind == RSGPU_PREC_DIAGONAL && M->precision == RSGPU_FP32) {
+        if (M->n)
+            hipLaunchKernelGGL(k_diag_apply<float>, dim3((M->n + 255) / 256), dim3(256), 0, s,
+                               M->n, M->invdiag32.p, b, x, sk1, sk2);
+        RS_CHECK_LAUNCH();
+        return;
+    }
+    if (M->precision != RSGPU_FP32 || M->kind == RSGPU_PREC_AMG ||
+        M->kind == RSGPU_PREC_SSOR)
+        throw RsError{RSGPU_BAD_ARGUMENT, "FP32 apply needs an FP32 ILU or diagonal preconditioner"};
     launch_apply<float, false>(M, M->L, M->ell32_L.p, nullptr, M->U, M->ell32_U.p, M->dgU32.p,
                                b, x, M->scratch32.p, sk1, sk2);
 }
@@ -842,6 +936,11 @@ void prec_trans_apply(rsgpu_prec *M, const double *b, double *x, const DevState 
 {
     if (M->kind == RSGPU_PREC_AMG) {
         amg_apply(M, b, x, st, true);
+        return;
+    }
+    // SSOR.transApply and DiagonalPreconditioner.transApply are apply
+    if (M->kind == RSGPU_PREC_SSOR || M->kind == RSGPU_PREC_DIAGONAL) {
+        prec_apply(M, b, x, st);
         return;
     }
     hipStream_t s = M->ctx->stream;

@@ -108,6 +108,7 @@ enum {
     FIN_OMEGA,       // BiCGstab: omega = (t . s) / (t . t)
     FIN_INIT_NORM,   // monitor(||r||) at iteration 0 (BiCGstab, IR)
     FIN_GM_NORMR,    // GMRES: normr = ||r||; outer monitor
+    FIN_CG_ALPHA,    // CG: alpha = rho / (p . q)
 };
 
 __device__ void finish_scalar(DevState *st, int fin, double t1, double t2)
@@ -129,6 +130,9 @@ __device__ void finish_scalar(DevState *st, int fin, double t1, double t2)
         st->normr = sqrt(t1);
         if (dev_monitor(st, st->normr))
             st->done = 1;
+        break;
+    case FIN_CG_ALPHA:
+        st->alpha = st->rho1 / t1;
         break;
     default:
         break;
@@ -299,6 +303,76 @@ __global__ void __launch_bounds__(RS_VBS)
         T ri = s[i];
         if (mo != T(0))
             ri += mo * t[i];
+        r[i] = ri;
+        a += (double)(ri * ri);
+    }
+    double v[1] = {a}, tot[1];
+    if (block_reduce_finish<RS_VBS, 1>(slot, v, tot) && threadIdx.x == 0) {
+        st->rho2 = st->rho1;
+        st->first = 0;
+        st->iter++;
+        if (dev_monitor(st, sqrt(tot[0])))
+            st->done = 1;
+    }
+}
+
+// =========================================================== CG kernels
+// rho = r . z; beta = rho / rho_1 after the first iteration (CG.java:71-78)
+template <typename T>
+__global__ void __launch_bounds__(RS_VBS)
+    k_cg_rho(int n, const T *__restrict__ r, const T *__restrict__ z, DevState *st,
+             RedSlot slot)
+{
+    if (st->done)
+        return;
+    double a = 0;
+    for (int i = blockIdx.x * RS_VBS + threadIdx.x; i < n; i += gridDim.x * RS_VBS)
+        a += (double)(r[i] * z[i]);
+    double v[1] = {a}, tot[1];
+    if (block_reduce_finish<RS_VBS, 1>(slot, v, tot) && threadIdx.x == 0) {
+        st->rho1 = tot[0];
+        if (!st->first)
+            st->beta = st->rho1 / st->rho2;
+    }
+}
+
+// first: p.set(z); then p.scale(beta).add(z) (AbstractVector.scale: beta 0
+// zeroes, beta 1 keeps; DenseVector.add adds in place)
+template <typename T>
+__global__ void __launch_bounds__(RS_VBS)
+    k_cg_p(int n, const T *__restrict__ z, T *__restrict__ p, const DevState *st)
+{
+    if (st->done)
+        return;
+    const int i = blockIdx.x * RS_VBS + threadIdx.x;
+    if (i >= n)
+        return;
+    if (st->first) {
+        p[i] = z[i];
+        return;
+    }
+    const T beta = (T)st->beta;
+    T pi = beta == T(0) ? T(0) : (beta == T(1) ? p[i] : beta * p[i]);
+    pi += z[i];
+    p[i] = pi;
+}
+
+// x.add(alpha, p); r.add(-alpha, q); monitor(||r||) (CG.java:82-87)
+template <typename T>
+__global__ void __launch_bounds__(RS_VBS)
+    k_cg_update(int n, const T *__restrict__ p, const T *__restrict__ q, T *__restrict__ x,
+                T *__restrict__ r, DevState *st, RedSlot slot)
+{
+    if (st->done)
+        return;
+    const T al = (T)st->alpha, ma = (T)(-st->alpha);
+    double a = 0;
+    for (int i = blockIdx.x * RS_VBS + threadIdx.x; i < n; i += gridDim.x * RS_VBS) {
+        if (al != T(0))
+            x[i] += al * p[i];
+        T ri = r[i];
+        if (ma != T(0))
+            ri += ma * q[i];
         r[i] = ri;
         a += (double)(ri * ri);
     }
@@ -762,6 +836,37 @@ void run_bicgstab(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const T *b,
     P.wait_last(done, cend);
 }
 
+// ------------------------------------------------------------ CG
+// work: r z p q (CG.java:51-55)
+template <typename T>
+void run_cg(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const T *b, T *x, T *work,
+            DevState *st)
+{
+    hipStream_t s = S->ctx->stream;
+    const int n = A->n;
+    const size_t L = S->ldv;
+    T *r = work, *z = work + L, *p = work + 2 * L, *q = work + 3 * L;
+    const int G = grid_for(n), E = (n + RS_VBS - 1) / RS_VBS;
+    RedSlot sl = slot_of(S);
+    // r = b - A x; monitor(||r||) at iter 0
+    launch_resid<T>(S, A, b, x, r, nullptr, nullptr, st, FIN_INIT_NORM);
+    Poller P{S, st};
+    int done = 0, cend = 0;
+    for (;;) {
+        papply(S, M, r, z, st);
+        hipLaunchKernelGGL(k_cg_rho<T>, dim3(G), dim3(RS_VBS), 0, s, n, r, z, st, sl);
+        hipLaunchKernelGGL(k_cg_p<T>, dim3(E), dim3(RS_VBS), 0, s, n, z, p, st);
+        launch_spmv_red<T>(S, A, p, EpiMultDot<T>{q, p, nullptr, 1}, st, FIN_CG_ALPHA, 0);
+        hipLaunchKernelGGL(k_cg_update<T>, dim3(G), dim3(RS_VBS), 0, s, n, p, q, x, r, st, sl);
+        RS_CHECK_LAUNCH();
+        P.post();
+        P.wait_prev(done, cend);
+        if (done)
+            break;
+    }
+    P.wait_last(done, cend);
+}
+
 // ------------------------------------------------------------ GMRES(m)
 // work: w u r + (m+1) basis vectors
 template <typename T>
@@ -854,7 +959,7 @@ extern "C" int rsgpu_solver_create(rsgpu_ctx *ctx, const rsgpu_solver_opts *opts
 {
     if (!ctx || !opts || !out || n < 0)
         return RSGPU_BAD_ARGUMENT;
-    if (opts->kind < 0 || opts->kind > 2)
+    if (opts->kind < 0 || opts->kind > 3)
         return RSGPU_BAD_ARGUMENT;
     if (opts->kind == RSGPU_GMRES && opts->restart <= 0) {
         ctx->err = "restart must be a positive integer";
@@ -883,6 +988,7 @@ extern "C" int rsgpu_solver_create(rsgpu_ctx *ctx, const rsgpu_solver_opts *opts
         if (!opts->mixed) {
             nvec64 = opts->kind == RSGPU_BICGSTAB ? 8
                      : opts->kind == RSGPU_GMRES  ? 3 + (size_t)m + 1
+                     : opts->kind == RSGPU_CG     ? 4
                                                   : 2;
         } else {
             nvec64 = 2;   // r, scratch
@@ -938,6 +1044,8 @@ void run_kind(rsgpu_solver *S, int kind, rsgpu_csr *A, rsgpu_prec *M, const T *b
 {
     if (kind == RSGPU_BICGSTAB)
         run_bicgstab<T>(S, A, M, b, x, work, st);
+    else if (kind == RSGPU_CG)
+        run_cg<T>(S, A, M, b, x, work, st);
     else if (kind == RSGPU_GMRES)
         run_gmres<T>(S, A, M, b, x, work, st, m);
     else

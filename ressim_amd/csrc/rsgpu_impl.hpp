@@ -160,9 +160,12 @@ struct rsgpu_prec {
     DevBuf<double> tmp64;        // FP32 apply on FP64 vectors
     DevBuf<float> tmpa32, tmpb32;
     int lds_rows = 0;            // rows that fit the LDS vector
-    // AMG hierarchy (amg.hip)
+    // AMG hierarchy (amg.hip); SSOR reuses it with a single smoothed level
     rsgpu_amg_params amg_params{};
     AmgData *amg = nullptr;
+    // DiagonalPreconditioner (ilu.hip)
+    DevBuf<double> invdiag;
+    DevBuf<float> invdiag32;
     ~rsgpu_prec() { amg_destroy(amg); }
 };
 
@@ -388,6 +391,9 @@ void prec_trans_apply(rsgpu_prec *M, const double *b, double *x,
 int amg_setup(rsgpu_prec *M, rsgpu_csr *A, int *bad_row);
 void amg_apply(rsgpu_prec *M, const double *b, double *x, const DevState *st, bool trans);
 int amg_levels(const rsgpu_prec *M, int *sizes, int *nnzs, int cap);
+int ssor_setup(rsgpu_prec *M, rsgpu_csr *A, int *bad_row);
+void ssor_apply(rsgpu_prec *M, const double *b, double *x, const DevState *st);
+int diag_setup(rsgpu_prec *M, rsgpu_csr *A, int *bad_row);
 int amg_get_level(const rsgpu_prec *M, int k, int32_t *rowptr, int32_t *colind,
                   double *data, int32_t *pt, int32_t *Prp, int32_t *Pci, double *Pv,
                   int *pnnz);

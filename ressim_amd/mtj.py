@@ -396,6 +396,40 @@ class BlockILU(Preconditioner):
         super().__init__(nblocks, precision, ctx)
 
 
+class DiagonalPreconditioner(Preconditioner):
+    """mtj/.../sparse/DiagonalPreconditioner.java: x = b / diag(A)."""
+    kind = L.PREC_DIAGONAL
+
+    def __init__(self, n: Optional[int] = None, precision: int = L.FP64,
+                 ctx: Optional[Context] = None):
+        super().__init__(1, precision, ctx)
+        self.size = n
+
+    def setMatrix(self, A: CompRowMatrix):
+        if self.size is not None and A.n != self.size:
+            raise ValueError("Matrix size differs from preconditioner size")
+        super().setMatrix(A)
+
+
+class SSOR(Preconditioner):
+    """mtj/.../sparse/SSOR.java: SSOR(F, reverse, omegaF, omegaR); ``SSOR()``
+    is SSOR(F) = reverse sweep, omega 1.  apply(b, x) starts from x's old
+    content.  block_rows / ordering: as the AMG smoother (0, 0 = SSOR.java)."""
+    kind = L.PREC_SSOR
+
+    def __init__(self, F: Optional[CompRowMatrix] = None, reverse: bool = True,
+                 omegaF: float = 1.0, omegaR: float = 1.0, block_rows: int = 0,
+                 ordering: int = 0, ctx: Optional[Context] = None):
+        if F is not None and not F.isSquare():
+            raise ValueError("SSOR only applies to square matrices")
+        super().__init__(1, L.FP64, ctx or (F.ctx if F is not None else None))
+        self.setOmega(omegaF, omegaR, reverse, block_rows, ordering)
+
+    def setOmega(self, omegaF, omegaR, reverse=True, block_rows=0, ordering=0):
+        _raise(L.load().rsgpu_prec_set_ssor(self.h, omegaF, omegaR, int(bool(reverse)),
+                                            block_rows, ordering), self.ctx.h)
+
+
 class AMG(Preconditioner):
     """Smoothed-aggregation AMG (mtj/.../sparse/AMG.java).  The positional
     arguments are AMG(omegaPreF, omegaPreR, omegaPostF, omegaPostR, nu1, nu2,
@@ -586,6 +620,11 @@ class AbstractIterativeSolver:
 class BiCGstab(AbstractIterativeSolver):
     """mtj/.../sparse/BiCGstab.java."""
     kind = L.BICGSTAB
+
+
+class CG(AbstractIterativeSolver):
+    """mtj/.../sparse/CG.java."""
+    kind = L.CG
 
 
 class GMRES(AbstractIterativeSolver):

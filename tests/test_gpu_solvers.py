@@ -166,3 +166,68 @@ def test_solve_device_vectors_repeat(gpu, oracle, c1):
         xo, so, _ = oracle.solve("bicgstab", rp, ci, d * scale, b, rtol=1e-12, prec="ilu")
         x = xd.cpu().numpy()
         assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-8
+
+
+# ---------------------------------------------------------------- CG, diagonal, SSOR
+def _spd(n1=24, eps=0.3):
+    I = sp.identity(n1)
+    T = sp.diags([-np.ones(n1 - 1), 2 * np.ones(n1), -np.ones(n1 - 1)], [-1, 0, 1])
+    D = sp.diags(np.exp(np.random.default_rng(7).standard_normal(n1 * n1)))
+    A = (D @ (sp.kron(I, T) + eps * sp.kron(T, I)) @ D).tocsr()
+    A.sort_indices()
+    return A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.copy()
+
+
+@pytest.mark.parametrize("prec", ["none", "diagonal", "ilu"])
+def test_cg_vs_oracle(gpu, oracle, prec):
+    rp, ci, v = _spd()
+    n = len(rp) - 1
+    b = np.random.default_rng(3).standard_normal(n)
+    xo, so, ho = oracle.solve("cg", rp, ci, v, b, rtol=1e-12, prec=prec, max_iter=5000)
+    assert so.status == 0
+    A = gpu.CompRowMatrix.from_csr(n, n, rp, ci, v)
+    M = {"none": None, "diagonal": gpu.DiagonalPreconditioner(n), "ilu": gpu.ILU()}[prec]
+    if M is not None:
+        M.setMatrix(A)
+    s = gpu.CG(b)
+    s.setIterationMonitor(gpu.DefaultIterationMonitor(5000, 1e-12, 1e-50, 1e5))
+    if M is not None:
+        s.setPreconditioner(M)
+    x = np.zeros(n)
+    s.solve(A, b, x)
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-8
+    it = s.getIterationMonitor()
+    assert abs(it.iterations() - so.iterations) <= max(2, so.iterations // 10)
+    np.testing.assert_allclose(it.history[:3], ho[:3], rtol=1e-10)
+
+
+def test_diagonal_apply_and_zero_diagonal(gpu):
+    rp, ci, v = _spd(8)
+    n = len(rp) - 1
+    A = gpu.CompRowMatrix.from_csr(n, n, rp, ci, v)
+    M = gpu.DiagonalPreconditioner(n)
+    M.setMatrix(A)
+    b = np.random.default_rng(1).standard_normal(n)
+    d = sp.csr_matrix((v, ci, rp)).diagonal()
+    np.testing.assert_array_equal(M.apply(b, np.zeros(n)), b * (1 / d))
+    B = sp.csr_matrix(np.array([[2.0, 1.0], [1.0, 0.0]]))
+    A2 = gpu.CompRowMatrix.from_csr(2, 2, B.indptr, B.indices, B.data)
+    with pytest.raises(RuntimeError, match="Zero diagonal on row 2"):
+        gpu.DiagonalPreconditioner(2).setMatrix(A2)
+
+
+@pytest.mark.parametrize("block_rows,ordering", [(0, 0), (0, 1), (40, 0), (40, 1)])
+def test_ssor_apply_bit_exact(gpu, oracle, block_rows, ordering):
+    """SSOR.apply(b, x) from x's old content, vs the oracle restatement."""
+    s = oracle_system(oracle, P.config1(nx=24, ny=20))
+    rp, ci, v = s["rowptr"], s["colind"], s["data"]
+    n = len(rp) - 1
+    A = gpu.CompRowMatrix.from_csr(n, n, rp, ci, v)
+    M = gpu.SSOR(A, True, 1.3, 0.7, block_rows=block_rows, ordering=ordering)
+    M.setMatrix(A)
+    rng = np.random.default_rng(2)
+    b, x0 = rng.standard_normal(n), rng.standard_normal(n)
+    ref = oracle.ssor_apply(rp, ci, v, b, x0, omega_f=1.3, omega_r=0.7, reverse=True,
+                            block_rows=block_rows, ordering=ordering)
+    assert np.array_equal(M.apply(b, x0.copy()), ref)
+    assert np.array_equal(M.transApply(b, x0.copy()), ref)

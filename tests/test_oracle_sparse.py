@@ -9,6 +9,7 @@ protocol), and the failure reasons of DefaultIterationMonitor and BiCGstab.
 import numpy as np
 import pytest
 import scipy.sparse as sp
+import scipy.sparse.linalg  # noqa: F401
 
 from ressim_amd import problems as P
 from tests.helpers import oracle_system, pentadiag, random_csr, scipy_csr, tridiag
@@ -224,3 +225,48 @@ def test_monitor_failure_reasons(oracle):
                                rtol=1e-12, prec="none")
     assert st.status == oracle.BREAKDOWN and st.iterations == 1
     np.testing.assert_allclose(hist, [np.sqrt(2)] * 3)
+
+
+# ---------------------------------------------------------------- CG, diagonal, SSOR
+def _spd(n1=20, eps=0.3):
+    """Anisotropic 2-D Laplacian with a badly scaled diagonal (D A D)."""
+    I = sp.identity(n1)
+    T = sp.diags([-np.ones(n1 - 1), 2 * np.ones(n1), -np.ones(n1 - 1)], [-1, 0, 1])
+    D = sp.diags(np.exp(np.random.default_rng(7).standard_normal(n1 * n1)))
+    A = (D @ (sp.kron(I, T) + eps * sp.kron(T, I)) @ D).tocsr()
+    A.sort_indices()
+    return A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.copy()
+
+
+@pytest.mark.parametrize("prec", ["none", "diagonal", "ilu"])
+def test_cg_solves_spd(oracle, prec):
+    """CG.java on an SPD matrix: solution equals spsolve; preconditioned CG
+    takes fewer iterations than plain CG."""
+    rp, ci, v = _spd()
+    n = len(rp) - 1
+    b = np.random.default_rng(3).standard_normal(n)
+    x, st, h = oracle.solve("cg", rp, ci, v, b, rtol=1e-12, prec=prec, max_iter=5000)
+    assert st.status == 0
+    xe = sp.linalg.spsolve(sp.csr_matrix((v, ci, rp)).tocsc(), b)
+    assert np.linalg.norm(x - xe) / np.linalg.norm(xe) < 1e-9
+    if prec != "none":
+        _, st0, _ = oracle.solve("cg", rp, ci, v, b, rtol=1e-12, prec="none", max_iter=5000)
+        assert st.iterations < st0.iterations
+
+
+def test_cg_finite_termination(oracle):
+    """A diagonal SPD matrix with k distinct eigenvalues: CG converges in
+    k iterations."""
+    d = np.repeat([1.0, 2.0, 5.0], 10)
+    A = sp.diags(d).tocsr()
+    rp, ci, v = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.copy()
+    x, st, _ = oracle.solve("cg", rp, ci, v, np.ones(30), rtol=1e-12, prec="none")
+    assert st.status == 0 and st.iterations == 3
+    np.testing.assert_allclose(x, 1 / d, rtol=1e-12)
+
+
+def test_diagonal_preconditioner_zero_diagonal(oracle):
+    A = sp.csr_matrix(np.array([[2.0, 1.0], [1.0, 0.0]]))
+    rp, ci, v = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.copy()
+    x, st, _ = oracle.solve("cg", rp, ci, v, np.ones(2), prec="diagonal")
+    assert st.status == oracle.ZERO_PIVOT and st.bad_row == 1
