@@ -74,7 +74,10 @@ def main():
     ap.add_argument("--adjoint", action="store_true",
                     help="measure the adjoint gradient workload (64x64x32 well-rate mismatch) "
                          "instead of the pressure solve and print its own JSON line")
-    ap.add_argument("--probe", choices=["spmv", "ilu", "solve"], default=None,
+    ap.add_argument("--restart", type=int, default=30)
+    ap.add_argument("--mgs", action="store_true",
+                    help="modified Gram-Schmidt in the FP32 inner GMRES (default: classical)")
+    ap.add_argument("--probe", choices=["spmv", "ilu", "solve", "mixed"], default=None,
                     help="profiling probe: run only the named kernel(s) on the workload "
                          "(plus a 512 MiB device copy as the traffic reference) and exit")
     a = ap.parse_args()
@@ -120,8 +123,9 @@ def main():
     g64 = mtj.GMRES(b, 30)
     g64.setIterationMonitor(mtj.DefaultIterationMonitor(20000, a.rtol, 1e-50, 1e5))
     g64.setPreconditioner(M64)
-    gmx = mtj.MixedPrecisionSolver(b, inner="gmres", restart=30, inner_rtol=a.inner_rtol,
-                                   inner_max_iter=600)
+    gmx = mtj.MixedPrecisionSolver(b, inner="gmres", restart=a.restart,
+                                   inner_rtol=a.inner_rtol, inner_max_iter=600,
+                                   cgs=not a.mgs)
     gmx.setIterationMonitor(mtj.DefaultIterationMonitor(200, a.rtol, 1e-50, 1e5))
     gmx.setPreconditioner(M32)
 
@@ -350,6 +354,19 @@ def probe(a, A, n, nnz, lib, mtj, _lib, ctx, stream, dev):
         torch.cuda.synchronize()
         for _ in range(reps):
             lib.rsgpu_csr_mult_f32(A.h, x32.data_ptr(), y32.data_ptr())
+    elif a.probe == "mixed":
+        A.sync_f32()
+        M = mtj.BlockILU(a.nblocks, precision=_lib.FP32, ctx=ctx)
+        M.setMatrix(A)
+        g = mtj.MixedPrecisionSolver(xin, inner="gmres", restart=a.restart,
+                                     inner_rtol=a.inner_rtol, inner_max_iter=600,
+                                     cgs=not a.mgs)
+        g.setIterationMonitor(mtj.DefaultIterationMonitor(200, a.rtol, 1e-50, 1e5))
+        g.setPreconditioner(M)
+        bb = torch.ones(n, dtype=torch.float64, device=dev)
+        y.zero_()
+        g.solve(A, bb, y)
+        log(f"probe mixed: inner iterations {g.last_stats.inner_iterations}")
     elif a.probe == "ilu":
         M = mtj.BlockILU(a.nblocks, precision=_lib.FP64, ctx=ctx)
         M.setMatrix(A)

@@ -219,6 +219,11 @@ typedef struct {
      * preconditioner set on the original matrix, applied through
      * transApply (ILU.java:72-78), i.e. (LU)^-T. */
     int prec_transpose;
+    /* Mixed form only: the FP32 inner GMRES orthogonalises with classical
+     * Gram-Schmidt (all projections of a column from one pass) instead of
+     * MTJ's modified Gram-Schmidt (GMRES.java:150-153); no reference
+     * counterpart.  The FP64 refinement still tests the true residual. */
+    int cgs;
 } rsgpu_solver_opts;
 
 typedef struct {

@@ -27,7 +27,7 @@ class SolverOpts(C.Structure):
                 ("rtol", C.c_double), ("atol", C.c_double), ("dtol", C.c_double),
                 ("mixed", C.c_int), ("inner_max_iter", C.c_int),
                 ("inner_rtol", C.c_double), ("check_every", C.c_int),
-                ("prec_transpose", C.c_int)]
+                ("prec_transpose", C.c_int), ("cgs", C.c_int)]
 
 
 class AmgParams(C.Structure):

@@ -413,6 +413,60 @@ __global__ void __launch_bounds__(RS_VBS)
     }
 }
 
+// H(i+1,i) = ||w|| from its square, previous Givens rotations on column i,
+// the new rotation (GivensRotation(x, y), GivensRotation.java:42-60), the
+// rotated rhs, iteration count and the inner monitor on |s_{i+1}|.
+__device__ void gm_finish_column(DevState *st, int i, double wnorm2)
+{
+    const int ld = st->m + 1;
+    double *Hc = st->H + i * ld;
+    Hc[i + 1] = sqrt(wnorm2);
+    st->hsub = Hc[i + 1];
+    // previous rotations on column i
+    for (int q = 0; q < i; ++q) {
+        const double c = st->gc[q], s = st->gs[q];
+        const double temp = c * Hc[q] + s * Hc[q + 1];
+        Hc[q + 1] = -s * Hc[q] + c * Hc[q + 1];
+        Hc[q] = temp;
+    }
+    // new rotation (GivensRotation(x, y))
+    const double x = Hc[i], y = Hc[i + 1];
+    const double roe = fabs(x) > fabs(y) ? x : y;
+    const double scale = fabs(x) + fabs(y);
+    double c, s;
+    if (scale != 0) {
+        const double xs = x / scale, ys = y / scale;
+        double rr = scale * sqrt(xs * xs + ys * ys);
+        if (roe < 0)
+            rr *= -1;
+        c = x / rr;
+        s = y / rr;
+    } else {
+        c = 1;
+        s = 0;
+    }
+    st->gc[i] = c;
+    st->gs[i] = s;
+    {
+        const double temp = c * Hc[i] + s * Hc[i + 1];
+        Hc[i + 1] = -s * Hc[i] + c * Hc[i + 1];
+        Hc[i] = temp;
+    }
+    {
+        double *sv = st->s;
+        const double temp = c * sv[i] + s * sv[i + 1];
+        sv[i + 1] = -s * sv[i] + c * sv[i + 1];
+        sv[i] = temp;
+    }
+    st->iter++;
+    st->gi = i + 1;
+    if (!(i + 1 < st->m) || dev_monitor(st, fabs(st->s[i + 1]))) {
+        st->cycle_end = 1;
+        if (st->status)
+            st->done = 1;
+    }
+}
+
 // MGS step k of column i: w -= H(k-1,i) v_{k-1} (k > 0), H(k,i) = w . v_k
 template <typename T>
 __global__ void __launch_bounds__(RS_VBS)
@@ -499,54 +553,144 @@ __global__ void __launch_bounds__(RS_VBS)
         a += (double)(wj * wj);
     }
     double v[1] = {a}, tot[1];
-    if (block_reduce_finish<RS_VBS, 1>(slot, v, tot) && threadIdx.x == 0) {
-        double *Hc = st->H + i * ld;
-        Hc[i + 1] = sqrt(tot[0]);
-        st->hsub = Hc[i + 1];
-        // previous rotations on column i
-        for (int q = 0; q < i; ++q) {
-            const double c = st->gc[q], s = st->gs[q];
-            const double temp = c * Hc[q] + s * Hc[q + 1];
-            Hc[q + 1] = -s * Hc[q] + c * Hc[q + 1];
-            Hc[q] = temp;
+    if (block_reduce_finish<RS_VBS, 1>(slot, v, tot) && threadIdx.x == 0)
+        gm_finish_column(st, i, tot[0]);
+}
+
+// Classical Gram-Schmidt for the FP32 inner solve of the mixed form (no
+// reference counterpart): all projections of column i from one pass over
+// w and the basis, H(k,i) = w . v_k for k = 0..i, reduced per k in a fixed
+// order by the finishing group.
+#define RS_CGS_CHUNK 8
+template <typename T>
+__global__ void __launch_bounds__(RS_VBS)
+    k_gm_cgs_dots(int n, int i, const T *__restrict__ w, const T *__restrict__ V, size_t ldv,
+                  DevState *st, RedSlot slot)
+{
+    if (stopped(st, true))
+        return;
+    using VT = typename V16<T>::type;
+    constexpr int N = V16<T>::N;
+    __shared__ double sh[RS_VBS / 64 + 1];
+    __shared__ int last;
+    const int cnt = i + 1, ld = st->m + 1;
+    const int G = (int)gridDim.x;
+    const int nv = n / N;
+    const int stride = G * RS_VBS;
+    for (int c0 = 0; c0 < cnt; c0 += RS_CGS_CHUNK) {
+        const int cc = min(RS_CGS_CHUNK, cnt - c0);
+        double a[RS_CGS_CHUNK];
+#pragma unroll
+        for (int q = 0; q < RS_CGS_CHUNK; ++q)
+            a[q] = 0;
+        for (int j = blockIdx.x * RS_VBS + threadIdx.x; j < nv; j += stride) {
+            const VT wv = reinterpret_cast<const VT *>(w)[j];
+            const T *we = reinterpret_cast<const T *>(&wv);
+#pragma unroll
+            for (int q = 0; q < RS_CGS_CHUNK; ++q)
+                if (q < cc) {
+                    const VT kv = reinterpret_cast<const VT *>(V + (size_t)(c0 + q) * ldv)[j];
+                    const T *ke = reinterpret_cast<const T *>(&kv);
+#pragma unroll
+                    for (int c = 0; c < N; ++c)
+                        a[q] += (double)(we[c] * ke[c]);
+                }
         }
-        // new rotation (GivensRotation(x, y))
-        const double x = Hc[i], y = Hc[i + 1];
-        const double roe = fabs(x) > fabs(y) ? x : y;
-        const double scale = fabs(x) + fabs(y);
-        double c, s;
-        if (scale != 0) {
-            const double xs = x / scale, ys = y / scale;
-            double rr = scale * sqrt(xs * xs + ys * ys);
-            if (roe < 0)
-                rr *= -1;
-            c = x / rr;
-            s = y / rr;
-        } else {
-            c = 1;
-            s = 0;
-        }
-        st->gc[i] = c;
-        st->gs[i] = s;
-        {
-            const double temp = c * Hc[i] + s * Hc[i + 1];
-            Hc[i + 1] = -s * Hc[i] + c * Hc[i + 1];
-            Hc[i] = temp;
-        }
-        {
-            double *sv = st->s;
-            const double temp = c * sv[i] + s * sv[i + 1];
-            sv[i + 1] = -s * sv[i] + c * sv[i + 1];
-            sv[i] = temp;
-        }
-        st->iter++;
-        st->gi = i + 1;
-        if (!(i + 1 < st->m) || dev_monitor(st, fabs(st->s[i + 1]))) {
-            st->cycle_end = 1;
-            if (st->status)
-                st->done = 1;
+        for (int j = nv * N + blockIdx.x * RS_VBS + threadIdx.x; j < n; j += stride)
+#pragma unroll
+            for (int q = 0; q < RS_CGS_CHUNK; ++q)
+                if (q < cc)
+                    a[q] += (double)(w[j] * V[(size_t)(c0 + q) * ldv + j]);
+        for (int q = 0; q < cc; ++q) {
+            const double r = block_sum<RS_VBS>(a[q], sh);
+            if (threadIdx.x == 0)
+                __hip_atomic_store(&slot.partials[(size_t)(c0 + q) * G + blockIdx.x], r,
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
+    // arrival ticket (as block_reduce_finish)
+    if (threadIdx.x == 0) {
+        wait_vm_stores();
+        const int ng = G < RS_NGROUP ? G : RS_NGROUP;
+        const int g = (int)blockIdx.x % ng;
+        const int gsize = (G - g + ng - 1) / ng;
+        unsigned *gt = slot.ticket + (1 + g) * RS_TPAD;
+        int fin = 0;
+        const unsigned t =
+            __hip_atomic_fetch_add(gt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((int)t == gsize - 1) {
+            __hip_atomic_store(gt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned u = __hip_atomic_fetch_add(slot.ticket, 1u, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT);
+            if ((int)u == ng - 1) {
+                __hip_atomic_store(slot.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                fin = 1;
+            }
+        }
+        last = fin;
+    }
+    __syncthreads();
+    if (!last)
+        return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    for (int k = 0; k < cnt; ++k) {
+        double acc = 0;
+        for (int b = threadIdx.x; b < G; b += RS_VBS)
+            acc += __hip_atomic_load(&slot.partials[(size_t)k * G + b], __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+        acc = block_sum<RS_VBS>(acc, sh);
+        if (threadIdx.x == 0)
+            st->H[k + i * ld] = acc;
+    }
+}
+
+// w -= sum_k H(k,i) v_k (each element in k order, add's alpha == 0 skip),
+// then H(i+1,i) = ||w||, Givens, monitor
+template <typename T>
+__global__ void __launch_bounds__(RS_VBS)
+    k_gm_cgs_last(int n, int i, T *__restrict__ w, const T *__restrict__ V, size_t ldv,
+                  DevState *st, RedSlot slot)
+{
+    if (stopped(st, true))
+        return;
+    using VT = typename V16<T>::type;
+    constexpr int N = V16<T>::N;
+    const int ld = st->m + 1;
+    const double *Hc = st->H + i * ld;
+    double a = 0;
+    const int nv = n / N;
+    const int stride = gridDim.x * RS_VBS;
+    for (int j = blockIdx.x * RS_VBS + threadIdx.x; j < nv; j += stride) {
+        VT wv = reinterpret_cast<const VT *>(w)[j];
+        T *we = reinterpret_cast<T *>(&wv);
+        for (int k = 0; k <= i; ++k) {
+            const T h = (T)(-Hc[k]);
+            if (h == T(0))
+                continue;
+            const VT kv = reinterpret_cast<const VT *>(V + (size_t)k * ldv)[j];
+            const T *ke = reinterpret_cast<const T *>(&kv);
+#pragma unroll
+            for (int c = 0; c < N; ++c)
+                we[c] += h * ke[c];
+        }
+        reinterpret_cast<VT *>(w)[j] = wv;
+#pragma unroll
+        for (int c = 0; c < N; ++c)
+            a += (double)(we[c] * we[c]);
+    }
+    for (int j = nv * N + blockIdx.x * RS_VBS + threadIdx.x; j < n; j += stride) {
+        T wj = w[j];
+        for (int k = 0; k <= i; ++k) {
+            const T h = (T)(-Hc[k]);
+            if (h != T(0))
+                wj += h * V[(size_t)k * ldv + j];
+        }
+        w[j] = wj;
+        a += (double)(wj * wj);
+    }
+    double v[1] = {a}, tot[1];
+    if (block_reduce_finish<RS_VBS, 1>(slot, v, tot) && threadIdx.x == 0)
+        gm_finish_column(st, i, tot[0]);
 }
 
 // v_{i+1} = (1/H(i+1,i)) w
@@ -896,12 +1040,19 @@ void run_gmres(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const T *b, T *x,
                                A->plan.wg_rows.p, A->rowptr.p, A->colind.p,
                                csr_values<T>(A), vi, u, st, 1);
             papply(S, M, u, w, st);
-            for (int k = 0; k <= i; ++k)
-                hipLaunchKernelGGL(k_gm_mgs<T>, dim3(G), dim3(RS_VBS), 0, s, n, i,
-                                   k, w, k > 0 ? V + (size_t)(k - 1) * L : V,
-                                   V + (size_t)k * L, st, sl);
-            hipLaunchKernelGGL(k_gm_last<T>, dim3(G), dim3(RS_VBS), 0, s, n, i, w,
-                               vi, st, sl);
+            if (S->o.cgs && sizeof(T) == 4) {
+                hipLaunchKernelGGL(k_gm_cgs_dots<T>, dim3(G), dim3(RS_VBS), 0, s, n, i, w, V, L,
+                                   st, sl);
+                hipLaunchKernelGGL(k_gm_cgs_last<T>, dim3(G), dim3(RS_VBS), 0, s, n, i, w, V, L,
+                                   st, sl);
+            } else {
+                for (int k = 0; k <= i; ++k)
+                    hipLaunchKernelGGL(k_gm_mgs<T>, dim3(G), dim3(RS_VBS), 0, s, n, i, k, w,
+                                       k > 0 ? V + (size_t)(k - 1) * L : V, V + (size_t)k * L,
+                                       st, sl);
+                hipLaunchKernelGGL(k_gm_last<T>, dim3(G), dim3(RS_VBS), 0, s, n, i, w, vi, st,
+                                   sl);
+            }
             hipLaunchKernelGGL(k_gm_scale<T>, dim3(E), dim3(RS_VBS), 0, s, n, w,
                                V + (size_t)(i + 1) * L, st);
             RS_CHECK_LAUNCH();
@@ -952,6 +1103,7 @@ extern "C" void rsgpu_solver_opts_default(rsgpu_solver_opts *o)
     o->inner_rtol = 1e-4;
     o->check_every = 4;
     o->prec_transpose = 0;
+    o->cgs = 0;
 }
 
 extern "C" int rsgpu_solver_create(rsgpu_ctx *ctx, const rsgpu_solver_opts *opts,
@@ -1195,8 +1347,9 @@ extern "C" int rsgpu_solver_solve(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M,
         RS_HIP(hipSetDevice(ctx->device));
         const int m = std::max(1, S->o.restart);
         const size_t need = (size_t)2 * std::max(A->plan.nwg, grid_for(A->n)) + 2;
-        if (S->partials.n < need + 4096)
-            S->partials.alloc(need + 4096);
+        const size_t need_cgs = (size_t)(m + 2) * grid_for(A->n);
+        if (S->partials.n < std::max(need, need_cgs) + 4096)
+            S->partials.alloc(std::max(need, need_cgs) + 4096);
         state_init(S, S->st, S->hist, S->o.max_iter, S->o.rtol, S->o.atol,
                    S->o.dtol, m, S->gmres_small.p);
         if (stats)

@@ -536,6 +536,8 @@ class AbstractIterativeSolver:
         # transposed solve: A' x = b with the preconditioner of A applied by
         # transApply (the adjoint's solve)
         self.prec_transpose = False
+        # mixed form: classical Gram-Schmidt in the FP32 inner GMRES
+        self.cgs = False
         self._h = None
         self._key = None
         self.last_stats = None
@@ -555,7 +557,7 @@ class AbstractIterativeSolver:
     def _handle(self, ctx: Context):
         key = (self.kind, self.restart, self.iter.maxIter, self.iter.rtol, self.iter.atol,
                self.iter.dtol, self.mixed, self.inner_rtol, self.inner_max_iter,
-               self.check_every, self.prec_transpose, id(ctx))
+               self.check_every, self.prec_transpose, self.cgs, id(ctx))
         if self._h is not None and key == self._key:
             return self._h
         lib = L.load()
@@ -570,6 +572,7 @@ class AbstractIterativeSolver:
         o.inner_rtol, o.inner_max_iter = self.inner_rtol, self.inner_max_iter
         o.check_every = self.check_every
         o.prec_transpose = int(self.prec_transpose)
+        o.cgs = int(self.cgs)
         h = C.c_void_p()
         _raise(lib.rsgpu_solver_create(ctx.h, C.byref(o), self.n, C.byref(h)), ctx.h)
         self._h, self._key = h, key
@@ -653,12 +656,13 @@ class MixedPrecisionSolver(AbstractIterativeSolver):
     No reference counterpart; the outer loop has IR.java's form."""
 
     def __init__(self, template, inner: str = "gmres", restart: int = 30,
-                 inner_rtol: float = 1e-4, inner_max_iter: int = 200):
+                 inner_rtol: float = 1e-4, inner_max_iter: int = 200, cgs: bool = False):
         super().__init__(template, restart)
         self.kind = {"gmres": L.GMRES, "bicgstab": L.BICGSTAB}[inner]
         self.mixed = True
         self.inner_rtol = inner_rtol
         self.inner_max_iter = inner_max_iter
+        self.cgs = cgs
 
 
 # ------------------------------------------------------------------ MPFA-O

@@ -92,7 +92,8 @@ def test_ir_diag_dominant(gpu, oracle):
     assert solver.getIterationMonitor().iterations() == so.iterations
 
 
-def test_mixed_precision(gpu, oracle, s3):
+@pytest.mark.parametrize("cgs", [False, True])
+def test_mixed_precision(gpu, oracle, s3, cgs):
     s = s3
     xo, so, _ = oracle.solve("gmres", s["rowptr"], s["colind"], s["data"], s["b"],
                              rtol=1e-13, prec="ilu")
@@ -103,7 +104,7 @@ def test_mixed_precision(gpu, oracle, s3):
     A.sync_f32()
     M = gpu.BlockILU(4, precision=_lib.FP32)
     M.setMatrix(A)
-    solver = gpu.MixedPrecisionSolver(b, inner="gmres", inner_rtol=1e-4)
+    solver = gpu.MixedPrecisionSolver(b, inner="gmres", inner_rtol=1e-4, restart=20, cgs=cgs)
     solver.setPreconditioner(M)
     solver.setIterationMonitor(gpu.DefaultIterationMonitor(100, 1e-10, 1e-50, 1e5))
     x = np.zeros(n)
