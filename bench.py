@@ -77,7 +77,7 @@ def main():
     ap.add_argument("--restart", type=int, default=30)
     ap.add_argument("--mgs", action="store_true",
                     help="modified Gram-Schmidt in the FP32 inner GMRES (default: classical)")
-    ap.add_argument("--probe", choices=["spmv", "ilu", "solve", "mixed"], default=None,
+    ap.add_argument("--probe", choices=["spmv", "ilu", "ilu32", "solve", "mixed"], default=None,
                     help="profiling probe: run only the named kernel(s) on the workload "
                          "(plus a 512 MiB device copy as the traffic reference) and exit")
     a = ap.parse_args()
@@ -372,6 +372,14 @@ def probe(a, A, n, nnz, lib, mtj, _lib, ctx, stream, dev):
         M.setMatrix(A)
         for _ in range(max(5, reps // 5)):
             lib.rsgpu_prec_apply(M.h, xin.data_ptr(), y.data_ptr())
+    elif a.probe == "ilu32":
+        M = mtj.BlockILU(a.nblocks, precision=_lib.FP32, ctx=ctx)
+        M.setMatrix(A)
+        x32 = xin.float()
+        y32 = torch.empty(n, dtype=torch.float32, device=dev)
+        torch.cuda.synchronize()
+        for _ in range(max(5, reps // 5)):
+            lib.rsgpu_prec_apply_f32(M.h, x32.data_ptr(), y32.data_ptr())
     else:
         M = mtj.BlockILU(a.nblocks, precision=_lib.FP64, ctx=ctx)
         g = mtj.GMRES(xin, 30)

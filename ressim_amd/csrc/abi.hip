@@ -115,7 +115,7 @@ extern "C" int rsgpu_csr_create(rsgpu_ctx *ctx, int n, int ncols, int nnz,
     A->colind.upload(A->h_colind, s);
     A->val.alloc(std::max(nnz, 1));
     A->val.zero(s);
-    spmv_build_plan(A->h_rowptr, n, A->plan, s);
+    spmv_build_plans(A->h_rowptr, n, A, s);
     A->pattern_id = next_pattern_id();
     RS_HIP(hipStreamSynchronize(s));
     *out = A;
@@ -156,7 +156,7 @@ extern "C" int rsgpu_csr_transpose(const rsgpu_csr *A, rsgpu_csr **out)
     DevBuf<int> perm;
     perm.upload(tpm, s);
     csr_launch_gather(s, perm.p, A->val.p, T->val.p, nnz);
-    spmv_build_plan(T->h_rowptr, m, T->plan, s);
+    spmv_build_plans(T->h_rowptr, m, T, s);
     T->pattern_id = next_pattern_id();
     T->transpose_of = A->pattern_id;
     RS_HIP(hipStreamSynchronize(s));

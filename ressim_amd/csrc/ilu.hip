@@ -29,6 +29,8 @@
 // keeps four levels of operands in flight (depth-4 register pipeline); the
 // barrier between levels only orders LDS traffic, so the loads cross it.
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 
 #include "rsgpu_impl.hpp"
 
@@ -503,58 +505,103 @@ struct alignas(16) CRow {
 template <typename T, int W>
 struct LevelRegs {
     int r;
+    bool act;
     T d;
     VRow<T, W> v;
     CRow<W> c;
 };
 
 // All loads of a level are independent of each other and of the vector:
-// row index, level-ordered diagonal, and the lane's W values / columns.
-// Inactive lanes redo the level's last row and write it to the junk slot
+// row index, the lane's W columns and values, level-ordered diagonal (in
+// that order, so one wait covers every gather address).  Inactive lanes
+// (t >= count, or every lane of a level index past the sweep's end) redo a
+// real row of the clamped level and write the result to the junk slot
 // nb + 1, so neither loads nor updates sit behind branches (branches make
-// the compiler's wait counting fall back to waiting for every load).
+// the compiler's wait counting fall back to waiting for every load, and
+// let it sink prefetches next to their use).
 template <typename T, int W, bool DIV>
-__device__ __forceinline__ void level_load(const LevelMeta &m, const int *__restrict__ rows,
+__device__ __forceinline__ void level_load(const LevelMeta &m, bool valid,
+                                           const int *__restrict__ rows,
                                            const int *__restrict__ ecol,
                                            const T *__restrict__ eval,
-                                           const T *__restrict__ dgl, int junk,
-                                           LevelRegs<T, W> &R)
+                                           const T *__restrict__ dgl, LevelRegs<T, W> &R)
 {
+    // the metadata is wave-uniform: scalar bases, 32-bit lane offsets
+    const int count = __builtin_amdgcn_readfirstlane(m.count);
+    const int off = __builtin_amdgcn_readfirstlane(m.off);
+    const int roff = __builtin_amdgcn_readfirstlane(m.roff);
     const int t = threadIdx.x;
-    const bool act = t < m.count;
-    const int tt = act ? t : m.count - 1;
-    const int r = rows[m.roff + tt];
-    R.r = act ? r : -1 - r;   // inactive: -1 - (row to read)
-    (void)junk;
+    const bool act = valid && t < count;
+    const int tt = t < count ? t : count - 1;
+    // the row index is only consumed by the compute stage: nothing here may
+    // depend on a loaded value, or the prefetch would wait for itself
+    R.r = (rows + roff)[tt];
+    R.act = act;
+    R.c = reinterpret_cast<const CRow<W> *>(ecol + off)[tt];
+    R.v = reinterpret_cast<const VRow<T, W> *>(eval + off)[tt];
     if (DIV)
-        R.d = dgl[m.roff + tt];
-    const size_t q = (size_t)m.off + (size_t)tt * W;
-    R.v = *reinterpret_cast<const VRow<T, W> *>(eval + q);
-    R.c = *reinterpret_cast<const CRow<W> *>(ecol + q);
+        R.d = (dgl + roff)[tt];
 }
 
-template <typename T, int W, int FORM, bool DIV>
+// Every gathered operand of a level passes through one empty asm statement
+// as an in/out register: all the level's LDS reads are issued before it and
+// one wait covers them, instead of the scheduler interleaving arithmetic
+// between batches of reads (one LDS round trip per level instead of two).
+// The statement emits no instruction and performs no memory access.
+template <typename T, int WL>
+__device__ __forceinline__ void pin_gathers(T (&x)[WL], T &r)
+{
+#define RS_P(i) "+v"(x[i])
+    if constexpr (WL == 3)
+        asm volatile("" : "+v"(r), RS_P(0), RS_P(1), RS_P(2));
+    else if constexpr (WL == 4)
+        asm volatile("" : "+v"(r), RS_P(0), RS_P(1), RS_P(2), RS_P(3));
+    else if constexpr (WL == 8)
+        asm volatile("" : "+v"(r), RS_P(0), RS_P(1), RS_P(2), RS_P(3), RS_P(4), RS_P(5), RS_P(6),
+                     RS_P(7));
+    else if constexpr (WL == 12)
+        asm volatile("" : "+v"(r), RS_P(0), RS_P(1), RS_P(2), RS_P(3), RS_P(4), RS_P(5), RS_P(6),
+                     RS_P(7), RS_P(8), RS_P(9), RS_P(10), RS_P(11));
+    else if constexpr (WL == 13)
+        asm volatile("" : "+v"(r), RS_P(0), RS_P(1), RS_P(2), RS_P(3), RS_P(4), RS_P(5), RS_P(6),
+                     RS_P(7), RS_P(8), RS_P(9), RS_P(10), RS_P(11), RS_P(12));
+    else if constexpr (WL == 16)
+        asm volatile("" : "+v"(r), RS_P(0), RS_P(1), RS_P(2), RS_P(3), RS_P(4), RS_P(5), RS_P(6),
+                     RS_P(7), RS_P(8), RS_P(9), RS_P(10), RS_P(11), RS_P(12), RS_P(13), RS_P(14),
+                     RS_P(15));
+    else
+        static_assert(WL == 3, "no pin for this width");
+#undef RS_P
+}
+
+// Only the first WL <= W slots are used: when no row of either sweep has
+// more than WL entries, slots WL..W-1 are padding everywhere, and a padding
+// update is an exact identity (sum + (-0.0 * 0) == sum, acc - (+0.0 * 0) ==
+// acc, signed zeros included), so skipping them keeps the bits.  Fewer
+// gathers let one batch of LDS reads cover the level.
+template <typename T, int W, int WL, int FORM, bool DIV>
 __device__ __forceinline__ void level_compute(const LevelRegs<T, W> &R, T *xv, int junk)
 {
-    const bool act = R.r >= 0;
-    const int rr = act ? R.r : -1 - R.r;
-    const int wr = act ? R.r : junk;
-    T xs[W];
+    static_assert(WL <= W, "used slots exceed the stored width");
+    const int wr = R.act ? R.r : junk;
+    T rhs_ = xv[R.r];
+    T xs[WL];
 #pragma unroll
-    for (int s = 0; s < W; ++s)
+    for (int s = 0; s < WL; ++s)
         xs[s] = xv[R.c.c[s]];
-    const T rhs = xv[rr];
+    pin_gathers<T, WL>(xs, rhs_);
+    const T rhs = rhs_;
     T acc;
     if (FORM == FORM_SUM) {
         T sum = 0;
 #pragma unroll
-        for (int s = 0; s < W; ++s)
+        for (int s = 0; s < WL; ++s)
             sum += R.v.v[s] * xs[s];
         acc = rhs - sum;
     } else {
         acc = rhs;
 #pragma unroll
-        for (int s = 0; s < W; ++s)
+        for (int s = 0; s < WL; ++s)
             acc -= R.v.v[s] * xs[s];
     }
     if (DIV)
@@ -562,51 +609,75 @@ __device__ __forceinline__ void level_compute(const LevelRegs<T, W> &R, T *xv, i
     xv[wr] = acc;
 }
 
-// Four register sets in rotation: while level l computes, levels l+1..l+3
-// are loading.  The prefetch index is clamped instead of branched on (the
-// tail reloads the last level), so every load and wait is unconditional.
-template <typename T, int W, int FORM, bool DIV>
+// Loads in flight per level (row index, columns, values, U diagonal).
+template <typename T, int W, bool DIV>
+constexpr int level_loads()
+{
+    return 1 + (W * 4 + 15) / 16 + (W * (int)sizeof(T) + 15) / 16 + (DIV ? 1 : 0);
+}
+
+// Prefetch depth: levels whose loads are in flight while one computes.  A
+// memory latency spans several levels of compute, so narrow rows keep up
+// to 8 levels in flight; the depth is capped by the 63 loads the wave's
+// counter can track and by the registers one level's operands occupy.
+template <typename T, int W>
+constexpr int pf_depth()
+{
+    constexpr int loads = level_loads<T, W, true>();
+    constexpr int regs = 2 + W + W * (int)sizeof(T) / 4 + (int)sizeof(T) / 4;
+    int d = 8;
+    while (d > 2 && (d * loads > 60 || d * regs > 210))
+        --d;
+    return d;
+}
+
+// DEPTH register sets in rotation: while level l computes, levels
+// l+1..l+DEPTH-1 are loading.  The stage loop is fully unrolled, so each
+// set is a fixed group of registers (no copies), and the loop body is one
+// basic block: level indices past the end load the last level again and
+// compute into the junk slot, so no branch separates a prefetch from the
+// stage it belongs to.  The level metadata a prefetch needs is read from
+// LDS one stage earlier, so its LDS round trip hides behind the previous
+// level's compute.  The kernel is one wave and a wave's LDS operations
+// complete in issue order, so levels are separated by a wave-scope fence
+// (no wait for the store to land).
+template <typename T, int W, int WL, int FORM, bool DIV>
 __device__ __forceinline__ void sweep_pf(int nl, const LevelMeta *sm,
                                          const int *__restrict__ rows,
                                          const int *__restrict__ ecol,
                                          const T *__restrict__ eval,
                                          const T *__restrict__ dgl, T *xv, int junk)
 {
+    constexpr int D = pf_depth<T, W>();
     if (nl <= 0)
         return;
     const int last = nl - 1;
-    LevelRegs<T, W> r0, r1, r2, r3;
-    level_load<T, W, DIV>(sm[0], rows, ecol, eval, dgl, junk, r0);
-    level_load<T, W, DIV>(sm[min(1, last)], rows, ecol, eval, dgl, junk, r1);
-    level_load<T, W, DIV>(sm[min(2, last)], rows, ecol, eval, dgl, junk, r2);
-// the scheduling barriers keep each prefetch issued before the compute of
-// the current level (the scheduler would otherwise sink the loads past the
-// LDS barrier and batch them, exposing a full memory latency per batch)
-#define RS_PF_STAGE(LOADREG, LOADLEV, COMPREG)                                  \
-    level_load<T, W, DIV>(sm[min((LOADLEV), last)], rows, ecol, eval, dgl, junk, \
-                          LOADREG);                                             \
-    asm volatile("" ::: "memory");                                              \
-    level_compute<T, W, FORM, DIV>(COMPREG, xv, junk);                          \
-    lds_barrier();                                                              \
-    __builtin_amdgcn_sched_barrier(0);
-    for (int l = 0; l < nl; l += 4) {
-        RS_PF_STAGE(r3, l + 3, r0)
-        if (l + 1 >= nl)
-            break;
-        RS_PF_STAGE(r0, l + 4, r1)
-        if (l + 2 >= nl)
-            break;
-        RS_PF_STAGE(r1, l + 5, r2)
-        if (l + 3 >= nl)
-            break;
-        RS_PF_STAGE(r2, l + 6, r3)
+    LevelRegs<T, W> r[D];
+#pragma unroll
+    for (int k = 0; k < D - 1; ++k)
+        level_load<T, W, DIV>(sm[min(k, last)], k <= last, rows, ecol, eval, dgl, r[k]);
+    LevelMeta mq = sm[min(D - 1, last)];
+    for (int l = 0; l < nl; l += D) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            // prefetch level l+k+D-1 into the set level l+k-1 used
+            const int lev = l + k + D - 1;
+            level_load<T, W, DIV>(mq, lev <= last, rows, ecol, eval, dgl, r[(k + D - 1) % D]);
+            mq = sm[min(lev + 1, last)];
+            // the scheduling barriers keep each prefetch issued before the
+            // compute of the current level (the scheduler would otherwise
+            // sink the loads past the compute and batch them)
+            asm volatile("" ::: "memory");
+            level_compute<T, W, WL, FORM, DIV>(r[k], xv, junk);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_sched_barrier(0);
+        }
     }
-#undef RS_PF_STAGE
 }
 
 // One wave per block (every level has at most 64 rows); vector and both
 // sweeps' level metadata in LDS.
-template <typename T, int W, bool TRANS>
+template <typename T, int W, int WL, bool TRANS>
 __global__ void __launch_bounds__(64)
     k_ilu_apply_pf(const int *__restrict__ bstart,
                    const int *__restrict__ blkA, const LevelMeta *__restrict__ metaA,
@@ -638,14 +709,154 @@ __global__ void __launch_bounds__(64)
     __syncthreads();
     const int junk = nb + 1;
     if (!TRANS) {
-        sweep_pf<T, W, FORM_SUM, false>(nla, sm, rowsA, colA, valA, dglA, xv, junk);
-        sweep_pf<T, W, FORM_SUM, true>(nlb, sm + nla, rowsB, colB, valB, dglB, xv, junk);
+        sweep_pf<T, W, WL, FORM_SUM, false>(nla, sm, rowsA, colA, valA, dglA, xv, junk);
+        sweep_pf<T, W, WL, FORM_SUM, true>(nlb, sm + nla, rowsB, colB, valB, dglB, xv, junk);
     } else {
-        sweep_pf<T, W, FORM_SUBSEQ, true>(nla, sm, rowsA, colA, valA, dglA, xv, junk);
-        sweep_pf<T, W, FORM_SUBSEQ, false>(nlb, sm + nla, rowsB, colB, valB, dglB, xv, junk);
+        sweep_pf<T, W, WL, FORM_SUBSEQ, true>(nla, sm, rowsA, colA, valA, dglA, xv, junk);
+        sweep_pf<T, W, WL, FORM_SUBSEQ, false>(nlb, sm + nla, rowsB, colB, valB, dglB, xv,
+                                               junk);
     }
     __syncthreads();
     for (int r = threadIdx.x; r < nb; r += blockDim.x)
+        x[b0 + r] = xv[r];
+}
+
+// ---------------------------------------------------------------- LDS-resident
+// The whole block segment of both sweeps (values, columns and rows as
+// 16-bit local indices, level metadata, diagonals) and the block's vector
+// are staged into LDS by the workgroup; then one wave walks the levels with
+// LDS traffic only.  The other waves exit after staging; the sweeping wave
+// orders its LDS accesses with wave-scope fences (no workgroup barrier).
+// Every row update is row_update's operation sequence (bit-identical).
+struct LdsLayout {
+    int xv, meta, vA, cA, rA, dA, vB, cB, rB, dB, bytes;
+};
+
+template <typename T, int W>
+__host__ __device__ inline LdsLayout lds_layout(int xv_cap, int maxlev, int nbmax)
+{
+    auto al = [](int v) { return (v + 15) & ~15; };
+    LdsLayout L;
+    int o = 0;
+    L.xv = o;
+    o = al(o + xv_cap * (int)sizeof(T));
+    L.meta = o;
+    o = al(o + maxlev * (int)sizeof(LevelMeta));
+    L.vA = o;
+    o = al(o + nbmax * W * (int)sizeof(T));
+    L.vB = o;
+    o = al(o + nbmax * W * (int)sizeof(T));
+    L.dA = o;
+    o = al(o + nbmax * (int)sizeof(T));
+    L.dB = o;
+    o = al(o + nbmax * (int)sizeof(T));
+    L.cA = o;
+    o = al(o + nbmax * W * 2);
+    L.cB = o;
+    o = al(o + nbmax * W * 2);
+    L.rA = o;
+    o = al(o + nbmax * 2);
+    L.rB = o;
+    o = al(o + nbmax * 2);
+    L.bytes = o;
+    return L;
+}
+
+template <typename T, int W, int FORM, bool DIV>
+__device__ __forceinline__ void sweep_lds(int nl, const LevelMeta *sm, int off0, int roff0,
+                                          const T *v, const short *c, const short *rows,
+                                          const T *dg, T *xv)
+{
+    const int t = threadIdx.x;
+    for (int l = 0; l < nl; ++l) {
+        const LevelMeta m = sm[l];
+        if (t < m.count) {
+            const int q = m.off - off0 + t * W;
+            const int rq = m.roff - roff0 + t;
+            const int r = rows[rq];
+            T acc;
+            if (FORM == FORM_SUM) {
+                T sum = 0;
+#pragma unroll
+                for (int s = 0; s < W; ++s)
+                    sum += v[q + s] * xv[c[q + s]];
+                acc = xv[r] - sum;
+            } else {
+                acc = xv[r];
+#pragma unroll
+                for (int s = 0; s < W; ++s)
+                    acc -= v[q + s] * xv[c[q + s]];
+            }
+            if (DIV)
+                acc = acc / dg[rq];
+            xv[r] = acc;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+}
+
+template <typename T, int W, bool TRANS>
+__global__ void __launch_bounds__(256)
+    k_ilu_apply_lds(const int *__restrict__ bstart, const int *__restrict__ blkA,
+                    const LevelMeta *__restrict__ metaA, const int *__restrict__ rowsA,
+                    const int *__restrict__ colA, const T *__restrict__ valA,
+                    const T *__restrict__ dglA, const int *__restrict__ blkB,
+                    const LevelMeta *__restrict__ metaB, const int *__restrict__ rowsB,
+                    const int *__restrict__ colB, const T *__restrict__ valB,
+                    const T *__restrict__ dglB, const T *__restrict__ b, T *__restrict__ x,
+                    int xv_cap, int maxlev, int nbmax, const int *skip1, const int *skip2)
+{
+    if ((skip1 && *skip1) || (skip2 && *skip2))
+        return;
+    extern __shared__ __align__(16) unsigned char rs_smem[];
+    const LdsLayout Ly = lds_layout<T, W>(xv_cap, maxlev, nbmax);
+    T *xv = (T *)(rs_smem + Ly.xv);
+    LevelMeta *sm = (LevelMeta *)(rs_smem + Ly.meta);
+    T *vA = (T *)(rs_smem + Ly.vA), *vB = (T *)(rs_smem + Ly.vB);
+    T *dA = (T *)(rs_smem + Ly.dA), *dB = (T *)(rs_smem + Ly.dB);
+    short *cA = (short *)(rs_smem + Ly.cA), *cB = (short *)(rs_smem + Ly.cB);
+    short *rA = (short *)(rs_smem + Ly.rA), *rB = (short *)(rs_smem + Ly.rB);
+    const int blk = blockIdx.x;
+    const int b0 = bstart[blk], nb = bstart[blk + 1] - b0;
+    const int la0 = blkA[blk], nla = blkA[blk + 1] - la0;
+    const int lb0 = blkB[blk], nlb = blkB[blk + 1] - lb0;
+    const int tid = threadIdx.x;
+    for (int q = tid; q < nla; q += blockDim.x)
+        sm[q] = metaA[la0 + q];
+    for (int q = tid; q < nlb; q += blockDim.x)
+        sm[nla + q] = metaB[lb0 + q];
+    // every row of the block appears once per sweep: segments of nb rows
+    // and nb * W slots starting at the first level's offsets
+    const int offA = nla ? metaA[la0].off : 0, roffA = nla ? metaA[la0].roff : 0;
+    const int offB = nlb ? metaB[lb0].off : 0, roffB = nlb ? metaB[lb0].roff : 0;
+    for (int q = tid; q < nb * W; q += blockDim.x) {
+        vA[q] = valA[offA + q];
+        cA[q] = (short)colA[offA + q];
+        vB[q] = valB[offB + q];
+        cB[q] = (short)colB[offB + q];
+    }
+    for (int q = tid; q < nb; q += blockDim.x) {
+        rA[q] = (short)rowsA[roffA + q];
+        rB[q] = (short)rowsB[roffB + q];
+        if (dglA)
+            dA[q] = dglA[roffA + q];
+        if (dglB)
+            dB[q] = dglB[roffB + q];
+        xv[q] = b[b0 + q];
+    }
+    if (tid == 0)
+        xv[nb] = T(0);
+    __syncthreads();
+    if (tid >= 64)
+        return;
+    if (!TRANS) {
+        sweep_lds<T, W, FORM_SUM, false>(nla, sm, offA, roffA, vA, cA, rA, dA, xv);
+        sweep_lds<T, W, FORM_SUM, true>(nlb, sm + nla, offB, roffB, vB, cB, rB, dB, xv);
+    } else {
+        sweep_lds<T, W, FORM_SUBSEQ, true>(nla, sm, offA, roffA, vA, cA, rA, dA, xv);
+        sweep_lds<T, W, FORM_SUBSEQ, false>(nlb, sm + nla, offB, roffB, vB, cB, rB, dB, xv);
+    }
+    for (int r = tid; r < nb; r += 64)
         x[b0 + r] = xv[r];
 }
 
@@ -719,16 +930,85 @@ int prec_factor(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
     return RSGPU_OK;
 }
 
-template <typename T, bool TRANS, int W>
+template <typename T, bool TRANS, int W, int WL = W>
 static void launch_pf(rsgpu_prec *M, const SweepData &A_, const T *va, const T *da,
                       const SweepData &B_, const T *vb, const T *db, const T *b, T *x,
                       size_t xv_bytes, size_t shm, const int *sk1, const int *sk2)
 {
-    hipLaunchKernelGGL((k_ilu_apply_pf<T, W, TRANS>), dim3(M->nblocks), dim3(64), shm,
+    // dynamic LDS beyond the 64 KiB default needs a per-kernel opt-in
+    static size_t granted = 65536;
+    if (shm > granted) {
+        RS_HIP(hipFuncSetAttribute((const void *)k_ilu_apply_pf<T, W, WL, TRANS>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        granted = shm;
+    }
+    hipLaunchKernelGGL((k_ilu_apply_pf<T, W, WL, TRANS>), dim3(M->nblocks), dim3(64), shm,
                        M->ctx->stream, M->bstart.p, A_.blk_lev.p,
                        (const LevelMeta *)A_.lev_meta.p, A_.rows.p, A_.ell_col.p, va, da,
                        B_.blk_lev.p, (const LevelMeta *)B_.lev_meta.p, B_.rows.p,
                        B_.ell_col.p, vb, db, b, x, (int)xv_bytes, sk1, sk2);
+}
+
+// Largest dynamic LDS a workgroup may request on this device (opt-in).
+static int lds_limit(int device)
+{
+    static int cached[64];
+    static bool have[64];
+    if (device < 0 || device >= 64)
+        return 65536;
+    if (!have[device]) {
+        int a = 0, b = 0;
+        (void)hipDeviceGetAttribute(&a, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
+        (void)hipDeviceGetAttribute(&b, hipDeviceAttributeSharedMemPerBlockOptin, device);
+        cached[device] = std::max(a, b);
+        have[device] = true;
+    }
+    return cached[device];
+}
+
+template <typename T, int W, bool TRANS>
+static bool try_launch_lds(rsgpu_prec *M, const SweepData &A_, const T *va, const T *da,
+                           const SweepData &B_, const T *vb, const T *db, const T *b, T *x,
+                           const int *sk1, const int *sk2)
+{
+    const int xv_cap = M->max_block_rows + 2;
+    const int maxlev = A_.max_levels + B_.max_levels;
+    const LdsLayout Ly = lds_layout<T, W>(xv_cap, maxlev, M->max_block_rows);
+    if (Ly.bytes > lds_limit(M->ctx->device))
+        return false;
+    auto kern = k_ilu_apply_lds<T, W, TRANS>;
+    static int granted = 0;
+    if (Ly.bytes > 65536 && Ly.bytes > granted) {
+        if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                Ly.bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return false;
+        }
+        granted = Ly.bytes;
+    }
+    hipLaunchKernelGGL(kern, dim3(M->nblocks), dim3(256), Ly.bytes, M->ctx->stream, M->bstart.p,
+                       A_.blk_lev.p, (const LevelMeta *)A_.lev_meta.p, A_.rows.p, A_.ell_col.p,
+                       va, da, B_.blk_lev.p, (const LevelMeta *)B_.lev_meta.p, B_.rows.p,
+                       B_.ell_col.p, vb, db, b, x, xv_cap, maxlev, M->max_block_rows, sk1, sk2);
+    RS_CHECK_LAUNCH();
+    return true;
+}
+
+// Sweep kernel choice: RSGPU_ILU_KERNEL = lds | pf | generic forces one
+// (tests and tuning); by default the LDS-resident kernel when the block
+// segment fits, else the pipelined one, else the generic one.
+static int ilu_kernel_choice()
+{
+    const char *e = getenv("RSGPU_ILU_KERNEL");
+    if (!e)
+        return 0;
+    if (!strcmp(e, "lds"))
+        return 1;
+    if (!strcmp(e, "pf"))
+        return 2;
+    if (!strcmp(e, "generic"))
+        return 3;
+    return 0;
 }
 
 template <typename T, bool TRANS>
@@ -740,13 +1020,48 @@ static void launch_apply(rsgpu_prec *M, const SweepData &A_, const T *va, const 
     if (M->n == 0)
         return;
     const int maxc = std::max(A_.max_count, B_.max_count);
+    const int choice = ilu_kernel_choice();
+    if ((choice == 0 || choice == 1) && maxc <= 64 && A_.W == B_.W &&
+        M->max_block_rows + 2 <= 32767) {
+        bool ok = false;
+        switch (A_.W) {
+        case 4:
+            ok = try_launch_lds<T, 4, TRANS>(M, A_, va, da, B_, vb, db, b, x, sk1, sk2);
+            break;
+        case 8:
+            ok = try_launch_lds<T, 8, TRANS>(M, A_, va, da, B_, vb, db, b, x, sk1, sk2);
+            break;
+        case 12:
+            ok = try_launch_lds<T, 12, TRANS>(M, A_, va, da, B_, vb, db, b, x, sk1, sk2);
+            break;
+        case 16:
+            ok = try_launch_lds<T, 16, TRANS>(M, A_, va, da, B_, vb, db, b, x, sk1, sk2);
+            break;
+        default:
+            break;
+        }
+        if (ok)
+            return;
+    }
     const bool lds = M->max_block_rows <= (int)(RS_LDS_BYTES / sizeof(T)) - 2;
     const size_t xv_bytes = ((size_t)(M->max_block_rows + 2) * sizeof(T) + 15) & ~(size_t)15;
     const size_t pf_shm =
         xv_bytes + (size_t)(A_.max_levels + B_.max_levels) * sizeof(LevelMeta);
     const int W = A_.W;
-    if (lds && maxc <= 64 && A_.W == B_.W && W <= 16 && pf_shm <= RS_LDS_BYTES) {
-        if (W <= 4)
+    // the pipelined kernel keeps the vector in LDS up to the device's
+    // opt-in limit (160 KiB on gfx950: 20k FP64 / 40k FP32 rows per block)
+    if (choice != 3 && maxc <= 64 && A_.W == B_.W && W <= 16 &&
+        pf_shm <= (size_t)lds_limit(M->ctx->device)) {
+        // used width: the widest row of either sweep (3 for 7-point and
+        // 13 for 27-point stencils get their own instances)
+        const int wu = std::max(A_.max_width, B_.max_width);
+        if (W == 4 && wu <= 3)
+            launch_pf<T, TRANS, 4, 3>(M, A_, va, da, B_, vb, db, b, x, xv_bytes, pf_shm, sk1,
+                                      sk2);
+        else if (W == 16 && wu <= 13)
+            launch_pf<T, TRANS, 16, 13>(M, A_, va, da, B_, vb, db, b, x, xv_bytes, pf_shm, sk1,
+                                        sk2);
+        else if (W <= 4)
             launch_pf<T, TRANS, 4>(M, A_, va, da, B_, vb, db, b, x, xv_bytes, pf_shm, sk1, sk2);
         else if (W <= 8)
             launch_pf<T, TRANS, 8>(M, A_, va, da, B_, vb, db, b, x, xv_bytes, pf_shm, sk1, sk2);

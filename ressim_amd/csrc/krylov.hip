@@ -148,7 +148,7 @@ __global__ void __launch_bounds__(RS_SPMV_BS)
 {
     if (stopped(st, inner_step))
         return;
-    spmv_core<T, RS_SPMV_BS, RS_SPMV_NNZ>(wg_rows, rowptr, colind, val, x, epi);
+    spmv_core<T, RS_SPMV_BS, spmv_nnz<T>()>(wg_rows, rowptr, colind, val, x, epi);
     double v[NR], tot[NR];
     v[0] = epi.a1;
     if (NR > 1)
@@ -168,7 +168,7 @@ __global__ void __launch_bounds__(RS_SPMV_BS)
     if (stopped(st, inner_step))
         return;
     EpiMult<T> epi{y};
-    spmv_core<T, RS_SPMV_BS, RS_SPMV_NNZ>(wg_rows, rowptr, colind, val, x, epi);
+    spmv_core<T, RS_SPMV_BS, spmv_nnz<T>()>(wg_rows, rowptr, colind, val, x, epi);
 }
 
 // 16-byte vector of T for the solver's internal vectors (leading dimension
@@ -559,9 +559,41 @@ __global__ void __launch_bounds__(RS_VBS)
 
 // Classical Gram-Schmidt for the FP32 inner solve of the mixed form (no
 // reference counterpart): all projections of column i from one pass over
-// w and the basis, H(k,i) = w . v_k for k = 0..i, reduced per k in a fixed
-// order by the finishing group.
-#define RS_CGS_CHUNK 8
+// w and the basis, H(k,i) = w . v_k for k = 0..i.
+#define RS_CGS_CHUNK 16
+// Every thread accumulates up to 16 dot products in registers over its
+// elements, each wave reduces them with shuffles, one LDS exchange per 16
+// columns gives the group's partials, and the last group to arrive sums
+// the partials with one wave per column.  The summation order is fixed by
+// the grid, so results are reproducible run to run.
+template <typename T, int CC>
+__device__ __forceinline__ void cgs_accum(int n, int nv, int stride, const T *__restrict__ w,
+                                          const T *__restrict__ V0, size_t ldv,
+                                          double (&a)[RS_CGS_CHUNK])
+{
+    using VT = typename V16<T>::type;
+    constexpr int N = V16<T>::N;
+    for (int j = blockIdx.x * RS_VBS + threadIdx.x; j < nv; j += stride) {
+        const VT wv = reinterpret_cast<const VT *>(w)[j];
+        const T *we = reinterpret_cast<const T *>(&wv);
+        VT kv[CC];
+#pragma unroll
+        for (int q = 0; q < CC; ++q)
+            kv[q] = reinterpret_cast<const VT *>(V0 + (size_t)q * ldv)[j];
+#pragma unroll
+        for (int q = 0; q < CC; ++q) {
+            const T *ke = reinterpret_cast<const T *>(&kv[q]);
+#pragma unroll
+            for (int c = 0; c < N; ++c)
+                a[q] += (double)(we[c] * ke[c]);
+        }
+    }
+    for (int j = nv * N + blockIdx.x * RS_VBS + threadIdx.x; j < n; j += stride)
+#pragma unroll
+        for (int q = 0; q < CC; ++q)
+            a[q] += (double)(w[j] * V0[(size_t)q * ldv + j]);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(RS_VBS)
     k_gm_cgs_dots(int n, int i, const T *__restrict__ w, const T *__restrict__ V, size_t ldv,
@@ -569,46 +601,59 @@ __global__ void __launch_bounds__(RS_VBS)
 {
     if (stopped(st, true))
         return;
-    using VT = typename V16<T>::type;
     constexpr int N = V16<T>::N;
-    __shared__ double sh[RS_VBS / 64 + 1];
+    constexpr int NW = RS_VBS / 64;
+    __shared__ double sh[NW][RS_CGS_CHUNK];
     __shared__ int last;
     const int cnt = i + 1, ld = st->m + 1;
     const int G = (int)gridDim.x;
     const int nv = n / N;
     const int stride = G * RS_VBS;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int c0 = 0; c0 < cnt; c0 += RS_CGS_CHUNK) {
         const int cc = min(RS_CGS_CHUNK, cnt - c0);
+        const T *V0 = V + (size_t)c0 * ldv;
         double a[RS_CGS_CHUNK];
 #pragma unroll
         for (int q = 0; q < RS_CGS_CHUNK; ++q)
             a[q] = 0;
-        for (int j = blockIdx.x * RS_VBS + threadIdx.x; j < nv; j += stride) {
-            const VT wv = reinterpret_cast<const VT *>(w)[j];
-            const T *we = reinterpret_cast<const T *>(&wv);
-#pragma unroll
-            for (int q = 0; q < RS_CGS_CHUNK; ++q)
-                if (q < cc) {
-                    const VT kv = reinterpret_cast<const VT *>(V + (size_t)(c0 + q) * ldv)[j];
-                    const T *ke = reinterpret_cast<const T *>(&kv);
-#pragma unroll
-                    for (int c = 0; c < N; ++c)
-                        a[q] += (double)(we[c] * ke[c]);
-                }
+        // the column count is a compile-time constant in each case, so all
+        // of a chunk's loads are unconditional and in flight together
+        switch (cc) {
+#define RS_CGS_CASE(K)                                                      \
+    case K:                                                                 \
+        cgs_accum<T, K>(n, nv, stride, w, V0, ldv, a);                       \
+        break;
+            RS_CGS_CASE(1) RS_CGS_CASE(2) RS_CGS_CASE(3) RS_CGS_CASE(4)
+            RS_CGS_CASE(5) RS_CGS_CASE(6) RS_CGS_CASE(7) RS_CGS_CASE(8)
+            RS_CGS_CASE(9) RS_CGS_CASE(10) RS_CGS_CASE(11) RS_CGS_CASE(12)
+            RS_CGS_CASE(13) RS_CGS_CASE(14) RS_CGS_CASE(15) RS_CGS_CASE(16)
+#undef RS_CGS_CASE
+        default:
+            break;
         }
-        for (int j = nv * N + blockIdx.x * RS_VBS + threadIdx.x; j < n; j += stride)
 #pragma unroll
-            for (int q = 0; q < RS_CGS_CHUNK; ++q)
-                if (q < cc)
-                    a[q] += (double)(w[j] * V[(size_t)(c0 + q) * ldv + j]);
-        for (int q = 0; q < cc; ++q) {
-            const double r = block_sum<RS_VBS>(a[q], sh);
-            if (threadIdx.x == 0)
-                __hip_atomic_store(&slot.partials[(size_t)(c0 + q) * G + blockIdx.x], r,
-                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int q = 0; q < RS_CGS_CHUNK; ++q)
+            if (q < cc) {
+                const double r = wave_sum(a[q]);
+                if (lane == 0)
+                    sh[wid][q] = r;
+            }
+        __syncthreads();
+        // wave 0 publishes the group's partials (its own store wait below
+        // covers them before the arrival ticket)
+        if (threadIdx.x < cc) {
+            double r = 0;
+#pragma unroll
+            for (int u = 0; u < NW; ++u)
+                r += sh[u][threadIdx.x];
+            __hip_atomic_store(&slot.partials[(size_t)(c0 + threadIdx.x) * G + blockIdx.x], r,
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
+        __syncthreads();
     }
-    // arrival ticket (as block_reduce_finish)
+    // arrival ticket (as block_reduce_finish); thread 0 is in wave 0, which
+    // made every partial store of this group
     if (threadIdx.x == 0) {
         wait_vm_stores();
         const int ng = G < RS_NGROUP ? G : RS_NGROUP;
@@ -633,13 +678,13 @@ __global__ void __launch_bounds__(RS_VBS)
     if (!last)
         return;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    for (int k = 0; k < cnt; ++k) {
+    for (int k = wid; k < cnt; k += NW) {
         double acc = 0;
-        for (int b = threadIdx.x; b < G; b += RS_VBS)
+        for (int b = lane; b < G; b += 64)
             acc += __hip_atomic_load(&slot.partials[(size_t)k * G + b], __ATOMIC_RELAXED,
                                      __HIP_MEMORY_SCOPE_AGENT);
-        acc = block_sum<RS_VBS>(acc, sh);
-        if (threadIdx.x == 0)
+        acc = wave_sum(acc);
+        if (lane == 0)
             st->H[k + i * ld] = acc;
     }
 }
@@ -663,15 +708,23 @@ __global__ void __launch_bounds__(RS_VBS)
     for (int j = blockIdx.x * RS_VBS + threadIdx.x; j < nv; j += stride) {
         VT wv = reinterpret_cast<const VT *>(w)[j];
         T *we = reinterpret_cast<T *>(&wv);
-        for (int k = 0; k <= i; ++k) {
-            const T h = (T)(-Hc[k]);
-            if (h == T(0))
-                continue;
-            const VT kv = reinterpret_cast<const VT *>(V + (size_t)k * ldv)[j];
-            const T *ke = reinterpret_cast<const T *>(&kv);
+        // eight basis vectors per step, loaded unconditionally (indices past
+        // i clamp to v_i) so the loads overlap; updates stay in k order
+        for (int k0 = 0; k0 <= i; k0 += 8) {
+            VT kv[8];
 #pragma unroll
-            for (int c = 0; c < N; ++c)
-                we[c] += h * ke[c];
+            for (int q = 0; q < 8; ++q)
+                kv[q] = reinterpret_cast<const VT *>(V + (size_t)min(k0 + q, i) * ldv)[j];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const T h = (T)(-Hc[min(k0 + q, i)]);
+                if (k0 + q <= i && h != T(0)) {
+                    const T *ke = reinterpret_cast<const T *>(&kv[q]);
+#pragma unroll
+                    for (int c = 0; c < N; ++c)
+                        we[c] += h * ke[c];
+                }
+            }
         }
         reinterpret_cast<VT *>(w)[j] = wv;
 #pragma unroll
@@ -892,14 +945,15 @@ void launch_spmv_red(rsgpu_solver *S, rsgpu_csr *A, const T *x, EpiMultDot<T> ep
                      DevState *st, int fin, int inner)
 {
     hipStream_t s = S->ctx->stream;
-    dim3 g(A->plan.nwg), b(RS_SPMV_BS);
+    const SpmvPlan &pl = spmv_plan<T>(A);
+    dim3 g(pl.nwg), b(RS_SPMV_BS);
     if (epi.nd <= 1)
         hipLaunchKernelGGL((k_spmv_red<T, EpiMultDot<T>, 1>), g, b, 0, s,
-                           A->plan.wg_rows.p, A->rowptr.p, A->colind.p,
+                           pl.wg_rows.p, A->rowptr.p, A->colind.p,
                            csr_values<T>(A), x, epi, st, slot_of(S), fin, inner);
     else
         hipLaunchKernelGGL((k_spmv_red<T, EpiMultDot<T>, 2>), g, b, 0, s,
-                           A->plan.wg_rows.p, A->rowptr.p, A->colind.p,
+                           pl.wg_rows.p, A->rowptr.p, A->colind.p,
                            csr_values<T>(A), x, epi, st, slot_of(S), fin, inner);
     RS_CHECK_LAUNCH();
 }
@@ -910,8 +964,9 @@ void launch_resid(rsgpu_solver *S, rsgpu_csr *A, const T *b, const T *x, T *r,
 {
     hipStream_t s = S->ctx->stream;
     EpiResidNorm<T> epi{r, copy, b, r32};
-    hipLaunchKernelGGL((k_spmv_red<T, EpiResidNorm<T>, 1>), dim3(A->plan.nwg),
-                       dim3(RS_SPMV_BS), 0, s, A->plan.wg_rows.p, A->rowptr.p,
+    const SpmvPlan &pl = spmv_plan<T>(A);
+    hipLaunchKernelGGL((k_spmv_red<T, EpiResidNorm<T>, 1>), dim3(pl.nwg),
+                       dim3(RS_SPMV_BS), 0, s, pl.wg_rows.p, A->rowptr.p,
                        A->colind.p, csr_values<T>(A), x, epi, st, slot_of(S),
                        fin, 0);
     RS_CHECK_LAUNCH();
@@ -1036,13 +1091,15 @@ void run_gmres(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const T *b, T *x,
                            st, sl);
         for (int i = 0; i < m; ++i) {
             T *vi = V + (size_t)i * L;
-            hipLaunchKernelGGL(k_spmv_skip<T>, dim3(A->plan.nwg), dim3(RS_SPMV_BS), 0, s,
-                               A->plan.wg_rows.p, A->rowptr.p, A->colind.p,
+            hipLaunchKernelGGL(k_spmv_skip<T>, dim3(spmv_plan<T>(A).nwg), dim3(RS_SPMV_BS),
+                               0, s, spmv_plan<T>(A).wg_rows.p, A->rowptr.p, A->colind.p,
                                csr_values<T>(A), vi, u, st, 1);
             papply(S, M, u, w, st);
             if (S->o.cgs && sizeof(T) == 4) {
-                hipLaunchKernelGGL(k_gm_cgs_dots<T>, dim3(G), dim3(RS_VBS), 0, s, n, i, w, V, L,
-                                   st, sl);
+                // a quarter of the vector grid: 16 elements per thread
+                // amortise the in-group reductions
+                hipLaunchKernelGGL(k_gm_cgs_dots<T>, dim3(std::max(1, G / 4)), dim3(RS_VBS), 0,
+                                   s, n, i, w, V, L, st, sl);
                 hipLaunchKernelGGL(k_gm_cgs_last<T>, dim3(G), dim3(RS_VBS), 0, s, n, i, w, V, L,
                                    st, sl);
             } else {
@@ -1346,7 +1403,8 @@ extern "C" int rsgpu_solver_solve(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M,
     try {
         RS_HIP(hipSetDevice(ctx->device));
         const int m = std::max(1, S->o.restart);
-        const size_t need = (size_t)2 * std::max(A->plan.nwg, grid_for(A->n)) + 2;
+        const size_t need =
+            (size_t)2 * std::max(std::max(A->plan.nwg, A->plan32.nwg), grid_for(A->n)) + 2;
         const size_t need_cgs = (size_t)(m + 2) * grid_for(A->n);
         if (S->partials.n < std::max(need, need_cgs) + 4096)
             S->partials.alloc(std::max(need, need_cgs) + 4096);

@@ -1318,7 +1318,7 @@ extern "C" int rsgpu_mpfa_o_assemble(rsgpu_ctx *ctx, int dim, int nx, int ny, in
         if (nnz)
             RS_HIP(hipMemcpyAsync(A->h_colind.data(), A->colind.p, sizeof(int) * nnz,
                                   hipMemcpyDeviceToHost, s));
-        spmv_build_plan(A->h_rowptr, n, A->plan, s);
+        spmv_build_plans(A->h_rowptr, n, A, s);
         A->pattern_id = next_pattern_id();
         RS_HIP(hipStreamSynchronize(s));
         if (check_fail)

@@ -100,7 +100,8 @@ struct rsgpu_csr {
     DevBuf<int> rowptr, colind;
     DevBuf<double> val;
     DevBuf<float> val32;
-    SpmvPlan plan;
+    SpmvPlan plan;     // FP64 kernels: 2048-nonzero stages
+    SpmvPlan plan32;   // FP32 kernels: 4096-nonzero stages (same LDS bytes)
     // transposed copy (lazy): rows = columns of A, values gathered via tperm
     bool has_t = false;
     DevBuf<int> trowptr, tcolind, tperm;
@@ -365,7 +366,9 @@ __device__ __forceinline__ bool dev_monitor(DevState *st, double r)
 // ---------------------------------------------------------------- host API
 // spmv.hip
 void spmv_build_plan(const std::vector<int> &rowptr, int n, SpmvPlan &plan,
-                     hipStream_t s);
+                     hipStream_t s, int nnz_wg = 2048);
+// both plans of a CSR matrix (FP64 and FP32 stage sizes)
+void spmv_build_plans(const std::vector<int> &rowptr, int n, rsgpu_csr *A, hipStream_t s);
 template <typename T>
 void spmv_launch(hipStream_t s, const rsgpu_csr *A, const T *val, const T *x,
                  T *y, int mode, double alpha, const T *b);

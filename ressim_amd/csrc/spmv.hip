@@ -7,7 +7,7 @@
 // Greedy contiguous row partition: a group takes rows until the next one
 // would overflow the LDS stage or the lane count; a longer row is alone.
 void spmv_build_plan(const std::vector<int> &rowptr, int n, SpmvPlan &plan,
-                     hipStream_t s)
+                     hipStream_t s, int nnz_wg)
 {
     std::vector<int> starts;
     starts.reserve(n / 16 + 2);
@@ -16,9 +16,9 @@ void spmv_build_plan(const std::vector<int> &rowptr, int n, SpmvPlan &plan,
         starts.push_back(r);
         const int j0 = rowptr[r];
         int e = r + 1;
-        if (rowptr[r + 1] - j0 <= RS_SPMV_NNZ)
+        if (rowptr[r + 1] - j0 <= nnz_wg)
             while (e < n && e - r < RS_SPMV_BS &&
-                   rowptr[e + 1] - j0 <= RS_SPMV_NNZ)
+                   rowptr[e + 1] - j0 <= nnz_wg)
                 ++e;
         r = e;
     }
@@ -27,14 +27,19 @@ void spmv_build_plan(const std::vector<int> &rowptr, int n, SpmvPlan &plan,
     plan.wg_rows.upload(starts, s);
 }
 
+void spmv_build_plans(const std::vector<int> &rowptr, int n, rsgpu_csr *A, hipStream_t s)
+{
+    spmv_build_plan(rowptr, n, A->plan, s, RS_SPMV_NNZ);
+    spmv_build_plan(rowptr, n, A->plan32, s, RS_SPMV_NNZ32);
+}
+
 template <typename T, class Epi>
 __global__ void __launch_bounds__(RS_SPMV_BS)
     k_spmv(const int *__restrict__ wg_rows, const int *__restrict__ rowptr,
            const int *__restrict__ colind, const T *__restrict__ val,
            const T *__restrict__ x, Epi epi)
 {
-    spmv_core<T, RS_SPMV_BS, RS_SPMV_NNZ>(wg_rows, rowptr, colind, val, x,
-                                          epi);
+    spmv_core<T, RS_SPMV_BS, spmv_nnz<T>()>(wg_rows, rowptr, colind, val, x, epi);
 }
 
 // mode: 0 mult, 1 multAdd(alpha), 2 residual b + alpha*Ax, 3 plain sum,
@@ -45,8 +50,9 @@ void spmv_launch(hipStream_t s, const rsgpu_csr *A, const T *val, const T *x,
 {
     if (A->n == 0)
         return;
-    dim3 g(A->plan.nwg), bl(RS_SPMV_BS);
-    const int *wr = A->plan.wg_rows.p, *rp = A->rowptr.p, *ci = A->colind.p;
+    const SpmvPlan &pl = spmv_plan<T>(A);
+    dim3 g(pl.nwg), bl(RS_SPMV_BS);
+    const int *wr = pl.wg_rows.p, *rp = A->rowptr.p, *ci = A->colind.p;
     switch (mode) {
     case 0:
         hipLaunchKernelGGL((k_spmv<T, EpiMult<T>>), g, bl, 0, s, wr, rp, ci,

@@ -17,6 +17,21 @@
 
 #define RS_SPMV_BS 256
 #define RS_SPMV_NNZ 2048
+#define RS_SPMV_NNZ32 4096
+
+// Stage size and plan by value type: FP32 groups stage twice the nonzeros
+// (the same 16 KiB of products), halving the fixed per-group work.
+template <typename T>
+constexpr int spmv_nnz()
+{
+    return sizeof(T) == 4 ? RS_SPMV_NNZ32 : RS_SPMV_NNZ;
+}
+
+template <typename T>
+inline const SpmvPlan &spmv_plan(const rsgpu_csr *A)
+{
+    return sizeof(T) == 4 ? A->plan32 : A->plan;
+}
 
 template <typename T, int BS, int NNZ_WG, class Epi>
 __device__ __forceinline__ void spmv_core(const int *__restrict__ wg_rows,
@@ -30,12 +45,31 @@ __device__ __forceinline__ void spmv_core(const int *__restrict__ wg_rows,
     const int r0 = wg_rows[g], r1 = wg_rows[g + 1];
     const int j0 = rowptr[r0], j1 = rowptr[r1];
     if (j1 - j0 <= NNZ_WG) {
+        // Every lane issues all of its value/column loads, then all of its
+        // gathers, before the first product is stored: indices past the
+        // group's last nonzero are clamped to it (a valid, already-fetched
+        // address) and only the store is predicated.  A guarded load per
+        // slot would serialise the slots (one memory latency each).
+        constexpr int Q = NNZ_WG / BS;
+        if (j1 > j0) {
+            T v[Q];
+            int c[Q];
 #pragma unroll
-        for (int q = 0; q < NNZ_WG / BS; ++q) {
-            const int j = j0 + (int)threadIdx.x + q * BS;
-            if (j < j1)
-                prod[j - j0] = __builtin_nontemporal_load(&val[j]) *
-                               x[__builtin_nontemporal_load(&colind[j])];
+            for (int q = 0; q < Q; ++q) {
+                const int j = min(j0 + (int)threadIdx.x + q * BS, j1 - 1);
+                v[q] = __builtin_nontemporal_load(&val[j]);
+                c[q] = __builtin_nontemporal_load(&colind[j]);
+            }
+            T xg[Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+                xg[q] = x[c[q]];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int jl = (int)threadIdx.x + q * BS;
+                if (j0 + jl < j1)
+                    prod[jl] = v[q] * xg[q];
+            }
         }
         __syncthreads();
         for (int r = r0 + (int)threadIdx.x; r < r1; r += BS) {

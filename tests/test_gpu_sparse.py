@@ -156,9 +156,54 @@ def test_block_ilu_bitexact(gpu, oracle, systems, nblocks):
     np.testing.assert_array_equal(x, oracle.ilu_apply(brp, bci, lu, dg, b))
 
 
-def test_ilu_global_scratch_path(gpu, oracle):
-    """A block larger than the LDS vector (9000 rows) runs the sweeps through
-    the global scratch path; still bit-exact."""
+@pytest.mark.parametrize("kernel", ["lds", "pf", "generic"])
+@pytest.mark.parametrize("nblocks", [3, 16])
+def test_block_ilu_sweep_kernels(gpu, oracle, systems, monkeypatch, kernel, nblocks):
+    """Every sweep kernel (LDS-resident factor, pipelined, generic) gives the
+    oracle's bits for apply and transApply, FP64 and FP32.  nblocks=3 puts
+    280 rows in a block: an FP64 LDS-resident block of about 97 KB, above
+    the 64 KB default workgroup allocation."""
+    import torch
+    from ressim_amd import _lib
+    monkeypatch.setenv("RSGPU_ILU_KERNEL", kernel)
+    s = systems["s3"]
+    rp, ci, d = s["rowptr"], s["colind"], s["data"]
+    n = len(rp) - 1
+    print(f"sweep kernel {kernel} nblocks {nblocks} n {n}", flush=True)
+    A = _mat(gpu, rp, ci, d, n, n)
+    brp, bci, bd, _ = oracle.block_filter(rp, ci, d, nblocks)
+    st, lu, dg, _ = oracle.ilu_factor(brp, bci, bd)
+    rng = np.random.default_rng(nblocks + 100)
+    M = gpu.BlockILU(nblocks)
+    M.setMatrix(A)
+    b = rng.standard_normal(n)
+    x = np.empty(n)
+    M.apply(b, x)
+    np.testing.assert_array_equal(x, oracle.ilu_apply(brp, bci, lu, dg, b))
+    xt = np.empty(n)
+    M.transApply(b, xt)
+    np.testing.assert_array_equal(xt, oracle.ilu_transapply(brp, bci, lu, dg, b))
+    M32 = gpu.BlockILU(nblocks, precision=_lib.FP32)
+    M32.setMatrix(A)
+    st, lu32, dg32, _ = oracle.ilu_factor_f32(brp, bci, bd.astype(np.float32))
+    b32 = b.astype(np.float32)
+    bd32 = torch.tensor(b32, device="cuda")
+    xd32 = torch.empty(n, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    assert _lib.load().rsgpu_prec_apply_f32(M32.h, bd32.data_ptr(), xd32.data_ptr()) == 0
+    M32.ctx.synchronize()
+    np.testing.assert_array_equal(xd32.cpu().numpy(),
+                                  oracle.ilu_apply_f32(brp, bci, lu32, dg32, b32))
+
+
+@pytest.mark.parametrize("kernel", ["default", "generic"])
+def test_ilu_global_scratch_path(gpu, oracle, monkeypatch, kernel):
+    """A 9000-row block: the pipelined kernel keeps its FP64 vector in LDS
+    beyond the 64 KiB default allocation (opt-in); the generic kernel runs
+    the sweeps through the global scratch region.  Both bit-exact."""
+    if kernel != "default":
+        monkeypatch.setenv("RSGPU_ILU_KERNEL", kernel)
+    print(f"9000-row ILU block, kernel {kernel}", flush=True)
     s = oracle_system(oracle, P.config1(nx=100, ny=90))
     rp, ci, d = s["rowptr"], s["colind"], s["data"]
     n = len(rp) - 1
