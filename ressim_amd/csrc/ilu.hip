@@ -136,7 +136,9 @@ static int max_entries(const std::vector<std::vector<std::pair<int, int>>> &ent)
     return (int)w;
 }
 
-static void upload_sweep(const HostSweep &H, SweepData &D, hipStream_t s)
+// es: element size of the apply's vector; the pipelined kernel reads row
+// and operand positions as LDS byte offsets (position * es)
+static void upload_sweep(const HostSweep &H, SweepData &D, hipStream_t s, int es)
 {
     D.W = H.W;
     D.blk_lev.upload(H.blk_lev, s);
@@ -144,6 +146,15 @@ static void upload_sweep(const HostSweep &H, SweepData &D, hipStream_t s)
     D.rows.upload(H.rows, s);
     D.row_glob.upload(H.row_glob, s);
     D.ell_col.upload(H.col, s);
+    {
+        std::vector<int> cb(H.col.size()), rb(H.rows.size());
+        for (size_t q = 0; q < cb.size(); ++q)
+            cb[q] = H.col[q] * es;
+        for (size_t q = 0; q < rb.size(); ++q)
+            rb[q] = H.rows[q] * es;
+        D.colb.upload(cb, s);
+        D.rowsb.upload(rb.empty() ? std::vector<int>(1, 0) : rb, s);
+    }
     D.ell_src.upload(H.src, s);
     D.ell_size = H.col.size();
     D.nrows_lev = (int)H.row_glob.size();
@@ -267,9 +278,9 @@ void prec_setup(rsgpu_prec *M, rsgpu_csr *A)
     const int W = pad_width(std::max(max_entries(eL), max_entries(eU)));
     HostSweep H;
     build_sweep(nb, M->h_bstart, eL, true, W, H);
-    upload_sweep(H, M->L, s);
+    upload_sweep(H, M->L, s, M->precision == RSGPU_FP64 ? 8 : 4);
     build_sweep(nb, M->h_bstart, eU, false, W, H);
-    upload_sweep(H, M->U, s);
+    upload_sweep(H, M->U, s, M->precision == RSGPU_FP64 ? 8 : 4);
     M->status.alloc(2);
     if (M->precision == RSGPU_FP64) {
         M->lu64.alloc(std::max(M->nnz, 1));
@@ -317,9 +328,9 @@ static void prec_setup_transpose(rsgpu_prec *M)
     const int W = pad_width(std::max(max_entries(eUt), max_entries(eLt)));
     HostSweep H;
     build_sweep(M->nblocks, M->h_bstart, eUt, true, W, H);
-    upload_sweep(H, M->Ut, s);
+    upload_sweep(H, M->Ut, s, M->precision == RSGPU_FP64 ? 8 : 4);
     build_sweep(M->nblocks, M->h_bstart, eLt, false, W, H);
-    upload_sweep(H, M->Lt, s);
+    upload_sweep(H, M->Lt, s, M->precision == RSGPU_FP64 ? 8 : 4);
     M->ell64_Ut.alloc(M->Ut.ell_size);
     M->ell64_Lt.alloc(M->Lt.ell_size);
     M->dgUt64.alloc(M->Ut.nrows_lev);
@@ -504,43 +515,97 @@ struct alignas(16) CRow {
 
 template <typename T, int W>
 struct LevelRegs {
-    int r;
+    int r;      // LDS byte offset of the row (inactive lanes: a real row)
     bool act;
     T d;
     VRow<T, W> v;
-    CRow<W> c;
+    CRow<W> c;  // LDS byte offsets of the operands
 };
 
-// All loads of a level are independent of each other and of the vector:
-// row index, the lane's W columns and values, level-ordered diagonal (in
-// that order, so one wait covers every gather address).  Inactive lanes
-// (t >= count, or every lane of a level index past the sweep's end) redo a
-// real row of the clamped level and write the result to the junk slot
-// nb + 1, so neither loads nor updates sit behind branches (branches make
-// the compiler's wait counting fall back to waiting for every load, and
-// let it sink prefetches next to their use).
-template <typename T, int W, bool DIV>
-__device__ __forceinline__ void level_load(const LevelMeta &m, bool valid,
-                                           const int *__restrict__ rows,
-                                           const int *__restrict__ ecol,
-                                           const T *__restrict__ eval,
-                                           const T *__restrict__ dgl, LevelRegs<T, W> &R)
+// One sweep as the pipelined kernel reads it: level ranges and metadata,
+// plus 32-bit buffer resources over the level-ordered row byte offsets,
+// operand byte offsets, values and U diagonals (loads beyond a resource's
+// size return zero instead of faulting).
+struct PfSweep {
+    const int *blk;
+    const LevelMeta *meta;
+    const int *rowsb;
+    const int *colb;
+    const void *val;
+    const void *dgl;
+    int rows_bytes, col_bytes, val_bytes, dgl_bytes;
+};
+
+struct PfRsrc {
+    __amdgpu_buffer_rsrc_t rows, cols, vals, dgl;
+};
+
+// buffer resource word 3 for raw (unformatted, stride 0) buffers on gfx9
+#define RS_BUF_WORD3 0x00020000
+
+__device__ __forceinline__ PfRsrc pf_rsrc(const PfSweep &S)
 {
-    // the metadata is wave-uniform: scalar bases, 32-bit lane offsets
+    PfRsrc R;
+    R.rows = __builtin_amdgcn_make_buffer_rsrc((void *)S.rowsb, (short)0, S.rows_bytes,
+                                               RS_BUF_WORD3);
+    R.cols = __builtin_amdgcn_make_buffer_rsrc((void *)S.colb, (short)0, S.col_bytes,
+                                               RS_BUF_WORD3);
+    R.vals = __builtin_amdgcn_make_buffer_rsrc((void *)S.val, (short)0, S.val_bytes,
+                                               RS_BUF_WORD3);
+    R.dgl = __builtin_amdgcn_make_buffer_rsrc((void *)S.dgl, (short)0, S.dgl_bytes,
+                                              RS_BUF_WORD3);
+    return R;
+}
+
+typedef int rs_v4i __attribute__((ext_vector_type(4)));
+typedef int rs_v2i __attribute__((ext_vector_type(2)));
+
+// NB bytes (a multiple of 16) from a raw buffer into dst
+template <int NB>
+__device__ __forceinline__ void buf_load(void *dst, __amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    static_assert(NB % 16 == 0, "row segments are whole 16-byte chunks");
+#pragma unroll
+    for (int k = 0; k < NB / 16; ++k) {
+        const rs_v4i q = __builtin_amdgcn_raw_buffer_load_b128(r, voff + 16 * k, soff, 0);
+        __builtin_memcpy((char *)dst + 16 * k, &q, 16);
+    }
+}
+
+// All loads of a level are independent of each other and of the vector:
+// row offset, the lane's W operand offsets and values, level-ordered
+// diagonal (in that order, so one wait covers every gather address).  The
+// level metadata is wave-uniform: scalar base offsets, 32-bit lane
+// offsets.  Inactive lanes (t >= count, or every lane of a level index past
+// the sweep's end) redo a real row of the clamped level and write the
+// result to the junk slot, so neither loads nor updates sit behind
+// branches (branches make the compiler's wait counting fall back to
+// waiting for every load, and let it sink prefetches next to their use).
+template <typename T, int W, bool DIV>
+__device__ __forceinline__ void level_load(const LevelMeta &m, bool valid, const PfRsrc &rs,
+                                           LevelRegs<T, W> &R)
+{
     const int count = __builtin_amdgcn_readfirstlane(m.count);
     const int off = __builtin_amdgcn_readfirstlane(m.off);
     const int roff = __builtin_amdgcn_readfirstlane(m.roff);
     const int t = threadIdx.x;
-    const bool act = valid && t < count;
-    const int tt = t < count ? t : count - 1;
-    // the row index is only consumed by the compute stage: nothing here may
-    // depend on a loaded value, or the prefetch would wait for itself
-    R.r = (rows + roff)[tt];
-    R.act = act;
-    R.c = reinterpret_cast<const CRow<W> *>(ecol + off)[tt];
-    R.v = reinterpret_cast<const VRow<T, W> *>(eval + off)[tt];
-    if (DIV)
-        R.d = (dgl + roff)[tt];
+    const int tt = min(t, count - 1);
+    // nothing here may depend on a loaded value, or the prefetch would
+    // wait for itself
+    R.act = valid && t < count;
+    R.r = __builtin_amdgcn_raw_buffer_load_b32(rs.rows, tt * 4, roff * 4, 0);
+    buf_load<W * 4>(&R.c, rs.cols, tt * (W * 4), off * 4);
+    buf_load<W * (int)sizeof(T)>(&R.v, rs.vals, tt * (W * (int)sizeof(T)),
+                                 off * (int)sizeof(T));
+    if (DIV) {
+        if (sizeof(T) == 8) {
+            const rs_v2i q = __builtin_amdgcn_raw_buffer_load_b64(rs.dgl, tt * 8, roff * 8, 0);
+            __builtin_memcpy(&R.d, &q, 8);
+        } else {
+            const int q = __builtin_amdgcn_raw_buffer_load_b32(rs.dgl, tt * 4, roff * 4, 0);
+            __builtin_memcpy(&R.d, &q, 4);
+        }
+    }
 }
 
 // Every gathered operand of a level passes through one empty asm statement
@@ -548,68 +613,77 @@ __device__ __forceinline__ void level_load(const LevelMeta &m, bool valid,
 // one wait covers them, instead of the scheduler interleaving arithmetic
 // between batches of reads (one LDS round trip per level instead of two).
 // The statement emits no instruction and performs no memory access.
-template <typename T, int WL>
-__device__ __forceinline__ void pin_gathers(T (&x)[WL], T &r)
+template <typename T, int W>
+__device__ __forceinline__ void pin_gathers(T (&x)[W], T &r)
 {
 #define RS_P(i) "+v"(x[i])
-    if constexpr (WL == 3)
-        asm volatile("" : "+v"(r), RS_P(0), RS_P(1), RS_P(2));
-    else if constexpr (WL == 4)
+    if constexpr (W == 4)
         asm volatile("" : "+v"(r), RS_P(0), RS_P(1), RS_P(2), RS_P(3));
-    else if constexpr (WL == 8)
+    else if constexpr (W == 8)
         asm volatile("" : "+v"(r), RS_P(0), RS_P(1), RS_P(2), RS_P(3), RS_P(4), RS_P(5), RS_P(6),
                      RS_P(7));
-    else if constexpr (WL == 12)
+    else if constexpr (W == 12)
         asm volatile("" : "+v"(r), RS_P(0), RS_P(1), RS_P(2), RS_P(3), RS_P(4), RS_P(5), RS_P(6),
                      RS_P(7), RS_P(8), RS_P(9), RS_P(10), RS_P(11));
-    else if constexpr (WL == 13)
-        asm volatile("" : "+v"(r), RS_P(0), RS_P(1), RS_P(2), RS_P(3), RS_P(4), RS_P(5), RS_P(6),
-                     RS_P(7), RS_P(8), RS_P(9), RS_P(10), RS_P(11), RS_P(12));
-    else if constexpr (WL == 16)
+    else if constexpr (W == 16)
         asm volatile("" : "+v"(r), RS_P(0), RS_P(1), RS_P(2), RS_P(3), RS_P(4), RS_P(5), RS_P(6),
                      RS_P(7), RS_P(8), RS_P(9), RS_P(10), RS_P(11), RS_P(12), RS_P(13), RS_P(14),
                      RS_P(15));
     else
-        static_assert(WL == 3, "no pin for this width");
+        static_assert(W == 4, "no pin for this width");
 #undef RS_P
 }
 
-// Only the first WL <= W slots are used: when no row of either sweep has
-// more than WL entries, slots WL..W-1 are padding everywhere, and a padding
-// update is an exact identity (sum + (-0.0 * 0) == sum, acc - (+0.0 * 0) ==
-// acc, signed zeros included), so skipping them keeps the bits.  Fewer
-// gathers let one batch of LDS reads cover the level.
-template <typename T, int W, int WL, int FORM, bool DIV>
-__device__ __forceinline__ void level_compute(const LevelRegs<T, W> &R, T *xv, int junk)
+template <typename T, int W>
+struct Gathered {
+    T rhs;
+    T xs[W];
+};
+
+template <typename T>
+__device__ __forceinline__ T &lds_at(unsigned char *base, int byte_off)
 {
-    static_assert(WL <= W, "used slots exceed the stored width");
-    const int wr = R.act ? R.r : junk;
-    T rhs_ = xv[R.r];
-    T xs[WL];
+    return *reinterpret_cast<T *>(base + byte_off);
+}
+
+// Issue the level's LDS reads: the row's right-hand side and its operands.
+template <typename T, int W>
+__device__ __forceinline__ void level_gather(const LevelRegs<T, W> &R, unsigned char *xv,
+                                             Gathered<T, W> &g)
+{
+    g.rhs = lds_at<T>(xv, R.r);
 #pragma unroll
-    for (int s = 0; s < WL; ++s)
-        xs[s] = xv[R.c.c[s]];
-    pin_gathers<T, WL>(xs, rhs_);
-    const T rhs = rhs_;
+    for (int s = 0; s < W; ++s)
+        g.xs[s] = lds_at<T>(xv, R.c.c[s]);
+}
+
+// Wait for the gathers once, then the row's operation sequence and store.
+template <typename T, int W, int FORM, bool DIV>
+__device__ __forceinline__ void level_finish(const LevelRegs<T, W> &R, Gathered<T, W> &g,
+                                             unsigned char *xv, int junkb)
+{
+    pin_gathers<T, W>(g.xs, g.rhs);
+    const int wr = R.act ? R.r : junkb;
+    const T rhs = g.rhs;
     T acc;
     if (FORM == FORM_SUM) {
         T sum = 0;
 #pragma unroll
-        for (int s = 0; s < WL; ++s)
-            sum += R.v.v[s] * xs[s];
+        for (int s = 0; s < W; ++s)
+            sum += R.v.v[s] * g.xs[s];
         acc = rhs - sum;
     } else {
         acc = rhs;
 #pragma unroll
-        for (int s = 0; s < WL; ++s)
-            acc -= R.v.v[s] * xs[s];
+        for (int s = 0; s < W; ++s)
+            acc -= R.v.v[s] * g.xs[s];
     }
     if (DIV)
         acc = acc / R.d;
-    xv[wr] = acc;
+    lds_at<T>(xv, wr) = acc;
 }
 
-// Loads in flight per level (row index, columns, values, U diagonal).
+// Loads in flight per level (row offset, operand offsets, values, U diagonal).
 template <typename T, int W, bool DIV>
 constexpr int level_loads()
 {
@@ -641,34 +715,40 @@ constexpr int pf_depth()
 // level's compute.  The kernel is one wave and a wave's LDS operations
 // complete in issue order, so levels are separated by a wave-scope fence
 // (no wait for the store to land).
-template <typename T, int W, int WL, int FORM, bool DIV>
-__device__ __forceinline__ void sweep_pf(int nl, const LevelMeta *sm,
-                                         const int *__restrict__ rows,
-                                         const int *__restrict__ ecol,
-                                         const T *__restrict__ eval,
-                                         const T *__restrict__ dgl, T *xv, int junk)
+template <typename T, int W, int FORM, bool DIV>
+__device__ __forceinline__ void sweep_pf(int nl, const LevelMeta *sm, const PfRsrc &rs,
+                                         unsigned char *xv, int junkb)
 {
     constexpr int D = pf_depth<T, W>();
     if (nl <= 0)
         return;
     const int last = nl - 1;
     LevelRegs<T, W> r[D];
+    // prologue in the steady state's issue order (level by level), so the
+    // wait counts at the loop head are the same from either predecessor
 #pragma unroll
-    for (int k = 0; k < D - 1; ++k)
-        level_load<T, W, DIV>(sm[min(k, last)], k <= last, rows, ecol, eval, dgl, r[k]);
+    for (int k = 0; k < D - 1; ++k) {
+        level_load<T, W, DIV>(sm[min(k, last)], k <= last, rs, r[k]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
     LevelMeta mq = sm[min(D - 1, last)];
     for (int l = 0; l < nl; l += D) {
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            // prefetch level l+k+D-1 into the set level l+k-1 used
+            // the current level's gathers go out first; the prefetch of
+            // level l+k+D-1 (into the set level l+k-1 used) is issued while
+            // they are in flight, then one wait and the row updates.  The
+            // scheduling barriers keep the three groups in this order (the
+            // scheduler would otherwise sink the prefetch past the compute)
+            Gathered<T, W> g;
+            level_gather<T, W>(r[k], xv, g);
+            __builtin_amdgcn_sched_barrier(0);
             const int lev = l + k + D - 1;
-            level_load<T, W, DIV>(mq, lev <= last, rows, ecol, eval, dgl, r[(k + D - 1) % D]);
+            level_load<T, W, DIV>(mq, lev <= last, rs, r[(k + D - 1) % D]);
             mq = sm[min(lev + 1, last)];
-            // the scheduling barriers keep each prefetch issued before the
-            // compute of the current level (the scheduler would otherwise
-            // sink the loads past the compute and batch them)
             asm volatile("" ::: "memory");
-            level_compute<T, W, WL, FORM, DIV>(r[k], xv, junk);
+            __builtin_amdgcn_sched_barrier(0);
+            level_finish<T, W, FORM, DIV>(r[k], g, xv, junkb);
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_sched_barrier(0);
         }
@@ -677,15 +757,9 @@ __device__ __forceinline__ void sweep_pf(int nl, const LevelMeta *sm,
 
 // One wave per block (every level has at most 64 rows); vector and both
 // sweeps' level metadata in LDS.
-template <typename T, int W, int WL, bool TRANS>
+template <typename T, int W, bool TRANS>
 __global__ void __launch_bounds__(64)
-    k_ilu_apply_pf(const int *__restrict__ bstart,
-                   const int *__restrict__ blkA, const LevelMeta *__restrict__ metaA,
-                   const int *__restrict__ rowsA, const int *__restrict__ colA,
-                   const T *__restrict__ valA, const T *__restrict__ dglA,
-                   const int *__restrict__ blkB, const LevelMeta *__restrict__ metaB,
-                   const int *__restrict__ rowsB, const int *__restrict__ colB,
-                   const T *__restrict__ valB, const T *__restrict__ dglB,
+    k_ilu_apply_pf(const int *__restrict__ bstart, PfSweep A, PfSweep B,
                    const T *__restrict__ b, T *__restrict__ x, int xv_bytes,
                    const int *skip1, const int *skip2)
 {
@@ -696,167 +770,28 @@ __global__ void __launch_bounds__(64)
     const int b0 = bstart[blk], b1 = bstart[blk + 1], nb = b1 - b0;
     T *xv = (T *)rs_smem;
     LevelMeta *sm = (LevelMeta *)(rs_smem + xv_bytes);
-    const int la0 = blkA[blk], nla = blkA[blk + 1] - la0;
-    const int lb0 = blkB[blk], nlb = blkB[blk + 1] - lb0;
+    const int la0 = A.blk[blk], nla = A.blk[blk + 1] - la0;
+    const int lb0 = B.blk[blk], nlb = B.blk[blk + 1] - lb0;
     for (int q = threadIdx.x; q < nla; q += blockDim.x)
-        sm[q] = metaA[la0 + q];
+        sm[q] = A.meta[la0 + q];
     for (int q = threadIdx.x; q < nlb; q += blockDim.x)
-        sm[nla + q] = metaB[lb0 + q];
+        sm[nla + q] = B.meta[lb0 + q];
     for (int r = threadIdx.x; r < nb; r += blockDim.x)
         xv[r] = b[b0 + r];
     if (threadIdx.x == 0)
         xv[nb] = T(0);
     __syncthreads();
-    const int junk = nb + 1;
+    const int junkb = (nb + 1) * (int)sizeof(T);
+    const PfRsrc ra = pf_rsrc(A), rb = pf_rsrc(B);
     if (!TRANS) {
-        sweep_pf<T, W, WL, FORM_SUM, false>(nla, sm, rowsA, colA, valA, dglA, xv, junk);
-        sweep_pf<T, W, WL, FORM_SUM, true>(nlb, sm + nla, rowsB, colB, valB, dglB, xv, junk);
+        sweep_pf<T, W, FORM_SUM, false>(nla, sm, ra, rs_smem, junkb);
+        sweep_pf<T, W, FORM_SUM, true>(nlb, sm + nla, rb, rs_smem, junkb);
     } else {
-        sweep_pf<T, W, WL, FORM_SUBSEQ, true>(nla, sm, rowsA, colA, valA, dglA, xv, junk);
-        sweep_pf<T, W, WL, FORM_SUBSEQ, false>(nlb, sm + nla, rowsB, colB, valB, dglB, xv,
-                                               junk);
+        sweep_pf<T, W, FORM_SUBSEQ, true>(nla, sm, ra, rs_smem, junkb);
+        sweep_pf<T, W, FORM_SUBSEQ, false>(nlb, sm + nla, rb, rs_smem, junkb);
     }
     __syncthreads();
     for (int r = threadIdx.x; r < nb; r += blockDim.x)
-        x[b0 + r] = xv[r];
-}
-
-// ---------------------------------------------------------------- LDS-resident
-// The whole block segment of both sweeps (values, columns and rows as
-// 16-bit local indices, level metadata, diagonals) and the block's vector
-// are staged into LDS by the workgroup; then one wave walks the levels with
-// LDS traffic only.  The other waves exit after staging; the sweeping wave
-// orders its LDS accesses with wave-scope fences (no workgroup barrier).
-// Every row update is row_update's operation sequence (bit-identical).
-struct LdsLayout {
-    int xv, meta, vA, cA, rA, dA, vB, cB, rB, dB, bytes;
-};
-
-template <typename T, int W>
-__host__ __device__ inline LdsLayout lds_layout(int xv_cap, int maxlev, int nbmax)
-{
-    auto al = [](int v) { return (v + 15) & ~15; };
-    LdsLayout L;
-    int o = 0;
-    L.xv = o;
-    o = al(o + xv_cap * (int)sizeof(T));
-    L.meta = o;
-    o = al(o + maxlev * (int)sizeof(LevelMeta));
-    L.vA = o;
-    o = al(o + nbmax * W * (int)sizeof(T));
-    L.vB = o;
-    o = al(o + nbmax * W * (int)sizeof(T));
-    L.dA = o;
-    o = al(o + nbmax * (int)sizeof(T));
-    L.dB = o;
-    o = al(o + nbmax * (int)sizeof(T));
-    L.cA = o;
-    o = al(o + nbmax * W * 2);
-    L.cB = o;
-    o = al(o + nbmax * W * 2);
-    L.rA = o;
-    o = al(o + nbmax * 2);
-    L.rB = o;
-    o = al(o + nbmax * 2);
-    L.bytes = o;
-    return L;
-}
-
-template <typename T, int W, int FORM, bool DIV>
-__device__ __forceinline__ void sweep_lds(int nl, const LevelMeta *sm, int off0, int roff0,
-                                          const T *v, const short *c, const short *rows,
-                                          const T *dg, T *xv)
-{
-    const int t = threadIdx.x;
-    for (int l = 0; l < nl; ++l) {
-        const LevelMeta m = sm[l];
-        if (t < m.count) {
-            const int q = m.off - off0 + t * W;
-            const int rq = m.roff - roff0 + t;
-            const int r = rows[rq];
-            T acc;
-            if (FORM == FORM_SUM) {
-                T sum = 0;
-#pragma unroll
-                for (int s = 0; s < W; ++s)
-                    sum += v[q + s] * xv[c[q + s]];
-                acc = xv[r] - sum;
-            } else {
-                acc = xv[r];
-#pragma unroll
-                for (int s = 0; s < W; ++s)
-                    acc -= v[q + s] * xv[c[q + s]];
-            }
-            if (DIV)
-                acc = acc / dg[rq];
-            xv[r] = acc;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    }
-}
-
-template <typename T, int W, bool TRANS>
-__global__ void __launch_bounds__(256)
-    k_ilu_apply_lds(const int *__restrict__ bstart, const int *__restrict__ blkA,
-                    const LevelMeta *__restrict__ metaA, const int *__restrict__ rowsA,
-                    const int *__restrict__ colA, const T *__restrict__ valA,
-                    const T *__restrict__ dglA, const int *__restrict__ blkB,
-                    const LevelMeta *__restrict__ metaB, const int *__restrict__ rowsB,
-                    const int *__restrict__ colB, const T *__restrict__ valB,
-                    const T *__restrict__ dglB, const T *__restrict__ b, T *__restrict__ x,
-                    int xv_cap, int maxlev, int nbmax, const int *skip1, const int *skip2)
-{
-    if ((skip1 && *skip1) || (skip2 && *skip2))
-        return;
-    extern __shared__ __align__(16) unsigned char rs_smem[];
-    const LdsLayout Ly = lds_layout<T, W>(xv_cap, maxlev, nbmax);
-    T *xv = (T *)(rs_smem + Ly.xv);
-    LevelMeta *sm = (LevelMeta *)(rs_smem + Ly.meta);
-    T *vA = (T *)(rs_smem + Ly.vA), *vB = (T *)(rs_smem + Ly.vB);
-    T *dA = (T *)(rs_smem + Ly.dA), *dB = (T *)(rs_smem + Ly.dB);
-    short *cA = (short *)(rs_smem + Ly.cA), *cB = (short *)(rs_smem + Ly.cB);
-    short *rA = (short *)(rs_smem + Ly.rA), *rB = (short *)(rs_smem + Ly.rB);
-    const int blk = blockIdx.x;
-    const int b0 = bstart[blk], nb = bstart[blk + 1] - b0;
-    const int la0 = blkA[blk], nla = blkA[blk + 1] - la0;
-    const int lb0 = blkB[blk], nlb = blkB[blk + 1] - lb0;
-    const int tid = threadIdx.x;
-    for (int q = tid; q < nla; q += blockDim.x)
-        sm[q] = metaA[la0 + q];
-    for (int q = tid; q < nlb; q += blockDim.x)
-        sm[nla + q] = metaB[lb0 + q];
-    // every row of the block appears once per sweep: segments of nb rows
-    // and nb * W slots starting at the first level's offsets
-    const int offA = nla ? metaA[la0].off : 0, roffA = nla ? metaA[la0].roff : 0;
-    const int offB = nlb ? metaB[lb0].off : 0, roffB = nlb ? metaB[lb0].roff : 0;
-    for (int q = tid; q < nb * W; q += blockDim.x) {
-        vA[q] = valA[offA + q];
-        cA[q] = (short)colA[offA + q];
-        vB[q] = valB[offB + q];
-        cB[q] = (short)colB[offB + q];
-    }
-    for (int q = tid; q < nb; q += blockDim.x) {
-        rA[q] = (short)rowsA[roffA + q];
-        rB[q] = (short)rowsB[roffB + q];
-        if (dglA)
-            dA[q] = dglA[roffA + q];
-        if (dglB)
-            dB[q] = dglB[roffB + q];
-        xv[q] = b[b0 + q];
-    }
-    if (tid == 0)
-        xv[nb] = T(0);
-    __syncthreads();
-    if (tid >= 64)
-        return;
-    if (!TRANS) {
-        sweep_lds<T, W, FORM_SUM, false>(nla, sm, offA, roffA, vA, cA, rA, dA, xv);
-        sweep_lds<T, W, FORM_SUM, true>(nlb, sm + nla, offB, roffB, vB, cB, rB, dB, xv);
-    } else {
-        sweep_lds<T, W, FORM_SUBSEQ, true>(nla, sm, offA, roffA, vA, cA, rA, dA, xv);
-        sweep_lds<T, W, FORM_SUBSEQ, false>(nlb, sm + nla, offB, roffB, vB, cB, rB, dB, xv);
-    }
-    for (int r = tid; r < nb; r += 64)
         x[b0 + r] = xv[r];
 }
 
@@ -930,7 +865,32 @@ int prec_factor(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
     return RSGPU_OK;
 }
 
-template <typename T, bool TRANS, int W, int WL = W>
+template <typename T>
+static PfSweep pf_sweep(const SweepData &D, const T *val, const T *dgl)
+{
+    PfSweep P;
+    P.blk = D.blk_lev.p;
+    P.meta = (const LevelMeta *)D.lev_meta.p;
+    P.rowsb = D.rowsb.p;
+    P.colb = D.colb.p;
+    P.val = val;
+    P.dgl = dgl;
+    P.rows_bytes = (int)(D.rowsb.n * 4);
+    P.col_bytes = (int)(D.colb.n * 4);
+    P.val_bytes = (int)(D.ell_size * sizeof(T));
+    P.dgl_bytes = dgl ? (int)((size_t)D.nrows_lev * sizeof(T)) : 0;
+    return P;
+}
+
+// the pipelined kernel addresses each sweep array with 32-bit byte offsets
+static bool pf_addressable(const SweepData &D, size_t es)
+{
+    const size_t lim = (size_t)INT32_MAX;
+    return D.colb.n * 4 <= lim && D.ell_size * es <= lim && D.rowsb.n * 4 <= lim &&
+           (size_t)D.nrows_lev * es <= lim;
+}
+
+template <typename T, bool TRANS, int W>
 static void launch_pf(rsgpu_prec *M, const SweepData &A_, const T *va, const T *da,
                       const SweepData &B_, const T *vb, const T *db, const T *b, T *x,
                       size_t xv_bytes, size_t shm, const int *sk1, const int *sk2)
@@ -938,15 +898,13 @@ static void launch_pf(rsgpu_prec *M, const SweepData &A_, const T *va, const T *
     // dynamic LDS beyond the 64 KiB default needs a per-kernel opt-in
     static size_t granted = 65536;
     if (shm > granted) {
-        RS_HIP(hipFuncSetAttribute((const void *)k_ilu_apply_pf<T, W, WL, TRANS>,
+        RS_HIP(hipFuncSetAttribute((const void *)k_ilu_apply_pf<T, W, TRANS>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
         granted = shm;
     }
-    hipLaunchKernelGGL((k_ilu_apply_pf<T, W, WL, TRANS>), dim3(M->nblocks), dim3(64), shm,
-                       M->ctx->stream, M->bstart.p, A_.blk_lev.p,
-                       (const LevelMeta *)A_.lev_meta.p, A_.rows.p, A_.ell_col.p, va, da,
-                       B_.blk_lev.p, (const LevelMeta *)B_.lev_meta.p, B_.rows.p,
-                       B_.ell_col.p, vb, db, b, x, (int)xv_bytes, sk1, sk2);
+    hipLaunchKernelGGL((k_ilu_apply_pf<T, W, TRANS>), dim3(M->nblocks), dim3(64), shm,
+                       M->ctx->stream, M->bstart.p, pf_sweep<T>(A_, va, da),
+                       pf_sweep<T>(B_, vb, db), b, x, (int)xv_bytes, sk1, sk2);
 }
 
 // Largest dynamic LDS a workgroup may request on this device (opt-in).
@@ -966,44 +924,14 @@ static int lds_limit(int device)
     return cached[device];
 }
 
-template <typename T, int W, bool TRANS>
-static bool try_launch_lds(rsgpu_prec *M, const SweepData &A_, const T *va, const T *da,
-                           const SweepData &B_, const T *vb, const T *db, const T *b, T *x,
-                           const int *sk1, const int *sk2)
-{
-    const int xv_cap = M->max_block_rows + 2;
-    const int maxlev = A_.max_levels + B_.max_levels;
-    const LdsLayout Ly = lds_layout<T, W>(xv_cap, maxlev, M->max_block_rows);
-    if (Ly.bytes > lds_limit(M->ctx->device))
-        return false;
-    auto kern = k_ilu_apply_lds<T, W, TRANS>;
-    static int granted = 0;
-    if (Ly.bytes > 65536 && Ly.bytes > granted) {
-        if (hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                Ly.bytes) != hipSuccess) {
-            (void)hipGetLastError();
-            return false;
-        }
-        granted = Ly.bytes;
-    }
-    hipLaunchKernelGGL(kern, dim3(M->nblocks), dim3(256), Ly.bytes, M->ctx->stream, M->bstart.p,
-                       A_.blk_lev.p, (const LevelMeta *)A_.lev_meta.p, A_.rows.p, A_.ell_col.p,
-                       va, da, B_.blk_lev.p, (const LevelMeta *)B_.lev_meta.p, B_.rows.p,
-                       B_.ell_col.p, vb, db, b, x, xv_cap, maxlev, M->max_block_rows, sk1, sk2);
-    RS_CHECK_LAUNCH();
-    return true;
-}
-
-// Sweep kernel choice: RSGPU_ILU_KERNEL = lds | pf | generic forces one
-// (tests and tuning); by default the LDS-resident kernel when the block
-// segment fits, else the pipelined one, else the generic one.
+// Sweep kernel choice: RSGPU_ILU_KERNEL = pf | generic forces one (tests
+// and tuning); by default the pipelined kernel wherever it applies, else
+// the generic one.
 static int ilu_kernel_choice()
 {
     const char *e = getenv("RSGPU_ILU_KERNEL");
     if (!e)
         return 0;
-    if (!strcmp(e, "lds"))
-        return 1;
     if (!strcmp(e, "pf"))
         return 2;
     if (!strcmp(e, "generic"))
@@ -1021,28 +949,6 @@ static void launch_apply(rsgpu_prec *M, const SweepData &A_, const T *va, const 
         return;
     const int maxc = std::max(A_.max_count, B_.max_count);
     const int choice = ilu_kernel_choice();
-    if ((choice == 0 || choice == 1) && maxc <= 64 && A_.W == B_.W &&
-        M->max_block_rows + 2 <= 32767) {
-        bool ok = false;
-        switch (A_.W) {
-        case 4:
-            ok = try_launch_lds<T, 4, TRANS>(M, A_, va, da, B_, vb, db, b, x, sk1, sk2);
-            break;
-        case 8:
-            ok = try_launch_lds<T, 8, TRANS>(M, A_, va, da, B_, vb, db, b, x, sk1, sk2);
-            break;
-        case 12:
-            ok = try_launch_lds<T, 12, TRANS>(M, A_, va, da, B_, vb, db, b, x, sk1, sk2);
-            break;
-        case 16:
-            ok = try_launch_lds<T, 16, TRANS>(M, A_, va, da, B_, vb, db, b, x, sk1, sk2);
-            break;
-        default:
-            break;
-        }
-        if (ok)
-            return;
-    }
     const bool lds = M->max_block_rows <= (int)(RS_LDS_BYTES / sizeof(T)) - 2;
     const size_t xv_bytes = ((size_t)(M->max_block_rows + 2) * sizeof(T) + 15) & ~(size_t)15;
     const size_t pf_shm =
@@ -1051,17 +957,9 @@ static void launch_apply(rsgpu_prec *M, const SweepData &A_, const T *va, const 
     // the pipelined kernel keeps the vector in LDS up to the device's
     // opt-in limit (160 KiB on gfx950: 20k FP64 / 40k FP32 rows per block)
     if (choice != 3 && maxc <= 64 && A_.W == B_.W && W <= 16 &&
-        pf_shm <= (size_t)lds_limit(M->ctx->device)) {
-        // used width: the widest row of either sweep (3 for 7-point and
-        // 13 for 27-point stencils get their own instances)
-        const int wu = std::max(A_.max_width, B_.max_width);
-        if (W == 4 && wu <= 3)
-            launch_pf<T, TRANS, 4, 3>(M, A_, va, da, B_, vb, db, b, x, xv_bytes, pf_shm, sk1,
-                                      sk2);
-        else if (W == 16 && wu <= 13)
-            launch_pf<T, TRANS, 16, 13>(M, A_, va, da, B_, vb, db, b, x, xv_bytes, pf_shm, sk1,
-                                        sk2);
-        else if (W <= 4)
+        pf_shm <= (size_t)lds_limit(M->ctx->device) && pf_addressable(A_, sizeof(T)) &&
+        pf_addressable(B_, sizeof(T))) {
+        if (W <= 4)
             launch_pf<T, TRANS, 4>(M, A_, va, da, B_, vb, db, b, x, xv_bytes, pf_shm, sk1, sk2);
         else if (W <= 8)
             launch_pf<T, TRANS, 8>(M, A_, va, da, B_, vb, db, b, x, xv_bytes, pf_shm, sk1, sk2);

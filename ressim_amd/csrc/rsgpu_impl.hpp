@@ -121,6 +121,7 @@ struct SweepData {
     DevBuf<int> rows;        // local row per (level,t)
     DevBuf<int> row_glob;    // global row per (level,t) (diagonal gather)
     DevBuf<int> ell_col;     // local column per slot (nb = zero slot)
+    DevBuf<int> colb, rowsb; // the same as LDS byte offsets (times element size)
     DevBuf<int> ell_src;     // source position in LU, -1 padding
     size_t ell_size = 0;
     int nrows_lev = 0;
