@@ -16,18 +16,18 @@
 // reference's per-row operation sequence, so factors and sweeps are
 // bit-identical to the sequential loops.
 //
-// Sweep layout.  The block's vector lives in LDS (blocks up to 8191 FP64 /
-// 16383 FP32 rows; larger blocks use a global scratch region) and is
-// initialised with the block's right-hand side, so both sweeps run in place.
-// The factor values of each level are stored lane-major: row t of a level
-// owns W consecutive (value, local column) slots, W the sweep's largest row
-// padded to a multiple of 4, so a lane fetches its operands with a few
-// 16-byte loads.  Padding slots are identity updates (value -0.0 for the
-// sum form, +0.0 for the subtract form, column = the zero slot), which keeps
-// results bit-identical including signed zeros.  The U sweeps' diagonals
-// are stored in level order.  With one wave per block, the apply kernel
-// keeps four levels of operands in flight (depth-4 register pipeline); the
-// barrier between levels only orders LDS traffic, so the loads cross it.
+// Sweep layout.  The block's vector lives in LDS and is initialised with the
+// block's right-hand side, so both sweeps run in place (the pipelined kernel
+// up to the device's opt-in LDS limit; the generic kernel up to 64 KiB, and
+// through a global scratch region beyond).  The factor values of each level
+// are stored lane-major: row t of a level owns W consecutive (value, local
+// column) slots, W the sweep's largest row padded to a multiple of 4, so a
+// lane fetches its operands with a few 16-byte loads.  Padding slots are
+// identity updates (value -0.0 for the sum form, +0.0 for the subtract
+// form, column = the zero slot), which keeps results bit-identical including
+// signed zeros.  The U sweeps' diagonals are stored in level order.  With
+// one wave per block, the pipelined apply keeps up to eight levels of
+// operands in flight in a register ring (see sweep_pf).
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
