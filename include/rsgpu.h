@@ -127,6 +127,13 @@ enum rsgpu_precision { RSGPU_FP64 = 0, RSGPU_FP32 = 1 };
  * precision: arithmetic of the factors and sweeps. */
 int rsgpu_prec_create(rsgpu_ctx *ctx, int kind, int nblocks, int precision,
                       rsgpu_prec **out);
+/* BLOCK_ILU0 only: the caller's block boundaries instead of equal ranges,
+ * bstart_host[0] = 0 < bstart_host[1] < ... < bstart_host[nblocks] = n, as
+ * the row ownership of a DistRowMatrix whose ranks own unequal row counts
+ * (distributed/DistRowMatrix.java:56-76).  Blocks aligned to the systems of
+ * a block-diagonal batch make the block-Jacobi ILU(0) exactly the ILU(0) of
+ * each system.  Takes effect at the next rsgpu_prec_set_matrix. */
+int rsgpu_prec_set_blocks(rsgpu_prec *M, int nblocks, const int32_t *bstart_host);
 int rsgpu_prec_destroy(rsgpu_prec *M);
 /* setMatrix(A) (ILU.java:80-84): copy values, factor.  The pattern analysis
  * (level schedule) is done on the first call and reused while A's pattern
@@ -251,6 +258,51 @@ int rsgpu_solver_solve_host(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M,
                             rsgpu_stats *st);
 /* Residuals the monitor saw, in order (IterationReporter.monitor replay). */
 int rsgpu_solver_history(const rsgpu_solver *S, double *out_host, int cap);
+
+/* ---- Batched ensemble solves (no reference counterpart) ------------------ */
+/* The reference solves one system per BiCGstab.solve call.  A batch solves
+ * many independent systems in the same launches: the systems are the
+ * diagonal blocks of one block-diagonal matrix, and each keeps its own
+ * BiCGstab scalars and DefaultIterationMonitor (BiCGstab.java:65-116,
+ * DefaultIterationMonitor.java:133-156), so each stops on its own. */
+typedef struct rsgpu_batch rsgpu_batch;
+
+/* The block-diagonal matrix diag(parts[0], ..., parts[count-1]): pattern
+ * and current values copied (row and column offsets added). */
+int rsgpu_csr_block_diag(rsgpu_ctx *ctx, int count, rsgpu_csr *const *parts,
+                         rsgpu_csr **out);
+
+enum rsgpu_batch_flags {
+    RSGPU_BATCH_NO_GRAPH = 1,     /* enqueue iterations directly instead of
+                                     replaying a captured HIP graph */
+    RSGPU_BATCH_NO_AUTOTUNE = 2   /* default SpMV stage sizes, no timing */
+};
+/* nsys systems; system k owns rows sys_start_host[k] ..
+ * sys_start_host[k+1]-1 (strictly rising from 0).  opts: kind must be
+ * RSGPU_BICGSTAB; max_iter/rtol/atol/dtol are each system's monitor.  With
+ * opts->mixed each system runs FP64 refinement on its true residual
+ * (IR.java:57-70) around an FP32 BiCGstab stopped at inner_rtol /
+ * inner_max_iter, independently of the other systems.  check_every =
+ * iterations per captured graph (the host reads one flag per replay). */
+int rsgpu_batch_create(rsgpu_ctx *ctx, const rsgpu_solver_opts *opts, int nsys,
+                       const int32_t *sys_start_host, int flags,
+                       rsgpu_batch **out);
+int rsgpu_batch_destroy(rsgpu_batch *B);
+/* Solve every system of A x = b from the initial guess x (device vectors
+ * of the batch's total size).  M: NULL, DIAGONAL, ILU0 or BLOCK_ILU0 whose
+ * blocks do not cross system boundaries (rsgpu_prec_set_blocks), FP64 for
+ * the plain form and FP32 for the mixed form; set on A.  stats_host (may be
+ * NULL) receives nsys records.  Returns RSGPU_OK when every system
+ * converged, else the status of the first system that did not. */
+int rsgpu_batch_solve(rsgpu_batch *B, rsgpu_csr *A, rsgpu_prec *M,
+                      const double *b_dev, double *x_dev,
+                      rsgpu_stats *stats_host);
+/* The SpMV stage sizes (nonzeros per workgroup) the autotuner picked for
+ * the last matrix pattern, their timed durations (microseconds; 0 without
+ * tuning), and the graph replays of the last solve. */
+int rsgpu_batch_info(const rsgpu_batch *B, int *spmv_stage_fp64,
+                     int *spmv_stage_fp32, double *spmv_us_fp64,
+                     double *spmv_us_fp32, int *graph_launches);
 
 /* ---- MPFA-O assembly (rs geometry/flux/O_MPFA.java, LinearSolver.java) --- */
 /* Cartesian grid, dim 2 or 3 (nz ignored for dim 2), K6_host per cell

@@ -20,6 +20,7 @@ HIP_ERROR, OUT_OF_MEMORY, NOT_BUILT = -1, -2, -3
 PREC_IDENTITY, PREC_ILU0, PREC_BLOCK_ILU0, PREC_DIAGONAL, PREC_AMG, PREC_SSOR = 0, 1, 2, 3, 4, 5
 FP64, FP32 = 0, 1
 BICGSTAB, GMRES, IR, CG = 0, 1, 2, 3
+BATCH_NO_GRAPH, BATCH_NO_AUTOTUNE = 1, 2
 
 
 class SolverOpts(C.Structure):
@@ -101,6 +102,14 @@ SIGNATURES = {
     "rsgpu_solver_solve": (C.c_int, [_h, _h, _h, _dp, _dp, C.POINTER(Stats)]),
     "rsgpu_solver_solve_host": (C.c_int, [_h, _h, _h, _dp, _dp, C.POINTER(Stats)]),
     "rsgpu_solver_history": (C.c_int, [_h, _dp, C.c_int]),
+    "rsgpu_prec_set_blocks": (C.c_int, [_h, C.c_int, _ip]),
+    "rsgpu_csr_block_diag": (C.c_int, [_h, C.c_int, C.POINTER(C.c_void_p), _hp]),
+    "rsgpu_batch_create": (C.c_int, [_h, C.POINTER(SolverOpts), C.c_int, _ip, C.c_int, _hp]),
+    "rsgpu_batch_destroy": (C.c_int, [_h]),
+    "rsgpu_batch_solve": (C.c_int, [_h, _h, _h, _dp, _dp, C.POINTER(Stats)]),
+    "rsgpu_batch_info": (C.c_int, [_h, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                   C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_int)]),
     "rsgpu_mpfa_o": (C.c_int, [_h, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                C.c_double, C.c_double, _dp, C.c_double, _ip, _ip, _dp,
                                C.POINTER(C.c_int)]),

@@ -507,6 +507,25 @@ extern "C" int rsgpu_prec_set_ssor(rsgpu_prec *M, double omega_f, double omega_r
     return RSGPU_OK;
 }
 
+extern "C" int rsgpu_prec_set_blocks(rsgpu_prec *M, int nblocks, const int32_t *bstart_host)
+{
+    if (!M || M->kind != RSGPU_PREC_BLOCK_ILU0 || nblocks < 1 || !bstart_host)
+        return RSGPU_BAD_ARGUMENT;
+    if (bstart_host[0] != 0) {
+        M->ctx->err = "block boundaries must start at row 0";
+        return RSGPU_BAD_ARGUMENT;
+    }
+    for (int b = 0; b < nblocks; ++b)
+        if (bstart_host[b + 1] <= bstart_host[b]) {
+            M->ctx->err = "block boundaries must rise strictly";
+            return RSGPU_BAD_ARGUMENT;
+        }
+    M->user_bstart.assign(bstart_host, bstart_host + nblocks + 1);
+    M->nblocks = nblocks;
+    M->ready = false;
+    return RSGPU_OK;
+}
+
 extern "C" int rsgpu_prec_set_amg_params(rsgpu_prec *M, const rsgpu_amg_params *p)
 {
     if (!M || !p || M->kind != RSGPU_PREC_AMG)

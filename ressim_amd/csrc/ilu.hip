@@ -55,10 +55,30 @@ static int pad_width(int w)
     return (w + 3) & ~3;
 }
 
-// Levels of one triangular part.  ent[r] lists (column, LU position) in
-// operation order.  forward: dependencies on smaller rows; else larger.
-static void build_sweep(int nblocks, const std::vector<int> &bstart,
-                        const std::vector<std::vector<std::pair<int, int>>> &ent,
+// Entries of one triangular part in operation order.  Row r's entries are
+// positions j in [begin(r), end(r)), each with a column col(j) and a position
+// pos(j) in the factor's CSR values.
+struct RangeEntries {   // a contiguous range of each factor row (L or U)
+    const int *b0, *e0;  // begin/end per row
+    int boff, eoff;      // added to b0[r] / e0[r]
+    const int *ci;
+    int begin(int r) const { return b0[r] + boff; }
+    int end(int r) const { return e0[r] + eoff; }
+    int col(int j) const { return ci[j]; }
+    int pos(int j) const { return j; }
+};
+struct FlatEntries {    // an explicit list per row (transposed factors)
+    std::vector<int> ptr, cols, poss;
+    int begin(int r) const { return ptr[r]; }
+    int end(int r) const { return ptr[r + 1]; }
+    int col(int j) const { return cols[j]; }
+    int pos(int j) const { return poss[j]; }
+};
+
+// Levels of one triangular part.  forward: dependencies on smaller rows;
+// else larger.
+template <class Ent>
+static void build_sweep(int nblocks, const std::vector<int> &bstart, const Ent &ent,
                         bool forward, int W, HostSweep &H)
 {
     H.W = W;
@@ -72,50 +92,64 @@ static void build_sweep(int nblocks, const std::vector<int> &bstart,
     H.max_width = 0;
     const int n = bstart[nblocks];
     std::vector<int> level(n, 0);
+    H.rows.reserve(n);
+    H.row_glob.reserve(n + 1);
+    H.col.reserve((size_t)W * n + W);
+    H.src.reserve((size_t)W * n + W);
+    std::vector<int> cnt, first, order;
     for (int b = 0; b < nblocks; ++b) {
         const int b0 = bstart[b], b1 = bstart[b + 1], nb = b1 - b0;
         int maxl = -1;
         if (forward) {
             for (int r = b0; r < b1; ++r) {
                 int l = 0;
-                for (auto &e : ent[r])
-                    l = std::max(l, level[e.first] + 1);
+                for (int j = ent.begin(r); j < ent.end(r); ++j)
+                    l = std::max(l, level[ent.col(j)] + 1);
                 level[r] = l;
                 maxl = std::max(maxl, l);
             }
         } else {
             for (int r = b1 - 1; r >= b0; --r) {
                 int l = 0;
-                for (auto &e : ent[r])
-                    l = std::max(l, level[e.first] + 1);
+                for (int j = ent.begin(r); j < ent.end(r); ++j)
+                    l = std::max(l, level[ent.col(j)] + 1);
                 level[r] = l;
                 maxl = std::max(maxl, l);
             }
         }
         const int nlev = maxl + 1;
-        std::vector<std::vector<int>> byl(nlev);
+        // rows by level, ascending within a level (counting sort)
+        cnt.assign(nlev + 1, 0);
         for (int r = b0; r < b1; ++r)
-            byl[level[r]].push_back(r);
+            cnt[level[r] + 1]++;
+        for (int l = 0; l < nlev; ++l)
+            cnt[l + 1] += cnt[l];
+        first.assign(cnt.begin(), cnt.end());
+        order.resize(nb);
+        for (int r = b0; r < b1; ++r)
+            order[first[level[r]]++] = r;
         H.blk_lev[b] = (int)H.meta.size();
         for (int l = 0; l < nlev; ++l) {
             LevelMeta m;
-            m.count = (int)byl[l].size();
+            m.count = cnt[l + 1] - cnt[l];
             m.off = (int)H.col.size();
             m.roff = (int)H.rows.size();
             m.pad = 0;
             H.max_count = std::max(H.max_count, m.count);
-            for (int r : byl[l]) {
+            for (int t = 0; t < m.count; ++t) {
+                const int r = order[cnt[l] + t];
                 H.rows.push_back(r - b0);
                 H.row_glob.push_back(r);
             }
             H.col.resize(H.col.size() + (size_t)W * m.count, nb);
             H.src.resize(H.src.size() + (size_t)W * m.count, -1);
             for (int t = 0; t < m.count; ++t) {
-                const auto &E = ent[byl[l][t]];
-                H.max_width = std::max(H.max_width, (int)E.size());
-                for (int s = 0; s < (int)E.size(); ++s) {
-                    H.col[m.off + (size_t)t * W + s] = E[s].first - b0;
-                    H.src[m.off + (size_t)t * W + s] = E[s].second;
+                const int r = order[cnt[l] + t];
+                const int j0 = ent.begin(r), j1 = ent.end(r);
+                H.max_width = std::max(H.max_width, j1 - j0);
+                for (int j = j0; j < j1; ++j) {
+                    H.col[m.off + (size_t)t * W + (j - j0)] = ent.col(j) - b0;
+                    H.src[m.off + (size_t)t * W + (j - j0)] = ent.pos(j);
                 }
             }
             H.meta.push_back(m);
@@ -128,12 +162,13 @@ static void build_sweep(int nblocks, const std::vector<int> &bstart,
     H.row_glob.push_back(0);
 }
 
-static int max_entries(const std::vector<std::vector<std::pair<int, int>>> &ent)
+template <class Ent>
+static int max_entries(const Ent &ent, int n)
 {
-    size_t w = 0;
-    for (auto &e : ent)
-        w = std::max(w, e.size());
-    return (int)w;
+    int w = 0;
+    for (int r = 0; r < n; ++r)
+        w = std::max(w, ent.end(r) - ent.begin(r));
+    return w;
 }
 
 // es: element size of the apply's vector; the pipelined kernel reads row
@@ -216,8 +251,19 @@ void prec_setup(rsgpu_prec *M, rsgpu_csr *A)
         nb = 1;
     M->nblocks = nb;
     M->h_bstart.resize(nb + 1);
-    for (int b = 0; b <= nb; ++b)
-        M->h_bstart[b] = (int)(((long long)n * b) / nb);
+    if (!M->user_bstart.empty()) {
+        const std::vector<int> &u = M->user_bstart;
+        bool ok = (int)u.size() == nb + 1 && u[0] == 0 && u[nb] == n;
+        for (int b = 0; ok && b < nb; ++b)
+            ok = u[b] < u[b + 1];
+        if (!ok)
+            throw RsError{RSGPU_BAD_ARGUMENT,
+                          "block boundaries must rise strictly from 0 to the matrix size"};
+        M->h_bstart = u;
+    } else {
+        for (int b = 0; b <= nb; ++b)
+            M->h_bstart[b] = (int)(((long long)n * b) / nb);
+    }
     M->max_block_rows = 0;
     for (int b = 0; b < nb; ++b)
         M->max_block_rows =
@@ -267,20 +313,19 @@ void prec_setup(rsgpu_prec *M, rsgpu_csr *A)
     if (!ident)
         M->src.upload(src, s);
 
-    // L (unit lower, forward), U (upper, backward) in operation order
-    std::vector<std::vector<std::pair<int, int>>> eL(n), eU(n);
-    for (int r = 0; r < n; ++r) {
-        for (int j = rp[r]; j < dg[r]; ++j)
-            eL[r].push_back({ci[j], j});
-        for (int j = dg[r] + 1; j < rp[r + 1]; ++j)
-            eU[r].push_back({ci[j], j});
+    // L (unit lower, forward), U (upper, backward) in operation order: the
+    // factor row's entries left and right of the diagonal
+    const RangeEntries eL{rp.data(), dg.data(), 0, 0, ci.data()};
+    const RangeEntries eU{dg.data(), rp.data() + 1, 1, 0, ci.data()};
+    const int W = pad_width(std::max(max_entries(eL, n), max_entries(eU, n)));
+    {
+        HostSweep H;
+        build_sweep(nb, M->h_bstart, eL, true, W, H);
+        upload_sweep(H, M->L, s, M->precision == RSGPU_FP64 ? 8 : 4);
+        build_sweep(nb, M->h_bstart, eU, false, W, H);
+        upload_sweep(H, M->U, s, M->precision == RSGPU_FP64 ? 8 : 4);
+        RS_HIP(hipStreamSynchronize(s));   // H's host arrays are released next
     }
-    const int W = pad_width(std::max(max_entries(eL), max_entries(eU)));
-    HostSweep H;
-    build_sweep(nb, M->h_bstart, eL, true, W, H);
-    upload_sweep(H, M->L, s, M->precision == RSGPU_FP64 ? 8 : 4);
-    build_sweep(nb, M->h_bstart, eU, false, W, H);
-    upload_sweep(H, M->U, s, M->precision == RSGPU_FP64 ? 8 : 4);
     M->status.alloc(2);
     if (M->precision == RSGPU_FP64) {
         M->lu64.alloc(std::max(M->nnz, 1));
@@ -318,14 +363,42 @@ static void prec_setup_transpose(rsgpu_prec *M)
     for (int k = 0; k < n; ++k)
         dg[k] = (int)(std::lower_bound(ci.begin() + rp[k], ci.begin() + rp[k + 1], k) -
                       ci.begin());
-    std::vector<std::vector<std::pair<int, int>>> eUt(n), eLt(n);
-    for (int i = 0; i < n; ++i)
+    // U' row c: the rows i < c with U(i, c) != 0, ascending i; L' row c: the
+    // rows i > c with L(i, c) != 0, descending i (the column sweeps' order)
+    FlatEntries eUt, eLt;
+    eUt.ptr.assign(n + 1, 0);
+    eLt.ptr.assign(n + 1, 0);
+    for (int i = 0; i < n; ++i) {
         for (int j = dg[i] + 1; j < rp[i + 1]; ++j)
-            eUt[ci[j]].push_back({i, j});
-    for (int i = n - 1; i >= 0; --i)
+            eUt.ptr[ci[j] + 1]++;
         for (int j = rp[i]; j < dg[i]; ++j)
-            eLt[ci[j]].push_back({i, j});
-    const int W = pad_width(std::max(max_entries(eUt), max_entries(eLt)));
+            eLt.ptr[ci[j] + 1]++;
+    }
+    for (int c = 0; c < n; ++c) {
+        eUt.ptr[c + 1] += eUt.ptr[c];
+        eLt.ptr[c + 1] += eLt.ptr[c];
+    }
+    eUt.cols.resize(eUt.ptr[n]);
+    eUt.poss.resize(eUt.ptr[n]);
+    eLt.cols.resize(eLt.ptr[n]);
+    eLt.poss.resize(eLt.ptr[n]);
+    {
+        std::vector<int> fu(eUt.ptr.begin(), eUt.ptr.end() - 1);
+        std::vector<int> fl(eLt.ptr.begin(), eLt.ptr.end() - 1);
+        for (int i = 0; i < n; ++i)
+            for (int j = dg[i] + 1; j < rp[i + 1]; ++j) {
+                const int q = fu[ci[j]]++;
+                eUt.cols[q] = i;
+                eUt.poss[q] = j;
+            }
+        for (int i = n - 1; i >= 0; --i)
+            for (int j = rp[i]; j < dg[i]; ++j) {
+                const int q = fl[ci[j]]++;
+                eLt.cols[q] = i;
+                eLt.poss[q] = j;
+            }
+    }
+    const int W = pad_width(std::max(max_entries(eUt, n), max_entries(eLt, n)));
     HostSweep H;
     build_sweep(M->nblocks, M->h_bstart, eUt, true, W, H);
     upload_sweep(H, M->Ut, s, M->precision == RSGPU_FP64 ? 8 : 4);
@@ -475,9 +548,11 @@ __global__ void __launch_bounds__(256)
                 const int *__restrict__ rowsB, const int *__restrict__ colB,
                 const T *__restrict__ valB, const T *__restrict__ dglB,
                 const T *__restrict__ b, T *__restrict__ x, T *scratch,
-                const int *skip1, const int *skip2)
+                const int *skip1, const int *skip2, BlkSkip bskip)
 {
     if ((skip1 && *skip1) || (skip2 && *skip2))
+        return;
+    if (blk_skipped(bskip, blockIdx.x))
         return;
     extern __shared__ unsigned char rs_smem[];
     const int blk = blockIdx.x;
@@ -761,9 +836,11 @@ template <typename T, int W, bool TRANS>
 __global__ void __launch_bounds__(64)
     k_ilu_apply_pf(const int *__restrict__ bstart, PfSweep A, PfSweep B,
                    const T *__restrict__ b, T *__restrict__ x, int xv_bytes,
-                   const int *skip1, const int *skip2)
+                   const int *skip1, const int *skip2, BlkSkip bskip)
 {
     if ((skip1 && *skip1) || (skip2 && *skip2))
+        return;
+    if (blk_skipped(bskip, blockIdx.x))
         return;
     extern __shared__ unsigned char rs_smem[];
     const int blk = blockIdx.x;
@@ -904,7 +981,7 @@ static void launch_pf(rsgpu_prec *M, const SweepData &A_, const T *va, const T *
     }
     hipLaunchKernelGGL((k_ilu_apply_pf<T, W, TRANS>), dim3(M->nblocks), dim3(64), shm,
                        M->ctx->stream, M->bstart.p, pf_sweep<T>(A_, va, da),
-                       pf_sweep<T>(B_, vb, db), b, x, (int)xv_bytes, sk1, sk2);
+                       pf_sweep<T>(B_, vb, db), b, x, (int)xv_bytes, sk1, sk2, M->skip);
 }
 
 // Largest dynamic LDS a workgroup may request on this device (opt-in).
@@ -978,7 +1055,7 @@ static void launch_apply(rsgpu_prec *M, const SweepData &A_, const T *va, const 
     hipLaunchKernelGGL(kern, dim3(M->nblocks), dim3(bs), shm, s, M->bstart.p, A_.W,
                        A_.blk_lev.p, (const LevelMeta *)A_.lev_meta.p, A_.rows.p,
                        A_.ell_col.p, va, da, B_.blk_lev.p, (const LevelMeta *)B_.lev_meta.p,
-                       B_.rows.p, B_.ell_col.p, vb, db, b, x, scratch, sk1, sk2);
+                       B_.rows.p, B_.ell_col.p, vb, db, b, x, scratch, sk1, sk2, M->skip);
     RS_CHECK_LAUNCH();
 }
 

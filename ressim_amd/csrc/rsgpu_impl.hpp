@@ -131,13 +131,29 @@ struct SweepData {
 struct AmgData;
 void amg_destroy(AmgData *d);
 
+// Block b of a block-Jacobi sweep is skipped while
+// done[(blk_sys ? blk_sys[b] : b) * stride] != 0 (the done flag of the
+// system that owns the block, in an array of per-system states).
+struct BlkSkip {
+    const int *blk_sys = nullptr;
+    const int *done = nullptr;
+    int stride = 0;
+};
+
+__device__ __forceinline__ bool blk_skipped(const BlkSkip &k, int b)
+{
+    return k.done && k.done[(size_t)(k.blk_sys ? k.blk_sys[b] : b) * k.stride];
+}
+
 struct rsgpu_prec {
     rsgpu_ctx *ctx = nullptr;
     int kind = 0, nblocks = 1, precision = RSGPU_FP64;
     int n = 0;
     uint64_t pattern_id = 0;
     bool ready = false;
-    // block structure
+    // block structure (user_bstart: caller-given block boundaries, else
+    // nblocks equal contiguous ranges)
+    std::vector<int> user_bstart;
     std::vector<int> h_bstart;
     DevBuf<int> bstart;
     int max_block_rows = 0;
@@ -168,6 +184,9 @@ struct rsgpu_prec {
     // DiagonalPreconditioner (ilu.hip)
     DevBuf<double> invdiag;
     DevBuf<float> invdiag32;
+    // per-block skip for the ILU sweeps (batched solves); set only while a
+    // batched solve enqueues its applies
+    BlkSkip skip{};
     ~rsgpu_prec() { amg_destroy(amg); }
 };
 

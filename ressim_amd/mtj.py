@@ -395,6 +395,13 @@ class BlockILU(Preconditioner):
                  ctx: Optional[Context] = None):
         super().__init__(nblocks, precision, ctx)
 
+    def setBlocks(self, bstart):
+        """Caller-given block boundaries (0 = bstart[0] < ... < bstart[-1] =
+        n), as a DistRowMatrix whose ranks own unequal row ranges; takes
+        effect at the next setMatrix."""
+        bs = np.ascontiguousarray(bstart, np.int32)
+        _raise(L.load().rsgpu_prec_set_blocks(self.h, len(bs) - 1, _hptr(bs)), self.ctx.h)
+
 
 class DiagonalPreconditioner(Preconditioner):
     """mtj/.../sparse/DiagonalPreconditioner.java: x = b / diag(A)."""

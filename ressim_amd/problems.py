@@ -134,6 +134,32 @@ def config2(seed: int = 2, nx: int = 128, ny: int = 128, nz: int = 64,
                            accumulation=accumulation)
 
 
+def config4(seed: int = 4, count: int = 64, emin: int = 32, emax: int = 96,
+            sigma: float = 1.0, accumulation: float = 1e-3) -> list:
+    """Ensemble of ``count`` single-phase pressure systems on cubic grids,
+    edge drawn uniformly from {emin..emax}; system k has its own log-normal
+    permeability (sigma 1, normal stream k+1), isotropic with the
+    0.1 sqrt(k_ii k_jj) off-diagonals; +1 / -1 in opposite corners.  With
+    accumulation = 0 the systems are incompressible and the centre cell is
+    locked, as the 2-D potential problem; otherwise each is the pressure
+    Newton Jacobian with that accumulation diagonal and no locked cell, as
+    the 128x128x64 system."""
+    u = uniform01(seed, 0, count)
+    edges = emin + np.floor(u * (emax - emin + 1)).astype(np.int64)
+    out = []
+    for k, e in enumerate(edges.tolist()):
+        n = e * e * e
+        kk = np.exp(sigma * normal(seed, 1 + k, n))
+        K = full_tensor(kk, kk, kk)
+        src, pin = _pm1(n, 3, e, e, e)
+        if accumulation:
+            pin = []
+        out.append(PressureProblem("C4-%d-%dx%dx%d" % (k, e, e, e), 3, e, e, e, 1.0, 1.0, -1.0,
+                                   K, sources=src, pinned=pin, seed=seed,
+                                   accumulation=accumulation))
+    return out
+
+
 def config5(seed: int = 5, nx: int = 64, ny: int = 64, nz: int = 32,
             sigma: float = 1.0) -> PressureProblem:
     """3-D 64x64x32 with scalar log-permeability (sigma 1) times a fixed
