@@ -106,6 +106,21 @@ int rsgpu_csr_sync_f32(rsgpu_csr *A);
  * scatter visits them, CompRowMatrix.java:356-372); values copied now. */
 int rsgpu_csr_transpose(const rsgpu_csr *A, rsgpu_csr **At);
 int rsgpu_csr_mult_f32(rsgpu_csr *A, const float *x_dev, float *y_dev);
+/* 3x3 block rows (no MTJ counterpart: MTJ has no block-CSR storage).  The
+ * block pattern browptr (nb+1) / bcolind (nnzb; sorted per block row as
+ * Arrays.sort would, CompRowMatrix.java:188) defines the expanded scalar
+ * matrix of 3 nb rows in which block row I is scalar rows 3I..3I+2, each
+ * listing its blocks' three columns in ascending order.  The value array
+ * (getData, rsgpu_csr_set_values) is that expanded CSR's, so value
+ * (row a, block b, column c) of block row I sits at 9 browptr[I] +
+ * 3 a (browptr[I+1] - browptr[I]) + 3 (b - browptr[I]) + c.  Every CSR entry
+ * point accepts the matrix (mult, solvers and ILU act on the expanded
+ * matrix); SpMV reads one block column index per block. */
+int rsgpu_csr_create_bsr3(rsgpu_ctx *ctx, int nb, int nnzb,
+                          const int32_t *browptr_host,
+                          const int32_t *bcolind_host, rsgpu_csr **out);
+/* Block size (1 or 3), block rows and block nonzeros (scalar ones for bs 1). */
+int rsgpu_csr_block_info(const rsgpu_csr *A, int *bs, int *nb, int *nnzb);
 /* Host-array conveniences (DenseVector.getData() round trip). */
 int rsgpu_csr_mult_host(rsgpu_csr *A, const double *x_host, double *y_host);
 int rsgpu_csr_transmult_host(rsgpu_csr *A, const double *x_host,
@@ -258,6 +273,48 @@ int rsgpu_solver_solve_host(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M,
                             rsgpu_stats *st);
 /* Residuals the monitor saw, in order (IterationReporter.monitor replay). */
 int rsgpu_solver_history(const rsgpu_solver *S, double *out_host, int cap);
+
+/* ---- Fully-implicit black-oil Jacobian (no reference counterpart) -------- */
+/* rs is IMPES (numerics/Discretisation.java:208-225 solves pressure only);
+ * this is the Newton Jacobian of a three-phase black-oil time step with
+ * dissolved gas on a Cartesian grid: unknowns (p, Sw, Sg) per cell, TPFA
+ * 7-point fluxes with phase upwinding and gravity, Corey relative
+ * permeabilities in CoreyRockFluid's form (rock/CoreyRockFluid.java:
+ * 129-152, integer exponents), linear shrinkage factors b = 1/B for water
+ * and oil, b_g and Rs proportional to p.  Residual rows are the water, oil
+ * and gas component balances in surface volumes. */
+typedef struct {
+    double dx, dy, dz;      /* cell sizes [m] */
+    double dt;              /* time step [s] */
+    double g;               /* gravity [m/s^2]; depth grows with k */
+    double phi0, cr;        /* porosity at pref, rock compressibility [1/Pa] */
+    double pref;            /* reference pressure [Pa] */
+    double kz_ratio;        /* k_z / k_h */
+    double bw0, cw;         /* bw = bw0 (1 + cw (p - pref)) */
+    double bo0, co;         /* bo = bo0 (1 + co (p - pref)) */
+    double bg0;             /* bg = bg0 p / pref */
+    double rs0;             /* Rs = rs0 p / pref */
+    double muw, muo, mug;   /* viscosities [Pa s] */
+    double rhow_s, rhoo_s, rhog_s; /* surface densities [kg/m^3] */
+    double swr, sor, sgr;   /* Corey residual saturations (Swi, Sor, Sgr) */
+    double kw, ko, kg;      /* Corey end-point relative permeabilities */
+    int nw, no, ng, pad;    /* Corey exponents (integers 0..16) */
+} rsgpu_blackoil_params;
+typedef struct rsgpu_blackoil rsgpu_blackoil;
+/* nx*ny*nz cells numbered i + j nx + k nx ny; kh_host = horizontal
+ * permeability per cell [m^2]. */
+int rsgpu_blackoil_create(rsgpu_ctx *ctx, const rsgpu_blackoil_params *p,
+                          int nx, int ny, int nz, const double *kh_host,
+                          rsgpu_blackoil **out);
+int rsgpu_blackoil_destroy(rsgpu_blackoil *B);
+/* The Jacobian's matrix: 3x3 blocks on the 7-point pattern
+ * (rsgpu_csr_create_bsr3 layout), values zero. */
+int rsgpu_blackoil_matrix(rsgpu_blackoil *B, rsgpu_csr **J_out);
+/* J = dR/du and R(u) at state u_dev (3 per cell: p, Sw, Sg) with the
+ * previous time level u_old_dev; J or residual_dev may be NULL. */
+int rsgpu_blackoil_assemble(rsgpu_blackoil *B, const double *u_dev,
+                            const double *u_old_dev, rsgpu_csr *J,
+                            double *residual_dev);
 
 /* ---- Batched ensemble solves (no reference counterpart) ------------------ */
 /* The reference solves one system per BiCGstab.solve call.  A batch solves

@@ -218,6 +218,30 @@ void or_cell_rates(int nconn, const int *here, const int *there,
                    const double *flux, int nsel, const int *cells,
                    double *rates);
 
+/* ---- Fully-implicit black-oil Jacobian (blackoil.c) ---------------------- */
+typedef struct {
+    double dx, dy, dz;      /* cell sizes [m] */
+    double dt;              /* time step [s] */
+    double g;               /* gravity [m/s^2] */
+    double phi0, cr;        /* porosity at pref, rock compressibility [1/Pa] */
+    double pref;            /* reference pressure [Pa] */
+    double kz_ratio;        /* k_z / k_h */
+    double bw0, cw;         /* bw = bw0 (1 + cw (p - pref)) */
+    double bo0, co;         /* bo = bo0 (1 + co (p - pref)) */
+    double bg0;             /* bg = bg0 p / pref */
+    double rs0;             /* Rs = rs0 p / pref */
+    double muw, muo, mug;   /* viscosities [Pa s] */
+    double rhow_s, rhoo_s, rhog_s; /* surface densities [kg/m^3] */
+    double swr, sor, sgr;   /* Corey residual saturations */
+    double kw, ko, kg;      /* Corey end-point relative permeabilities */
+    int nw, no, ng, pad;    /* Corey exponents */
+} or_bo_params;
+int or_bo_pattern(int nx, int ny, int nz, int *browptr, int *bcolind);
+void or_bo_assemble(const or_bo_params *P, int nx, int ny, int nz, const double *kh,
+                    const double *u, const double *u_old, double *val, double *R);
+int or_bo_expand(int nb, const int *browptr, const int *bcolind, int *rowptr,
+                 int *colind);
+
 #ifdef __cplusplus
 }
 #endif

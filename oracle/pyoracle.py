@@ -55,6 +55,14 @@ class Stats(C.Structure):
                 ("bad_row", C.c_int), ("hist_len", C.c_int)]
 
 
+class BoParams(C.Structure):
+    """or_bo_params (oracle.h); field order identical to rsgpu_blackoil_params."""
+    _fields_ = [(f, C.c_double) for f in (
+        "dx", "dy", "dz", "dt", "g", "phi0", "cr", "pref", "kz_ratio", "bw0", "cw", "bo0", "co",
+        "bg0", "rs0", "muw", "muo", "mug", "rhow_s", "rhoo_s", "rhog_s", "swr", "sor", "sgr",
+        "kw", "ko", "kg")] + [(f, C.c_int) for f in ("nw", "no", "ng", "pad")]
+
+
 def build() -> str:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
     return LIB_PATH
@@ -135,6 +143,11 @@ def lib():
         L.or_mpfa_o_vjp.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                     C.c_double, _f64p, C.c_double, _i32p, _i32p, _f64p, _f64p]
         L.or_cell_rates.restype = None
+        L.or_bo_pattern.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
+        L.or_bo_assemble.argtypes = [C.POINTER(BoParams), C.c_int, C.c_int, C.c_int, _f64p,
+                                     _f64p, _f64p, C.c_void_p, C.c_void_p]
+        L.or_bo_assemble.restype = None
+        L.or_bo_expand.argtypes = [C.c_int, _i32p, _i32p, _i32p, _i32p]
         _lib = L
     return _lib
 
@@ -515,3 +528,42 @@ def connection_flux(md_ptr, md_elem, md_t, p):
     f = np.zeros(max(nconn, 1))
     lib().or_connection_flux(nconn, _i32(md_ptr), _i32(md_elem), _f64(md_t), _f64(p), f)
     return f[:nconn]
+
+
+# ---------------------------------------------------------------- black-oil
+def _bo_params(params: dict) -> BoParams:
+    p = BoParams()
+    for k, v in params.items():
+        setattr(p, k, v)
+    return p
+
+
+def bo_pattern(nx, ny, nz):
+    """7-point block pattern (browptr, bcolind)."""
+    n = nx * ny * nz
+    nnzb = lib().or_bo_pattern(nx, ny, nz, None, None)
+    brp = np.zeros(n + 1, np.int32)
+    bci = np.zeros(nnzb, np.int32)
+    lib().or_bo_pattern(nx, ny, nz, brp.ctypes.data, bci.ctypes.data)
+    return brp, bci
+
+
+def bo_expand(brp, bci):
+    """Scalar CSR pattern of a 3x3 block pattern (block rows interleaved)."""
+    nb = len(brp) - 1
+    rp = np.zeros(3 * nb + 1, np.int32)
+    ci = np.zeros(9 * len(bci), np.int32)
+    lib().or_bo_expand(nb, _i32(brp), _i32(bci), rp, ci)
+    return rp, ci
+
+
+def bo_assemble(params: dict, nx, ny, nz, kh, u, u_old, need_jac=True):
+    """(values in the row-interleaved block layout or None, residual)."""
+    n = nx * ny * nz
+    nnzb = lib().or_bo_pattern(nx, ny, nz, None, None)
+    val = np.zeros(9 * nnzb) if need_jac else None
+    R = np.zeros(3 * n)
+    P = _bo_params(params)
+    lib().or_bo_assemble(C.byref(P), nx, ny, nz, _f64(kh), _f64(u), _f64(u_old),
+                         val.ctypes.data if need_jac else None, R.ctypes.data)
+    return val, R

@@ -46,6 +46,14 @@ class Stats(C.Structure):
                 ("b_norm", C.c_double)]
 
 
+class BlackOilParams(C.Structure):
+    """rsgpu_blackoil_params (include/rsgpu.h)."""
+    _fields_ = [(f, C.c_double) for f in (
+        "dx", "dy", "dz", "dt", "g", "phi0", "cr", "pref", "kz_ratio", "bw0", "cw", "bo0", "co",
+        "bg0", "rs0", "muw", "muo", "mug", "rhow_s", "rhoo_s", "rhog_s", "swr", "sor", "sgr",
+        "kw", "ko", "kg")] + [(f, C.c_int) for f in ("nw", "no", "ng", "pad")]
+
+
 _vp = C.c_void_p
 _ip = C.c_void_p      # int32 pointers passed as addresses
 _dp = C.c_void_p
@@ -103,6 +111,14 @@ SIGNATURES = {
     "rsgpu_solver_solve_host": (C.c_int, [_h, _h, _h, _dp, _dp, C.POINTER(Stats)]),
     "rsgpu_solver_history": (C.c_int, [_h, _dp, C.c_int]),
     "rsgpu_prec_set_blocks": (C.c_int, [_h, C.c_int, _ip]),
+    "rsgpu_csr_create_bsr3": (C.c_int, [_h, C.c_int, C.c_int, _ip, _ip, _hp]),
+    "rsgpu_csr_block_info": (C.c_int, [_h, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int)]),
+    "rsgpu_blackoil_create": (C.c_int, [_h, C.POINTER(BlackOilParams), C.c_int, C.c_int,
+                                        C.c_int, _dp, _hp]),
+    "rsgpu_blackoil_destroy": (C.c_int, [_h]),
+    "rsgpu_blackoil_matrix": (C.c_int, [_h, _hp]),
+    "rsgpu_blackoil_assemble": (C.c_int, [_h, _dp, _dp, _h, _dp]),
     "rsgpu_csr_block_diag": (C.c_int, [_h, C.c_int, C.POINTER(C.c_void_p), _hp]),
     "rsgpu_batch_create": (C.c_int, [_h, C.POINTER(SolverOpts), C.c_int, _ip, C.c_int, _hp]),
     "rsgpu_batch_destroy": (C.c_int, [_h]),

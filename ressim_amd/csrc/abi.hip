@@ -4,7 +4,7 @@
 #include <algorithm>
 #include <atomic>
 
-#include "rsgpu_impl.hpp"
+#include "spmv_core.hpp"
 
 static std::atomic<uint64_t> g_pattern_id{1};
 uint64_t next_pattern_id() { return g_pattern_id++; }
@@ -121,6 +121,84 @@ extern "C" int rsgpu_csr_create(rsgpu_ctx *ctx, int n, int ncols, int nnz,
     *out = A;
     return RSGPU_OK;
     RS_CATCH(ctx)
+}
+
+// 3x3 block matrix: the expanded CSR with each block row's three scalar rows
+// listing the row's blocks in ascending block column (three scalars per
+// block), plus the block pattern for the block-row SpMV core.
+extern "C" int rsgpu_csr_create_bsr3(rsgpu_ctx *ctx, int nb, int nnzb, const int32_t *brp,
+                                     const int32_t *bcol, rsgpu_csr **out)
+{
+    if (!ctx || !out || nb < 0 || nnzb < 0 || !brp || (nnzb > 0 && !bcol))
+        return RSGPU_BAD_ARGUMENT;
+    if ((long long)nnzb * 9 > 0x7fffffffLL || (long long)nb * 3 > 0x7fffffffLL) {
+        ctx->err = "block matrix exceeds 32-bit scalar indices";
+        return RSGPU_BAD_ARGUMENT;
+    }
+    rsgpu_csr *A = new rsgpu_csr();
+    RS_TRY(ctx)
+    RS_HIP(hipSetDevice(ctx->device));
+    if (brp[0] != 0 || brp[nb] != nnzb)
+        throw RsError{RSGPU_BAD_ARGUMENT, "block rowptr must start at 0 and end at nnzb"};
+    std::vector<int> hb(brp, brp + nb + 1), hc(bcol, bcol + nnzb);
+    for (int I = 0; I < nb; ++I) {
+        if (hb[I + 1] < hb[I])
+            throw RsError{RSGPU_BAD_ARGUMENT, "block rowptr must be non-decreasing"};
+        for (int b = hb[I]; b < hb[I + 1]; ++b)
+            if (hc[b] < 0 || hc[b] >= nb)
+                throw RsError{RSGPU_BAD_ARGUMENT, "block column out of range"};
+        // Arrays.sort of each row (CompRowMatrix.java:188), at block level
+        std::sort(hc.begin() + hb[I], hc.begin() + hb[I + 1]);
+        for (int b = hb[I] + 1; b < hb[I + 1]; ++b)
+            if (hc[b] == hc[b - 1])
+                throw RsError{RSGPU_BAD_ARGUMENT, "duplicate block column"};
+    }
+    const int n = 3 * nb, nnz = 9 * nnzb;
+    A->ctx = ctx;
+    A->n = A->ncols = n;
+    A->nnz = nnz;
+    A->bs = 3;
+    A->nb = nb;
+    A->nnzb = nnzb;
+    A->h_rowptr.resize(n + 1);
+    A->h_colind.resize(nnz);
+    int p = 0;
+    A->h_rowptr[0] = 0;
+    for (int I = 0; I < nb; ++I)
+        for (int a = 0; a < 3; ++a) {
+            for (int b = hb[I]; b < hb[I + 1]; ++b)
+                for (int c = 0; c < 3; ++c)
+                    A->h_colind[p++] = 3 * hc[b] + c;
+            A->h_rowptr[3 * I + a + 1] = p;
+        }
+    hipStream_t s = ctx->stream;
+    A->rowptr.upload(A->h_rowptr, s);
+    A->colind.upload(A->h_colind, s);
+    A->brp.upload(hb, s);
+    A->bcol.upload(hc.empty() ? std::vector<int>(1, 0) : hc, s);
+    A->val.alloc(std::max(nnz, 1));
+    A->val.zero(s);
+    spmv_build_plans(A->h_rowptr, n, A, s);
+    spmv_build_plan_b3(hb, nb, A->bplan, s, RS_SPMV_NNZ);
+    spmv_build_plan_b3(hb, nb, A->bplan32, s, RS_SPMV_NNZ32);
+    A->pattern_id = next_pattern_id();
+    RS_HIP(hipStreamSynchronize(s));
+    *out = A;
+    return RSGPU_OK;
+    RS_CATCH(ctx)
+}
+
+extern "C" int rsgpu_csr_block_info(const rsgpu_csr *A, int *bs, int *nb, int *nnzb)
+{
+    if (!A)
+        return RSGPU_BAD_ARGUMENT;
+    if (bs)
+        *bs = A->bs;
+    if (nb)
+        *nb = A->bs == 3 ? A->nb : A->n;
+    if (nnzb)
+        *nnzb = A->bs == 3 ? A->nnzb : A->nnz;
+    return RSGPU_OK;
 }
 
 extern "C" int rsgpu_csr_transpose(const rsgpu_csr *A, rsgpu_csr **out)

@@ -59,6 +59,11 @@ __device__ __forceinline__ bool stopped(const DevState *st, bool inner_step)
     return st->done || (inner_step && st->cycle_end);
 }
 
+// block pattern of a 3x3 block matrix (null for a plain CSR)
+struct BMat {
+    const int *brp, *bcol;
+};
+
 // =========================================================== fused SpMV
 // y = A x (mult rule), plus up to two partial dot products over the rows,
 // finished by Fin.
@@ -139,16 +144,29 @@ __device__ void finish_scalar(DevState *st, int fin, double t1, double t2)
     }
 }
 
-template <typename T, class Epi, int NR>
+// the block-row core for 3x3 block matrices, the CSR core otherwise
+template <typename T, bool B3, class Epi>
+__device__ __forceinline__ void spmv_any(const int *wg_rows, const int *rowptr,
+                                         const int *colind, const BMat &bm, const T *val,
+                                         const T *x, Epi &epi)
+{
+    if (B3)
+        spmv_core_b3<T, RS_SPMV_BS, spmv_nnz<T>()>(wg_rows, rowptr, colind, bm.brp, bm.bcol,
+                                                   val, x, epi);
+    else
+        spmv_core<T, RS_SPMV_BS, spmv_nnz<T>()>(wg_rows, rowptr, colind, val, x, epi);
+}
+
+template <typename T, class Epi, int NR, bool B3>
 __global__ void __launch_bounds__(RS_SPMV_BS)
     k_spmv_red(const int *__restrict__ wg_rows, const int *__restrict__ rowptr,
-               const int *__restrict__ colind, const T *__restrict__ val,
+               const int *__restrict__ colind, BMat bm, const T *__restrict__ val,
                const T *__restrict__ x, Epi epi, DevState *st, RedSlot slot,
                int fin, int inner_step)
 {
     if (stopped(st, inner_step))
         return;
-    spmv_core<T, RS_SPMV_BS, spmv_nnz<T>()>(wg_rows, rowptr, colind, val, x, epi);
+    spmv_any<T, B3>(wg_rows, rowptr, colind, bm, val, x, epi);
     double v[NR], tot[NR];
     v[0] = epi.a1;
     if (NR > 1)
@@ -158,17 +176,17 @@ __global__ void __launch_bounds__(RS_SPMV_BS)
 }
 
 // y = A x without a reduction, skipped when the solve has stopped
-template <typename T>
+template <typename T, bool B3>
 __global__ void __launch_bounds__(RS_SPMV_BS)
     k_spmv_skip(const int *__restrict__ wg_rows, const int *__restrict__ rowptr,
-                const int *__restrict__ colind, const T *__restrict__ val,
+                const int *__restrict__ colind, BMat bm, const T *__restrict__ val,
                 const T *__restrict__ x, T *__restrict__ y, const DevState *st,
                 int inner_step)
 {
     if (stopped(st, inner_step))
         return;
     EpiMult<T> epi{y};
-    spmv_core<T, RS_SPMV_BS, spmv_nnz<T>()>(wg_rows, rowptr, colind, val, x, epi);
+    spmv_any<T, B3>(wg_rows, rowptr, colind, bm, val, x, epi);
 }
 
 // 16-byte vector of T for the solver's internal vectors (leading dimension
@@ -940,36 +958,55 @@ struct Poller {
     }
 };
 
+BMat bmat_of(const rsgpu_csr *A) { return BMat{A->brp.p, A->bcol.p}; }
+
+template <typename T, class Epi, int NR>
+void launch_red(rsgpu_solver *S, rsgpu_csr *A, const T *x, Epi epi, DevState *st, int fin,
+                int inner)
+{
+    hipStream_t s = S->ctx->stream;
+    const SpmvPlan &pl = spmv_plan<T>(A);
+    const dim3 g(pl.nwg), b(RS_SPMV_BS);
+    if (A->bs == 3)
+        hipLaunchKernelGGL((k_spmv_red<T, Epi, NR, true>), g, b, 0, s, pl.wg_rows.p,
+                           A->rowptr.p, A->colind.p, bmat_of(A), csr_values<T>(A), x, epi, st,
+                           slot_of(S), fin, inner);
+    else
+        hipLaunchKernelGGL((k_spmv_red<T, Epi, NR, false>), g, b, 0, s, pl.wg_rows.p,
+                           A->rowptr.p, A->colind.p, bmat_of(A), csr_values<T>(A), x, epi, st,
+                           slot_of(S), fin, inner);
+    RS_CHECK_LAUNCH();
+}
+
 template <typename T>
 void launch_spmv_red(rsgpu_solver *S, rsgpu_csr *A, const T *x, EpiMultDot<T> epi,
                      DevState *st, int fin, int inner)
 {
-    hipStream_t s = S->ctx->stream;
-    const SpmvPlan &pl = spmv_plan<T>(A);
-    dim3 g(pl.nwg), b(RS_SPMV_BS);
     if (epi.nd <= 1)
-        hipLaunchKernelGGL((k_spmv_red<T, EpiMultDot<T>, 1>), g, b, 0, s,
-                           pl.wg_rows.p, A->rowptr.p, A->colind.p,
-                           csr_values<T>(A), x, epi, st, slot_of(S), fin, inner);
+        launch_red<T, EpiMultDot<T>, 1>(S, A, x, epi, st, fin, inner);
     else
-        hipLaunchKernelGGL((k_spmv_red<T, EpiMultDot<T>, 2>), g, b, 0, s,
-                           pl.wg_rows.p, A->rowptr.p, A->colind.p,
-                           csr_values<T>(A), x, epi, st, slot_of(S), fin, inner);
-    RS_CHECK_LAUNCH();
+        launch_red<T, EpiMultDot<T>, 2>(S, A, x, epi, st, fin, inner);
 }
 
 template <typename T>
 void launch_resid(rsgpu_solver *S, rsgpu_csr *A, const T *b, const T *x, T *r,
                   T *copy, float *r32, DevState *st, int fin)
 {
+    launch_red<T, EpiResidNorm<T>, 1>(S, A, x, EpiResidNorm<T>{r, copy, b, r32}, st, fin, 0);
+}
+
+template <typename T>
+void launch_spmv_skip(rsgpu_solver *S, rsgpu_csr *A, const T *x, T *y, DevState *st)
+{
     hipStream_t s = S->ctx->stream;
-    EpiResidNorm<T> epi{r, copy, b, r32};
     const SpmvPlan &pl = spmv_plan<T>(A);
-    hipLaunchKernelGGL((k_spmv_red<T, EpiResidNorm<T>, 1>), dim3(pl.nwg),
-                       dim3(RS_SPMV_BS), 0, s, pl.wg_rows.p, A->rowptr.p,
-                       A->colind.p, csr_values<T>(A), x, epi, st, slot_of(S),
-                       fin, 0);
-    RS_CHECK_LAUNCH();
+    const dim3 g(pl.nwg), b(RS_SPMV_BS);
+    if (A->bs == 3)
+        hipLaunchKernelGGL((k_spmv_skip<T, true>), g, b, 0, s, pl.wg_rows.p, A->rowptr.p,
+                           A->colind.p, bmat_of(A), csr_values<T>(A), x, y, st, 1);
+    else
+        hipLaunchKernelGGL((k_spmv_skip<T, false>), g, b, 0, s, pl.wg_rows.p, A->rowptr.p,
+                           A->colind.p, bmat_of(A), csr_values<T>(A), x, y, st, 1);
 }
 
 // ------------------------------------------------------------ BiCGstab
@@ -1091,9 +1128,7 @@ void run_gmres(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const T *b, T *x,
                            st, sl);
         for (int i = 0; i < m; ++i) {
             T *vi = V + (size_t)i * L;
-            hipLaunchKernelGGL(k_spmv_skip<T>, dim3(spmv_plan<T>(A).nwg), dim3(RS_SPMV_BS),
-                               0, s, spmv_plan<T>(A).wg_rows.p, A->rowptr.p, A->colind.p,
-                               csr_values<T>(A), vi, u, st, 1);
+            launch_spmv_skip<T>(S, A, vi, u, st);
             papply(S, M, u, w, st);
             if (S->o.cgs && sizeof(T) == 4) {
                 // a quarter of the vector grid: 16 elements per thread
@@ -1404,7 +1439,8 @@ extern "C" int rsgpu_solver_solve(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M,
         RS_HIP(hipSetDevice(ctx->device));
         const int m = std::max(1, S->o.restart);
         const size_t need =
-            (size_t)2 * std::max(std::max(A->plan.nwg, A->plan32.nwg), grid_for(A->n)) + 2;
+            (size_t)2 * std::max({A->plan.nwg, A->plan32.nwg, A->bplan.nwg, A->bplan32.nwg,
+                                  grid_for(A->n)}) + 2;
         const size_t need_cgs = (size_t)(m + 2) * grid_for(A->n);
         if (S->partials.n < std::max(need, need_cgs) + 4096)
             S->partials.alloc(std::max(need, need_cgs) + 4096);

@@ -109,6 +109,13 @@ struct rsgpu_csr {
     DevBuf<double> tval;
     // pattern id of the matrix this one is the transpose of (rsgpu_csr_transpose)
     uint64_t transpose_of = 0;
+    // 3x3 block rows (rsgpu_csr_create_bsr3): the CSR above is the expanded
+    // matrix with each block row's scalar rows interleaved; brp/bcol is the
+    // block pattern and bplan/bplan32 group whole block rows
+    int bs = 1;
+    int nb = 0, nnzb = 0;
+    DevBuf<int> brp, bcol;
+    SpmvPlan bplan, bplan32;
 };
 
 // Level-scheduled sweep data of one triangular factor (see ilu.hip)
@@ -389,6 +396,9 @@ void spmv_build_plan(const std::vector<int> &rowptr, int n, SpmvPlan &plan,
                      hipStream_t s, int nnz_wg = 2048);
 // both plans of a CSR matrix (FP64 and FP32 stage sizes)
 void spmv_build_plans(const std::vector<int> &rowptr, int n, rsgpu_csr *A, hipStream_t s);
+// block-row plans of a 3x3 block matrix (scalar row bounds at block rows)
+void spmv_build_plan_b3(const std::vector<int> &brp, int nb, SpmvPlan &plan, hipStream_t s,
+                        int nnz_wg);
 template <typename T>
 void spmv_launch(hipStream_t s, const rsgpu_csr *A, const T *val, const T *x,
                  T *y, int mode, double alpha, const T *b);

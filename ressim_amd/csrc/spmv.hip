@@ -33,13 +33,60 @@ void spmv_build_plans(const std::vector<int> &rowptr, int n, rsgpu_csr *A, hipSt
     spmv_build_plan(rowptr, n, A->plan32, s, RS_SPMV_NNZ32);
 }
 
-template <typename T, class Epi>
+// Block-row plan of a 3x3 block matrix: whole block rows per group (at most
+// RS_SPMV_BS / 3 of them and nnz_wg / 9 blocks); a block row with more
+// blocks than the stage becomes three single-row groups.
+void spmv_build_plan_b3(const std::vector<int> &brp, int nb, SpmvPlan &plan, hipStream_t s,
+                        int nnz_wg)
+{
+    const int maxb = nnz_wg / 9, maxrows = RS_SPMV_BS / 3;
+    std::vector<int> starts;
+    starts.reserve(nb / 16 + 2);
+    int I = 0;
+    while (I < nb) {
+        const int b0 = brp[I];
+        if (brp[I + 1] - b0 > maxb) {
+            for (int a = 0; a < 3; ++a)
+                starts.push_back(3 * I + a);
+            ++I;
+            continue;
+        }
+        starts.push_back(3 * I);
+        int e = I + 1;
+        while (e < nb && e - I < maxrows && brp[e + 1] - b0 <= maxb)
+            ++e;
+        I = e;
+    }
+    starts.push_back(3 * nb);
+    plan.nwg = (int)starts.size() - 1;
+    plan.wg_rows.upload(starts, s);
+}
+
+template <typename T, class Epi, bool B3>
 __global__ void __launch_bounds__(RS_SPMV_BS)
     k_spmv(const int *__restrict__ wg_rows, const int *__restrict__ rowptr,
-           const int *__restrict__ colind, const T *__restrict__ val,
-           const T *__restrict__ x, Epi epi)
+           const int *__restrict__ colind, const int *__restrict__ brp,
+           const int *__restrict__ bcol, const T *__restrict__ val, const T *__restrict__ x,
+           Epi epi)
 {
-    spmv_core<T, RS_SPMV_BS, spmv_nnz<T>()>(wg_rows, rowptr, colind, val, x, epi);
+    if (B3)
+        spmv_core_b3<T, RS_SPMV_BS, spmv_nnz<T>()>(wg_rows, rowptr, colind, brp, bcol, val, x,
+                                                   epi);
+    else
+        spmv_core<T, RS_SPMV_BS, spmv_nnz<T>()>(wg_rows, rowptr, colind, val, x, epi);
+}
+
+template <typename T, class Epi>
+static void launch_k_spmv(hipStream_t s, const rsgpu_csr *A, const T *val, const T *x, Epi epi)
+{
+    const SpmvPlan &pl = spmv_plan<T>(A);
+    const dim3 g(pl.nwg), bl(RS_SPMV_BS);
+    if (A->bs == 3)
+        hipLaunchKernelGGL((k_spmv<T, Epi, true>), g, bl, 0, s, pl.wg_rows.p, A->rowptr.p,
+                           A->colind.p, A->brp.p, A->bcol.p, val, x, epi);
+    else
+        hipLaunchKernelGGL((k_spmv<T, Epi, false>), g, bl, 0, s, pl.wg_rows.p, A->rowptr.p,
+                           A->colind.p, nullptr, nullptr, val, x, epi);
 }
 
 // mode: 0 mult, 1 multAdd(alpha), 2 residual b + alpha*Ax, 3 plain sum,
@@ -50,25 +97,18 @@ void spmv_launch(hipStream_t s, const rsgpu_csr *A, const T *val, const T *x,
 {
     if (A->n == 0)
         return;
-    const SpmvPlan &pl = spmv_plan<T>(A);
-    dim3 g(pl.nwg), bl(RS_SPMV_BS);
-    const int *wr = pl.wg_rows.p, *rp = A->rowptr.p, *ci = A->colind.p;
     switch (mode) {
     case 0:
-        hipLaunchKernelGGL((k_spmv<T, EpiMult<T>>), g, bl, 0, s, wr, rp, ci,
-                           val, x, EpiMult<T>{y});
+        launch_k_spmv<T>(s, A, val, x, EpiMult<T>{y});
         break;
     case 1:
-        hipLaunchKernelGGL((k_spmv<T, EpiMultAdd<T>>), g, bl, 0, s, wr, rp, ci,
-                           val, x, EpiMultAdd<T>{y, (T)alpha});
+        launch_k_spmv<T>(s, A, val, x, EpiMultAdd<T>{y, (T)alpha});
         break;
     case 2:
-        hipLaunchKernelGGL((k_spmv<T, EpiResid<T>>), g, bl, 0, s, wr, rp, ci,
-                           val, x, EpiResid<T>{y, b, (T)alpha});
+        launch_k_spmv<T>(s, A, val, x, EpiResid<T>{y, b, (T)alpha});
         break;
     default:
-        hipLaunchKernelGGL((k_spmv<T, EpiPlain<T>>), g, bl, 0, s, wr, rp, ci,
-                           val, x, EpiPlain<T>{y});
+        launch_k_spmv<T>(s, A, val, x, EpiPlain<T>{y});
         break;
     }
     RS_CHECK_LAUNCH();

@@ -30,6 +30,8 @@ constexpr int spmv_nnz()
 template <typename T>
 inline const SpmvPlan &spmv_plan(const rsgpu_csr *A)
 {
+    if (A->bs == 3)
+        return sizeof(T) == 4 ? A->bplan32 : A->bplan;
     return sizeof(T) == 4 ? A->plan32 : A->plan;
 }
 
@@ -94,6 +96,108 @@ __device__ __forceinline__ void spmv_core(const int *__restrict__ wg_rows,
         }
         if (threadIdx.x == 0)
             epi.row(r0, dot);
+    }
+}
+
+// 3x3 block rows (row-interleaved layout: block row I is scalar rows 3I..3I+2,
+// each listing its blocks in ascending block column, three scalars per
+// block, so the value array is the CSR of the expanded matrix and scalar row
+// 3I+a starts at 9 brp[I] + 3a (brp[I+1] - brp[I])).  The plan's groups are
+// whole block rows.  Phase 1 streams the group's values coalesced, one per
+// lane, exactly as the CSR core does, but without a column-index stream: an
+// LDS map gives each staged value its scalar row, the row's base gives the
+// block and the column inside it, and one block column index per block (a
+// cached gather, shared by the 9 values of the block) gives the column of x.
+// Phase 2 is the CSR phase 2: each scalar row summed left to right,
+// bit-identical to CompRowMatrix.mult on the expanded matrix.
+template <typename T, int BS, int NNZ_WG, class Epi>
+__device__ __forceinline__ void spmv_core_b3(const int *__restrict__ wg_rows,
+                                             const int *__restrict__ rowptr,
+                                             const int *__restrict__ colind,
+                                             const int *__restrict__ brp,
+                                             const int *__restrict__ bcol,
+                                             const T *__restrict__ val,
+                                             const T *__restrict__ x, Epi &epi)
+{
+    static_assert(BS <= 256 * 3, "scalar rows of a group are numbered in one byte");
+    __shared__ int s_brp[BS / 3 + 2];
+    __shared__ int s_base[BS + 1];
+    __shared__ unsigned char s_row[NNZ_WG];
+    const int g = xcd_remap((int)blockIdx.x, (int)gridDim.x);
+    const int r0 = wg_rows[g], r1 = wg_rows[g + 1];
+    if ((r0 % 3) || (r1 % 3) || r1 - r0 > BS) {
+        // one scalar row of a block row longer than the stage (the plan
+        // splits such block rows into their three scalar rows): CSR core
+        spmv_core<T, BS, NNZ_WG>(wg_rows, rowptr, colind, val, x, epi);
+        return;
+    }
+    const int I0 = r0 / 3, nI = (r1 - r0) / 3;
+    // the group's value range is known from two block row pointers; the
+    // value loads are issued before the row map is built, and the barriers
+    // below wait for LDS traffic only, so those loads stay in flight
+    const int B0 = brp[I0], nv = 9 * (brp[I0 + nI] - B0);
+    if (nv > NNZ_WG) {
+        spmv_core<T, BS, NNZ_WG>(wg_rows, rowptr, colind, val, x, epi);
+        return;
+    }
+    __shared__ T prod[NNZ_WG];
+    const int j0 = 9 * B0;   // rowptr[r0]
+    constexpr int Q = NNZ_WG / BS;
+    T v[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+        v[q] = __builtin_nontemporal_load(&val[j0 + min((int)threadIdx.x + q * BS, nv - 1)]);
+    for (int t = threadIdx.x; t <= nI; t += BS)
+        s_brp[t] = brp[I0 + t];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    // scalar row bases (relative to j0) and the row of every staged value
+    for (int rr = threadIdx.x; rr < 3 * nI; rr += BS) {
+        const int r = rr / 3, a = rr - 3 * r;
+        const int len = 3 * (s_brp[r + 1] - s_brp[r]);
+        const int base = 9 * (s_brp[r] - B0) + a * len;
+        s_base[rr] = base;
+        for (int k = 0; k < len; ++k)
+            s_row[base + k] = (unsigned char)rr;
+    }
+    if (threadIdx.x == 0)
+        s_base[3 * nI] = nv;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    if (nv > 0) {
+        int blk[Q], col[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int jl = min((int)threadIdx.x + q * BS, nv - 1);
+            const int rr = s_row[jl];
+            const int off = jl - s_base[rr];
+            const int bl = off / 3;
+            blk[q] = s_brp[rr / 3] + bl;
+            col[q] = off - 3 * bl;   // column inside the block
+        }
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+            col[q] += 3 * bcol[blk[q]];
+        T xg[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+            xg[q] = x[col[q]];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int jl = (int)threadIdx.x + q * BS;
+            if (jl < nv)
+                prod[jl] = v[q] * xg[q];
+        }
+    }
+    __syncthreads();
+    for (int r = r0 + (int)threadIdx.x; r < r1; r += BS) {
+        const int a = s_base[r - r0], e = s_base[r - r0 + 1];
+        T dot = epi.init(r);
+        for (int j = a; j < e; ++j)
+            dot += prod[j];
+        epi.row(r, dot);
     }
 }
 

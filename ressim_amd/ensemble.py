@@ -88,7 +88,7 @@ def grid_blocks(problems, max_rows: int) -> np.ndarray:
     base = 0
     for pr in problems:
         nx, ny = pr.nx, pr.ny
-        nz = pr.nz if pr.dim == 3 else 1
+        nz = pr.nz if getattr(pr, "dim", 3) == 3 else 1
         lines = max(1, min(ny, max_rows // nx))
         nb = -(-ny // lines)
         cuts = [(ny * q) // nb for q in range(1, nb + 1)]

@@ -167,6 +167,34 @@ class CompRowMatrix:
         return self
 
     @classmethod
+    def from_bsr3(cls, nb, browptr, bcolind, data=None,
+                  ctx: Optional[Context] = None) -> "CompRowMatrix":
+        """A matrix of 3x3 blocks on the block pattern (browptr, bcolind): the
+        expanded scalar matrix with each block row's three rows interleaved
+        (rsgpu_csr_create_bsr3); getData() is in that layout."""
+        self = cls.__new__(cls)
+        self.ctx = ctx or Context.default()
+        brp = np.ascontiguousarray(browptr, np.int32)
+        bci = np.ascontiguousarray(bcolind, np.int32)
+        h = C.c_void_p()
+        _raise(L.load().rsgpu_csr_create_bsr3(self.ctx.h, nb, int(brp[-1]), _hptr(brp),
+                                              _hptr(bci) if bci.size else None, C.byref(h)),
+               self.ctx.h)
+        self.h = h
+        self.n = self.m = 3 * nb
+        self.nnz = 9 * int(brp[-1])
+        if data is not None:
+            self.set_data(data)
+        return self
+
+    def block_info(self):
+        """(block size, block rows, block nonzeros)."""
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _raise(L.load().rsgpu_csr_block_info(self.h, C.byref(a), C.byref(b), C.byref(c)),
+               self.ctx.h)
+        return a.value, b.value, c.value
+
+    @classmethod
     def _wrap(cls, handle, n, m, nnz, ctx) -> "CompRowMatrix":
         self = cls.__new__(cls)
         self.ctx = ctx

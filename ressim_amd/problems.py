@@ -160,6 +160,63 @@ def config4(seed: int = 4, count: int = 64, emin: int = 32, emax: int = 96,
     return out
 
 
+# Black-oil fluid, rock and grid data of the fully-implicit Jacobian (SI
+# units); field names as rsgpu_blackoil_params.  Corey residuals and
+# exponents as CoreyRockFluid's Swi, Sor, Sgr, nw, no, ng.
+BLACKOIL_DEFAULTS = dict(
+    dx=20.0, dy=20.0, dz=4.0, dt=86400.0, g=9.81,
+    phi0=0.2, cr=1e-9, pref=200e5, kz_ratio=0.1,
+    bw0=1.0 / 1.01, cw=4.5e-10, bo0=1.0 / 1.2, co=1.5e-9, bg0=200.0, rs0=100.0,
+    muw=5e-4, muo=2e-3, mug=2e-5,
+    rhow_s=1000.0, rhoo_s=800.0, rhog_s=1.0,
+    swr=0.05, sor=0.1, sgr=0.05, kw=1.0, ko=1.0, kg=1.0, nw=2, no=2, ng=2, pad=0)
+
+
+@dataclass
+class BlackOilProblem:
+    """One Newton step of a fully-implicit black-oil time step: state u =
+    (p, Sw, Sg) per cell, previous state u_old, horizontal permeability kh
+    per cell (k_z = kz_ratio kh)."""
+    name: str
+    nx: int
+    ny: int
+    nz: int
+    kh: np.ndarray
+    u: np.ndarray
+    u_old: np.ndarray
+    params: dict
+
+    @property
+    def n(self) -> int:
+        return self.nx * self.ny * self.nz
+
+    @property
+    def nnzb(self) -> int:
+        nx, ny, nz = self.nx, self.ny, self.nz
+        return nx * ny * nz + 2 * ((nx - 1) * ny * nz + nx * (ny - 1) * nz + nx * ny * (nz - 1))
+
+
+def config3(seed: int = 3, nx: int = 100, ny: int = 100, nz: int = 50, sigma: float = 1.5,
+            **overrides) -> BlackOilProblem:
+    """100x100x50 black-oil Newton step: p = 200 bar + the water hydrostatic
+    gradient (rho_w,s g z at the cell-centre depth), Sw and Sg uniform in
+    [0.1, 0.4] (seeded), log-normal permeability (sigma 1.5, mean log
+    100 mD), Corey exponents 2; the previous state equals the current one
+    (first Newton iterate of a step), so the residual is the flux
+    imbalance."""
+    n = nx * ny * nz
+    params = dict(BLACKOIL_DEFAULTS, **overrides)
+    kh = 1e-13 * np.exp(sigma * normal(seed, 0, n))
+    z = ((np.arange(n) // (nx * ny)) + 0.5) * params["dz"]
+    u = np.empty(3 * n)
+    u[0::3] = params["pref"] + params["rhow_s"] * params["g"] * z
+    s = 0.1 + 0.3 * uniform01(seed, 1, 2 * n)
+    u[1::3] = s[:n]
+    u[2::3] = s[n:]
+    return BlackOilProblem("C3-blackoil-%dx%dx%d" % (nx, ny, nz), nx, ny, nz, kh, u, u.copy(),
+                           params)
+
+
 def config5(seed: int = 5, nx: int = 64, ny: int = 64, nz: int = 32,
             sigma: float = 1.0) -> PressureProblem:
     """3-D 64x64x32 with scalar log-permeability (sigma 1) times a fixed

@@ -1,0 +1,177 @@
+"""Device black-oil Jacobian and the 3x3 block matrix path against the oracle.
+
+Assembly: pattern, values and residual bit-exact against oracle/blackoil.c.
+Block SpMV: bit-exact against CompRowMatrix.mult on the expanded CSR (the
+oracle's or_csr_mult).  ILU(0) on the block matrix: bit-exact against MTJ's
+scalar ILU(0) on the expanded CSR (the reference-side oracle for the block
+path).  Solves: BiCGstab + block-Jacobi ILU(0), FP64 and with the FP32
+preconditioner, against the oracle's solution.
+"""
+import numpy as np
+import pytest
+
+from ressim_amd import problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def _device_jacobian(gpu, pr, u=None, u_old=None):
+    import torch
+    from ressim_amd.blackoil import BlackOilJacobian
+    bo = BlackOilJacobian(pr)
+    J = bo.matrix()
+    ud = torch.tensor(pr.u if u is None else u, device="cuda")
+    uo = torch.tensor(pr.u_old if u_old is None else u_old, device="cuda")
+    R = torch.empty(3 * pr.n, dtype=torch.float64, device="cuda")
+    bo.assemble(ud, uo, J, R)
+    return bo, J, R
+
+
+@pytest.mark.parametrize("shape", [(7, 5, 4), (1, 6, 3), (9, 1, 2)])
+def test_assembly_bitexact(gpu, oracle, shape):
+    nx, ny, nz = shape
+    pr = P.config3(nx=nx, ny=ny, nz=nz)
+    rng = np.random.default_rng(1)
+    u = pr.u.copy()
+    u[0::3] += rng.uniform(-300, 300, pr.n)
+    u_old = pr.u.copy()
+    u_old[0::3] -= 1e5
+    bo, J, R = _device_jacobian(gpu, pr, u, u_old)
+    brp, bci = oracle.bo_pattern(nx, ny, nz)
+    rp, ci = oracle.bo_expand(brp, bci)
+    assert J.block_info() == (3, pr.n, len(bci))
+    assert np.array_equal(J.getRowPointers(), rp)
+    assert np.array_equal(J.getColumnIndices(), ci)
+    val, Ro = oracle.bo_assemble(pr.params, nx, ny, nz, pr.kh, u, u_old)
+    assert np.array_equal(J.getData(), val)
+    assert np.array_equal(R.cpu().numpy(), Ro)
+
+
+def test_block_spmv_bitexact(gpu, oracle):
+    import torch
+    pr = P.config3(nx=12, ny=9, nz=5)
+    bo, J, R = _device_jacobian(gpu, pr)
+    rp, ci, val = J.getRowPointers(), J.getColumnIndices(), J.getData()
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(J.n)
+    y = torch.zeros(J.n, dtype=torch.float64, device="cuda")
+    J.mult(torch.tensor(x, device="cuda"), y)
+    assert np.array_equal(y.cpu().numpy(), oracle.mult(rp, ci, val, x))
+    # FP32 values and vectors
+    from ressim_amd import _lib
+    J.sync_f32()
+    x32 = x.astype(np.float32)
+    y32 = torch.zeros(J.n, dtype=torch.float32, device="cuda")
+    xd = torch.tensor(x32, device="cuda")
+    torch.cuda.synchronize()
+    assert _lib.load().rsgpu_csr_mult_f32(J.h, xd.data_ptr(), y32.data_ptr()) == 0
+    J.ctx.synchronize()
+    assert np.array_equal(y32.cpu().numpy(), oracle.mult_f32(rp, ci, val.astype(np.float32),
+                                                             x32))
+
+
+def test_generic_bsr3_with_long_and_empty_block_rows(gpu, oracle):
+    """A block pattern with an empty block row and one block row longer
+    than the SpMV stage (its three scalar rows are summed one by one)."""
+    import torch
+    rng = np.random.default_rng(4)
+    nb = 400
+    rows = []
+    for I in range(nb):
+        if I == 7:
+            rows.append(np.zeros(0, np.int32))
+        elif I == 11:
+            rows.append(np.arange(0, nb, 1, dtype=np.int32)[:330])
+        else:
+            k = int(rng.integers(1, 9))
+            rows.append(np.sort(rng.choice(nb, size=k, replace=False)).astype(np.int32))
+    brp = np.zeros(nb + 1, np.int32)
+    brp[1:] = np.cumsum([len(r) for r in rows])
+    bci = np.concatenate(rows)
+    A = gpu.CompRowMatrix.from_bsr3(nb, brp, bci)
+    rp, ci = oracle.bo_expand(brp, bci)
+    assert np.array_equal(A.getRowPointers(), rp) and np.array_equal(A.getColumnIndices(), ci)
+    data = rng.standard_normal(len(ci))
+    A.set_data(data)
+    x = rng.standard_normal(3 * nb)
+    y = torch.zeros(3 * nb, dtype=torch.float64, device="cuda")
+    A.mult(torch.tensor(x, device="cuda"), y)
+    assert np.array_equal(y.cpu().numpy(), oracle.mult(rp, ci, data, x))
+
+
+def test_ilu_on_block_matrix_is_scalar_ilu(gpu, oracle):
+    import torch
+    pr = P.config3(nx=6, ny=5, nz=3)
+    bo, J, R = _device_jacobian(gpu, pr)
+    M = gpu.ILU(J)
+    M.setMatrix(J)
+    rp, ci, lu = M.factors()
+    st, luo, diag, _ = oracle.ilu_factor(J.getRowPointers(), J.getColumnIndices(), J.getData())
+    assert st == 0 and np.array_equal(lu, luo)
+    rng = np.random.default_rng(6)
+    b = rng.standard_normal(J.n)
+    z = torch.zeros(J.n, dtype=torch.float64, device="cuda")
+    M.apply(torch.tensor(b, device="cuda"), z)
+    assert np.array_equal(z.cpu().numpy(), oracle.ilu_apply(rp, ci, luo, diag, b))
+
+
+def _plane_blocks(pr, lines):
+    """Block boundaries of whole x-lines of one plane (3 scalar rows per cell)."""
+    from ressim_amd.ensemble import grid_blocks
+    return 3 * grid_blocks([pr], lines * pr.nx)
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_bicgstab_block_ilu_solves_newton_system(gpu, oracle, mixed):
+    """FP64 BiCGstab + FP64 block ILU(0); and the mixed form: FP64
+    refinement on the true residual around FP32 BiCGstab with the FP32
+    matrix and FP32 block ILU(0)."""
+    import torch
+    from ressim_amd import _lib
+    pr = P.config3(nx=16, ny=12, nz=6)
+    bo, J, R = _device_jacobian(gpu, pr)
+    b = -R
+    M = gpu.BlockILU(1, precision=_lib.FP32 if mixed else _lib.FP64)
+    M.setBlocks(_plane_blocks(pr, 6))
+    if mixed:
+        J.sync_f32()
+    M.setMatrix(J)
+    if mixed:
+        s = gpu.MixedPrecisionSolver(b, inner="bicgstab", inner_rtol=1e-3, inner_max_iter=1000)
+        s.setIterationMonitor(gpu.DefaultIterationMonitor(50, 1e-10, 1e-300, 1e5))
+    else:
+        s = gpu.BiCGstab(b)
+        s.setIterationMonitor(gpu.DefaultIterationMonitor(2000, 1e-10, 1e-300, 1e5))
+    s.setPreconditioner(M)
+    x = torch.zeros(J.n, dtype=torch.float64, device="cuda")
+    s.solve(J, b, x)
+    st = s.last_stats
+    assert st.true_residual <= 1e-9 * st.b_norm
+    xo, so, _ = oracle.solve("bicgstab", J.getRowPointers(), J.getColumnIndices(), J.getData(),
+                             b.cpu().numpy(), rtol=1e-12, atol=1e-300, prec="ilu")
+    assert so.status == 0
+    err = np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo)
+    assert err < 1e-7, err
+
+
+def test_gmres_on_block_matrix(gpu, oracle):
+    """GMRES(20) + ILU(0) on the block matrix (the block-row SpMV inside the
+    inner steps) against the oracle's GMRES on the expanded CSR."""
+    import torch
+    pr = P.config3(nx=10, ny=8, nz=4)
+    bo, J, R = _device_jacobian(gpu, pr)
+    b = -R
+    M = gpu.ILU(J)
+    M.setMatrix(J)
+    s = gpu.GMRES(b, 20)
+    s.setPreconditioner(M)
+    s.setIterationMonitor(gpu.DefaultIterationMonitor(3000, 1e-11, 1e-300, 1e5))
+    x = torch.zeros(J.n, dtype=torch.float64, device="cuda")
+    s.solve(J, b, x)
+    st = s.last_stats
+    assert st.true_residual <= 1e-10 * st.b_norm
+    xo, so, _ = oracle.solve("gmres", J.getRowPointers(), J.getColumnIndices(), J.getData(),
+                             b.cpu().numpy(), rtol=1e-11, atol=1e-300, restart=20, prec="ilu")
+    assert so.status == 0
+    assert abs(st.iterations - so.iterations) <= max(3, so.iterations // 10)
+    assert np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo) < 1e-8
