@@ -103,13 +103,22 @@ __device__ __forceinline__ void spmv_core(const int *__restrict__ wg_rows,
 // each listing its blocks in ascending block column, three scalars per
 // block, so the value array is the CSR of the expanded matrix and scalar row
 // 3I+a starts at 9 brp[I] + 3a (brp[I+1] - brp[I])).  The plan's groups are
-// whole block rows.  Phase 1 streams the group's values coalesced, one per
-// lane, exactly as the CSR core does, but without a column-index stream: an
-// LDS map gives each staged value its scalar row, the row's base gives the
-// block and the column inside it, and one block column index per block (a
-// cached gather, shared by the 9 values of the block) gives the column of x.
-// Phase 2 is the CSR phase 2: each scalar row summed left to right,
-// bit-identical to CompRowMatrix.mult on the expanded matrix.
+// whole block rows.  Phase 1a streams the group's values into LDS
+// coalesced, one per lane, as the CSR core does but with no column-index
+// stream.  Phase 1b gives one block per lane: one coalesced block column
+// index and one 3-element gather of x per block (instead of nine column
+// indices and nine gathers), and the block's nine staged values are
+// multiplied in place.  Phase 2 is the CSR phase 2: each scalar row summed
+// left to right, bit-identical to CompRowMatrix.mult on the expanded matrix.
+// The block-column and x loads are issued before the LDS barriers, which
+// wait for LDS traffic only, so they overlap the staging.
+__device__ __forceinline__ void lds_only_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <typename T, int BS, int NNZ_WG, class Epi>
 __device__ __forceinline__ void spmv_core_b3(const int *__restrict__ wg_rows,
                                              const int *__restrict__ rowptr,
@@ -119,10 +128,12 @@ __device__ __forceinline__ void spmv_core_b3(const int *__restrict__ wg_rows,
                                              const T *__restrict__ val,
                                              const T *__restrict__ x, Epi &epi)
 {
-    static_assert(BS <= 256 * 3, "scalar rows of a group are numbered in one byte");
+    constexpr int MAXB = NNZ_WG / 9;          // blocks per stage
+    constexpr int QB = (MAXB + BS - 1) / BS;  // blocks per lane
+    constexpr int Q = NNZ_WG / BS;            // staged values per lane
+    static_assert(BS / 3 <= 255, "local block rows are numbered in one byte");
     __shared__ int s_brp[BS / 3 + 2];
-    __shared__ int s_base[BS + 1];
-    __shared__ unsigned char s_row[NNZ_WG];
+    __shared__ unsigned char s_brow[MAXB + 1];
     const int g = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     const int r0 = wg_rows[g], r1 = wg_rows[g + 1];
     if ((r0 % 3) || (r1 % 3) || r1 - r0 > BS) {
@@ -132,70 +143,68 @@ __device__ __forceinline__ void spmv_core_b3(const int *__restrict__ wg_rows,
         return;
     }
     const int I0 = r0 / 3, nI = (r1 - r0) / 3;
-    // the group's value range is known from two block row pointers; the
-    // value loads are issued before the row map is built, and the barriers
-    // below wait for LDS traffic only, so those loads stay in flight
-    const int B0 = brp[I0], nv = 9 * (brp[I0 + nI] - B0);
-    if (nv > NNZ_WG) {
+    const int B0 = brp[I0], nbk = brp[I0 + nI] - B0, nv = 9 * nbk;
+    if (nbk > MAXB) {
         spmv_core<T, BS, NNZ_WG>(wg_rows, rowptr, colind, val, x, epi);
         return;
     }
     __shared__ T prod[NNZ_WG];
     const int j0 = 9 * B0;   // rowptr[r0]
-    constexpr int Q = NNZ_WG / BS;
-    T v[Q];
+    if (nbk > 0) {
+        T v[Q];
 #pragma unroll
-    for (int q = 0; q < Q; ++q)
-        v[q] = __builtin_nontemporal_load(&val[j0 + min((int)threadIdx.x + q * BS, nv - 1)]);
-    for (int t = threadIdx.x; t <= nI; t += BS)
-        s_brp[t] = brp[I0 + t];
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-    // scalar row bases (relative to j0) and the row of every staged value
-    for (int rr = threadIdx.x; rr < 3 * nI; rr += BS) {
-        const int r = rr / 3, a = rr - 3 * r;
-        const int len = 3 * (s_brp[r + 1] - s_brp[r]);
-        const int base = 9 * (s_brp[r] - B0) + a * len;
-        s_base[rr] = base;
-        for (int k = 0; k < len; ++k)
-            s_row[base + k] = (unsigned char)rr;
-    }
-    if (threadIdx.x == 0)
-        s_base[3 * nI] = nv;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-    if (nv > 0) {
-        int blk[Q], col[Q];
+        for (int q = 0; q < Q; ++q)
+            v[q] = __builtin_nontemporal_load(&val[j0 + min((int)threadIdx.x + q * BS, nv - 1)]);
+        T xg[QB][3];
 #pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            const int jl = min((int)threadIdx.x + q * BS, nv - 1);
-            const int rr = s_row[jl];
-            const int off = jl - s_base[rr];
-            const int bl = off / 3;
-            blk[q] = s_brp[rr / 3] + bl;
-            col[q] = off - 3 * bl;   // column inside the block
+        for (int q = 0; q < QB; ++q) {
+            const int t = min((int)threadIdx.x + q * BS, nbk - 1);
+            const int c = 3 * __builtin_nontemporal_load(&bcol[B0 + t]);
+#pragma unroll
+            for (int e = 0; e < 3; ++e)
+                xg[q][e] = x[c + e];
         }
-#pragma unroll
-        for (int q = 0; q < Q; ++q)
-            col[q] += 3 * bcol[blk[q]];
-        T xg[Q];
-#pragma unroll
-        for (int q = 0; q < Q; ++q)
-            xg[q] = x[col[q]];
+        for (int t = threadIdx.x; t <= nI; t += BS)
+            s_brp[t] = brp[I0 + t];
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const int jl = (int)threadIdx.x + q * BS;
             if (jl < nv)
-                prod[jl] = v[q] * xg[q];
+                prod[jl] = v[q];
         }
+        lds_only_barrier();
+        // local block row of every staged block
+        for (int r = threadIdx.x; r < nI; r += BS)
+            for (int b = s_brp[r]; b < s_brp[r + 1]; ++b)
+                s_brow[b - B0] = (unsigned char)r;
+        lds_only_barrier();
+#pragma unroll
+        for (int q = 0; q < QB; ++q) {
+            const int t = (int)threadIdx.x + q * BS;
+            if (t < nbk) {
+                const int r = s_brow[t];
+                const int b0r = s_brp[r], nbr = s_brp[r + 1] - b0r;
+                const int pos0 = 9 * (b0r - B0) + 3 * (B0 + t - b0r);
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int e = 0; e < 3; ++e) {
+                        const int p = pos0 + a * 3 * nbr + e;
+                        prod[p] = prod[p] * xg[q][e];
+                    }
+            }
+        }
+    } else {
+        for (int t = threadIdx.x; t <= nI; t += BS)
+            s_brp[t] = brp[I0 + t];
     }
     __syncthreads();
     for (int r = r0 + (int)threadIdx.x; r < r1; r += BS) {
-        const int a = s_base[r - r0], e = s_base[r - r0 + 1];
+        const int rr = r - r0, rl = rr / 3, a = rr - 3 * rl;
+        const int b0r = s_brp[rl], len = 3 * (s_brp[rl + 1] - b0r);
+        const int s0 = 9 * (b0r - B0) + a * len;
         T dot = epi.init(r);
-        for (int j = a; j < e; ++j)
+        for (int j = s0; j < s0 + len; ++j)
             dot += prod[j];
         epi.row(r, dot);
     }
