@@ -93,7 +93,7 @@ def main():
     ap.add_argument("--blackoil", action="store_true",
                     help="measure the fully-implicit black-oil Newton step (100x100x50, 3x3 "
                          "blocks) instead of the pressure solve and print its own JSON line")
-    ap.add_argument("--bo-lines", type=int, default=32,
+    ap.add_argument("--bo-lines", type=int, default=16,
                     help="x-lines per block-Jacobi ILU(0) block of the black-oil Jacobian")
     ap.add_argument("--bo-nz", type=int, default=50)
     ap.add_argument("--restart", type=int, default=30)
@@ -547,6 +547,28 @@ def bench_blackoil(a):
     n, nnzb = J.n, pr.nnzb
     log(f"black-oil {pr.name}: {pr.n} cells, n={n}, block nnz={nnzb}, set up in "
         f"{time.time() - t0:.1f}s")
+    if a.probe == "spmv":
+        # kernel-only run for rocprofv3: block SpMV FP64 and FP32, then the
+        # 512 MiB copy that calibrates FETCH_SIZE / WRITE_SIZE
+        xin = torch.randn(n, dtype=torch.float64, device=dev)
+        y = torch.empty(n, dtype=torch.float64, device=dev)
+        J.sync_f32()
+        x32 = xin.float()
+        y32 = torch.empty(n, dtype=torch.float32, device=dev)
+        torch.cuda.synchronize()
+        for _ in range(a.spmv_reps):
+            lib.rsgpu_csr_mult(J.h, xin.data_ptr(), y.data_ptr())
+        for _ in range(a.spmv_reps):
+            lib.rsgpu_csr_mult_f32(J.h, x32.data_ptr(), y32.data_ptr())
+        torch.cuda.synchronize()
+        src = torch.ones(64 << 20, dtype=torch.float64, device=dev)     # 512 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        ctx.set_stream(None)
+        log(f"probe spmv: block rows {pr.n}, block nnz {nnzb}, reps {a.spmv_reps}; copy 512 MiB x3")
+        return
     # assembly kernel time
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
