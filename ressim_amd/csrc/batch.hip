@@ -125,13 +125,18 @@ struct BPlanDev {
     const int *wg_rows;
     const int *wg_sys;
     const int *sys_g0;
+    const short *coff;   // 16-bit column offsets from each group's first row, or null
 };
 
 struct BPlan {
     int nnz_stage = 0;
     int nwg = 0;
     DevBuf<int> wg_rows, wg_sys, sys_g0;
-    BPlanDev dev() const { return BPlanDev{wg_rows.p, wg_sys.p, sys_g0.p}; }
+    DevBuf<short> coff;
+    BPlanDev dev() const
+    {
+        return BPlanDev{wg_rows.p, wg_sys.p, sys_g0.p, coff.n ? coff.p : nullptr};
+    }
 };
 
 struct RedSeg {
@@ -374,7 +379,7 @@ __global__ void __launch_bounds__(RS_SPMV_BS)
     BatchSys *S = st + sys;
     if (!b_gate<FIN>(S))
         return;
-    spmv_core<T, RS_SPMV_BS, NNZ>(pl.wg_rows, rowptr, colind, val, x, epi);
+    spmv_core_off<T, RS_SPMV_BS, NNZ>(pl.wg_rows, rowptr, colind, pl.coff, val, x, epi);
     if (NR == 0)
         return;
     constexpr int NRR = (NR > 0) ? NR : 1;
@@ -590,8 +595,8 @@ struct rsgpu_batch {
 
 namespace {
 
-void build_bplan(const std::vector<int> &rp, const std::vector<int> &sys, int nnz_stage,
-                 BPlan &pl, hipStream_t s)
+void build_bplan(const std::vector<int> &rp, const std::vector<int> &ci,
+                 const std::vector<int> &sys, int nnz_stage, BPlan &pl, hipStream_t s)
 {
     const int nsys = (int)sys.size() - 1;
     std::vector<int> starts, wsys, g0(nsys + 1, 0);
@@ -618,6 +623,8 @@ void build_bplan(const std::vector<int> &rp, const std::vector<int> &sys, int nn
     pl.wg_rows.upload(starts, s);
     pl.wg_sys.upload(wsys, s);
     pl.sys_g0.upload(g0, s);
+    spmv_build_coff(rp, ci, starts, pl.coff, s);
+    RS_HIP(hipStreamSynchronize(s));
 }
 
 RedSeg red_of(rsgpu_batch *B)
@@ -683,7 +690,7 @@ float autotune(rsgpu_batch *B, rsgpu_csr *A, BPlan &best)
         BPlan pl[3];
         size_t need = 1;
         for (int c = 0; c < 3; ++c) {
-            build_bplan(A->h_rowptr, B->h_sys, cand[c], pl[c], s);
+            build_bplan(A->h_rowptr, A->h_colind, B->h_sys, cand[c], pl[c], s);
             need = std::max(need, (size_t)pl[c].nwg);
         }
         if (B->partials.n < 2 * need)
@@ -713,7 +720,7 @@ float autotune(rsgpu_batch *B, rsgpu_csr *A, BPlan &best)
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
     }
-    build_bplan(A->h_rowptr, B->h_sys, best_stage, best, s);
+    build_bplan(A->h_rowptr, A->h_colind, B->h_sys, best_stage, best, s);
     RS_HIP(hipStreamSynchronize(s));
     return best_us;
 }

@@ -59,9 +59,11 @@ __device__ __forceinline__ bool stopped(const DevState *st, bool inner_step)
     return st->done || (inner_step && st->cycle_end);
 }
 
-// block pattern of a 3x3 block matrix (null for a plain CSR)
+// block pattern of a 3x3 block matrix (null for a plain CSR) and the
+// plan's 16-bit column offsets (null when absent)
 struct BMat {
     const int *brp, *bcol;
+    const short *coff;
 };
 
 // =========================================================== fused SpMV
@@ -154,7 +156,8 @@ __device__ __forceinline__ void spmv_any(const int *wg_rows, const int *rowptr,
         spmv_core_b3<T, RS_SPMV_BS, spmv_nnz<T>()>(wg_rows, rowptr, colind, bm.brp, bm.bcol,
                                                    val, x, epi);
     else
-        spmv_core<T, RS_SPMV_BS, spmv_nnz<T>()>(wg_rows, rowptr, colind, val, x, epi);
+        spmv_core_off<T, RS_SPMV_BS, spmv_nnz<T>()>(wg_rows, rowptr, colind, bm.coff, val, x,
+                                                    epi);
 }
 
 template <typename T, class Epi, int NR, bool B3>
@@ -958,7 +961,11 @@ struct Poller {
     }
 };
 
-BMat bmat_of(const rsgpu_csr *A) { return BMat{A->brp.p, A->bcol.p}; }
+template <typename T>
+BMat bmat_of(const rsgpu_csr *A)
+{
+    return BMat{A->brp.p, A->bcol.p, spmv_plan<T>(A).coff_ptr()};
+}
 
 template <typename T, class Epi, int NR>
 void launch_red(rsgpu_solver *S, rsgpu_csr *A, const T *x, Epi epi, DevState *st, int fin,
@@ -969,11 +976,11 @@ void launch_red(rsgpu_solver *S, rsgpu_csr *A, const T *x, Epi epi, DevState *st
     const dim3 g(pl.nwg), b(RS_SPMV_BS);
     if (A->bs == 3)
         hipLaunchKernelGGL((k_spmv_red<T, Epi, NR, true>), g, b, 0, s, pl.wg_rows.p,
-                           A->rowptr.p, A->colind.p, bmat_of(A), csr_values<T>(A), x, epi, st,
+                           A->rowptr.p, A->colind.p, bmat_of<T>(A), csr_values<T>(A), x, epi, st,
                            slot_of(S), fin, inner);
     else
         hipLaunchKernelGGL((k_spmv_red<T, Epi, NR, false>), g, b, 0, s, pl.wg_rows.p,
-                           A->rowptr.p, A->colind.p, bmat_of(A), csr_values<T>(A), x, epi, st,
+                           A->rowptr.p, A->colind.p, bmat_of<T>(A), csr_values<T>(A), x, epi, st,
                            slot_of(S), fin, inner);
     RS_CHECK_LAUNCH();
 }
@@ -1003,10 +1010,10 @@ void launch_spmv_skip(rsgpu_solver *S, rsgpu_csr *A, const T *x, T *y, DevState 
     const dim3 g(pl.nwg), b(RS_SPMV_BS);
     if (A->bs == 3)
         hipLaunchKernelGGL((k_spmv_skip<T, true>), g, b, 0, s, pl.wg_rows.p, A->rowptr.p,
-                           A->colind.p, bmat_of(A), csr_values<T>(A), x, y, st, 1);
+                           A->colind.p, bmat_of<T>(A), csr_values<T>(A), x, y, st, 1);
     else
         hipLaunchKernelGGL((k_spmv_skip<T, false>), g, b, 0, s, pl.wg_rows.p, A->rowptr.p,
-                           A->colind.p, bmat_of(A), csr_values<T>(A), x, y, st, 1);
+                           A->colind.p, bmat_of<T>(A), csr_values<T>(A), x, y, st, 1);
 }
 
 // ------------------------------------------------------------ BiCGstab

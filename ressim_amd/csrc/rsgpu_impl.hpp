@@ -90,6 +90,11 @@ struct SpmvPlan {
     // contiguous row ranges per workgroup; long rows get their own group
     DevBuf<int> wg_rows;
     int nwg = 0;
+    // column - (first row of the entry's workgroup) as 16 bits, when every
+    // entry fits (stencil matrices): the SpMV then streams 2 index bytes per
+    // nonzero instead of 4; empty otherwise
+    DevBuf<short> coff;
+    const short *coff_ptr() const { return coff.n ? coff.p : nullptr; }
 };
 
 struct rsgpu_csr {
@@ -393,7 +398,11 @@ __device__ __forceinline__ bool dev_monitor(DevState *st, double r)
 // ---------------------------------------------------------------- host API
 // spmv.hip
 void spmv_build_plan(const std::vector<int> &rowptr, int n, SpmvPlan &plan,
-                     hipStream_t s, int nnz_wg = 2048);
+                     hipStream_t s, int nnz_wg = 2048,
+                     const std::vector<int> *colind = nullptr);
+// 16-bit column offsets of a plan (left empty when an entry does not fit)
+void spmv_build_coff(const std::vector<int> &rowptr, const std::vector<int> &colind,
+                     const std::vector<int> &starts, DevBuf<short> &coff, hipStream_t s);
 // both plans of a CSR matrix (FP64 and FP32 stage sizes)
 void spmv_build_plans(const std::vector<int> &rowptr, int n, rsgpu_csr *A, hipStream_t s);
 // block-row plans of a 3x3 block matrix (scalar row bounds at block rows)

@@ -6,8 +6,33 @@
 
 // Greedy contiguous row partition: a group takes rows until the next one
 // would overflow the LDS stage or the lane count; a longer row is alone.
+void spmv_build_coff(const std::vector<int> &rowptr, const std::vector<int> &colind,
+                     const std::vector<int> &starts, DevBuf<short> &coff, hipStream_t s)
+{
+    const int ng = (int)starts.size() - 1;
+    std::vector<short> off(colind.size());
+    for (int g = 0; g < ng; ++g) {
+        const int r0 = starts[g];
+        for (int j = rowptr[r0]; j < rowptr[starts[g + 1]]; ++j) {
+            const int d = colind[j] - r0;
+            if (d < -32768 || d > 32767) {
+                coff.release();
+                return;
+            }
+            off[j] = (short)d;
+        }
+    }
+    if (off.empty()) {
+        coff.release();
+        return;
+    }
+    coff.upload(off, s);
+    // the host vector is released on return: wait for the copy
+    RS_HIP(hipStreamSynchronize(s));
+}
+
 void spmv_build_plan(const std::vector<int> &rowptr, int n, SpmvPlan &plan,
-                     hipStream_t s, int nnz_wg)
+                     hipStream_t s, int nnz_wg, const std::vector<int> *colind)
 {
     std::vector<int> starts;
     starts.reserve(n / 16 + 2);
@@ -25,12 +50,17 @@ void spmv_build_plan(const std::vector<int> &rowptr, int n, SpmvPlan &plan,
     starts.push_back(n);
     plan.nwg = (int)starts.size() - 1;
     plan.wg_rows.upload(starts, s);
+    if (colind)
+        spmv_build_coff(rowptr, *colind, starts, plan.coff, s);
+    else
+        plan.coff.release();
+    RS_HIP(hipStreamSynchronize(s));
 }
 
 void spmv_build_plans(const std::vector<int> &rowptr, int n, rsgpu_csr *A, hipStream_t s)
 {
-    spmv_build_plan(rowptr, n, A->plan, s, RS_SPMV_NNZ);
-    spmv_build_plan(rowptr, n, A->plan32, s, RS_SPMV_NNZ32);
+    spmv_build_plan(rowptr, n, A->plan, s, RS_SPMV_NNZ, &A->h_colind);
+    spmv_build_plan(rowptr, n, A->plan32, s, RS_SPMV_NNZ32, &A->h_colind);
 }
 
 // Block-row plan of a 3x3 block matrix: whole block rows per group (at most
@@ -65,15 +95,15 @@ void spmv_build_plan_b3(const std::vector<int> &brp, int nb, SpmvPlan &plan, hip
 template <typename T, class Epi, bool B3>
 __global__ void __launch_bounds__(RS_SPMV_BS)
     k_spmv(const int *__restrict__ wg_rows, const int *__restrict__ rowptr,
-           const int *__restrict__ colind, const int *__restrict__ brp,
-           const int *__restrict__ bcol, const T *__restrict__ val, const T *__restrict__ x,
-           Epi epi)
+           const int *__restrict__ colind, const short *__restrict__ coff,
+           const int *__restrict__ brp, const int *__restrict__ bcol, const T *__restrict__ val,
+           const T *__restrict__ x, Epi epi)
 {
     if (B3)
         spmv_core_b3<T, RS_SPMV_BS, spmv_nnz<T>()>(wg_rows, rowptr, colind, brp, bcol, val, x,
                                                    epi);
     else
-        spmv_core<T, RS_SPMV_BS, spmv_nnz<T>()>(wg_rows, rowptr, colind, val, x, epi);
+        spmv_core_off<T, RS_SPMV_BS, spmv_nnz<T>()>(wg_rows, rowptr, colind, coff, val, x, epi);
 }
 
 template <typename T, class Epi>
@@ -83,10 +113,10 @@ static void launch_k_spmv(hipStream_t s, const rsgpu_csr *A, const T *val, const
     const dim3 g(pl.nwg), bl(RS_SPMV_BS);
     if (A->bs == 3)
         hipLaunchKernelGGL((k_spmv<T, Epi, true>), g, bl, 0, s, pl.wg_rows.p, A->rowptr.p,
-                           A->colind.p, A->brp.p, A->bcol.p, val, x, epi);
+                           A->colind.p, nullptr, A->brp.p, A->bcol.p, val, x, epi);
     else
         hipLaunchKernelGGL((k_spmv<T, Epi, false>), g, bl, 0, s, pl.wg_rows.p, A->rowptr.p,
-                           A->colind.p, nullptr, nullptr, val, x, epi);
+                           A->colind.p, pl.coff_ptr(), nullptr, nullptr, val, x, epi);
 }
 
 // mode: 0 mult, 1 multAdd(alpha), 2 residual b + alpha*Ax, 3 plain sum,
@@ -167,7 +197,7 @@ void csr_build_transpose(rsgpu_csr *A)
     A->tcolind.upload(tci, s);
     A->tperm.upload(tpm, s);
     A->tval.alloc(std::max(nnz, 1));
-    spmv_build_plan(trp, m, A->tplan, s);
+    spmv_build_plan(trp, m, A->tplan, s, RS_SPMV_NNZ);
     RS_HIP(hipStreamSynchronize(s));
     A->has_t = true;
 }

@@ -35,12 +35,15 @@ inline const SpmvPlan &spmv_plan(const rsgpu_csr *A)
     return sizeof(T) == 4 ? A->plan32 : A->plan;
 }
 
+// coff (may be null): per nonzero, its column minus the first row of its
+// workgroup, in 16 bits (SpmvPlan::coff); used instead of colind when given.
 template <typename T, int BS, int NNZ_WG, class Epi>
-__device__ __forceinline__ void spmv_core(const int *__restrict__ wg_rows,
-                                          const int *__restrict__ rowptr,
-                                          const int *__restrict__ colind,
-                                          const T *__restrict__ val,
-                                          const T *__restrict__ x, Epi &epi)
+__device__ __forceinline__ void spmv_core_off(const int *__restrict__ wg_rows,
+                                              const int *__restrict__ rowptr,
+                                              const int *__restrict__ colind,
+                                              const short *__restrict__ coff,
+                                              const T *__restrict__ val,
+                                              const T *__restrict__ x, Epi &epi)
 {
     __shared__ T prod[NNZ_WG];
     const int g = xcd_remap((int)blockIdx.x, (int)gridDim.x);
@@ -56,11 +59,20 @@ __device__ __forceinline__ void spmv_core(const int *__restrict__ wg_rows,
         if (j1 > j0) {
             T v[Q];
             int c[Q];
+            if (coff) {
 #pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                const int j = min(j0 + (int)threadIdx.x + q * BS, j1 - 1);
-                v[q] = __builtin_nontemporal_load(&val[j]);
-                c[q] = __builtin_nontemporal_load(&colind[j]);
+                for (int q = 0; q < Q; ++q) {
+                    const int j = min(j0 + (int)threadIdx.x + q * BS, j1 - 1);
+                    v[q] = __builtin_nontemporal_load(&val[j]);
+                    c[q] = r0 + (int)__builtin_nontemporal_load(&coff[j]);
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < Q; ++q) {
+                    const int j = min(j0 + (int)threadIdx.x + q * BS, j1 - 1);
+                    v[q] = __builtin_nontemporal_load(&val[j]);
+                    c[q] = __builtin_nontemporal_load(&colind[j]);
+                }
             }
             T xg[Q];
 #pragma unroll
@@ -97,6 +109,17 @@ __device__ __forceinline__ void spmv_core(const int *__restrict__ wg_rows,
         if (threadIdx.x == 0)
             epi.row(r0, dot);
     }
+}
+
+template <typename T, int BS, int NNZ_WG, class Epi>
+__device__ __forceinline__ void spmv_core(const int *__restrict__ wg_rows,
+                                          const int *__restrict__ rowptr,
+                                          const int *__restrict__ colind,
+                                          const T *__restrict__ val,
+                                          const T *__restrict__ x, Epi &epi)
+{
+    spmv_core_off<T, BS, NNZ_WG>(wg_rows, rowptr, colind, (const short *)nullptr, val, x,
+                                 epi);
 }
 
 // 3x3 block rows (row-interleaved layout: block row I is scalar rows 3I..3I+2,

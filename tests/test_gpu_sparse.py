@@ -57,6 +57,41 @@ def test_spmv_long_empty_rows_and_device_vectors(gpu, oracle):
     np.testing.assert_array_equal(ytd.cpu().numpy(), oracle.transmultadd(rp, ci, d, 1.3, xt, y1))
 
 
+@pytest.mark.parametrize("far", [False, True])
+def test_spmv_column_offsets_fit_or_not(gpu, oracle, far):
+    """70,000 rows: a banded pattern (every column within 16 bits of its
+    workgroup's first row: the SpMV reads 16-bit column offsets) and the same
+    band plus a column 50,000 away (it does not fit: 32-bit column indices)."""
+    import torch
+    rng = np.random.default_rng(11)
+    n = 70000
+    rows = []
+    for i in range(n):
+        cols = {max(0, i - 300), i, min(n - 1, i + 7)}
+        if far:
+            cols.add((i + 50000) % n)
+        rows.append(sorted(cols))
+    rp = np.zeros(n + 1, np.int32)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    ci = np.concatenate([np.asarray(r, np.int32) for r in rows])
+    d = rng.standard_normal(len(ci))
+    A = _mat(gpu, rp, ci, d, n, n)
+    x = rng.standard_normal(n)
+    y = torch.zeros(n, dtype=torch.float64, device="cuda")
+    A.mult(torch.tensor(x, device="cuda"), y)
+    np.testing.assert_array_equal(y.cpu().numpy(), oracle.mult(rp, ci, d, x))
+    A.sync_f32()
+    from ressim_amd import _lib
+    x32 = torch.tensor(x.astype(np.float32), device="cuda")
+    y32 = torch.zeros(n, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    assert _lib.load().rsgpu_csr_mult_f32(A.h, x32.data_ptr(), y32.data_ptr()) == 0
+    A.ctx.synchronize()
+    np.testing.assert_array_equal(y32.cpu().numpy(),
+                                  oracle.mult_f32(rp, ci, d.astype(np.float32),
+                                                  x.astype(np.float32)))
+
+
 def test_spmv_f32_bitexact(gpu, oracle, systems):
     import torch
     s = systems["s3"]
