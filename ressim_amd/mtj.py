@@ -187,6 +187,20 @@ class CompRowMatrix:
             self.set_data(data)
         return self
 
+    @classmethod
+    def from_matrix_market(cls, src, ctx: Optional[Context] = None) -> "CompRowMatrix":
+        """CompRowMatrix(MatrixVectorReader) (CompRowMatrix.java:68-148)."""
+        from .mmio import read_matrix
+        n, m, rp, ci, d = read_matrix(src)
+        return cls.from_csr(n, m, rp, ci, d, ctx=ctx)
+
+    def to_matrix_market(self, digits: int = 12) -> str:
+        """The matrix in Matrix Market coordinate form (AbstractMatrix.toString
+        with MatrixVectorWriter's header)."""
+        from .mmio import write_matrix
+        return write_matrix(self.getRowPointers(), self.getColumnIndices(), self.getData(),
+                            self.n, self.m, digits=digits)
+
     def block_info(self):
         """(block size, block rows, block nonzeros)."""
         a, b, c = C.c_int(), C.c_int(), C.c_int()

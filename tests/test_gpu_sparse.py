@@ -282,3 +282,21 @@ def test_ilu_errors_gpu(gpu):
     M2 = gpu.ILU(B)
     with pytest.raises(RuntimeError, match="Missing diagonal entry on row 2"):
         M2.setMatrix(B)
+
+
+def test_matrix_market_round_trip_through_device(gpu, oracle, systems):
+    """A CompRowMatrix written in Matrix Market form (17 digits) and read back
+    (CompRowMatrix(MatrixVectorReader)) holds the same pattern and values, and
+    its device SpMV equals the oracle's."""
+    import io
+    s = systems["c1"]
+    rp, ci, d = s["rowptr"], s["colind"], s["data"]
+    n = len(rp) - 1
+    A = _mat(gpu, rp, ci, d, n, n)
+    B = gpu.CompRowMatrix.from_matrix_market(io.StringIO(A.to_matrix_market(digits=17)))
+    nz = d != 0
+    assert B.nnz == int(nz.sum())
+    x = np.random.default_rng(9).standard_normal(n)
+    y = np.empty(n)
+    B.mult(x, y)
+    np.testing.assert_array_equal(y, oracle.mult(rp, ci, d, x))
