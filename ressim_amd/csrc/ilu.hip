@@ -47,7 +47,12 @@ struct HostSweep {
     std::vector<LevelMeta> meta;
     std::vector<int> rows, row_glob, col, src;
     int max_count = 0, max_width = 0;
+    int R = 1;                         // rows per level entry (3: cells)
+    std::vector<int> intra_src;        // R = 3: 3 per cell
 };
+
+static int lds_limit(int device);
+static int ilu_kernel_choice();
 
 static int pad_width(int w)
 {
@@ -199,6 +204,149 @@ static void upload_sweep(const HostSweep &H, SweepData &D, hipStream_t s, int es
         D.max_levels = std::max(D.max_levels, H.blk_lev[b + 1] - H.blk_lev[b]);
     D.max_count = H.max_count;
     D.max_width = H.max_width;
+    D.R = H.R;
+    if (H.R == 3)
+        D.intra_src.upload(H.intra_src, s);
+    else
+        D.intra_src.release();
+}
+
+// ---------------------------------------------------------------- cell sweeps
+// For 3x3 block matrices (rows 3I..3I+2 of cell I, full diagonal blocks)
+// the rows of a cell depend on each other, so a row schedule needs three
+// levels per cell.  A cell schedule puts whole cells on a level; the lane of
+// a cell computes its three rows in order with the reference's per-row
+// operation sequence: in the L sweep row 3I+a sums its entries in cells
+// before I (ascending columns), then its strictly lower diagonal-block
+// entries (3I, 3I+1); in the U sweep row 3I+a first sums its strictly upper
+// diagonal-block entries, then its entries in later cells.  External
+// entries live in W padded slots per row (identity padding as in the row
+// ELL); the three intra-cell coefficients per cell are gathered separately.
+
+// position of column c in row r of the factor pattern, or -1
+static int find_pos(const std::vector<int> &rp, const std::vector<int> &ci, int r, int c)
+{
+    auto b = ci.begin() + rp[r], e = ci.begin() + rp[r + 1];
+    auto it = std::lower_bound(b, e, c);
+    return (it != e && *it == c) ? (int)(it - ci.begin()) : -1;
+}
+
+// true when every cell has its full diagonal block; w_ext = the largest
+// number of external entries of a row in either triangular part
+static bool cell_pattern(int n, const std::vector<int> &rp, const std::vector<int> &ci,
+                         const std::vector<int> &dg, int &w_ext)
+{
+    w_ext = 0;
+    if (n % 3)
+        return false;
+    for (int r = 0; r < n; ++r) {
+        const int I = r / 3, a = r - 3 * I;
+        for (int c = 0; c < 3; ++c)
+            if (find_pos(rp, ci, r, 3 * I + c) < 0)
+                return false;
+        int ext_l = 0, ext_u = 0;
+        for (int j = rp[r]; j < dg[r]; ++j)
+            ext_l += ci[j] < 3 * I;
+        for (int j = dg[r] + 1; j < rp[r + 1]; ++j)
+            ext_u += ci[j] >= 3 * I + 3;
+        // the remaining entries of each part are the a (L) / 2-a (U) intra ones
+        if (dg[r] - rp[r] - ext_l != a || rp[r + 1] - dg[r] - 1 - ext_u != 2 - a)
+            return false;
+        w_ext = std::max(w_ext, std::max(ext_l, ext_u));
+    }
+    return true;
+}
+
+static void build_sweep_cells(int nblocks, const std::vector<int> &bstart,
+                              const std::vector<int> &rp, const std::vector<int> &ci,
+                              const std::vector<int> &dg, bool forward, int W, HostSweep &H)
+{
+    H.W = W;
+    H.R = 3;
+    H.blk_lev.assign(nblocks + 1, 0);
+    H.meta.clear();
+    H.rows.clear();
+    H.row_glob.clear();
+    H.col.clear();
+    H.src.clear();
+    H.intra_src.clear();
+    H.max_count = 0;
+    H.max_width = 0;
+    const int n = bstart[nblocks], ncell = n / 3;
+    std::vector<int> level(ncell, 0), cnt, first, order;
+    // external entries of row r in operation (column) order
+    auto ext_begin = [&](int r) { return forward ? rp[r] : dg[r] + 1; };
+    auto ext_end = [&](int r) { return forward ? dg[r] : rp[r + 1]; };
+    auto is_ext = [&](int r, int c) { return forward ? c < 3 * (r / 3) : c >= 3 * (r / 3) + 3; };
+    for (int b = 0; b < nblocks; ++b) {
+        const int c0 = bstart[b] / 3, c1 = bstart[b + 1] / 3, b0 = bstart[b];
+        int maxl = -1;
+        for (int q = 0; q < c1 - c0; ++q) {
+            const int I = forward ? c0 + q : c1 - 1 - q;
+            int l = 0;
+            for (int a = 0; a < 3; ++a) {
+                const int r = 3 * I + a;
+                for (int j = ext_begin(r); j < ext_end(r); ++j)
+                    if (is_ext(r, ci[j]))
+                        l = std::max(l, level[ci[j] / 3] + 1);
+            }
+            level[I] = l;
+            maxl = std::max(maxl, l);
+        }
+        const int nlev = maxl + 1;
+        cnt.assign(nlev + 1, 0);
+        for (int I = c0; I < c1; ++I)
+            cnt[level[I] + 1]++;
+        for (int l = 0; l < nlev; ++l)
+            cnt[l + 1] += cnt[l];
+        first.assign(cnt.begin(), cnt.end());
+        order.resize(c1 - c0);
+        for (int I = c0; I < c1; ++I)
+            order[first[level[I]]++] = I;
+        H.blk_lev[b] = (int)H.meta.size();
+        for (int l = 0; l < nlev; ++l) {
+            LevelMeta m;
+            m.count = cnt[l + 1] - cnt[l];
+            m.off = (int)H.col.size();
+            m.roff = (int)H.rows.size();
+            m.pad = 0;
+            H.max_count = std::max(H.max_count, m.count);
+            for (int t = 0; t < m.count; ++t) {
+                const int I = order[cnt[l] + t];
+                for (int a = 0; a < 3; ++a) {
+                    const int r = 3 * I + a;
+                    H.rows.push_back(r - b0);
+                    H.row_glob.push_back(r);
+                    const size_t base = H.col.size();
+                    H.col.resize(base + W, bstart[b + 1] - b0);
+                    H.src.resize(base + W, -1);
+                    int s = 0;
+                    for (int j = ext_begin(r); j < ext_end(r); ++j)
+                        if (is_ext(r, ci[j])) {
+                            H.col[base + s] = ci[j] - b0;
+                            H.src[base + s] = j;
+                            ++s;
+                        }
+                    H.max_width = std::max(H.max_width, s);
+                }
+                if (forward) {
+                    H.intra_src.push_back(find_pos(rp, ci, 3 * I + 1, 3 * I));
+                    H.intra_src.push_back(find_pos(rp, ci, 3 * I + 2, 3 * I));
+                    H.intra_src.push_back(find_pos(rp, ci, 3 * I + 2, 3 * I + 1));
+                } else {
+                    H.intra_src.push_back(find_pos(rp, ci, 3 * I, 3 * I + 1));
+                    H.intra_src.push_back(find_pos(rp, ci, 3 * I, 3 * I + 2));
+                    H.intra_src.push_back(find_pos(rp, ci, 3 * I + 1, 3 * I + 2));
+                }
+            }
+            H.meta.push_back(m);
+        }
+    }
+    H.blk_lev[nblocks] = (int)H.meta.size();
+    H.col.resize(H.col.size() + 3 * W, 0);
+    H.src.resize(H.src.size() + 3 * W, -1);
+    H.row_glob.push_back(0);
+    H.intra_src.resize(H.intra_src.size() + 3, -1);
 }
 
 // MTJ's factor loop reads colind[rowptr[k+1]] when row `index` has a column
@@ -315,15 +463,48 @@ void prec_setup(rsgpu_prec *M, rsgpu_csr *A)
 
     // L (unit lower, forward), U (upper, backward) in operation order: the
     // factor row's entries left and right of the diagonal
+    const int es = M->precision == RSGPU_FP64 ? 8 : 4;
+    // 3x3 block matrices with cell-aligned blocks: cell schedules, when the
+    // pipelined cell sweep applies (at most 64 cells per level, the block in
+    // LDS)
+    M->cell = false;
+    bool aligned = A->bs == 3 && getenv("RSGPU_ILU_CELLS") == nullptr &&
+                   ilu_kernel_choice() != 3;
+    for (int b = 0; aligned && b <= nb; ++b)
+        aligned = M->h_bstart[b] % 3 == 0;
+    int w_ext = 0;
+    if (aligned && cell_pattern(n, rp, ci, dg, w_ext) && w_ext <= 16) {
+        const int Wc = pad_width(w_ext);
+        HostSweep HL, HU;
+        build_sweep_cells(nb, M->h_bstart, rp, ci, dg, true, Wc, HL);
+        build_sweep_cells(nb, M->h_bstart, rp, ci, dg, false, Wc, HU);
+        int maxlev = 0;
+        for (int b = 0; b < nb; ++b)
+            maxlev = std::max(maxlev, (HL.blk_lev[b + 1] - HL.blk_lev[b]) +
+                                          (HU.blk_lev[b + 1] - HU.blk_lev[b]));
+        const size_t shm = (((size_t)(M->max_block_rows + 2) * es + 15) & ~(size_t)15) +
+                           (size_t)maxlev * sizeof(LevelMeta);
+        // 32-bit byte offsets into every sweep array
+        const size_t lim = (size_t)INT32_MAX / 8;
+        const bool addressable = HL.col.size() <= lim && HU.col.size() <= lim &&
+                                 HL.rows.size() + 1 <= lim && HU.rows.size() + 1 <= lim;
+        if (HL.max_count <= 64 && HU.max_count <= 64 && addressable &&
+            shm <= (size_t)lds_limit(M->ctx->device)) {
+            upload_sweep(HL, M->L, s, es);
+            upload_sweep(HU, M->U, s, es);
+            RS_HIP(hipStreamSynchronize(s));
+            M->cell = true;
+        }
+    }
     const RangeEntries eL{rp.data(), dg.data(), 0, 0, ci.data()};
     const RangeEntries eU{dg.data(), rp.data() + 1, 1, 0, ci.data()};
     const int W = pad_width(std::max(max_entries(eL, n), max_entries(eU, n)));
-    {
+    if (!M->cell) {
         HostSweep H;
         build_sweep(nb, M->h_bstart, eL, true, W, H);
-        upload_sweep(H, M->L, s, M->precision == RSGPU_FP64 ? 8 : 4);
+        upload_sweep(H, M->L, s, es);
         build_sweep(nb, M->h_bstart, eU, false, W, H);
-        upload_sweep(H, M->U, s, M->precision == RSGPU_FP64 ? 8 : 4);
+        upload_sweep(H, M->U, s, es);
         RS_HIP(hipStreamSynchronize(s));   // H's host arrays are released next
     }
     M->status.alloc(2);
@@ -332,11 +513,19 @@ void prec_setup(rsgpu_prec *M, rsgpu_csr *A)
         M->ell64_L.alloc(M->L.ell_size);
         M->ell64_U.alloc(M->U.ell_size);
         M->dgU64.alloc(M->U.nrows_lev);
+        if (M->cell) {
+            M->in64_L.alloc(M->L.intra_src.n);
+            M->in64_U.alloc(M->U.intra_src.n);
+        }
     } else {
         M->lu32.alloc(std::max(M->nnz, 1));
         M->ell32_L.alloc(M->L.ell_size);
         M->ell32_U.alloc(M->U.ell_size);
         M->dgU32.alloc(M->U.nrows_lev);
+        if (M->cell) {
+            M->in32_L.alloc(M->L.intra_src.n);
+            M->in32_U.alloc(M->U.intra_src.n);
+        }
     }
     const int elem = M->precision == RSGPU_FP64 ? 8 : 4;
     M->lds_rows = RS_LDS_BYTES / elem - 1;
@@ -425,18 +614,21 @@ __global__ void k_load_values(const double *__restrict__ a, const int *__restric
 
 // One workgroup per block; levels in order; one lane per row performing
 // the IKJ update sequence of ILU.java:99-122 for that row.
+// With a cell schedule (R = 3) a level entry is a cell and its lane
+// factors the cell's three rows in order.
 template <typename T>
 __global__ void __launch_bounds__(256)
     k_ilu_factor(const int *__restrict__ bstart, const int *__restrict__ blk_lev,
                  const LevelMeta *__restrict__ meta, const int *__restrict__ rows,
                  const int *__restrict__ rowptr, const int *__restrict__ colind,
-                 const int *__restrict__ diag, T *lu, int *status)
+                 const int *__restrict__ diag, T *lu, int *status, int R)
 {
     const int b = blockIdx.x;
     const int b0 = bstart[b];
     for (int l = blk_lev[b]; l < blk_lev[b + 1]; ++l) {
         const LevelMeta m = meta[l];
-        for (int t = threadIdx.x; t < m.count; t += blockDim.x) {
+        for (int tr = threadIdx.x * R; tr < m.count * R; tr += blockDim.x * R)
+            for (int t = tr; t < tr + R; ++t) {
             const int k = b0 + rows[m.roff + t];
             const int rk0 = rowptr[k], rk1 = rowptr[k + 1], dk = diag[k];
             for (int i = rk0; i < dk; ++i) {
@@ -458,7 +650,7 @@ __global__ void __launch_bounds__(256)
                         lu[l2] -= LUki * lu[j];
                 }
             }
-        }
+            }
         __syncthreads();
     }
 }
@@ -872,6 +1064,241 @@ __global__ void __launch_bounds__(64)
         x[b0 + r] = xv[r];
 }
 
+// ---------------------------------------------------------------- cell sweeps
+// One lane per cell (3 rows); a level's external operands of all three rows
+// are gathered at once, and the rows are finished in order with the
+// intra-cell terms taken from registers, in the reference's summation order
+// (L: external entries, then the strictly lower diagonal-block ones; U: the
+// strictly upper diagonal-block ones, then the external entries).
+struct PfSweep3 {
+    PfSweep S;
+    const void *intra;
+    int intra_bytes;
+};
+
+struct PfRsrc3 {
+    PfRsrc R;
+    __amdgpu_buffer_rsrc_t intra;
+};
+
+__device__ __forceinline__ PfRsrc3 pf_rsrc3(const PfSweep3 &S)
+{
+    PfRsrc3 R;
+    R.R = pf_rsrc(S.S);
+    R.intra = __builtin_amdgcn_make_buffer_rsrc((void *)S.intra, (short)0, S.intra_bytes,
+                                                RS_BUF_WORD3);
+    return R;
+}
+
+template <typename T, int W>
+struct CellRegs {
+    int r;        // LDS byte offset of the cell's first row (inactive lanes: a real cell)
+    bool act;
+    T d[3];       // U diagonals of the three rows
+    T in[3];      // intra-cell coefficients
+    VRow<T, W> v[3];
+    CRow<W> c[3];
+};
+
+template <typename T>
+__device__ __forceinline__ T buf_scalar(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    T out;
+    if (sizeof(T) == 8) {
+        const rs_v2i q = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+        __builtin_memcpy(&out, &q, 8);
+    } else {
+        const int q = __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+        __builtin_memcpy(&out, &q, 4);
+    }
+    return out;
+}
+
+template <typename T, int W, bool DIV>
+__device__ __forceinline__ void cell_load(const LevelMeta &m, bool valid, const PfRsrc3 &rs,
+                                          CellRegs<T, W> &R)
+{
+    const int count = __builtin_amdgcn_readfirstlane(m.count);
+    const int off = __builtin_amdgcn_readfirstlane(m.off);
+    const int roff = __builtin_amdgcn_readfirstlane(m.roff);
+    const int t = threadIdx.x;
+    const int tt = min(t, count - 1);
+    constexpr int es = (int)sizeof(T);
+    R.act = valid && t < count;
+    R.r = __builtin_amdgcn_raw_buffer_load_b32(rs.R.rows, tt * 12, roff * 4, 0);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        buf_load<W * 4>(&R.c[a], rs.R.cols, (3 * tt + a) * (W * 4), off * 4);
+        buf_load<W * es>(&R.v[a], rs.R.vals, (3 * tt + a) * (W * es), off * es);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        R.in[k] = buf_scalar<T>(rs.intra, (3 * tt + k) * es, roff * es);
+    if (DIV) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+            R.d[a] = buf_scalar<T>(rs.R.dgl, (3 * tt + a) * es, roff * es);
+    }
+}
+
+template <typename T, int W>
+struct CellGathered {
+    T rhs[3];
+    T xs[3][W];
+};
+
+template <typename T, int W>
+__device__ __forceinline__ void cell_gather(const CellRegs<T, W> &R, unsigned char *xv,
+                                            CellGathered<T, W> &g)
+{
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        g.rhs[a] = lds_at<T>(xv, R.r + a * (int)sizeof(T));
+#pragma unroll
+        for (int s = 0; s < W; ++s)
+            g.xs[a][s] = lds_at<T>(xv, R.c[a].c[s]);
+    }
+}
+
+template <typename T, int W, bool DIV>
+__device__ __forceinline__ void cell_finish(const CellRegs<T, W> &R, CellGathered<T, W> &g,
+                                            unsigned char *xv, int junkb)
+{
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        pin_gathers<T, W>(g.xs[a], g.rhs[a]);
+    constexpr int es = (int)sizeof(T);
+    T y[3];
+    if (!DIV) {
+        // L: y_a = rhs_a - (external terms, then L(a,0) y_0, L(a,1) y_1)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            T sum = 0;
+#pragma unroll
+            for (int s = 0; s < W; ++s)
+                sum += R.v[a].v[s] * g.xs[a][s];
+            if (a == 1)
+                sum += R.in[0] * y[0];
+            if (a == 2) {
+                sum += R.in[1] * y[0];
+                sum += R.in[2] * y[1];
+            }
+            y[a] = g.rhs[a] - sum;
+        }
+    } else {
+        // U: rows 2, 1, 0; y_a = (rhs_a - (U(a,a+1..2) terms, then external)) / d_a
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const int a = 2 - q;
+            T sum = 0;
+            if (a == 1)
+                sum += R.in[2] * y[2];
+            if (a == 0) {
+                sum += R.in[0] * y[1];
+                sum += R.in[1] * y[2];
+            }
+#pragma unroll
+            for (int s = 0; s < W; ++s)
+                sum += R.v[a].v[s] * g.xs[a][s];
+            y[a] = (g.rhs[a] - sum) / R.d[a];
+        }
+    }
+    const int wr = R.act ? R.r : junkb;
+    const int st = R.act ? es : 0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        lds_at<T>(xv, wr + a * st) = y[a];
+}
+
+template <typename T, int W, bool DIV>
+constexpr int cell_loads()
+{
+    return 1 + 3 * ((W * 4 + 15) / 16) + 3 * ((W * (int)sizeof(T) + 15) / 16) + 3 +
+           (DIV ? 3 : 0);
+}
+
+template <typename T, int W>
+constexpr int pf3_depth()
+{
+    constexpr int loads = cell_loads<T, W, true>();
+    constexpr int regs = 1 + 3 * (W + W * (int)sizeof(T) / 4) + 6 * (int)sizeof(T) / 4;
+    int d = 4;
+    while (d > 2 && (d * loads > 60 || d * regs > 210))
+        --d;
+    return d;
+}
+
+// The sweep_pf pipeline with cell level entries.
+template <typename T, int W, bool DIV>
+__device__ __forceinline__ void sweep_pf3(int nl, const LevelMeta *sm, const PfRsrc3 &rs,
+                                          unsigned char *xv, int junkb)
+{
+    constexpr int D = pf3_depth<T, W>();
+    if (nl <= 0)
+        return;
+    const int last = nl - 1;
+    CellRegs<T, W> r[D];
+#pragma unroll
+    for (int k = 0; k < D - 1; ++k) {
+        cell_load<T, W, DIV>(sm[min(k, last)], k <= last, rs, r[k]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    LevelMeta mq = sm[min(D - 1, last)];
+    for (int l = 0; l < nl; l += D) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            CellGathered<T, W> g;
+            cell_gather<T, W>(r[k], xv, g);
+            __builtin_amdgcn_sched_barrier(0);
+            const int lev = l + k + D - 1;
+            cell_load<T, W, DIV>(mq, lev <= last, rs, r[(k + D - 1) % D]);
+            mq = sm[min(lev + 1, last)];
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_sched_barrier(0);
+            cell_finish<T, W, DIV>(r[k], g, xv, junkb);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+}
+
+// One wave per block (at most 64 cells per level); L then U cell sweeps.
+template <typename T, int W>
+__global__ void __launch_bounds__(64)
+    k_ilu_apply_pf3(const int *__restrict__ bstart, PfSweep3 A, PfSweep3 B,
+                    const T *__restrict__ b, T *__restrict__ x, int xv_bytes,
+                    const int *skip1, const int *skip2, BlkSkip bskip)
+{
+    if ((skip1 && *skip1) || (skip2 && *skip2))
+        return;
+    if (blk_skipped(bskip, blockIdx.x))
+        return;
+    extern __shared__ unsigned char rs_smem[];
+    const int blk = blockIdx.x;
+    const int b0 = bstart[blk], b1 = bstart[blk + 1], nb = b1 - b0;
+    T *xv = (T *)rs_smem;
+    LevelMeta *sm = (LevelMeta *)(rs_smem + xv_bytes);
+    const int la0 = A.S.blk[blk], nla = A.S.blk[blk + 1] - la0;
+    const int lb0 = B.S.blk[blk], nlb = B.S.blk[blk + 1] - lb0;
+    for (int q = threadIdx.x; q < nla; q += blockDim.x)
+        sm[q] = A.S.meta[la0 + q];
+    for (int q = threadIdx.x; q < nlb; q += blockDim.x)
+        sm[nla + q] = B.S.meta[lb0 + q];
+    for (int r = threadIdx.x; r < nb; r += blockDim.x)
+        xv[r] = b[b0 + r];
+    if (threadIdx.x == 0)
+        xv[nb] = T(0);
+    __syncthreads();
+    // three junk slots after the zero slot, for inactive lanes
+    const int junkb = (nb + 1) * (int)sizeof(T);
+    const PfRsrc3 ra = pf_rsrc3(A), rb = pf_rsrc3(B);
+    sweep_pf3<T, W, false>(nla, sm, ra, rs_smem, junkb);
+    sweep_pf3<T, W, true>(nlb, sm + nla, rb, rs_smem, junkb);
+    __syncthreads();
+    for (int r = threadIdx.x; r < nb; r += blockDim.x)
+        x[b0 + r] = xv[r];
+}
+
 // ---------------------------------------------------------------- host ops
 template <typename T>
 static void launch_factor(rsgpu_prec *M, T *lu, const double *aval)
@@ -887,7 +1314,8 @@ static void launch_factor(rsgpu_prec *M, T *lu, const double *aval)
     RS_HIP(hipMemcpyAsync(M->status.p, init, sizeof(init), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_ilu_factor<T>, dim3(M->nblocks), dim3(M->L.max_count > 64 ? 256 : 64),
                        0, s, M->bstart.p, M->L.blk_lev.p, (const LevelMeta *)M->L.lev_meta.p,
-                       M->L.rows.p, M->rowptr.p, M->colind.p, M->diag.p, lu, M->status.p);
+                       M->L.rows.p, M->rowptr.p, M->colind.p, M->diag.p, lu, M->status.p,
+                       M->L.R);
     RS_CHECK_LAUNCH();
 }
 
@@ -898,6 +1326,17 @@ static void launch_ell(hipStream_t s, const SweepData &D, const T *lu, T *ell, b
         return;
     hipLaunchKernelGGL(k_gather_ell<T>, dim3((unsigned)((D.ell_size + 255) / 256)), dim3(256),
                        0, s, D.ell_src.p, lu, ell, D.ell_size, sum_form ? T(-0.0) : T(0.0));
+    RS_CHECK_LAUNCH();
+}
+
+// the intra-cell coefficients of a cell schedule
+template <typename T>
+static void launch_intra(hipStream_t s, const SweepData &D, const T *lu, T *dst)
+{
+    if (D.intra_src.n == 0)
+        return;
+    hipLaunchKernelGGL(k_gather_ell<T>, dim3((unsigned)((D.intra_src.n + 255) / 256)), dim3(256),
+                       0, s, D.intra_src.p, lu, dst, D.intra_src.n, T(0.0));
     RS_CHECK_LAUNCH();
 }
 
@@ -921,6 +1360,10 @@ int prec_factor(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
         launch_ell<double>(s, M->L, M->lu64.p, M->ell64_L.p, true);
         launch_ell<double>(s, M->U, M->lu64.p, M->ell64_U.p, true);
         launch_rowdiag<double>(s, M->U, M->diag.p, M->lu64.p, M->dgU64.p);
+        if (M->cell) {
+            launch_intra<double>(s, M->L, M->lu64.p, M->in64_L.p);
+            launch_intra<double>(s, M->U, M->lu64.p, M->in64_U.p);
+        }
         if (M->has_t) {
             launch_ell<double>(s, M->Ut, M->lu64.p, M->ell64_Ut.p, false);
             launch_ell<double>(s, M->Lt, M->lu64.p, M->ell64_Lt.p, false);
@@ -931,6 +1374,10 @@ int prec_factor(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
         launch_ell<float>(s, M->L, M->lu32.p, M->ell32_L.p, true);
         launch_ell<float>(s, M->U, M->lu32.p, M->ell32_U.p, true);
         launch_rowdiag<float>(s, M->U, M->diag.p, M->lu32.p, M->dgU32.p);
+        if (M->cell) {
+            launch_intra<float>(s, M->L, M->lu32.p, M->in32_L.p);
+            launch_intra<float>(s, M->U, M->lu32.p, M->in32_U.p);
+        }
     }
     RS_HIP(hipMemcpyAsync(st, M->status.p, sizeof(st), hipMemcpyDeviceToHost, s));
     RS_HIP(hipStreamSynchronize(s));
@@ -1059,6 +1506,47 @@ static void launch_apply(rsgpu_prec *M, const SweepData &A_, const T *va, const 
     RS_CHECK_LAUNCH();
 }
 
+// Cell schedules (3x3 block matrices): the pipelined cell sweeps, forward L
+// then backward U; prec_setup only selects them when every level holds at
+// most 64 cells, the block fits LDS and the arrays are 32-bit addressable.
+template <typename T, int W>
+static void launch_pf3(rsgpu_prec *M, PfSweep3 A, PfSweep3 B, const T *b, T *x,
+                       size_t xv_bytes, size_t shm, const int *sk1, const int *sk2)
+{
+    static size_t granted = 65536;
+    if (shm > granted) {
+        RS_HIP(hipFuncSetAttribute((const void *)k_ilu_apply_pf3<T, W>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        granted = shm;
+    }
+    hipLaunchKernelGGL((k_ilu_apply_pf3<T, W>), dim3(M->nblocks), dim3(64), shm, M->ctx->stream,
+                       M->bstart.p, A, B, b, x, (int)xv_bytes, sk1, sk2, M->skip);
+}
+
+template <typename T>
+static void launch_apply_cells(rsgpu_prec *M, const T *vl, const T *il, const T *vu,
+                               const T *iu, const T *du, const T *b, T *x, const int *sk1,
+                               const int *sk2)
+{
+    if (M->n == 0)
+        return;
+    const size_t xv_bytes = ((size_t)(M->max_block_rows + 2) * sizeof(T) + 15) & ~(size_t)15;
+    const size_t shm =
+        xv_bytes + (size_t)(M->L.max_levels + M->U.max_levels) * sizeof(LevelMeta);
+    const PfSweep3 A{pf_sweep<T>(M->L, vl, nullptr), il, (int)(M->L.intra_src.n * sizeof(T))};
+    const PfSweep3 B{pf_sweep<T>(M->U, vu, du), iu, (int)(M->U.intra_src.n * sizeof(T))};
+    const int W = M->L.W;
+    if (W <= 4)
+        launch_pf3<T, 4>(M, A, B, b, x, xv_bytes, shm, sk1, sk2);
+    else if (W <= 8)
+        launch_pf3<T, 8>(M, A, B, b, x, xv_bytes, shm, sk1, sk2);
+    else if (W <= 12)
+        launch_pf3<T, 12>(M, A, B, b, x, xv_bytes, shm, sk1, sk2);
+    else
+        launch_pf3<T, 16>(M, A, B, b, x, xv_bytes, shm, sk1, sk2);
+    RS_CHECK_LAUNCH();
+}
+
 __global__ void k_copy_skip(const double *__restrict__ a, double *__restrict__ b, int n,
                             const int *sk1, const int *sk2)
 {
@@ -1183,16 +1671,25 @@ void prec_apply(rsgpu_prec *M, const double *b, double *x, const DevState *st)
         return;
     }
     if (M->precision == RSGPU_FP64) {
-        launch_apply<double, false>(M, M->L, M->ell64_L.p, nullptr, M->U, M->ell64_U.p,
-                                    M->dgU64.p, b, x, M->scratch64.p, sk1, sk2);
+        if (M->cell)
+            launch_apply_cells<double>(M, M->ell64_L.p, M->in64_L.p, M->ell64_U.p, M->in64_U.p,
+                                       M->dgU64.p, b, x, sk1, sk2);
+        else
+            launch_apply<double, false>(M, M->L, M->ell64_L.p, nullptr, M->U, M->ell64_U.p,
+                                        M->dgU64.p, b, x, M->scratch64.p, sk1, sk2);
         return;
     }
     // FP32 factors on FP64 vectors
     M->tmpa32.alloc(M->n);
     M->tmpb32.alloc(M->n);
     convert_f64_to_f32(s, b, M->tmpa32.p, M->n);
-    launch_apply<float, false>(M, M->L, M->ell32_L.p, nullptr, M->U, M->ell32_U.p, M->dgU32.p,
-                               M->tmpa32.p, M->tmpb32.p, M->scratch32.p, sk1, sk2);
+    if (M->cell)
+        launch_apply_cells<float>(M, M->ell32_L.p, M->in32_L.p, M->ell32_U.p, M->in32_U.p,
+                                  M->dgU32.p, M->tmpa32.p, M->tmpb32.p, sk1, sk2);
+    else
+        launch_apply<float, false>(M, M->L, M->ell32_L.p, nullptr, M->U, M->ell32_U.p,
+                                   M->dgU32.p, M->tmpa32.p, M->tmpb32.p, M->scratch32.p, sk1,
+                                   sk2);
     convert_f32_to_f64(s, M->tmpb32.p, x, M->n);
 }
 
@@ -1218,8 +1715,12 @@ void prec_apply(rsgpu_prec *M, const float *b, float *x, const DevState *st)
     if (M->precision != RSGPU_FP32 || M->kind == RSGPU_PREC_AMG ||
         M->kind == RSGPU_PREC_SSOR)
         throw RsError{RSGPU_BAD_ARGUMENT, "FP32 apply needs an FP32 ILU or diagonal preconditioner"};
-    launch_apply<float, false>(M, M->L, M->ell32_L.p, nullptr, M->U, M->ell32_U.p, M->dgU32.p,
-                               b, x, M->scratch32.p, sk1, sk2);
+    if (M->cell)
+        launch_apply_cells<float>(M, M->ell32_L.p, M->in32_L.p, M->ell32_U.p, M->in32_U.p,
+                                  M->dgU32.p, b, x, sk1, sk2);
+    else
+        launch_apply<float, false>(M, M->L, M->ell32_L.p, nullptr, M->U, M->ell32_U.p,
+                                   M->dgU32.p, b, x, M->scratch32.p, sk1, sk2);
 }
 
 void prec_trans_apply(rsgpu_prec *M, const double *b, double *x, const DevState *st)

@@ -175,3 +175,96 @@ def test_gmres_on_block_matrix(gpu, oracle):
     assert so.status == 0
     assert abs(st.iterations - so.iterations) <= max(3, so.iterations // 10)
     assert np.linalg.norm(x.cpu().numpy() - xo) / np.linalg.norm(xo) < 1e-8
+
+
+def _filter_blocks(rp, ci, val, bstart):
+    """Entries of A whose row and column lie in the same block (the
+    block-Jacobi pattern), as CSR."""
+    n = len(rp) - 1
+    blk = np.searchsorted(bstart, np.arange(n), side="right") - 1
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    keep = blk[rows] == blk[ci]
+    frp = np.zeros(n + 1, np.int32)
+    frp[1:] = np.cumsum(np.bincount(rows[keep], minlength=n))
+    return frp, ci[keep].astype(np.int32), val[keep]
+
+
+@pytest.mark.parametrize("blocks", ["whole", "lines2", "lines6", "planes"])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_cell_ilu_bitexact(gpu, oracle, monkeypatch, blocks, prec):
+    """ILU(0) on a 3x3 block matrix with cell-aligned blocks runs cell
+    schedules (three rows per lane, one level per cell wavefront): factors
+    and apply bit-exact against the oracle's scalar ILU(0) of the
+    block-filtered matrix, in FP64 and FP32; and with fewer levels than the
+    row schedule of the same blocks."""
+    import torch
+    from ressim_amd import _lib
+    pr = P.config3(nx=11, ny=7, nz=4)
+    rng = np.random.default_rng(12)
+    u = pr.u.copy()
+    u[0::3] += rng.uniform(-300, 300, pr.n)
+    bo, J, R = _device_jacobian(gpu, pr, u)
+    n = J.n
+    if blocks == "whole":
+        bstart = np.array([0, n], np.int32)
+    elif blocks == "planes":
+        bstart = np.arange(0, n + 1, 3 * pr.nx * pr.ny * 2, dtype=np.int32)
+    else:
+        bstart = _plane_blocks(pr, int(blocks[5:])).astype(np.int32)
+    print(f"cell ILU blocks {blocks} ({len(bstart) - 1}) prec {prec}", flush=True)
+    precision = _lib.FP32 if prec == "f32" else _lib.FP64
+    if prec == "f32":
+        J.sync_f32()
+
+    def make():
+        M = gpu.BlockILU(1, precision=precision)
+        M.setBlocks(bstart)
+        M.setMatrix(J)
+        return M
+
+    M = make()
+    frp, fci, fval = _filter_blocks(J.getRowPointers(), J.getColumnIndices(), J.getData(),
+                                    bstart)
+    rp_g, ci_g, lu_g = M.factors()
+    assert np.array_equal(rp_g, frp) and np.array_equal(ci_g, fci)
+    b = rng.standard_normal(n)
+    if prec == "f64":
+        st, lu, dg, _ = oracle.ilu_factor(frp, fci, fval)
+        assert st == 0 and np.array_equal(lu_g, lu)
+        z = torch.zeros(n, dtype=torch.float64, device="cuda")
+        M.apply(torch.tensor(b, device="cuda"), z)
+        assert np.array_equal(z.cpu().numpy(), oracle.ilu_apply(frp, fci, lu, dg, b))
+    else:
+        st, lu, dg, _ = oracle.ilu_factor_f32(frp, fci, fval.astype(np.float32))
+        assert st == 0 and np.array_equal(lu_g.astype(np.float32), lu)
+        b32 = b.astype(np.float32)
+        bd = torch.tensor(b32, device="cuda")
+        z = torch.zeros(n, dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        assert _lib.load().rsgpu_prec_apply_f32(M.h, bd.data_ptr(), z.data_ptr()) == 0
+        M.ctx.synchronize()
+        assert np.array_equal(z.cpu().numpy(), oracle.ilu_apply_f32(frp, fci, lu, dg, b32))
+    # the cell schedule has about a third of the row schedule's levels
+    cells = M.info()
+    monkeypatch.setenv("RSGPU_ILU_CELLS", "0")
+    rows = make().info()
+    assert 2 * cells["levels_lower"] < rows["levels_lower"]
+    assert 2 * cells["levels_upper"] < rows["levels_upper"]
+
+
+def test_cell_ilu_transapply(gpu, oracle):
+    """transApply of a cell-scheduled ILU keeps the row sweeps of the
+    transposed factors: bit-exact against the oracle."""
+    import torch
+    pr = P.config3(nx=9, ny=6, nz=3)
+    bo, J, R = _device_jacobian(gpu, pr)
+    M = gpu.BlockILU(1)
+    M.setBlocks(_plane_blocks(pr, 3))
+    M.setMatrix(J)
+    frp, fci, fval = _filter_blocks(J.getRowPointers(), J.getColumnIndices(), J.getData(),
+                                    _plane_blocks(pr, 3).astype(np.int32))
+    st, lu, dg, _ = oracle.ilu_factor(frp, fci, fval)
+    b = np.random.default_rng(3).standard_normal(J.n)
+    xt = torch.zeros(J.n, dtype=torch.float64, device="cuda")
+    M.transApply(torch.tensor(b, device="cuda"), xt)
+    assert np.array_equal(xt.cpu().numpy(), oracle.ilu_transapply(frp, fci, lu, dg, b))
