@@ -48,11 +48,12 @@ struct HostSweep {
     std::vector<int> rows, row_glob, col, src;
     int max_count = 0, max_width = 0;
     int R = 1;                         // rows per level entry (3: cells)
-    std::vector<int> intra_src;        // R = 3: 3 per cell
+    int irec = 0, vrec = 0;            // R = 3: record sizes per cell
 };
 
 static int lds_limit(int device);
 static int ilu_kernel_choice();
+static size_t cell_xv_bytes(int max_block_rows, int es);
 
 static int pad_width(int w)
 {
@@ -181,12 +182,20 @@ static int max_entries(const Ent &ent, int n)
 static void upload_sweep(const HostSweep &H, SweepData &D, hipStream_t s, int es)
 {
     D.W = H.W;
+    D.R = H.R;
+    D.irec = H.irec;
+    D.vrec = H.vrec;
     D.blk_lev.upload(H.blk_lev, s);
     D.lev_meta.upload((const int *)H.meta.data(), H.meta.size() * 4, s);
     D.rows.upload(H.rows, s);
     D.row_glob.upload(H.row_glob, s);
-    D.ell_col.upload(H.col, s);
-    {
+    if (H.R == 3) {
+        // cell records: colb already holds LDS byte offsets
+        D.colb.upload(H.col, s);
+        D.rowsb.upload(std::vector<int>(1, 0), s);
+        D.ell_col.release();
+    } else {
+        D.ell_col.upload(H.col, s);
         std::vector<int> cb(H.col.size()), rb(H.rows.size());
         for (size_t q = 0; q < cb.size(); ++q)
             cb[q] = H.col[q] * es;
@@ -196,7 +205,7 @@ static void upload_sweep(const HostSweep &H, SweepData &D, hipStream_t s, int es
         D.rowsb.upload(rb.empty() ? std::vector<int>(1, 0) : rb, s);
     }
     D.ell_src.upload(H.src, s);
-    D.ell_size = H.col.size();
+    D.ell_size = H.src.size();
     D.nrows_lev = (int)H.row_glob.size();
     D.nlev_total = (int)H.meta.size();
     D.max_levels = 0;
@@ -204,24 +213,21 @@ static void upload_sweep(const HostSweep &H, SweepData &D, hipStream_t s, int es
         D.max_levels = std::max(D.max_levels, H.blk_lev[b + 1] - H.blk_lev[b]);
     D.max_count = H.max_count;
     D.max_width = H.max_width;
-    D.R = H.R;
-    if (H.R == 3)
-        D.intra_src.upload(H.intra_src, s);
-    else
-        D.intra_src.release();
 }
 
 // ---------------------------------------------------------------- cell sweeps
-// For 3x3 block matrices (rows 3I..3I+2 of cell I, full diagonal blocks)
-// the rows of a cell depend on each other, so a row schedule needs three
-// levels per cell.  A cell schedule puts whole cells on a level; the lane of
-// a cell computes its three rows in order with the reference's per-row
-// operation sequence: in the L sweep row 3I+a sums its entries in cells
-// before I (ascending columns), then its strictly lower diagonal-block
-// entries (3I, 3I+1); in the U sweep row 3I+a first sums its strictly upper
-// diagonal-block entries, then its entries in later cells.  External
-// entries live in W padded slots per row (identity padding as in the row
-// ELL); the three intra-cell coefficients per cell are gathered separately.
+// For 3x3 block matrices (rows 3I..3I+2 of cell I, full 3x3 blocks) the rows
+// of a cell depend on each other, so a row schedule needs three levels per
+// cell.  A cell schedule puts whole cells on a level; the lane of a cell
+// computes its three rows in order with the reference's per-row operation
+// sequence: in the L sweep row 3I+a sums its entries in cells before I
+// (ascending columns), then its strictly lower diagonal-block entries; in
+// the U sweep row 3I+a first sums its strictly upper diagonal-block
+// entries, then its entries in later cells.  The three rows of a cell have
+// the same neighbour blocks, so the lane gathers each neighbour's three
+// vector entries once for all three rows.
+
+static int round4(int x) { return (x + 3) & ~3; }
 
 // position of column c in row r of the factor pattern, or -1
 static int find_pos(const std::vector<int> &rp, const std::vector<int> &ci, int r, int c)
@@ -231,65 +237,93 @@ static int find_pos(const std::vector<int> &rp, const std::vector<int> &ci, int 
     return (it != e && *it == c) ? (int)(it - ci.begin()) : -1;
 }
 
-// true when every cell has its full diagonal block; w_ext = the largest
-// number of external entries of a row in either triangular part
-static bool cell_pattern(int n, const std::vector<int> &rp, const std::vector<int> &ci,
-                         const std::vector<int> &dg, int &w_ext)
+// First external entry of row r = 3I+a in one triangle and their count:
+// L: [rp[r], dg[r] - a) (the a intra entries 3I..r-1 follow); U:
+// [dg[r] + 1 + (2 - a), rp[r+1]) (after the intra entries r+1..3I+2).
+static void cell_ext(const std::vector<int> &rp, const std::vector<int> &dg, int r,
+                     bool forward, int &j0, int &cnt)
 {
-    w_ext = 0;
+    const int a = r % 3;
+    if (forward) {
+        j0 = rp[r];
+        cnt = dg[r] - a - rp[r];
+    } else {
+        j0 = dg[r] + 3 - a;
+        cnt = rp[r + 1] - j0;
+    }
+}
+
+// true when every cell has its full diagonal block and its three rows have
+// the same external columns, whole blocks 3J..3J+2, in both triangles; wb =
+// the most neighbour blocks of a cell in either triangle
+static bool cell_pattern(int n, const std::vector<int> &rp, const std::vector<int> &ci,
+                         const std::vector<int> &dg, int &wb)
+{
+    wb = 0;
     if (n % 3)
         return false;
-    for (int r = 0; r < n; ++r) {
-        const int I = r / 3, a = r - 3 * I;
-        for (int c = 0; c < 3; ++c)
-            if (find_pos(rp, ci, r, 3 * I + c) < 0)
+    for (int I = 0; I < n / 3; ++I) {
+        for (int a = 0; a < 3; ++a)
+            for (int c = 0; c < 3; ++c)
+                if (find_pos(rp, ci, 3 * I + a, 3 * I + c) < 0)
+                    return false;
+        for (int f = 0; f < 2; ++f) {
+            int j00, cnt0;
+            cell_ext(rp, dg, 3 * I, f == 0, j00, cnt0);
+            if (cnt0 < 0 || cnt0 % 3)
                 return false;
-        int ext_l = 0, ext_u = 0;
-        for (int j = rp[r]; j < dg[r]; ++j)
-            ext_l += ci[j] < 3 * I;
-        for (int j = dg[r] + 1; j < rp[r + 1]; ++j)
-            ext_u += ci[j] >= 3 * I + 3;
-        // the remaining entries of each part are the a (L) / 2-a (U) intra ones
-        if (dg[r] - rp[r] - ext_l != a || rp[r + 1] - dg[r] - 1 - ext_u != 2 - a)
-            return false;
-        w_ext = std::max(w_ext, std::max(ext_l, ext_u));
+            for (int a = 0; a < 3; ++a) {
+                int j0, cnt;
+                cell_ext(rp, dg, 3 * I + a, f == 0, j0, cnt);
+                if (cnt != cnt0)
+                    return false;
+                for (int q = 0; q < cnt; ++q) {
+                    const int J = ci[j00 + q - q % 3] / 3;
+                    if (ci[j00 + q - q % 3] % 3 || ci[j0 + q] != 3 * J + q % 3 ||
+                        (f == 0 ? J >= I : J <= I))
+                        return false;
+                }
+            }
+            wb = std::max(wb, cnt0 / 3);
+        }
     }
     return true;
 }
 
+// Cell schedule of one triangle with WB neighbour-block slots per cell.
+// Level meta: count = cells, off = cells before (record index), roff = rows
+// before (3 per cell, the factorisation's row list).
 static void build_sweep_cells(int nblocks, const std::vector<int> &bstart,
                               const std::vector<int> &rp, const std::vector<int> &ci,
-                              const std::vector<int> &dg, bool forward, int W, HostSweep &H)
+                              const std::vector<int> &dg, bool forward, int WB, int es,
+                              HostSweep &H)
 {
-    H.W = W;
+    const int irec = round4(1 + WB), vrec = round4(9 * WB + (forward ? 3 : 6));
+    H.W = WB;
     H.R = 3;
+    H.irec = irec;
+    H.vrec = vrec;
     H.blk_lev.assign(nblocks + 1, 0);
     H.meta.clear();
     H.rows.clear();
     H.row_glob.clear();
     H.col.clear();
     H.src.clear();
-    H.intra_src.clear();
     H.max_count = 0;
     H.max_width = 0;
     const int n = bstart[nblocks], ncell = n / 3;
     std::vector<int> level(ncell, 0), cnt, first, order;
-    // external entries of row r in operation (column) order
-    auto ext_begin = [&](int r) { return forward ? rp[r] : dg[r] + 1; };
-    auto ext_end = [&](int r) { return forward ? dg[r] : rp[r + 1]; };
-    auto is_ext = [&](int r, int c) { return forward ? c < 3 * (r / 3) : c >= 3 * (r / 3) + 3; };
+    int cells = 0;
     for (int b = 0; b < nblocks; ++b) {
         const int c0 = bstart[b] / 3, c1 = bstart[b + 1] / 3, b0 = bstart[b];
+        const int zb = (bstart[b + 1] - b0) * es;   // the zero slots
         int maxl = -1;
         for (int q = 0; q < c1 - c0; ++q) {
             const int I = forward ? c0 + q : c1 - 1 - q;
-            int l = 0;
-            for (int a = 0; a < 3; ++a) {
-                const int r = 3 * I + a;
-                for (int j = ext_begin(r); j < ext_end(r); ++j)
-                    if (is_ext(r, ci[j]))
-                        l = std::max(l, level[ci[j] / 3] + 1);
-            }
+            int j0, ne, l = 0;
+            cell_ext(rp, dg, 3 * I, forward, j0, ne);
+            for (int k = 0; k < ne; k += 3)
+                l = std::max(l, level[ci[j0 + k] / 3] + 1);
             level[I] = l;
             maxl = std::max(maxl, l);
         }
@@ -307,46 +341,53 @@ static void build_sweep_cells(int nblocks, const std::vector<int> &bstart,
         for (int l = 0; l < nlev; ++l) {
             LevelMeta m;
             m.count = cnt[l + 1] - cnt[l];
-            m.off = (int)H.col.size();
-            m.roff = (int)H.rows.size();
+            m.off = cells;
+            m.roff = 3 * cells;
             m.pad = 0;
             H.max_count = std::max(H.max_count, m.count);
             for (int t = 0; t < m.count; ++t) {
                 const int I = order[cnt[l] + t];
+                int j0, ne;
+                cell_ext(rp, dg, 3 * I, forward, j0, ne);
+                const int nbr = ne / 3;
+                H.max_width = std::max(H.max_width, nbr);
+                const size_t ib = H.col.size(), vb = H.src.size();
+                H.col.resize(ib + irec, zb);
+                H.src.resize(vb + vrec, -1);
+                H.col[ib] = (3 * I - b0) * es;
+                for (int k = 0; k < nbr; ++k)
+                    H.col[ib + 1 + k] = (ci[j0 + 3 * k] - b0) * es;
                 for (int a = 0; a < 3; ++a) {
                     const int r = 3 * I + a;
                     H.rows.push_back(r - b0);
                     H.row_glob.push_back(r);
-                    const size_t base = H.col.size();
-                    H.col.resize(base + W, bstart[b + 1] - b0);
-                    H.src.resize(base + W, -1);
-                    int s = 0;
-                    for (int j = ext_begin(r); j < ext_end(r); ++j)
-                        if (is_ext(r, ci[j])) {
-                            H.col[base + s] = ci[j] - b0;
-                            H.src[base + s] = j;
-                            ++s;
-                        }
-                    H.max_width = std::max(H.max_width, s);
+                    int ja, na;
+                    cell_ext(rp, dg, r, forward, ja, na);
+                    for (int q = 0; q < na; ++q)
+                        H.src[vb + 3 * WB * a + q] = ja + q;
                 }
+                const size_t iv = vb + 9 * WB;
                 if (forward) {
-                    H.intra_src.push_back(find_pos(rp, ci, 3 * I + 1, 3 * I));
-                    H.intra_src.push_back(find_pos(rp, ci, 3 * I + 2, 3 * I));
-                    H.intra_src.push_back(find_pos(rp, ci, 3 * I + 2, 3 * I + 1));
+                    H.src[iv + 0] = find_pos(rp, ci, 3 * I + 1, 3 * I);
+                    H.src[iv + 1] = find_pos(rp, ci, 3 * I + 2, 3 * I);
+                    H.src[iv + 2] = find_pos(rp, ci, 3 * I + 2, 3 * I + 1);
                 } else {
-                    H.intra_src.push_back(find_pos(rp, ci, 3 * I, 3 * I + 1));
-                    H.intra_src.push_back(find_pos(rp, ci, 3 * I, 3 * I + 2));
-                    H.intra_src.push_back(find_pos(rp, ci, 3 * I + 1, 3 * I + 2));
+                    H.src[iv + 0] = find_pos(rp, ci, 3 * I, 3 * I + 1);
+                    H.src[iv + 1] = find_pos(rp, ci, 3 * I, 3 * I + 2);
+                    H.src[iv + 2] = find_pos(rp, ci, 3 * I + 1, 3 * I + 2);
+                    for (int a = 0; a < 3; ++a)
+                        H.src[iv + 3 + a] = dg[3 * I + a];
                 }
+                ++cells;
             }
             H.meta.push_back(m);
         }
     }
     H.blk_lev[nblocks] = (int)H.meta.size();
-    H.col.resize(H.col.size() + 3 * W, 0);
-    H.src.resize(H.src.size() + 3 * W, -1);
+    // trailing dummy record: the arrays are never empty
+    H.col.resize(H.col.size() + irec, 0);
+    H.src.resize(H.src.size() + vrec, -1);
     H.row_glob.push_back(0);
-    H.intra_src.resize(H.intra_src.size() + 3, -1);
 }
 
 // MTJ's factor loop reads colind[rowptr[k+1]] when row `index` has a column
@@ -472,22 +513,22 @@ void prec_setup(rsgpu_prec *M, rsgpu_csr *A)
                    ilu_kernel_choice() != 3;
     for (int b = 0; aligned && b <= nb; ++b)
         aligned = M->h_bstart[b] % 3 == 0;
-    int w_ext = 0;
-    if (aligned && cell_pattern(n, rp, ci, dg, w_ext) && w_ext <= 16) {
-        const int Wc = pad_width(w_ext);
+    int wb = 0;
+    if (aligned && cell_pattern(n, rp, ci, dg, wb) && wb <= 8) {
+        const int WB = wb <= 2 ? 2 : wb <= 4 ? wb : 8;   // the kernel's instances
         HostSweep HL, HU;
-        build_sweep_cells(nb, M->h_bstart, rp, ci, dg, true, Wc, HL);
-        build_sweep_cells(nb, M->h_bstart, rp, ci, dg, false, Wc, HU);
+        build_sweep_cells(nb, M->h_bstart, rp, ci, dg, true, WB, es, HL);
+        build_sweep_cells(nb, M->h_bstart, rp, ci, dg, false, WB, es, HU);
         int maxlev = 0;
         for (int b = 0; b < nb; ++b)
             maxlev = std::max(maxlev, (HL.blk_lev[b + 1] - HL.blk_lev[b]) +
                                           (HU.blk_lev[b + 1] - HU.blk_lev[b]));
-        const size_t shm = (((size_t)(M->max_block_rows + 2) * es + 15) & ~(size_t)15) +
-                           (size_t)maxlev * sizeof(LevelMeta);
-        // 32-bit byte offsets into every sweep array
-        const size_t lim = (size_t)INT32_MAX / 8;
-        const bool addressable = HL.col.size() <= lim && HU.col.size() <= lim &&
-                                 HL.rows.size() + 1 <= lim && HU.rows.size() + 1 <= lim;
+        const size_t shm =
+            cell_xv_bytes(M->max_block_rows, es) + (size_t)maxlev * sizeof(LevelMeta);
+        // 32-bit byte offsets into the record arrays
+        const size_t lim = (size_t)INT32_MAX;
+        const bool addressable = HL.col.size() * 4 <= lim && HU.col.size() * 4 <= lim &&
+                                 HL.src.size() * es <= lim && HU.src.size() * es <= lim;
         if (HL.max_count <= 64 && HU.max_count <= 64 && addressable &&
             shm <= (size_t)lds_limit(M->ctx->device)) {
             upload_sweep(HL, M->L, s, es);
@@ -496,10 +537,10 @@ void prec_setup(rsgpu_prec *M, rsgpu_csr *A)
             M->cell = true;
         }
     }
-    const RangeEntries eL{rp.data(), dg.data(), 0, 0, ci.data()};
-    const RangeEntries eU{dg.data(), rp.data() + 1, 1, 0, ci.data()};
-    const int W = pad_width(std::max(max_entries(eL, n), max_entries(eU, n)));
     if (!M->cell) {
+        const RangeEntries eL{rp.data(), dg.data(), 0, 0, ci.data()};
+        const RangeEntries eU{dg.data(), rp.data() + 1, 1, 0, ci.data()};
+        const int W = pad_width(std::max(max_entries(eL, n), max_entries(eU, n)));
         HostSweep H;
         build_sweep(nb, M->h_bstart, eL, true, W, H);
         upload_sweep(H, M->L, s, es);
@@ -512,20 +553,14 @@ void prec_setup(rsgpu_prec *M, rsgpu_csr *A)
         M->lu64.alloc(std::max(M->nnz, 1));
         M->ell64_L.alloc(M->L.ell_size);
         M->ell64_U.alloc(M->U.ell_size);
-        M->dgU64.alloc(M->U.nrows_lev);
-        if (M->cell) {
-            M->in64_L.alloc(M->L.intra_src.n);
-            M->in64_U.alloc(M->U.intra_src.n);
-        }
+        if (!M->cell)
+            M->dgU64.alloc(M->U.nrows_lev);
     } else {
         M->lu32.alloc(std::max(M->nnz, 1));
         M->ell32_L.alloc(M->L.ell_size);
         M->ell32_U.alloc(M->U.ell_size);
-        M->dgU32.alloc(M->U.nrows_lev);
-        if (M->cell) {
-            M->in32_L.alloc(M->L.intra_src.n);
-            M->in32_U.alloc(M->U.intra_src.n);
-        }
+        if (!M->cell)
+            M->dgU32.alloc(M->U.nrows_lev);
     }
     const int elem = M->precision == RSGPU_FP64 ? 8 : 4;
     M->lds_rows = RS_LDS_BYTES / elem - 1;
@@ -1065,197 +1100,175 @@ __global__ void __launch_bounds__(64)
 }
 
 // ---------------------------------------------------------------- cell sweeps
-// One lane per cell (3 rows); a level's external operands of all three rows
-// are gathered at once, and the rows are finished in order with the
-// intra-cell terms taken from registers, in the reference's summation order
-// (L: external entries, then the strictly lower diagonal-block ones; U: the
-// strictly upper diagonal-block ones, then the external entries).
-struct PfSweep3 {
-    PfSweep S;
-    const void *intra;
-    int intra_bytes;
-};
-
-struct PfRsrc3 {
-    PfRsrc R;
-    __amdgpu_buffer_rsrc_t intra;
-};
-
-__device__ __forceinline__ PfRsrc3 pf_rsrc3(const PfSweep3 &S)
+// One lane per cell (3 rows).  A level's loads per lane are two records: the
+// IREC ints (LDS byte offsets of the cell and of its neighbour blocks) and
+// the VREC values (row a's 3*WB neighbour-block entries at [3*WB*a, ...),
+// the intra-cell coefficients at 9*WB, for U the diagonals at 9*WB+3).
+// Each neighbour block's three vector entries are gathered once for the
+// three rows.  Padding neighbour slots point at the three zero slots with
+// -0.0 values (s + (-0)*0 == s).
+template <int WB>
+constexpr int cell_irec()
 {
-    PfRsrc3 R;
-    R.R = pf_rsrc(S.S);
-    R.intra = __builtin_amdgcn_make_buffer_rsrc((void *)S.intra, (short)0, S.intra_bytes,
-                                                RS_BUF_WORD3);
-    return R;
+    return (1 + WB + 3) & ~3;
+}
+template <int WB, bool DIV>
+constexpr int cell_vrec()
+{
+    return (9 * WB + (DIV ? 6 : 3) + 3) & ~3;
 }
 
-template <typename T, int W>
+template <typename T, int WB, bool DIV>
 struct CellRegs {
-    int r;        // LDS byte offset of the cell's first row (inactive lanes: a real cell)
     bool act;
-    T d[3];       // U diagonals of the three rows
-    T in[3];      // intra-cell coefficients
-    VRow<T, W> v[3];
-    CRow<W> c[3];
+    int idx[cell_irec<WB>()];
+    T v[cell_vrec<WB, DIV>()];
 };
 
-template <typename T>
-__device__ __forceinline__ T buf_scalar(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+template <typename T, int WB, bool DIV>
+__device__ __forceinline__ void cell_load(const LevelMeta &m, bool valid, const PfRsrc &rs,
+                                          CellRegs<T, WB, DIV> &R)
 {
-    T out;
-    if (sizeof(T) == 8) {
-        const rs_v2i q = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
-        __builtin_memcpy(&out, &q, 8);
-    } else {
-        const int q = __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
-        __builtin_memcpy(&out, &q, 4);
-    }
-    return out;
-}
-
-template <typename T, int W, bool DIV>
-__device__ __forceinline__ void cell_load(const LevelMeta &m, bool valid, const PfRsrc3 &rs,
-                                          CellRegs<T, W> &R)
-{
+    constexpr int IR = cell_irec<WB>(), VR = cell_vrec<WB, DIV>();
+    constexpr int es = (int)sizeof(T);
     const int count = __builtin_amdgcn_readfirstlane(m.count);
     const int off = __builtin_amdgcn_readfirstlane(m.off);
-    const int roff = __builtin_amdgcn_readfirstlane(m.roff);
     const int t = threadIdx.x;
     const int tt = min(t, count - 1);
-    constexpr int es = (int)sizeof(T);
     R.act = valid && t < count;
-    R.r = __builtin_amdgcn_raw_buffer_load_b32(rs.R.rows, tt * 12, roff * 4, 0);
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        buf_load<W * 4>(&R.c[a], rs.R.cols, (3 * tt + a) * (W * 4), off * 4);
-        buf_load<W * es>(&R.v[a], rs.R.vals, (3 * tt + a) * (W * es), off * es);
-    }
-#pragma unroll
-    for (int k = 0; k < 3; ++k)
-        R.in[k] = buf_scalar<T>(rs.intra, (3 * tt + k) * es, roff * es);
-    if (DIV) {
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-            R.d[a] = buf_scalar<T>(rs.R.dgl, (3 * tt + a) * es, roff * es);
-    }
+    buf_load<IR * 4>(R.idx, rs.cols, tt * (IR * 4), off * (IR * 4));
+    buf_load<VR * es>(R.v, rs.vals, tt * (VR * es), off * (VR * es));
 }
 
-template <typename T, int W>
+template <typename T, int WB>
 struct CellGathered {
     T rhs[3];
-    T xs[3][W];
+    T xs[WB][3];
 };
 
-template <typename T, int W>
-__device__ __forceinline__ void cell_gather(const CellRegs<T, W> &R, unsigned char *xv,
-                                            CellGathered<T, W> &g)
+template <typename T, int WB, bool DIV>
+__device__ __forceinline__ void cell_gather(const CellRegs<T, WB, DIV> &R, unsigned char *xv,
+                                            CellGathered<T, WB> &g)
 {
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        g.rhs[a] = lds_at<T>(xv, R.r + a * (int)sizeof(T));
-#pragma unroll
-        for (int s = 0; s < W; ++s)
-            g.xs[a][s] = lds_at<T>(xv, R.c[a].c[s]);
-    }
-}
-
-template <typename T, int W, bool DIV>
-__device__ __forceinline__ void cell_finish(const CellRegs<T, W> &R, CellGathered<T, W> &g,
-                                            unsigned char *xv, int junkb)
-{
+    constexpr int es = (int)sizeof(T);
 #pragma unroll
     for (int a = 0; a < 3; ++a)
-        pin_gathers<T, W>(g.xs[a], g.rhs[a]);
+        g.rhs[a] = lds_at<T>(xv, R.idx[0] + a * es);
+#pragma unroll
+    for (int k = 0; k < WB; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            g.xs[k][c] = lds_at<T>(xv, R.idx[1 + k] + c * es);
+}
+
+// an empty asm statement with the three gathered values as in/out registers:
+// every gather is issued before the first of these, and they wait in order
+template <typename T>
+__device__ __forceinline__ void pin3(T (&x)[3])
+{
+    asm volatile("" : "+v"(x[0]), "+v"(x[1]), "+v"(x[2]));
+}
+
+template <typename T, int WB, bool DIV>
+__device__ __forceinline__ void cell_finish(const CellRegs<T, WB, DIV> &R,
+                                            CellGathered<T, WB> &g, unsigned char *xv,
+                                            int junkb)
+{
+    pin3<T>(g.rhs);
+#pragma unroll
+    for (int k = 0; k < WB; ++k)
+        pin3<T>(g.xs[k]);
     constexpr int es = (int)sizeof(T);
+    constexpr int iv = 9 * WB;
     T y[3];
     if (!DIV) {
-        // L: y_a = rhs_a - (external terms, then L(a,0) y_0, L(a,1) y_1)
+        // L: y_a = rhs_a - (neighbour entries, then L(a,0) y_0, L(a,1) y_1)
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             T sum = 0;
 #pragma unroll
-            for (int s = 0; s < W; ++s)
-                sum += R.v[a].v[s] * g.xs[a][s];
+            for (int k = 0; k < WB; ++k)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    sum += R.v[3 * WB * a + 3 * k + c] * g.xs[k][c];
             if (a == 1)
-                sum += R.in[0] * y[0];
+                sum += R.v[iv + 0] * y[0];
             if (a == 2) {
-                sum += R.in[1] * y[0];
-                sum += R.in[2] * y[1];
+                sum += R.v[iv + 1] * y[0];
+                sum += R.v[iv + 2] * y[1];
             }
             y[a] = g.rhs[a] - sum;
         }
     } else {
-        // U: rows 2, 1, 0; y_a = (rhs_a - (U(a,a+1..2) terms, then external)) / d_a
+        // U: rows 2, 1, 0; y_a = (rhs_a - (U(a,a+1..2) terms, then neighbour
+        // entries)) / d_a
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             const int a = 2 - q;
             T sum = 0;
             if (a == 1)
-                sum += R.in[2] * y[2];
+                sum += R.v[iv + 2] * y[2];
             if (a == 0) {
-                sum += R.in[0] * y[1];
-                sum += R.in[1] * y[2];
+                sum += R.v[iv + 0] * y[1];
+                sum += R.v[iv + 1] * y[2];
             }
 #pragma unroll
-            for (int s = 0; s < W; ++s)
-                sum += R.v[a].v[s] * g.xs[a][s];
-            y[a] = (g.rhs[a] - sum) / R.d[a];
+            for (int k = 0; k < WB; ++k)
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    sum += R.v[3 * WB * a + 3 * k + c] * g.xs[k][c];
+            y[a] = (g.rhs[a] - sum) / R.v[iv + 3 + a];
         }
     }
-    const int wr = R.act ? R.r : junkb;
+    const int wr = R.act ? R.idx[0] : junkb;
     const int st = R.act ? es : 0;
 #pragma unroll
     for (int a = 0; a < 3; ++a)
         lds_at<T>(xv, wr + a * st) = y[a];
 }
 
-template <typename T, int W, bool DIV>
-constexpr int cell_loads()
-{
-    return 1 + 3 * ((W * 4 + 15) / 16) + 3 * ((W * (int)sizeof(T) + 15) / 16) + 3 +
-           (DIV ? 3 : 0);
-}
-
-template <typename T, int W>
+// Prefetch depth of the cell pipeline: (depth-1) levels of loads in flight
+// within the wave's 63-load counter, and the register sets within ~216 VGPRs.
+template <typename T, int WB>
 constexpr int pf3_depth()
 {
-    constexpr int loads = cell_loads<T, W, true>();
-    constexpr int regs = 1 + 3 * (W + W * (int)sizeof(T) / 4) + 6 * (int)sizeof(T) / 4;
-    int d = 4;
-    while (d > 2 && (d * loads > 60 || d * regs > 210))
+    constexpr int es = (int)sizeof(T);
+    constexpr int loads = cell_irec<WB>() / 4 + cell_vrec<WB, true>() * es / 16;
+    constexpr int regs = 1 + cell_irec<WB>() + cell_vrec<WB, true>() * es / 4;
+    int d = 6;
+    while (d > 2 && ((d - 1) * loads > 60 || d * regs > 216))
         --d;
     return d;
 }
 
 // The sweep_pf pipeline with cell level entries.
-template <typename T, int W, bool DIV>
-__device__ __forceinline__ void sweep_pf3(int nl, const LevelMeta *sm, const PfRsrc3 &rs,
+template <typename T, int WB, bool DIV>
+__device__ __forceinline__ void sweep_pf3(int nl, const LevelMeta *sm, const PfRsrc &rs,
                                           unsigned char *xv, int junkb)
 {
-    constexpr int D = pf3_depth<T, W>();
+    constexpr int D = pf3_depth<T, WB>();
     if (nl <= 0)
         return;
     const int last = nl - 1;
-    CellRegs<T, W> r[D];
+    CellRegs<T, WB, DIV> r[D];
 #pragma unroll
     for (int k = 0; k < D - 1; ++k) {
-        cell_load<T, W, DIV>(sm[min(k, last)], k <= last, rs, r[k]);
+        cell_load<T, WB, DIV>(sm[min(k, last)], k <= last, rs, r[k]);
         __builtin_amdgcn_sched_barrier(0);
     }
     LevelMeta mq = sm[min(D - 1, last)];
     for (int l = 0; l < nl; l += D) {
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            CellGathered<T, W> g;
-            cell_gather<T, W>(r[k], xv, g);
+            CellGathered<T, WB> g;
+            cell_gather<T, WB, DIV>(r[k], xv, g);
             __builtin_amdgcn_sched_barrier(0);
             const int lev = l + k + D - 1;
-            cell_load<T, W, DIV>(mq, lev <= last, rs, r[(k + D - 1) % D]);
+            cell_load<T, WB, DIV>(mq, lev <= last, rs, r[(k + D - 1) % D]);
             mq = sm[min(lev + 1, last)];
             asm volatile("" ::: "memory");
             __builtin_amdgcn_sched_barrier(0);
-            cell_finish<T, W, DIV>(r[k], g, xv, junkb);
+            cell_finish<T, WB, DIV>(r[k], g, xv, junkb);
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             __builtin_amdgcn_sched_barrier(0);
         }
@@ -1263,9 +1276,10 @@ __device__ __forceinline__ void sweep_pf3(int nl, const LevelMeta *sm, const PfR
 }
 
 // One wave per block (at most 64 cells per level); L then U cell sweeps.
-template <typename T, int W>
+// LDS: the block's vector, three zero slots, one junk slot, level metadata.
+template <typename T, int WB>
 __global__ void __launch_bounds__(64)
-    k_ilu_apply_pf3(const int *__restrict__ bstart, PfSweep3 A, PfSweep3 B,
+    k_ilu_apply_pf3(const int *__restrict__ bstart, PfSweep A, PfSweep B,
                     const T *__restrict__ b, T *__restrict__ x, int xv_bytes,
                     const int *skip1, const int *skip2, BlkSkip bskip)
 {
@@ -1278,22 +1292,21 @@ __global__ void __launch_bounds__(64)
     const int b0 = bstart[blk], b1 = bstart[blk + 1], nb = b1 - b0;
     T *xv = (T *)rs_smem;
     LevelMeta *sm = (LevelMeta *)(rs_smem + xv_bytes);
-    const int la0 = A.S.blk[blk], nla = A.S.blk[blk + 1] - la0;
-    const int lb0 = B.S.blk[blk], nlb = B.S.blk[blk + 1] - lb0;
+    const int la0 = A.blk[blk], nla = A.blk[blk + 1] - la0;
+    const int lb0 = B.blk[blk], nlb = B.blk[blk + 1] - lb0;
     for (int q = threadIdx.x; q < nla; q += blockDim.x)
-        sm[q] = A.S.meta[la0 + q];
+        sm[q] = A.meta[la0 + q];
     for (int q = threadIdx.x; q < nlb; q += blockDim.x)
-        sm[nla + q] = B.S.meta[lb0 + q];
+        sm[nla + q] = B.meta[lb0 + q];
     for (int r = threadIdx.x; r < nb; r += blockDim.x)
         xv[r] = b[b0 + r];
-    if (threadIdx.x == 0)
-        xv[nb] = T(0);
+    if (threadIdx.x < 3)
+        xv[nb + threadIdx.x] = T(0);
     __syncthreads();
-    // three junk slots after the zero slot, for inactive lanes
-    const int junkb = (nb + 1) * (int)sizeof(T);
-    const PfRsrc3 ra = pf_rsrc3(A), rb = pf_rsrc3(B);
-    sweep_pf3<T, W, false>(nla, sm, ra, rs_smem, junkb);
-    sweep_pf3<T, W, true>(nlb, sm + nla, rb, rs_smem, junkb);
+    const int junkb = (nb + 3) * (int)sizeof(T);
+    const PfRsrc ra = pf_rsrc(A), rb = pf_rsrc(B);
+    sweep_pf3<T, WB, false>(nla, sm, ra, rs_smem, junkb);
+    sweep_pf3<T, WB, true>(nlb, sm + nla, rb, rs_smem, junkb);
     __syncthreads();
     for (int r = threadIdx.x; r < nb; r += blockDim.x)
         x[b0 + r] = xv[r];
@@ -1329,17 +1342,6 @@ static void launch_ell(hipStream_t s, const SweepData &D, const T *lu, T *ell, b
     RS_CHECK_LAUNCH();
 }
 
-// the intra-cell coefficients of a cell schedule
-template <typename T>
-static void launch_intra(hipStream_t s, const SweepData &D, const T *lu, T *dst)
-{
-    if (D.intra_src.n == 0)
-        return;
-    hipLaunchKernelGGL(k_gather_ell<T>, dim3((unsigned)((D.intra_src.n + 255) / 256)), dim3(256),
-                       0, s, D.intra_src.p, lu, dst, D.intra_src.n, T(0.0));
-    RS_CHECK_LAUNCH();
-}
-
 template <typename T>
 static void launch_rowdiag(hipStream_t s, const SweepData &D, const int *diag, const T *lu,
                            T *dgl)
@@ -1359,11 +1361,8 @@ int prec_factor(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
         launch_factor<double>(M, M->lu64.p, A->val.p);
         launch_ell<double>(s, M->L, M->lu64.p, M->ell64_L.p, true);
         launch_ell<double>(s, M->U, M->lu64.p, M->ell64_U.p, true);
-        launch_rowdiag<double>(s, M->U, M->diag.p, M->lu64.p, M->dgU64.p);
-        if (M->cell) {
-            launch_intra<double>(s, M->L, M->lu64.p, M->in64_L.p);
-            launch_intra<double>(s, M->U, M->lu64.p, M->in64_U.p);
-        }
+        if (!M->cell)
+            launch_rowdiag<double>(s, M->U, M->diag.p, M->lu64.p, M->dgU64.p);
         if (M->has_t) {
             launch_ell<double>(s, M->Ut, M->lu64.p, M->ell64_Ut.p, false);
             launch_ell<double>(s, M->Lt, M->lu64.p, M->ell64_Lt.p, false);
@@ -1373,11 +1372,8 @@ int prec_factor(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
         launch_factor<float>(M, M->lu32.p, A->val.p);
         launch_ell<float>(s, M->L, M->lu32.p, M->ell32_L.p, true);
         launch_ell<float>(s, M->U, M->lu32.p, M->ell32_U.p, true);
-        launch_rowdiag<float>(s, M->U, M->diag.p, M->lu32.p, M->dgU32.p);
-        if (M->cell) {
-            launch_intra<float>(s, M->L, M->lu32.p, M->in32_L.p);
-            launch_intra<float>(s, M->U, M->lu32.p, M->in32_U.p);
-        }
+        if (!M->cell)
+            launch_rowdiag<float>(s, M->U, M->diag.p, M->lu32.p, M->dgU32.p);
     }
     RS_HIP(hipMemcpyAsync(st, M->status.p, sizeof(st), hipMemcpyDeviceToHost, s));
     RS_HIP(hipStreamSynchronize(s));
@@ -1509,41 +1505,50 @@ static void launch_apply(rsgpu_prec *M, const SweepData &A_, const T *va, const 
 // Cell schedules (3x3 block matrices): the pipelined cell sweeps, forward L
 // then backward U; prec_setup only selects them when every level holds at
 // most 64 cells, the block fits LDS and the arrays are 32-bit addressable.
-template <typename T, int W>
-static void launch_pf3(rsgpu_prec *M, PfSweep3 A, PfSweep3 B, const T *b, T *x,
+template <typename T, int WB>
+static void launch_pf3(rsgpu_prec *M, const PfSweep &A, const PfSweep &B, const T *b, T *x,
                        size_t xv_bytes, size_t shm, const int *sk1, const int *sk2)
 {
     static size_t granted = 65536;
     if (shm > granted) {
-        RS_HIP(hipFuncSetAttribute((const void *)k_ilu_apply_pf3<T, W>,
+        RS_HIP(hipFuncSetAttribute((const void *)k_ilu_apply_pf3<T, WB>,
                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
         granted = shm;
     }
-    hipLaunchKernelGGL((k_ilu_apply_pf3<T, W>), dim3(M->nblocks), dim3(64), shm, M->ctx->stream,
-                       M->bstart.p, A, B, b, x, (int)xv_bytes, sk1, sk2, M->skip);
+    hipLaunchKernelGGL((k_ilu_apply_pf3<T, WB>), dim3(M->nblocks), dim3(64), shm,
+                       M->ctx->stream, M->bstart.p, A, B, b, x, (int)xv_bytes, sk1, sk2,
+                       M->skip);
+}
+
+static size_t cell_xv_bytes(int max_block_rows, int es)
+{
+    return ((size_t)(max_block_rows + 4) * es + 15) & ~(size_t)15;
 }
 
 template <typename T>
-static void launch_apply_cells(rsgpu_prec *M, const T *vl, const T *il, const T *vu,
-                               const T *iu, const T *du, const T *b, T *x, const int *sk1,
-                               const int *sk2)
+static void launch_apply_cells(rsgpu_prec *M, const T *vl, const T *vu, const T *b, T *x,
+                               const int *sk1, const int *sk2)
 {
     if (M->n == 0)
         return;
-    const size_t xv_bytes = ((size_t)(M->max_block_rows + 2) * sizeof(T) + 15) & ~(size_t)15;
+    const size_t xv_bytes = cell_xv_bytes(M->max_block_rows, (int)sizeof(T));
     const size_t shm =
         xv_bytes + (size_t)(M->L.max_levels + M->U.max_levels) * sizeof(LevelMeta);
-    const PfSweep3 A{pf_sweep<T>(M->L, vl, nullptr), il, (int)(M->L.intra_src.n * sizeof(T))};
-    const PfSweep3 B{pf_sweep<T>(M->U, vu, du), iu, (int)(M->U.intra_src.n * sizeof(T))};
-    const int W = M->L.W;
-    if (W <= 4)
+    const PfSweep A = pf_sweep<T>(M->L, vl, nullptr), B = pf_sweep<T>(M->U, vu, nullptr);
+    switch (M->L.W) {
+    case 2:
+        launch_pf3<T, 2>(M, A, B, b, x, xv_bytes, shm, sk1, sk2);
+        break;
+    case 3:
+        launch_pf3<T, 3>(M, A, B, b, x, xv_bytes, shm, sk1, sk2);
+        break;
+    case 4:
         launch_pf3<T, 4>(M, A, B, b, x, xv_bytes, shm, sk1, sk2);
-    else if (W <= 8)
+        break;
+    default:
         launch_pf3<T, 8>(M, A, B, b, x, xv_bytes, shm, sk1, sk2);
-    else if (W <= 12)
-        launch_pf3<T, 12>(M, A, B, b, x, xv_bytes, shm, sk1, sk2);
-    else
-        launch_pf3<T, 16>(M, A, B, b, x, xv_bytes, shm, sk1, sk2);
+        break;
+    }
     RS_CHECK_LAUNCH();
 }
 
@@ -1672,8 +1677,7 @@ void prec_apply(rsgpu_prec *M, const double *b, double *x, const DevState *st)
     }
     if (M->precision == RSGPU_FP64) {
         if (M->cell)
-            launch_apply_cells<double>(M, M->ell64_L.p, M->in64_L.p, M->ell64_U.p, M->in64_U.p,
-                                       M->dgU64.p, b, x, sk1, sk2);
+            launch_apply_cells<double>(M, M->ell64_L.p, M->ell64_U.p, b, x, sk1, sk2);
         else
             launch_apply<double, false>(M, M->L, M->ell64_L.p, nullptr, M->U, M->ell64_U.p,
                                         M->dgU64.p, b, x, M->scratch64.p, sk1, sk2);
@@ -1684,8 +1688,8 @@ void prec_apply(rsgpu_prec *M, const double *b, double *x, const DevState *st)
     M->tmpb32.alloc(M->n);
     convert_f64_to_f32(s, b, M->tmpa32.p, M->n);
     if (M->cell)
-        launch_apply_cells<float>(M, M->ell32_L.p, M->in32_L.p, M->ell32_U.p, M->in32_U.p,
-                                  M->dgU32.p, M->tmpa32.p, M->tmpb32.p, sk1, sk2);
+        launch_apply_cells<float>(M, M->ell32_L.p, M->ell32_U.p, M->tmpa32.p, M->tmpb32.p,
+                                  sk1, sk2);
     else
         launch_apply<float, false>(M, M->L, M->ell32_L.p, nullptr, M->U, M->ell32_U.p,
                                    M->dgU32.p, M->tmpa32.p, M->tmpb32.p, M->scratch32.p, sk1,
@@ -1716,8 +1720,7 @@ void prec_apply(rsgpu_prec *M, const float *b, float *x, const DevState *st)
         M->kind == RSGPU_PREC_SSOR)
         throw RsError{RSGPU_BAD_ARGUMENT, "FP32 apply needs an FP32 ILU or diagonal preconditioner"};
     if (M->cell)
-        launch_apply_cells<float>(M, M->ell32_L.p, M->in32_L.p, M->ell32_U.p, M->in32_U.p,
-                                  M->dgU32.p, b, x, sk1, sk2);
+        launch_apply_cells<float>(M, M->ell32_L.p, M->ell32_U.p, b, x, sk1, sk2);
     else
         launch_apply<float, false>(M, M->L, M->ell32_L.p, nullptr, M->U, M->ell32_U.p,
                                    M->dgU32.p, b, x, M->scratch32.p, sk1, sk2);

@@ -160,16 +160,33 @@ __device__ __forceinline__ void spmv_any(const int *wg_rows, const int *rowptr,
                                                     epi);
 }
 
+// SpMV with fused dot products.  Each workgroup computes a contiguous run
+// of plan groups (XCD-aware: the runs of one XCD are adjacent) and pays the
+// cross-workgroup reduction once, at the end: with one group per workgroup
+// the reduction's atomic round trip at every group's end held each
+// workgroup slot well beyond its loads.
 template <typename T, class Epi, int NR, bool B3>
 __global__ void __launch_bounds__(RS_SPMV_BS)
-    k_spmv_red(const int *__restrict__ wg_rows, const int *__restrict__ rowptr,
+    k_spmv_red(const int *__restrict__ wg_rows, int ngroups, const int *__restrict__ rowptr,
                const int *__restrict__ colind, BMat bm, const T *__restrict__ val,
                const T *__restrict__ x, Epi epi, DevState *st, RedSlot slot,
                int fin, int inner_step)
 {
     if (stopped(st, inner_step))
         return;
-    spmv_any<T, B3>(wg_rows, rowptr, colind, bm, val, x, epi);
+    const int G = (int)gridDim.x, w = xcd_remap((int)blockIdx.x, G);
+    const int per = ngroups / G, rem = ngroups % G;
+    const int g0 = w * per + min(w, rem), g1 = g0 + per + (w < rem ? 1 : 0);
+    for (int g = g0; g < g1; ++g) {
+        if (g > g0)
+            __syncthreads();   // the previous group's LDS reads are done
+        if (B3)
+            spmv_core_b3_g<T, RS_SPMV_BS, spmv_nnz<T>()>(g, wg_rows, rowptr, colind, bm.brp,
+                                                         bm.bcol, val, x, epi);
+        else
+            spmv_core_off_g<T, RS_SPMV_BS, spmv_nnz<T>()>(g, wg_rows, rowptr, colind, bm.coff,
+                                                          val, x, epi);
+    }
     double v[NR], tot[NR];
     v[0] = epi.a1;
     if (NR > 1)
@@ -967,19 +984,38 @@ BMat bmat_of(const rsgpu_csr *A)
     return BMat{A->brp.p, A->bcol.p, spmv_plan<T>(A).coff_ptr()};
 }
 
+// k_spmv_red's grid: the workgroups the device holds at once (occupancy x
+// CUs, queried once per instance), at most one per plan group
+template <typename T, class Epi, int NR, bool B3>
+static int red_grid(int ngroups)
+{
+    static int cap = 0;
+    if (cap == 0) {
+        int per_cu = 0, cus = 0, dev = 0;
+        RS_HIP(hipGetDevice(&dev));
+        RS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, (const void *)k_spmv_red<T, Epi, NR, B3>, RS_SPMV_BS, 0));
+        RS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        cap = std::max(1, per_cu) * std::max(1, cus);
+    }
+    return std::max(1, std::min(ngroups, cap));
+}
+
 template <typename T, class Epi, int NR>
 void launch_red(rsgpu_solver *S, rsgpu_csr *A, const T *x, Epi epi, DevState *st, int fin,
                 int inner)
 {
     hipStream_t s = S->ctx->stream;
     const SpmvPlan &pl = spmv_plan<T>(A);
-    const dim3 g(pl.nwg), b(RS_SPMV_BS);
+    const dim3 g(A->bs == 3 ? red_grid<T, Epi, NR, true>(pl.nwg)
+                            : red_grid<T, Epi, NR, false>(pl.nwg)),
+        b(RS_SPMV_BS);
     if (A->bs == 3)
-        hipLaunchKernelGGL((k_spmv_red<T, Epi, NR, true>), g, b, 0, s, pl.wg_rows.p,
+        hipLaunchKernelGGL((k_spmv_red<T, Epi, NR, true>), g, b, 0, s, pl.wg_rows.p, pl.nwg,
                            A->rowptr.p, A->colind.p, bmat_of<T>(A), csr_values<T>(A), x, epi, st,
                            slot_of(S), fin, inner);
     else
-        hipLaunchKernelGGL((k_spmv_red<T, Epi, NR, false>), g, b, 0, s, pl.wg_rows.p,
+        hipLaunchKernelGGL((k_spmv_red<T, Epi, NR, false>), g, b, 0, s, pl.wg_rows.p, pl.nwg,
                            A->rowptr.p, A->colind.p, bmat_of<T>(A), csr_values<T>(A), x, epi, st,
                            slot_of(S), fin, inner);
     RS_CHECK_LAUNCH();

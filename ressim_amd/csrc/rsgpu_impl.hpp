@@ -139,12 +139,14 @@ struct SweepData {
     int nrows_lev = 0;
     int max_count = 0, max_width = 0;
     // R = 3: a level lists cells (3x3 block rows) and one lane handles the
-    // cell's three rows in order; `rows`, `row_glob` and the U diagonals hold
-    // three entries per cell, the ELL three rows of W external slots, and
-    // intra_src the positions of the cell's three strictly lower (L sweep)
-    // or strictly upper (U sweep) diagonal-block entries
+    // cell's three rows in order.  `rows` holds three entries per cell (the
+    // factorisation); per cell, `colb` holds `irec` ints (the cell's LDS byte
+    // offset, then those of its W neighbour blocks in this triangle, the zero
+    // slots as padding) and `ell_src` `vrec` positions of the cell record
+    // (the three rows' neighbour-block entries, the three intra-cell
+    // coefficients, for U the three diagonals)
     int R = 1;
-    DevBuf<int> intra_src;
+    int irec = 0, vrec = 0;
 };
 
 struct AmgData;
@@ -192,11 +194,9 @@ struct rsgpu_prec {
     bool has_t = false;
     DevBuf<double> ell64_L, ell64_U, ell64_Lt, ell64_Ut, dgU64, dgUt64;
     DevBuf<float> ell32_L, ell32_U, dgU32;
-    // cell sweeps (3x3 block matrices): L and U are cell schedules and
-    // these hold their intra-cell coefficients
+    // cell sweeps (3x3 block matrices): L and U are cell schedules, their
+    // ELL arrays the cell records
     bool cell = false;
-    DevBuf<double> in64_L, in64_U;
-    DevBuf<float> in32_L, in32_U;
     DevBuf<double> scratch64;    // global fallback vectors for large blocks
     DevBuf<float> scratch32;
     DevBuf<double> tmp64;        // FP32 apply on FP64 vectors

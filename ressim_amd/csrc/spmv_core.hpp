@@ -37,16 +37,16 @@ inline const SpmvPlan &spmv_plan(const rsgpu_csr *A)
 
 // coff (may be null): per nonzero, its column minus the first row of its
 // workgroup, in 16 bits (SpmvPlan::coff); used instead of colind when given.
+// g: the plan group this workgroup computes.
 template <typename T, int BS, int NNZ_WG, class Epi>
-__device__ __forceinline__ void spmv_core_off(const int *__restrict__ wg_rows,
-                                              const int *__restrict__ rowptr,
-                                              const int *__restrict__ colind,
-                                              const short *__restrict__ coff,
-                                              const T *__restrict__ val,
-                                              const T *__restrict__ x, Epi &epi)
+__device__ __forceinline__ void spmv_core_off_g(int g, const int *__restrict__ wg_rows,
+                                                const int *__restrict__ rowptr,
+                                                const int *__restrict__ colind,
+                                                const short *__restrict__ coff,
+                                                const T *__restrict__ val,
+                                                const T *__restrict__ x, Epi &epi)
 {
     __shared__ T prod[NNZ_WG];
-    const int g = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     const int r0 = wg_rows[g], r1 = wg_rows[g + 1];
     const int j0 = rowptr[r0], j1 = rowptr[r1];
     if (j1 - j0 <= NNZ_WG) {
@@ -111,6 +111,19 @@ __device__ __forceinline__ void spmv_core_off(const int *__restrict__ wg_rows,
     }
 }
 
+// one group per workgroup, XCD-aware
+template <typename T, int BS, int NNZ_WG, class Epi>
+__device__ __forceinline__ void spmv_core_off(const int *__restrict__ wg_rows,
+                                              const int *__restrict__ rowptr,
+                                              const int *__restrict__ colind,
+                                              const short *__restrict__ coff,
+                                              const T *__restrict__ val,
+                                              const T *__restrict__ x, Epi &epi)
+{
+    spmv_core_off_g<T, BS, NNZ_WG>(xcd_remap((int)blockIdx.x, (int)gridDim.x), wg_rows,
+                                   rowptr, colind, coff, val, x, epi);
+}
+
 template <typename T, int BS, int NNZ_WG, class Epi>
 __device__ __forceinline__ void spmv_core(const int *__restrict__ wg_rows,
                                           const int *__restrict__ rowptr,
@@ -143,13 +156,13 @@ __device__ __forceinline__ void lds_only_barrier()
 }
 
 template <typename T, int BS, int NNZ_WG, class Epi>
-__device__ __forceinline__ void spmv_core_b3(const int *__restrict__ wg_rows,
-                                             const int *__restrict__ rowptr,
-                                             const int *__restrict__ colind,
-                                             const int *__restrict__ brp,
-                                             const int *__restrict__ bcol,
-                                             const T *__restrict__ val,
-                                             const T *__restrict__ x, Epi &epi)
+__device__ __forceinline__ void spmv_core_b3_g(int g, const int *__restrict__ wg_rows,
+                                               const int *__restrict__ rowptr,
+                                               const int *__restrict__ colind,
+                                               const int *__restrict__ brp,
+                                               const int *__restrict__ bcol,
+                                               const T *__restrict__ val,
+                                               const T *__restrict__ x, Epi &epi)
 {
     constexpr int MAXB = NNZ_WG / 9;          // blocks per stage
     constexpr int QB = (MAXB + BS - 1) / BS;  // blocks per lane
@@ -157,18 +170,19 @@ __device__ __forceinline__ void spmv_core_b3(const int *__restrict__ wg_rows,
     static_assert(BS / 3 <= 255, "local block rows are numbered in one byte");
     __shared__ int s_brp[BS / 3 + 2];
     __shared__ unsigned char s_brow[MAXB + 1];
-    const int g = xcd_remap((int)blockIdx.x, (int)gridDim.x);
     const int r0 = wg_rows[g], r1 = wg_rows[g + 1];
     if ((r0 % 3) || (r1 % 3) || r1 - r0 > BS) {
         // one scalar row of a block row longer than the stage (the plan
         // splits such block rows into their three scalar rows): CSR core
-        spmv_core<T, BS, NNZ_WG>(wg_rows, rowptr, colind, val, x, epi);
+        spmv_core_off_g<T, BS, NNZ_WG>(g, wg_rows, rowptr, colind, (const short *)nullptr, val,
+                                       x, epi);
         return;
     }
     const int I0 = r0 / 3, nI = (r1 - r0) / 3;
     const int B0 = brp[I0], nbk = brp[I0 + nI] - B0, nv = 9 * nbk;
     if (nbk > MAXB) {
-        spmv_core<T, BS, NNZ_WG>(wg_rows, rowptr, colind, val, x, epi);
+        spmv_core_off_g<T, BS, NNZ_WG>(g, wg_rows, rowptr, colind, (const short *)nullptr, val,
+                                       x, epi);
         return;
     }
     __shared__ T prod[NNZ_WG];
@@ -231,6 +245,19 @@ __device__ __forceinline__ void spmv_core_b3(const int *__restrict__ wg_rows,
             dot += prod[j];
         epi.row(r, dot);
     }
+}
+
+template <typename T, int BS, int NNZ_WG, class Epi>
+__device__ __forceinline__ void spmv_core_b3(const int *__restrict__ wg_rows,
+                                             const int *__restrict__ rowptr,
+                                             const int *__restrict__ colind,
+                                             const int *__restrict__ brp,
+                                             const int *__restrict__ bcol,
+                                             const T *__restrict__ val,
+                                             const T *__restrict__ x, Epi &epi)
+{
+    spmv_core_b3_g<T, BS, NNZ_WG>(xcd_remap((int)blockIdx.x, (int)gridDim.x), wg_rows, rowptr,
+                                  colind, brp, bcol, val, x, epi);
 }
 
 // y = A x with CompRowMatrix.mult's rule: y[i] = dot only if dot != 0.
