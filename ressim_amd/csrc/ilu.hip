@@ -40,6 +40,11 @@ struct LevelMeta {
     int count, off, roff, pad;
 };
 
+// cell factorisation records: at most RS_CELL_NB blocks per block row
+#define RS_CELL_NB 8
+#define RS_CREC (4 + 4 * RS_CELL_NB)
+#define RS_CQ (3 * RS_CELL_NB)
+
 // ---------------------------------------------------------------- host setup
 struct HostSweep {
     int W = 4;
@@ -390,6 +395,48 @@ static void build_sweep_cells(int nblocks, const std::vector<int> &bstart,
     H.row_glob.push_back(0);
 }
 
+// Records of the cell factorisation (k_ilu_factor_cells); false when a
+// block row has more than RS_CELL_NB blocks.
+static bool build_cell_records(int n, const std::vector<int> &rp, const std::vector<int> &ci,
+                               const std::vector<int> &dg, std::vector<int> &rec)
+{
+    const int ncell = n / 3;
+    rec.assign((size_t)ncell * RS_CREC, 0);
+    std::vector<int> bcI;
+    for (int I = 0; I < ncell; ++I) {
+        const int r = 3 * I, nb = (rp[r + 1] - rp[r]) / 3, dI = (dg[r] - rp[r]) / 3;
+        if (nb > RS_CELL_NB)
+            return false;
+        int *R = rec.data() + (size_t)RS_CREC * I;
+        R[0] = rp[r];
+        R[1] = nb;
+        R[2] = dI;
+        bcI.resize(nb);
+        for (int q = 0; q < nb; ++q)
+            bcI[q] = ci[rp[r] + 3 * q] / 3;
+        for (int q = 0; q < dI; ++q) {
+            const int J = bcI[q], rJ = 3 * J;
+            const int nbJ = (rp[rJ + 1] - rp[rJ]) / 3, dJ = (dg[rJ] - rp[rJ]) / 3;
+            int nm = 0;
+            unsigned code[2] = {0, 0};
+            for (int sJ = dJ + 1; sJ < nbJ; ++sJ) {
+                const int K = ci[rp[rJ] + 3 * sJ] / 3;
+                const auto it = std::lower_bound(bcI.begin(), bcI.end(), K);
+                if (it == bcI.end() || *it != K)
+                    continue;
+                const int sI = (int)(it - bcI.begin());
+                code[nm >> 2] |= (unsigned)((sI << 4) | sJ) << (8 * (nm & 3));
+                ++nm;
+            }
+            R[4 + 4 * q] = rp[rJ];
+            R[5 + 4 * q] = (nbJ << 16) | (dJ << 8) | nm;
+            R[6 + 4 * q] = (int)code[0];
+            R[7 + 4 * q] = (int)code[1];
+        }
+    }
+    return true;
+}
+
 // MTJ's factor loop reads colind[rowptr[k+1]] when row `index` has a column
 // beyond row k's last one (ILU.java:113-120) and updates the next row's
 // first entry when that column matches.  That cannot happen when the pattern
@@ -537,6 +584,17 @@ void prec_setup(rsgpu_prec *M, rsgpu_csr *A)
             M->cell = true;
         }
     }
+    M->cell_factor = false;
+    if (M->cell && getenv("RSGPU_ILU_CELL_FACTOR") == nullptr) {
+        std::vector<int> rec;
+        if (build_cell_records(n, rp, ci, dg, rec)) {
+            M->cell_rec.upload(rec, s);
+            RS_HIP(hipStreamSynchronize(s));
+            M->cell_factor = true;
+        }
+    }
+    if (!M->cell_factor)
+        M->cell_rec.release();
     if (!M->cell) {
         const RangeEntries eL{rp.data(), dg.data(), 0, 0, ci.data()};
         const RangeEntries eU{dg.data(), rp.data() + 1, 1, 0, ci.data()};
@@ -688,6 +746,140 @@ __global__ void __launch_bounds__(256)
             }
         __syncthreads();
     }
+}
+
+// Cell factorisation (3x3 block matrices with a cell schedule).  The IKJ
+// sequence of ILU.java:99-122 for row k = 3I+a visits its L entries in
+// column order: the entries of each neighbour block J < I (columns 3J+c,
+// c = 0..2), then the diagonal block's (3I+c, c < a).  Entry (k, 3J+c) is
+// divided by the pivot of row 3J+c and then updates row k at the columns of
+// row 3J+c's U part that row k has: block J's columns 3J+c' (c' > c), then
+// every block K > J of row J that is also a block of row I.  With whole
+// blocks these positions follow from the block slots, so the lane of cell I
+// needs no column search: its three rows sit in LDS, the host lists per
+// neighbour block J the slot pairs (slot of K in row I, slot in row J), and
+// each J's pivots and U entries are loaded once for the three rows.  Rows
+// are independent until the diagonal block, so the lane runs every row's
+// external steps first (each row's own order is unchanged: J ascending,
+// then c), then the diagonal-block steps of rows 1 and 2, which read rows 0
+// and 1 after they are final.  Per target the updates arrive in the
+// reference's order, so the factors are bit-identical.
+//
+// Cell record (ints, RS_CREC): [rp(3I), nb(I), slot of I, -] then per
+// neighbour slot s < slot of I: [rp(3J), nb(J) << 16 | slot of J in row J
+// << 8 | matches, 4 match bytes, 4 match bytes], a match byte = slot in row
+// I << 4 | slot in row J, in ascending slot of row J.
+
+template <typename T>
+__global__ void __launch_bounds__(64)
+    k_ilu_factor_cells(const int *__restrict__ bstart, const int *__restrict__ blk_lev,
+                       const LevelMeta *__restrict__ meta, const int *__restrict__ rows,
+                       const int *__restrict__ crec, T *lu, int *status)
+{
+    __shared__ T own[3 * RS_CQ * 64];
+    const int t = threadIdx.x;
+#define RS_OWN(a, q) own[((a) * RS_CQ + (q)) * 64 + t]
+    const int blk = blockIdx.x;
+    const int b0 = bstart[blk];
+    for (int l = blk_lev[blk]; l < blk_lev[blk + 1]; ++l) {
+        const LevelMeta m = meta[l];
+        if (t < m.count) {
+            const int I = (b0 + rows[m.roff + 3 * t]) / 3;
+            const int *rec = crec + (size_t)RS_CREC * I;
+            const int rp3 = rec[0], nb = rec[1], dI = rec[2], n3 = 3 * nb;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int q = 0; q < RS_CQ; ++q)
+                    if (q < n3)
+                        RS_OWN(a, q) = lu[rp3 + a * n3 + q];
+            bool bad[3] = {false, false, false};
+            for (int s = 0; s < dI; ++s) {
+                const int rpJ = rec[4 + 4 * s], w = rec[5 + 4 * s];
+                const int n3J = 3 * (w >> 16), dJ = (w >> 8) & 255, nm = w & 255;
+                const int dpos = rpJ + 3 * dJ;
+                // pivots and block J's U entries of rows 3J, 3J+1
+                T piv[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    piv[c] = lu[dpos + c * n3J + c];
+                const T u01 = lu[dpos + 1], u02 = lu[dpos + 2], u12 = lu[dpos + n3J + 2];
+                T Lv[3][3];
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        Lv[a][c] = T(0);
+                        if (bad[a])
+                            continue;
+                        const int iq = 3 * s + c;
+                        if (piv[c] == T(0)) {
+                            atomicMin(&status[1], rp3 + a * n3 + iq);
+                            atomicMax(&status[0], (int)RSGPU_ZERO_PIVOT);
+                            bad[a] = true;
+                            continue;
+                        }
+                        const T L = RS_OWN(a, iq) / piv[c];
+                        RS_OWN(a, iq) = L;
+                        Lv[a][c] = L;
+                        if (c == 0) {
+                            RS_OWN(a, iq + 1) -= L * u01;
+                            RS_OWN(a, iq + 2) -= L * u02;
+                        } else if (c == 1) {
+                            RS_OWN(a, iq + 1) -= L * u12;
+                        }
+                    }
+                }
+                // blocks K > J of row J that row I has
+                for (int mi = 0; mi < nm; ++mi) {
+                    const int code = (rec[6 + 4 * s + (mi >> 2)] >> (8 * (mi & 3))) & 255;
+                    const int sI = code >> 4, sJ = code & 15;
+                    T src[3][3];
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+#pragma unroll
+                        for (int e = 0; e < 3; ++e)
+                            src[c][e] = lu[rpJ + c * n3J + 3 * sJ + e];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        if (bad[a])
+                            continue;
+#pragma unroll
+                        for (int c = 0; c < 3; ++c)
+#pragma unroll
+                            for (int e = 0; e < 3; ++e)
+                                RS_OWN(a, 3 * sI + e) -= Lv[a][c] * src[c][e];
+                    }
+                }
+            }
+            // the diagonal block: rows 1 and 2 against the final rows above
+#pragma unroll
+            for (int a = 1; a < 3; ++a) {
+                for (int c = 0; c < a && !bad[a]; ++c) {
+                    const int iq = 3 * dI + c;
+                    const T pv = RS_OWN(c, iq);
+                    if (pv == T(0)) {
+                        atomicMin(&status[1], rp3 + a * n3 + iq);
+                        atomicMax(&status[0], (int)RSGPU_ZERO_PIVOT);
+                        bad[a] = true;
+                        break;
+                    }
+                    const T L = RS_OWN(a, iq) / pv;
+                    RS_OWN(a, iq) = L;
+                    for (int q = iq + 1; q < n3; ++q)
+                        RS_OWN(a, q) -= L * RS_OWN(c, q);
+                }
+            }
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int q = 0; q < RS_CQ; ++q)
+                    if (q < n3)
+                        lu[rp3 + a * n3 + q] = RS_OWN(a, q);
+        }
+        __syncthreads();
+    }
+#undef RS_OWN
 }
 
 // Padding slots hold the identity of the sweep's update: -0.0 for the
@@ -1325,6 +1517,13 @@ static void launch_factor(rsgpu_prec *M, T *lu, const double *aval)
     }
     int init[2] = {0, 0x7fffffff};
     RS_HIP(hipMemcpyAsync(M->status.p, init, sizeof(init), hipMemcpyHostToDevice, s));
+    if (M->cell_factor) {
+        hipLaunchKernelGGL(k_ilu_factor_cells<T>, dim3(M->nblocks), dim3(64), 0, s,
+                           M->bstart.p, M->L.blk_lev.p, (const LevelMeta *)M->L.lev_meta.p,
+                           M->L.rows.p, M->cell_rec.p, lu, M->status.p);
+        RS_CHECK_LAUNCH();
+        return;
+    }
     hipLaunchKernelGGL(k_ilu_factor<T>, dim3(M->nblocks), dim3(M->L.max_count > 64 ? 256 : 64),
                        0, s, M->bstart.p, M->L.blk_lev.p, (const LevelMeta *)M->L.lev_meta.p,
                        M->L.rows.p, M->rowptr.p, M->colind.p, M->diag.p, lu, M->status.p,

@@ -195,8 +195,10 @@ struct rsgpu_prec {
     DevBuf<double> ell64_L, ell64_U, ell64_Lt, ell64_Ut, dgU64, dgUt64;
     DevBuf<float> ell32_L, ell32_U, dgU32;
     // cell sweeps (3x3 block matrices): L and U are cell schedules, their
-    // ELL arrays the cell records
-    bool cell = false;
+    // ELL arrays the cell records; cell_factor: the block-slot factorisation
+    // with its per-cell records
+    bool cell = false, cell_factor = false;
+    DevBuf<int> cell_rec;
     DevBuf<double> scratch64;    // global fallback vectors for large blocks
     DevBuf<float> scratch32;
     DevBuf<double> tmp64;        // FP32 apply on FP64 vectors

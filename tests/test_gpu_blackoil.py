@@ -268,3 +268,22 @@ def test_cell_ilu_transapply(gpu, oracle):
     xt = torch.zeros(J.n, dtype=torch.float64, device="cuda")
     M.transApply(torch.tensor(b, device="cuda"), xt)
     assert np.array_equal(xt.cpu().numpy(), oracle.ilu_transapply(frp, fci, lu, dg, b))
+
+
+@pytest.mark.parametrize("factor", ["cells", "rows"])
+def test_cell_ilu_zero_pivot(gpu, monkeypatch, factor):
+    """A zero diagonal in the first cell: both factorisations of the cell
+    schedule stop with "Zero pivot" at the first entry the reference divides
+    by it, (row 1, column 0)."""
+    if factor == "rows":
+        monkeypatch.setenv("RSGPU_ILU_CELL_FACTOR", "0")
+    print(f"cell ILU zero pivot, factor {factor}", flush=True)
+    pr = P.config3(nx=5, ny=4, nz=2)
+    bo, J, R = _device_jacobian(gpu, pr)
+    rp, ci, val = J.getRowPointers(), J.getColumnIndices(), J.getData().copy()
+    val[rp[0] + list(ci[rp[0]:rp[1]]).index(0)] = 0.0
+    J.set_data(val)
+    M = gpu.ILU(J)
+    with pytest.raises(RuntimeError, match="Zero pivot"):
+        M.setMatrix(J)
+    assert M.bad_row == rp[1] + list(ci[rp[1]:rp[2]]).index(0)
