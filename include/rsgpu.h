@@ -407,6 +407,77 @@ int rsgpu_flux_apply(rsgpu_flux *F, const double *p_dev, double *flux_dev);
 int rsgpu_flux_rates(rsgpu_flux *F, const double *flux_dev, int ncells_sel,
                      const int32_t *cells_host, double *rates_host);
 
+/* ---- IMPES pressure Jacobian (rs numerics/Discretisation.java, field/CS.java)
+ * The step before the solve in rs's pressure Newton loop
+ * (Discretisation.java:284-289): control-surface secondaries (CS.java:94-170),
+ * then assemblePressureJacobianFlux (Discretisation.java:459-561) and
+ * assemblePressureJacobianDiagonal with its source terms (:365-453), values
+ * added into A's pattern (LinearSolver.addToJacobian / addToResidual,
+ * LinearSolver.java:226-235) in the reference's call order, so A and b are
+ * bit-identical to the Java loops' (A and b start from zero, as
+ * LinearSolver.solve leaves them, :287-288).  The control-volume
+ * secondaries (flash, PVT, rock-fluid) are inputs.  Phases are water, oil,
+ * gas (Phase.all()); per-phase arrays are phase-major [3][n]. */
+typedef struct rsgpu_impes rsgpu_impes;
+
+typedef struct rsgpu_impes_params {
+    int thermal;     /* 1: convective heat and Fourier terms */
+    int pad;
+    double dt;       /* time step */
+    double g;        /* gravity acceleration (CS.bodyForce) */
+    double omega;    /* centrifuge rotational speed */
+    double ax, ay;   /* centrifuge axis */
+} rsgpu_impes_params;
+
+/* Device pointers (double unless noted), one element per cell. */
+typedef struct rsgpu_impes_state {
+    const double *pressure;        /* [n] CV.getPressure */
+    const double *phase_pressure;  /* [3][n] CV.getPhasePressure */
+    const double *saturation;      /* [3][n] CV.getSaturation */
+    const double *mass_density;    /* [3][n] CV.getMassDensity */
+    const double *phase_mobility;  /* [3][n] CV.getPhaseMobility */
+    const uint8_t *present;        /* [3][n] CV.isPhasePresent */
+    const double *comp_mobility;   /* [3][ncomp][n] CV.getComponentMobility */
+    const double *dRdN;            /* [ncomp][n] getResidualVolumeDerivativeMolarMass */
+    const double *dRdp;            /* [n] getResidualVolumeDerivativePressure */
+    const double *heat_capacity;   /* [n] thermal only */
+    const double *dRdT;            /* [n] thermal only */
+    const double *energy_mobility; /* [3][n] thermal only */
+    const double *temperature;     /* [n] thermal only */
+    const double *p0, *R0;         /* pressure and residual volume stored at the
+                                      step start (storeCurrentState) */
+    const double *cx, *cy, *cz;    /* element centres (CS.bodyForce) */
+} rsgpu_impes_state;
+
+/* F: the mesh's MD lists and connections (rsgpu_flux_create or
+ * rsgpu_mpfa_o_flux); Fmf: the Fourier (MF) lists of the same connections,
+ * required for thermal assembly, may be NULL otherwise. */
+int rsgpu_impes_create(rsgpu_flux *F, const rsgpu_flux *Fmf, int ncomp,
+                       rsgpu_impes **out);
+int rsgpu_impes_destroy(rsgpu_impes *E);
+/* Source terms in the order rs applies them (field.sources(), then each
+ * source's elements): kind 0 = regular (q[t][0][nu] is the component rate,
+ * Discretisation.java:405-424), kind 1 = outlet (q[t][phase][nu], summed over
+ * phases, :429-452); qe[t] the energy rate (thermal).  Host arrays; nterms 0
+ * clears them. */
+int rsgpu_impes_set_sources(rsgpu_impes *E, int nterms, const int32_t *elem_host,
+                            const int32_t *kind_host, const double *q_host,
+                            const double *qe_host);
+/* CS.calculateSecondaries for every connection: rho[3][nconn] (saturation-
+ * weighted density, 0 where it is NaN), upstream[3][nconn] (here when the
+ * phase's potential sum is positive, else there), darcy[3][nconn]
+ * (upstream mobility times that sum).  Device arrays. */
+int rsgpu_impes_control_surfaces(rsgpu_impes *E, const rsgpu_impes_params *P,
+                                 const rsgpu_impes_state *S, double *rho_dev,
+                                 int32_t *upstream_dev, double *darcy_dev);
+/* A's values (pattern of LinearSolver.determineSparsityPattern) and b: the
+ * flux part, then the diagonal part and the source terms.  Returns
+ * RSGPU_NOT_IN_PATTERN when an entry is missing from A's pattern (A's
+ * values are then unspecified). */
+int rsgpu_impes_assemble(rsgpu_impes *E, rsgpu_csr *A, const rsgpu_impes_params *P,
+                         const rsgpu_impes_state *S, const double *rho_dev,
+                         const int32_t *upstream_dev, double *b_dev);
+
 /* ---- Adjoint (no reference counterpart: rs has no gradients) ------------ */
 /* tbar[q] = (w[here(c)] - w[there(c)]) * p[md_elem[q]]: gradient of
  * sum_i w_i * (net outflow of cell i) with respect to the MD

@@ -23,6 +23,8 @@
  *     QuadCell,BilinearMap,TrilinearMap,StructuredGeometry2D,
  *     StructuredGeometry3D}.java
  *   rs/src/no/uib/cipr/rs/numerics/{LinearSolver,PotentialCalculator}.java
+ *   rs/src/no/uib/cipr/rs/numerics/Discretisation.java, rs/.../field/CS.java
+ *     (impes.c: the IMPES pressure Jacobian)
  *
  * Dense LAPACK/BLAS calls of the reference (dgesv, dtrtrs, dgemm) are
  * restated from the published reference-BLAS / LAPACK 3.0 algorithms
@@ -241,6 +243,43 @@ void or_bo_assemble(const or_bo_params *P, int nx, int ny, int nz, const double 
                     const double *u, const double *u_old, double *val, double *R);
 int or_bo_expand(int nb, const int *browptr, const int *bcolind, int *rowptr,
                  int *colind);
+
+/* ---- IMPES pressure Jacobian (impes.c; rs numerics/Discretisation.java,
+ * field/CS.java) ---------------------------------------------------------- */
+/* Field names and layouts identical to rsgpu_impes_params / _state. */
+typedef struct or_impes_params {
+    int n, nconn, ncomp, thermal;
+    double dt, g, omega, ax, ay;
+} or_impes_params;
+
+typedef struct or_impes_state {
+    const double *pressure;        /* [n] CV.getPressure */
+    const double *phase_pressure;  /* [3][n] water, oil, gas */
+    const double *saturation;      /* [3][n] */
+    const double *mass_density;    /* [3][n] */
+    const double *phase_mobility;  /* [3][n] */
+    const uint8_t *present;        /* [3][n] */
+    const double *comp_mobility;   /* [3][ncomp][n] */
+    const double *dRdN;            /* [ncomp][n] */
+    const double *dRdp;            /* [n] */
+    const double *heat_capacity;   /* [n] thermal only */
+    const double *dRdT;            /* [n] thermal only */
+    const double *energy_mobility; /* [3][n] thermal only */
+    const double *temperature;     /* [n] thermal only */
+    const double *p0, *R0;         /* pressure and residual volume stored at the step start */
+    const double *cx, *cy, *cz;    /* element centres */
+} or_impes_state;
+
+void or_impes_cs(const or_impes_params *P, const or_impes_state *S, const int *here,
+                 const int *there, const int *md_ptr, const int *md_elem, const double *md_t,
+                 double *rho, int *upstream, double *darcy);
+int or_impes_assemble(const or_impes_params *P, const or_impes_state *S, const int *here,
+                      const int *there, const int *md_ptr, const int *md_elem,
+                      const double *md_t, const int *mf_ptr, const int *mf_elem,
+                      const double *mf_t, const double *rho, const int *upstream, int nterms,
+                      const int *src_elem, const int *src_kind, const double *src_q,
+                      const double *src_qe, const int *rowptr, const int *colind, double *val,
+                      double *b);
 
 #ifdef __cplusplus
 }

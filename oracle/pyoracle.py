@@ -63,6 +63,22 @@ class BoParams(C.Structure):
         "kw", "ko", "kg")] + [(f, C.c_int) for f in ("nw", "no", "ng", "pad")]
 
 
+IMPES_FIELDS = ("pressure", "phase_pressure", "saturation", "mass_density", "phase_mobility",
+                "present", "comp_mobility", "dRdN", "dRdp", "heat_capacity", "dRdT",
+                "energy_mobility", "temperature", "p0", "R0", "cx", "cy", "cz")
+
+
+class ImpesParams(C.Structure):
+    """or_impes_params (oracle.h)."""
+    _fields_ = [(f, C.c_int) for f in ("n", "nconn", "ncomp", "thermal")] + \
+               [(f, C.c_double) for f in ("dt", "g", "omega", "ax", "ay")]
+
+
+class ImpesState(C.Structure):
+    """or_impes_state (oracle.h): one pointer per field of IMPES_FIELDS."""
+    _fields_ = [(f, C.c_void_p) for f in IMPES_FIELDS]
+
+
 def build() -> str:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
     return LIB_PATH
@@ -148,6 +164,14 @@ def lib():
                                      _f64p, _f64p, C.c_void_p, C.c_void_p]
         L.or_bo_assemble.restype = None
         L.or_bo_expand.argtypes = [C.c_int, _i32p, _i32p, _i32p, _i32p]
+        L.or_impes_cs.argtypes = [C.POINTER(ImpesParams), C.POINTER(ImpesState), _i32p, _i32p,
+                                  _i32p, _i32p, _f64p, _f64p, _i32p, _f64p]
+        L.or_impes_cs.restype = None
+        L.or_impes_assemble.argtypes = [C.POINTER(ImpesParams), C.POINTER(ImpesState), _i32p,
+                                        _i32p, _i32p, _i32p, _f64p, C.c_void_p, C.c_void_p,
+                                        C.c_void_p, _f64p, _i32p, C.c_int, C.c_void_p,
+                                        C.c_void_p, C.c_void_p, C.c_void_p, _i32p, _i32p,
+                                        _f64p, _f64p]
         _lib = L
     return _lib
 
@@ -567,3 +591,65 @@ def bo_assemble(params: dict, nx, ny, nz, kh, u, u_old, need_jac=True):
     lib().or_bo_assemble(C.byref(P), nx, ny, nz, _f64(kh), _f64(u), _f64(u_old),
                          val.ctypes.data if need_jac else None, R.ctypes.data)
     return val, R
+
+
+# ---------------------------------------------------------------- IMPES
+def _impes_structs(params: dict, state: dict):
+    """(ImpesParams, ImpesState, keep-alive list) from a parameter dict (n,
+    nconn, ncomp, thermal, dt, g, omega, ax, ay) and a dict of numpy arrays
+    keyed by IMPES_FIELDS (thermal fields may be absent)."""
+    P = ImpesParams()
+    for k, v in params.items():
+        setattr(P, k, v)
+    S = ImpesState()
+    keep = []
+    for f in IMPES_FIELDS:
+        a = state.get(f)
+        if a is None:
+            continue
+        a = np.ascontiguousarray(a, dtype=np.uint8 if f == "present" else np.float64)
+        keep.append(a)
+        setattr(S, f, a.ctypes.data)
+    return P, S, keep
+
+
+def impes_cs(params, state, here, there, md_ptr, md_elem, md_t):
+    """Control-surface secondaries: (rho [3][nconn], upstream [3][nconn],
+    darcy [3][nconn])."""
+    P, S, keep = _impes_structs(params, state)
+    nc = params["nconn"]
+    rho = np.zeros(3 * nc)
+    up = np.zeros(3 * nc, np.int32)
+    u = np.zeros(3 * nc)
+    lib().or_impes_cs(C.byref(P), C.byref(S), _i32(here), _i32(there), _i32(md_ptr),
+                      _i32(md_elem), _f64(md_t), rho, up, u)
+    return rho, up, u
+
+
+def impes_assemble(params, state, here, there, md_ptr, md_elem, md_t, rho, upstream,
+                   rowptr, colind, mf=None, sources=None):
+    """(status, values on the (rowptr, colind) pattern, residual b).  mf =
+    (mf_ptr, mf_elem, mf_t) for the thermal Fourier term; sources = (elem,
+    kind, q [nterms][3][ncomp], qe) in application order."""
+    P, S, keep = _impes_structs(params, state)
+    n = params["n"]
+    val = np.zeros(int(rowptr[n]))
+    b = np.zeros(n)
+    if mf is not None:
+        mfp, mfe, mft = _i32(mf[0]), _i32(mf[1]), _f64(mf[2])
+        keep += [mfp, mfe, mft]
+        mfa = (mfp.ctypes.data, mfe.ctypes.data, mft.ctypes.data)
+    else:
+        mfa = (None, None, None)
+    if sources is not None:
+        se, sk, sq, sqe = (_i32(sources[0]), _i32(sources[1]), _f64(sources[2]),
+                           _f64(sources[3]))
+        keep += [se, sk, sq, sqe]
+        nt = len(se)
+        sa = (se.ctypes.data, sk.ctypes.data, sq.ctypes.data, sqe.ctypes.data)
+    else:
+        nt, sa = 0, (None, None, None, None)
+    st = lib().or_impes_assemble(C.byref(P), C.byref(S), _i32(here), _i32(there),
+                                 _i32(md_ptr), _i32(md_elem), _f64(md_t), *mfa, _f64(rho),
+                                 _i32(upstream), nt, *sa, _i32(rowptr), _i32(colind), val, b)
+    return st, val, b

@@ -54,6 +54,22 @@ class BlackOilParams(C.Structure):
         "kw", "ko", "kg")] + [(f, C.c_int) for f in ("nw", "no", "ng", "pad")]
 
 
+IMPES_FIELDS = ("pressure", "phase_pressure", "saturation", "mass_density", "phase_mobility",
+                "present", "comp_mobility", "dRdN", "dRdp", "heat_capacity", "dRdT",
+                "energy_mobility", "temperature", "p0", "R0", "cx", "cy", "cz")
+
+
+class ImpesParams(C.Structure):
+    """rsgpu_impes_params (include/rsgpu.h)."""
+    _fields_ = [("thermal", C.c_int), ("pad", C.c_int)] + \
+               [(f, C.c_double) for f in ("dt", "g", "omega", "ax", "ay")]
+
+
+class ImpesState(C.Structure):
+    """rsgpu_impes_state (include/rsgpu.h): device addresses, in field order."""
+    _fields_ = [(f, C.c_void_p) for f in IMPES_FIELDS]
+
+
 _vp = C.c_void_p
 _ip = C.c_void_p      # int32 pointers passed as addresses
 _dp = C.c_void_p
@@ -144,6 +160,13 @@ SIGNATURES = {
     "rsgpu_flux_vjp": (C.c_int, [_h, _dp, _dp, _dp]),
     "rsgpu_mpfa_o_vjp": (C.c_int, [_h, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                    C.c_double, C.c_double, _dp, C.c_double, _h, _dp, _dp]),
+    "rsgpu_impes_create": (C.c_int, [_h, _h, C.c_int, _hp]),
+    "rsgpu_impes_destroy": (C.c_int, [_h]),
+    "rsgpu_impes_set_sources": (C.c_int, [_h, C.c_int, _ip, _ip, _dp, _dp]),
+    "rsgpu_impes_control_surfaces": (C.c_int, [_h, C.POINTER(ImpesParams),
+                                               C.POINTER(ImpesState), _dp, _ip, _dp]),
+    "rsgpu_impes_assemble": (C.c_int, [_h, _h, C.POINTER(ImpesParams), C.POINTER(ImpesState),
+                                       _dp, _ip, _dp]),
 }
 
 _lib = None

@@ -13,17 +13,6 @@
 
 #include "spmv_core.hpp"
 
-struct rsgpu_flux {
-    rsgpu_ctx *ctx = nullptr;
-    int nconn = 0, ncells = 0;
-    DevBuf<int> md_ptr, md_elem;
-    DevBuf<double> md_t;
-    SpmvPlan plan;
-    DevBuf<int> cell_ptr, cell_conn;   // connections of each cell, ascending,
-    DevBuf<signed char> cell_sign;     // +1 here, -1 there
-    DevBuf<int> here, there;
-    int nentries = 0;                  // MD entries (md_ptr[nconn])
-};
 
 int flux_md_lists(const rsgpu_flux *F, const int **md_ptr, const int **md_elem, int *nconn,
                   int *ncells)

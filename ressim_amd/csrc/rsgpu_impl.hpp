@@ -123,6 +123,21 @@ struct rsgpu_csr {
     SpmvPlan bplan, bplan32;
 };
 
+// Per-connection transmissibility lists of a mesh (flux.hip): rows =
+// connections with their MD entries in the reference's order, plus each
+// cell's incident connections
+struct rsgpu_flux {
+    rsgpu_ctx *ctx = nullptr;
+    int nconn = 0, ncells = 0;
+    DevBuf<int> md_ptr, md_elem;
+    DevBuf<double> md_t;
+    SpmvPlan plan;
+    DevBuf<int> cell_ptr, cell_conn;   // connections of each cell, ascending,
+    DevBuf<signed char> cell_sign;     // +1 here, -1 there
+    DevBuf<int> here, there;
+    int nentries = 0;                  // MD entries (md_ptr[nconn])
+};
+
 // Level-scheduled sweep data of one triangular factor (see ilu.hip)
 struct SweepData {
     int W = 4;               // padded row width (entries per lane)

@@ -248,3 +248,61 @@ def stencil_nnz(dim: int, nx: int, ny: int, nz: int = 1) -> int:
     if dim == 2:
         return (3 * nx - 2) * (3 * ny - 2)
     return (3 * nx - 2) * (3 * ny - 2) * (3 * nz - 2)
+
+
+# ---------------------------------------------------------------- IMPES state
+IMPES_PARAMS = dict(dt=86400.0, g=9.81, omega=0.0, ax=0.0, ay=0.0, thermal=0)
+
+
+def impes_state(seed: int, nx: int, ny: int, nz: int, dx: float, dy: float, dz: float,
+                ncomp: int = 3, thermal: bool = False, absent: float = 0.15) -> dict:
+    """Seeded control-volume secondaries of a three-phase, ncomp-component
+    IMPES step on a Cartesian grid, as numpy arrays keyed like
+    rsgpu_impes_state (per-phase arrays phase-major).  About ``absent`` of
+    the (cell, phase) pairs have zero saturation and the phase absent, so
+    the density average meets 0/0 and the upstream presence test skips
+    phases.  Synthetic values of the right sign and scale (no PVT model)."""
+    n = nx * ny * nz
+    u = lambda s, m: uniform01(seed, s, m)          # noqa: E731
+    i = np.arange(n)
+    cx = (i % nx + 0.5) * dx
+    cy = ((i // nx) % ny + 0.5) * dy
+    cz = (i // (nx * ny) + 0.5) * dz
+    p = 2.0e7 + 9.81 * 1000.0 * cz + 2.0e5 * (u(1, n) - 0.5)
+    S = u(2, 3 * n).reshape(3, n) + 0.05
+    S[u(3, 3 * n).reshape(3, n) < absent] = 0.0
+    S[0, S.sum(axis=0) == 0] = 1.0
+    S = S / S.sum(axis=0)
+    present = (S > 0).astype(np.uint8)
+    pc = np.stack([-1.0e4 * u(4, n), np.zeros(n), 5.0e3 * u(5, n)])
+    rho = np.stack([1000.0 + 20 * u(6, n), 800.0 + 40 * u(7, n), 120.0 + 30 * u(8, n)])
+    mob = np.stack([S[0] ** 2 / 5e-4, S[1] ** 2 / 2e-3, S[2] ** 2 / 2e-5])
+    frac = u(9, 3 * ncomp * n).reshape(3, ncomp, n) + 0.01
+    frac = frac / frac.sum(axis=1, keepdims=True)
+    dens = np.array([55.0e3, 8.0e3, 4.0e3])[:, None, None]          # mol/m^3 per phase
+    comp_mob = mob[:, None, :] * frac * dens
+    st = dict(
+        pressure=p, phase_pressure=p[None, :] + pc, saturation=S, mass_density=rho,
+        phase_mobility=mob, present=present, comp_mobility=comp_mob,
+        dRdN=(1.0e-5 * (0.5 + u(10, ncomp * n))).reshape(ncomp, n),
+        dRdp=-1.0e-9 * (0.5 + u(11, n)) * dx * dy * dz,
+        p0=p - 1.0e4 * (u(12, n) - 0.3), R0=1.0e-3 * (u(13, n) - 0.5) * dx * dy * dz,
+        cx=cx, cy=cy, cz=cz)
+    if thermal:
+        st.update(heat_capacity=2.5e6 * (1 + u(14, n)) * dx * dy * dz,
+                  dRdT=1.0e-4 * (u(15, n) - 0.3) * dx * dy * dz,
+                  energy_mobility=mob * (4.0e6 + 1.0e6 * u(16, 3 * n).reshape(3, n)),
+                  temperature=350.0 + 10 * u(17, n))
+    return {k: np.ascontiguousarray(v.reshape(-1)) for k, v in st.items()}
+
+
+def impes_sources(seed: int, n: int, ncomp: int, count: int = 6):
+    """(elem, kind, q [count][3][ncomp], qe): alternating regular and outlet
+    terms on seeded cells, with repeats (one cell in two sources)."""
+    elem = (uniform01(seed, 20, count) * n).astype(np.int32)
+    elem[-1] = elem[0]
+    kind = (np.arange(count) % 2).astype(np.int32)
+    q = (uniform01(seed, 21, 3 * ncomp * count) - 0.5).reshape(count, 3, ncomp) * 1.0e-2
+    q[kind == 0, 1:, :] = 0.0
+    qe = (uniform01(seed, 22, count) - 0.5) * 1.0e3
+    return elem, kind, q, qe
