@@ -11,6 +11,7 @@
 // Design.  The reference walks connections and scatters into rows; every
 // matrix entry and residual element is a sum, starting from zero, of its
 // contributions in call order.  Here the work splits into
+//   k_impes_bf    one lane per cell: its body force (gravity, centrifuge);
 //   k_impes_cs    one lane per connection: the CS secondaries
 //                 (density average, potential sum over MD, upstream, flux);
 //   k_impes_conn  one lane per connection: per MD entry the here/there
@@ -18,17 +19,21 @@
 //                 residual contributions ri, rj, each in the reference's
 //                 loop order (MD entry, phase, component, heat term), so
 //                 every Ji, Jj, ri, rj is the Java value bit for bit;
-//   k_impes_rows  one lane per cell: its row of A and its b element as the
-//                 sums of its contributions over its incident connections
-//                 in ascending order (the order of the reference's
-//                 connection loop), then the diagonal term and the cell's
-//                 source terms in application order.
+//   k_impes_vals  one lane per entry of A: the sum of its contributions in
+//                 connection order, from a per-entry list built once per
+//                 matrix pattern (k_map_*: positions by binary search,
+//                 counting sort, each entry's short list sorted by MD
+//                 index, which is connection order);
+//   k_impes_rows  one lane per cell: its b element as the sum of its
+//                 connections' ri / rj in ascending connection order, then
+//                 the diagonal term, the diagonal entry of A, and the
+//                 cell's source terms in application order.
 // Only +, -, *, / are used (no fused multiply-add, -ffp-contract=off), so
-// A and b equal the reference's loops bit for bit.  Bytes per cell are a
-// few hundred (the secondaries), so the kernels are HBM/latency bound and
-// small next to one solve.
+// A and b equal the reference's loops bit for bit.
 #include <algorithm>
 #include <vector>
+
+#include <hipcub/hipcub.hpp>
 
 #include "rsgpu_impl.hpp"
 
@@ -37,7 +42,15 @@ struct rsgpu_impes {
     rsgpu_flux *F = nullptr;
     const rsgpu_flux *Fmf = nullptr;
     int ncomp = 0;
-    DevBuf<double> jh, jt, rh, rt;        // per MD entry / per connection
+    int S = 0;                            // ELL slots of the MD view (0: CSR)
+    DevBuf<int> ell_e;
+    DevBuf<double> ell_t;
+    DevBuf<double> jj, rh, rt;            // per MD entry (here, there) / per connection
+    DevBuf<double> bf;                    // per cell body force
+    // per-entry contribution lists of the matrix pattern `pattern_id`
+    uint64_t pattern_id = 0;
+    int map_status = RSGPU_OK;
+    DevBuf<int> cptr, clist, diag_pos;
     int nterms = 0;
     DevBuf<int> src_ptr, src_term, src_kind;
     DevBuf<double> src_q, src_qe;
@@ -59,14 +72,44 @@ __device__ __forceinline__ double body_force(const ImpesK &K, int e)
     return K.g * z - (K.omega * K.omega / 2) * distance;
 }
 
+__global__ void k_impes_bf(ImpesK K, double *__restrict__ bf)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < K.n)
+        bf[e] = body_force(K, e);
+}
+
 __device__ __forceinline__ double ph_at(const double *a, int n, int ph, int e)
 {
     return a[(size_t)ph * n + e];
 }
 
-__global__ void k_impes_cs(ImpesK K, const int *__restrict__ md_ptr,
-                           const int *__restrict__ md_elem, const double *__restrict__ md_t,
-                           const int *__restrict__ here, const int *__restrict__ there,
+// MD lists as the kernels read them: CSR (md_ptr, md_elem, md_t; any
+// length, one entry after another) or slot-major ELL (entry s of
+// connection c at s * nconn + c, S slots unrolled, loads coalesced across
+// lanes; padding slots hold element `here` and t = 0 and are skipped).
+template <int S>
+struct MdView {
+    static constexpr int slots = S;
+    const int *ptr, *elem;
+    const double *t;
+    int nconn;
+    __device__ __forceinline__ int count(int c) const { return ptr[c + 1] - ptr[c]; }
+    __device__ __forceinline__ int idx(int c, int s) const
+    {
+        return S > 0 ? s * nconn + c : ptr[c] + s;
+    }
+    // the entry's Jacobian slot: the MD index itself (slot-major in ELL, so
+    // the connection kernel's stores coalesce; a connection-major order
+    // measured slower overall)
+    __device__ __forceinline__ int jidx(int c, int s) const { return idx(c, s); }
+};
+
+// CS.calculateMassDensity and calculateDarcyFlux for connection c; the three
+// phases' potential sums run in one pass over MD (each in MD order).
+template <int S>
+__global__ void k_impes_cs(ImpesK K, MdView<S> md, const int *__restrict__ here,
+                           const int *__restrict__ there, const double *__restrict__ bf,
                            double *__restrict__ rho, int *__restrict__ upstream,
                            double *__restrict__ darcy)
 {
@@ -75,6 +118,7 @@ __global__ void k_impes_cs(ImpesK K, const int *__restrict__ md_ptr,
         return;
     const int n = K.n, nc = K.nconn, h = here[c], t = there[c];
     double r[3];
+#pragma unroll
     for (int ph = 0; ph < 3; ++ph) {
         const double Sh = ph_at(K.S.saturation, n, ph, h);
         const double St = ph_at(K.S.saturation, n, ph, t);
@@ -85,33 +129,51 @@ __global__ void k_impes_cs(ImpesK K, const int *__restrict__ md_ptr,
         r[ph] = isnan(av) ? 0.0 : av;
         rho[(size_t)ph * nc + c] = r[ph];
     }
-    const int q0 = md_ptr[c], q1 = md_ptr[c + 1];
-    for (int ph = 0; ph < 3; ++ph) {
-        double D = 0;
-        for (int q = q0; q < q1; ++q) {
-            const int ek = md_elem[q];
-            const double psi = ph_at(K.S.phase_pressure, n, ph, ek) + r[ph] * body_force(K, ek);
-            D += md_t[q] * psi;
+    double D[3] = {0, 0, 0};
+    const int cnt = md.count(c);
+    auto entry = [&](int s) {
+        const int q = md.idx(c, s);
+        const int ek = md.elem[q];
+        const double tk = md.t[q], b = bf[ek];
+#pragma unroll
+        for (int ph = 0; ph < 3; ++ph) {
+            const double psi = ph_at(K.S.phase_pressure, n, ph, ek) + r[ph] * b;
+            if (S == 0 || s < cnt)
+                D[ph] += tk * psi;
         }
-        const int up = (D > 0) ? h : t;
+    };
+    if (S > 0) {
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+            entry(s);
+    } else {
+        for (int s = 0; s < cnt; ++s)
+            entry(s);
+    }
+#pragma unroll
+    for (int ph = 0; ph < 3; ++ph) {
+        const int up = (D[ph] > 0) ? h : t;
         upstream[(size_t)ph * nc + c] = up;
-        darcy[(size_t)ph * nc + c] = ph_at(K.S.phase_mobility, n, ph, up) * D;
+        darcy[(size_t)ph * nc + c] = ph_at(K.S.phase_mobility, n, ph, up) * D[ph];
     }
 }
 
-__global__ void k_impes_conn(ImpesK K, const int *__restrict__ md_ptr,
-                             const int *__restrict__ md_elem, const double *__restrict__ md_t,
-                             const int *__restrict__ mf_ptr, const int *__restrict__ mf_elem,
-                             const double *__restrict__ mf_t, const int *__restrict__ here,
-                             const int *__restrict__ there, const double *__restrict__ rho,
-                             const int *__restrict__ upstream, double *__restrict__ jh,
-                             double *__restrict__ jt, double *__restrict__ rh,
-                             double *__restrict__ rt)
+// assemblePressureJacobianFlux for connection c: per MD entry the here/there
+// Jacobian contributions (jj at 2 * Jacobian slot + side) and the
+// connection's residual contributions.  NC > 0: ncomp ==
+// NC with the connection's component mobilities and dRdN in registers.
+template <int S, int NC>
+__global__ void k_impes_conn(ImpesK K, MdView<S> md, const int *__restrict__ mf_ptr,
+                             const int *__restrict__ mf_elem, const double *__restrict__ mf_t,
+                             const int *__restrict__ here, const int *__restrict__ there,
+                             const double *__restrict__ rho, const int *__restrict__ upstream,
+                             const double *__restrict__ bf, double *__restrict__ jj,
+                             double *__restrict__ rh, double *__restrict__ rt)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= K.nconn)
         return;
-    const int n = K.n, nc = K.nconn, ncomp = K.ncomp;
+    const int n = K.n, nc = K.nconn, ncomp = NC > 0 ? NC : K.ncomp;
     const int i = here[c], j = there[c];
     const bool thermal = K.thermal != 0;
     const double deltaHere = thermal ? K.S.heat_capacity[i] : 0.0;
@@ -119,29 +181,52 @@ __global__ void k_impes_conn(ImpesK K, const int *__restrict__ md_ptr,
     const double dRdT_here = thermal ? K.S.dRdT[i] : 0.0;
     const double dRdT_there = thermal ? K.S.dRdT[j] : 0.0;
     int up[3];
-    double r[3];
+    double r[3], em[3];
     bool pres[3];
+#pragma unroll
     for (int ph = 0; ph < 3; ++ph) {
         up[ph] = upstream[(size_t)ph * nc + c];
         r[ph] = rho[(size_t)ph * nc + c];
         pres[ph] = K.S.present[(size_t)ph * n + up[ph]] != 0;
+        em[ph] = thermal ? ph_at(K.S.energy_mobility, n, ph, up[ph]) : 0.0;
+    }
+    constexpr int NR = NC > 0 ? NC : 1;
+    double mob[3][NR], dRh[NR], dRt[NR];
+    if (NC > 0) {
+#pragma unroll
+        for (int nu = 0; nu < NR; ++nu) {
+            dRh[nu] = K.S.dRdN[(size_t)nu * n + i];
+            dRt[nu] = K.S.dRdN[(size_t)nu * n + j];
+#pragma unroll
+            for (int ph = 0; ph < 3; ++ph)
+                mob[ph][nu] = K.S.comp_mobility[((size_t)ph * NR + nu) * n + up[ph]];
+        }
     }
     double ri = 0, rj = 0;
-    for (int q = md_ptr[c]; q < md_ptr[c + 1]; ++q) {
-        const int ik = md_elem[q];
-        const double tk = md_t[q];
-        const double b = body_force(K, ik);
+    const int cnt = md.count(c);
+    auto entry = [&](int s) {
+        const int q = md.idx(c, s);
+        const int ik = md.elem[q];
+        const double tk = md.t[q];
+        const double b = bf[ik];
+        double psi[3];
+#pragma unroll
+        for (int ph = 0; ph < 3; ++ph)
+            psi[ph] = ph_at(K.S.phase_pressure, n, ph, ik) + r[ph] * b;
+        if (S > 0 && s >= cnt)
+            return;
         double Ji = 0, Jj = 0;
+#pragma unroll
         for (int ph = 0; ph < 3; ++ph) {
             if (!pres[ph])
                 continue;
-            const double psi = ph_at(K.S.phase_pressure, n, ph, ik) + r[ph] * b;
             for (int nu = 0; nu < ncomp; ++nu) {
                 const double phaseCompMob =
-                    K.S.comp_mobility[((size_t)ph * ncomp + nu) * n + up[ph]];
-                const double dRdNhere = K.S.dRdN[(size_t)nu * n + i];
-                const double dRdNthere = K.S.dRdN[(size_t)nu * n + j];
-                const double dNdt = -phaseCompMob * tk * psi;
+                    NC > 0 ? mob[ph][nu]
+                           : K.S.comp_mobility[((size_t)ph * ncomp + nu) * n + up[ph]];
+                const double dRdNhere = NC > 0 ? dRh[nu] : K.S.dRdN[(size_t)nu * n + i];
+                const double dRdNthere = NC > 0 ? dRt[nu] : K.S.dRdN[(size_t)nu * n + j];
+                const double dNdt = -phaseCompMob * tk * psi[ph];
                 const double d2Ndtp = -phaseCompMob * tk;
                 ri += dRdNhere * dNdt;
                 rj += -dRdNthere * dNdt;
@@ -149,8 +234,8 @@ __global__ void k_impes_conn(ImpesK K, const int *__restrict__ md_ptr,
                 Jj += -dRdNthere * d2Ndtp;
             }
             if (thermal) {
-                const double phaseEnergyMob = ph_at(K.S.energy_mobility, n, ph, up[ph]);
-                const double dTdt = -phaseEnergyMob * tk * psi;
+                const double phaseEnergyMob = em[ph];
+                const double dTdt = -phaseEnergyMob * tk * psi[ph];
                 const double d2Tdtp = -phaseEnergyMob * tk;
                 ri += dRdT_here * dTdt / deltaHere;
                 rj += -dRdT_there * dTdt / deltaThere;
@@ -158,8 +243,17 @@ __global__ void k_impes_conn(ImpesK K, const int *__restrict__ md_ptr,
                 Jj += -dRdT_there * d2Tdtp / deltaThere;
             }
         }
-        jh[q] = Ji;
-        jt[q] = Jj;
+        const size_t w = (size_t)md.jidx(c, s);
+        jj[2 * w] = Ji;
+        jj[2 * w + 1] = Jj;
+    };
+    if (S > 0) {
+#pragma unroll
+        for (int s = 0; s < S; ++s)
+            entry(s);
+    } else {
+        for (int s = 0; s < cnt; ++s)
+            entry(s);
     }
     if (thermal)
         for (int q = mf_ptr[c]; q < mf_ptr[c + 1]; ++q) {
@@ -169,6 +263,24 @@ __global__ void k_impes_conn(ImpesK K, const int *__restrict__ md_ptr,
         }
     rh[c] = ri;
     rt[c] = rj;
+}
+
+// slot-major copy of the MD lists (padding: element `here`, t = 0)
+template <int S>
+__global__ void k_impes_ell(int nconn, const int *__restrict__ md_ptr,
+                            const int *__restrict__ md_elem, const double *__restrict__ md_t,
+                            const int *__restrict__ here, int *__restrict__ ell_e,
+                            double *__restrict__ ell_t)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nconn)
+        return;
+    const int q0 = md_ptr[c], cnt = md_ptr[c + 1] - q0;
+    for (int s = 0; s < S; ++s) {
+        const bool v = s < cnt;
+        ell_e[(size_t)s * nconn + c] = v ? md_elem[q0 + s] : here[c];
+        ell_t[(size_t)s * nconn + c] = v ? md_t[q0 + s] : 0.0;
+    }
 }
 
 __device__ __forceinline__ int find_col(const int *__restrict__ colind, int lo, int hi, int k)
@@ -184,54 +296,133 @@ __device__ __forceinline__ int find_col(const int *__restrict__ colind, int lo, 
     return lo;
 }
 
+// ---- per-entry contribution lists (once per matrix pattern)
+// pos[2 v + side] = the position in A of entry v's element in row here[c]
+// (side 0) / there[c] (side 1), v the entry's Jacobian slot; -1 for ELL
+// padding
+template <int S>
+__global__ void k_map_count(MdView<S> md, const int *__restrict__ md_ptr,
+                            const int *__restrict__ md_elem, const int *__restrict__ here,
+                            const int *__restrict__ there, const int *__restrict__ rowptr,
+                            const int *__restrict__ colind, int *__restrict__ pos,
+                            int *__restrict__ cnt, int *__restrict__ err)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= md.nconn)
+        return;
+    const int rows[2] = {here[c], there[c]};
+    const int q0 = md_ptr[c], m = md_ptr[c + 1] - q0;
+    for (int s = 0; s < (S > 0 ? S : m); ++s) {
+        const size_t v = (size_t)md.jidx(c, s);
+        if (s >= m) {
+            pos[2 * v] = pos[2 * v + 1] = -1;
+            continue;
+        }
+        const int k = md_elem[q0 + s];
+        for (int side = 0; side < 2; ++side) {
+            const int r = rows[side], r0 = rowptr[r], r1 = rowptr[r + 1];
+            const int p = find_col(colind, r0, r1, k);
+            if (p >= r1 || colind[p] != k) {
+                atomicOr(err, 1);
+                pos[2 * v + side] = -1;
+                continue;
+            }
+            pos[2 * v + side] = p;
+            atomicAdd(&cnt[p], 1);
+        }
+    }
+}
+
+__global__ void k_map_fill(size_t nitems, const int *__restrict__ pos,
+                           const int *__restrict__ cptr, int *__restrict__ fill,
+                           int *__restrict__ clist)
+{
+    const size_t it = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= nitems)
+        return;
+    const int p = pos[it];
+    if (p < 0)
+        return;
+    clist[cptr[p] + atomicAdd(&fill[p], 1)] = (int)it;
+}
+
+// each entry's list in connection order (the item's connection: item / 2
+// in CSR layout, (item / 2) mod nconn in slot-major layout)
+__global__ void k_map_sort(int nnz, int ell_nconn, const int *__restrict__ cptr,
+                           int *__restrict__ clist)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nnz)
+        return;
+    auto key = [&](int it) { return ell_nconn > 0 ? (it >> 1) % ell_nconn : it; };
+    const int a = cptr[p], e = cptr[p + 1];
+    for (int i = a + 1; i < e; ++i) {
+        const int v = clist[i], kv = key(v);
+        int j = i - 1;
+        while (j >= a && key(clist[j]) > kv) {
+            clist[j + 1] = clist[j];
+            --j;
+        }
+        clist[j + 1] = v;
+    }
+}
+
+__global__ void k_map_diag(int n, const int *__restrict__ rowptr,
+                           const int *__restrict__ colind, int *__restrict__ dpos,
+                           int *__restrict__ err)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n)
+        return;
+    const int r0 = rowptr[r], r1 = rowptr[r + 1];
+    const int p = find_col(colind, r0, r1, r);
+    if (p >= r1 || colind[p] != r) {
+        atomicOr(err, 1);
+        dpos[r] = -1;
+    } else {
+        dpos[r] = p;
+    }
+}
+
+// ---- assembly
+__global__ void k_impes_vals(int nnz, const int *__restrict__ cptr,
+                             const int *__restrict__ clist, const double *__restrict__ jj,
+                             double *__restrict__ val)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nnz)
+        return;
+    double v = 0;
+    for (int k = cptr[p]; k < cptr[p + 1]; ++k)
+        v += jj[clist[k]];
+    val[p] = v;
+}
+
 __global__ void k_impes_rows(ImpesK K, const int *__restrict__ cell_ptr,
                              const int *__restrict__ cell_conn,
                              const signed char *__restrict__ cell_sign,
-                             const int *__restrict__ md_ptr, const int *__restrict__ md_elem,
-                             const double *__restrict__ jh, const double *__restrict__ jt,
                              const double *__restrict__ rh, const double *__restrict__ rt,
-                             const int *__restrict__ rowptr, const int *__restrict__ colind,
-                             double *__restrict__ val, const int *__restrict__ src_ptr,
-                             const int *__restrict__ src_term, const int *__restrict__ src_kind,
+                             const int *__restrict__ dpos, double *__restrict__ val,
+                             const int *__restrict__ src_ptr, const int *__restrict__ src_term,
+                             const int *__restrict__ src_kind,
                              const double *__restrict__ src_q,
-                             const double *__restrict__ src_qe, double *__restrict__ bvec,
-                             int *__restrict__ err)
+                             const double *__restrict__ src_qe, double *__restrict__ bvec)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= K.n)
         return;
     const int n = K.n, ncomp = K.ncomp;
-    const int r0 = rowptr[r], r1 = rowptr[r + 1];
-    for (int p = r0; p < r1; ++p)
-        val[p] = 0.0;
     double acc = 0;
     // flux part: incident connections in ascending order
     for (int a = cell_ptr[r]; a < cell_ptr[r + 1]; ++a) {
         const int c = cell_conn[a];
-        const bool h = cell_sign[a] > 0;
-        const double *J = h ? jh : jt;
-        for (int q = md_ptr[c]; q < md_ptr[c + 1]; ++q) {
-            const int k = md_elem[q];
-            const int p = find_col(colind, r0, r1, k);
-            if (p >= r1 || colind[p] != k) {
-                atomicOr(err, 1);
-                continue;
-            }
-            val[p] += J[q];
-        }
-        acc += h ? rh[c] : rt[c];
+        acc += cell_sign[a] > 0 ? rh[c] : rt[c];
     }
     // diagonal part
     const double dp = K.S.pressure[r] - K.S.p0[r];
     const double dRdp = K.S.dRdp[r];
     acc += (K.S.R0[r] + dRdp * dp) / K.dt;
-    {
-        const int p = find_col(colind, r0, r1, r);
-        if (p >= r1 || colind[p] != r)
-            atomicOr(err, 1);
-        else
-            val[p] += dRdp / K.dt;
-    }
+    val[dpos[r]] += dRdp / K.dt;
     // source terms of this cell, in application order
     for (int a = src_ptr[r]; a < src_ptr[r + 1]; ++a) {
         const int t = src_term[a];
@@ -299,6 +490,79 @@ void check_state(const rsgpu_impes *E, const rsgpu_impes_params *P,
     }
 }
 
+template <int S>
+MdView<S> md_view(const rsgpu_impes *E)
+{
+    const rsgpu_flux *F = E->F;
+    MdView<S> v;
+    v.ptr = F->md_ptr.p;
+    v.elem = S > 0 ? E->ell_e.p : F->md_elem.p;
+    v.t = S > 0 ? E->ell_t.p : F->md_t.p;
+    v.nconn = F->nconn;
+    return v;
+}
+
+// f(view) with the view type of E's layout
+template <class Fn>
+void with_view(const rsgpu_impes *E, Fn f)
+{
+    if (E->S == 6)
+        f(md_view<6>(E));
+    else if (E->S == 18)
+        f(md_view<18>(E));
+    else
+        f(md_view<0>(E));
+}
+
+// Per-entry contribution lists of A's pattern: for entry p of row r, the
+// indices 2v + side of the MD entries (here side 0, there side 1; v the
+// entry's Jacobian slot) whose element is p's column, over r's incident
+// connections, in connection order.
+void build_map(rsgpu_impes *E, const rsgpu_csr *A)
+{
+    const rsgpu_flux *F = E->F;
+    hipStream_t s = E->ctx->stream;
+    const int nnz = A->nnz, n = F->ncells;
+    const size_t nit = E->jj.n;   // 2 x view entries
+    DevBuf<int> pos, cnt, fill;
+    pos.alloc(std::max(nit, (size_t)1));
+    cnt.alloc((size_t)nnz + 1);
+    fill.alloc(std::max(nnz, 1));
+    cnt.zero(s);
+    fill.zero(s);
+    E->cptr.alloc((size_t)nnz + 1);
+    E->clist.alloc(std::max(nit, (size_t)1));
+    E->diag_pos.alloc(std::max(n, 1));
+    hipLaunchKernelGGL(k_impes_clear, dim3(1), dim3(1), 0, s, E->err.p);
+    if (F->nconn)
+        with_view(E, [&](auto v) {
+            hipLaunchKernelGGL(k_map_count<decltype(v)::slots>, dim3((F->nconn + 255) / 256),
+                               dim3(256), 0, s, v, F->md_ptr.p, F->md_elem.p, F->here.p,
+                               F->there.p, A->rowptr.p, A->colind.p, pos.p, cnt.p, E->err.p);
+        });
+    RS_CHECK_LAUNCH();
+    size_t tmp = 0;
+    RS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, cnt.p, E->cptr.p, nnz + 1, s));
+    DevBuf<unsigned char> work;
+    work.alloc(std::max(tmp, (size_t)1));
+    RS_HIP(hipcub::DeviceScan::ExclusiveSum(work.p, tmp, cnt.p, E->cptr.p, nnz + 1, s));
+    if (nit)
+        hipLaunchKernelGGL(k_map_fill, dim3((unsigned)((nit + 255) / 256)), dim3(256), 0, s,
+                           nit, pos.p, E->cptr.p, fill.p, E->clist.p);
+    if (nnz)
+        hipLaunchKernelGGL(k_map_sort, dim3((nnz + 255) / 256), dim3(256), 0, s, nnz,
+                           E->S > 0 ? F->nconn : 0, E->cptr.p, E->clist.p);
+    if (n)
+        hipLaunchKernelGGL(k_map_diag, dim3((n + 255) / 256), dim3(256), 0, s, n, A->rowptr.p,
+                           A->colind.p, E->diag_pos.p, E->err.p);
+    RS_CHECK_LAUNCH();
+    int h = 0;
+    RS_HIP(hipMemcpyAsync(&h, E->err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    RS_HIP(hipStreamSynchronize(s));
+    E->map_status = h ? RSGPU_NOT_IN_PATTERN : RSGPU_OK;
+    E->pattern_id = A->pattern_id;
+}
+
 }  // namespace
 
 extern "C" int rsgpu_impes_create(rsgpu_flux *F, const rsgpu_flux *Fmf, int ncomp,
@@ -314,8 +578,31 @@ extern "C" int rsgpu_impes_create(rsgpu_flux *F, const rsgpu_flux *Fmf, int ncom
         E->F = F;
         E->Fmf = Fmf;
         E->ncomp = ncomp;
-        E->jh.alloc(std::max(F->nentries, 1));
-        E->jt.alloc(std::max(F->nentries, 1));
+        // slot-major copy of the MD lists when every list is short
+        int maxlen = 0;
+        if (F->nconn) {
+            std::vector<int> mp(F->nconn + 1);
+            RS_HIP(hipMemcpy(mp.data(), F->md_ptr.p, sizeof(int) * mp.size(),
+                             hipMemcpyDeviceToHost));
+            for (int c = 0; c < F->nconn; ++c)
+                maxlen = std::max(maxlen, mp[c + 1] - mp[c]);
+        }
+        E->S = getenv("RSGPU_IMPES_CSR") ? 0 : maxlen <= 6 ? 6 : maxlen <= 18 ? 18 : 0;
+        if (F->nconn == 0)
+            E->S = 0;
+        if (E->S > 0) {
+            E->ell_e.alloc((size_t)E->S * F->nconn);
+            E->ell_t.alloc((size_t)E->S * F->nconn);
+            hipLaunchKernelGGL(E->S == 6 ? k_impes_ell<6> : k_impes_ell<18>,
+                               dim3((F->nconn + 255) / 256), dim3(256), 0, F->ctx->stream,
+                               F->nconn, F->md_ptr.p, F->md_elem.p, F->md_t.p, F->here.p,
+                               E->ell_e.p, E->ell_t.p);
+            RS_CHECK_LAUNCH();
+            E->jj.alloc(2 * (size_t)E->S * F->nconn);
+        } else {
+            E->jj.alloc(2 * (size_t)std::max(F->nentries, 1));
+        }
+        E->bf.alloc(std::max(F->ncells, 1));
         E->rh.alloc(std::max(F->nconn, 1));
         E->rt.alloc(std::max(F->nconn, 1));
         E->err.alloc(1);
@@ -390,9 +677,14 @@ extern "C" int rsgpu_impes_control_surfaces(rsgpu_impes *E, const rsgpu_impes_pa
         check_state(E, P, S, false);
         const rsgpu_flux *F = E->F;
         if (F->nconn) {
-            hipLaunchKernelGGL(k_impes_cs, dim3((F->nconn + 255) / 256), dim3(256), 0,
-                               E->ctx->stream, make_k(E, P, S), F->md_ptr.p, F->md_elem.p,
-                               F->md_t.p, F->here.p, F->there.p, rho, upstream, darcy);
+            const ImpesK K = make_k(E, P, S);
+            hipLaunchKernelGGL(k_impes_bf, dim3((F->ncells + 255) / 256), dim3(256), 0,
+                               E->ctx->stream, K, E->bf.p);
+            with_view(E, [&](auto v) {
+                hipLaunchKernelGGL(k_impes_cs<decltype(v)::slots>, dim3((F->nconn + 255) / 256),
+                                   dim3(256), 0, E->ctx->stream, K, v, F->here.p, F->there.p,
+                                   E->bf.p, rho, upstream, darcy);
+            });
             RS_CHECK_LAUNCH();
         }
         return RSGPU_OK;
@@ -415,29 +707,58 @@ extern "C" int rsgpu_impes_assemble(rsgpu_impes *E, rsgpu_csr *A, const rsgpu_im
         if (P->dt == 0.0)
             throw RsError{RSGPU_BAD_ARGUMENT, "time step must be nonzero"};
         hipStream_t s = E->ctx->stream;
+        if (E->pattern_id != A->pattern_id)
+            build_map(E, A);
+        if (E->map_status != RSGPU_OK)
+            throw RsError{E->map_status, "Jacobian entry outside the matrix pattern"};
         const ImpesK K = make_k(E, P, S);
         const rsgpu_flux *M = E->Fmf ? E->Fmf : F;   // MF lists read only when thermal
-        hipLaunchKernelGGL(k_impes_clear, dim3(1), dim3(1), 0, s, E->err.p);
+        if (F->ncells) {
+            hipLaunchKernelGGL(k_impes_bf, dim3((F->ncells + 255) / 256), dim3(256), 0, s, K,
+                               E->bf.p);
+            RS_CHECK_LAUNCH();
+        }
         if (F->nconn) {
-            hipLaunchKernelGGL(k_impes_conn, dim3((F->nconn + 255) / 256), dim3(256), 0, s, K,
-                               F->md_ptr.p, F->md_elem.p, F->md_t.p, M->md_ptr.p, M->md_elem.p,
-                               M->md_t.p, F->here.p, F->there.p, rho, upstream, E->jh.p,
-                               E->jt.p, E->rh.p, E->rt.p);
+            with_view(E, [&](auto v) {
+                constexpr int SL = decltype(v)::slots;
+                const dim3 g((F->nconn + 255) / 256), bl(256);
+                auto go = [&](auto kern) {
+                    hipLaunchKernelGGL(kern, g, bl, 0, s, K, v, M->md_ptr.p, M->md_elem.p,
+                                       M->md_t.p, F->here.p, F->there.p, rho, upstream,
+                                       E->bf.p, E->jj.p, E->rh.p, E->rt.p);
+                };
+                switch (E->ncomp) {
+                case 1:
+                    go(k_impes_conn<SL, 1>);
+                    break;
+                case 2:
+                    go(k_impes_conn<SL, 2>);
+                    break;
+                case 3:
+                    go(k_impes_conn<SL, 3>);
+                    break;
+                case 4:
+                    go(k_impes_conn<SL, 4>);
+                    break;
+                default:
+                    go(k_impes_conn<SL, 0>);
+                    break;
+                }
+            });
+            RS_CHECK_LAUNCH();
+        }
+        if (A->nnz) {
+            hipLaunchKernelGGL(k_impes_vals, dim3((A->nnz + 255) / 256), dim3(256), 0, s,
+                               A->nnz, E->cptr.p, E->clist.p, E->jj.p, A->val.p);
             RS_CHECK_LAUNCH();
         }
         if (F->ncells) {
             hipLaunchKernelGGL(k_impes_rows, dim3((F->ncells + 255) / 256), dim3(256), 0, s, K,
-                               F->cell_ptr.p, F->cell_conn.p, F->cell_sign.p, F->md_ptr.p,
-                               F->md_elem.p, E->jh.p, E->jt.p, E->rh.p, E->rt.p, A->rowptr.p,
-                               A->colind.p, A->val.p, E->src_ptr.p, E->src_term.p,
-                               E->src_kind.p, E->src_q.p, E->src_qe.p, b, E->err.p);
+                               F->cell_ptr.p, F->cell_conn.p, F->cell_sign.p, E->rh.p, E->rt.p,
+                               E->diag_pos.p, A->val.p, E->src_ptr.p, E->src_term.p,
+                               E->src_kind.p, E->src_q.p, E->src_qe.p, b);
             RS_CHECK_LAUNCH();
         }
-        int h = 0;
-        RS_HIP(hipMemcpyAsync(&h, E->err.p, sizeof(int), hipMemcpyDeviceToHost, s));
-        RS_HIP(hipStreamSynchronize(s));
-        if (h)
-            throw RsError{RSGPU_NOT_IN_PATTERN, "Jacobian entry outside the matrix pattern"};
         return RSGPU_OK;
     } catch (const RsError &e) {
         return impes_fail(E->ctx, e);

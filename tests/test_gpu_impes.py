@@ -31,12 +31,19 @@ def _device(st):
     return {k: torch.tensor(v, device="cuda") for k, v in st.items()}
 
 
-@pytest.mark.parametrize("case", ["isothermal", "thermal", "centrifuge"])
-def test_impes_assembly_bitexact(gpu, oracle, mesh, case):
+@pytest.mark.parametrize("case", ["isothermal", "thermal", "centrifuge", "csr-lists",
+                                  "five-components"])
+def test_impes_assembly_bitexact(gpu, oracle, mesh, monkeypatch, case):
+    """Slot-major MD lists (the default for MPFA-O lists) and the CSR lists
+    (RSGPU_IMPES_CSR), component counts with register-resident mobilities
+    (3) and without (5)."""
     import torch
     from ressim_amd.impes import PressureJacobian
     from ressim_amd.potential import FluxOperator
-    m, n, ncomp = mesh, mesh["n"], 3
+    if case == "csr-lists":
+        monkeypatch.setenv("RSGPU_IMPES_CSR", "1")
+    m, n = mesh, mesh["n"]
+    ncomp = 5 if case == "five-components" else 3
     thermal = case == "thermal"
     print(f"IMPES assembly {case}", flush=True)
     g = GRID
