@@ -217,7 +217,10 @@ enum rsgpu_solver_kind {
     RSGPU_BICGSTAB = 0,  /* mtj/sparse/BiCGstab.java */
     RSGPU_GMRES = 1,     /* mtj/sparse/GMRES.java    */
     RSGPU_IR = 2,        /* mtj/sparse/IR.java       */
-    RSGPU_CG = 3         /* mtj/sparse/CG.java       */
+    RSGPU_CG = 3,        /* mtj/sparse/CG.java       */
+    RSGPU_BICG = 4,      /* mtj/sparse/BiCG.java     */
+    RSGPU_CGS = 5,       /* mtj/sparse/CGS.java      */
+    RSGPU_QMR = 6        /* mtj/sparse/QMR.java (M1 = M2 = M) */
 };
 
 /* DefaultIterationMonitor(maxIter, rtol, atol, dtol)

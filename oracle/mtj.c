@@ -852,6 +852,256 @@ int or_cg(int n, const int *rowptr, const int *colind, const double *data,
     return st;
 }
 
+/* M.transApply: ILU.transApply (ILU.java:72-78), AMG.transApply; the
+ * diagonal, SSOR and identity preconditioners' transApply is apply. */
+static void prec_tapply(prec_t *p, const double *b, double *x)
+{
+    if (p->kind == 1 || p->kind == 2) {
+        or_ilu_transapply(p->n, p->rowptr, p->colind, p->lu, p->diagind, b, x, p->y);
+        return;
+    }
+    if (p->kind == 3) {
+        or_amg_apply(p->amg, b, x, 1);
+        return;
+    }
+    prec_apply(p, b, x);
+}
+
+/* x = beta*x + y with AbstractVector.scale (beta 0 zeroes, 1 keeps) then
+ * DenseVector.add */
+static void v_scale_add(int n, double *x, double beta, const double *y)
+{
+    for (int i = 0; i < n; ++i) {
+        double xi = beta == 0 ? 0.0 : (beta == 1 ? x[i] : beta * x[i]);
+        x[i] = xi + y[i];
+    }
+}
+
+/* x.scale(beta) (AbstractVector.scale) */
+static void v_scale(int n, double *x, double beta)
+{
+    if (beta == 1)
+        return;
+    for (int i = 0; i < n; ++i)
+        x[i] = beta == 0 ? 0.0 : beta * x[i];
+}
+
+#define SOLVER_PROLOGUE                                                          \
+    prec_t M;                                                                    \
+    int bad = -1;                                                                \
+    int st = prec_setup(&M, n, rowptr, colind, data, opts, &bad);                \
+    monitor_t it;                                                                \
+    mon_init(&it, opts, hist, hist_cap);                                         \
+    if (st) {                                                                    \
+        prec_free(&M);                                                           \
+        fill_stats(stats, &it, st);                                              \
+        if (stats)                                                               \
+            stats->bad_row = bad;                                                \
+        return st;                                                               \
+    }
+
+/* ---- BiCG (BiCG.java:74-121) ---------------------------------------------- */
+int or_bicg(int n, const int *rowptr, const int *colind, const double *data,
+            const double *b, double *x, const or_opts *opts, or_stats *stats,
+            double *hist, int hist_cap)
+{
+    SOLVER_PROLOGUE
+    double *w = (double *)calloc((size_t)n * 8 + 1, sizeof(double));
+    double *r = w, *rt = w + n, *z = w + 2 * (size_t)n, *zt = w + 3 * (size_t)n;
+    double *p = w + 4 * (size_t)n, *pt = w + 5 * (size_t)n, *q = w + 6 * (size_t)n,
+           *qt = w + 7 * (size_t)n;
+    double rho_1 = 1, rho_2 = 1, alpha = 1, beta = 1;
+    int conv = 0;
+    v_copy(n, r, b);
+    or_csr_multadd(n, rowptr, colind, data, -1, x, r);
+    v_copy(n, rt, r);
+    for (it.iter = 0;; it.iter++) {
+        st = mon_converged(&it, or_norm2(n, r), &conv);
+        if (st || conv)
+            break;
+        prec_apply(&M, r, z);
+        prec_tapply(&M, rt, zt);
+        rho_1 = or_dot(n, z, rt);
+        if (rho_1 == 0.) {
+            st = OR_BREAKDOWN;
+            break;
+        }
+        if (it.iter == 0) {
+            v_copy(n, p, z);
+            v_copy(n, pt, zt);
+        } else {
+            beta = rho_1 / rho_2;
+            v_scale_add(n, p, beta, z);
+            v_scale_add(n, pt, beta, zt);
+        }
+        or_csr_mult(n, rowptr, colind, data, p, q);
+        or_csr_transmult(n, n, rowptr, colind, data, pt, qt);
+        alpha = rho_1 / or_dot(n, pt, q);
+        v_add_scaled(n, x, alpha, p);
+        v_add_scaled(n, r, -alpha, q);
+        v_add_scaled(n, rt, -alpha, qt);
+        rho_2 = rho_1;
+    }
+    (void)beta;
+    free(w);
+    prec_free(&M);
+    fill_stats(stats, &it, st);
+    return st;
+}
+
+/* ---- CGS (CGS.java:78-121) ------------------------------------------------ */
+int or_cgs(int n, const int *rowptr, const int *colind, const double *data,
+           const double *b, double *x, const or_opts *opts, or_stats *stats,
+           double *hist, int hist_cap)
+{
+    SOLVER_PROLOGUE
+    double *w = (double *)calloc((size_t)n * 10 + 1, sizeof(double));
+    double *r = w, *rt = w + n, *u = w + 2 * (size_t)n, *p = w + 3 * (size_t)n;
+    double *ph = w + 4 * (size_t)n, *vh = w + 5 * (size_t)n, *q = w + 6 * (size_t)n;
+    double *sum = w + 7 * (size_t)n, *uh = w + 8 * (size_t)n, *qh = w + 9 * (size_t)n;
+    double rho_1 = 0, rho_2 = 0, alpha = 0, beta = 0;
+    int conv = 0;
+    v_copy(n, r, b);
+    or_csr_multadd(n, rowptr, colind, data, -1, x, r);
+    v_copy(n, rt, r);
+    for (it.iter = 0;; it.iter++) {
+        st = mon_converged(&it, or_norm2(n, r), &conv);
+        if (st || conv)
+            break;
+        rho_1 = or_dot(n, rt, r);
+        if (rho_1 == 0) {
+            st = OR_BREAKDOWN;
+            break;
+        }
+        if (it.iter == 0) {
+            v_copy(n, u, r);
+            v_copy(n, p, u);
+        } else {
+            beta = rho_1 / rho_2;
+            v_copy(n, u, r);
+            v_add_scaled(n, u, beta, q);
+            v_copy(n, sum, q);
+            v_add_scaled(n, sum, beta, p);
+            v_copy(n, p, u);
+            v_add_scaled(n, p, beta, sum);
+        }
+        prec_apply(&M, p, ph);
+        or_csr_mult(n, rowptr, colind, data, ph, vh);
+        alpha = rho_1 / or_dot(n, rt, vh);
+        v_set_scaled(n, q, -alpha, vh);
+        v_add(n, q, u);
+        v_copy(n, sum, u);
+        v_add(n, sum, q);
+        prec_apply(&M, sum, uh);
+        v_add_scaled(n, x, alpha, uh);
+        or_csr_mult(n, rowptr, colind, data, uh, qh);
+        v_add_scaled(n, r, -alpha, qh);
+        rho_2 = rho_1;
+    }
+    free(w);
+    prec_free(&M);
+    fill_stats(stats, &it, st);
+    return st;
+}
+
+/* ---- QMR (QMR.java:136-243), M1 = M2 = M ----------------------------------- */
+int or_qmr(int n, const int *rowptr, const int *colind, const double *data,
+           const double *b, double *x, const or_opts *opts, or_stats *stats,
+           double *hist, int hist_cap)
+{
+    SOLVER_PROLOGUE
+    double *wk = (double *)calloc((size_t)n * 14 + 1, sizeof(double));
+    double *V[14];
+    for (int k = 0; k < 14; ++k)
+        V[k] = wk + (size_t)k * n;
+    double *r = V[0], *y = V[1], *z = V[2], *v = V[3], *w = V[4], *p = V[5], *q = V[6];
+    double *d = V[7], *s = V[8], *v_tld = V[9], *w_tld = V[10], *y_tld = V[11],
+           *z_tld = V[12], *p_tld = V[13];
+    double rho = 0, rho_1 = 0, xi = 0, gamma = 1., gamma_1 = 0, theta = 0, theta_1 = 0,
+           eta = -1., delta = 0, ep = 0, beta = 0;
+    int conv = 0;
+    v_copy(n, r, b);
+    or_csr_multadd(n, rowptr, colind, data, -1, x, r);
+    v_copy(n, v_tld, r);
+    prec_apply(&M, v_tld, y);
+    rho = or_norm2(n, y);
+    v_copy(n, w_tld, r);
+    prec_tapply(&M, w_tld, z);
+    xi = or_norm2(n, z);
+    for (it.iter = 0;; it.iter++) {
+        st = mon_converged(&it, or_norm2(n, r), &conv);
+        if (st || conv)
+            break;
+        if (rho == 0 || xi == 0) {
+            st = OR_BREAKDOWN;
+            break;
+        }
+        v_set_scaled(n, v, 1 / rho, v_tld);
+        v_scale(n, y, 1 / rho);
+        v_set_scaled(n, w, 1 / xi, w_tld);
+        v_scale(n, z, 1 / xi);
+        delta = or_dot(n, z, y);
+        if (delta == 0) {
+            st = OR_BREAKDOWN;
+            break;
+        }
+        prec_apply(&M, y, y_tld);
+        prec_tapply(&M, z, z_tld);
+        if (it.iter == 0) {
+            v_copy(n, p, y_tld);
+            v_copy(n, q, z_tld);
+        } else {
+            v_scale_add(n, p, -xi * delta / ep, y_tld);
+            v_scale_add(n, q, -rho * delta / ep, z_tld);
+        }
+        or_csr_mult(n, rowptr, colind, data, p, p_tld);
+        ep = or_dot(n, q, p_tld);
+        if (ep == 0) {
+            st = OR_BREAKDOWN;
+            break;
+        }
+        beta = ep / delta;
+        if (beta == 0) {
+            st = OR_BREAKDOWN;
+            break;
+        }
+        v_set_scaled(n, v_tld, -beta, v);
+        v_add(n, v_tld, p_tld);
+        prec_apply(&M, v_tld, y);
+        rho_1 = rho;
+        rho = or_norm2(n, y);
+        v_set_scaled(n, w_tld, -beta, w);
+        or_csr_transmultadd(n, n, rowptr, colind, data, 1, q, w_tld);
+        prec_tapply(&M, w_tld, z);
+        xi = or_norm2(n, z);
+        gamma_1 = gamma;
+        theta_1 = theta;
+        theta = rho / (gamma_1 * beta);
+        gamma = 1 / sqrt(1 + theta * theta);
+        if (gamma == 0) {
+            st = OR_BREAKDOWN;
+            break;
+        }
+        eta = -eta * rho_1 * gamma * gamma / (beta * gamma_1 * gamma_1);
+        if (it.iter == 0) {
+            v_set_scaled(n, d, eta, p);
+            v_set_scaled(n, s, eta, p_tld);
+        } else {
+            const double val = theta_1 * theta_1 * gamma * gamma;
+            v_scale(n, d, val);
+            v_add_scaled(n, d, eta, p);
+            v_scale(n, s, val);
+            v_add_scaled(n, s, eta, p_tld);
+        }
+        v_add(n, x, d);
+        v_add_scaled(n, r, -1, s);
+    }
+    free(wk);
+    prec_free(&M);
+    fill_stats(stats, &it, st);
+    return st;
+}
+
 /* ---- IR (IR.java:57-70) ------------------------------------------------- */
 int or_ir(int n, const int *rowptr, const int *colind, const double *data,
           const double *b, double *x, const or_opts *opts, or_stats *stats,

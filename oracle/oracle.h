@@ -169,6 +169,15 @@ int or_cg(int n, const int *rowptr, const int *colind, const double *data,
 int or_ir(int n, const int *rowptr, const int *colind, const double *data,
           const double *b, double *x, const or_opts *opts, or_stats *stats,
           double *hist, int hist_cap);
+int or_bicg(int n, const int *rowptr, const int *colind, const double *data,
+            const double *b, double *x, const or_opts *opts, or_stats *stats,
+            double *hist, int hist_cap);
+int or_cgs(int n, const int *rowptr, const int *colind, const double *data,
+           const double *b, double *x, const or_opts *opts, or_stats *stats,
+           double *hist, int hist_cap);
+int or_qmr(int n, const int *rowptr, const int *colind, const double *data,
+           const double *b, double *x, const or_opts *opts, or_stats *stats,
+           double *hist, int hist_cap);
 int or_block_starts(int n, int nblocks, int *bstart);
 
 /* ---- Dense LU (LAPACK 3.0 dgetf2/dgetrs restatement) --------------------- */

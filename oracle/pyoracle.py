@@ -116,7 +116,7 @@ def lib():
         L.or_block_filter.argtypes = [C.c_int, _i32p, _i32p, _f64p, C.c_int, _i32p, _i32p,
                                       _i32p, _f64p, _i32p]
         L.or_block_starts.argtypes = [C.c_int, C.c_int, _i32p]
-        for f in (L.or_bicgstab, L.or_gmres, L.or_ir, L.or_cg):
+        for f in (L.or_bicgstab, L.or_gmres, L.or_ir, L.or_cg, L.or_bicg, L.or_cgs, L.or_qmr):
             f.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.POINTER(Opts),
                           C.POINTER(Stats), _f64p, C.c_int]
         L.or_dgesv.argtypes = [C.c_int, C.c_int, _f64p, C.c_int, _i32p, _f64p, C.c_int]
@@ -326,7 +326,8 @@ PREC = {"none": 0, "identity": 0, "ilu": 1, "block_ilu": 2, "amg": 3, "diagonal"
 
 def solve(kind, rowptr, colind, data, b, x0=None, *, max_iter=100000, rtol=1e-5, atol=1e-50,
           dtol=1e5, restart=30, prec="ilu", nblocks=1, hist_cap=None, amg=None):
-    """Run or_bicgstab / or_gmres / or_ir.  Returns (x, Stats, history).
+    """Run or_bicgstab / or_gmres / or_ir / or_cg / or_bicg / or_cgs / or_qmr.
+    Returns (x, Stats, history).
     amg: AmgParams (or dict of overrides) when prec == "amg"."""
     n = len(rowptr) - 1
     x = np.zeros(n) if x0 is None else _f64(x0).copy()
@@ -338,7 +339,8 @@ def solve(kind, rowptr, colind, data, b, x0=None, *, max_iter=100000, rtol=1e-5,
     cap = max(1, min(cap, 10_000_000))
     hist = np.zeros(cap)
     fn = {"bicgstab": lib().or_bicgstab, "gmres": lib().or_gmres, "ir": lib().or_ir,
-          "cg": lib().or_cg}[kind]
+          "cg": lib().or_cg, "bicg": lib().or_bicg, "cgs": lib().or_cgs,
+          "qmr": lib().or_qmr}[kind]
     fn(n, _i32(rowptr), _i32(colind), _f64(data), _f64(b), x, C.byref(o), C.byref(s), hist, cap)
     return x, s, hist[: min(s.hist_len, cap)]
 

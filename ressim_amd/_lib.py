@@ -19,7 +19,7 @@ HIP_ERROR, OUT_OF_MEMORY, NOT_BUILT = -1, -2, -3
 
 PREC_IDENTITY, PREC_ILU0, PREC_BLOCK_ILU0, PREC_DIAGONAL, PREC_AMG, PREC_SSOR = 0, 1, 2, 3, 4, 5
 FP64, FP32 = 0, 1
-BICGSTAB, GMRES, IR, CG = 0, 1, 2, 3
+BICGSTAB, GMRES, IR, CG, BICG, CGS, QMR = 0, 1, 2, 3, 4, 5, 6
 BATCH_NO_GRAPH, BATCH_NO_AUTOTUNE = 1, 2
 
 

@@ -116,7 +116,20 @@ enum {
     FIN_INIT_NORM,   // monitor(||r||) at iteration 0 (BiCGstab, IR)
     FIN_GM_NORMR,    // GMRES: normr = ||r||; outer monitor
     FIN_CG_ALPHA,    // CG: alpha = rho / (p . q)
+    FIN_RHO_BREAK,   // BiCG, CGS: rho1 = t; breakdown if 0; beta = rho1 / rho2
+    FIN_QMR_RHO0,    // QMR: rho = ||y|| before the loop
+    FIN_QMR_XI0,     // QMR: xi = ||z|| before the loop
+    FIN_QMR_DELTA,   // QMR: delta = z . y; breakdown if 0; p / q scale factors
+    FIN_QMR_EP,      // QMR: ep = q . p_tld; beta = ep / delta; breakdowns
+    FIN_QMR_RHO,     // QMR: rho_1 = rho; rho = ||y||
+    FIN_QMR_XI,      // QMR: xi = ||z||; theta, gamma, eta; d / s scale factor
 };
+
+__device__ __forceinline__ void breakdown(DevState *st)
+{
+    st->status = RSGPU_BREAKDOWN;
+    st->done = 1;
+}
 
 __device__ void finish_scalar(DevState *st, int fin, double t1, double t2)
 {
@@ -141,6 +154,61 @@ __device__ void finish_scalar(DevState *st, int fin, double t1, double t2)
     case FIN_CG_ALPHA:
         st->alpha = st->rho1 / t1;
         break;
+    case FIN_RHO_BREAK:
+        st->rho1 = t1;
+        if (t1 == 0) {
+            breakdown(st);
+            break;
+        }
+        if (!st->first)
+            st->beta = st->rho1 / st->rho2;
+        break;
+    case FIN_QMR_RHO0:
+        st->q_rho = sqrt(t1);
+        break;
+    case FIN_QMR_XI0:
+        st->q_xi = sqrt(t1);
+        break;
+    case FIN_QMR_DELTA:
+        st->q_delta = t1;
+        if (t1 == 0) {
+            breakdown(st);
+            break;
+        }
+        if (!st->first) {
+            st->q_c1 = -st->q_xi * st->q_delta / st->q_ep;
+            st->q_c2 = -st->q_rho * st->q_delta / st->q_ep;
+        }
+        break;
+    case FIN_QMR_EP:
+        st->q_ep = t1;
+        if (t1 == 0) {
+            breakdown(st);
+            break;
+        }
+        st->q_beta = st->q_ep / st->q_delta;
+        if (st->q_beta == 0)
+            breakdown(st);
+        break;
+    case FIN_QMR_RHO:
+        st->q_rho1 = st->q_rho;
+        st->q_rho = sqrt(t1);
+        break;
+    case FIN_QMR_XI: {
+        st->q_xi = sqrt(t1);
+        st->q_gamma1 = st->q_gamma;
+        st->q_theta1 = st->q_theta;
+        st->q_theta = st->q_rho / (st->q_gamma1 * st->q_beta);
+        st->q_gamma = 1 / sqrt(1 + st->q_theta * st->q_theta);
+        if (st->q_gamma == 0) {
+            breakdown(st);
+            break;
+        }
+        st->q_eta = -st->q_eta * st->q_rho1 * st->q_gamma * st->q_gamma /
+                    (st->q_beta * st->q_gamma1 * st->q_gamma1);
+        st->q_val = st->q_theta1 * st->q_theta1 * st->q_gamma * st->q_gamma;
+        break;
+    }
     default:
         break;
     }
@@ -422,6 +490,272 @@ __global__ void __launch_bounds__(RS_VBS)
         if (dev_monitor(st, sqrt(tot[0])))
             st->done = 1;
     }
+}
+
+// =========================================================== BiCG, CGS, QMR kernels
+// (BiCG.java:74-121, CGS.java:78-121, QMR.java:136-243).  MTJ's vector
+// operations keep their special cases: scale(0) zeroes, scale(1) keeps,
+// add(0, y) and set(0, y) are a no-op and a zeroing.
+
+// dot(x, y) (norm: dot(x, x)) reduced and handed to finish_scalar
+__global__ void __launch_bounds__(RS_VBS)
+    k_dot_fin(int n, const double *__restrict__ x, const double *__restrict__ y, DevState *st,
+              RedSlot slot, int fin)
+{
+    if (st->done)
+        return;
+    double a = 0;
+    for (int i = blockIdx.x * RS_VBS + threadIdx.x; i < n; i += gridDim.x * RS_VBS)
+        a += x[i] * y[i];
+    double v[1] = {a}, tot[1];
+    if (block_reduce_finish<RS_VBS, 1>(slot, v, tot) && threadIdx.x == 0)
+        finish_scalar(st, fin, tot[0], 0.0);
+}
+
+// a.scale(beta).add(b) on two vector pairs, or a = b on the first iteration
+__global__ void __launch_bounds__(RS_VBS)
+    k_two_scale_add(int n, double *__restrict__ a1, const double *__restrict__ b1,
+                    double *__restrict__ a2, const double *__restrict__ b2,
+                    const DevState *st, const double *beta1, const double *beta2)
+{
+    if (st->done)
+        return;
+    const int i = blockIdx.x * RS_VBS + threadIdx.x;
+    if (i >= n)
+        return;
+    if (st->first) {
+        a1[i] = b1[i];
+        a2[i] = b2[i];
+        return;
+    }
+    const double c1 = *beta1, c2 = *beta2;
+    const double x1 = c1 == 0 ? 0.0 : (c1 == 1 ? a1[i] : c1 * a1[i]);
+    const double x2 = c2 == 0 ? 0.0 : (c2 == 1 ? a2[i] : c2 * a2[i]);
+    a1[i] = x1 + b1[i];
+    a2[i] = x2 + b2[i];
+}
+
+// BiCG: x += alpha p; r += -alpha q; rtilde += -alpha qtilde; rho2 = rho1;
+// monitor(||r||)
+__global__ void __launch_bounds__(RS_VBS)
+    k_bicg_upd(int n, const double *__restrict__ p, const double *__restrict__ q,
+               const double *__restrict__ qt, double *__restrict__ x, double *__restrict__ r,
+               double *__restrict__ rt, DevState *st, RedSlot slot)
+{
+    if (st->done)
+        return;
+    const double al = st->alpha, ma = -st->alpha;
+    double a = 0;
+    for (int i = blockIdx.x * RS_VBS + threadIdx.x; i < n; i += gridDim.x * RS_VBS) {
+        if (al != 0)
+            x[i] += al * p[i];
+        double ri = r[i];
+        if (ma != 0) {
+            ri += ma * q[i];
+            rt[i] += ma * qt[i];
+        }
+        r[i] = ri;
+        a += ri * ri;
+    }
+    double v[1] = {a}, tot[1];
+    if (block_reduce_finish<RS_VBS, 1>(slot, v, tot) && threadIdx.x == 0) {
+        st->rho2 = st->rho1;
+        st->first = 0;
+        st->iter++;
+        if (dev_monitor(st, sqrt(tot[0])))
+            st->done = 1;
+    }
+}
+
+// CGS: first u = r, p = u; else u = r + beta q; sum = q + beta p;
+// p = u + beta sum (set then add, add(0, .) a no-op)
+__global__ void __launch_bounds__(RS_VBS)
+    k_cgs_ups(int n, const double *__restrict__ r, const double *__restrict__ q,
+              double *__restrict__ u, double *__restrict__ sum, double *__restrict__ p,
+              const DevState *st)
+{
+    if (st->done)
+        return;
+    const int i = blockIdx.x * RS_VBS + threadIdx.x;
+    if (i >= n)
+        return;
+    if (st->first) {
+        u[i] = r[i];
+        p[i] = r[i];
+        return;
+    }
+    const double beta = st->beta;
+    double ui = r[i], si = q[i], pi = p[i];
+    if (beta != 0) {
+        ui += beta * q[i];
+        si += beta * pi;
+    }
+    pi = ui;
+    if (beta != 0)
+        pi += beta * si;
+    u[i] = ui;
+    sum[i] = si;
+    p[i] = pi;
+}
+
+// CGS: q = -alpha vhat + u (set(-alpha, vhat).add(u)); sum = u + q
+__global__ void __launch_bounds__(RS_VBS)
+    k_cgs_qsum(int n, const double *__restrict__ vh, const double *__restrict__ u,
+               double *__restrict__ q, double *__restrict__ sum, const DevState *st)
+{
+    if (st->done)
+        return;
+    const int i = blockIdx.x * RS_VBS + threadIdx.x;
+    if (i >= n)
+        return;
+    const double ma = -st->alpha;
+    double qi = ma == 0 ? 0.0 : ma * vh[i];
+    qi += u[i];
+    q[i] = qi;
+    sum[i] = u[i] + qi;
+}
+
+// CGS: x += alpha uhat; r += -alpha qhat; rho2 = rho1; monitor(||r||)
+__global__ void __launch_bounds__(RS_VBS)
+    k_cgs_upd(int n, const double *__restrict__ uh, const double *__restrict__ qh,
+              double *__restrict__ x, double *__restrict__ r, DevState *st, RedSlot slot)
+{
+    if (st->done)
+        return;
+    const double al = st->alpha, ma = -st->alpha;
+    double a = 0;
+    for (int i = blockIdx.x * RS_VBS + threadIdx.x; i < n; i += gridDim.x * RS_VBS) {
+        if (al != 0)
+            x[i] += al * uh[i];
+        double ri = r[i];
+        if (ma != 0)
+            ri += ma * qh[i];
+        r[i] = ri;
+        a += ri * ri;
+    }
+    double v[1] = {a}, tot[1];
+    if (block_reduce_finish<RS_VBS, 1>(slot, v, tot) && threadIdx.x == 0) {
+        st->rho2 = st->rho1;
+        st->first = 0;
+        st->iter++;
+        if (dev_monitor(st, sqrt(tot[0])))
+            st->done = 1;
+    }
+}
+
+__global__ void __launch_bounds__(RS_VBS)
+    k_copy2(int n, const double *__restrict__ src, double *__restrict__ d1,
+            double *__restrict__ d2, const DevState *st)
+{
+    if (st->done)
+        return;
+    const int i = blockIdx.x * RS_VBS + threadIdx.x;
+    if (i < n) {
+        d1[i] = src[i];
+        d2[i] = src[i];
+    }
+}
+
+// QMR loop head: breakdown on rho or xi == 0; v = (1/rho) v_tld;
+// y.scale(1/rho); w = (1/xi) w_tld; z.scale(1/xi); delta = z . y
+__global__ void __launch_bounds__(RS_VBS)
+    k_qmr_head(int n, const double *__restrict__ vt, const double *__restrict__ wt,
+               double *__restrict__ v, double *__restrict__ y, double *__restrict__ w,
+               double *__restrict__ z, DevState *st, RedSlot slot)
+{
+    if (st->done)
+        return;
+    const double rho = st->q_rho, xi = st->q_xi;
+    if (rho == 0 || xi == 0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+            breakdown(st);
+        return;
+    }
+    const double ir = 1 / rho, ix = 1 / xi;
+    double a = 0;
+    for (int i = blockIdx.x * RS_VBS + threadIdx.x; i < n; i += gridDim.x * RS_VBS) {
+        v[i] = ir == 0 ? 0.0 : ir * vt[i];
+        const double yi = ir == 0 ? 0.0 : (ir == 1 ? y[i] : ir * y[i]);
+        y[i] = yi;
+        w[i] = ix == 0 ? 0.0 : ix * wt[i];
+        const double zi = ix == 0 ? 0.0 : (ix == 1 ? z[i] : ix * z[i]);
+        z[i] = zi;
+        a += zi * yi;
+    }
+    double vv[1] = {a}, tot[1];
+    if (block_reduce_finish<RS_VBS, 1>(slot, vv, tot) && threadIdx.x == 0)
+        finish_scalar(st, FIN_QMR_DELTA, tot[0], 0.0);
+}
+
+// QMR: v_tld = -beta v + p_tld (set then add); w_tld = -beta w
+__global__ void __launch_bounds__(RS_VBS)
+    k_qmr_vw(int n, const double *__restrict__ v, const double *__restrict__ pt,
+             const double *__restrict__ w, double *__restrict__ vt, double *__restrict__ wt,
+             const DevState *st)
+{
+    if (st->done)
+        return;
+    const int i = blockIdx.x * RS_VBS + threadIdx.x;
+    if (i >= n)
+        return;
+    const double mb = -st->q_beta;
+    double vi = mb == 0 ? 0.0 : mb * v[i];
+    vt[i] = vi + pt[i];
+    wt[i] = mb == 0 ? 0.0 : mb * w[i];
+}
+
+// QMR: first d = eta p, s = eta p_tld; else d.scale(val).add(eta, p),
+// s likewise; x += d; r += -1 s; monitor(||r||)
+__global__ void __launch_bounds__(RS_VBS)
+    k_qmr_upd(int n, const double *__restrict__ p, const double *__restrict__ pt,
+              double *__restrict__ d, double *__restrict__ s, double *__restrict__ x,
+              double *__restrict__ r, DevState *st, RedSlot slot)
+{
+    if (st->done)
+        return;
+    const double eta = st->q_eta, val = st->q_val;
+    const bool first = st->first != 0;
+    double a = 0;
+    for (int i = blockIdx.x * RS_VBS + threadIdx.x; i < n; i += gridDim.x * RS_VBS) {
+        double di, si;
+        if (first) {
+            di = eta == 0 ? 0.0 : eta * p[i];
+            si = eta == 0 ? 0.0 : eta * pt[i];
+        } else {
+            di = val == 0 ? 0.0 : (val == 1 ? d[i] : val * d[i]);
+            si = val == 0 ? 0.0 : (val == 1 ? s[i] : val * s[i]);
+            if (eta != 0) {
+                di += eta * p[i];
+                si += eta * pt[i];
+            }
+        }
+        d[i] = di;
+        s[i] = si;
+        x[i] += di;
+        const double ri = r[i] + -1.0 * si;
+        r[i] = ri;
+        a += ri * ri;
+    }
+    double v[1] = {a}, tot[1];
+    if (block_reduce_finish<RS_VBS, 1>(slot, v, tot) && threadIdx.x == 0) {
+        st->first = 0;
+        st->iter++;
+        if (dev_monitor(st, sqrt(tot[0])))
+            st->done = 1;
+    }
+}
+
+// y = A' x (add = 0) or y = y + A' x (add = 1, transMultAdd with alpha 1) on
+// the transposed CSR, skipped when the solve has stopped
+template <class Epi>
+__global__ void __launch_bounds__(RS_SPMV_BS)
+    k_spmvt_skip(const int *__restrict__ wg_rows, const int *__restrict__ rowptr,
+                 const int *__restrict__ colind, const double *__restrict__ val,
+                 const double *__restrict__ x, Epi epi, const DevState *st)
+{
+    if (st->done)
+        return;
+    spmv_core<double, RS_SPMV_BS, RS_SPMV_NNZ>(wg_rows, rowptr, colind, val, x, epi);
 }
 
 // =========================================================== GMRES kernels
@@ -925,6 +1259,8 @@ void state_init(rsgpu_solver *S, DevState *dst, DevBuf<double> &hist,
     h.max_iter = max_iter;
     h.first = 1;
     h.rho1 = h.rho2 = h.alpha = h.beta = h.omega = 1;
+    h.q_gamma = 1;     // QMR.java:138-139 initial values
+    h.q_eta = -1;
     h.hist = hist.p;
     h.hist_cap = (int)hist.n;
     h.m = m;
@@ -1146,6 +1482,169 @@ void run_cg(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const T *b, T *x, T *w
     P.wait_last(done, cend);
 }
 
+// ------------------------------------------------------------ BiCG, CGS, QMR
+// M.transApply (the solve's apply when the solve is the transposed one)
+static void ptapply(rsgpu_solver *S, rsgpu_prec *M, const double *b, double *x,
+                    const DevState *st)
+{
+    if (S->o.prec_transpose)
+        prec_apply(M, b, x, st);
+    else
+        prec_trans_apply(M, b, x, st);
+}
+
+// transposed CSR of A with current values (once per solve)
+static void prepare_transpose(rsgpu_csr *A)
+{
+    csr_build_transpose(A);
+    if (A->nnz)
+        csr_launch_gather(A->ctx->stream, A->tperm.p, A->val.p, A->tval.p, A->nnz);
+}
+
+// y = A' x, or y = y + A' x (transMultAdd, alpha 1)
+static void launch_spmvt(rsgpu_solver *S, rsgpu_csr *A, const double *x, double *y, bool add,
+                         const DevState *st)
+{
+    if (A->ncols == 0)
+        return;
+    hipStream_t s = S->ctx->stream;
+    const dim3 g(A->tplan.nwg), b(RS_SPMV_BS);
+    if (!add)
+        hipLaunchKernelGGL((k_spmvt_skip<EpiPlain<double>>), g, b, 0, s, A->tplan.wg_rows.p,
+                           A->trowptr.p, A->tcolind.p, A->tval.p, x, EpiPlain<double>{y}, st);
+    else
+        hipLaunchKernelGGL((k_spmvt_skip<EpiTransAdd<double>>), g, b, 0, s,
+                           A->tplan.wg_rows.p, A->trowptr.p, A->tcolind.p, A->tval.p, x,
+                           EpiTransAdd<double>{y, 1.0, 1.0}, st);
+    RS_CHECK_LAUNCH();
+}
+
+// BiCG.java:74-121; work: r rtilde z ztilde p ptilde q qtilde
+void run_bicg(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const double *b, double *x,
+              double *work, DevState *st)
+{
+    hipStream_t s = S->ctx->stream;
+    const int n = A->n;
+    const size_t L = S->ldv;
+    double *r = work, *rt = work + L, *z = work + 2 * L, *zt = work + 3 * L;
+    double *p = work + 4 * L, *pt = work + 5 * L, *q = work + 6 * L, *qt = work + 7 * L;
+    const int G = grid_for(n), E = (n + RS_VBS - 1) / RS_VBS;
+    RedSlot sl = slot_of(S);
+    prepare_transpose(A);
+    launch_resid<double>(S, A, b, x, r, rt, nullptr, st, FIN_INIT_NORM);
+    Poller P{S, st};
+    int done = 0, cend = 0;
+    for (;;) {
+        papply(S, M, r, z, st);
+        ptapply(S, M, rt, zt, st);
+        hipLaunchKernelGGL(k_dot_fin, dim3(G), dim3(RS_VBS), 0, s, n, z, rt, st, sl,
+                           (int)FIN_RHO_BREAK);
+        hipLaunchKernelGGL(k_two_scale_add, dim3(E), dim3(RS_VBS), 0, s, n, p, z, pt, zt, st,
+                           &st->beta, &st->beta);
+        launch_spmv_red<double>(S, A, p, EpiMultDot<double>{q, pt, nullptr, 1}, st,
+                                FIN_CG_ALPHA, 0);
+        launch_spmvt(S, A, pt, qt, false, st);
+        hipLaunchKernelGGL(k_bicg_upd, dim3(G), dim3(RS_VBS), 0, s, n, p, q, qt, x, r, rt, st,
+                           sl);
+        RS_CHECK_LAUNCH();
+        P.post();
+        P.wait_prev(done, cend);
+        if (done)
+            break;
+    }
+    P.wait_last(done, cend);
+}
+
+// CGS.java:78-121; work: r rtilde u p phat vhat q sum uhat qhat
+void run_cgs(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const double *b, double *x,
+             double *work, DevState *st)
+{
+    hipStream_t s = S->ctx->stream;
+    const int n = A->n;
+    const size_t L = S->ldv;
+    double *r = work, *rt = work + L, *u = work + 2 * L, *p = work + 3 * L;
+    double *ph = work + 4 * L, *vh = work + 5 * L, *q = work + 6 * L, *sum = work + 7 * L;
+    double *uh = work + 8 * L, *qh = work + 9 * L;
+    const int G = grid_for(n), E = (n + RS_VBS - 1) / RS_VBS;
+    RedSlot sl = slot_of(S);
+    launch_resid<double>(S, A, b, x, r, rt, nullptr, st, FIN_INIT_NORM);
+    Poller P{S, st};
+    int done = 0, cend = 0;
+    for (;;) {
+        hipLaunchKernelGGL(k_dot_fin, dim3(G), dim3(RS_VBS), 0, s, n, rt, r, st, sl,
+                           (int)FIN_RHO_BREAK);
+        hipLaunchKernelGGL(k_cgs_ups, dim3(E), dim3(RS_VBS), 0, s, n, r, q, u, sum, p, st);
+        papply(S, M, p, ph, st);
+        launch_spmv_red<double>(S, A, ph, EpiMultDot<double>{vh, rt, nullptr, 1}, st,
+                                FIN_CG_ALPHA, 0);
+        hipLaunchKernelGGL(k_cgs_qsum, dim3(E), dim3(RS_VBS), 0, s, n, vh, u, q, sum, st);
+        papply(S, M, sum, uh, st);
+        launch_spmv_skip<double>(S, A, uh, qh, st);
+        hipLaunchKernelGGL(k_cgs_upd, dim3(G), dim3(RS_VBS), 0, s, n, uh, qh, x, r, st, sl);
+        RS_CHECK_LAUNCH();
+        P.post();
+        P.wait_prev(done, cend);
+        if (done)
+            break;
+    }
+    P.wait_last(done, cend);
+}
+
+// QMR.java:136-243 with M1 = M2 = M; work: r y z v w p q d s v_tld w_tld
+// y_tld z_tld p_tld
+void run_qmr(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const double *b, double *x,
+             double *work, DevState *st)
+{
+    hipStream_t s = S->ctx->stream;
+    const int n = A->n;
+    const size_t L = S->ldv;
+    double *V[14];
+    for (int k = 0; k < 14; ++k)
+        V[k] = work + (size_t)k * L;
+    double *r = V[0], *y = V[1], *z = V[2], *v = V[3], *w = V[4], *p = V[5], *q = V[6];
+    double *d = V[7], *sv = V[8], *vt = V[9], *wt = V[10], *yt = V[11], *zt = V[12],
+           *pt = V[13];
+    const int G = grid_for(n), E = (n + RS_VBS - 1) / RS_VBS;
+    RedSlot sl = slot_of(S);
+    prepare_transpose(A);
+    launch_resid<double>(S, A, b, x, r, nullptr, nullptr, st, FIN_INIT_NORM);
+    hipLaunchKernelGGL(k_copy2, dim3(E), dim3(RS_VBS), 0, s, n, r, vt, wt, st);
+    papply(S, M, vt, y, st);
+    hipLaunchKernelGGL(k_dot_fin, dim3(G), dim3(RS_VBS), 0, s, n, y, y, st, sl,
+                       (int)FIN_QMR_RHO0);
+    ptapply(S, M, wt, z, st);
+    hipLaunchKernelGGL(k_dot_fin, dim3(G), dim3(RS_VBS), 0, s, n, z, z, st, sl,
+                       (int)FIN_QMR_XI0);
+    Poller P{S, st};
+    int done = 0, cend = 0;
+    for (;;) {
+        hipLaunchKernelGGL(k_qmr_head, dim3(G), dim3(RS_VBS), 0, s, n, vt, wt, v, y, w, z, st,
+                           sl);
+        papply(S, M, y, yt, st);
+        ptapply(S, M, z, zt, st);
+        hipLaunchKernelGGL(k_two_scale_add, dim3(E), dim3(RS_VBS), 0, s, n, p, yt, q, zt, st,
+                           &st->q_c1, &st->q_c2);
+        launch_spmv_red<double>(S, A, p, EpiMultDot<double>{pt, q, nullptr, 1}, st,
+                                FIN_QMR_EP, 0);
+        hipLaunchKernelGGL(k_qmr_vw, dim3(E), dim3(RS_VBS), 0, s, n, v, pt, w, vt, wt, st);
+        papply(S, M, vt, y, st);
+        hipLaunchKernelGGL(k_dot_fin, dim3(G), dim3(RS_VBS), 0, s, n, y, y, st, sl,
+                           (int)FIN_QMR_RHO);
+        launch_spmvt(S, A, q, wt, true, st);
+        ptapply(S, M, wt, z, st);
+        hipLaunchKernelGGL(k_dot_fin, dim3(G), dim3(RS_VBS), 0, s, n, z, z, st, sl,
+                           (int)FIN_QMR_XI);
+        hipLaunchKernelGGL(k_qmr_upd, dim3(G), dim3(RS_VBS), 0, s, n, p, pt, d, sv, x, r, st,
+                           sl);
+        RS_CHECK_LAUNCH();
+        P.post();
+        P.wait_prev(done, cend);
+        if (done)
+            break;
+    }
+    P.wait_last(done, cend);
+}
+
 // ------------------------------------------------------------ GMRES(m)
 // work: w u r + (m+1) basis vectors
 template <typename T>
@@ -1246,8 +1745,12 @@ extern "C" int rsgpu_solver_create(rsgpu_ctx *ctx, const rsgpu_solver_opts *opts
 {
     if (!ctx || !opts || !out || n < 0)
         return RSGPU_BAD_ARGUMENT;
-    if (opts->kind < 0 || opts->kind > 3)
+    if (opts->kind < 0 || opts->kind > RSGPU_QMR)
         return RSGPU_BAD_ARGUMENT;
+    if (opts->mixed && opts->kind > RSGPU_CG) {
+        ctx->err = "the mixed form's inner solver is BiCGstab, GMRES or CG";
+        return RSGPU_BAD_ARGUMENT;
+    }
     if (opts->kind == RSGPU_GMRES && opts->restart <= 0) {
         ctx->err = "restart must be a positive integer";
         return RSGPU_BAD_ARGUMENT;
@@ -1276,6 +1779,9 @@ extern "C" int rsgpu_solver_create(rsgpu_ctx *ctx, const rsgpu_solver_opts *opts
             nvec64 = opts->kind == RSGPU_BICGSTAB ? 8
                      : opts->kind == RSGPU_GMRES  ? 3 + (size_t)m + 1
                      : opts->kind == RSGPU_CG     ? 4
+                     : opts->kind == RSGPU_BICG   ? 8
+                     : opts->kind == RSGPU_CGS    ? 10
+                     : opts->kind == RSGPU_QMR    ? 14
                                                   : 2;
         } else {
             nvec64 = 2;   // r, scratch
@@ -1335,7 +1841,16 @@ void run_kind(rsgpu_solver *S, int kind, rsgpu_csr *A, rsgpu_prec *M, const T *b
         run_cg<T>(S, A, M, b, x, work, st);
     else if (kind == RSGPU_GMRES)
         run_gmres<T>(S, A, M, b, x, work, st, m);
-    else
+    else if constexpr (sizeof(T) == 8) {
+        if (kind == RSGPU_BICG)
+            run_bicg(S, A, M, b, x, work, st);
+        else if (kind == RSGPU_CGS)
+            run_cgs(S, A, M, b, x, work, st);
+        else if (kind == RSGPU_QMR)
+            run_qmr(S, A, M, b, x, work, st);
+        else
+            throw RsError{RSGPU_BAD_ARGUMENT, "unsupported solver"};
+    } else
         throw RsError{RSGPU_BAD_ARGUMENT, "unsupported inner solver"};
 }
 

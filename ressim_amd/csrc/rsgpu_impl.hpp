@@ -263,6 +263,10 @@ struct DevState {
     double *H, *s, *gc, *gs;
     int m;
     int pad1;
+    // QMR (QMR.java:136-243); c1, c2: the p / q scale factors, val: the d / s
+    // scale factor of the coming update
+    double q_rho, q_rho1, q_xi, q_gamma, q_gamma1, q_theta, q_theta1, q_eta, q_delta, q_ep,
+        q_beta, q_c1, q_c2, q_val;
 };
 
 // ---------------------------------------------------------------- reductions

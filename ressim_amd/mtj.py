@@ -679,6 +679,22 @@ class CG(AbstractIterativeSolver):
     kind = L.CG
 
 
+class BiCG(AbstractIterativeSolver):
+    """mtj/.../sparse/BiCG.java (M.apply and M.transApply, A.transMult)."""
+    kind = L.BICG
+
+
+class CGS(AbstractIterativeSolver):
+    """mtj/.../sparse/CGS.java."""
+    kind = L.CGS
+
+
+class QMR(AbstractIterativeSolver):
+    """mtj/.../sparse/QMR.java with M1 = M2 = the preconditioner
+    (QMR.setPreconditioner)."""
+    kind = L.QMR
+
+
 class GMRES(AbstractIterativeSolver):
     """mtj/.../sparse/GMRES.java; restart 30 by default (:84-86)."""
     kind = L.GMRES

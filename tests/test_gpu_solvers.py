@@ -232,3 +232,30 @@ def test_ssor_apply_bit_exact(gpu, oracle, block_rows, ordering):
                             block_rows=block_rows, ordering=ordering)
     assert np.array_equal(M.apply(b, x0.copy()), ref)
     assert np.array_equal(M.transApply(b, x0.copy()), ref)
+
+
+@pytest.mark.parametrize("kind,prec,name", [
+    ("bicg", "ilu", "c1"), ("bicg", "diagonal", "c1"), ("cgs", "ilu", "s3"),
+    ("cgs", "block_ilu", "s3"), ("cgs", "diagonal", "c1"), ("qmr", "ilu", "c1"),
+    ("qmr", "none", "s3"), ("qmr", "block_ilu", "c1")])
+def test_bicg_cgs_qmr_vs_oracle(gpu, oracle, request, kind, prec, name):
+    """BiCG, CGS and QMR (M.transApply and the transposed SpMV on the
+    device in BiCG and QMR) against the oracle's restatement of BiCG.java,
+    CGS.java and QMR.java (CGS with ILU stagnates above 1e-12 on c1 in the
+    oracle too, so it runs on s3)."""
+    s = request.getfixturevalue(name)
+    print(f"{kind} + {prec} on {name}", flush=True)
+    rp, ci, d, b = s["rowptr"], s["colind"], s["data"], s["b"]
+    n = len(rp) - 1
+    xo, so, ho = oracle.solve(kind, rp, ci, d, b, rtol=1e-12, prec=prec, nblocks=8,
+                              max_iter=20000, dtol=1e30)
+    assert so.status == 0
+    solver = {"bicg": gpu.BiCG, "cgs": gpu.CGS, "qmr": gpu.QMR}[kind](b)
+    solver.setIterationMonitor(gpu.DefaultIterationMonitor(20000, 1e-12, 1e-50, 1e30))
+    M = {"none": None, "ilu": gpu.ILU(), "diagonal": gpu.DiagonalPreconditioner(n),
+         "block_ilu": gpu.BlockILU(8)}[prec]
+    x, _ = _run(gpu, s, solver, M)
+    _check(s, x, xo)
+    it = solver.getIterationMonitor()
+    assert abs(it.iterations() - so.iterations) <= max(3, so.iterations // 10)
+    np.testing.assert_allclose(it.history[:3], ho[:3], rtol=1e-9)
