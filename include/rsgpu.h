@@ -133,7 +133,8 @@ enum rsgpu_prec_kind {
     RSGPU_PREC_BLOCK_ILU0 = 2, /* distributed/BlockDiagonalPreconditioner + ILU  */
     RSGPU_PREC_DIAGONAL = 3,   /* mtj/sparse/DiagonalPreconditioner.java         */
     RSGPU_PREC_AMG = 4,        /* mtj/sparse/AMG.java (smoothed aggregation)     */
-    RSGPU_PREC_SSOR = 5        /* mtj/sparse/SSOR.java                           */
+    RSGPU_PREC_SSOR = 5,       /* mtj/sparse/SSOR.java                           */
+    RSGPU_PREC_ICC = 6         /* mtj/sparse/ICC.java (FP64)                     */
 };
 enum rsgpu_precision { RSGPU_FP64 = 0, RSGPU_FP32 = 1 };
 

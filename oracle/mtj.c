@@ -296,6 +296,77 @@ int or_ilu_factor_f32(int n, const int *rowptr, const int *colind, float *lu,
     ILU_FACTOR_BODY(float)
 }
 
+/* ---- ICC (ICC.java:71-125) ---------------------------------------------- */
+/* R.set(A) then factor(): a dense work row Rk per row k, eliminated against
+ * every earlier row i (skipped when Rk[i] / R(i,i) == 0) with row i's stored
+ * upper entries; row k's entries from the diagonal on become Rk / sqrt(Rk[k]).
+ * Entries left of the diagonal keep A's values.  bad_row: the 1-based row of
+ * the error (as the reference's messages). */
+int or_icc_factor(int n, const int *rowptr, const int *colind, double *data, int *diagind,
+                  int *bad_row)
+{
+    for (int k = 0; k < n; ++k) {
+        int d = or_binary_search(colind, k, rowptr[k], rowptr[k + 1]);
+        if (d < 0) {
+            if (bad_row)
+                *bad_row = k + 1;
+            return OR_MISSING_DIAGONAL;
+        }
+        diagind[k] = d;
+    }
+    double *Rk = (double *)malloc(sizeof(double) * (n > 0 ? n : 1));
+    for (int k = 0; k < n; ++k) {
+        for (int i = 0; i < n; ++i)
+            Rk[i] = 0;
+        for (int i = rowptr[k]; i < rowptr[k + 1]; ++i)
+            Rk[colind[i]] = data[i];
+        for (int i = 0; i < k; ++i) {
+            const double Rii = data[diagind[i]];
+            if (Rii == 0) {
+                if (bad_row)
+                    *bad_row = i + 1;
+                free(Rk);
+                return OR_ZERO_PIVOT;
+            }
+            const double Rki = Rk[i] / Rii;
+            if (Rki == 0)
+                continue;
+            for (int j = diagind[i] + 1; j < rowptr[i + 1]; ++j)
+                Rk[colind[j]] -= Rki * data[j];
+        }
+        if (Rk[k] == 0) {
+            if (bad_row)
+                *bad_row = k + 1;
+            free(Rk);
+            return OR_ZERO_PIVOT;
+        }
+        const double sqRkk = sqrt(Rk[k]);
+        for (int i = diagind[k]; i < rowptr[k + 1]; ++i)
+            data[i] = Rk[colind[i]] / sqRkk;
+    }
+    free(Rk);
+    return OR_OK;
+}
+
+/* ICC.apply = transApply: UpperCompRowMatrix.transSolve (:70-90) then
+ * UpperCompRowMatrix.solve (:49-67) with R's upper part. */
+void or_icc_apply(int n, const int *rowptr, const int *colind, const double *R,
+                  const int *diagind, const double *b, double *x, double *y)
+{
+    memcpy(y, b, sizeof(double) * n);
+    for (int i = 0; i < n; ++i) {
+        y[i] /= R[diagind[i]];
+        for (int j = diagind[i] + 1; j < rowptr[i + 1]; ++j)
+            y[colind[j]] -= R[j] * y[i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        double sum = 0;
+        for (int j = diagind[i] + 1; j < rowptr[i + 1]; ++j)
+            sum += R[j] * x[colind[j]];
+        x[i] = (y[i] - sum) / R[diagind[i]];
+    }
+}
+
 /* ILU.apply: UnitLowerCompRowMatrix.solve (:49-68) then
  * UpperCompRowMatrix.solve (:49-68). */
 #define ILU_APPLY_BODY(T)                                                     \
@@ -478,6 +549,17 @@ static int prec_setup(prec_t *p, int n, const int *rowptr, const int *colind,
     }
     int nnz;
     p->rowptr = (int *)malloc(sizeof(int) * (n + 1));
+    if (p->kind == 6) {
+        /* ICC: R = copy of A, factored in place */
+        nnz = rowptr[n];
+        p->colind = (int *)malloc(sizeof(int) * (nnz > 0 ? nnz : 1));
+        p->lu = (double *)malloc(sizeof(double) * (nnz > 0 ? nnz : 1));
+        memcpy(p->rowptr, rowptr, sizeof(int) * (n + 1));
+        memcpy(p->colind, colind, sizeof(int) * nnz);
+        memcpy(p->lu, data, sizeof(double) * nnz);
+        p->diagind = (int *)malloc(sizeof(int) * (n > 0 ? n : 1));
+        return or_icc_factor(n, p->rowptr, p->colind, p->lu, p->diagind, bad_row);
+    }
     if (p->kind == 1) {
         nnz = rowptr[n];
         p->colind = (int *)malloc(sizeof(int) * (nnz > 0 ? nnz : 1));
@@ -524,6 +606,10 @@ static void prec_apply(prec_t *p, const double *b, double *x)
         /* SSOR.apply starts from x's old content (SSOR.java:143-194) */
         or_ssor_apply(p->n, p->arp, p->aci, p->av, p->sp.block_rows, p->sp.ordering,
                       p->sp.omega_pre_f, p->sp.omega_pre_r, p->sp.reverse, b, x, NULL);
+        return;
+    }
+    if (p->kind == 6) {
+        or_icc_apply(p->n, p->rowptr, p->colind, p->lu, p->diagind, b, x, p->y);
         return;
     }
     or_ilu_apply(p->n, p->rowptr, p->colind, p->lu, p->diagind, b, x, p->y);

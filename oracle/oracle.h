@@ -142,7 +142,7 @@ typedef struct {
     int restart;        /* GMRES(m) */
     /* preconditioner: 0 identity, 1 ILU(0), 2 block-Jacobi ILU(0), 3 AMG,
      * 4 diagonal, 5 SSOR (parameters in amg: omega_pre_f/r, reverse,
-     * block_rows, ordering) */
+     * block_rows, ordering), 6 ICC */
     int prec_kind;
     int nblocks;        /* for block-Jacobi, contiguous equal row ranges */
     or_amg_params amg;  /* for AMG */
@@ -169,6 +169,10 @@ int or_cg(int n, const int *rowptr, const int *colind, const double *data,
 int or_ir(int n, const int *rowptr, const int *colind, const double *data,
           const double *b, double *x, const or_opts *opts, or_stats *stats,
           double *hist, int hist_cap);
+int or_icc_factor(int n, const int *rowptr, const int *colind, double *data, int *diagind,
+                  int *bad_row);
+void or_icc_apply(int n, const int *rowptr, const int *colind, const double *R,
+                  const int *diagind, const double *b, double *x, double *y);
 int or_bicg(int n, const int *rowptr, const int *colind, const double *data,
             const double *b, double *x, const or_opts *opts, or_stats *stats,
             double *hist, int hist_cap);

@@ -119,6 +119,9 @@ def lib():
         for f in (L.or_bicgstab, L.or_gmres, L.or_ir, L.or_cg, L.or_bicg, L.or_cgs, L.or_qmr):
             f.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.POINTER(Opts),
                           C.POINTER(Stats), _f64p, C.c_int]
+        L.or_icc_factor.argtypes = [C.c_int, _i32p, _i32p, _f64p, _i32p, C.c_void_p]
+        L.or_icc_apply.argtypes = [C.c_int, _i32p, _i32p, _f64p, _i32p, _f64p, _f64p, _f64p]
+        L.or_icc_apply.restype = None
         L.or_dgesv.argtypes = [C.c_int, C.c_int, _f64p, C.c_int, _i32p, _f64p, C.c_int]
         L.or_amg_params_default.argtypes = [C.c_void_p]
         L.or_amg_setup.argtypes = [C.c_int, _i32p, _i32p, _f64p, C.c_void_p, C.c_void_p,
@@ -298,6 +301,25 @@ def ilu_transapply(rowptr, colind, lu, diag, b):
     return x
 
 
+def icc_factor(rowptr, colind, data):
+    """ICC.setMatrix: (status, R values on the same pattern, diagind, bad_row
+    1-based)."""
+    n = len(rowptr) - 1
+    R = _f64(data).copy()
+    dg = np.zeros(max(n, 1), np.int32)
+    bad = C.c_int(-1)
+    st = lib().or_icc_factor(n, _i32(rowptr), _i32(colind), R, dg, C.byref(bad))
+    return st, R, dg[:n], bad.value
+
+
+def icc_apply(rowptr, colind, R, diag, b):
+    n = len(rowptr) - 1
+    x = np.zeros(n)
+    y = np.zeros(n)
+    lib().or_icc_apply(n, _i32(rowptr), _i32(colind), _f64(R), _i32(diag), _f64(b), x, y)
+    return x
+
+
 def block_starts(n, nblocks):
     bs = np.zeros(nblocks + 1, np.int32)
     st = lib().or_block_starts(n, nblocks, bs)
@@ -321,7 +343,7 @@ def block_filter(rowptr, colind, data, nblocks):
 
 # ---------------------------------------------------------------- solvers
 PREC = {"none": 0, "identity": 0, "ilu": 1, "block_ilu": 2, "amg": 3, "diagonal": 4,
-        "ssor": 5}
+        "ssor": 5, "icc": 6}
 
 
 def solve(kind, rowptr, colind, data, b, x0=None, *, max_iter=100000, rtol=1e-5, atol=1e-50,

@@ -17,7 +17,8 @@ BREAKDOWN, DIVERGENCE, ITERATIONS = 1, 2, 3
 ZERO_PIVOT, MISSING_DIAGONAL, NOT_IN_PATTERN, BAD_ARGUMENT, SINGULAR = 10, 11, 12, 13, 14
 HIP_ERROR, OUT_OF_MEMORY, NOT_BUILT = -1, -2, -3
 
-PREC_IDENTITY, PREC_ILU0, PREC_BLOCK_ILU0, PREC_DIAGONAL, PREC_AMG, PREC_SSOR = 0, 1, 2, 3, 4, 5
+PREC_IDENTITY, PREC_ILU0, PREC_BLOCK_ILU0, PREC_DIAGONAL, PREC_AMG, PREC_SSOR, PREC_ICC = (
+    0, 1, 2, 3, 4, 5, 6)
 FP64, FP32 = 0, 1
 BICGSTAB, GMRES, IR, CG, BICG, CGS, QMR = 0, 1, 2, 3, 4, 5, 6
 BATCH_NO_GRAPH, BATCH_NO_AUTOTUNE = 1, 2

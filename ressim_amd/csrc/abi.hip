@@ -534,14 +534,15 @@ extern "C" int rsgpu_prec_create(rsgpu_ctx *ctx, int kind, int nblocks,
         return RSGPU_BAD_ARGUMENT;
     if (kind != RSGPU_PREC_IDENTITY && kind != RSGPU_PREC_ILU0 &&
         kind != RSGPU_PREC_BLOCK_ILU0 && kind != RSGPU_PREC_AMG &&
-        kind != RSGPU_PREC_DIAGONAL && kind != RSGPU_PREC_SSOR) {
+        kind != RSGPU_PREC_DIAGONAL && kind != RSGPU_PREC_SSOR && kind != RSGPU_PREC_ICC) {
         ctx->err = "preconditioner kind not available";
         return RSGPU_BAD_ARGUMENT;
     }
     if (precision != RSGPU_FP64 && precision != RSGPU_FP32)
         return RSGPU_BAD_ARGUMENT;
-    if ((kind == RSGPU_PREC_AMG || kind == RSGPU_PREC_SSOR) && precision != RSGPU_FP64) {
-        ctx->err = "AMG and SSOR run in FP64";
+    if ((kind == RSGPU_PREC_AMG || kind == RSGPU_PREC_SSOR || kind == RSGPU_PREC_ICC) &&
+        precision != RSGPU_FP64) {
+        ctx->err = "AMG, SSOR and ICC run in FP64";
         return RSGPU_BAD_ARGUMENT;
     }
     if (kind == RSGPU_PREC_BLOCK_ILU0 && nblocks < 1)
@@ -677,6 +678,24 @@ extern "C" int rsgpu_prec_set_matrix(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
                                                     : diag_setup(M, A, bad_row);
         M->ready = st == RSGPU_OK;
         M->nnz = A->nnz;
+        return st;
+    }
+    if (M->kind == RSGPU_PREC_ICC) {
+        int st = RSGPU_OK;
+        try {
+            st = icc_set_matrix(M, A, bad_row);
+        } catch (const RsError &e) {
+            if (e.code == RSGPU_MISSING_DIAGONAL && bad_row) {
+                auto p = e.msg.rfind(' ');
+                *bad_row = std::stoi(e.msg.substr(p + 1)) - 1;
+            }
+            M->ready = false;
+            throw;
+        }
+        if (st == RSGPU_ZERO_PIVOT)
+            M->ctx->err = "Zero diagonal entry encountered on row " +
+                          std::to_string((bad_row ? *bad_row : 0) + 1) +
+                          " during ICC process";
         return st;
     }
     if (!M->ready || M->pattern_id != A->pattern_id) {

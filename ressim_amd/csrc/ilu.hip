@@ -478,8 +478,10 @@ void prec_setup(rsgpu_prec *M, rsgpu_csr *A)
     M->pattern_id = A->pattern_id;
     M->ready = false;
     M->has_t = false;
+    const bool icc = M->kind == RSGPU_PREC_ICC;
     if (A->n != A->ncols)
-        throw RsError{RSGPU_BAD_ARGUMENT, "ILU only applies to square matrices"};
+        throw RsError{RSGPU_BAD_ARGUMENT, icc ? "ICC only applies to square matrices"
+                                              : "ILU only applies to square matrices"};
     int nb = (M->kind == RSGPU_PREC_BLOCK_ILU0) ? M->nblocks : 1;
     if (nb < 1 || (n > 0 && nb > n))
         throw RsError{RSGPU_BAD_ARGUMENT, "block count must be in [1, n]"};
@@ -539,7 +541,7 @@ void prec_setup(rsgpu_prec *M, rsgpu_csr *A)
                           "Missing diagonal entry on row " + std::to_string(k + 1)};
         dg[k] = (int)(it - ci.begin());
     }
-    if (ilu_quirk_active(n, rp, ci, dg))
+    if (!icc && ilu_quirk_active(n, rp, ci, dg))
         throw RsError{RSGPU_BAD_ARGUMENT,
                       "pattern makes MTJ's ILU update the next row's first entry; "
                       "not supported"};
@@ -556,7 +558,7 @@ void prec_setup(rsgpu_prec *M, rsgpu_csr *A)
     // pipelined cell sweep applies (at most 64 cells per level, the block in
     // LDS)
     M->cell = false;
-    bool aligned = A->bs == 3 && getenv("RSGPU_ILU_CELLS") == nullptr &&
+    bool aligned = A->bs == 3 && !icc && getenv("RSGPU_ILU_CELLS") == nullptr &&
                    ilu_kernel_choice() != 3;
     for (int b = 0; aligned && b <= nb; ++b)
         aligned = M->h_bstart[b] % 3 == 0;
@@ -634,7 +636,7 @@ void prec_setup(rsgpu_prec *M, rsgpu_csr *A)
 
 // Transposed sweeps: U' (forward, subtract in ascending source row, divide)
 // then L' (backward, subtract in descending source row).
-static void prec_setup_transpose(rsgpu_prec *M)
+static void prec_setup_transpose(rsgpu_prec *M, int wmin = 0)
 {
     if (M->has_t)
         return;
@@ -680,7 +682,7 @@ static void prec_setup_transpose(rsgpu_prec *M)
                 eLt.poss[q] = j;
             }
     }
-    const int W = pad_width(std::max(max_entries(eUt, n), max_entries(eLt, n)));
+    const int W = pad_width(std::max({max_entries(eUt, n), max_entries(eLt, n), wmin}));
     HostSweep H;
     build_sweep(M->nblocks, M->h_bstart, eUt, true, W, H);
     upload_sweep(H, M->Ut, s, M->precision == RSGPU_FP64 ? 8 : 4);
@@ -957,7 +959,14 @@ __device__ void sweep_generic(int lb, int le, int W, const LevelMeta *__restrict
     }
 }
 
-template <typename T, bool LDS, bool TRANS>
+// MODE 0: ILU apply (L: sum form; U: sum form, divide); 1: ILU transApply
+// (U': subtract form, divide; L': subtract form); 2: ICC apply (R':
+// subtract form, divide; R: sum form, divide)
+#define RS_MODE_ILU 0
+#define RS_MODE_ILU_T 1
+#define RS_MODE_ICC 2
+
+template <typename T, bool LDS, int MODE>
 __global__ void __launch_bounds__(256)
     k_ilu_apply(const int *__restrict__ bstart, int W,
                 const int *__restrict__ blkA, const LevelMeta *__restrict__ metaA,
@@ -982,16 +991,21 @@ __global__ void __launch_bounds__(256)
     if (threadIdx.x == 0)
         xv[nb] = T(0);
     __syncthreads();
-    if (!TRANS) {
+    if (MODE == RS_MODE_ILU) {
         sweep_generic<T, FORM_SUM, false, LDS>(blkA[blk], blkA[blk + 1], W, metaA, rowsA, colA,
                                                valA, dglA, xv);
         sweep_generic<T, FORM_SUM, true, LDS>(blkB[blk], blkB[blk + 1], W, metaB, rowsB, colB,
                                               valB, dglB, xv);
-    } else {
+    } else if (MODE == RS_MODE_ILU_T) {
         sweep_generic<T, FORM_SUBSEQ, true, LDS>(blkA[blk], blkA[blk + 1], W, metaA, rowsA,
                                                  colA, valA, dglA, xv);
         sweep_generic<T, FORM_SUBSEQ, false, LDS>(blkB[blk], blkB[blk + 1], W, metaB, rowsB,
                                                   colB, valB, dglB, xv);
+    } else {
+        sweep_generic<T, FORM_SUBSEQ, true, LDS>(blkA[blk], blkA[blk + 1], W, metaA, rowsA,
+                                                 colA, valA, dglA, xv);
+        sweep_generic<T, FORM_SUM, true, LDS>(blkB[blk], blkB[blk + 1], W, metaB, rowsB, colB,
+                                              valB, dglB, xv);
     }
     for (int r = threadIdx.x; r < nb; r += blockDim.x)
         x[b0 + r] = xv[r];
@@ -1251,7 +1265,7 @@ __device__ __forceinline__ void sweep_pf(int nl, const LevelMeta *sm, const PfRs
 
 // One wave per block (every level has at most 64 rows); vector and both
 // sweeps' level metadata in LDS.
-template <typename T, int W, bool TRANS>
+template <typename T, int W, int MODE>
 __global__ void __launch_bounds__(64)
     k_ilu_apply_pf(const int *__restrict__ bstart, PfSweep A, PfSweep B,
                    const T *__restrict__ b, T *__restrict__ x, int xv_bytes,
@@ -1279,12 +1293,15 @@ __global__ void __launch_bounds__(64)
     __syncthreads();
     const int junkb = (nb + 1) * (int)sizeof(T);
     const PfRsrc ra = pf_rsrc(A), rb = pf_rsrc(B);
-    if (!TRANS) {
+    if (MODE == RS_MODE_ILU) {
         sweep_pf<T, W, FORM_SUM, false>(nla, sm, ra, rs_smem, junkb);
         sweep_pf<T, W, FORM_SUM, true>(nlb, sm + nla, rb, rs_smem, junkb);
-    } else {
+    } else if (MODE == RS_MODE_ILU_T) {
         sweep_pf<T, W, FORM_SUBSEQ, true>(nla, sm, ra, rs_smem, junkb);
         sweep_pf<T, W, FORM_SUBSEQ, false>(nlb, sm + nla, rb, rs_smem, junkb);
+    } else {
+        sweep_pf<T, W, FORM_SUBSEQ, true>(nla, sm, ra, rs_smem, junkb);
+        sweep_pf<T, W, FORM_SUM, true>(nlb, sm + nla, rb, rs_smem, junkb);
     }
     __syncthreads();
     for (int r = threadIdx.x; r < nb; r += blockDim.x)
@@ -1609,7 +1626,7 @@ static bool pf_addressable(const SweepData &D, size_t es)
            (size_t)D.nrows_lev * es <= lim;
 }
 
-template <typename T, bool TRANS, int W>
+template <typename T, int TRANS, int W>
 static void launch_pf(rsgpu_prec *M, const SweepData &A_, const T *va, const T *da,
                       const SweepData &B_, const T *vb, const T *db, const T *b, T *x,
                       size_t xv_bytes, size_t shm, const int *sk1, const int *sk2)
@@ -1658,7 +1675,7 @@ static int ilu_kernel_choice()
     return 0;
 }
 
-template <typename T, bool TRANS>
+template <typename T, int TRANS>
 static void launch_apply(rsgpu_prec *M, const SweepData &A_, const T *va, const T *da,
                          const SweepData &B_, const T *vb, const T *db, const T *b, T *x,
                          T *scratch, const int *sk1, const int *sk2)
@@ -1699,6 +1716,251 @@ static void launch_apply(rsgpu_prec *M, const SweepData &A_, const T *va, const 
                        A_.ell_col.p, va, da, B_.blk_lev.p, (const LevelMeta *)B_.lev_meta.p,
                        B_.rows.p, B_.ell_col.p, vb, db, b, x, scratch, sk1, sk2, M->skip);
     RS_CHECK_LAUNCH();
+}
+
+// ---------------------------------------------------------------- ICC
+// ICC.factor (ICC.java:83-118) keeps a dense work row Rk per row k and
+// eliminates it against every earlier row i with Rk[i] / R(i,i) != 0, using
+// row i's stored entries right of its diagonal; the work row fills in at
+// columns left of k that A's row k lacks, and those fill values take part
+// in the later eliminations of the same row (only positions of A's pattern
+// from the diagonal on are stored).  Structurally, the rows i that touch row
+// k form its reach: A's columns of row k left of k, closed under "row i's
+// upper columns left of k".  The host lists per row its work slots (reach
+// and row k's pattern from the diagonal on), per reach row i its work slot,
+// pivot position and (target slot, source position) pairs; updates to
+// columns right of k outside the pattern are dropped (never read).  Rows
+// are factored level by level (a row after every row of its reach), one
+// lane per row over its slice of a work array, in the reference's
+// operation order per slot: the same R bit for bit.
+#define RS_ICC_MAX_PAIRS (1LL << 28)
+
+struct IccData {
+    DevBuf<int> woff;    // work-slot range per row
+    DevBuf<int> pslot;   // per pattern position: work slot (-1: left of diag, not reached)
+    DevBuf<int> rptr;    // reach-entry range per row
+    DevBuf<int4> rent;   // reach entry: work slot of i, pivot position, pair range
+    DevBuf<int2> pairs;  // (target slot, source position)
+    DevBuf<int> lev_ptr, lev_rows;
+    int nlev = 0;
+    DevBuf<double> work;
+};
+
+void icc_destroy(IccData *d) { delete d; }
+
+static void icc_symbolic(rsgpu_prec *M)
+{
+    hipStream_t s = M->ctx->stream;
+    const int n = M->n;
+    const auto &rp = M->h_rowptr, &ci = M->h_colind;
+    std::vector<int> dg(n);
+    for (int k = 0; k < n; ++k)
+        dg[k] = (int)(std::lower_bound(ci.begin() + rp[k], ci.begin() + rp[k + 1], k) -
+                      ci.begin());
+    std::vector<int> woff(n + 1, 0), pslot(std::max(M->nnz, 1), -1), rptr(n + 1, 0);
+    std::vector<int4> rent;
+    std::vector<int2> pairs;
+    std::vector<int> level(n, 0);
+    std::vector<int> mark(n, -1), slot(n, -1), reach, wcols;
+    std::vector<int> heap;
+    long long npairs = 0;
+    for (int k = 0; k < n; ++k) {
+        // reach: ascending closure with a min-heap
+        reach.clear();
+        heap.clear();
+        auto push = [&](int c) {
+            if (mark[c] != k) {
+                mark[c] = k;
+                heap.push_back(c);
+                std::push_heap(heap.begin(), heap.end(), std::greater<int>());
+            }
+        };
+        for (int p = rp[k]; p < dg[k]; ++p)
+            push(ci[p]);
+        while (!heap.empty()) {
+            std::pop_heap(heap.begin(), heap.end(), std::greater<int>());
+            const int i = heap.back();
+            heap.pop_back();
+            reach.push_back(i);
+            for (int j = dg[i] + 1; j < rp[i + 1]; ++j)
+                if (ci[j] < k)
+                    push(ci[j]);
+        }
+        // work slots: reach (ascending) then row k's columns from the diagonal
+        wcols.assign(reach.begin(), reach.end());
+        for (int p = dg[k]; p < rp[k + 1]; ++p)
+            wcols.push_back(ci[p]);
+        woff[k + 1] = woff[k] + (int)wcols.size();
+        for (size_t q = 0; q < wcols.size(); ++q)
+            slot[wcols[q]] = (int)q;
+        for (int p = rp[k]; p < rp[k + 1]; ++p)
+            pslot[p] = slot[ci[p]];
+        int lv = 0;
+        for (int i : reach) {
+            lv = std::max(lv, level[i] + 1);
+            int4 e;
+            e.x = slot[i];
+            e.y = dg[i];
+            e.z = (int)pairs.size();
+            for (int j = dg[i] + 1; j < rp[i + 1]; ++j) {
+                const int c = ci[j];
+                const bool in = c < k ? true
+                                      : std::binary_search(ci.begin() + dg[k],
+                                                           ci.begin() + rp[k + 1], c);
+                if (in)
+                    pairs.push_back(int2{slot[c], j});
+            }
+            e.w = (int)pairs.size();
+            rent.push_back(e);
+            npairs = (long long)pairs.size();
+            if (npairs > RS_ICC_MAX_PAIRS)
+                throw RsError{RSGPU_BAD_ARGUMENT,
+                              "ICC: the fill of this matrix exceeds the factorisation's "
+                              "update list limit"};
+        }
+        rptr[k + 1] = (int)rent.size();
+        level[k] = lv;
+        for (int c : wcols)
+            slot[c] = -1;
+    }
+    int nlev = 0;
+    for (int k = 0; k < n; ++k)
+        nlev = std::max(nlev, level[k] + 1);
+    std::vector<int> lev_ptr(nlev + 1, 0), lev_rows(std::max(n, 1));
+    for (int k = 0; k < n; ++k)
+        lev_ptr[level[k] + 1]++;
+    for (int l = 0; l < nlev; ++l)
+        lev_ptr[l + 1] += lev_ptr[l];
+    {
+        std::vector<int> f(lev_ptr.begin(), lev_ptr.end() - 1);
+        for (int k = 0; k < n; ++k)
+            lev_rows[f[level[k]]++] = k;
+    }
+    if (!M->icc)
+        M->icc = new IccData;
+    IccData &D = *M->icc;
+    D.woff.upload(woff, s);
+    D.pslot.upload(pslot, s);
+    D.rptr.upload(rptr, s);
+    D.rent.upload(rent.empty() ? std::vector<int4>(1, int4{0, 0, 0, 0}) : rent, s);
+    D.pairs.upload(pairs.empty() ? std::vector<int2>(1, int2{0, 0}) : pairs, s);
+    D.lev_ptr.upload(lev_ptr, s);
+    D.lev_rows.upload(lev_rows, s);
+    D.nlev = nlev;
+    D.work.alloc(std::max(woff[n], 1));
+    RS_HIP(hipStreamSynchronize(s));
+}
+
+// One workgroup; levels in order; one lane per row (ICC.java:89-117).
+// status: [code, first failing row].
+__global__ void __launch_bounds__(256)
+    k_icc_factor(int nlev, const int *__restrict__ lev_ptr, const int *__restrict__ lev_rows,
+                 const int *__restrict__ rowptr, const int *__restrict__ diag,
+                 const int *__restrict__ woff, const int *__restrict__ pslot,
+                 const int *__restrict__ rptr, const int4 *__restrict__ rent,
+                 const int2 *__restrict__ pairs, double *work, double *lu, int *status)
+{
+    for (int l = 0; l < nlev; ++l) {
+        for (int t = lev_ptr[l] + (int)threadIdx.x; t < lev_ptr[l + 1]; t += blockDim.x) {
+            const int k = lev_rows[t];
+            double *w = work + woff[k];
+            for (int q = 0; q < woff[k + 1] - woff[k]; ++q)
+                w[q] = 0.0;
+            for (int p = rowptr[k]; p < rowptr[k + 1]; ++p)
+                if (pslot[p] >= 0)
+                    w[pslot[p]] = lu[p];
+            for (int e = rptr[k]; e < rptr[k + 1]; ++e) {
+                const int4 E = rent[e];
+                const double Rki = w[E.x] / lu[E.y];
+                if (Rki == 0)
+                    continue;
+                for (int q = E.z; q < E.w; ++q) {
+                    const int2 P = pairs[q];
+                    w[P.x] -= Rki * lu[P.y];
+                }
+            }
+            const int d = diag[k];
+            const double Rkk = w[pslot[d]];
+            if (Rkk == 0) {
+                atomicMin(&status[1], k);
+                atomicMax(&status[0], (int)RSGPU_ZERO_PIVOT);
+                continue;
+            }
+            const double sqRkk = sqrt(Rkk);
+            for (int p = d; p < rowptr[k + 1]; ++p)
+                lu[p] = w[pslot[p]] / sqRkk;
+        }
+        __syncthreads();
+    }
+}
+
+// ICC.setMatrix: R.set(A) then the factorisation; the R' and R sweeps'
+// values.  Returns RSGPU_ZERO_PIVOT with the 0-based row of the zero
+// diagonal (the first in row order).
+static int icc_factor(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
+{
+    hipStream_t s = M->ctx->stream;
+    IccData &D = *M->icc;
+    const int nnz = M->nnz;
+    if (nnz > 0) {
+        hipLaunchKernelGGL(k_load_values<double>, dim3((nnz + 255) / 256), dim3(256), 0, s,
+                           A->val.p, M->src_identity ? nullptr : M->src.p, M->lu64.p, nnz);
+        RS_CHECK_LAUNCH();
+    }
+    int init[2] = {0, 0x7fffffff};
+    RS_HIP(hipMemcpyAsync(M->status.p, init, sizeof(init), hipMemcpyHostToDevice, s));
+    if (M->n > 0)
+        hipLaunchKernelGGL(k_icc_factor, dim3(1), dim3(256), 0, s, D.nlev, D.lev_ptr.p,
+                           D.lev_rows.p, M->rowptr.p, M->diag.p, D.woff.p, D.pslot.p, D.rptr.p,
+                           D.rent.p, D.pairs.p, D.work.p, M->lu64.p, M->status.p);
+    RS_CHECK_LAUNCH();
+    launch_ell<double>(s, M->U, M->lu64.p, M->ell64_U.p, true);
+    launch_rowdiag<double>(s, M->U, M->diag.p, M->lu64.p, M->dgU64.p);
+    launch_ell<double>(s, M->Ut, M->lu64.p, M->ell64_Ut.p, false);
+    launch_rowdiag<double>(s, M->Ut, M->diag.p, M->lu64.p, M->dgUt64.p);
+    int st[2];
+    RS_HIP(hipMemcpyAsync(st, M->status.p, sizeof(st), hipMemcpyDeviceToHost, s));
+    RS_HIP(hipStreamSynchronize(s));
+    if (st[0] != 0) {
+        if (bad_row)
+            *bad_row = st[1];
+        return st[0];
+    }
+    return RSGPU_OK;
+}
+
+int icc_set_matrix(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
+{
+    if (!M->ready || M->pattern_id != A->pattern_id || !M->icc) {
+        prec_setup(M, A);
+        M->ready = false;
+        // R' and R sweeps run in one kernel with one slot width
+        prec_setup_transpose(M, M->U.W);
+        if (M->Ut.W > M->U.W) {
+            const auto &rp = M->h_rowptr, &ci = M->h_colind;
+            std::vector<int> dg(M->n);
+            for (int k = 0; k < M->n; ++k)
+                dg[k] = (int)(std::lower_bound(ci.begin() + rp[k], ci.begin() + rp[k + 1], k) -
+                              ci.begin());
+            const RangeEntries eU{dg.data(), rp.data() + 1, 1, 0, ci.data()};
+            HostSweep H;
+            build_sweep(M->nblocks, M->h_bstart, eU, false, M->Ut.W, H);
+            upload_sweep(H, M->U, M->ctx->stream, 8);
+            M->ell64_U.alloc(M->U.ell_size);
+            M->dgU64.alloc(M->U.nrows_lev);
+            RS_HIP(hipStreamSynchronize(M->ctx->stream));
+        }
+        icc_symbolic(M);
+        M->ready = true;
+    }
+    return icc_factor(M, A, bad_row);
+}
+
+static void icc_apply(rsgpu_prec *M, const double *b, double *x, const int *sk1,
+                      const int *sk2)
+{
+    launch_apply<double, RS_MODE_ICC>(M, M->Ut, M->ell64_Ut.p, M->dgUt64.p, M->U,
+                                      M->ell64_U.p, M->dgU64.p, b, x, M->scratch64.p, sk1, sk2);
 }
 
 // Cell schedules (3x3 block matrices): the pipelined cell sweeps, forward L
@@ -1874,11 +2136,16 @@ void prec_apply(rsgpu_prec *M, const double *b, double *x, const DevState *st)
         RS_CHECK_LAUNCH();
         return;
     }
+    if (M->kind == RSGPU_PREC_ICC) {
+        if (M->n)
+            icc_apply(M, b, x, sk1, sk2);
+        return;
+    }
     if (M->precision == RSGPU_FP64) {
         if (M->cell)
             launch_apply_cells<double>(M, M->ell64_L.p, M->ell64_U.p, b, x, sk1, sk2);
         else
-            launch_apply<double, false>(M, M->L, M->ell64_L.p, nullptr, M->U, M->ell64_U.p,
+            launch_apply<double, RS_MODE_ILU>(M, M->L, M->ell64_L.p, nullptr, M->U, M->ell64_U.p,
                                         M->dgU64.p, b, x, M->scratch64.p, sk1, sk2);
         return;
     }
@@ -1890,7 +2157,7 @@ void prec_apply(rsgpu_prec *M, const double *b, double *x, const DevState *st)
         launch_apply_cells<float>(M, M->ell32_L.p, M->ell32_U.p, M->tmpa32.p, M->tmpb32.p,
                                   sk1, sk2);
     else
-        launch_apply<float, false>(M, M->L, M->ell32_L.p, nullptr, M->U, M->ell32_U.p,
+        launch_apply<float, RS_MODE_ILU>(M, M->L, M->ell32_L.p, nullptr, M->U, M->ell32_U.p,
                                    M->dgU32.p, M->tmpa32.p, M->tmpb32.p, M->scratch32.p, sk1,
                                    sk2);
     convert_f32_to_f64(s, M->tmpb32.p, x, M->n);
@@ -1921,7 +2188,7 @@ void prec_apply(rsgpu_prec *M, const float *b, float *x, const DevState *st)
     if (M->cell)
         launch_apply_cells<float>(M, M->ell32_L.p, M->ell32_U.p, b, x, sk1, sk2);
     else
-        launch_apply<float, false>(M, M->L, M->ell32_L.p, nullptr, M->U, M->ell32_U.p,
+        launch_apply<float, RS_MODE_ILU>(M, M->L, M->ell32_L.p, nullptr, M->U, M->ell32_U.p,
                                    M->dgU32.p, b, x, M->scratch32.p, sk1, sk2);
 }
 
@@ -1931,8 +2198,9 @@ void prec_trans_apply(rsgpu_prec *M, const double *b, double *x, const DevState 
         amg_apply(M, b, x, st, true);
         return;
     }
-    // SSOR.transApply and DiagonalPreconditioner.transApply are apply
-    if (M->kind == RSGPU_PREC_SSOR || M->kind == RSGPU_PREC_DIAGONAL) {
+    // SSOR, DiagonalPreconditioner and ICC: transApply is apply
+    if (M->kind == RSGPU_PREC_SSOR || M->kind == RSGPU_PREC_DIAGONAL ||
+        M->kind == RSGPU_PREC_ICC) {
         prec_apply(M, b, x, st);
         return;
     }
@@ -1951,6 +2219,6 @@ void prec_trans_apply(rsgpu_prec *M, const double *b, double *x, const DevState 
         launch_ell<double>(s, M->Lt, M->lu64.p, M->ell64_Lt.p, false);
         launch_rowdiag<double>(s, M->Ut, M->diag.p, M->lu64.p, M->dgUt64.p);
     }
-    launch_apply<double, true>(M, M->Ut, M->ell64_Ut.p, M->dgUt64.p, M->Lt, M->ell64_Lt.p,
+    launch_apply<double, RS_MODE_ILU_T>(M, M->Ut, M->ell64_Ut.p, M->dgUt64.p, M->Lt, M->ell64_Lt.p,
                                nullptr, b, x, M->scratch64.p, sk1, sk2);
 }

@@ -166,6 +166,8 @@ struct SweepData {
 
 struct AmgData;
 void amg_destroy(AmgData *d);
+struct IccData;
+void icc_destroy(IccData *d);
 
 // Block b of a block-Jacobi sweep is skipped while
 // done[(blk_sys ? blk_sys[b] : b) * stride] != 0 (the done flag of the
@@ -228,7 +230,12 @@ struct rsgpu_prec {
     // per-block skip for the ILU sweeps (batched solves); set only while a
     // batched solve enqueues its applies
     BlkSkip skip{};
-    ~rsgpu_prec() { amg_destroy(amg); }
+    IccData *icc = nullptr;     // ICC: fill-in reach lists and factor levels
+    ~rsgpu_prec()
+    {
+        amg_destroy(amg);
+        icc_destroy(icc);
+    }
 };
 
 // ---------------------------------------------------------------- device state
@@ -457,6 +464,7 @@ void convert_f32_to_f64(hipStream_t s, const float *a, double *b, size_t n);
 // ilu.hip
 void prec_setup(rsgpu_prec *M, rsgpu_csr *A);
 int prec_factor(rsgpu_prec *M, rsgpu_csr *A, int *bad_row);
+int icc_set_matrix(rsgpu_prec *M, rsgpu_csr *A, int *bad_row);
 void prec_apply(rsgpu_prec *M, const double *b, double *x, const DevState *st);
 void prec_apply(rsgpu_prec *M, const float *b, float *x, const DevState *st);
 void prec_trans_apply(rsgpu_prec *M, const double *b, double *x,

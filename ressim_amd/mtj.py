@@ -460,6 +460,18 @@ class DiagonalPreconditioner(Preconditioner):
         super().setMatrix(A)
 
 
+class ICC(Preconditioner):
+    """mtj/.../sparse/ICC.java: incomplete Cholesky on A's pattern (FP64);
+    apply = transApply = R' then R solves.  ``ICC(R)`` accepts the matrix the
+    reference passes as the factor storage; only its pattern matters here."""
+    kind = L.PREC_ICC
+
+    def __init__(self, R: Optional[CompRowMatrix] = None, ctx: Optional[Context] = None):
+        if R is not None and not R.isSquare():
+            raise ValueError("ICC only applies to square matrices")
+        super().__init__(1, L.FP64, ctx or (R.ctx if R is not None else None))
+
+
 class SSOR(Preconditioner):
     """mtj/.../sparse/SSOR.java: SSOR(F, reverse, omegaF, omegaR); ``SSOR()``
     is SSOR(F) = reverse sweep, omega 1.  apply(b, x) starts from x's old

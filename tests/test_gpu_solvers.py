@@ -259,3 +259,54 @@ def test_bicg_cgs_qmr_vs_oracle(gpu, oracle, request, kind, prec, name):
     it = solver.getIterationMonitor()
     assert abs(it.iterations() - so.iterations) <= max(3, so.iterations // 10)
     np.testing.assert_allclose(it.history[:3], ho[:3], rtol=1e-9)
+
+
+@pytest.mark.parametrize("name", ["spd", "c1"])
+def test_icc_bit_exact(gpu, oracle, request, name):
+    """ICC.setMatrix: the device factor (fill-in reach lists, level by
+    level) bit-exact against the oracle's ICC.factor; apply = transApply
+    (R' then R sweeps) bit-exact against the oracle's solves."""
+    if name == "spd":
+        rp, ci, d = _spd()
+    else:
+        s = request.getfixturevalue("c1")
+        rp, ci, d = s["rowptr"], s["colind"], s["data"]
+    n = len(rp) - 1
+    print(f"ICC on {name} (n={n})", flush=True)
+    A = gpu.CompRowMatrix.from_csr(n, n, rp, ci, d)
+    M = gpu.ICC(A)
+    M.setMatrix(A)
+    st, R, dg, _ = oracle.icc_factor(rp, ci, d)
+    assert st == 0
+    _, _, R_gpu = M.factors()
+    np.testing.assert_array_equal(R_gpu, R)
+    b = np.random.default_rng(11).standard_normal(n)
+    x = np.empty(n)
+    M.apply(b, x)
+    np.testing.assert_array_equal(x, oracle.icc_apply(rp, ci, R, dg, b))
+    xt = np.empty(n)
+    M.transApply(b, xt)
+    np.testing.assert_array_equal(xt, x)
+
+
+def test_icc_cg_and_zero_diagonal(gpu, oracle):
+    rp, ci, v = _spd()
+    n = len(rp) - 1
+    b = np.random.default_rng(12).standard_normal(n)
+    xo, so, ho = oracle.solve("cg", rp, ci, v, b, rtol=1e-12, prec="icc", max_iter=5000)
+    assert so.status == 0
+    A = gpu.CompRowMatrix.from_csr(n, n, rp, ci, v)
+    M = gpu.ICC(A)
+    M.setMatrix(A)
+    s = gpu.CG(b)
+    s.setIterationMonitor(gpu.DefaultIterationMonitor(5000, 1e-12, 1e-50, 1e5))
+    s.setPreconditioner(M)
+    x = np.zeros(n)
+    s.solve(A, b, x)
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-8
+    assert abs(s.getIterationMonitor().iterations() - so.iterations) <= max(2, so.iterations // 10)
+    Z = gpu.CompRowMatrix.from_csr(2, 2, np.array([0, 2, 4]), np.array([0, 1, 0, 1]),
+                                   np.array([0.0, 1.0, 1.0, 2.0]))
+    Mz = gpu.ICC(Z)
+    with pytest.raises(RuntimeError, match="Zero diagonal entry encountered on row 1"):
+        Mz.setMatrix(Z)

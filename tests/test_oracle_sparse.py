@@ -341,3 +341,61 @@ def test_qmr_and_cgs_first_steps(oracle):
                              max_iter=1, prec="none")
     assert h[1] == pytest.approx(np.linalg.norm(b - A @ x1), rel=1e-10)
     assert h[1] < h[0]
+
+
+def _upper_dense(rp, ci, R, n):
+    U = np.zeros((n, n))
+    for i in range(n):
+        for j in range(rp[i], rp[i + 1]):
+            if ci[j] >= i:
+                U[i, ci[j]] = R[j]
+    return U
+
+
+@pytest.mark.parametrize("case", ["tridiagonal", "dense", "arrow"])
+def test_icc_known_answers(oracle, case):
+    """ICC's factor equals the exact Cholesky factor where the pattern
+    holds it: a tridiagonal, a full matrix and an arrow matrix with its
+    dense row and column last (none of them fills in)."""
+    rng = np.random.default_rng(3)
+    n = 24
+    if case == "tridiagonal":
+        A = sp.diags([-np.ones(n - 1), 4 * np.ones(n), -np.ones(n - 1)], [-1, 0, 1]).toarray()
+    elif case == "dense":
+        B = rng.standard_normal((n, n))
+        A = B @ B.T + n * np.eye(n)
+    else:
+        A = np.eye(n) * 10.0
+        A[-1, :] = A[:, -1] = rng.uniform(0.1, 1.0, n)
+        A[-1, -1] = 50.0
+    Asp = sp.csr_matrix(A)
+    Asp.sort_indices()
+    st, R, dg, bad = oracle.icc_factor(Asp.indptr, Asp.indices, Asp.data)
+    assert st == 0
+    L = np.linalg.cholesky(A)
+    np.testing.assert_allclose(_upper_dense(Asp.indptr, Asp.indices, R, n), L.T, rtol=1e-13,
+                               atol=1e-13)
+    # entries left of the diagonal keep A's values
+    for i in range(n):
+        for j in range(Asp.indptr[i], Asp.indptr[i + 1]):
+            if Asp.indices[j] < i:
+                assert R[j] == Asp.data[j]
+    b = rng.standard_normal(n)
+    x = oracle.icc_apply(Asp.indptr, Asp.indices, R, dg, b)
+    np.testing.assert_allclose(A @ x, b, rtol=1e-10, atol=1e-12)
+
+
+def test_icc_zero_diagonal_and_cg(oracle):
+    rp2, ci2 = np.array([0, 2, 4], np.int32), np.array([0, 1, 0, 1], np.int32)
+    st, _, _, bad = oracle.icc_factor(rp2, ci2, np.array([0.0, 1.0, 1.0, 2.0]))
+    assert st == 10 and bad == 1          # "Zero diagonal entry ... on row 1"
+    st, _, _, bad = oracle.icc_factor(np.array([0, 1, 2], np.int32),
+                                      np.array([1, 1], np.int32), np.ones(2))
+    assert st == 11 and bad == 1          # "Missing diagonal entry on row 1"
+    rp, ci, v = _spd()
+    b = np.random.default_rng(8).standard_normal(len(rp) - 1)
+    x, s, _ = oracle.solve("cg", rp, ci, v, b, rtol=1e-12, prec="icc", max_iter=5000)
+    _, s0, _ = oracle.solve("cg", rp, ci, v, b, rtol=1e-12, prec="none", max_iter=5000)
+    assert s.status == 0 and s.iterations < s0.iterations / 3
+    Asp = scipy_csr(rp, ci, v)
+    assert np.linalg.norm(Asp @ x - b) < 1e-10 * np.linalg.norm(b)
