@@ -451,6 +451,10 @@ typedef struct rsgpu_impes_state {
     const double *p0, *R0;         /* pressure and residual volume stored at the
                                       step start (storeCurrentState) */
     const double *cx, *cy, *cz;    /* element centres (CS.bodyForce) */
+    /* explicit molar-mass update and temperature system */
+    const double *mole_fraction;   /* [3][ncomp][n] the phases' EOS compositions */
+    const double *molar_density;   /* [3][n] EquationOfStateData.getMolarDensity */
+    const double *T0;              /* [n] temperature stored at the step start */
 } rsgpu_impes_state;
 
 /* F: the mesh's MD lists and connections (rsgpu_flux_create or
@@ -481,6 +485,29 @@ int rsgpu_impes_control_surfaces(rsgpu_impes *E, const rsgpu_impes_params *P,
 int rsgpu_impes_assemble(rsgpu_impes *E, rsgpu_csr *A, const rsgpu_impes_params *P,
                          const rsgpu_impes_state *S, const double *rho_dev,
                          const int32_t *upstream_dev, double *b_dev);
+
+/* The explicit molar-mass update (MolarMassExplicit.solve,
+ * Discretisation.java:768-919): dN from the connections' component fluxes
+ * (upstream mole fraction x molar density x Darcy flux of each present
+ * phase, CS secondaries from rsgpu_impes_control_surfaces) and the source
+ * terms, zero in the locked cells, N_new = N_old + dN ([ncomp][n] device
+ * arrays).  A negative N_new is set to 0 when truncate != 0 (the count goes
+ * to *truncated), else the call returns RSGPU_BAD_ARGUMENT with the first
+ * element * ncomp + component in the reference's order in *bad. */
+int rsgpu_impes_molar_update(rsgpu_impes *E, const rsgpu_impes_params *P,
+                             const rsgpu_impes_state *S, const double *darcy_dev,
+                             const int32_t *upstream_dev, int nlocked,
+                             const int32_t *locked_host, int truncate,
+                             const double *N_old_dev, double *N_new_dev, int *truncated,
+                             int *bad);
+/* The temperature system (TemperatureNewtonRaphson.solve,
+ * Discretisation.java:577-590): advective flux (residual only), Fourier
+ * flux (requires the MF lists), diagonal part and the energy source terms,
+ * into A (LinearSolver's pattern) and b in the reference's call order. */
+int rsgpu_impes_assemble_temperature(rsgpu_impes *E, rsgpu_csr *A,
+                                     const rsgpu_impes_params *P,
+                                     const rsgpu_impes_state *S, const double *rho_dev,
+                                     const int32_t *upstream_dev, double *b_dev);
 
 /* ---- Adjoint (no reference counterpart: rs has no gradients) ------------ */
 /* tbar[q] = (w[here(c)] - w[there(c)]) * p[md_elem[q]]: gradient of

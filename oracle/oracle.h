@@ -281,6 +281,9 @@ typedef struct or_impes_state {
     const double *temperature;     /* [n] thermal only */
     const double *p0, *R0;         /* pressure and residual volume stored at the step start */
     const double *cx, *cy, *cz;    /* element centres */
+    const double *mole_fraction;   /* [3][ncomp][n] */
+    const double *molar_density;   /* [3][n] */
+    const double *T0;              /* [n] */
 } or_impes_state;
 
 void or_impes_cs(const or_impes_params *P, const or_impes_state *S, const int *here,
@@ -293,6 +296,17 @@ int or_impes_assemble(const or_impes_params *P, const or_impes_state *S, const i
                       const int *src_elem, const int *src_kind, const double *src_q,
                       const double *src_qe, const int *rowptr, const int *colind, double *val,
                       double *b);
+int or_impes_molar_update(const or_impes_params *P, const or_impes_state *S, const int *here,
+                          const int *there, const double *darcy, const int *upstream,
+                          int nterms, const int *src_elem, const int *src_kind,
+                          const double *src_q, const unsigned char *locked, int truncate,
+                          const double *N_old, double *N_new, int *truncated, int *bad);
+int or_impes_temperature(const or_impes_params *P, const or_impes_state *S, const int *here,
+                         const int *there, const int *md_ptr, const int *md_elem,
+                         const double *md_t, const int *mf_ptr, const int *mf_elem,
+                         const double *mf_t, const double *rho, const int *upstream,
+                         int nterms, const double *src_qe, const int *src_elem,
+                         const int *rowptr, const int *colind, double *val, double *b);
 
 #ifdef __cplusplus
 }

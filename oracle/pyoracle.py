@@ -65,7 +65,8 @@ class BoParams(C.Structure):
 
 IMPES_FIELDS = ("pressure", "phase_pressure", "saturation", "mass_density", "phase_mobility",
                 "present", "comp_mobility", "dRdN", "dRdp", "heat_capacity", "dRdT",
-                "energy_mobility", "temperature", "p0", "R0", "cx", "cy", "cz")
+                "energy_mobility", "temperature", "p0", "R0", "cx", "cy", "cz",
+                "mole_fraction", "molar_density", "T0")
 
 
 class ImpesParams(C.Structure):
@@ -170,6 +171,14 @@ def lib():
         L.or_impes_cs.argtypes = [C.POINTER(ImpesParams), C.POINTER(ImpesState), _i32p, _i32p,
                                   _i32p, _i32p, _f64p, _f64p, _i32p, _f64p]
         L.or_impes_cs.restype = None
+        L.or_impes_molar_update.argtypes = [C.POINTER(ImpesParams), C.POINTER(ImpesState),
+                                            _i32p, _i32p, _f64p, _i32p, C.c_int, C.c_void_p,
+                                            C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, _f64p,
+                                            _f64p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.or_impes_temperature.argtypes = [C.POINTER(ImpesParams), C.POINTER(ImpesState),
+                                           _i32p, _i32p, _i32p, _i32p, _f64p, _i32p, _i32p,
+                                           _f64p, _f64p, _i32p, C.c_int, C.c_void_p,
+                                           C.c_void_p, _i32p, _i32p, _f64p, _f64p]
         L.or_impes_assemble.argtypes = [C.POINTER(ImpesParams), C.POINTER(ImpesState), _i32p,
                                         _i32p, _i32p, _i32p, _f64p, C.c_void_p, C.c_void_p,
                                         C.c_void_p, _f64p, _i32p, C.c_int, C.c_void_p,
@@ -676,4 +685,50 @@ def impes_assemble(params, state, here, there, md_ptr, md_elem, md_t, rho, upstr
     st = lib().or_impes_assemble(C.byref(P), C.byref(S), _i32(here), _i32(there),
                                  _i32(md_ptr), _i32(md_elem), _f64(md_t), *mfa, _f64(rho),
                                  _i32(upstream), nt, *sa, _i32(rowptr), _i32(colind), val, b)
+    return st, val, b
+
+
+def impes_molar_update(params, state, here, there, darcy, upstream, N_old, sources=None,
+                       locked=None, truncate=False):
+    """(status, N_new [ncomp][n], truncated count, bad index)."""
+    P, S, keep = _impes_structs(params, state)
+    n = params["n"]
+    N_new = np.zeros_like(_f64(N_old))
+    if sources is not None:
+        se, sk, sq = _i32(sources[0]), _i32(sources[1]), _f64(sources[2])
+        keep += [se, sk, sq]
+        nt, sa = len(se), (se.ctypes.data, sk.ctypes.data, sq.ctypes.data)
+    else:
+        nt, sa = 0, (None, None, None)
+    lk = None
+    if locked is not None:
+        lk = np.zeros(n, np.uint8)
+        lk[np.asarray(locked, np.int64)] = 1
+        keep.append(lk)
+    tr, bad = C.c_int(0), C.c_int(-1)
+    st = lib().or_impes_molar_update(C.byref(P), C.byref(S), _i32(here), _i32(there),
+                                     _f64(darcy), _i32(upstream), nt, *sa,
+                                     lk.ctypes.data if lk is not None else None,
+                                     int(truncate), _f64(N_old), N_new, C.byref(tr),
+                                     C.byref(bad))
+    return st, N_new, tr.value, bad.value
+
+
+def impes_temperature(params, state, here, there, md_ptr, md_elem, md_t, mf, rho, upstream,
+                      rowptr, colind, sources=None):
+    """(status, A values on the pattern, b) of the temperature system."""
+    P, S, keep = _impes_structs(params, state)
+    n = params["n"]
+    val = np.zeros(int(rowptr[n]))
+    b = np.zeros(n)
+    if sources is not None:
+        se, sqe = _i32(sources[0]), _f64(sources[3])
+        keep += [se, sqe]
+        nt, sa = len(se), (sqe.ctypes.data, se.ctypes.data)
+    else:
+        nt, sa = 0, (None, None)
+    st = lib().or_impes_temperature(C.byref(P), C.byref(S), _i32(here), _i32(there),
+                                    _i32(md_ptr), _i32(md_elem), _f64(md_t), _i32(mf[0]),
+                                    _i32(mf[1]), _f64(mf[2]), _f64(rho), _i32(upstream), nt,
+                                    *sa, _i32(rowptr), _i32(colind), val, b)
     return st, val, b

@@ -57,7 +57,8 @@ class BlackOilParams(C.Structure):
 
 IMPES_FIELDS = ("pressure", "phase_pressure", "saturation", "mass_density", "phase_mobility",
                 "present", "comp_mobility", "dRdN", "dRdp", "heat_capacity", "dRdT",
-                "energy_mobility", "temperature", "p0", "R0", "cx", "cy", "cz")
+                "energy_mobility", "temperature", "p0", "R0", "cx", "cy", "cz",
+                "mole_fraction", "molar_density", "T0")
 
 
 class ImpesParams(C.Structure):
@@ -168,6 +169,11 @@ SIGNATURES = {
                                                C.POINTER(ImpesState), _dp, _ip, _dp]),
     "rsgpu_impes_assemble": (C.c_int, [_h, _h, C.POINTER(ImpesParams), C.POINTER(ImpesState),
                                        _dp, _ip, _dp]),
+    "rsgpu_impes_molar_update": (C.c_int, [_h, C.POINTER(ImpesParams), C.POINTER(ImpesState),
+                                           _dp, _ip, C.c_int, _ip, C.c_int, _dp, _dp,
+                                           C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "rsgpu_impes_assemble_temperature": (C.c_int, [_h, _h, C.POINTER(ImpesParams),
+                                                   C.POINTER(ImpesState), _dp, _ip, _dp]),
 }
 
 _lib = None

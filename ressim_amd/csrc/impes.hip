@@ -51,6 +51,13 @@ struct rsgpu_impes {
     uint64_t pattern_id = 0;
     int map_status = RSGPU_OK;
     DevBuf<int> cptr, clist, diag_pos;
+    // the same for the Fourier (MF) lists (temperature system)
+    uint64_t mf_pattern_id = 0;
+    int mf_map_status = RSGPU_OK;
+    DevBuf<int> mf_cptr, mf_clist;
+    DevBuf<double> cf, adv;               // component fluxes [ncomp][nconn]; energy flux
+    DevBuf<unsigned char> locked;
+    DevBuf<int> mm_stat;                  // truncated count, first bad index
     int nterms = 0;
     DevBuf<int> src_ptr, src_term, src_kind;
     DevBuf<double> src_q, src_qe;
@@ -446,6 +453,178 @@ __global__ void k_impes_rows(ImpesK K, const int *__restrict__ cell_ptr,
 
 __global__ void k_impes_clear(int *err) { *err = 0; }
 
+// ---- explicit molar-mass update (MolarMassExplicit, Discretisation.java:768-919)
+// component fluxes of connection c: sum over present upstream phases of
+// (mole fraction x molar density) x Darcy flux, phases in order
+__global__ void k_mm_flux(ImpesK K, const double *__restrict__ darcy,
+                          const int *__restrict__ upstream, double *__restrict__ cf)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= K.nconn)
+        return;
+    const int n = K.n, nc = K.nconn, ncomp = K.ncomp;
+    for (int nu = 0; nu < ncomp; ++nu) {
+        double f = 0;
+        for (int ph = 0; ph < 3; ++ph) {
+            const int up = upstream[(size_t)ph * nc + c];
+            if (!K.S.present[(size_t)ph * n + up])
+                continue;
+            const double D = darcy[(size_t)ph * nc + c];
+            const double xi = ph_at(K.S.molar_density, n, ph, up);
+            const double C = K.S.mole_fraction[((size_t)ph * ncomp + nu) * n + up];
+            f += C * xi * D;
+        }
+        cf[(size_t)nu * nc + c] = f;
+    }
+}
+
+// dN of cell r per component: its connections in ascending order (here:
+// -= dt f, there: += dt f), then its source terms; locked cells keep N;
+// N_new = N_old + dN, negatives truncated or reported (stat[0]: count,
+// stat[1]: first cell * ncomp + component)
+__global__ void k_mm_rows(ImpesK K, const int *__restrict__ cell_ptr,
+                          const int *__restrict__ cell_conn,
+                          const signed char *__restrict__ cell_sign,
+                          const double *__restrict__ cf, const int *__restrict__ src_ptr,
+                          const int *__restrict__ src_term, const int *__restrict__ src_kind,
+                          const double *__restrict__ src_q,
+                          const unsigned char *__restrict__ locked, int truncate,
+                          const double *__restrict__ N_old, double *__restrict__ N_new,
+                          int *stat)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= K.n)
+        return;
+    const int n = K.n, nc = K.nconn, ncomp = K.ncomp;
+    const double dt = K.dt;
+    for (int nu = 0; nu < ncomp; ++nu) {
+        double dN = 0;
+        for (int a = cell_ptr[r]; a < cell_ptr[r + 1]; ++a) {
+            const double f = cf[(size_t)nu * nc + cell_conn[a]];
+            if (cell_sign[a] > 0)
+                dN -= dt * f;
+            else
+                dN += dt * f;
+        }
+        for (int a = src_ptr[r]; a < src_ptr[r + 1]; ++a) {
+            const int t = src_term[a];
+            const double *q = src_q + (size_t)t * 3 * ncomp;
+            if (src_kind[t] == 0)
+                dN += dt * q[nu];
+            else
+                for (int ph = 0; ph < 3; ++ph)
+                    dN += dt * q[ph * ncomp + nu];
+        }
+        if (locked[r])
+            dN = 0;
+        double Nn = N_old[(size_t)nu * n + r] + dN;
+        if (Nn < 0) {
+            if (truncate) {
+                Nn = 0;
+                atomicAdd(&stat[0], 1);
+            } else {
+                atomicMin(&stat[1], r * ncomp + nu);
+            }
+        }
+        N_new[(size_t)nu * n + r] = Nn;
+    }
+}
+
+// ---- temperature system (TemperatureNewtonRaphson, Discretisation.java:577-752)
+// advective energy flux of connection c: phases in order, MD entries inside
+template <int S>
+__global__ void k_temp_adv(ImpesK K, MdView<S> md, const double *__restrict__ rho,
+                           const int *__restrict__ upstream, const double *__restrict__ bf,
+                           double *__restrict__ adv)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= K.nconn)
+        return;
+    const int n = K.n, nc = K.nconn;
+    const int cnt = md.count(c);
+    double energyFlux = 0;
+#pragma unroll
+    for (int ph = 0; ph < 3; ++ph) {
+        const int up = upstream[(size_t)ph * nc + c];
+        if (!K.S.present[(size_t)ph * n + up])
+            continue;
+        const double energyMob = ph_at(K.S.energy_mobility, n, ph, up);
+        const double r = rho[(size_t)ph * nc + c];
+        auto entry = [&](int s) {
+            const int q = md.idx(c, s);
+            const int ek = md.elem[q];
+            const double psi = ph_at(K.S.phase_pressure, n, ph, ek) + r * bf[ek];
+            if (S == 0 || s < cnt)
+                energyFlux += energyMob * md.t[q] * psi;
+        };
+        if (S > 0) {
+#pragma unroll
+            for (int s = 0; s < S; ++s)
+                entry(s);
+        } else {
+            for (int s = 0; s < cnt; ++s)
+                entry(s);
+        }
+    }
+    adv[c] = energyFlux;
+}
+
+// A's entries from the Fourier lists: +t (here row), -t (there row) in
+// connection order
+__global__ void k_temp_vals(int nnz, const int *__restrict__ cptr,
+                            const int *__restrict__ clist, const double *__restrict__ mf_t,
+                            double *__restrict__ val)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nnz)
+        return;
+    double v = 0;
+    for (int k = cptr[p]; k < cptr[p + 1]; ++k) {
+        const int it = clist[k];
+        const double t = mf_t[it >> 1];
+        v += (it & 1) ? -t : t;
+    }
+    val[p] = v;
+}
+
+// b of cell r: advective fluxes of its connections, then their Fourier
+// terms, then the diagonal term (and A's diagonal entry), then -qe of its
+// source terms
+__global__ void k_temp_rows(ImpesK K, const int *__restrict__ cell_ptr,
+                            const int *__restrict__ cell_conn,
+                            const signed char *__restrict__ cell_sign,
+                            const double *__restrict__ adv, const int *__restrict__ mf_ptr,
+                            const int *__restrict__ mf_elem, const double *__restrict__ mf_t,
+                            const int *__restrict__ dpos, double *__restrict__ val,
+                            const int *__restrict__ src_ptr, const int *__restrict__ src_term,
+                            const double *__restrict__ src_qe, double *__restrict__ bvec)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= K.n)
+        return;
+    double acc = 0;
+    for (int a = cell_ptr[r]; a < cell_ptr[r + 1]; ++a) {
+        const double e = adv[cell_conn[a]];
+        acc += cell_sign[a] > 0 ? e : -e;
+    }
+    for (int a = cell_ptr[r]; a < cell_ptr[r + 1]; ++a) {
+        const int c = cell_conn[a];
+        const bool h = cell_sign[a] > 0;
+        for (int q = mf_ptr[c]; q < mf_ptr[c + 1]; ++q) {
+            const double tk = mf_t[q];
+            const double T = K.S.temperature[mf_elem[q]];
+            acc += h ? tk * T : -tk * T;
+        }
+    }
+    const double delta = K.S.heat_capacity[r];
+    const double dT = K.S.temperature[r] - K.S.T0[r];
+    acc += delta * dT / K.dt;
+    val[dpos[r]] += delta / K.dt;
+    for (int a = src_ptr[r]; a < src_ptr[r + 1]; ++a)
+        acc += -src_qe[src_term[a]];
+    bvec[r] = acc;
+}
+
 int impes_fail(rsgpu_ctx *ctx, const RsError &e)
 {
     if (ctx)
@@ -561,6 +740,51 @@ void build_map(rsgpu_impes *E, const rsgpu_csr *A)
     RS_HIP(hipStreamSynchronize(s));
     E->map_status = h ? RSGPU_NOT_IN_PATTERN : RSGPU_OK;
     E->pattern_id = A->pattern_id;
+}
+
+// The per-entry lists of the Fourier (MF) lists on A's pattern (CSR view).
+void build_map_mf(rsgpu_impes *E, const rsgpu_csr *A)
+{
+    const rsgpu_flux *F = E->F, *Mf = E->Fmf;
+    hipStream_t s = E->ctx->stream;
+    const int nnz = A->nnz;
+    const size_t nit = 2 * (size_t)Mf->nentries;
+    DevBuf<int> pos, cnt, fill;
+    pos.alloc(std::max(nit, (size_t)1));
+    cnt.alloc((size_t)nnz + 1);
+    fill.alloc(std::max(nnz, 1));
+    cnt.zero(s);
+    fill.zero(s);
+    E->mf_cptr.alloc((size_t)nnz + 1);
+    E->mf_clist.alloc(std::max(nit, (size_t)1));
+    hipLaunchKernelGGL(k_impes_clear, dim3(1), dim3(1), 0, s, E->err.p);
+    MdView<0> v;
+    v.ptr = Mf->md_ptr.p;
+    v.elem = Mf->md_elem.p;
+    v.t = Mf->md_t.p;
+    v.nconn = Mf->nconn;
+    if (Mf->nconn)
+        hipLaunchKernelGGL(k_map_count<0>, dim3((Mf->nconn + 255) / 256), dim3(256), 0, s, v,
+                           Mf->md_ptr.p, Mf->md_elem.p, F->here.p, F->there.p, A->rowptr.p,
+                           A->colind.p, pos.p, cnt.p, E->err.p);
+    RS_CHECK_LAUNCH();
+    size_t tmp = 0;
+    RS_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, cnt.p, E->mf_cptr.p, nnz + 1, s));
+    DevBuf<unsigned char> work;
+    work.alloc(std::max(tmp, (size_t)1));
+    RS_HIP(hipcub::DeviceScan::ExclusiveSum(work.p, tmp, cnt.p, E->mf_cptr.p, nnz + 1, s));
+    if (nit)
+        hipLaunchKernelGGL(k_map_fill, dim3((unsigned)((nit + 255) / 256)), dim3(256), 0, s,
+                           nit, pos.p, E->mf_cptr.p, fill.p, E->mf_clist.p);
+    if (nnz)
+        hipLaunchKernelGGL(k_map_sort, dim3((nnz + 255) / 256), dim3(256), 0, s, nnz, 0,
+                           E->mf_cptr.p, E->mf_clist.p);
+    RS_CHECK_LAUNCH();
+    int h = 0;
+    RS_HIP(hipMemcpyAsync(&h, E->err.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    RS_HIP(hipStreamSynchronize(s));
+    E->mf_map_status = h ? RSGPU_NOT_IN_PATTERN : RSGPU_OK;
+    E->mf_pattern_id = A->pattern_id;
 }
 
 }  // namespace
@@ -759,6 +983,118 @@ extern "C" int rsgpu_impes_assemble(rsgpu_impes *E, rsgpu_csr *A, const rsgpu_im
                                E->src_kind.p, E->src_q.p, E->src_qe.p, b);
             RS_CHECK_LAUNCH();
         }
+        return RSGPU_OK;
+    } catch (const RsError &e) {
+        return impes_fail(E->ctx, e);
+    }
+}
+
+extern "C" int rsgpu_impes_molar_update(rsgpu_impes *E, const rsgpu_impes_params *P,
+                                        const rsgpu_impes_state *S, const double *darcy,
+                                        const int32_t *upstream, int nlocked,
+                                        const int32_t *locked_host, int truncate,
+                                        const double *N_old, double *N_new, int *truncated,
+                                        int *bad)
+{
+    if (!E || !P || !S || nlocked < 0 || (nlocked && !locked_host) ||
+        (E->F->nconn && (!darcy || !upstream)) || (E->F->ncells && (!N_old || !N_new)))
+        return RSGPU_BAD_ARGUMENT;
+    try {
+        const rsgpu_flux *F = E->F;
+        const int n = F->ncells;
+        if (P->dt == 0.0)
+            throw RsError{RSGPU_BAD_ARGUMENT, "time step must be nonzero"};
+        if (n && (!S->present || !S->mole_fraction || !S->molar_density))
+            throw RsError{RSGPU_BAD_ARGUMENT, "missing control-volume arrays"};
+        hipStream_t s = E->ctx->stream;
+        std::vector<unsigned char> lk(std::max(n, 1), 0);
+        for (int t = 0; t < nlocked; ++t) {
+            if (locked_host[t] < 0 || locked_host[t] >= n)
+                throw RsError{RSGPU_BAD_ARGUMENT, "locked cell out of range"};
+            lk[locked_host[t]] = 1;
+        }
+        E->locked.upload(lk, s);
+        E->cf.alloc((size_t)std::max(E->ncomp * F->nconn, 1));
+        E->mm_stat.alloc(2);
+        const int init[2] = {0, 0x7fffffff};
+        RS_HIP(hipMemcpyAsync(E->mm_stat.p, init, sizeof(init), hipMemcpyHostToDevice, s));
+        const ImpesK K = make_k(E, P, S);
+        if (F->nconn)
+            hipLaunchKernelGGL(k_mm_flux, dim3((F->nconn + 255) / 256), dim3(256), 0, s, K,
+                               darcy, upstream, E->cf.p);
+        if (n)
+            hipLaunchKernelGGL(k_mm_rows, dim3((n + 255) / 256), dim3(256), 0, s, K,
+                               F->cell_ptr.p, F->cell_conn.p, F->cell_sign.p, E->cf.p,
+                               E->src_ptr.p, E->src_term.p, E->src_kind.p, E->src_q.p,
+                               E->locked.p, truncate ? 1 : 0, N_old, N_new, E->mm_stat.p);
+        RS_CHECK_LAUNCH();
+        int h[2];
+        RS_HIP(hipMemcpyAsync(h, E->mm_stat.p, sizeof(h), hipMemcpyDeviceToHost, s));
+        RS_HIP(hipStreamSynchronize(s));
+        if (truncated)
+            *truncated = h[0];
+        if (bad)
+            *bad = h[1] == 0x7fffffff ? -1 : h[1];
+        if (h[1] != 0x7fffffff) {
+            const int e = h[1] / E->ncomp, nu = h[1] % E->ncomp;
+            E->ctx->err = "Negative number of moles of component " + std::to_string(nu) +
+                          " in element " + std::to_string(e + 1);
+            return RSGPU_BAD_ARGUMENT;
+        }
+        return RSGPU_OK;
+    } catch (const RsError &e) {
+        return impes_fail(E->ctx, e);
+    }
+}
+
+extern "C" int rsgpu_impes_assemble_temperature(rsgpu_impes *E, rsgpu_csr *A,
+                                                const rsgpu_impes_params *P,
+                                                const rsgpu_impes_state *S, const double *rho,
+                                                const int32_t *upstream, double *b)
+{
+    if (!E || !A || !P || !S || (E->F->nconn && (!rho || !upstream)) || (E->F->ncells && !b))
+        return RSGPU_BAD_ARGUMENT;
+    try {
+        const rsgpu_flux *F = E->F;
+        const int n = F->ncells;
+        if (!E->Fmf)
+            throw RsError{RSGPU_BAD_ARGUMENT, "the temperature system needs the Fourier lists"};
+        if (A->n != n || A->ncols != n)
+            throw RsError{RSGPU_BAD_ARGUMENT, "matrix size differs from the mesh"};
+        if (P->dt == 0.0)
+            throw RsError{RSGPU_BAD_ARGUMENT, "time step must be nonzero"};
+        if (n && (!S->present || !S->energy_mobility || !S->phase_pressure || !S->temperature ||
+                  !S->heat_capacity || !S->T0 || !S->cx || !S->cy || !S->cz))
+            throw RsError{RSGPU_BAD_ARGUMENT, "missing control-volume arrays"};
+        hipStream_t s = E->ctx->stream;
+        if (E->pattern_id != A->pattern_id)
+            build_map(E, A);
+        if (E->mf_pattern_id != A->pattern_id)
+            build_map_mf(E, A);
+        if (E->map_status != RSGPU_OK || E->mf_map_status != RSGPU_OK)
+            throw RsError{RSGPU_NOT_IN_PATTERN, "Jacobian entry outside the matrix pattern"};
+        const ImpesK K = make_k(E, P, S);
+        const rsgpu_flux *Mf = E->Fmf;
+        E->adv.alloc(std::max(F->nconn, 1));
+        if (n) {
+            hipLaunchKernelGGL(k_impes_bf, dim3((n + 255) / 256), dim3(256), 0, s, K, E->bf.p);
+            RS_CHECK_LAUNCH();
+        }
+        if (F->nconn)
+            with_view(E, [&](auto v) {
+                hipLaunchKernelGGL(k_temp_adv<decltype(v)::slots>, dim3((F->nconn + 255) / 256),
+                                   dim3(256), 0, s, K, v, rho, upstream, E->bf.p, E->adv.p);
+            });
+        RS_CHECK_LAUNCH();
+        if (A->nnz)
+            hipLaunchKernelGGL(k_temp_vals, dim3((A->nnz + 255) / 256), dim3(256), 0, s, A->nnz,
+                               E->mf_cptr.p, E->mf_clist.p, Mf->md_t.p, A->val.p);
+        if (n)
+            hipLaunchKernelGGL(k_temp_rows, dim3((n + 255) / 256), dim3(256), 0, s, K,
+                               F->cell_ptr.p, F->cell_conn.p, F->cell_sign.p, E->adv.p,
+                               Mf->md_ptr.p, Mf->md_elem.p, Mf->md_t.p, E->diag_pos.p, A->val.p,
+                               E->src_ptr.p, E->src_term.p, E->src_qe.p, b);
+        RS_CHECK_LAUNCH();
         return RSGPU_OK;
     } catch (const RsError &e) {
         return impes_fail(E->ctx, e);

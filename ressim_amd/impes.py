@@ -15,8 +15,12 @@ results: ``CV`` getters) are inputs: a dict of device tensors keyed by
     rho, up, u = J.controlSurfaces(params, state)
     J.setSources(elem, kind, q, qe)                # optional, rs source order
     J.assemble(A, params, state, rho, up, b)       # A: CompRowMatrix, b: tensor
+    J.assembleTemperature(A, params, state, rho, up, b)   # thermal runs
+    J.molarUpdate(params, state, u, up, N_old, N_new)     # after the solves
 
-``params``: dict with dt, g, omega, ax, ay, thermal.
+``params``: dict with dt, g, omega, ax, ay, thermal.  The temperature
+system is TemperatureNewtonRaphson's (:577-752), the molar-mass update
+MolarMassExplicit.solve (:768-919).
 """
 from __future__ import annotations
 
@@ -112,3 +116,30 @@ class PressureJacobian:
         _raise(L.load().rsgpu_impes_assemble(self.h, A.h, C.byref(P), C.byref(S), _dptr(rho),
                                              _dptr(upstream), _dptr(b)), self.ctx.h)
         return A, b
+
+    def assembleTemperature(self, A, params: dict, state: dict, rho, upstream, b):
+        """The temperature system's A and b (TemperatureNewtonRaphson):
+        advective and Fourier fluxes, diagonal part, energy sources."""
+        P, S = _params(params), _state(state)
+        _torch_sync(rho, b)
+        _raise(L.load().rsgpu_impes_assemble_temperature(
+            self.h, A.h, C.byref(P), C.byref(S), _dptr(rho), _dptr(upstream), _dptr(b)),
+            self.ctx.h)
+        return A, b
+
+    def molarUpdate(self, params: dict, state: dict, darcy, upstream, N_old, N_new,
+                    locked=(), truncate=False):
+        """MolarMassExplicit.solve: N_new = N_old + dN ([ncomp][n] device
+        tensors); returns the number of truncated negative mole numbers
+        (truncate=True) or raises on the first negative one."""
+        P, S = _params(params), _state(state)
+        lk = np.ascontiguousarray(np.asarray(locked, np.int32).reshape(-1))
+        tr, bad = C.c_int(0), C.c_int(-1)
+        _torch_sync(N_old, N_new)
+        code = L.load().rsgpu_impes_molar_update(
+            self.h, C.byref(P), C.byref(S), _dptr(darcy), _dptr(upstream), lk.size,
+            _hptr(lk) if lk.size else None, int(truncate), _dptr(N_old), _dptr(N_new),
+            C.byref(tr), C.byref(bad))
+        self.bad = bad.value   # first element * ncomp + component with N < 0, or -1
+        _raise(code, self.ctx.h)
+        return tr.value

@@ -287,12 +287,14 @@ def impes_state(seed: int, nx: int, ny: int, nz: int, dx: float, dy: float, dz: 
         dRdN=(1.0e-5 * (0.5 + u(10, ncomp * n))).reshape(ncomp, n),
         dRdp=-1.0e-9 * (0.5 + u(11, n)) * dx * dy * dz,
         p0=p - 1.0e4 * (u(12, n) - 0.3), R0=1.0e-3 * (u(13, n) - 0.5) * dx * dy * dz,
-        cx=cx, cy=cy, cz=cz)
+        cx=cx, cy=cy, cz=cz, mole_fraction=frac,
+        molar_density=dens[:, 0, :] * (1 + 0.05 * u(18, 3 * n).reshape(3, n)))
     if thermal:
         st.update(heat_capacity=2.5e6 * (1 + u(14, n)) * dx * dy * dz,
                   dRdT=1.0e-4 * (u(15, n) - 0.3) * dx * dy * dz,
                   energy_mobility=mob * (4.0e6 + 1.0e6 * u(16, 3 * n).reshape(3, n)),
-                  temperature=350.0 + 10 * u(17, n))
+                  temperature=350.0 + 10 * u(17, n),
+                  T0=350.0 + 10 * u(17, n) - 0.5 * u(19, n))
     return {k: np.ascontiguousarray(v.reshape(-1)) for k, v in st.items()}
 
 

@@ -119,3 +119,77 @@ def test_impes_errors(gpu, oracle, mesh):
         J.assemble(A, dict(P.IMPES_PARAMS, thermal=1), sd, rho, up, b)
     with pytest.raises(ValueError):
         J.setSources([n], [0], np.zeros((1, 3, 2)), [0.0])
+
+
+@pytest.mark.parametrize("truncate", [False, True])
+def test_molar_update_bitexact(gpu, oracle, mesh, truncate):
+    """MolarMassExplicit.solve on the device: N_new bit-exact against the
+    oracle with sources and locked cells; negative mole numbers truncated
+    (count) or reported (first element and component)."""
+    import torch
+    from ressim_amd.impes import PressureJacobian
+    from ressim_amd.potential import FluxOperator
+    m, n, ncomp = mesh, mesh["n"], 3
+    g = GRID
+    print(f"molar update truncate={truncate}", flush=True)
+    st = P.impes_state(13, g["nx"], g["ny"], g["nz"], g["dx"], g["dy"], g["dz"], ncomp=ncomp)
+    F = FluxOperator.from_lists(n, m["md_ptr"], m["md_elem"], m["md_t"], m["here"], m["there"])
+    J = PressureJacobian(F, ncomp)
+    elem, kind, q, qe = P.impes_sources(7, n, ncomp)
+    J.setSources(elem, kind, q, qe)
+    params = dict(P.IMPES_PARAMS)
+    sd = _device(st)
+    rho, up, u = J.controlSurfaces(params, sd)
+    op = dict(params, n=n, nconn=len(m["here"]), ncomp=ncomp)
+    _, up_o, u_o = oracle.impes_cs(op, st, m["here"], m["there"], m["md_ptr"], m["md_elem"],
+                                   m["md_t"])
+    locked = [5, 40, 41]
+    N_old = np.where(np.arange(ncomp * n) % 5 == 0, 1.0, 1.0e10)
+    code_o, N_o, tr_o, bad_o = oracle.impes_molar_update(
+        op, st, m["here"], m["there"], u_o, up_o, N_old, sources=(elem, kind, q, qe),
+        locked=locked, truncate=truncate)
+    Nd = torch.tensor(N_old, device="cuda")
+    Nn = torch.empty_like(Nd)
+    if truncate or code_o == 0:
+        tr = J.molarUpdate(params, sd, u, up, Nd, Nn, locked=locked, truncate=truncate)
+        assert tr == tr_o
+    else:
+        with pytest.raises(ValueError, match="Negative number of moles"):
+            J.molarUpdate(params, sd, u, up, Nd, Nn, locked=locked, truncate=truncate)
+        assert J.bad == bad_o
+    assert np.array_equal(Nn.cpu().numpy(), N_o)
+
+
+def test_temperature_system_bitexact(gpu, oracle, mesh):
+    """TemperatureNewtonRaphson's A and b on the device bit-exact against
+    the oracle (advective flux, Fourier flux, diagonal part, sources)."""
+    import torch
+    from ressim_amd.impes import PressureJacobian
+    from ressim_amd.potential import FluxOperator
+    m, n, ncomp = mesh, mesh["n"], 2
+    g = GRID
+    st = P.impes_state(17, g["nx"], g["ny"], g["nz"], g["dx"], g["dy"], g["dz"], ncomp=ncomp,
+                       thermal=True)
+    F = FluxOperator.from_lists(n, m["md_ptr"], m["md_elem"], m["md_t"], m["here"], m["there"])
+    mf_t = 0.21 * m["md_t"]
+    Fmf = FluxOperator.from_lists(n, m["md_ptr"], m["md_elem"], mf_t, m["here"], m["there"])
+    J = PressureJacobian(F, ncomp, fourier=Fmf)
+    elem, kind, q, qe = P.impes_sources(9, n, ncomp)
+    J.setSources(elem, kind, q, qe)
+    params = dict(P.IMPES_PARAMS, thermal=1, omega=0.2, ax=3.0, ay=4.0)
+    sd = _device(st)
+    rho, up, u = J.controlSurfaces(params, sd)
+    A = gpu.CompRowMatrix.from_csr(n, n, m["rowptr"], m["colind"])
+    b = torch.empty(n, dtype=torch.float64, device="cuda")
+    J.assembleTemperature(A, params, sd, rho, up, b)
+    op = dict(params, n=n, nconn=len(m["here"]), ncomp=ncomp)
+    rho_o, up_o, _ = oracle.impes_cs(op, st, m["here"], m["there"], m["md_ptr"], m["md_elem"],
+                                     m["md_t"])
+    code, val_o, b_o = oracle.impes_temperature(op, st, m["here"], m["there"], m["md_ptr"],
+                                                m["md_elem"], m["md_t"],
+                                                (m["md_ptr"], m["md_elem"], mf_t), rho_o, up_o,
+                                                m["rowptr"], m["colind"],
+                                                sources=(elem, kind, q, qe))
+    assert code == 0
+    assert np.array_equal(A.getData(), val_o)
+    assert np.array_equal(b.cpu().numpy(), b_o)

@@ -182,3 +182,119 @@ def test_missing_pattern_entry(oracle, mesh):
     code, _, _ = oracle.impes_assemble(params, st, mesh["here"], mesh["there"], mesh["md_ptr"],
                                        mesh["md_elem"], mesh["md_t"], rho, up, rp2, ci2)
     assert code == 12
+
+
+def test_molar_update_conserves_and_reduces(oracle, mesh):
+    """Without sources and locks the connection fluxes move moles between
+    cells only (the total of each component is unchanged up to round-off);
+    with one phase, one component, unit mole fraction and molar density, dN
+    is dt times the cells' net Darcy inflow (the negated cell rates of the
+    control surfaces' Darcy fluxes)."""
+    n, nconn, ncomp = mesh["n"], len(mesh["here"]), 2
+    st = _state(ncomp)
+    params = _params(mesh, ncomp)
+    rho, up, u = oracle.impes_cs(params, st, mesh["here"], mesh["there"], mesh["md_ptr"],
+                                 mesh["md_elem"], mesh["md_t"])
+    N_old = 1.0e10 * (1 + np.arange(ncomp * n) % 7)
+    code, N_new, tr, bad = oracle.impes_molar_update(params, st, mesh["here"], mesh["there"],
+                                                     u, up, N_old)
+    assert code == 0
+    dN = (N_new - N_old).reshape(ncomp, n)
+    for nu in range(ncomp):
+        assert abs(dN[nu].sum()) <= 1e-9 * np.abs(dN[nu]).sum()
+    # one phase, one component
+    st1 = dict(st)
+    st1["present"] = np.concatenate([np.ones(n), np.zeros(2 * n)]).astype(np.uint8)
+    st1["mole_fraction"] = np.concatenate([np.ones(n), np.zeros(2 * n)])
+    st1["molar_density"] = np.ones(3 * n)
+    p1 = _params(mesh, 1)
+    rho1, up1, u1 = oracle.impes_cs(p1, st1, mesh["here"], mesh["there"], mesh["md_ptr"],
+                                    mesh["md_elem"], mesh["md_t"])
+    # N_old = 0: N_new is dN itself (negative entries are reported, not truncated)
+    code, N1, _, _ = oracle.impes_molar_update(p1, st1, mesh["here"], mesh["there"], u1, up1,
+                                               np.zeros(n))
+    assert code in (0, 13)
+    rates = oracle.cell_rates(mesh["here"], mesh["there"], u1[:nconn], np.arange(n))
+    np.testing.assert_allclose(N1, -p1["dt"] * rates, rtol=1e-12,
+                               atol=1e-14 * p1["dt"] * np.abs(rates).max())
+
+
+def test_molar_update_sources_locks_and_negatives(oracle, mesh):
+    n, ncomp = mesh["n"], 2
+    st = _state(ncomp)
+    params = _params(mesh, ncomp)
+    rho, up, u = oracle.impes_cs(params, st, mesh["here"], mesh["there"], mesh["md_ptr"],
+                                 mesh["md_elem"], mesh["md_t"])
+    elem, kind, q, qe = P.impes_sources(5, n, ncomp)
+    N_old = np.full(ncomp * n, 1.0e10)
+    _, Na, _, _ = oracle.impes_molar_update(params, st, mesh["here"], mesh["there"], u, up,
+                                            N_old)
+    _, Nb, _, _ = oracle.impes_molar_update(params, st, mesh["here"], mesh["there"], u, up,
+                                            N_old, sources=(elem, kind, q, qe))
+    want = np.zeros((ncomp, n))
+    for t in range(len(elem)):
+        want[:, elem[t]] += params["dt"] * (q[t, 0] if kind[t] == 0 else q[t].sum(axis=0))
+    np.testing.assert_allclose((Nb - Na).reshape(ncomp, n), want, rtol=1e-6,
+                               atol=1e-9 * params["dt"])
+    locked = [3, 17]
+    _, Nl, _, _ = oracle.impes_molar_update(params, st, mesh["here"], mesh["there"], u, up,
+                                            N_old, locked=locked)
+    for c in locked:
+        assert np.all(Nl.reshape(ncomp, n)[:, c] == 1.0e10)
+    # tiny old mole numbers: negatives truncated (counted) or reported
+    N_small = np.full(ncomp * n, 1.0)
+    code, Nt, tr, _ = oracle.impes_molar_update(params, st, mesh["here"], mesh["there"], u,
+                                                up, N_small, truncate=True)
+    assert code == 0 and tr > 0 and Nt.min() == 0.0
+    code, _, _, bad = oracle.impes_molar_update(params, st, mesh["here"], mesh["there"], u,
+                                                up, N_small)
+    assert code == 13
+    neg = [e * ncomp + nu for e in range(n) for nu in range(ncomp) if Nt[nu * n + e] == 0.0]
+    assert bad == min(neg) and tr == len(neg)
+
+
+def test_temperature_system(oracle, mesh):
+    """With every phase absent the advective part vanishes and the
+    temperature system's A is PotentialCalculator's flux assembly of the
+    Fourier lists plus delta/dt on the diagonal, b the cells' net Fourier
+    outflow of T plus delta dT/dt; the advective energy flux of a connection
+    is the upstream energy mobility times the control surface's potential
+    sum."""
+    n, ncomp = mesh["n"], 2
+    st = _state(ncomp, thermal=True)
+    params = _params(mesh, ncomp, thermal=1)
+    mf = (mesh["md_ptr"], mesh["md_elem"], 0.3 * mesh["md_t"])
+    rho, up, u = oracle.impes_cs(params, st, mesh["here"], mesh["there"], mesh["md_ptr"],
+                                 mesh["md_elem"], mesh["md_t"])
+    s0 = dict(st)
+    s0["present"] = np.zeros(3 * n, np.uint8)
+    code, val, b = oracle.impes_temperature(params, s0, mesh["here"], mesh["there"],
+                                            mesh["md_ptr"], mesh["md_elem"], mesh["md_t"], mf,
+                                            rho, up, mesh["rowptr"], mesh["colind"])
+    assert code == 0
+    Af = oracle.assemble_flux(n, mesh["here"], mesh["there"], mf[0], mf[1], mf[2],
+                              mesh["rowptr"], mesh["colind"])
+    A = _dense(mesh["rowptr"], mesh["colind"], val, n)
+    Ad = _dense(mesh["rowptr"], mesh["colind"], Af, n) + np.diag(st["heat_capacity"] /
+                                                               params["dt"])
+    np.testing.assert_allclose(A, Ad, rtol=1e-14, atol=0)
+    fl = oracle.connection_flux(mf[0], mf[1], mf[2], st["temperature"])
+    rates = oracle.cell_rates(mesh["here"], mesh["there"], fl, np.arange(n))
+    diag = st["heat_capacity"] * (st["temperature"] - st["T0"]) / params["dt"]
+    np.testing.assert_allclose(b, rates + diag, rtol=1e-10,
+                               atol=1e-12 * np.abs(rates).max())
+    # advective part: one connection's contribution against the CS flux
+    code, _, b1 = oracle.impes_temperature(params, st, mesh["here"], mesh["there"],
+                                           mesh["md_ptr"], mesh["md_elem"], mesh["md_t"],
+                                           (mesh["md_ptr"], mesh["md_elem"],
+                                            0.0 * mesh["md_t"]), rho, up, mesh["rowptr"],
+                                           mesh["colind"])
+    nconn = len(mesh["here"])
+    adv = np.zeros(nconn)
+    for ph in range(3):
+        upc = up[ph * nconn:(ph + 1) * nconn]
+        pres = st["present"][ph * n + upc] != 0
+        D = u[ph * nconn:(ph + 1) * nconn] / st["phase_mobility"][ph * n + upc].clip(1e-300)
+        adv += np.where(pres, st["energy_mobility"][ph * n + upc] * D, 0.0)
+    want = oracle.cell_rates(mesh["here"], mesh["there"], adv, np.arange(n)) + diag
+    np.testing.assert_allclose(b1, want, rtol=1e-8, atol=1e-10 * np.abs(want).max())
