@@ -435,6 +435,33 @@ def bench_impes(a):
     wall = (time.perf_counter() - t) / a.steps
     t_cs = float(np.median([e[0].elapsed_time(e[1]) for e in evs])) * 1e-3
     t_asm = float(np.median([e[1].elapsed_time(e[2]) for e in evs])) * 1e-3
+    # the rest of the IMPES step on the same state: the explicit molar-mass
+    # update and (thermal runs) the temperature system
+    N_old = torch.full((ncomp * n,), 1.0e12, dtype=torch.float64, device=dev)
+    N_new = torch.empty_like(N_old)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    J.molarUpdate(params, sd, u, up, N_old, N_new, truncate=True)
+    e0.record(stream)
+    for _ in range(a.steps):
+        J.molarUpdate(params, sd, u, up, N_old, N_new, truncate=True)
+    e1.record(stream)
+    e1.synchronize()
+    t_mm = e0.elapsed_time(e1) / a.steps * 1e-3
+    t_temp = None
+    if thermal:
+        At, _ = mtj.mpfa_o_assemble(pr.dim, pr.nx, pr.ny, pr.nz, pr.dx, pr.dy, pr.dz, pr.K6,
+                                    ctx=ctx)
+        bt = torch.empty(n, dtype=torch.float64, device=dev)
+        J.assembleTemperature(At, params, sd, rho, up, bt)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(a.steps):
+            J.assembleTemperature(At, params, sd, rho, up, bt)
+        e1.record(stream)
+        e1.synchronize()
+        t_temp = e0.elapsed_time(e1) / a.steps * 1e-3
+    log(f"molar-mass update {t_mm * 1e6:.0f} us per call (host syncs for the status)"
+        + (f"; temperature system {t_temp * 1e6:.0f} us" if t_temp else ""))
     # algorithmic bytes: each API call's inputs read once and outputs written once
     cells_cs = 8 * (3 * 4 + 3) * n                       # 4 phase arrays + centres
     cells_asm = (4 * 8 + 24 + 3 + 24 * ncomp + 8 * ncomp + 24) * n
@@ -488,6 +515,8 @@ def bench_impes(a):
                                   f"{ncomp} components, thermal {int(thermal)}",
                       "n": n, "connections": nconn, "md_entries": nent, "nnz": nnz},
            "control_surfaces_us": t_cs * 1e6, "assembly_us": t_asm * 1e6,
+           "molar_update_us": t_mm * 1e6,
+           "temperature_system_us": t_temp * 1e6 if t_temp else None,
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                         "frac": achieved / 8000.0, "traffic": None,
                         "kernel": "k_impes_cs + k_impes_conn + k_impes_rows",
