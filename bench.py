@@ -41,6 +41,13 @@ def spmv_bytes(n, nnz, s_val, s_vec):
     return nnz * (s_val + 4) + (n + 1) * 4 + n * s_vec + n * s_vec
 
 
+def sliced_bytes(n, nchunks, nslots, s_val):
+    """Bytes the sliced SpMV must move: every value slot (zeros where a row
+    lacks the slot), the 4-byte row masks, two ints per chunk, x read once
+    and y written once (the offset tables are a few hundred bytes)."""
+    return nslots * s_val + n * 4 + nchunks * 8 + n * s_val + n * s_val
+
+
 # HBM bytes per launch from the committed rocprofv3 counter passes
 # (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors from a 512 MiB copy in
 # the same process); valid for the 128x128x64 workload they were taken on.
@@ -240,6 +247,26 @@ def main():
     t_spmv32 = time_kernel(lambda: lib.rsgpu_csr_mult_f32(A.h, xin32.data_ptr(),
                                                           y32.data_ptr()), a.spmv_reps)
     gbs32 = spmv_bytes(n, nnz, 4, 4) / t_spmv32 / 1e9
+    # the sliced copy the solvers stream (FP64 filled per FP64 solve, FP32
+    # with the FP32 values)
+    nch, nslots, ntab = A.sliced_info()
+    sl = None
+    if nch:
+        assert lib.rsgpu_csr_sync_sliced(A.h) == 0
+        torch.cuda.synchronize()
+        t_sl64 = time_kernel(lambda: lib.rsgpu_csr_mult_sliced(A.h, xin.data_ptr(),
+                                                               yout.data_ptr()), a.spmv_reps)
+        t_sl32 = time_kernel(lambda: lib.rsgpu_csr_mult_sliced_f32(A.h, xin32.data_ptr(),
+                                                                   y32.data_ptr()), a.spmv_reps)
+        bs64, bs32 = sliced_bytes(n, nch, nslots, 8), sliced_bytes(n, nch, nslots, 4)
+        sl = {"fp64_us": t_sl64 * 1e6, "fp64_GBps": bs64 / t_sl64 / 1e9, "bytes_fp64": bs64,
+              "fp32_us": t_sl32 * 1e6, "fp32_GBps": bs32 / t_sl32 / 1e9, "bytes_fp32": bs32,
+              "chunks": nch, "slots": nslots, "tables": ntab,
+              "csr_equivalent_fp64_GBps": b64 / t_sl64 / 1e9}
+        log(f"sliced SpMV fp64 {t_sl64 * 1e6:.1f} us {sl['fp64_GBps']:.0f} GB/s "
+            f"({sl['fp64_GBps'] / PEAK_HBM_GBS:.1%}); fp32 {t_sl32 * 1e6:.1f} us "
+            f"{sl['fp32_GBps']:.0f} GB/s; {nch} chunks, {ntab} tables, "
+            f"{nslots / nnz - 1:.1%} padding")
     # block ILU(0) apply (L and U sweeps in one kernel)
     M64.setMatrix(A)
     info = M64.info()
@@ -288,13 +315,20 @@ def main():
                   "rel_err_vs_fp64": err_mx},
         "spmv": {"fp64_GBps": gbs64, "fp64_us": t_spmv * 1e6, "fp32_GBps": gbs32,
                  "fp32_us": t_spmv32 * 1e6, "bytes_fp64": b64},
+        "spmv_sliced": sl,
         "ilu_apply": {"us": t_ilu * 1e6, "GBps": ilu_bytes / t_ilu / 1e9,
                       "levels_per_block": info["levels_lower"]},
-        "roofline": {"bound": "hbm", "achieved": gbs64, "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": gbs64 / PEAK_HBM_GBS,
-                     "traffic": measured_traffic(pr, "k_spmv<double>"),
-                     "kernel": "k_spmv<double,EpiMult> (CompRowMatrix.mult)",
-                     "algorithmic_bytes": b64},
+        "roofline": ({"bound": "hbm", "achieved": sl["fp64_GBps"], "peak": PEAK_HBM_GBS,
+                      "unit": "GB/s", "frac": sl["fp64_GBps"] / PEAK_HBM_GBS,
+                      "traffic": measured_traffic(pr, "k_spmv_sliced<double>"),
+                      "kernel": "k_spmv_sliced<double> (the FP64 solves' SpMV, "
+                                "CompRowMatrix.mult's sums over the sliced copy)",
+                      "algorithmic_bytes": sl["bytes_fp64"]} if sl else
+                     {"bound": "hbm", "achieved": gbs64, "peak": PEAK_HBM_GBS,
+                      "unit": "GB/s", "frac": gbs64 / PEAK_HBM_GBS,
+                      "traffic": measured_traffic(pr, "k_spmv<double>"),
+                      "kernel": "k_spmv<double,EpiMult> (CompRowMatrix.mult)",
+                      "algorithmic_bytes": b64}),
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -972,6 +1006,12 @@ def probe(a, A, n, nnz, lib, mtj, _lib, ctx, stream, dev):
         torch.cuda.synchronize()
         for _ in range(reps):
             lib.rsgpu_csr_mult_f32(A.h, x32.data_ptr(), y32.data_ptr())
+        if A.sliced_info()[0]:
+            assert lib.rsgpu_csr_sync_sliced(A.h) == 0
+            for _ in range(reps):
+                lib.rsgpu_csr_mult_sliced(A.h, xin.data_ptr(), y.data_ptr())
+            for _ in range(reps):
+                lib.rsgpu_csr_mult_sliced_f32(A.h, x32.data_ptr(), y32.data_ptr())
     elif a.probe == "mixed":
         A.sync_f32()
         M = mtj.BlockILU(a.nblocks, precision=_lib.FP32, ctx=ctx)

@@ -106,6 +106,21 @@ int rsgpu_csr_sync_f32(rsgpu_csr *A);
  * scatter visits them, CompRowMatrix.java:356-372); values copied now. */
 int rsgpu_csr_transpose(const rsgpu_csr *A, rsgpu_csr **At);
 int rsgpu_csr_mult_f32(rsgpu_csr *A, const float *x_dev, float *y_dev);
+/* Sliced copy (no MTJ counterpart; the storage the solvers' SpMV streams
+ * for stencil matrices).  Rows in chunks of 64, each chunk with the sorted
+ * union of its rows' column offsets (column - row) as a table and per row a
+ * mask of the slots it holds; values slot-major per chunk.  Built with the
+ * pattern when every chunk's union has at most 32 offsets (nchunks = 0
+ * otherwise; RSGPU_SLICED=0 in the environment leaves it out).  nslots =
+ * values per copy.  y = A x over it is bit-identical to rsgpu_csr_mult
+ * (mult_sliced_f32: to rsgpu_csr_mult_f32).  The FP64 copy is refreshed by
+ * rsgpu_csr_sync_sliced (and by every FP64 solve at its start), the FP32
+ * copy by rsgpu_csr_sync_f32. */
+int rsgpu_csr_sliced_info(const rsgpu_csr *A, int *nchunks, int64_t *nslots,
+                          int *ntables);
+int rsgpu_csr_sync_sliced(rsgpu_csr *A);
+int rsgpu_csr_mult_sliced(rsgpu_csr *A, const double *x_dev, double *y_dev);
+int rsgpu_csr_mult_sliced_f32(rsgpu_csr *A, const float *x_dev, float *y_dev);
 /* 3x3 block rows (no MTJ counterpart: MTJ has no block-CSR storage).  The
  * block pattern browptr (nb+1) / bcolind (nnzb; sorted per block row as
  * Arrays.sort would, CompRowMatrix.java:188) defines the expanded scalar

@@ -102,6 +102,11 @@ SIGNATURES = {
     "rsgpu_csr_transmultadd": (C.c_int, [_h, C.c_double, _dp, _dp]),
     "rsgpu_csr_sync_f32": (C.c_int, [_h]),
     "rsgpu_csr_mult_f32": (C.c_int, [_h, _vp, _vp]),
+    "rsgpu_csr_sliced_info": (C.c_int, [_h, C.POINTER(C.c_int), C.POINTER(C.c_int64),
+                                        C.POINTER(C.c_int)]),
+    "rsgpu_csr_sync_sliced": (C.c_int, [_h]),
+    "rsgpu_csr_mult_sliced": (C.c_int, [_h, _vp, _vp]),
+    "rsgpu_csr_mult_sliced_f32": (C.c_int, [_h, _vp, _vp]),
     "rsgpu_csr_mult_host": (C.c_int, [_h, _dp, _dp]),
     "rsgpu_csr_transmult_host": (C.c_int, [_h, _dp, _dp]),
     "rsgpu_prec_create": (C.c_int, [_h, C.c_int, C.c_int, C.c_int, _hp]),

@@ -178,7 +178,7 @@ extern "C" int rsgpu_csr_create_bsr3(rsgpu_ctx *ctx, int nb, int nnzb, const int
     A->bcol.upload(hc.empty() ? std::vector<int>(1, 0) : hc, s);
     A->val.alloc(std::max(nnz, 1));
     A->val.zero(s);
-    spmv_build_plans(A->h_rowptr, n, A, s);
+    spmv_build_plans(A->h_rowptr, n, A, s, false);
     spmv_build_plan_b3(hb, nb, A->bplan, s, RS_SPMV_NNZ);
     spmv_build_plan_b3(hb, nb, A->bplan32, s, RS_SPMV_NNZ32);
     A->pattern_id = next_pattern_id();
@@ -470,6 +470,61 @@ extern "C" int rsgpu_csr_sync_f32(rsgpu_csr *A)
     RS_TRY(A->ctx)
     A->val32.alloc(std::max(A->nnz, 1));
     convert_f64_to_f32(A->ctx->stream, A->val.p, A->val32.p, A->nnz);
+    sliced_fill<float>(A, A->ctx->stream);
+    return RSGPU_OK;
+    RS_CATCH(A->ctx)
+}
+
+extern "C" int rsgpu_csr_sliced_info(const rsgpu_csr *A, int *nchunks, int64_t *nslots,
+                                     int *ntables)
+{
+    if (!A)
+        return RSGPU_BAD_ARGUMENT;
+    if (nchunks)
+        *nchunks = A->sl.nchunks;
+    if (nslots)
+        *nslots = A->sl.nslots;
+    if (ntables)
+        *ntables = A->sl.ntables;
+    return RSGPU_OK;
+}
+
+extern "C" int rsgpu_csr_sync_sliced(rsgpu_csr *A)
+{
+    if (!A)
+        return RSGPU_BAD_ARGUMENT;
+    RS_TRY(A->ctx)
+    if (!A->sl.ok()) {
+        A->ctx->err = "the matrix has no sliced copy";
+        return RSGPU_BAD_ARGUMENT;
+    }
+    sliced_fill<double>(A, A->ctx->stream);
+    return RSGPU_OK;
+    RS_CATCH(A->ctx)
+}
+
+extern "C" int rsgpu_csr_mult_sliced(rsgpu_csr *A, const double *x, double *y)
+{
+    if (!A || !A->sl.ok() || !A->sl.v64_ready) {
+        if (A)
+            A->ctx->err = "mult_sliced needs rsgpu_csr_sync_sliced after the last value change";
+        return RSGPU_BAD_ARGUMENT;
+    }
+    RS_TRY(A->ctx)
+    sliced_mult<double>(A->ctx->stream, A, x, y);
+    return RSGPU_OK;
+    RS_CATCH(A->ctx)
+}
+
+extern "C" int rsgpu_csr_mult_sliced_f32(rsgpu_csr *A, const float *x, float *y)
+{
+    if (!A || !A->sl.ok() || !A->sl.v32.p) {
+        if (A)
+            A->ctx->err = "mult_sliced_f32 needs rsgpu_csr_sync_f32 after the last value change";
+        return RSGPU_BAD_ARGUMENT;
+    }
+    RS_TRY(A->ctx)
+    sliced_mult<float>(A->ctx->stream, A, x, y);
     return RSGPU_OK;
     RS_CATCH(A->ctx)
 }

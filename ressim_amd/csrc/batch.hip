@@ -89,7 +89,7 @@ extern "C" int rsgpu_csr_block_diag(rsgpu_ctx *ctx, int count, rsgpu_csr *const 
                                       s));
             off += parts[k]->nnz;
         }
-        spmv_build_plans(A->h_rowptr, A->n, A, s);
+        spmv_build_plans(A->h_rowptr, A->n, A, s, false);
         A->pattern_id = next_pattern_id();
         RS_HIP(hipStreamSynchronize(s));
     } catch (const RsError &e) {
@@ -199,13 +199,8 @@ __device__ bool seg_finish(const RedSeg &R, int sys, int g, int g0, int g1,
         return false;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 #pragma unroll
-    for (int q = 0; q < NR; ++q) {
-        double acc = 0;
-        for (int i = g0 + (int)threadIdx.x; i < g1; i += BS)
-            acc += __hip_atomic_load(&R.partials[(size_t)q * R.P + i], __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT);
-        tot[q] = block_sum<BS>(acc, sh);
-    }
+    for (int q = 0; q < NR; ++q)
+        tot[q] = block_sum<BS>(gather_sum<BS>(R.partials + (size_t)q * R.P, g0, g1), sh);
     return true;
 }
 

@@ -45,6 +45,9 @@ struct rsgpu_solver {
     std::vector<double> h_hist;
     hipEvent_t ev[2] = {nullptr, nullptr};
     int *flags_host = nullptr;    // pinned [2][4]
+    // the matrix whose FP64 sliced copy this solve filled (its SpMVs then
+    // stream the sliced copy)
+    const rsgpu_csr *sl64 = nullptr;
 };
 
 // vector kernels: at least 4 elements per lane, at most 1024 groups
@@ -263,6 +266,45 @@ __global__ void __launch_bounds__(RS_SPMV_BS)
         finish_scalar(st, fin, tot[0], NR > 1 ? tot[NR - 1] : 0.0);
 }
 
+// k_spmv_red over the sliced copy: each workgroup computes a contiguous run
+// of 4-chunk tiles, then the same deterministic reduction
+template <typename T, class Epi, int NR>
+__global__ void __launch_bounds__(RS_SPMV_BS)
+    k_spmv_red_sl(SlView sv, int ntiles, const T *__restrict__ val, const T *__restrict__ x,
+                  Epi epi, DevState *st, RedSlot slot, int fin, int inner_step)
+{
+    if (stopped(st, inner_step))
+        return;
+    const int G = (int)gridDim.x, w = xcd_remap((int)blockIdx.x, G);
+    const int per = ntiles / G, rem = ntiles % G;
+    const int g0 = w * per + min(w, rem), g1 = g0 + per + (w < rem ? 1 : 0);
+    for (int g = g0; g < g1; ++g) {
+        const int row0 = sliced_wave_row(g);
+        if (row0 < sv.n)
+            sliced_rows<T>(row0, sv, val, x, epi);
+    }
+    double v[NR], tot[NR];
+    v[0] = epi.a1;
+    if (NR > 1)
+        v[NR - 1] = epi.a2;
+    if (block_reduce_finish<RS_SPMV_BS, NR>(slot, v, tot) && threadIdx.x == 0)
+        finish_scalar(st, fin, tot[0], NR > 1 ? tot[NR - 1] : 0.0);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(RS_SPMV_BS)
+    k_spmv_skip_sl(SlView sv, const T *__restrict__ val, const T *__restrict__ x,
+                   T *__restrict__ y, const DevState *st, int inner_step)
+{
+    if (stopped(st, inner_step))
+        return;
+    const int row0 = sliced_wave_row(xcd_remap((int)blockIdx.x, (int)gridDim.x));
+    if (row0 >= sv.n)
+        return;
+    EpiMult<T> epi{y};
+    sliced_rows<T>(row0, sv, val, x, epi);
+}
+
 // y = A x without a reduction, skipped when the solve has stopped
 template <typename T, bool B3>
 __global__ void __launch_bounds__(RS_SPMV_BS)
@@ -275,6 +317,18 @@ __global__ void __launch_bounds__(RS_SPMV_BS)
         return;
     EpiMult<T> epi{y};
     spmv_any<T, B3>(wg_rows, rowptr, colind, bm, val, x, epi);
+}
+
+// An empty asm statement taking a loaded vector's elements as inputs: the
+// loads must complete before it, so they cannot be sunk under a later
+// condition.  It emits no instruction.
+template <typename T, int N>
+__device__ __forceinline__ void pin_vec(const T *e)
+{
+    if constexpr (N == 4)
+        asm volatile("" ::"v"(e[0]), "v"(e[1]), "v"(e[2]), "v"(e[3]));
+    else
+        asm volatile("" ::"v"(e[0]), "v"(e[1]));
 }
 
 // 16-byte vector of T for the solver's internal vectors (leading dimension
@@ -1050,14 +1104,42 @@ __global__ void __launch_bounds__(RS_VBS)
     if (!last)
         return;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    for (int k = wid; k < cnt; k += NW) {
-        double acc = 0;
-        for (int b = lane; b < G; b += 64)
-            acc += __hip_atomic_load(&slot.partials[(size_t)k * G + b], __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT);
-        acc = wave_sum(acc);
-        if (lane == 0)
-            st->H[k + i * ld] = acc;
+    // wave wid sums columns wid, wid + NW, ...: per lane the partials
+    // b = lane, lane + 64, ... in that order.  All loads of a batch (8
+    // columns x 4 partials per lane) are issued before the first add, so the
+    // finishing group waits for one memory round trip per batch instead of
+    // one per column and partial.
+    for (int k0 = wid; k0 < cnt; k0 += 8 * NW) {
+        double acc[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            acc[c] = 0;
+        for (int b0 = lane; b0 < G; b0 += 256) {
+            double t[8][4];
+#pragma unroll
+            for (int c = 0; c < 8; ++c)
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    t[c][u] = __hip_atomic_load(
+                        &slot.partials[(size_t)min(k0 + c * NW, cnt - 1) * G +
+                                       min(b0 + 64 * u, G - 1)],
+                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                acc[c] += t[c][0];
+#pragma unroll
+                for (int u = 1; u < 4; ++u) {
+                    const double a = acc[c] + t[c][u];
+                    acc[c] = b0 + 64 * u < G ? a : acc[c];
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const double r = wave_sum(acc[c]);
+            if (lane == 0 && k0 + c * NW < cnt)
+                st->H[k0 + c * NW + i * ld] = r;
+        }
     }
 }
 
@@ -1080,21 +1162,30 @@ __global__ void __launch_bounds__(RS_VBS)
     for (int j = blockIdx.x * RS_VBS + threadIdx.x; j < nv; j += stride) {
         VT wv = reinterpret_cast<const VT *>(w)[j];
         T *we = reinterpret_cast<T *>(&wv);
-        // eight basis vectors per step, loaded unconditionally (indices past
-        // i clamp to v_i) so the loads overlap; updates stay in k order
-        for (int k0 = 0; k0 <= i; k0 += 8) {
-            VT kv[8];
+        // sixteen basis vectors and coefficients per step, loaded
+        // unconditionally (indices past i clamp to v_i) and pinned before any
+        // update, so all loads overlap and none sinks under the skip rule;
+        // updates stay in k order, skipped ones selected away
+        for (int k0 = 0; k0 <= i; k0 += 16) {
+            VT kv[16];
+            T hq[16];
 #pragma unroll
-            for (int q = 0; q < 8; ++q)
-                kv[q] = reinterpret_cast<const VT *>(V + (size_t)min(k0 + q, i) * ldv)[j];
+            for (int q = 0; q < 16; ++q) {
+                const int k = min(k0 + q, i);
+                kv[q] = reinterpret_cast<const VT *>(V + (size_t)k * ldv)[j];
+                hq[q] = (T)(-Hc[k]);
+            }
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const T h = (T)(-Hc[min(k0 + q, i)]);
-                if (k0 + q <= i && h != T(0)) {
-                    const T *ke = reinterpret_cast<const T *>(&kv[q]);
+            for (int q = 0; q < 16; ++q)
+                pin_vec<T, N>(reinterpret_cast<T *>(&kv[q]));
 #pragma unroll
-                    for (int c = 0; c < N; ++c)
-                        we[c] += h * ke[c];
+            for (int q = 0; q < 16; ++q) {
+                const bool on = k0 + q <= i && hq[q] != T(0);
+                const T *ke = reinterpret_cast<const T *>(&kv[q]);
+#pragma unroll
+                for (int c = 0; c < N; ++c) {
+                    const T u = we[c] + hq[q] * ke[c];
+                    we[c] = on ? u : we[c];
                 }
             }
         }
@@ -1338,10 +1429,46 @@ static int red_grid(int ngroups)
 }
 
 template <typename T, class Epi, int NR>
+static int red_grid_sl(int ntiles)
+{
+    static int cap = 0;
+    if (cap == 0) {
+        int per_cu = 0, cus = 0, dev = 0;
+        RS_HIP(hipGetDevice(&dev));
+        RS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, (const void *)k_spmv_red_sl<T, Epi, NR>, RS_SPMV_BS, 0));
+        RS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        cap = std::max(1, per_cu) * std::max(1, cus);
+    }
+    return std::max(1, std::min(ntiles, cap));
+}
+
+// The solve streams the sliced copy of A in precision T when it holds one:
+// FP64 when this solve filled it, FP32 whenever it exists (rsgpu_csr_sync_f32
+// fills it with the FP32 CSR copy).
+template <typename T>
+static const T *sliced_vals(const rsgpu_solver *S, const rsgpu_csr *A)
+{
+    if (!A->sl.ok())
+        return nullptr;
+    if (sizeof(T) == 8)
+        return S->sl64 == A ? (const T *)A->sl.v64.p : nullptr;
+    return (const T *)A->sl.v32.p;
+}
+
+template <typename T, class Epi, int NR>
 void launch_red(rsgpu_solver *S, rsgpu_csr *A, const T *x, Epi epi, DevState *st, int fin,
                 int inner)
 {
     hipStream_t s = S->ctx->stream;
+    if (const T *sv = sliced_vals<T>(S, A)) {
+        const int ntiles = sliced_tiles(A->n);
+        hipLaunchKernelGGL((k_spmv_red_sl<T, Epi, NR>), dim3(red_grid_sl<T, Epi, NR>(ntiles)),
+                           dim3(RS_SPMV_BS), 0, s, A->slview(), ntiles, sv, x, epi, st,
+                           slot_of(S), fin, inner);
+        RS_CHECK_LAUNCH();
+        return;
+    }
     const SpmvPlan &pl = spmv_plan<T>(A);
     const dim3 g(A->bs == 3 ? red_grid<T, Epi, NR, true>(pl.nwg)
                             : red_grid<T, Epi, NR, false>(pl.nwg)),
@@ -1378,6 +1505,12 @@ template <typename T>
 void launch_spmv_skip(rsgpu_solver *S, rsgpu_csr *A, const T *x, T *y, DevState *st)
 {
     hipStream_t s = S->ctx->stream;
+    if (const T *sv = sliced_vals<T>(S, A)) {
+        hipLaunchKernelGGL(k_spmv_skip_sl<T>, dim3(sliced_tiles(A->n)), dim3(RS_SPMV_BS), 0,
+                           s, A->slview(), sv, x, y, st, 1);
+        RS_CHECK_LAUNCH();
+        return;
+    }
     const SpmvPlan &pl = spmv_plan<T>(A);
     const dim3 g(pl.nwg), b(RS_SPMV_BS);
     if (A->bs == 3)
@@ -1998,7 +2131,7 @@ extern "C" int rsgpu_solver_solve(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M,
         const int m = std::max(1, S->o.restart);
         const size_t need =
             (size_t)2 * std::max({A->plan.nwg, A->plan32.nwg, A->bplan.nwg, A->bplan32.nwg,
-                                  grid_for(A->n)}) + 2;
+                                  grid_for(A->n), sliced_tiles(A->n)}) + 2;
         const size_t need_cgs = (size_t)(m + 2) * grid_for(A->n);
         if (S->partials.n < std::max(need, need_cgs) + 4096)
             S->partials.alloc(std::max(need, need_cgs) + 4096);
@@ -2006,6 +2139,12 @@ extern "C" int rsgpu_solver_solve(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M,
                    S->o.dtol, m, S->gmres_small.p);
         if (stats)
             std::memset(stats, 0, sizeof(*stats));
+        // FP64 solves stream the sliced copy, refreshed from A's values here
+        S->sl64 = nullptr;
+        if (!S->o.mixed && A->sl.ok()) {
+            sliced_fill<double>(A, ctx->stream);
+            S->sl64 = A;
+        }
         if (S->o.mixed) {
             run_mixed(S, A, M, b_dev, x_dev, S->st, stats);
         } else if (S->o.kind == RSGPU_IR) {

@@ -97,6 +97,38 @@ struct SpmvPlan {
     const short *coff_ptr() const { return coff.n ? coff.p : nullptr; }
 };
 
+// Sliced copy of a CSR pattern for stencil-like matrices: rows in chunks of
+// RS_SL_ROWS (one workgroup of four waves), each chunk with one table, the
+// ascending union of its rows' column offsets (column - row, shared by
+// chunks with equal unions), and per row a bit mask of the table slots the
+// row holds.  Values are stored slot-major per chunk (value of chunk c,
+// slot s, row c * RS_SL_ROWS + i at (cbase[c] + s) * RS_SL_ROWS + i, zero
+// where the row lacks the slot), so a wave reads one slot of its 64 rows as
+// one coalesced run and gathers x at row + offset, contiguous across the
+// wave.  Built when every chunk's union has at most 32 offsets; absent
+// otherwise (nchunks == 0).
+#define RS_SL_ROWS 256
+struct SlicedFmt {
+    int nchunks = 0, ntables = 0;
+    long long nslots = 0;              // values per copy (64 per slot row)
+    DevBuf<int> ctab, cbase, tptr, toff;
+    DevBuf<unsigned> mask;
+    DevBuf<double> v64;                // filled by sliced_fill (per solve)
+    DevBuf<float> v32;                 // filled with the FP32 CSR copy
+    bool v64_ready = false;
+    bool ok() const { return nchunks > 0; }
+};
+
+// 256-row tiles (one workgroup each) of the sliced kernels
+inline int sliced_tiles(int n) { return (n + 255) / 256; }
+
+// what the sliced kernels read (plain pointers)
+struct SlView {
+    int n, ncols, nchunks;
+    const int *ctab, *cbase, *tptr, *toff;
+    const unsigned *mask;
+};
+
 struct rsgpu_csr {
     rsgpu_ctx *ctx = nullptr;
     int n = 0, ncols = 0, nnz = 0;
@@ -121,6 +153,12 @@ struct rsgpu_csr {
     int nb = 0, nnzb = 0;
     DevBuf<int> brp, bcol;
     SpmvPlan bplan, bplan32;
+    SlicedFmt sl;
+    SlView slview() const
+    {
+        return SlView{n, ncols, sl.nchunks, sl.ctab.p, sl.cbase.p, sl.tptr.p, sl.toff.p,
+                      sl.mask.p};
+    }
 };
 
 // Per-connection transmissibility lists of a mesh (flux.hip): rows =
@@ -313,6 +351,32 @@ __device__ __forceinline__ double block_sum(double v, double *sh)
     return v;   // valid in thread 0
 }
 
+// Sum of p[i] for i = i0 + threadIdx.x, i0 + threadIdx.x + BS, ... < i1, in
+// that order, read with agent-scope atomic loads (the partials of other
+// workgroups).  Eight loads are issued before the first add: a load-add
+// loop waits for each coherent load in turn, a memory round trip per
+// iteration.  Indices past i1 re-read the last valid partial and are not
+// added.
+template <int BS>
+__device__ __forceinline__ double gather_sum(const double *p, int i0, int i1)
+{
+    double acc = 0;
+    for (int i = i0 + (int)threadIdx.x; i < i1; i += 8 * BS) {
+        double t[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            t[u] = __hip_atomic_load(&p[min(i + u * BS, i1 - 1)], __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+        acc += t[0];
+#pragma unroll
+        for (int u = 1; u < 8; ++u) {
+            const double a = acc + t[u];
+            acc = i + u * BS < i1 ? a : acc;
+        }
+    }
+    return acc;
+}
+
 // Publish this group's partial(s); returns true in every thread of the
 // finishing group, whose thread 0 then holds the totals in tot[].
 //
@@ -374,13 +438,9 @@ __device__ bool block_reduce_finish(RedSlot slot, const double (&v)[NR],
         return false;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
 #pragma unroll
-    for (int q = 0; q < NR; ++q) {
-        double acc = 0;
-        for (int i = threadIdx.x; i < (int)gridDim.x; i += BS)
-            acc += __hip_atomic_load(&slot.partials[(size_t)q * gridDim.x + i],
-                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        tot[q] = block_sum<BS>(acc, sh);
-    }
+    for (int q = 0; q < NR; ++q)
+        tot[q] = block_sum<BS>(
+            gather_sum<BS>(slot.partials + (size_t)q * gridDim.x, 0, (int)gridDim.x), sh);
     return true;
 }
 
@@ -444,7 +504,14 @@ void spmv_build_plan(const std::vector<int> &rowptr, int n, SpmvPlan &plan,
 void spmv_build_coff(const std::vector<int> &rowptr, const std::vector<int> &colind,
                      const std::vector<int> &starts, DevBuf<short> &coff, hipStream_t s);
 // both plans of a CSR matrix (FP64 and FP32 stage sizes)
-void spmv_build_plans(const std::vector<int> &rowptr, int n, rsgpu_csr *A, hipStream_t s);
+void spmv_build_plans(const std::vector<int> &rowptr, int n, rsgpu_csr *A, hipStream_t s,
+                      bool sliced = true);
+// sliced values from A->val (T = double: A->sl.v64, float: A->sl.v32)
+template <typename T>
+void sliced_fill(rsgpu_csr *A, hipStream_t s);
+// y = A x over the sliced copy of precision T (mult's zero rule)
+template <typename T>
+void sliced_mult(hipStream_t s, const rsgpu_csr *A, const T *x, T *y);
 // block-row plans of a 3x3 block matrix (scalar row bounds at block rows)
 void spmv_build_plan_b3(const std::vector<int> &brp, int nb, SpmvPlan &plan, hipStream_t s,
                         int nnz_wg);

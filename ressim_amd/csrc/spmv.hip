@@ -1,5 +1,7 @@
 // CSR SpMV entry kernels, transposed copy, zeroRows, conversions.
 #include <algorithm>
+#include <cstdlib>
+#include <map>
 #include <numeric>
 
 #include "spmv_core.hpp"
@@ -57,11 +59,181 @@ void spmv_build_plan(const std::vector<int> &rowptr, int n, SpmvPlan &plan,
     RS_HIP(hipStreamSynchronize(s));
 }
 
-void spmv_build_plans(const std::vector<int> &rowptr, int n, rsgpu_csr *A, hipStream_t s)
+// ---------------------------------------------------------------- sliced
+// RSGPU_SLICED=0 leaves the sliced copy out (the solvers then run the CSR core)
+static bool sliced_enabled()
+{
+    const char *e = std::getenv("RSGPU_SLICED");
+    return !(e && e[0] == '0');
+}
+
+// Chunk tables, shared tables and row masks of SlicedFmt from the host
+// pattern.  The format is left out when a chunk's union exceeds 32 offsets,
+// a row repeats a column, or the slot count passes 2^31 slot rows.
+static void sliced_build(rsgpu_csr *A, hipStream_t s)
+{
+    SlicedFmt &F = A->sl;
+    F.nchunks = F.ntables = 0;
+    F.nslots = 0;
+    F.v64_ready = false;
+    F.v64.release();
+    F.v32.release();
+    const int n = A->n;
+    if (n == 0 || A->bs != 1 || !sliced_enabled())
+        return;
+    const std::vector<int> &rp = A->h_rowptr, &ci = A->h_colind;
+    const int nch = (n + RS_SL_ROWS - 1) / RS_SL_ROWS;
+    std::vector<int> ctab(nch), cbase(nch), tptr{0}, toff;
+    std::vector<unsigned> mask(n, 0u);
+    std::map<std::vector<int>, int> tables;
+    std::vector<int> u, merged;
+    long long rows = 0;
+    for (int c = 0; c < nch; ++c) {
+        const int r0 = RS_SL_ROWS * c, r1 = std::min(n, r0 + RS_SL_ROWS);
+        u.clear();
+        for (int r = r0; r < r1; ++r) {
+            size_t i = 0;
+            int j = rp[r];
+            const int e = rp[r + 1];
+            merged.clear();
+            while (i < u.size() || j < e) {
+                if (j > rp[r] && j < e && ci[j] == ci[j - 1])
+                    return;
+                const int d = j < e ? ci[j] - r : 0;
+                if (j == e || (i < u.size() && u[i] < d)) {
+                    merged.push_back(u[i++]);
+                } else if (i == u.size() || d < u[i]) {
+                    merged.push_back(d);
+                    ++j;
+                } else {
+                    merged.push_back(u[i++]);
+                    ++j;
+                }
+            }
+            u.swap(merged);
+            if (u.size() > 32)
+                return;
+        }
+        for (int r = r0; r < r1; ++r) {
+            unsigned mk = 0;
+            size_t i = 0;
+            for (int j = rp[r]; j < rp[r + 1]; ++j) {
+                while (u[i] != ci[j] - r)
+                    ++i;
+                mk |= 1u << i;
+            }
+            mask[r] = mk;
+        }
+        auto it = tables.find(u);
+        int t;
+        if (it == tables.end()) {
+            t = (int)tables.size();
+            tables.emplace(u, t);
+            toff.insert(toff.end(), u.begin(), u.end());
+            tptr.push_back((int)toff.size());
+        } else {
+            t = it->second;
+        }
+        ctab[c] = t;
+        cbase[c] = (int)rows;
+        rows += (long long)u.size();
+        if (rows >= (1LL << 31))
+            return;
+    }
+    // the kernels read 32 offsets from every table's start
+    toff.insert(toff.end(), 32, 0);
+    F.ctab.upload(ctab, s);
+    F.cbase.upload(cbase, s);
+    F.tptr.upload(tptr, s);
+    F.toff.upload(toff, s);
+    F.mask.upload(mask, s);
+    RS_HIP(hipStreamSynchronize(s));
+    F.ntables = (int)tables.size();
+    F.nslots = rows * RS_SL_ROWS;
+    F.nchunks = nch;
+}
+
+void spmv_build_plans(const std::vector<int> &rowptr, int n, rsgpu_csr *A, hipStream_t s,
+                      bool sliced)
 {
     spmv_build_plan(rowptr, n, A->plan, s, RS_SPMV_NNZ, &A->h_colind);
     spmv_build_plan(rowptr, n, A->plan32, s, RS_SPMV_NNZ32, &A->h_colind);
+    if (sliced)
+        sliced_build(A, s);
 }
+
+// sliced values of chunk c, slot s from the CSR values (zero where the row
+// lacks the slot); one row per thread
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_sliced_fill(SlView S, const int *__restrict__ rowptr, const double *__restrict__ val,
+                  T *__restrict__ out)
+{
+    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+    const int c = (int)(g / RS_SL_ROWS), i = (int)(g % RS_SL_ROWS);
+    if (c >= S.nchunks)
+        return;
+    const int r = c * RS_SL_ROWS + i;
+    const int t = S.ctab[c];
+    const int W = S.tptr[t + 1] - S.tptr[t];
+    const unsigned m = r < S.n ? S.mask[r] : 0u;
+    int j = r < S.n ? rowptr[r] : 0;
+    T *o = out + (size_t)S.cbase[c] * RS_SL_ROWS + i;
+    for (int sl = 0; sl < W; ++sl) {
+        T v = T(0);
+        if ((m >> sl) & 1u)
+            v = (T)val[j++];
+        o[(size_t)sl * RS_SL_ROWS] = v;
+    }
+}
+
+template <typename T>
+static DevBuf<T> &sliced_buf(rsgpu_csr *A);
+template <>
+DevBuf<double> &sliced_buf<double>(rsgpu_csr *A) { return A->sl.v64; }
+template <>
+DevBuf<float> &sliced_buf<float>(rsgpu_csr *A) { return A->sl.v32; }
+
+template <typename T>
+void sliced_fill(rsgpu_csr *A, hipStream_t s)
+{
+    SlicedFmt &F = A->sl;
+    if (!F.ok())
+        return;
+    DevBuf<T> &b = sliced_buf<T>(A);
+    b.alloc((size_t)F.nslots);
+    const long long threads = (long long)F.nchunks * RS_SL_ROWS;
+    hipLaunchKernelGGL(k_sliced_fill<T>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s,
+                       A->slview(), A->rowptr.p, A->val.p, b.p);
+    RS_CHECK_LAUNCH();
+    if (sizeof(T) == 8)
+        F.v64_ready = true;
+}
+template void sliced_fill<double>(rsgpu_csr *, hipStream_t);
+template void sliced_fill<float>(rsgpu_csr *, hipStream_t);
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_spmv_sliced(SlView S, const T *__restrict__ val, const T *__restrict__ x,
+                  T *__restrict__ y)
+{
+    const int row0 = sliced_wave_row(xcd_remap((int)blockIdx.x, (int)gridDim.x));
+    if (row0 >= S.n)
+        return;
+    EpiMult<T> epi{y};
+    sliced_rows<T>(row0, S, val, x, epi);
+}
+
+template <typename T>
+void sliced_mult(hipStream_t s, const rsgpu_csr *A, const T *x, T *y)
+{
+    const T *val = sizeof(T) == 8 ? (const T *)A->sl.v64.p : (const T *)A->sl.v32.p;
+    hipLaunchKernelGGL(k_spmv_sliced<T>, dim3(sliced_tiles(A->n)), dim3(256), 0, s,
+                       A->slview(), val, x, y);
+    RS_CHECK_LAUNCH();
+}
+template void sliced_mult<double>(hipStream_t, const rsgpu_csr *, const double *, double *);
+template void sliced_mult<float>(hipStream_t, const rsgpu_csr *, const float *, float *);
 
 // Block-row plan of a 3x3 block matrix: whole block rows per group (at most
 // RS_SPMV_BS / 3 of them and nnz_wg / 9 blocks); a block row with more

@@ -63,15 +63,15 @@ __device__ __forceinline__ void spmv_core_off_g(int g, const int *__restrict__ w
 #pragma unroll
                 for (int q = 0; q < Q; ++q) {
                     const int j = min(j0 + (int)threadIdx.x + q * BS, j1 - 1);
-                    v[q] = __builtin_nontemporal_load(&val[j]);
-                    c[q] = r0 + (int)__builtin_nontemporal_load(&coff[j]);
+                    v[q] = val[j];
+                    c[q] = r0 + (int)coff[j];
                 }
             } else {
 #pragma unroll
                 for (int q = 0; q < Q; ++q) {
                     const int j = min(j0 + (int)threadIdx.x + q * BS, j1 - 1);
-                    v[q] = __builtin_nontemporal_load(&val[j]);
-                    c[q] = __builtin_nontemporal_load(&colind[j]);
+                    v[q] = val[j];
+                    c[q] = colind[j];
                 }
             }
             T xg[Q];
@@ -258,6 +258,91 @@ __device__ __forceinline__ void spmv_core_b3(const int *__restrict__ wg_rows,
 {
     spmv_core_b3_g<T, BS, NNZ_WG>(xcd_remap((int)blockIdx.x, (int)gridDim.x), wg_rows, rowptr,
                                   colind, brp, bcol, val, x, epi);
+}
+
+// Sliced core (SlicedFmt): chunks of RS_SL_ROWS rows, one wave per 64 of
+// them, one row per lane.  A row's present slots are visited in table
+// order, which is its CSR order (ascending columns), and each product is
+// rounded to T before it is added, as in the CSR core, so every row sum is
+// bit-identical to the sequential loop.  Absent slots are skipped rather
+// than added as zeros, so non-finite x entries outside a row's pattern never
+// reach it.  The chunk's table is read up front as 32 offsets (wide scalar
+// loads; the offset array is padded by 32 entries), slots past the table
+// reuse its last offset and value slot, and gather columns clamp into
+// [0, ncols), so every address is valid and all loads of a batch of 16
+// slots are in flight before the first product.  A workgroup's four waves
+// read adjacent 64-row pieces of each slot row, so one slot of the group is
+// one contiguous run.  Values are read with ordinary loads: the
+// nontemporal hint keeps them out of the last-level cache, which the FP32
+// inner solve's working set largely fits.
+#define RS_SL_BATCH 16
+template <typename T, class Epi>
+__device__ __forceinline__ void sliced_batch(int s0, int W, int r, int cmax, unsigned m,
+                                             const int (&offs)[32], const T *vc,
+                                             const T *__restrict__ x, T &dot)
+{
+    T v[RS_SL_BATCH], xg[RS_SL_BATCH];
+#pragma unroll
+    for (int k = 0; k < RS_SL_BATCH; ++k) {
+        const int sl = min(s0 + k, W - 1);
+        const int col = min(max(r + offs[s0 + k], 0), cmax);
+        v[k] = vc[(size_t)sl * RS_SL_ROWS];
+        xg[k] = x[col];
+    }
+    // every product and sum is formed and then selected: a conditional add
+    // lets the compiler sink the slot's loads under the condition, where
+    // each waits for memory on its own
+#pragma unroll
+    for (int k = 0; k < RS_SL_BATCH; ++k) {
+        const T q = dot + v[k] * xg[k];
+        dot = ((m >> (s0 + k)) & 1u) ? q : dot;
+    }
+}
+
+// the 64 rows from row0 (wave-uniform, row0 < n)
+template <typename T, class Epi>
+__device__ __forceinline__ void sliced_rows(int row0, const SlView &S, const T *__restrict__ val,
+                                            const T *__restrict__ x, Epi &epi)
+{
+    const int lane = threadIdx.x & 63;
+    const int c = row0 / RS_SL_ROWS, sub = row0 - c * RS_SL_ROWS;
+    const int r = row0 + lane;
+    const int t = S.ctab[c];
+    const int o0 = S.tptr[t], W = S.tptr[t + 1] - o0;
+    const T *vc = val + (size_t)S.cbase[c] * RS_SL_ROWS + sub + lane;
+    const int *off = S.toff + o0;
+    // lanes past the last row compute a copy of the last row and store
+    // nothing (no condition on them before the store, which would again
+    // let the compiler sink loads under it)
+    const bool live = r < S.n;
+    const int rr = min(r, S.n - 1);
+    const unsigned m = S.mask[rr];
+    const int cmax = S.ncols - 1;
+    T dot = epi.init(rr);
+    // an opaque start value: selecting between a product and a constant
+    // start is otherwise turned into a branch around the first slot's loads
+    asm volatile("" : "+v"(dot));
+    if (W > 0) {
+        const int lastoff = off[W - 1];
+        int offs[32];
+#pragma unroll
+        for (int k = 0; k < 32; ++k)
+            offs[k] = off[k];
+#pragma unroll
+        for (int k = 0; k < 32; ++k)
+            offs[k] = k < W ? offs[k] : lastoff;
+        sliced_batch<T, Epi>(0, W, rr, cmax, m, offs, vc, x, dot);
+        if (W > RS_SL_BATCH)
+            sliced_batch<T, Epi>(RS_SL_BATCH, W, rr, cmax, m, offs, vc, x, dot);
+    }
+    if (live)
+        epi.row(r, dot);
+}
+
+// first row of the calling wave in 256-row tile `tile` (wave-uniform)
+__device__ __forceinline__ int sliced_wave_row(int tile)
+{
+    return tile * 256 + 64 * __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
 }
 
 // y = A x with CompRowMatrix.mult's rule: y[i] = dot only if dot != 0.

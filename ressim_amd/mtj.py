@@ -293,6 +293,28 @@ class CompRowMatrix:
     def sync_f32(self):
         _raise(L.load().rsgpu_csr_sync_f32(self.h), self.ctx.h)
 
+    def sliced_info(self):
+        """(chunks, value slots, tables) of the sliced copy; chunks 0 when the
+        pattern has none (a 64-row chunk spans more than 32 column offsets)."""
+        a, b, c = C.c_int(), C.c_int64(), C.c_int()
+        _raise(L.load().rsgpu_csr_sliced_info(self.h, C.byref(a), C.byref(b), C.byref(c)),
+               self.ctx.h)
+        return a.value, b.value, c.value
+
+    def sync_sliced(self):
+        """Refresh the FP64 sliced copy from the current values."""
+        _raise(L.load().rsgpu_csr_sync_sliced(self.h), self.ctx.h)
+
+    def mult_sliced(self, x, y):
+        """y = A x over the sliced copy (device tensors, FP64 after
+        sync_sliced or FP32 after sync_f32)."""
+        _torch_sync(x, y)
+        f = (L.load().rsgpu_csr_mult_sliced_f32 if str(x.dtype) == "torch.float32"
+             else L.load().rsgpu_csr_mult_sliced)
+        _raise(f(self.h, _dptr(x), _dptr(y)), self.ctx.h)
+        self.ctx.synchronize()
+        return y
+
     def mult(self, x, y):
         """y = A x (CompRowMatrix.java:301-333)."""
         lib = L.load()

@@ -110,6 +110,108 @@ def test_spmv_f32_bitexact(gpu, oracle, systems):
                                                                    x))
 
 
+def _banded(rng, n, half, empty=()):
+    """Random pattern inside a band (every 64-row chunk's offsets fit 32
+    slots), rows of varying subsets, some empty rows."""
+    rows = []
+    for i in range(n):
+        if i in empty:
+            rows.append(np.zeros(0, np.int32))
+            continue
+        lo, hi = max(0, i - half), min(n, i + half + 1)
+        k = rng.integers(1, hi - lo + 1)
+        rows.append(np.sort(rng.choice(np.arange(lo, hi), size=k, replace=False)).astype(np.int32))
+    rp = np.zeros(n + 1, np.int32)
+    rp[1:] = np.cumsum([len(r) for r in rows])
+    return rp, np.concatenate(rows).astype(np.int32), rng.standard_normal(int(rp[-1]))
+
+
+def _sliced_products(gpu, A, x):
+    import torch
+    xd = torch.tensor(x, device="cuda")
+    yd = torch.empty_like(xd)
+    A.mult_sliced(xd, yd)
+    return yd.cpu().numpy()
+
+
+@pytest.mark.parametrize("case", ["c1", "s3", "banded"])
+def test_sliced_spmv_bitexact(gpu, oracle, systems, case):
+    """The sliced copy (64-row chunks sharing column-offset tables, per-row
+    slot masks): FP64 and FP32 products bit-identical to the oracle's
+    row-sequential sums, including rows next to non-finite x entries they do
+    not reference, a partial last chunk and empty rows."""
+    rng = np.random.default_rng(5)
+    if case == "banded":
+        n = 1000
+        rp, ci, d = _banded(rng, n, 12, empty={0, 63, 64, 999})
+    else:
+        s = systems[case]
+        rp, ci, d = s["rowptr"], s["colind"], s["data"]
+        n = len(rp) - 1
+    A = _mat(gpu, rp, ci, d, n, n)
+    nch, nslots, ntab = A.sliced_info()
+    assert nch == (n + 255) // 256 and nslots % 64 == 0 and nslots >= len(ci) and ntab >= 1
+    A.sync_sliced()
+    A.sync_f32()
+    x = rng.standard_normal(n)
+    np.testing.assert_array_equal(_sliced_products(gpu, A, x), oracle.mult(rp, ci, d, x))
+    x32 = x.astype(np.float32)
+    np.testing.assert_array_equal(_sliced_products(gpu, A, x32),
+                                  oracle.mult_f32(rp, ci, d.astype(np.float32), x32))
+    # inf and NaN in x reach exactly the rows whose pattern holds them
+    xb = x.copy()
+    xb[n // 3] = np.inf
+    xb[n // 2] = np.nan
+    yo = oracle.mult(rp, ci, d, xb)
+    assert np.isfinite(yo).sum() > n // 2
+    np.testing.assert_array_equal(_sliced_products(gpu, A, xb), yo)
+    # new values need a new sync; the copy follows them
+    d2 = rng.standard_normal(len(d))
+    A.set_data(d2)
+    A.sync_sliced()
+    np.testing.assert_array_equal(_sliced_products(gpu, A, x), oracle.mult(rp, ci, d2, x))
+
+
+def test_sliced_absent_when_offsets_spread(gpu, oracle):
+    """Chunks whose rows span more than 32 column offsets keep the CSR path
+    only: no sliced copy, and its entry points refuse."""
+    import torch
+    rng = np.random.default_rng(6)
+    n = 300
+    rp, ci, d = random_csr(rng, n, n, 0.05)
+    A = _mat(gpu, rp, ci, d, n, n)
+    assert A.sliced_info()[0] == 0
+    with pytest.raises(ValueError):
+        A.sync_sliced()
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError):
+        A.mult_sliced(x, torch.empty_like(x))
+
+
+def test_sliced_env_switch_and_solve(gpu, oracle, systems, monkeypatch):
+    """RSGPU_SLICED=0 builds no sliced copy; FP64 GMRES + ILU over the
+    sliced copy and over CSR reach the same solution."""
+    s = systems["s3"]
+    rp, ci, d, b = s["rowptr"], s["colind"], s["data"], s["b"]
+    n = len(rp) - 1
+    monkeypatch.setenv("RSGPU_SLICED", "0")
+    A0 = _mat(gpu, rp, ci, d, n, n)
+    monkeypatch.delenv("RSGPU_SLICED")
+    A1 = _mat(gpu, rp, ci, d, n, n)
+    assert A0.sliced_info()[0] == 0 and A1.sliced_info()[0] > 0
+    xs = []
+    for A in (A0, A1):
+        solver = gpu.GMRES(b, 30)
+        solver.setIterationMonitor(gpu.DefaultIterationMonitor(100000, 1e-12, 1e-50, 1e5))
+        M = gpu.ILU()
+        M.setMatrix(A)
+        solver.setPreconditioner(M)
+        x = np.zeros(n)
+        solver.solve(A, b, x)
+        xs.append(x)
+    assert np.linalg.norm(xs[0] - xs[1]) / np.linalg.norm(xs[0]) < 1e-9
+
+
 def test_add_zero_rows_patterns(gpu, oracle):
     rng = np.random.default_rng(3)
     rp, ci, d = random_csr(rng, 50, 50, 0.1, diag=4.0)
