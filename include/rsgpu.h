@@ -186,6 +186,12 @@ int rsgpu_prec_get_factors(const rsgpu_prec *M, int32_t *rowptr_host,
                            int32_t *colind_host, double *lu_host, int *nnz);
 int rsgpu_prec_info(const rsgpu_prec *M, int *nnz, int *nlevels_lower,
                     int *nlevels_upper);
+/* Which sweep kernel ILU.apply runs (no MTJ counterpart; all three are
+ * bit-identical to the sequential solves): 0 level schedules, 1 line
+ * wavefront (every block Ly <= 64 lines of Lx rows coupled among 9-point
+ * neighbours; RSGPU_ILU_BAND=0 in the environment turns it off), 2 3x3 cell
+ * schedules. */
+int rsgpu_prec_sweep_form(const rsgpu_prec *M, int *form);
 
 /* AMG parameters: the constructor arguments of AMG(omegaPreF, omegaPreR,
  * omegaPostF, omegaPostR, nu1, nu2, gamma, min, omega)

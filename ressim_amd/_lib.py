@@ -105,6 +105,7 @@ SIGNATURES = {
     "rsgpu_csr_sliced_info": (C.c_int, [_h, C.POINTER(C.c_int), C.POINTER(C.c_int64),
                                         C.POINTER(C.c_int)]),
     "rsgpu_csr_sync_sliced": (C.c_int, [_h]),
+    "rsgpu_prec_sweep_form": (C.c_int, [_h, C.POINTER(C.c_int)]),
     "rsgpu_csr_mult_sliced": (C.c_int, [_h, _vp, _vp]),
     "rsgpu_csr_mult_sliced_f32": (C.c_int, [_h, _vp, _vp]),
     "rsgpu_csr_mult_host": (C.c_int, [_h, _dp, _dp]),

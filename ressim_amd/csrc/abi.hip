@@ -856,6 +856,14 @@ extern "C" int rsgpu_prec_get_factors(const rsgpu_prec *M, int32_t *rowptr,
     RS_CATCH(M->ctx)
 }
 
+extern "C" int rsgpu_prec_sweep_form(const rsgpu_prec *M, int *form)
+{
+    if (!M || !form)
+        return RSGPU_BAD_ARGUMENT;
+    *form = M->cell ? 2 : M->band ? 1 : 0;
+    return RSGPU_OK;
+}
+
 extern "C" int rsgpu_prec_info(const rsgpu_prec *M, int *nnz, int *nl, int *nu)
 {
     if (!M)

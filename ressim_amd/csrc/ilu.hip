@@ -36,6 +36,12 @@
 
 #define RS_LDS_BYTES 65536
 
+// Records are padded by RS_BAND_PAD rows on both sides (zeros) and so is
+// the block's LDS vector, so every lane addresses its row at step s as
+// base +/- s without clamping: prefetches past either end of a line, a block
+// or the matrix read padding or a neighbouring line and are never used.
+#define RS_BAND_PAD 64
+
 struct LevelMeta {
     int count, off, roff, pad;
 };
@@ -58,6 +64,8 @@ struct HostSweep {
 
 static int lds_limit(int device);
 static int ilu_kernel_choice();
+static void band_setup(rsgpu_prec *M, const std::vector<int> &rp, const std::vector<int> &ci,
+                       const std::vector<int> &dg);
 static size_t cell_xv_bytes(int max_block_rows, int es);
 
 static int pad_width(int w)
@@ -622,6 +630,7 @@ void prec_setup(rsgpu_prec *M, rsgpu_csr *A)
         if (!M->cell)
             M->dgU32.alloc(M->U.nrows_lev);
     }
+    band_setup(M, rp, ci, dg);
     const int elem = M->precision == RSGPU_FP64 ? 8 : 4;
     M->lds_rows = RS_LDS_BYTES / elem - 1;
     if (M->max_block_rows > M->lds_rows) {
@@ -632,6 +641,132 @@ void prec_setup(rsgpu_prec *M, rsgpu_csr *A)
     }
     RS_HIP(hipStreamSynchronize(s));
     M->ready = true;
+}
+
+// Line-wavefront form of a block [b0, b1): Ly lines of Lx rows in natural
+// order (row b0 + Lx j + i), every in-block coupling of row (i, j) one of
+// L: (i-1, j-1), (i, j-1), (i+1, j-1), (i-1, j) and U: (i+1, j), (i-1, j+1),
+// (i, j+1), (i+1, j+1) -- in ascending column order the slots 0..3 of each
+// part.  Lx is read off the first row (its first column past b0 + 1); Lx >= 3
+// keeps the eight offsets distinct.  pos receives per row the factor
+// positions of L0..L3, the diagonal and U0..U3 (-1 absent).
+static bool band_block(int b0, int b1, const std::vector<int> &rp, const std::vector<int> &ci,
+                       const std::vector<int> &dg, int &Lx, int &Ly, std::vector<int> &pos)
+{
+    const int nbr = b1 - b0;
+    Lx = nbr;
+    for (int q = dg[b0] + 1; q < rp[b0 + 1]; ++q)
+        if (ci[q] >= b0 + 2) {
+            Lx = ci[q] - b0;
+            break;
+        }
+    if (Lx < 3 || nbr % Lx != 0)
+        return false;
+    Ly = nbr / Lx;
+    if (Ly > 64)
+        return false;
+    for (int r = b0; r < b1; ++r) {
+        const int i = (r - b0) % Lx, j = (r - b0) / Lx;
+        int *p = &pos[(size_t)9 * r];
+        for (int k = 0; k < 9; ++k)
+            p[k] = -1;
+        p[4] = dg[r];
+        for (int q = rp[r]; q < rp[r + 1]; ++q) {
+            const int d = ci[q] - r;
+            int slot = -1;
+            if (d == -Lx - 1 && i > 0 && j > 0)
+                slot = 0;
+            else if (d == -Lx && j > 0)
+                slot = 1;
+            else if (d == -Lx + 1 && i < Lx - 1 && j > 0)
+                slot = 2;
+            else if (d == -1 && i > 0)
+                slot = 3;
+            else if (d == 0)
+                continue;
+            else if (d == 1 && i < Lx - 1)
+                slot = 5;
+            else if (d == Lx - 1 && i > 0 && j < Ly - 1)
+                slot = 6;
+            else if (d == Lx && j < Ly - 1)
+                slot = 7;
+            else if (d == Lx + 1 && i < Lx - 1 && j < Ly - 1)
+                slot = 8;
+            if (slot < 0 || p[slot] >= 0)
+                return false;
+            p[slot] = q;
+        }
+    }
+    return true;
+}
+
+// RSGPU_ILU_BAND=0 keeps the level schedules for every block
+static void band_setup(rsgpu_prec *M, const std::vector<int> &rp, const std::vector<int> &ci,
+                       const std::vector<int> &dg)
+{
+    M->band = false;
+    M->band_dims.release();
+    M->band_pos.release();
+    M->band_mask.release();
+    M->band64.release();
+    M->band32.release();
+    const char *e = getenv("RSGPU_ILU_BAND");
+    if ((e && e[0] == '0') || M->cell || ilu_kernel_choice() == 3 || M->n == 0 ||
+        M->kind == RSGPU_PREC_ICC)
+        return;
+    const int n = M->n, nb = M->nblocks;
+    std::vector<int> pos((size_t)9 * n);
+    std::vector<int2> dims(nb);
+    for (int b = 0; b < nb; ++b) {
+        int Lx, Ly;
+        if (!band_block(M->h_bstart[b], M->h_bstart[b + 1], rp, ci, dg, Lx, Ly, pos))
+            return;
+        dims[b] = make_int2(Lx, Ly);
+    }
+    std::vector<unsigned char> mask(n);
+    for (int r = 0; r < n; ++r) {
+        unsigned m = 0;
+        for (int k = 0; k < 4; ++k) {
+            if (pos[(size_t)9 * r + k] >= 0)
+                m |= 1u << k;
+            if (pos[(size_t)9 * r + 5 + k] >= 0)
+                m |= 16u << k;
+        }
+        mask[r] = (unsigned char)m;
+    }
+    // blocks whose every row holds all of its 9-point neighbours compute the
+    // masks from the row position (marked Ly > 0); others read them (-Ly)
+    for (int b = 0; b < nb; ++b) {
+        const int b0 = M->h_bstart[b], Lx = dims[b].x, Ly = dims[b].y;
+        bool full = true;
+        for (int r = b0; full && r < M->h_bstart[b + 1]; ++r) {
+            const int i = (r - b0) % Lx, j = (r - b0) / Lx;
+            const bool w = i > 0, e = i < Lx - 1, s_ = j > 0, nn = j < Ly - 1;
+            const unsigned f = (w && s_ ? 1u : 0u) | (s_ ? 2u : 0u) | (e && s_ ? 4u : 0u) |
+                               (w ? 8u : 0u) | (e ? 16u : 0u) | (w && nn ? 32u : 0u) |
+                               (nn ? 64u : 0u) | (e && nn ? 128u : 0u);
+            full = mask[r] == f;
+        }
+        if (!full)
+            dims[b].y = -Ly;
+    }
+    hipStream_t s = M->ctx->stream;
+    M->band_dims.upload(dims, s);
+    M->band_pos.upload(pos, s);
+    // masks and records padded by RS_BAND_PAD zero rows on both sides
+    std::vector<unsigned char> mpad(mask.size() + 2 * RS_BAND_PAD, 0);
+    std::copy(mask.begin(), mask.end(), mpad.begin() + RS_BAND_PAD);
+    M->band_mask.upload(mpad, s);
+    const size_t np = (size_t)n + 2 * RS_BAND_PAD;
+    if (M->precision == RSGPU_FP64) {
+        M->band64.alloc(9 * np);
+        M->band64.zero(s);
+    } else {
+        M->band32.alloc(9 * np);
+        M->band32.zero(s);
+    }
+    RS_HIP(hipStreamSynchronize(s));
+    M->band = true;
 }
 
 // Transposed sweeps: U' (forward, subtract in ascending source row, divide)
@@ -1569,6 +1704,36 @@ static void launch_rowdiag(hipStream_t s, const SweepData &D, const int *diag, c
     RS_CHECK_LAUNCH();
 }
 
+// per-row line-wavefront records from the factor: L0..L3, U0..U3 (zero
+// where absent) of row r at 8 (r + PAD), diagonals at 8 (n + 2 PAD) + r + PAD
+// (the padding rows stay zero)
+template <typename T>
+__global__ void k_band_gather(int n, const int *__restrict__ pos, const T *__restrict__ lu,
+                              T *__restrict__ out)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n)
+        return;
+    const int *p = pos + (size_t)9 * r;
+    T *o = out + (size_t)8 * (r + RS_BAND_PAD);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        o[k] = p[k] >= 0 ? lu[p[k]] : T(0);
+        o[4 + k] = p[5 + k] >= 0 ? lu[p[5 + k]] : T(0);
+    }
+    out[(size_t)8 * ((size_t)n + 2 * RS_BAND_PAD) + r + RS_BAND_PAD] = lu[p[4]];
+}
+
+template <typename T>
+static void launch_band_gather(rsgpu_prec *M, const T *lu, T *out)
+{
+    if (!M->band)
+        return;
+    hipLaunchKernelGGL(k_band_gather<T>, dim3((M->n + 255) / 256), dim3(256), 0,
+                       M->ctx->stream, M->n, M->band_pos.p, lu, out);
+    RS_CHECK_LAUNCH();
+}
+
 int prec_factor(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
 {
     hipStream_t s = M->ctx->stream;
@@ -1579,6 +1744,7 @@ int prec_factor(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
         launch_ell<double>(s, M->U, M->lu64.p, M->ell64_U.p, true);
         if (!M->cell)
             launch_rowdiag<double>(s, M->U, M->diag.p, M->lu64.p, M->dgU64.p);
+        launch_band_gather<double>(M, M->lu64.p, M->band64.p);
         if (M->has_t) {
             launch_ell<double>(s, M->Ut, M->lu64.p, M->ell64_Ut.p, false);
             launch_ell<double>(s, M->Lt, M->lu64.p, M->ell64_Lt.p, false);
@@ -1590,6 +1756,7 @@ int prec_factor(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
         launch_ell<float>(s, M->U, M->lu32.p, M->ell32_U.p, true);
         if (!M->cell)
             launch_rowdiag<float>(s, M->U, M->diag.p, M->lu32.p, M->dgU32.p);
+        launch_band_gather<float>(M, M->lu32.p, M->band32.p);
     }
     RS_HIP(hipMemcpyAsync(st, M->status.p, sizeof(st), hipMemcpyDeviceToHost, s));
     RS_HIP(hipStreamSynchronize(s));
@@ -1675,6 +1842,221 @@ static int ilu_kernel_choice()
     return 0;
 }
 
+// ---------------------------------------------------------------- line wavefront
+// ILU.apply on blocks of line form (band_block): one wave per block, lane j
+// owns line j.  In the L sweep lane j treats row (i, j) at step s = i + 2j,
+// so its line j-1 couplings (i-1, j-1), (i, j-1), (i+1, j-1) were computed
+// by lane j-1 three, two and one steps earlier and (i-1, j) by itself one
+// step earlier: each step shifts lane j-1's latest result in with a DPP
+// wave shift and keeps the two before it in registers, and no level ever
+// waits for LDS.  The U sweep runs backwards (step s treats
+// i = Lx - 1 - s + 2 (Ly - 1 - j)) with the neighbours from lane j+1.  Every
+// row forms sum = 0, sum += v * x over its present slots in ascending column
+// order (absent slots selected away), then rhs - sum (and / diagonal): the
+// statement order of UnitLowerCompRowMatrix.solve and
+// UpperCompRowMatrix.solve, bit for bit.  The block's vector lives in LDS
+// (right-hand side, then y, then x, each written only by the lane that owns
+// the row); each lane's rows are consecutive, so the row records (8 values
+// plus diagonal and mask) are prefetched RS_BAND_D steps ahead in a register
+// ring.
+#define RS_BAND_D 8
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf,
+                                                      false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v)
+{
+    const long long q = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)q, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(q >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+#define RS_DPP_WAVE_SHR1 0x138   // lane k gets lane k-1 (lane 0: 0)
+#define RS_DPP_WAVE_SHL1 0x130   // lane k gets lane k+1 (lane 63: 0)
+
+template <typename T>
+struct alignas(16) BandRec {
+    T v[4];
+};
+
+
+// Global loads in flight: the ring depth D (steps ahead), within the 63
+// loads the wave's counter tracks and about 200 registers of ring.
+template <typename T, bool UPPER, bool FULL>
+constexpr int band_depth()
+{
+    constexpr int loads = (int)sizeof(T) / 4 + (UPPER ? 1 : 0) + (FULL ? 0 : 1);
+    constexpr int regs =
+        4 * (int)sizeof(T) / 4 + (UPPER ? (int)sizeof(T) / 4 : 0) + (FULL ? 0 : 1);
+    int d = 16;
+    while (d > 4 && (d * loads > 62 || d * regs > 200))
+        --d;
+    return d;
+}
+
+template <typename T, bool UPPER, bool FULL>
+struct BandStage {
+    BandRec<T> v;
+    T d;
+    unsigned m;
+};
+
+// record of row offset q (in rows) from base pointers at the lane's row
+template <typename T, bool UPPER, bool FULL>
+__device__ __forceinline__ void band_load(const T *rp, const T *dp, const unsigned char *mp,
+                                          int q, BandStage<T, UPPER, FULL> &st)
+{
+    st.v = *reinterpret_cast<const BandRec<T> *>(rp + 8 * q);
+    if (UPPER)
+        st.d = dp[q];
+    if (!FULL)
+        st.m = mp[q];
+}
+
+template <typename T, bool UPPER, bool FULL>
+__device__ __forceinline__ void band_sweep(int Lx, int Ly, int g0, const T *rec, const T *dgl,
+                                           const unsigned char *mask, T *xv, int junk)
+{
+    constexpr int D = band_depth<T, UPPER, FULL>();
+    constexpr int dir = UPPER ? -1 : 1;
+    const int j = threadIdx.x, jj = min(j, Ly - 1);
+    const bool live = j < Ly;
+    const int S = Lx + 2 * Ly - 2;
+    // row of the lane at step s: rb + dir * s; line position i0 + dir * s
+    const int i0 = UPPER ? Lx - 1 + 2 * (Ly - 1 - jj) : -2 * jj;
+    const int rb = jj * Lx + i0;
+    const bool js = jj > 0, jn = jj < Ly - 1;
+    const T *rp = rec + (size_t)8 * (g0 + rb) + (UPPER ? 4 : 0);
+    const T *dp = dgl + (g0 + rb);
+    const unsigned char *mp = mask + (g0 + rb);
+    const T *xr = xv + rb;
+    BandStage<T, UPPER, FULL> ring[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+        band_load<T, UPPER, FULL>(rp, dp, mp, dir * k, ring[k]);
+    T cur = T(0), h1 = T(0), h2 = T(0), h3 = T(0);
+    T rhs = xr[0];
+    for (int s0 = 0; s0 < S; s0 += D) {
+        // base pointers of this round (every access below has an immediate
+        // offset from them)
+        const T *rps = rp + (ptrdiff_t)8 * dir * s0;
+        const T *dps = dp + dir * s0;
+        const unsigned char *mps = mp + dir * s0;
+        const T *xrs = xr + dir * s0;
+        const int is0 = i0 + dir * s0;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            BandStage<T, UPPER, FULL> &st = ring[k];
+            const int i = is0 + dir * k;
+            const bool act = live && (unsigned)i < (unsigned)Lx;
+            const T rhs_s = rhs;
+            rhs = xrs[dir * (k + 1)];
+            // the neighbour line's latest result and the two before it
+            h3 = h2;
+            h2 = h1;
+            h1 = UPPER ? dpp_mov<RS_DPP_WAVE_SHL1>(cur) : dpp_mov<RS_DPP_WAVE_SHR1>(cur);
+            const bool w = i > 0, e = i < Lx - 1;
+            bool p0, p1, p2, p3;
+            if (FULL) {
+                // L: (i-1,j-1), (i,j-1), (i+1,j-1), (i-1,j); U: (i+1,j),
+                // (i-1,j+1), (i,j+1), (i+1,j+1)
+                p0 = UPPER ? e : (w && js);
+                p1 = UPPER ? (w && jn) : js;
+                p2 = UPPER ? jn : (e && js);
+                p3 = UPPER ? (e && jn) : w;
+            } else {
+                const unsigned m = UPPER ? st.m >> 4 : st.m;
+                p0 = m & 1u;
+                p1 = m & 2u;
+                p2 = m & 4u;
+                p3 = m & 8u;
+            }
+            const T o0 = UPPER ? cur : h3, o1 = UPPER ? h1 : h2;
+            const T o2 = UPPER ? h2 : h1, o3 = UPPER ? h3 : cur;
+            T sum = T(0);
+            T q = sum + st.v.v[0] * o0;
+            sum = p0 ? q : sum;
+            q = sum + st.v.v[1] * o1;
+            sum = p1 ? q : sum;
+            q = sum + st.v.v[2] * o2;
+            sum = p2 ? q : sum;
+            q = sum + st.v.v[3] * o3;
+            sum = p3 ? q : sum;
+            T val = rhs_s - sum;
+            if (UPPER)
+                val = val / st.d;
+            if (act)
+                xv[rb + dir * (s0 + k)] = val;
+            cur = act ? val : cur;
+            // refill the slot for step s + D
+            band_load<T, UPPER, FULL>(rps, dps, mps, dir * (k + D), st);
+        }
+    }
+    (void)junk;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64)
+    k_ilu_apply_band(const int *__restrict__ bstart, const int2 *__restrict__ dims,
+                     const unsigned char *__restrict__ mask, const T *__restrict__ rec,
+                     const T *__restrict__ dgl, const T *__restrict__ b, T *__restrict__ x,
+                     const int *skip1, const int *skip2, BlkSkip bskip)
+{
+    if ((skip1 && *skip1) || (skip2 && *skip2))
+        return;
+    if (blk_skipped(bskip, blockIdx.x))
+        return;
+    extern __shared__ unsigned char rs_smem[];
+    const int b0 = bstart[blockIdx.x], nb = bstart[blockIdx.x + 1] - b0;
+    T *xv = (T *)rs_smem + RS_BAND_PAD;
+    const int2 d = dims[blockIdx.x];   // Lx, Ly; Ly < 0: the block's pattern is not full
+    const int Lx = d.x, Ly = d.y < 0 ? -d.y : d.y;
+    for (int r = threadIdx.x; r < nb; r += 64)
+        xv[r] = b[b0 + r];
+    for (int r = threadIdx.x; r < RS_BAND_PAD; r += 64) {
+        xv[-1 - r] = T(0);
+        xv[nb + r] = T(0);
+    }
+    __syncthreads();
+    if (d.y > 0) {
+        band_sweep<T, false, true>(Lx, Ly, b0, rec, dgl, mask, xv, nb);
+        __syncthreads();
+        band_sweep<T, true, true>(Lx, Ly, b0, rec, dgl, mask, xv, nb);
+    } else {
+        band_sweep<T, false, false>(Lx, Ly, b0, rec, dgl, mask, xv, nb);
+        __syncthreads();
+        band_sweep<T, true, false>(Lx, Ly, b0, rec, dgl, mask, xv, nb);
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < nb; r += 64)
+        x[b0 + r] = xv[r];
+}
+
+template <typename T>
+static void launch_band(rsgpu_prec *M, const T *rec, const T *b, T *x, const int *sk1,
+                        const int *sk2)
+{
+    const size_t shm = (size_t)(M->max_block_rows + 2 * RS_BAND_PAD) * sizeof(T);
+    static bool attr = false;
+    if (!attr && shm > 65536) {
+        RS_HIP(hipFuncSetAttribute((const void *)k_ilu_apply_band<T>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   lds_limit(M->ctx->device)));
+        attr = true;
+    }
+    // padded layout: rows -PAD .. n+PAD-1 of 8 values, then the diagonals
+    const size_t np = (size_t)M->n + 2 * RS_BAND_PAD;
+    hipLaunchKernelGGL(k_ilu_apply_band<T>, dim3(M->nblocks), dim3(64), shm, M->ctx->stream,
+                       M->bstart.p, M->band_dims.p, M->band_mask.p + RS_BAND_PAD,
+                       rec + (size_t)8 * RS_BAND_PAD, rec + 8 * np + RS_BAND_PAD, b, x, sk1,
+                       sk2, M->skip);
+    RS_CHECK_LAUNCH();
+}
+
 template <typename T, int TRANS>
 static void launch_apply(rsgpu_prec *M, const SweepData &A_, const T *va, const T *da,
                          const SweepData &B_, const T *vb, const T *db, const T *b, T *x,
@@ -1683,6 +2065,13 @@ static void launch_apply(rsgpu_prec *M, const SweepData &A_, const T *va, const 
     hipStream_t s = M->ctx->stream;
     if (M->n == 0)
         return;
+    if (TRANS == RS_MODE_ILU && M->band && &A_ == &M->L &&
+        (size_t)(M->max_block_rows + 2 * RS_BAND_PAD) * sizeof(T) <=
+            (size_t)lds_limit(M->ctx->device)) {
+        launch_band<T>(M, sizeof(T) == 8 ? (const T *)M->band64.p : (const T *)M->band32.p, b,
+                       x, sk1, sk2);
+        return;
+    }
     const int maxc = std::max(A_.max_count, B_.max_count);
     const int choice = ilu_kernel_choice();
     const bool lds = M->max_block_rows <= (int)(RS_LDS_BYTES / sizeof(T)) - 2;

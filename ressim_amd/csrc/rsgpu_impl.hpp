@@ -269,6 +269,17 @@ struct rsgpu_prec {
     // batched solve enqueues its applies
     BlkSkip skip{};
     IccData *icc = nullptr;     // ICC: fill-in reach lists and factor levels
+    // line-wavefront sweeps (every block a grid of Ly <= 64 lines of Lx rows
+    // coupled only among 9-point neighbours): per block (Lx, Ly), per row the
+    // factor positions of the 4 L slots, the diagonal and the 4 U slots (-1
+    // absent), the slot masks, and the per-row records gathered after each
+    // factorisation
+    bool band = false;
+    DevBuf<int2> band_dims;
+    DevBuf<int> band_pos;
+    DevBuf<unsigned char> band_mask;
+    DevBuf<double> band64;       // per row: L[4], U[4] (8 values), diagonals after
+    DevBuf<float> band32;
     ~rsgpu_prec()
     {
         amg_destroy(amg);

@@ -430,7 +430,10 @@ class Preconditioner:
     def info(self):
         a, b, c = C.c_int(), C.c_int(), C.c_int()
         L.load().rsgpu_prec_info(self.h, C.byref(a), C.byref(b), C.byref(c))
-        return dict(nnz=a.value, levels_lower=b.value, levels_upper=c.value)
+        f = C.c_int(0)
+        L.load().rsgpu_prec_sweep_form(self.h, C.byref(f))
+        return dict(nnz=a.value, levels_lower=b.value, levels_upper=c.value,
+                    sweep_form=("levels", "lines", "cells")[f.value])
 
 
 class IdentityPreconditioner(Preconditioner):

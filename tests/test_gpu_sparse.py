@@ -126,6 +126,26 @@ def _banded(rng, n, half, empty=()):
     return rp, np.concatenate(rows).astype(np.int32), rng.standard_normal(int(rp[-1]))
 
 
+def five_point(nx, ny, rng):
+    """A diagonally dominant 5-point matrix on an nx x ny grid (natural order)."""
+    rows = []
+    for j in range(ny):
+        for i in range(nx):
+            r = i + nx * j
+            cols = [c for c, ok in ((r - nx, j > 0), (r - 1, i > 0), (r, True),
+                                    (r + 1, i < nx - 1), (r + nx, j < ny - 1)) if ok]
+            rows.append(cols)
+    rp = np.zeros(nx * ny + 1, np.int32)
+    rp[1:] = np.cumsum([len(c) for c in rows])
+    ci = np.concatenate(rows).astype(np.int32)
+    d = -rng.uniform(0.5, 1.5, len(ci))
+    for r in range(nx * ny):
+        for q in range(rp[r], rp[r + 1]):
+            if ci[q] == r:
+                d[q] = 6.0 + rng.uniform()
+    return rp, ci, d
+
+
 def _sliced_products(gpu, A, x):
     import torch
     xd = torch.tensor(x, device="cuda")
@@ -291,6 +311,59 @@ def test_block_ilu_bitexact(gpu, oracle, systems, nblocks):
     x = np.empty(n)
     M.apply(b, x)
     np.testing.assert_array_equal(x, oracle.ilu_apply(brp, bci, lu, dg, b))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("shape", [(64, 64, 1, "lines"), (64, 64, 16, "lines"),
+                                   (37, 50, 5, "lines"), (128, 64, 2, "lines"),
+                                   (20, 130, 2, "levels"), (9, 7, 1, "lines"),
+                                   (31, 33, 4, "levels"), (40, 48, -3, "lines"),
+                                   (64, 64, -16, "lines")])
+def test_block_ilu_line_wavefront(gpu, oracle, monkeypatch, precision, shape):
+    """Block ILU(0) apply on 2-D 9-point MPFA-O matrices: blocks of whole
+    lines (at most 64 of them) take the line-wavefront kernel, others the
+    level schedules; either way the result is the oracle's, bit for bit,
+    FP64 and FP32, and the same bits as with the wavefront switched off."""
+    import torch
+    from ressim_amd import _lib
+    nx, ny, nb, form = shape
+    if nb < 0:
+        # a 5-point pattern (no diagonal neighbours): masks read per row
+        nb = -nb
+        rp, ci, d = five_point(nx, ny, np.random.default_rng(nx))
+    else:
+        s = oracle_system(oracle, P.config1(nx=nx, ny=ny))
+        rp, ci, d = s["rowptr"], s["colind"], s["data"]
+    n = len(rp) - 1
+    brp, bci, bd, _ = oracle.block_filter(rp, ci, d, nb)
+    b = np.random.default_rng(nx * ny + nb).standard_normal(n)
+    b[n // 2] = np.inf          # propagates as in the sequential sweeps
+    out = []
+    for band in ("1", "0"):
+        monkeypatch.setenv("RSGPU_ILU_BAND", band)
+        A = _mat(gpu, rp, ci, d, n, n)
+        M = gpu.BlockILU(nb, precision=_lib.FP64 if precision == "fp64" else _lib.FP32)
+        M.setMatrix(A)
+        assert M.info()["sweep_form"] == (form if band == "1" else "levels")
+        if precision == "fp64":
+            x = np.empty(n)
+            M.apply(b, x)
+        else:
+            bd_ = torch.tensor(b.astype(np.float32), device="cuda")
+            xd = torch.empty(n, dtype=torch.float32, device="cuda")
+            torch.cuda.synchronize()
+            assert _lib.load().rsgpu_prec_apply_f32(M.h, bd_.data_ptr(), xd.data_ptr()) == 0
+            M.ctx.synchronize()
+            x = xd.cpu().numpy()
+        out.append(x)
+    if precision == "fp64":
+        _, lu, dg, _ = oracle.ilu_factor(brp, bci, bd)
+        want = oracle.ilu_apply(brp, bci, lu, dg, b)
+    else:
+        _, lu, dg, _ = oracle.ilu_factor_f32(brp, bci, bd.astype(np.float32))
+        want = oracle.ilu_apply_f32(brp, bci, lu, dg, b.astype(np.float32))
+    np.testing.assert_array_equal(out[0], want)
+    np.testing.assert_array_equal(out[1], want)
 
 
 @pytest.mark.parametrize("kernel", ["pf", "generic"])
