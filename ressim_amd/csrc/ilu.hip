@@ -1989,14 +1989,14 @@ __device__ __forceinline__ void band_sweep(int Lx, int Ly, int g0, const T *rec,
             T val = rhs_s - sum;
             if (UPPER)
                 val = val / st.d;
-            if (act)
-                xv[rb + dir * (s0 + k)] = val;
+            // inactive lanes store into a padding slot: a branch here would
+            // make the compiler's wait counting drain the prefetch ring
+            xv[act ? rb + dir * (s0 + k) : junk] = val;
             cur = act ? val : cur;
             // refill the slot for step s + D
             band_load<T, UPPER, FULL>(rps, dps, mps, dir * (k + D), st);
         }
     }
-    (void)junk;
 }
 
 template <typename T>
@@ -2022,14 +2022,17 @@ __global__ void __launch_bounds__(64)
         xv[nb + r] = T(0);
     }
     __syncthreads();
+    // junk slot: the last padding row (read only by prefetches of rows
+    // outside the block, whose values are never used)
+    const int junk = nb + RS_BAND_PAD - 1;
     if (d.y > 0) {
-        band_sweep<T, false, true>(Lx, Ly, b0, rec, dgl, mask, xv, nb);
+        band_sweep<T, false, true>(Lx, Ly, b0, rec, dgl, mask, xv, junk);
         __syncthreads();
-        band_sweep<T, true, true>(Lx, Ly, b0, rec, dgl, mask, xv, nb);
+        band_sweep<T, true, true>(Lx, Ly, b0, rec, dgl, mask, xv, junk);
     } else {
-        band_sweep<T, false, false>(Lx, Ly, b0, rec, dgl, mask, xv, nb);
+        band_sweep<T, false, false>(Lx, Ly, b0, rec, dgl, mask, xv, junk);
         __syncthreads();
-        band_sweep<T, true, false>(Lx, Ly, b0, rec, dgl, mask, xv, nb);
+        band_sweep<T, true, false>(Lx, Ly, b0, rec, dgl, mask, xv, junk);
     }
     __syncthreads();
     for (int r = threadIdx.x; r < nb; r += 64)
