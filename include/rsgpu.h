@@ -107,7 +107,7 @@ int rsgpu_csr_sync_f32(rsgpu_csr *A);
 int rsgpu_csr_transpose(const rsgpu_csr *A, rsgpu_csr **At);
 int rsgpu_csr_mult_f32(rsgpu_csr *A, const float *x_dev, float *y_dev);
 /* Sliced copy (no MTJ counterpart; the storage the solvers' SpMV streams
- * for stencil matrices).  Rows in chunks of 64, each chunk with the sorted
+ * for stencil matrices).  Rows in chunks of 256, each chunk with the sorted
  * union of its rows' column offsets (column - row) as a table and per row a
  * mask of the slots it holds; values slot-major per chunk.  Built with the
  * pattern when every chunk's union has at most 32 offsets (nchunks = 0
@@ -115,7 +115,12 @@ int rsgpu_csr_mult_f32(rsgpu_csr *A, const float *x_dev, float *y_dev);
  * values per copy.  y = A x over it is bit-identical to rsgpu_csr_mult
  * (mult_sliced_f32: to rsgpu_csr_mult_f32).  The FP64 copy is refreshed by
  * rsgpu_csr_sync_sliced (and by every FP64 solve at its start), the FP32
- * copy by rsgpu_csr_sync_f32. */
+ * copy by rsgpu_csr_sync_f32.  Every entry point that changes A's values
+ * (set_values*, zero, add_batch, zero_rows, the assembly calls) makes both
+ * copies stale: mult_sliced, mult_sliced_f32, mult_f32 and the mixed
+ * solves then return RSGPU_BAD_ARGUMENT until the copy is synced again.
+ * Writes through the rsgpu_csr_values_dev pointer are not tracked; sync
+ * after them. */
 int rsgpu_csr_sliced_info(const rsgpu_csr *A, int *nchunks, int64_t *nslots,
                           int *ntables);
 int rsgpu_csr_sync_sliced(rsgpu_csr *A);

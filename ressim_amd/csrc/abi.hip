@@ -85,20 +85,28 @@ extern "C" int rsgpu_csr_create(rsgpu_ctx *ctx, int n, int ncols, int nnz,
     if (!ctx || !out || n < 0 || ncols < 0 || nnz < 0 || !rowptr ||
         (nnz > 0 && !colind))
         return RSGPU_BAD_ARGUMENT;
+    // The whole rowptr is checked before any colind entry is read: a row
+    // range past nnz would otherwise read past the caller's colind array.
+    if (rowptr[0] != 0 || rowptr[n] != nnz) {
+        ctx->err = "rowptr must start at 0 and end at nnz";
+        return RSGPU_BAD_ARGUMENT;
+    }
+    for (int i = 0; i < n; ++i)
+        if (rowptr[i + 1] < rowptr[i] || rowptr[i + 1] > nnz) {
+            ctx->err = "rowptr must be non-decreasing and at most nnz (row " +
+                       std::to_string(i) + ")";
+            return RSGPU_BAD_ARGUMENT;
+        }
     rsgpu_csr *A = new rsgpu_csr();
-    RS_TRY(ctx)
+    try {
     RS_HIP(hipSetDevice(ctx->device));
     A->ctx = ctx;
     A->n = n;
     A->ncols = ncols;
     A->nnz = nnz;
-    if (rowptr[0] != 0 || rowptr[n] != nnz)
-        throw RsError{RSGPU_BAD_ARGUMENT, "rowptr must start at 0 and end at nnz"};
     A->h_rowptr.assign(rowptr, rowptr + n + 1);
     A->h_colind.assign(colind, colind + nnz);
     for (int i = 0; i < n; ++i) {
-        if (rowptr[i + 1] < rowptr[i])
-            throw RsError{RSGPU_BAD_ARGUMENT, "rowptr must be non-decreasing"};
         for (int j = rowptr[i]; j < rowptr[i + 1]; ++j)
             if (colind[j] < 0 || colind[j] >= ncols)
                 throw RsError{RSGPU_BAD_ARGUMENT,
@@ -120,7 +128,15 @@ extern "C" int rsgpu_csr_create(rsgpu_ctx *ctx, int n, int ncols, int nnz,
     RS_HIP(hipStreamSynchronize(s));
     *out = A;
     return RSGPU_OK;
-    RS_CATCH(ctx)
+    }
+    catch (const RsError &e) {
+        delete A;
+        return fail(ctx, e);
+    }
+    catch (const std::bad_alloc &) {
+        delete A;
+        return fail(ctx, RsError{RSGPU_OUT_OF_MEMORY, "host out of memory"});
+    }
 }
 
 // 3x3 block matrix: the expanded CSR with each block row's three scalar rows
@@ -135,15 +151,22 @@ extern "C" int rsgpu_csr_create_bsr3(rsgpu_ctx *ctx, int nb, int nnzb, const int
         ctx->err = "block matrix exceeds 32-bit scalar indices";
         return RSGPU_BAD_ARGUMENT;
     }
+    // The block rowptr is checked whole before any bcol entry is read.
+    if (brp[0] != 0 || brp[nb] != nnzb) {
+        ctx->err = "block rowptr must start at 0 and end at nnzb";
+        return RSGPU_BAD_ARGUMENT;
+    }
+    for (int I = 0; I < nb; ++I)
+        if (brp[I + 1] < brp[I] || brp[I + 1] > nnzb) {
+            ctx->err = "block rowptr must be non-decreasing and at most nnzb (block row " +
+                       std::to_string(I) + ")";
+            return RSGPU_BAD_ARGUMENT;
+        }
     rsgpu_csr *A = new rsgpu_csr();
-    RS_TRY(ctx)
+    try {
     RS_HIP(hipSetDevice(ctx->device));
-    if (brp[0] != 0 || brp[nb] != nnzb)
-        throw RsError{RSGPU_BAD_ARGUMENT, "block rowptr must start at 0 and end at nnzb"};
     std::vector<int> hb(brp, brp + nb + 1), hc(bcol, bcol + nnzb);
     for (int I = 0; I < nb; ++I) {
-        if (hb[I + 1] < hb[I])
-            throw RsError{RSGPU_BAD_ARGUMENT, "block rowptr must be non-decreasing"};
         for (int b = hb[I]; b < hb[I + 1]; ++b)
             if (hc[b] < 0 || hc[b] >= nb)
                 throw RsError{RSGPU_BAD_ARGUMENT, "block column out of range"};
@@ -185,7 +208,15 @@ extern "C" int rsgpu_csr_create_bsr3(rsgpu_ctx *ctx, int nb, int nnzb, const int
     RS_HIP(hipStreamSynchronize(s));
     *out = A;
     return RSGPU_OK;
-    RS_CATCH(ctx)
+    }
+    catch (const RsError &e) {
+        delete A;
+        return fail(ctx, e);
+    }
+    catch (const std::bad_alloc &) {
+        delete A;
+        return fail(ctx, RsError{RSGPU_OUT_OF_MEMORY, "host out of memory"});
+    }
 }
 
 extern "C" int rsgpu_csr_block_info(const rsgpu_csr *A, int *bs, int *nb, int *nnzb)
@@ -283,6 +314,7 @@ extern "C" int rsgpu_csr_set_values(rsgpu_csr *A, const double *data)
                               hipMemcpyHostToDevice, A->ctx->stream));
         RS_HIP(hipStreamSynchronize(A->ctx->stream));
     }
+    A->values_changed();
     return RSGPU_OK;
     RS_CATCH(A->ctx)
 }
@@ -295,6 +327,7 @@ extern "C" int rsgpu_csr_set_values_dev(rsgpu_csr *A, const double *data)
     if (A->nnz)
         RS_HIP(hipMemcpyAsync(A->val.p, data, sizeof(double) * A->nnz,
                               hipMemcpyDeviceToDevice, A->ctx->stream));
+    A->values_changed();
     return RSGPU_OK;
     RS_CATCH(A->ctx)
 }
@@ -321,6 +354,7 @@ extern "C" int rsgpu_csr_zero(rsgpu_csr *A)
         return RSGPU_BAD_ARGUMENT;
     RS_TRY(A->ctx)
     A->val.zero(A->ctx->stream);
+    A->values_changed();
     return RSGPU_OK;
     RS_CATCH(A->ctx)
 }
@@ -387,6 +421,7 @@ extern "C" int rsgpu_csr_add_batch(rsgpu_csr *A, int count, const int32_t *rows,
     hipLaunchKernelGGL(k_add_ordered, dim3((ng + 255) / 256), dim3(256), 0, s,
                        dgp.p, dpos.p, dv.p, A->val.p, ng);
     RS_CHECK_LAUNCH();
+    A->values_changed();
     RS_HIP(hipStreamSynchronize(s));
     return RSGPU_OK;
     RS_CATCH(A->ctx)
@@ -471,6 +506,7 @@ extern "C" int rsgpu_csr_sync_f32(rsgpu_csr *A)
     A->val32.alloc(std::max(A->nnz, 1));
     convert_f64_to_f32(A->ctx->stream, A->val.p, A->val32.p, A->nnz);
     sliced_fill<float>(A, A->ctx->stream);
+    A->gen32 = A->val_gen;
     return RSGPU_OK;
     RS_CATCH(A->ctx)
 }
@@ -505,7 +541,7 @@ extern "C" int rsgpu_csr_sync_sliced(rsgpu_csr *A)
 
 extern "C" int rsgpu_csr_mult_sliced(rsgpu_csr *A, const double *x, double *y)
 {
-    if (!A || !A->sl.ok() || !A->sl.v64_ready) {
+    if (!A || !A->sl.ok() || A->sl.gen64 != A->val_gen) {
         if (A)
             A->ctx->err = "mult_sliced needs rsgpu_csr_sync_sliced after the last value change";
         return RSGPU_BAD_ARGUMENT;
@@ -518,7 +554,7 @@ extern "C" int rsgpu_csr_mult_sliced(rsgpu_csr *A, const double *x, double *y)
 
 extern "C" int rsgpu_csr_mult_sliced_f32(rsgpu_csr *A, const float *x, float *y)
 {
-    if (!A || !A->sl.ok() || !A->sl.v32.p) {
+    if (!A || !A->sl.ok() || !A->sl.v32.p || !A->f32_current()) {
         if (A)
             A->ctx->err = "mult_sliced_f32 needs rsgpu_csr_sync_f32 after the last value change";
         return RSGPU_BAD_ARGUMENT;
@@ -531,8 +567,11 @@ extern "C" int rsgpu_csr_mult_sliced_f32(rsgpu_csr *A, const float *x, float *y)
 
 extern "C" int rsgpu_csr_mult_f32(rsgpu_csr *A, const float *x, float *y)
 {
-    if (!A || !A->val32.p)
+    if (!A || !A->f32_current()) {
+        if (A)
+            A->ctx->err = "mult_f32 needs rsgpu_csr_sync_f32 after the last value change";
         return RSGPU_BAD_ARGUMENT;
+    }
     RS_TRY(A->ctx)
     spmv_launch<float>(A->ctx->stream, A, A->val32.p, x, y, 0, 0.0, nullptr);
     return RSGPU_OK;

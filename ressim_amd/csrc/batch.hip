@@ -1007,7 +1007,7 @@ extern "C" int rsgpu_batch_solve(rsgpu_batch *B, rsgpu_csr *A, rsgpu_prec *M,
                               : "FP64 batched solve needs an FP64 preconditioner";
         return RSGPU_BAD_ARGUMENT;
     }
-    if (B->o.mixed && !A->val32.p) {
+    if (B->o.mixed && !A->f32_current()) {
         ctx->err = "mixed solve needs rsgpu_csr_sync_f32 after the last value change";
         return RSGPU_BAD_ARGUMENT;
     }

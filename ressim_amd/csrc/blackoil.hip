@@ -393,6 +393,8 @@ extern "C" int rsgpu_blackoil_assemble(rsgpu_blackoil *B, const double *u_dev,
                            B->P, G, B->kh.p, u_dev, u_old_dev, J ? J->brp.p : nullptr,
                            J ? J->val.p : nullptr, residual_dev);
         RS_CHECK_LAUNCH();
+        if (J)
+            J->values_changed();
         return RSGPU_OK;
     } catch (const RsError &e) {
         return bo_fail(ctx, e);

@@ -983,6 +983,7 @@ extern "C" int rsgpu_impes_assemble(rsgpu_impes *E, rsgpu_csr *A, const rsgpu_im
                                E->src_kind.p, E->src_q.p, E->src_qe.p, b);
             RS_CHECK_LAUNCH();
         }
+        A->values_changed();
         return RSGPU_OK;
     } catch (const RsError &e) {
         return impes_fail(E->ctx, e);
@@ -1095,6 +1096,7 @@ extern "C" int rsgpu_impes_assemble_temperature(rsgpu_impes *E, rsgpu_csr *A,
                                Mf->md_ptr.p, Mf->md_elem.p, Mf->md_t.p, E->diag_pos.p, A->val.p,
                                E->src_ptr.p, E->src_term.p, E->src_qe.p, b);
         RS_CHECK_LAUNCH();
+        A->values_changed();
         return RSGPU_OK;
     } catch (const RsError &e) {
         return impes_fail(E->ctx, e);

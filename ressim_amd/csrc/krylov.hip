@@ -2023,7 +2023,7 @@ void run_mixed(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const double *b,
     double *r = S->w64.p;
     float *r32 = S->w32.p, *z32 = S->w32.p + S->ldv, *work32 = S->w32.p + 2 * S->ldv;
     const int E = (n + RS_VBS - 1) / RS_VBS;
-    if (!A->val32.p)
+    if (!A->f32_current())
         throw RsError{RSGPU_BAD_ARGUMENT,
                       "mixed solve needs rsgpu_csr_sync_f32 after the last value change"};
     if (M && M->kind != RSGPU_PREC_IDENTITY && M->precision != RSGPU_FP32)

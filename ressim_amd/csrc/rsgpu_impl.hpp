@@ -115,7 +115,7 @@ struct SlicedFmt {
     DevBuf<unsigned> mask;
     DevBuf<double> v64;                // filled by sliced_fill (per solve)
     DevBuf<float> v32;                 // filled with the FP32 CSR copy
-    bool v64_ready = false;
+    uint64_t gen64 = 0;                // value generation v64 was filled from
     bool ok() const { return nchunks > 0; }
 };
 
@@ -137,6 +137,14 @@ struct rsgpu_csr {
     DevBuf<int> rowptr, colind;
     DevBuf<double> val;
     DevBuf<float> val32;
+    // Value generation: every entry point that changes val counts it up;
+    // the FP32 copy and the sliced FP64 copy record the generation they
+    // were filled from, and their users refuse a stale copy.  Writes
+    // through the rsgpu_csr_values_dev pointer are not seen.
+    uint64_t val_gen = 1;
+    uint64_t gen32 = 0;
+    void values_changed() { ++val_gen; }
+    bool f32_current() const { return val32.p && gen32 == val_gen; }
     SpmvPlan plan;     // FP64 kernels: 2048-nonzero stages
     SpmvPlan plan32;   // FP32 kernels: 4096-nonzero stages (same LDS bytes)
     // transposed copy (lazy): rows = columns of A, values gathered via tperm
@@ -406,7 +414,16 @@ __device__ __forceinline__ double gather_sum(const double *p, int i0, int i1)
 #define RS_TPAD 32
 #define RS_TICKET_WORDS ((RS_NGROUP + 1) * RS_TPAD)
 
-__device__ __forceinline__ void wait_vm_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+// Waits until this wave's vector-memory stores have completed.  The compiler
+// signal fences on both sides keep the compiler from moving memory accesses
+// (the relaxed partial stores before, the ticket atomics after) across the
+// wait, which is not a memory operation in its model; they emit nothing.
+__device__ __forceinline__ void wait_vm_stores()
+{
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 
 template <int BS, int NR>
 __device__ bool block_reduce_finish(RedSlot slot, const double (&v)[NR],

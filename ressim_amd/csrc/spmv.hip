@@ -75,7 +75,7 @@ static void sliced_build(rsgpu_csr *A, hipStream_t s)
     SlicedFmt &F = A->sl;
     F.nchunks = F.ntables = 0;
     F.nslots = 0;
-    F.v64_ready = false;
+    F.gen64 = 0;
     F.v64.release();
     F.v32.release();
     const int n = A->n;
@@ -207,7 +207,7 @@ void sliced_fill(rsgpu_csr *A, hipStream_t s)
                        A->slview(), A->rowptr.p, A->val.p, b.p);
     RS_CHECK_LAUNCH();
     if (sizeof(T) == 8)
-        F.v64_ready = true;
+        F.gen64 = A->val_gen;
 }
 template void sliced_fill<double>(rsgpu_csr *, hipStream_t);
 template void sliced_fill<float>(rsgpu_csr *, hipStream_t);
@@ -431,6 +431,7 @@ void csr_zero_rows_dev(rsgpu_csr *A, double diagonal, const int *rows_dev,
                        A->ctx->stream, A->rowptr.p, A->colind.p, A->val.p,
                        rows_dev, nrows, diagonal);
     RS_CHECK_LAUNCH();
+    A->values_changed();
 }
 
 // ---------------------------------------------------------------- conversions

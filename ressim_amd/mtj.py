@@ -176,6 +176,10 @@ class CompRowMatrix:
         self.ctx = ctx or Context.default()
         brp = np.ascontiguousarray(browptr, np.int32)
         bci = np.ascontiguousarray(bcolind, np.int32)
+        if brp.ndim != 1 or brp.size != nb + 1:
+            raise ValueError(f"browptr must hold nb + 1 = {nb + 1} entries, got {brp.size}")
+        if bci.size < int(brp[-1]):
+            raise ValueError(f"bcolind holds {bci.size} entries, browptr[-1] = {int(brp[-1])}")
         h = C.c_void_p()
         _raise(L.load().rsgpu_csr_create_bsr3(self.ctx.h, nb, int(brp[-1]), _hptr(brp),
                                               _hptr(bci) if bci.size else None, C.byref(h)),
@@ -222,7 +226,11 @@ class CompRowMatrix:
         h = C.c_void_p()
         rowptr = np.ascontiguousarray(rowptr, np.int32)
         colind = np.ascontiguousarray(colind, np.int32)
-        nnz = int(rowptr[-1]) if len(rowptr) else 0
+        if rowptr.ndim != 1 or rowptr.size != n + 1:
+            raise ValueError(f"rowptr must hold n + 1 = {n + 1} entries, got {rowptr.size}")
+        nnz = int(rowptr[-1])
+        if colind.size < nnz:
+            raise ValueError(f"colind holds {colind.size} entries, rowptr[-1] = {nnz}")
         _raise(lib.rsgpu_csr_create(self.ctx.h, n, m, nnz, _hptr(rowptr),
                                     _hptr(colind) if nnz else None, C.byref(h)),
                self.ctx.h)

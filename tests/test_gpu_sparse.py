@@ -475,3 +475,64 @@ def test_matrix_market_round_trip_through_device(gpu, oracle, systems):
     y = np.empty(n)
     B.mult(x, y)
     np.testing.assert_array_equal(y, oracle.mult(rp, ci, d, x))
+
+
+def test_create_rejects_rowptr_past_nnz(gpu):
+    """rowptr = [0, 3, 1] with nnz = 1: row 0 claims entries past the colind
+    array; rejected before any column index is read (scalar and 3x3 block
+    patterns)."""
+    import ctypes as C
+    from ressim_amd import _lib as L
+    lib = L.load()
+    ctx = gpu.Context.default()
+    rp = np.array([0, 3, 1], np.int32)
+    ci = np.array([0], np.int32)
+    h = C.c_void_p()
+    rc = lib.rsgpu_csr_create(ctx.h, 2, 2, 1, rp.ctypes.data_as(C.c_void_p),
+                              ci.ctypes.data_as(C.c_void_p), C.byref(h))
+    assert rc == L.BAD_ARGUMENT and not h.value
+    assert b"rowptr" in lib.rsgpu_last_error(ctx.h)
+    rc = lib.rsgpu_csr_create_bsr3(ctx.h, 2, 1, rp.ctypes.data_as(C.c_void_p),
+                                   ci.ctypes.data_as(C.c_void_p), C.byref(h))
+    assert rc == L.BAD_ARGUMENT and not h.value
+    with pytest.raises(ValueError):
+        gpu.CompRowMatrix.from_csr(3, 3, np.array([0, 1, 2], np.int32), np.zeros(2, np.int32),
+                                   np.zeros(2))
+
+
+def test_stale_copies_are_refused(gpu, oracle, systems):
+    """A value change after sync_f32 / sync_sliced makes the FP32 and sliced
+    copies stale: the calls that read them refuse until the next sync."""
+    import torch
+    from ressim_amd import _lib as L
+    lib = L.load()
+    s = systems["s3"]
+    rp, ci, d = s["rowptr"], s["colind"], s["data"]
+    n = len(rp) - 1
+    A = _mat(gpu, rp, ci, d, n, n)
+    x = torch.tensor(np.random.default_rng(5).standard_normal(n), device="cuda")
+    x32, y32 = x.float(), torch.empty(n, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    A.sync_f32()
+    assert lib.rsgpu_csr_mult_f32(A.h, x32.data_ptr(), y32.data_ptr()) == L.OK
+    A.add(np.array([0], np.int32), np.array([0], np.int32), np.array([1.0]))
+    assert lib.rsgpu_csr_mult_f32(A.h, x32.data_ptr(), y32.data_ptr()) == L.BAD_ARGUMENT
+    assert b"sync_f32" in lib.rsgpu_last_error(A.ctx.h)
+    A.sync_f32()
+    assert lib.rsgpu_csr_mult_f32(A.h, x32.data_ptr(), y32.data_ptr()) == L.OK
+    nch, _, _ = A.sliced_info()
+    assert nch > 0
+    y = torch.empty(n, dtype=torch.float64, device="cuda")
+    A.sync_sliced()
+    A.mult_sliced(x, y)
+    A.mult_sliced(x32, y32)
+    gpu.Matrices.zeroRows(A, 1.0, [1])
+    with pytest.raises(ValueError, match="sync_sliced"):
+        A.mult_sliced(x, y)
+    with pytest.raises(ValueError, match="sync_f32"):
+        A.mult_sliced(x32, y32)
+    A.sync_sliced()
+    A.mult_sliced(x, y)
+    ref = torch.empty_like(y)
+    A.mult(x, ref)
+    assert torch.equal(y, ref)
