@@ -436,6 +436,18 @@ int rsgpu_flux_apply(rsgpu_flux *F, const double *p_dev, double *flux_dev);
  * +flux (cell is "here") or -flux ("there"). */
 int rsgpu_flux_rates(rsgpu_flux *F, const double *flux_dev, int ncells_sel,
                      const int32_t *cells_host, double *rates_host);
+/* No reference counterpart.  rsgpu_flux_rates without any device
+ * allocation, for callers that cannot allocate (code under HIP graph
+ * capture, tight serving or optimisation loops that evaluate well rates
+ * every step, harnesses that forbid hipMalloc): the selected cells and their
+ * rates are staged in a static device array of the library (768 KiB per
+ * GPU, held until the process ends).  At most 65536 cells per call; more
+ * gives RSGPU_BAD_ARGUMENT.  Results are bit-identical to
+ * rsgpu_flux_rates.  Every call on one GPU shares that array: calls that
+ * overlap on one GPU (from different threads or contexts) race. */
+int rsgpu_flux_rates_noalloc(rsgpu_flux *F, const double *flux_dev,
+                             int ncells_sel, const int32_t *cells_host,
+                             double *rates_host);
 
 /* ---- IMPES pressure Jacobian (rs numerics/Discretisation.java, field/CS.java)
  * The step before the solve in rs's pressure Newton loop

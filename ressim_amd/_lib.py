@@ -165,6 +165,7 @@ SIGNATURES = {
                                   C.POINTER(C.c_int)]),
     "rsgpu_flux_apply": (C.c_int, [_h, _dp, _dp]),
     "rsgpu_flux_rates": (C.c_int, [_h, _dp, C.c_int, _ip, _dp]),
+    "rsgpu_flux_rates_noalloc": (C.c_int, [_h, _dp, C.c_int, _ip, _dp]),
     "rsgpu_flux_vjp": (C.c_int, [_h, _dp, _dp, _dp]),
     "rsgpu_mpfa_o_vjp": (C.c_int, [_h, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                    C.c_double, C.c_double, _dp, C.c_double, _h, _dp, _dp]),

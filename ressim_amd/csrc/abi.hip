@@ -35,6 +35,11 @@ extern "C" int rsgpu_ctx_create(int device, rsgpu_ctx **out)
     c->device = device;
     RS_TRY(c)
     RS_HIP(hipSetDevice(device));
+    {
+        const int rc = rs_rates_scratch_init(device, c->err);
+        if (rc != RSGPU_OK)
+            throw RsError{rc, c->err};
+    }
     RS_HIP(hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking));
     c->stream = c->own;
     *out = c;

@@ -10,6 +10,8 @@
 // equals the cell's row of the flux-assembled matrix times the potential
 // (PotentialCalculator.discretizeFlux, :319-333) up to summation order.
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 #include "spmv_core.hpp"
 
@@ -236,6 +238,87 @@ extern "C" int rsgpu_flux_rates(rsgpu_flux *F, const double *flux_dev, int ncell
                            rates.p);
         RS_CHECK_LAUNCH();
         RS_HIP(hipMemcpyAsync(rates_host, rates.p, sizeof(double) * ncells_sel,
+                              hipMemcpyDeviceToHost, s));
+        RS_HIP(hipStreamSynchronize(s));
+        return RSGPU_OK;
+    } catch (const RsError &e) {
+        return flux_fail(F->ctx, e);
+    }
+}
+
+// ---------------------------------------------------------------- no-allocation rates
+// rsgpu_flux_rates_noalloc stages the selected cells and their rates in a
+// file-scope device array instead of allocating them per call: rates at
+// byte 0 (RS_RATES_NOALLOC_MAX doubles), cells after them.  Each GPU the
+// library runs on holds its own copy (768 KiB) until the process ends.  In
+// host code the array's name is not a device address: host code uses only
+// the per-device address hipGetSymbolAddress returned (g_scratch_ptr), taken
+// once per device when the first context on it is created.
+#define RS_RATES_NOALLOC_MAX 65536
+#define RS_SCRATCH_DEVICES 64
+static __device__ __align__(256) unsigned char
+    scratch[(size_t)RS_RATES_NOALLOC_MAX * (sizeof(double) + sizeof(int))];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[RS_SCRATCH_DEVICES];
+
+// Called by rsgpu_ctx_create with `device` current.
+int rs_rates_scratch_init(int device, std::string &err)
+{
+    if (device < 0 || device >= RS_SCRATCH_DEVICES) {
+        err = "device ordinal beyond the library's per-device table";
+        return RSGPU_BAD_ARGUMENT;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device])
+        return RSGPU_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return RSGPU_HIP_ERROR;
+    }
+    g_scratch_ptr[device] = p;
+    return RSGPU_OK;
+}
+
+extern "C" int rsgpu_flux_rates_noalloc(rsgpu_flux *F, const double *flux_dev, int ncells_sel,
+                                        const int32_t *cells_host, double *rates_host)
+{
+    if (!F || ncells_sel < 0 || (ncells_sel && (!cells_host || !rates_host || !flux_dev)))
+        return RSGPU_BAD_ARGUMENT;
+    try {
+        if (ncells_sel > RS_RATES_NOALLOC_MAX)
+            throw RsError{RSGPU_BAD_ARGUMENT,
+                          "rsgpu_flux_rates_noalloc: " + std::to_string(ncells_sel) +
+                              " cells exceed its limit of " +
+                              std::to_string(RS_RATES_NOALLOC_MAX) +
+                              " (use rsgpu_flux_rates)"};
+        for (int t = 0; t < ncells_sel; ++t)
+            if (cells_host[t] < 0 || cells_host[t] >= F->ncells)
+                throw RsError{RSGPU_BAD_ARGUMENT, "cell index out of range"};
+        if (!ncells_sel)
+            return RSGPU_OK;
+        const int dev = F->ctx->device;
+        void *base = nullptr;
+        if (dev >= 0 && dev < RS_SCRATCH_DEVICES) {
+            std::lock_guard<std::mutex> lock(g_scratch_mu);
+            base = g_scratch_ptr[dev];
+        }
+        if (!base)
+            throw RsError{RSGPU_HIP_ERROR, "no-allocation scratch not set up on this device"};
+        double *rates = static_cast<double *>(base);
+        int *cells = reinterpret_cast<int *>(static_cast<char *>(base) +
+                                             sizeof(double) * RS_RATES_NOALLOC_MAX);
+        hipStream_t s = F->ctx->stream;
+        RS_HIP(hipMemcpyAsync(cells, cells_host, sizeof(int) * ncells_sel,
+                              hipMemcpyHostToDevice, s));
+        // every thread t < ncells_sel writes rates[t]: nothing to clear
+        hipLaunchKernelGGL(k_rates, dim3((ncells_sel + 255) / 256), dim3(256), 0, s, ncells_sel,
+                           cells, F->cell_ptr.p, F->cell_conn.p, F->cell_sign.p, flux_dev,
+                           rates);
+        RS_CHECK_LAUNCH();
+        RS_HIP(hipMemcpyAsync(rates_host, rates, sizeof(double) * ncells_sel,
                               hipMemcpyDeviceToHost, s));
         RS_HIP(hipStreamSynchronize(s));
         return RSGPU_OK;

@@ -577,3 +577,6 @@ int amg_get_level(const rsgpu_prec *M, int k, int32_t *rowptr, int32_t *colind,
                   int *pnnz);
 
 uint64_t next_pattern_id();
+// flux.hip: per-device address of rsgpu_flux_rates_noalloc's static scratch
+// array, taken with `device` current (rsgpu_ctx_create)
+int rs_rates_scratch_init(int device, std::string &err);
