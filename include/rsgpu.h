@@ -99,6 +99,14 @@ int rsgpu_csr_multadd(rsgpu_csr *A, double alpha, const double *x_dev,
 int rsgpu_csr_transmult(rsgpu_csr *A, const double *x_dev, double *y_dev);
 int rsgpu_csr_transmultadd(rsgpu_csr *A, double alpha, const double *x_dev,
                            double *y_dev);
+/* r = b - A x, the residual multAdd(-1, x, b) forms (:336-353), but with
+ * the products and row sums carried in twice the working precision
+ * (Ogita-Rump-Oishi Dot2) and rounded once, so r is exact to about one
+ * rounding of its own size however much the row cancels.  No reference
+ * counterpart: iterative refinement on it converges to the FP64-attainable
+ * solution (the adjoint's forward and adjoint solves use it). */
+int rsgpu_csr_residual_dd(rsgpu_csr *A, const double *b_dev,
+                          const double *x_dev, double *r_dev);
 /* Single-precision copy of the values (no reference counterpart; used by
  * the mixed-precision form) and y = A x in FP32. */
 int rsgpu_csr_sync_f32(rsgpu_csr *A);

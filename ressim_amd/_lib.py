@@ -101,6 +101,7 @@ SIGNATURES = {
     "rsgpu_csr_transmult": (C.c_int, [_h, _dp, _dp]),
     "rsgpu_csr_transmultadd": (C.c_int, [_h, C.c_double, _dp, _dp]),
     "rsgpu_csr_sync_f32": (C.c_int, [_h]),
+    "rsgpu_csr_residual_dd": (C.c_int, [_h, _dp, _dp, _dp]),
     "rsgpu_csr_mult_f32": (C.c_int, [_h, _vp, _vp]),
     "rsgpu_csr_sliced_info": (C.c_int, [_h, C.POINTER(C.c_int), C.POINTER(C.c_int64),
                                         C.POINTER(C.c_int)]),

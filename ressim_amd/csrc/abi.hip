@@ -503,6 +503,17 @@ extern "C" int rsgpu_csr_transmultadd(rsgpu_csr *A, double alpha, const double *
     RS_CATCH(A->ctx)
 }
 
+extern "C" int rsgpu_csr_residual_dd(rsgpu_csr *A, const double *b, const double *x,
+                                     double *r)
+{
+    if (!A || (A->n && (!b || !x || !r)))
+        return RSGPU_BAD_ARGUMENT;
+    RS_TRY(A->ctx)
+    csr_residual_dd(A, b, x, r);
+    return RSGPU_OK;
+    RS_CATCH(A->ctx)
+}
+
 extern "C" int rsgpu_csr_sync_f32(rsgpu_csr *A)
 {
     if (!A)

@@ -551,6 +551,7 @@ void csr_transmult_launch(rsgpu_csr *A, double alpha, const double *x,
                           double *y, bool add);
 void csr_launch_gather(hipStream_t s, const int *perm, const double *src,
                        double *dst, int n);
+void csr_residual_dd(rsgpu_csr *A, const double *b, const double *x, double *r);
 void csr_zero_rows_dev(rsgpu_csr *A, double diagonal, const int *rows_dev,
                        int nrows);
 void convert_f64_to_f32(hipStream_t s, const double *a, float *b, size_t n);

@@ -405,6 +405,66 @@ void csr_transmult_launch(rsgpu_csr *A, double alpha, const double *x,
     RS_CHECK_LAUNCH();
 }
 
+// ---------------------------------------------------------------- compensated residual
+// r = b - A x by Ogita, Rump and Oishi's Dot2 ("Accurate sum and dot
+// product", SIAM J. Sci. Comput. 26, 2005, Algorithm 5.3): every product is
+// split exactly (TwoProd through an FMA), the running sum p is carried by
+// Knuth's TwoSum, all rounding errors are summed plainly in s, and the
+// result is p + s.  Its error is at most eps |r| + gamma_n^2 sum |a_ij x_j|
+// (their Proposition 5.5): as if computed in twice the working precision
+// and rounded once.  Eight lanes per row, each over a strided share of the
+// row, then the lanes' (p, s) pairs are combined by the same TwoSum step
+// through lane shuffles.
+struct DD {
+    double hi, lo;   // running sum p, sum of rounding errors s
+};
+
+__device__ __forceinline__ DD dd_add(DD a, double b, double be)
+{
+    const double s = a.hi + b;
+    const double bb = s - a.hi;
+    const double e = (a.hi - (s - bb)) + (b - bb);
+    return DD{s, a.lo + (e + be)};
+}
+
+__global__ void __launch_bounds__(256)
+    k_residual_dd(int n, const int *__restrict__ rowptr, const int *__restrict__ colind,
+                  const double *__restrict__ val, const double *__restrict__ b,
+                  const double *__restrict__ x, double *__restrict__ r)
+{
+    const int row = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 3);
+    const int sub = threadIdx.x & 7;
+    DD acc{0.0, 0.0};
+    if (row < n) {
+        if (sub == 0)
+            acc.hi = b[row];
+        for (int j = rowptr[row] + sub; j < rowptr[row + 1]; j += 8) {
+            const double a = val[j], xv = x[colind[j]];
+            const double p = a * xv;
+            const double pe = __builtin_fma(a, xv, -p);
+            acc = dd_add(acc, -p, -pe);
+        }
+    }
+#pragma unroll
+    for (int off = 4; off > 0; off >>= 1) {
+        const double oh = __shfl_xor(acc.hi, off, 8);
+        const double ol = __shfl_xor(acc.lo, off, 8);
+        acc = dd_add(acc, oh, ol);
+    }
+    if (row < n && sub == 0)
+        r[row] = acc.hi + acc.lo;
+}
+
+void csr_residual_dd(rsgpu_csr *A, const double *b, const double *x, double *r)
+{
+    if (A->n == 0)
+        return;
+    const long long threads = (long long)A->n * 8;
+    hipLaunchKernelGGL(k_residual_dd, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
+                       A->ctx->stream, A->n, A->rowptr.p, A->colind.p, A->val.p, b, x, r);
+    RS_CHECK_LAUNCH();
+}
+
 // ---------------------------------------------------------------- zeroRows
 __global__ void k_zero_rows(const int *__restrict__ rowptr,
                             const int *__restrict__ colind, double *val,

@@ -360,6 +360,15 @@ class CompRowMatrix:
         self.ctx.synchronize()
         return y
 
+    def residual_dd(self, b, x, r):
+        """r = b - A x in twice the working precision, rounded once (device
+        tensors; rsgpu_csr_residual_dd)."""
+        _torch_sync(b, x, r)
+        _raise(L.load().rsgpu_csr_residual_dd(self.h, _dptr(b), _dptr(x), _dptr(r)),
+               self.ctx.h)
+        self.ctx.synchronize()
+        return r
+
 
 class Matrices:
     @staticmethod

@@ -16,8 +16,18 @@ def oracle_system(po, pr: P.PressureProblem, with_diag=True):
     A = po.assemble_flux(pr.n, here, there, md_ptr, md_el, md_t, rp, ci)
     flux_only = A.copy()
     if pr.accumulation:
-        for i in range(pr.n):
-            assert po.csr_add(rp, ci, A, i, i, pr.accumulation) == 0
+        # CompRowMatrix.add(i, i, acc) on every row (or_csr_add): one IEEE
+        # addition into each diagonal entry, vectorised (each pattern row
+        # holds its diagonal once)
+        rows = np.repeat(np.arange(pr.n, dtype=np.int32), np.diff(rp))
+        dpos = np.flatnonzero(ci == rows)
+        assert dpos.size == pr.n
+        A[dpos] += pr.accumulation
+        if pr.n <= 4096:
+            chk = flux_only.copy()
+            for i in range(pr.n):
+                assert po.csr_add(rp, ci, chk, i, i, pr.accumulation) == 0
+            assert np.array_equal(chk, A)
     A_unpinned = A.copy()
     st = po.zero_rows(rp, ci, A, 1.0, pr.pinned)
     assert st == 0

@@ -49,11 +49,12 @@ def test_mpfa_vjp_bitexact(gpu, oracle, pr):
 
 
 def test_gradient_vs_cpu_adjoint_and_fd(gpu, oracle):
+    """Reduced grid, MTJ's ILU (one block): J, rates and the gradient
+    against the oracle's CPU FP64 adjoint; central differences on 8 cells."""
     from ressim_amd.adjoint import WellRateMismatch
     pr, m, K0, obs = _problem()
     q_obs = oracle_well_mismatch(oracle, obs, np.log(obs.K6[:, 0]), np.zeros(4),
                                  need_grad=False)[1]
-    # MTJ's ILU (one block): the tightest forward solves for the differences
     W = WellRateMismatch(pr.dim, pr.nx, pr.ny, pr.nz, pr.dx, pr.dy, pr.dz, K0,
                          pr.sources, pr.pinned, q_obs=q_obs, nblocks=1)
     J = W.forward(m)
@@ -65,7 +66,6 @@ def test_gradient_vs_cpu_adjoint_and_fd(gpu, oracle):
     # producers take all the injected fluid
     assert abs(W.q.sum() + 1.0) < 1e-10
     assert np.linalg.norm(g - go) <= 1e-8 * np.linalg.norm(go)
-    # central differences on random cells, step 1e-6
     rng = np.random.default_rng(12)
     cells = rng.choice(pr.n, 8, replace=False)
     h = 1e-6
@@ -75,6 +75,92 @@ def test_gradient_vs_cpu_adjoint_and_fd(gpu, oracle):
         mm[e] -= h
         fd = (W.forward(mp) - W.forward(mm)) / (2 * h)
         assert abs(fd - g[e]) <= 1e-5 * max(abs(g[e]), 1e-3 * np.abs(g).max()), (e, fd, g[e])
+
+
+@pytest.mark.timeout(400)
+def test_gradient_full_size_vs_cpu_adjoint_and_fd(gpu, oracle):
+    """The 64x64x32 adjoint workload (4 producers, 1 injector, 131,072
+    log-permeabilities, q_obs from the second seeded field): the gradient
+    against the oracle's CPU FP64 adjoint, and central differences with the
+    step 1e-6 on 32 random cells among those whose gradient is at least 1e-3
+    of the largest: the maximum relative error is at most 1e-5."""
+    from ressim_amd.adjoint import WellRateMismatch
+    pr, m, K0, obs = _problem(64, 64, 32)
+    W0 = WellRateMismatch(pr.dim, pr.nx, pr.ny, pr.nz, pr.dx, pr.dy, pr.dz,
+                          obs.K6 / obs.K6[:, :1], obs.sources, obs.pinned)
+    q_obs = W0.rates(np.log(obs.K6[:, 0]))
+    W = WellRateMismatch(pr.dim, pr.nx, pr.ny, pr.nz, pr.dx, pr.dy, pr.dz, K0,
+                         pr.sources, pr.pinned, q_obs=q_obs)
+    J = W.forward(m)
+    g = W.backward()
+    print(f"C5 full: J={J:.6e} iterations {W.last_iterations} "
+          f"refinements {W.last_refinements}", flush=True)
+    Jo, qo, go = oracle_well_mismatch(oracle, pr, m, q_obs)
+    rel = np.linalg.norm(g - go) / np.linalg.norm(go)
+    print(f"C5 full: |J-Jo|/J={abs(J - Jo) / abs(Jo):.2e} grad vs CPU adjoint {rel:.2e}",
+          flush=True)
+    assert abs(J - Jo) <= 1e-10 * abs(Jo)
+    assert rel <= 1e-8
+    rng = np.random.default_rng(2024)
+    big = np.flatnonzero(np.abs(g) >= 1e-3 * np.abs(g).max())
+    cells = rng.choice(big, 32, replace=False)
+    h = 1e-6
+    errs = []
+    for e in cells:
+        mp, mm = m.copy(), m.copy()
+        mp[e] += h
+        mm[e] -= h
+        fd = (W.forward(mp) - W.forward(mm)) / (2 * h)
+        errs.append(abs(fd - g[e]) / abs(fd))
+    print(f"C5 full: FD h=1e-6 on 32 cells: max rel err {max(errs):.2e} "
+          f"median {np.median(errs):.2e}", flush=True)
+    assert max(errs) <= 1e-5
+
+
+def test_residual_dd_is_exact_to_rounding(gpu):
+    """rsgpu_csr_residual_dd against the exact rational residual rounded to
+    double: x is an FP64 solution of A x = b, so b - A x cancels to a few
+    units in the last place of the products and the FP64 residual keeps few
+    correct digits; the compensated one is the rounded exact value (within
+    one unit in its last place)."""
+    from fractions import Fraction
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spl
+    import torch
+    rng = np.random.default_rng(3)
+    n = 400
+    A = (sp.random(n, n, density=0.03, random_state=4, format="csr") * 10
+         + sp.diags(np.full(n, 30.0))).tocsr()
+    A.sort_indices()
+    # rows of 0 to 64+ entries: one long row exercises the strided lanes
+    b = rng.standard_normal(n)
+    x = spl.spsolve(A.tocsc(), b)
+    rp, ci, d = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data
+    M = gpu.CompRowMatrix.from_csr(n, n, rp, ci, d)
+    bd, xd = torch.tensor(b, device="cuda"), torch.tensor(x, device="cuda")
+    r = torch.empty_like(bd)
+    M.residual_dd(bd, xd, r)
+    r = r.cpu().numpy()
+    exact = np.empty(n)
+    for i in range(n):
+        acc = Fraction(b[i])
+        for j in range(rp[i], rp[i + 1]):
+            acc -= Fraction(d[j]) * Fraction(x[ci[j]])
+        exact[i] = float(acc)
+    # Dot2's bound (Ogita-Rump-Oishi 2005, Prop. 5.5), with a factor 2 of
+    # slack: |r - exact| <= eps |exact| + gamma_k^2 (|b| + sum |a x|)
+    u = np.finfo(np.float64).eps / 2
+    k = np.diff(rp) + 1
+    gam = k * u / (1 - k * u)
+    mag = np.abs(b) + np.abs(A).multiply(np.abs(x)[None, :]).sum(axis=1).A1
+    err = np.abs(r - exact)
+    assert np.all(err <= 2 * (2 * u * np.abs(exact) + gam ** 2 * mag))
+    rel = err / np.abs(exact)
+    r64 = b - A @ x
+    rel64 = np.abs(r64 - exact) / np.abs(exact)
+    print(f"residual_dd median rel err {np.median(rel):.1e}, FP64 {np.median(rel64):.1e}")
+    assert np.median(rel) <= 1e-14
+    assert np.median(rel64) > 1e-3
 
 
 def test_transposed_solve(gpu, oracle):
