@@ -48,20 +48,52 @@ def sliced_bytes(n, nchunks, nslots, s_val):
     return nslots * s_val + n * 4 + nchunks * 8 + n * s_val + n * s_val
 
 
+def ilu_bytes(n, nnz_lu, s):
+    """ILU(0) apply (L and U sweeps): factor values and int32 columns once,
+    two row-pointer arrays, the diagonal index, b read, y written and read,
+    x written (algorithmic bytes per application)."""
+    return nnz_lu * (s + 4) + 2 * (n + 1) * 4 + 4 * n + 4 * n * s
+
+
 # HBM bytes per launch from the committed rocprofv3 counter passes
-# (FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000, factors from a 512 MiB copy in
-# the same process); valid for the 128x128x64 workload they were taken on.
+# (FETCH_SIZE x factor + WRITE_SIZE x factor, the factors from a 512 MiB
+# copy in the same process); valid for the 128x128x64 workload they were
+# taken on.
 PMC_PROFILE = os.path.join(ROOT, "profiles", "k_spmv_mpfa3d_128x128x64_pmc.md")
+ILU_PMC_PROFILE = os.path.join(ROOT, "profiles", "k_ilu_apply_mpfa3d_128x128x64_pmc.md")
+ILU32_KERNEL = "k_ilu_apply_band<float>"
 
 
-def measured_traffic(pr, kernel):
-    if (pr.nx, pr.ny, pr.nz) != (128, 128, 64) or not os.path.exists(PMC_PROFILE):
+def measured_traffic(pr, kernel, profile=PMC_PROFILE):
+    if (pr.nx, pr.ny, pr.nz) != (128, 128, 64) or not os.path.exists(profile):
         return None
-    for line in open(PMC_PROFILE):
+    for line in open(profile):
         cells = [c.strip() for c in line.split("|")]
         if len(cells) > 4 and cells[1] == f"`{kernel}`" and cells[4].endswith(" B"):
             return int(cells[4][:-2].replace(",", ""))
     return None
+
+
+def host_cpu():
+    """(threads for the all-cores leg, CPU model): the threads OpenMP is
+    given (OMP_NUM_THREADS, else the CPUs this process may run on) and the
+    lscpu model name."""
+    try:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        threads = 0
+    if threads <= 0:
+        threads = len(os.sched_getaffinity(0))
+    model = "unknown"
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return threads, model
 
 
 def main():
@@ -219,75 +251,105 @@ def main():
     err_mx = float(torch.linalg.norm(xmx - x64) / torch.linalg.norm(x64))
     log(f"mixed vs fp64 relative solution difference {err_mx:.3e}")
 
-    # ---------------------------------------------------------- SpMV roofline
-    xin = torch.randn(n, dtype=torch.float64, device=dev)
-    yout = torch.empty(n, dtype=torch.float64, device=dev)
-    torch.cuda.synchronize()
+    # ---------------------------------------------------------- kernel timings
+    # Each kernel is timed with HIP events on the stream it runs on over many
+    # back-to-back launches: warm (the same operands every launch, so up to
+    # 256 MB of them stay in the last-level cache) and cold (the launches
+    # cycle over C copies of the operands, C x the kernel's bytes > 256 MB,
+    # so each launch reads its operands from HBM, as inside a solve whose
+    # working set far exceeds that cache).  roofline.achieved uses the cold
+    # time.
+    rp_h, ci_h, d_h = A.getRowPointers(), A.getColumnIndices(), A.getData()
+    NCOPY = 4
 
-    def time_kernel(launch, reps):
+    def time_launches(launchers, reps):
+        for f in launchers:
+            f()
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
-        for _ in range(3):
-            launch()
         e0.record(stream)
-        for _ in range(reps):
-            launch()
+        for r in range(reps):
+            launchers[r % len(launchers)]()
         e1.record(stream)
         e1.synchronize()
         return e0.elapsed_time(e1) / reps * 1e-3
 
-    t_spmv = time_kernel(lambda: lib.rsgpu_csr_mult(A.h, xin.data_ptr(), yout.data_ptr()),
-                         a.spmv_reps)
-    b64 = spmv_bytes(n, nnz, 8, 8)
-    gbs64 = b64 / t_spmv / 1e9
-    A.sync_f32()
-    xin32 = xin.float()
-    y32 = torch.empty(n, dtype=torch.float32, device=dev)
-    torch.cuda.synchronize()
-    t_spmv32 = time_kernel(lambda: lib.rsgpu_csr_mult_f32(A.h, xin32.data_ptr(),
-                                                          y32.data_ptr()), a.spmv_reps)
-    gbs32 = spmv_bytes(n, nnz, 4, 4) / t_spmv32 / 1e9
-    # the sliced copy the solvers stream (FP64 filled per FP64 solve, FP32
-    # with the FP32 values)
-    nch, nslots, ntab = A.sliced_info()
-    sl = None
-    if nch:
-        assert lib.rsgpu_csr_sync_sliced(A.h) == 0
-        torch.cuda.synchronize()
-        t_sl64 = time_kernel(lambda: lib.rsgpu_csr_mult_sliced(A.h, xin.data_ptr(),
-                                                               yout.data_ptr()), a.spmv_reps)
-        t_sl32 = time_kernel(lambda: lib.rsgpu_csr_mult_sliced_f32(A.h, xin32.data_ptr(),
-                                                                   y32.data_ptr()), a.spmv_reps)
-        bs64, bs32 = sliced_bytes(n, nch, nslots, 8), sliced_bytes(n, nch, nslots, 4)
-        sl = {"fp64_us": t_sl64 * 1e6, "fp64_GBps": bs64 / t_sl64 / 1e9, "bytes_fp64": bs64,
-              "fp32_us": t_sl32 * 1e6, "fp32_GBps": bs32 / t_sl32 / 1e9, "bytes_fp32": bs32,
-              "chunks": nch, "slots": nslots, "tables": ntab,
-              "csr_equivalent_fp64_GBps": b64 / t_sl64 / 1e9}
-        log(f"sliced SpMV fp64 {t_sl64 * 1e6:.1f} us {sl['fp64_GBps']:.0f} GB/s "
-            f"({sl['fp64_GBps'] / PEAK_HBM_GBS:.1%}); fp32 {t_sl32 * 1e6:.1f} us "
-            f"{sl['fp32_GBps']:.0f} GB/s; {nch} chunks, {ntab} tables, "
-            f"{nslots / nnz - 1:.1%} padding")
-    # block ILU(0) apply (L and U sweeps in one kernel)
+    mats = [A] + [mtj.CompRowMatrix.from_csr(n, n, rp_h, ci_h, d_h, ctx=ctx)
+                  for _ in range(NCOPY - 1)]
+    xs64 = [torch.randn(n, dtype=torch.float64, device=dev) for _ in range(NCOPY)]
+    ys64 = [torch.empty(n, dtype=torch.float64, device=dev) for _ in range(NCOPY)]
+    xs32 = [v.float() for v in xs64]
+    ys32 = [torch.empty(n, dtype=torch.float32, device=dev) for _ in range(NCOPY)]
+    for Mt in mats:
+        Mt.sync_f32()
     M64.setMatrix(A)
+    M32.setMatrix(A)
+    pre64 = [M64] + [mtj.BlockILU(a.nblocks, precision=_lib.FP64, ctx=ctx)
+                     for _ in range(NCOPY - 1)]
+    pre32 = [M32] + [mtj.BlockILU(a.nblocks, precision=_lib.FP32, ctx=ctx)
+                     for _ in range(NCOPY - 1)]
+    for Pm in pre64[1:] + pre32[1:]:
+        Pm.setMatrix(A)
+    torch.cuda.synchronize()
+    R = a.spmv_reps
+
+    def both(make, reps):
+        fs = [make(c) for c in range(NCOPY)]
+        return {"warm_us": time_launches(fs[:1], reps) * 1e6,
+                "cold_us": time_launches(fs, reps) * 1e6}
+
+    b64, b32 = spmv_bytes(n, nnz, 8, 8), spmv_bytes(n, nnz, 4, 4)
+    kt = {}
+    kt["spmv_csr_fp64"] = both(lambda c: lambda: lib.rsgpu_csr_mult(
+        mats[c].h, xs64[c].data_ptr(), ys64[c].data_ptr()), R)
+    kt["spmv_csr_fp32"] = both(lambda c: lambda: lib.rsgpu_csr_mult_f32(
+        mats[c].h, xs32[c].data_ptr(), ys32[c].data_ptr()), R)
+    nch, nslots, ntab = A.sliced_info()
+    if nch:
+        for Mt in mats:
+            assert lib.rsgpu_csr_sync_sliced(Mt.h) == 0
+        kt["spmv_sliced_fp64"] = both(lambda c: lambda: lib.rsgpu_csr_mult_sliced(
+            mats[c].h, xs64[c].data_ptr(), ys64[c].data_ptr()), R)
+        kt["spmv_sliced_fp32"] = both(lambda c: lambda: lib.rsgpu_csr_mult_sliced_f32(
+            mats[c].h, xs32[c].data_ptr(), ys32[c].data_ptr()), R)
     info = M64.info()
-    t_ilu = time_kernel(lambda: lib.rsgpu_prec_apply(M64.h, xin.data_ptr(), yout.data_ptr()),
-                        max(5, a.spmv_reps // 5))
     nnz_lu = info["nnz"]
-    ilu_bytes = nnz_lu * 12 + 2 * (n + 1) * 4 + 4 * n + 4 * n * 8
-    log(f"SpMV fp64 {t_spmv * 1e6:.1f} us {gbs64:.0f} GB/s ({gbs64 / PEAK_HBM_GBS:.1%}); "
-        f"fp32 {t_spmv32 * 1e6:.1f} us {gbs32:.0f} GB/s; "
-        f"BJ-ILU0 apply {t_ilu * 1e6:.1f} us {ilu_bytes / t_ilu / 1e9:.0f} GB/s "
-        f"levels={info['levels_lower']}")
+    kt["ilu_apply_fp64"] = both(lambda c: lambda: lib.rsgpu_prec_apply(
+        pre64[c].h, xs64[c].data_ptr(), ys64[c].data_ptr()), max(10, R // 2))
+    kt["ilu_apply_fp32"] = both(lambda c: lambda: lib.rsgpu_prec_apply_f32(
+        pre32[c].h, xs32[c].data_ptr(), ys32[c].data_ptr()), max(10, R // 2))
     ctx.set_stream(None)
+    del mats[1:], pre64[1:], pre32[1:]
+    ilu64_bytes, ilu32_bytes = ilu_bytes(n, nnz_lu, 8), ilu_bytes(n, nnz_lu, 4)
+    for k, byt in (("spmv_csr_fp64", b64), ("spmv_csr_fp32", b32), ("spmv_sliced_fp64", b64),
+                   ("spmv_sliced_fp32", b32), ("ilu_apply_fp64", ilu64_bytes),
+                   ("ilu_apply_fp32", ilu32_bytes)):
+        if k in kt:
+            v = kt[k]
+            v["algorithmic_bytes"] = byt
+            v["cold_GBps"] = byt / v["cold_us"] / 1e3
+            v["warm_GBps"] = byt / v["warm_us"] / 1e3
+            v["cold_frac"] = v["cold_GBps"] / PEAK_HBM_GBS
+            log(f"{k}: warm {v['warm_us']:.1f} us, cold {v['cold_us']:.1f} us = "
+                f"{v['cold_GBps']:.0f} GB/s ({v['cold_frac']:.1%} of 8 TB/s) of "
+                f"{byt / 1e6:.1f} MB algorithmic")
+    if nch:
+        kt["spmv_sliced_fp64"]["format_bytes"] = sliced_bytes(n, nch, nslots, 8)
+        kt["spmv_sliced_fp32"]["format_bytes"] = sliced_bytes(n, nch, nslots, 4)
+        kt["sliced_format"] = {"chunks": nch, "slots": nslots, "tables": ntab}
+    kt["cold_copies"] = NCOPY
 
     # ---------------------------------------------------------- CPU baseline
     cpu = None
     if not a.no_cpu:
         try:
-            cpu = cpu_baseline(A, pr, a, st64.iterations)
+            cpu = cpu_baseline(A, pr, a, mon_rtol, st64.iterations, x64, xmx)
         except Exception as e:  # the baseline must not hide the GPU result
             log(f"cpu baseline failed: {e!r}")
 
+    spmv_key = "spmv_sliced_fp64" if nch else "spmv_csr_fp64"
+    il = kt["ilu_apply_fp32"]
+    sp = kt[spmv_key]
     out = {
         "metric": "solves/s + SpMV HBM GB/s, 1M-cell MPFA, mixed vs FP64 at 1e-8 residual",
         "value": 1.0 / dtmx,
@@ -307,28 +369,25 @@ def main():
                    "n": n, "nnz": nnz, "nblocks": a.nblocks},
         "fp64": {"solves_per_s": 1.0 / dt64, "ms_per_solve": dt64 * 1e3,
                  "iterations": st64.iterations,
-                 "true_rel_residual": st64.true_residual / st64.b_norm},
+                 "true_rel_residual": st64.true_residual / st64.b_norm,
+                 "rel_err_vs_fp64_oracle": cpu and cpu.get("rel_err_fp64")},
         "mixed": {"solves_per_s": 1.0 / dtmx, "ms_per_solve": dtmx * 1e3,
                   "outer_iterations": stmx.outer_iterations,
                   "inner_iterations": stmx.inner_iterations,
                   "true_rel_residual": stmx.true_residual / stmx.b_norm,
-                  "rel_err_vs_fp64": err_mx},
-        "spmv": {"fp64_GBps": gbs64, "fp64_us": t_spmv * 1e6, "fp32_GBps": gbs32,
-                 "fp32_us": t_spmv32 * 1e6, "bytes_fp64": b64},
-        "spmv_sliced": sl,
-        "ilu_apply": {"us": t_ilu * 1e6, "GBps": ilu_bytes / t_ilu / 1e9,
-                      "levels_per_block": info["levels_lower"]},
-        "roofline": ({"bound": "hbm", "achieved": sl["fp64_GBps"], "peak": PEAK_HBM_GBS,
-                      "unit": "GB/s", "frac": sl["fp64_GBps"] / PEAK_HBM_GBS,
-                      "traffic": measured_traffic(pr, "k_spmv_sliced<double>"),
-                      "kernel": "k_spmv_sliced<double> (the FP64 solves' SpMV, "
-                                "CompRowMatrix.mult's sums over the sliced copy)",
-                      "algorithmic_bytes": sl["bytes_fp64"]} if sl else
-                     {"bound": "hbm", "achieved": gbs64, "peak": PEAK_HBM_GBS,
-                      "unit": "GB/s", "frac": gbs64 / PEAK_HBM_GBS,
-                      "traffic": measured_traffic(pr, "k_spmv<double>"),
-                      "kernel": "k_spmv<double,EpiMult> (CompRowMatrix.mult)",
-                      "algorithmic_bytes": b64}),
+                  "rel_err_vs_fp64_oracle": cpu and cpu.get("rel_err_mixed"),
+                  "rel_err_vs_gpu_fp64": err_mx},
+        "kernels": kt,
+        "spmv": {"kernel": spmv_key, "cold_GBps": sp["cold_GBps"], "cold_us": sp["cold_us"],
+                 "warm_us": sp["warm_us"], "frac": sp["cold_frac"],
+                 "algorithmic_bytes": b64},
+        "roofline": {"bound": "hbm", "achieved": il["cold_GBps"], "peak": PEAK_HBM_GBS,
+                     "unit": "GB/s", "frac": il["cold_frac"],
+                     "traffic": measured_traffic(pr, ILU32_KERNEL, ILU_PMC_PROFILE),
+                     "kernel": f"{ILU32_KERNEL} (block-Jacobi ILU(0) apply of the FP32 inner "
+                               "solve, the mixed solve's largest kernel time)",
+                     "algorithmic_bytes": ilu32_bytes, "cold_us": il["cold_us"],
+                     "warm_us": il["warm_us"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -1060,32 +1119,47 @@ def probe(a, A, n, nnz, lib, mtj, _lib, ctx, stream, dev):
     log(f"probe {a.probe}: n={n} nnz={nnz} reps={reps}; copy 512 MiB x3")
 
 
-def cpu_baseline(A, pr, a, gpu_iters):
-    """Oracle FP64 GMRES(30) + block-Jacobi ILU(0) on the host cores (one
-    thread, MTJ's concurrency): factor + k iterations timed for two k, the
-    per-iteration cost extrapolated to the GPU FP64 iteration count."""
+def cpu_baseline(A, pr, a, mon_rtol, gpu_iters, x64, xmx):
+    """The oracle's FP64 GMRES(30) + block-Jacobi ILU(0) on the same matrix
+    and right-hand side, with the GPU FP64 solve's monitor tolerance, solved
+    completely twice on the host cores: one thread (MTJ's concurrency; the
+    sequential restatement) and all cores (or_gmres_bjilu_omp: OpenMP SpMV,
+    vector updates and per-block ILU(0)).  The one-thread solution is the
+    FP64 oracle the GPU results' errors are measured against."""
     from oracle import pyoracle as po
     rp = A.getRowPointers()
     ci = A.getColumnIndices()
     d = A.getData()
     b = pr.rhs()
-    times = {}
-    for k in (2, a.cpu_iters):
-        t = time.perf_counter()
-        po.solve("gmres", rp, ci, d, b, rtol=a.rtol, prec="block_ilu", nblocks=a.nblocks,
-                 max_iter=k, hist_cap=k + 8)
-        times[k] = time.perf_counter() - t
-        log(f"cpu oracle: setup + {k} iterations in {times[k]:.2f}s")
-    per_it = (times[a.cpu_iters] - times[2]) / (a.cpu_iters - 2)
-    setup = max(times[2] - 2 * per_it, 0.0)
-    est = setup + per_it * gpu_iters
-    return {"value": 1.0 / est, "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": f"oracle GMRES(30)+block-ILU0 x{a.nblocks} FP64 on the same matrix: "
-                      f"setup+2 and setup+{a.cpu_iters} iterations timed "
-                      f"({times[2]:.1f}s, {times[a.cpu_iters]:.1f}s); solve time "
-                      f"extrapolated to the GPU FP64 iteration count {gpu_iters} "
-                      f"({est:.1f}s per solve)",
-            "setup_s": setup, "per_iteration_s": per_it}
+    threads, model = host_cpu()
+    log(f"cpu: {threads} threads for the all-cores leg, nproc {os.cpu_count()}, model {model}")
+    t = time.perf_counter()
+    x1, s1, _ = po.solve("gmres", rp, ci, d, b, rtol=mon_rtol, prec="block_ilu",
+                         nblocks=a.nblocks, max_iter=20000, hist_cap=16)
+    t1 = time.perf_counter() - t
+    t = time.perf_counter()
+    xn, sn = po.gmres_bjilu_omp(rp, ci, d, b, nthreads=threads, rtol=mon_rtol,
+                                nblocks=a.nblocks, max_iter=20000)
+    tn = time.perf_counter() - t
+    r1 = np.linalg.norm(b - po.mult(rp, ci, d, x1)) / np.linalg.norm(b)
+    rn = np.linalg.norm(b - po.mult(rp, ci, d, xn)) / np.linalg.norm(b)
+    nx1 = np.linalg.norm(x1)
+    e64 = float(np.linalg.norm(x64.cpu().numpy() - x1) / nx1)
+    emx = float(np.linalg.norm(xmx.cpu().numpy() - x1) / nx1)
+    log(f"cpu oracle 1 thread: {t1:.2f}s, {s1.iterations} iterations, true rel residual "
+        f"{r1:.2e}; {threads} threads: {tn:.2f}s, {sn.iterations} iterations, {rn:.2e}")
+    log(f"GPU vs FP64 oracle: FP64 {e64:.2e}, mixed {emx:.2e}")
+    return {"value": 1.0 / tn, "unit": "solves/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "nproc": os.cpu_count(),
+            "sample": f"one complete FP64 GMRES(30) + block ILU(0) x{a.nblocks} solve of the "
+                      f"same system (monitor rtol {mon_rtol:.3e}, as the GPU FP64 solve) by the "
+                      f"oracle's OpenMP form on {threads} threads: {tn:.2f}s, {sn.iterations} "
+                      f"iterations (GPU {gpu_iters}), true rel residual {rn:.2e}",
+            "one_thread": {"value": 1.0 / t1, "unit": "solves/s", "cores": 1,
+                           "seconds": t1, "iterations": s1.iterations,
+                           "true_rel_residual": r1},
+            "all_cores_seconds": tn,
+            "rel_err_fp64": e64, "rel_err_mixed": emx}
 
 
 if __name__ == "__main__":

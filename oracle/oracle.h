@@ -183,6 +183,12 @@ int or_qmr(int n, const int *rowptr, const int *colind, const double *data,
            const double *b, double *x, const or_opts *opts, or_stats *stats,
            double *hist, int hist_cap);
 int or_block_starts(int n, int nblocks, int *bstart);
+/* All-host-cores GMRES(m) + block-Jacobi ILU(0) over opts->nblocks equal
+ * row ranges (mtj_omp.c; the CPU baseline's multi-threaded leg): nthreads
+ * OpenMP threads (0 = OpenMP's default). */
+int or_gmres_bjilu_omp(int n, const int *rowptr, const int *colind, const double *data,
+                       const double *b, double *x, const or_opts *opts, or_stats *stats,
+                       int nthreads);
 
 /* ---- Dense LU (LAPACK 3.0 dgetf2/dgetrs restatement) --------------------- */
 int or_dgesv(int n, int nrhs, double *a, int lda, int *ipiv, double *b,

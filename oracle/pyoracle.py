@@ -120,6 +120,8 @@ def lib():
         for f in (L.or_bicgstab, L.or_gmres, L.or_ir, L.or_cg, L.or_bicg, L.or_cgs, L.or_qmr):
             f.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p, C.POINTER(Opts),
                           C.POINTER(Stats), _f64p, C.c_int]
+        L.or_gmres_bjilu_omp.argtypes = [C.c_int, _i32p, _i32p, _f64p, _f64p, _f64p,
+                                         C.POINTER(Opts), C.POINTER(Stats), C.c_int]
         L.or_icc_factor.argtypes = [C.c_int, _i32p, _i32p, _f64p, _i32p, C.c_void_p]
         L.or_icc_apply.argtypes = [C.c_int, _i32p, _i32p, _f64p, _i32p, _f64p, _f64p, _f64p]
         L.or_icc_apply.restype = None
@@ -374,6 +376,20 @@ def solve(kind, rowptr, colind, data, b, x0=None, *, max_iter=100000, rtol=1e-5,
           "qmr": lib().or_qmr}[kind]
     fn(n, _i32(rowptr), _i32(colind), _f64(data), _f64(b), x, C.byref(o), C.byref(s), hist, cap)
     return x, s, hist[: min(s.hist_len, cap)]
+
+
+def gmres_bjilu_omp(rowptr, colind, data, b, x0=None, *, nthreads=0, max_iter=100000,
+                    rtol=1e-5, atol=1e-50, dtol=1e5, restart=30, nblocks=1):
+    """or_gmres_bjilu_omp: GMRES(restart) + block-Jacobi ILU(0) over nblocks
+    equal row ranges on nthreads OpenMP threads (the CPU baseline's
+    all-cores leg).  Returns (x, Stats)."""
+    n = len(rowptr) - 1
+    x = np.zeros(n) if x0 is None else _f64(x0).copy()
+    o = Opts(max_iter, rtol, atol, dtol, restart, PREC["block_ilu"], nblocks, amg_params())
+    s = Stats()
+    lib().or_gmres_bjilu_omp(n, _i32(rowptr), _i32(colind), _f64(data), _f64(b), x,
+                             C.byref(o), C.byref(s), int(nthreads))
+    return x, s
 
 
 class AMG:

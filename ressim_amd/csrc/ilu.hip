@@ -1094,6 +1094,26 @@ __device__ void sweep_generic(int lb, int le, int W, const LevelMeta *__restrict
     }
 }
 
+// dst[0..n) = src[0..n) by the workgroup with 16 loads per thread in flight
+// per batch: a plain strided loop waits for each load before issuing the
+// next, one memory round trip per blockDim.x elements.  Indices past the end
+// clamp to the last element (n >= 1); only the stores are predicated.
+template <typename E>
+__device__ __forceinline__ void wg_copy(E *__restrict__ dst, const E *__restrict__ src, int n)
+{
+    const int nt = (int)blockDim.x;
+    for (int r0 = threadIdx.x; r0 < n; r0 += nt * 16) {
+        E v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            v[u] = src[min(r0 + nt * u, n - 1)];
+#pragma unroll
+        for (int u = 0; u < 16; ++u)
+            if (r0 + nt * u < n)
+                dst[r0 + nt * u] = v[u];
+    }
+}
+
 // MODE 0: ILU apply (L: sum form; U: sum form, divide); 1: ILU transApply
 // (U': subtract form, divide; L': subtract form); 2: ICC apply (R':
 // subtract form, divide; R: sum form, divide)
@@ -1121,8 +1141,8 @@ __global__ void __launch_bounds__(256)
     const int blk = blockIdx.x;
     const int b0 = bstart[blk], b1 = bstart[blk + 1], nb = b1 - b0;
     T *xv = LDS ? (T *)rs_smem : scratch + b0 + blk;
-    for (int r = threadIdx.x; r < nb; r += blockDim.x)
-        xv[r] = b[b0 + r];
+    if (nb > 0)
+        wg_copy(xv, b + b0, nb);
     if (threadIdx.x == 0)
         xv[nb] = T(0);
     __syncthreads();
@@ -1417,12 +1437,12 @@ __global__ void __launch_bounds__(64)
     LevelMeta *sm = (LevelMeta *)(rs_smem + xv_bytes);
     const int la0 = A.blk[blk], nla = A.blk[blk + 1] - la0;
     const int lb0 = B.blk[blk], nlb = B.blk[blk + 1] - lb0;
-    for (int q = threadIdx.x; q < nla; q += blockDim.x)
-        sm[q] = A.meta[la0 + q];
-    for (int q = threadIdx.x; q < nlb; q += blockDim.x)
-        sm[nla + q] = B.meta[lb0 + q];
-    for (int r = threadIdx.x; r < nb; r += blockDim.x)
-        xv[r] = b[b0 + r];
+    if (nla > 0)
+        wg_copy(sm, A.meta + la0, nla);
+    if (nlb > 0)
+        wg_copy(sm + nla, B.meta + lb0, nlb);
+    if (nb > 0)
+        wg_copy(xv, b + b0, nb);
     if (threadIdx.x == 0)
         xv[nb] = T(0);
     __syncthreads();
@@ -1638,12 +1658,12 @@ __global__ void __launch_bounds__(64)
     LevelMeta *sm = (LevelMeta *)(rs_smem + xv_bytes);
     const int la0 = A.blk[blk], nla = A.blk[blk + 1] - la0;
     const int lb0 = B.blk[blk], nlb = B.blk[blk + 1] - lb0;
-    for (int q = threadIdx.x; q < nla; q += blockDim.x)
-        sm[q] = A.meta[la0 + q];
-    for (int q = threadIdx.x; q < nlb; q += blockDim.x)
-        sm[nla + q] = B.meta[lb0 + q];
-    for (int r = threadIdx.x; r < nb; r += blockDim.x)
-        xv[r] = b[b0 + r];
+    if (nla > 0)
+        wg_copy(sm, A.meta + la0, nla);
+    if (nlb > 0)
+        wg_copy(sm + nla, B.meta + lb0, nlb);
+    if (nb > 0)
+        wg_copy(xv, b + b0, nb);
     if (threadIdx.x < 3)
         xv[nb + threadIdx.x] = T(0);
     __syncthreads();
@@ -1935,12 +1955,22 @@ __device__ __forceinline__ void band_sweep(int Lx, int Ly, int g0, const T *rec,
     const unsigned char *mp = mask + (g0 + rb);
     const T *xr = xv + rb;
     BandStage<T, UPPER, FULL> ring[D];
+    // The ring is a true D-deep prefetch only if the slots are loaded in
+    // slot order here and refilled in slot order in the loop: the compiler
+    // then waits for D-1 outstanding loads per step.  Any other order (the
+    // scheduler reorders free loads; a guard branch before the loop lets
+    // them sink past it) makes it wait for all of them at the loop head,
+    // one full memory round trip per D steps.  Hence the barriers and the
+    // do-while (S >= 1 always).
 #pragma unroll
-    for (int k = 0; k < D; ++k)
+    for (int k = 0; k < D; ++k) {
         band_load<T, UPPER, FULL>(rp, dp, mp, dir * k, ring[k]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
     T cur = T(0), h1 = T(0), h2 = T(0), h3 = T(0);
     T rhs = xr[0];
-    for (int s0 = 0; s0 < S; s0 += D) {
+    int s0 = 0;
+    do {
         // base pointers of this round (every access below has an immediate
         // offset from them)
         const T *rps = rp + (ptrdiff_t)8 * dir * s0;
@@ -1995,8 +2025,13 @@ __device__ __forceinline__ void band_sweep(int Lx, int Ly, int g0, const T *rec,
             cur = act ? val : cur;
             // refill the slot for step s + D
             band_load<T, UPPER, FULL>(rps, dps, mps, dir * (k + D), st);
+            // keep the refill in its step: left free, the scheduler gathers
+            // a round's refills at the round's end, and the next round then
+            // waits for all of them (one memory round trip per D steps)
+            __builtin_amdgcn_sched_barrier(0);
         }
-    }
+        s0 += D;
+    } while (s0 < S);
 }
 
 template <typename T>
@@ -2015,8 +2050,8 @@ __global__ void __launch_bounds__(64)
     T *xv = (T *)rs_smem + RS_BAND_PAD;
     const int2 d = dims[blockIdx.x];   // Lx, Ly; Ly < 0: the block's pattern is not full
     const int Lx = d.x, Ly = d.y < 0 ? -d.y : d.y;
-    for (int r = threadIdx.x; r < nb; r += 64)
-        xv[r] = b[b0 + r];
+    if (nb > 0)
+        wg_copy(xv, b + b0, nb);
     for (int r = threadIdx.x; r < RS_BAND_PAD; r += 64) {
         xv[-1 - r] = T(0);
         xv[nb + r] = T(0);

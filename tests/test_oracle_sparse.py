@@ -399,3 +399,25 @@ def test_icc_zero_diagonal_and_cg(oracle):
     assert s.status == 0 and s.iterations < s0.iterations / 3
     Asp = scipy_csr(rp, ci, v)
     assert np.linalg.norm(Asp @ x - b) < 1e-10 * np.linalg.norm(b)
+
+
+def test_gmres_bjilu_omp_equals_sequential_oracle(oracle):
+    """The all-cores CPU baseline (or_gmres_bjilu_omp): with one thread every
+    operation is the sequential oracle's, so iterate, iteration count and
+    residual are bit-identical to or_gmres + block ILU(0); with several
+    threads only the dot products' summation order changes."""
+    s = oracle_system(oracle, P.small3d(nx=14, ny=12, nz=9))
+    args = (s["rowptr"], s["colind"], s["data"], s["b"])
+    xo, so, _ = oracle.solve("gmres", *args, rtol=1e-11, prec="block_ilu", nblocks=7,
+                             restart=20)
+    assert so.status == 0
+    x1, s1 = oracle.gmres_bjilu_omp(*args, nthreads=1, rtol=1e-11, nblocks=7, restart=20)
+    assert s1.status == 0 and s1.iterations == so.iterations
+    assert s1.residual == so.residual
+    np.testing.assert_array_equal(x1, xo)
+    x4, s4 = oracle.gmres_bjilu_omp(*args, nthreads=4, rtol=1e-11, nblocks=7, restart=20)
+    assert s4.status == 0 and abs(s4.iterations - so.iterations) <= 2
+    assert np.linalg.norm(x4 - xo) <= 1e-9 * np.linalg.norm(xo)
+    # failure reasons follow DefaultIterationMonitor
+    _, s5 = oracle.gmres_bjilu_omp(*args, nthreads=2, rtol=1e-11, nblocks=7, max_iter=3)
+    assert s5.status == 3
