@@ -305,6 +305,28 @@ __global__ void __launch_bounds__(RS_SPMV_BS)
     sliced_rows<T>(row0, sv, val, x, epi);
 }
 
+// GMRES's scale step fused into the next step's SpMV: v = (1/H(i,i-1)) w is
+// formed on the fly for every gathered entry (k_gm_scale's rule, so the
+// products are bit-identical to scaling first), and each wave stores its own
+// 64 rows of v (the basis vector v_i).  Skipped when the step is.
+template <typename T>
+__global__ void __launch_bounds__(RS_SPMV_BS)
+    k_spmv_scale_sl(SlView sv, const T *__restrict__ val, const T *__restrict__ w,
+                    T *__restrict__ vout, T *__restrict__ y, const DevState *st)
+{
+    if (stopped(st, true))
+        return;
+    const int row0 = sliced_wave_row(xcd_remap((int)blockIdx.x, (int)gridDim.x));
+    if (row0 >= sv.n)
+        return;
+    const T sc = (T)(1. / st->hsub);
+    const int r = row0 + (int)(threadIdx.x & 63);
+    if (r < sv.n)
+        vout[r] = (sc == T(0)) ? T(0) : sc * w[r];
+    EpiMult<T> epi{y};
+    sliced_rows<T, EpiMult<T>, true>(row0, sv, val, w, epi, sc);
+}
+
 // y = A x without a reduction, skipped when the solve has stopped
 template <typename T, bool B3>
 __global__ void __launch_bounds__(RS_SPMV_BS)
@@ -1801,9 +1823,17 @@ void run_gmres(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const T *b, T *x,
     for (;;) {
         hipLaunchKernelGGL(k_gm_start<T>, dim3(G), dim3(RS_VBS), 0, s, n, r, V,
                            st, sl);
+        const T *slv = sliced_vals<T>(S, A);
         for (int i = 0; i < m; ++i) {
             T *vi = V + (size_t)i * L;
-            launch_spmv_skip<T>(S, A, vi, u, st);
+            if (i > 0 && slv) {
+                // v_i = w / H(i,i-1) stored by the SpMV that reads it
+                hipLaunchKernelGGL(k_spmv_scale_sl<T>, dim3(sliced_tiles(n)),
+                                   dim3(RS_SPMV_BS), 0, s, A->slview(), slv, w, vi, u, st);
+                RS_CHECK_LAUNCH();
+            } else {
+                launch_spmv_skip<T>(S, A, vi, u, st);
+            }
             papply(S, M, u, w, st);
             if (S->o.cgs && sizeof(T) == 4) {
                 // a quarter of the vector grid: 16 elements per thread
@@ -1820,8 +1850,9 @@ void run_gmres(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const T *b, T *x,
                 hipLaunchKernelGGL(k_gm_last<T>, dim3(G), dim3(RS_VBS), 0, s, n, i, w, vi, st,
                                    sl);
             }
-            hipLaunchKernelGGL(k_gm_scale<T>, dim3(E), dim3(RS_VBS), 0, s, n, w,
-                               V + (size_t)(i + 1) * L, st);
+            if (!slv)
+                hipLaunchKernelGGL(k_gm_scale<T>, dim3(E), dim3(RS_VBS), 0, s, n, w,
+                                   V + (size_t)(i + 1) * L, st);
             RS_CHECK_LAUNCH();
             if ((i + 1) % every == 0 && i + 1 < m) {
                 P.post();

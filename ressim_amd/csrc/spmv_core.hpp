@@ -280,10 +280,10 @@ __device__ __forceinline__ void spmv_core_b3(const int *__restrict__ wg_rows,
 // nontemporal hint keeps them out of the last-level cache, which the FP32
 // inner solve's working set largely fits.
 #define RS_SL_BATCH 16
-template <typename T, class Epi>
+template <typename T, class Epi, bool SCALE = false>
 __device__ __forceinline__ void sliced_batch(int s0, int W, int r, int cmax, unsigned m,
                                              const int (&offs)[32], const T *vc,
-                                             const T *__restrict__ x, T &dot)
+                                             const T *__restrict__ x, T &dot, T sc = T(1))
 {
     T v[RS_SL_BATCH], xg[RS_SL_BATCH];
 #pragma unroll
@@ -292,6 +292,13 @@ __device__ __forceinline__ void sliced_batch(int s0, int W, int r, int cmax, uns
         const int col = min(max(r + offs[s0 + k], 0), cmax);
         v[k] = vc[(size_t)sl * RS_SL_ROWS];
         xg[k] = x[col];
+    }
+    // SCALE: x is sc * x[col] (zero when sc is zero), formed exactly as a
+    // separate scaling pass would store it
+    if (SCALE) {
+#pragma unroll
+        for (int k = 0; k < RS_SL_BATCH; ++k)
+            xg[k] = (sc == T(0)) ? T(0) : sc * xg[k];
     }
     // every product and sum is formed and then selected: a conditional add
     // lets the compiler sink the slot's loads under the condition, where
@@ -304,9 +311,9 @@ __device__ __forceinline__ void sliced_batch(int s0, int W, int r, int cmax, uns
 }
 
 // the 64 rows from row0 (wave-uniform, row0 < n)
-template <typename T, class Epi>
+template <typename T, class Epi, bool SCALE = false>
 __device__ __forceinline__ void sliced_rows(int row0, const SlView &S, const T *__restrict__ val,
-                                            const T *__restrict__ x, Epi &epi)
+                                            const T *__restrict__ x, Epi &epi, T sc = T(1))
 {
     const int lane = threadIdx.x & 63;
     const int c = row0 / RS_SL_ROWS, sub = row0 - c * RS_SL_ROWS;
@@ -335,9 +342,9 @@ __device__ __forceinline__ void sliced_rows(int row0, const SlView &S, const T *
 #pragma unroll
         for (int k = 0; k < 32; ++k)
             offs[k] = k < W ? offs[k] : lastoff;
-        sliced_batch<T, Epi>(0, W, rr, cmax, m, offs, vc, x, dot);
+        sliced_batch<T, Epi, SCALE>(0, W, rr, cmax, m, offs, vc, x, dot, sc);
         if (W > RS_SL_BATCH)
-            sliced_batch<T, Epi>(RS_SL_BATCH, W, rr, cmax, m, offs, vc, x, dot);
+            sliced_batch<T, Epi, SCALE>(RS_SL_BATCH, W, rr, cmax, m, offs, vc, x, dot, sc);
     }
     if (live)
         epi.row(r, dot);
