@@ -142,6 +142,8 @@ def main():
     ap.add_argument("--impes-ncomp", type=int, default=3)
     ap.add_argument("--impes-thermal", action="store_true")
     ap.add_argument("--restart", type=int, default=30)
+    ap.add_argument("--fp64-mgs", action="store_true",
+                    help="FP64 GMRES with MTJ's modified Gram-Schmidt (default: CGS2)")
     ap.add_argument("--mgs", action="store_true",
                     help="modified Gram-Schmidt in the FP32 inner GMRES (default: classical)")
     ap.add_argument("--probe", choices=["spmv", "ilu", "ilu32", "solve", "mixed"], default=None,
@@ -194,6 +196,7 @@ def main():
     M64 = mtj.BlockILU(a.nblocks, precision=_lib.FP64, ctx=ctx)
     M32 = mtj.BlockILU(a.nblocks, precision=_lib.FP32, ctx=ctx)
     g64 = mtj.GMRES(b, 30)
+    g64.cgs = not a.fp64_mgs           # CGS2; MTJ's MGS measured beside it
     g64.setIterationMonitor(mtj.DefaultIterationMonitor(20000, a.rtol, 1e-50, 1e5))
     g64.setPreconditioner(M64)
     gmx = mtj.MixedPrecisionSolver(b, inner="gmres", restart=a.restart,
@@ -244,8 +247,17 @@ def main():
             f"true_rel_res={st.true_residual / st.b_norm:.3e}")
         return dt, st
 
-    dt64, st64 = timed(step_fp64, max(1, a.steps // 2), a.warmup, "fp64 GMRES(30)+BJ-ILU0")
+    dt64, st64 = timed(step_fp64, max(1, a.steps // 2), a.warmup,
+                       f"fp64 GMRES(30)+BJ-ILU0 ({'MGS' if a.fp64_mgs else 'CGS2'})")
     x64 = x.clone()
+    mgs = None
+    if not a.fp64_mgs:
+        # MTJ's own orthogonalisation (modified Gram-Schmidt), same monitor
+        g64.cgs = False
+        dtm, stm = timed(step_fp64, 1, 1, "fp64 GMRES(30)+BJ-ILU0 (MGS)")
+        mgs = {"ms_per_solve": dtm * 1e3, "iterations": stm.iterations,
+               "true_rel_residual": stm.true_residual / stm.b_norm}
+        g64.cgs = True
     dtmx, stmx = timed(step_mixed, a.steps, a.warmup, "mixed FP32-GMRES/FP64-IR")
     xmx = x.clone()
     err_mx = float(torch.linalg.norm(xmx - x64) / torch.linalg.norm(x64))
@@ -368,6 +380,8 @@ def main():
                                f"rel residual {a.rtol:g}",
                    "n": n, "nnz": nnz, "nblocks": a.nblocks},
         "fp64": {"solves_per_s": 1.0 / dt64, "ms_per_solve": dt64 * 1e3,
+                 "orthogonalisation": "MGS" if a.fp64_mgs else "CGS2",
+                 "mgs": mgs,
                  "iterations": st64.iterations,
                  "true_rel_residual": st64.true_residual / st64.b_norm,
                  "rel_err_vs_fp64_oracle": cpu and cpu.get("rel_err_fp64")},
@@ -948,6 +962,8 @@ def bench_blackoil(a):
                                f"ILU(0) of {a.bo_lines} x-lines, rel residual {a.rtol:g}",
                    "n": n, "nnzb": nnzb, "ilu_blocks": len(bs) - 1},
         "fp64": {"solves_per_s": 1.0 / dt64, "ms_per_solve": dt64 * 1e3,
+                 "orthogonalisation": "MGS" if a.fp64_mgs else "CGS2",
+                 "mgs": mgs,
                  "iterations": st64.iterations,
                  "true_rel_residual": st64.true_residual / st64.b_norm},
         "mixed": {"solves_per_s": 1.0 / dtmx, "ms_per_solve": dtmx * 1e3,

@@ -279,10 +279,13 @@ typedef struct {
      * preconditioner set on the original matrix, applied through
      * transApply (ILU.java:72-78), i.e. (LU)^-T. */
     int prec_transpose;
-    /* Mixed form only: the FP32 inner GMRES orthogonalises with classical
-     * Gram-Schmidt (all projections of a column from one pass) instead of
-     * MTJ's modified Gram-Schmidt (GMRES.java:150-153); no reference
-     * counterpart.  The FP64 refinement still tests the true residual. */
+    /* GMRES orthogonalisation (no reference counterpart; MTJ uses modified
+     * Gram-Schmidt, GMRES.java:150-153, the default 0): with cgs != 0 the
+     * FP32 inner GMRES of the mixed form uses classical Gram-Schmidt (all
+     * projections of a column from one pass) and the FP64 GMRES uses CGS2
+     * (two classical passes, H(:,i) = h1 + h2): four kernels per step
+     * instead of i + 2, same iterates up to round-off.  The FP64 refinement
+     * of the mixed form still tests the true residual. */
     int cgs;
 } rsgpu_solver_opts;
 

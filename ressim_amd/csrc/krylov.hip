@@ -57,9 +57,14 @@ static int grid_for(int n)
     return std::max(1, std::min(g, 1024));
 }
 
+// Both flags are read unconditionally: a short-circuit test makes the
+// second read wait for the first, one more memory round trip at the start
+// of every solver kernel.
 __device__ __forceinline__ bool stopped(const DevState *st, bool inner_step)
 {
-    return st->done || (inner_step && st->cycle_end);
+    const int d = st->done;
+    const int c = st->cycle_end;
+    return (d != 0) | (inner_step & (c != 0));
 }
 
 // block pattern of a 3x3 block matrix (null for a plain CSR) and the
@@ -1005,12 +1010,14 @@ __global__ void __launch_bounds__(RS_VBS)
         gm_finish_column(st, i, tot[0]);
 }
 
-// Classical Gram-Schmidt for the FP32 inner solve of the mixed form (no
-// reference counterpart): all projections of column i from one pass over
-// w and the basis, H(k,i) = w . v_k for k = 0..i.
-#define RS_CGS_CHUNK 16
-// Every thread accumulates up to 16 dot products in registers over its
-// elements, each wave reduces them with shuffles, one LDS exchange per 16
+// Classical Gram-Schmidt (no reference counterpart; the FP32 inner GMRES of
+// the mixed form, and both passes of the FP64 GMRES's CGS2): all
+// projections of column i from one pass over w and the basis,
+// H(k,i) = w . v_k for k = 0..i.
+#define RS_CGS_CHUNK 8
+#define RS_CGS_ET 4
+// Every thread accumulates up to 8 dot products in registers over its
+// elements, each wave reduces them with shuffles, one LDS exchange per 8
 // columns gives the group's partials, and the last group to arrive sums
 // the partials with one wave per column.  The summation order is fixed by
 // the grid, so results are reproducible run to run.
@@ -1021,19 +1028,34 @@ __device__ __forceinline__ void cgs_accum(int n, int nv, int stride, const T *__
 {
     using VT = typename V16<T>::type;
     constexpr int N = V16<T>::N;
-    for (int j = blockIdx.x * RS_VBS + threadIdx.x; j < nv; j += stride) {
-        const VT wv = reinterpret_cast<const VT *>(w)[j];
-        const T *we = reinterpret_cast<const T *>(&wv);
-        VT kv[CC];
+    // RS_CGS_ET elements per thread per pass, every load of the pass in
+    // flight before the first product (indices past the end clamp to the
+    // last element; their products are selected away), elements summed in
+    // the same order as one at a time
+    constexpr int ET = RS_CGS_ET;
+    for (int j0 = blockIdx.x * RS_VBS + threadIdx.x; j0 < nv; j0 += ET * stride) {
+        VT wv[ET], kv[ET][CC];
 #pragma unroll
-        for (int q = 0; q < CC; ++q)
-            kv[q] = reinterpret_cast<const VT *>(V0 + (size_t)q * ldv)[j];
+        for (int e = 0; e < ET; ++e) {
+            const int je = min(j0 + e * stride, nv - 1);
+            wv[e] = reinterpret_cast<const VT *>(w)[je];
 #pragma unroll
-        for (int q = 0; q < CC; ++q) {
-            const T *ke = reinterpret_cast<const T *>(&kv[q]);
+            for (int q = 0; q < CC; ++q)
+                kv[e][q] = reinterpret_cast<const VT *>(V0 + (size_t)q * ldv)[je];
+        }
 #pragma unroll
-            for (int c = 0; c < N; ++c)
-                a[q] += (double)(we[c] * ke[c]);
+        for (int e = 0; e < ET; ++e) {
+            const bool on = j0 + e * stride < nv;
+            const T *we = reinterpret_cast<const T *>(&wv[e]);
+#pragma unroll
+            for (int q = 0; q < CC; ++q) {
+                const T *ke = reinterpret_cast<const T *>(&kv[e][q]);
+#pragma unroll
+                for (int c = 0; c < N; ++c) {
+                    const double u = a[q] + (double)(we[c] * ke[c]);
+                    a[q] = on ? u : a[q];
+                }
+            }
         }
     }
     for (int j = nv * N + blockIdx.x * RS_VBS + threadIdx.x; j < n; j += stride)
@@ -1042,18 +1064,23 @@ __device__ __forceinline__ void cgs_accum(int n, int nv, int stride, const T *__
             a[q] += (double)(w[j] * V0[(size_t)q * ldv + j]);
 }
 
+// second: the projections go to st->h2 (CGS2's second pass) instead of
+// column i of H
 template <typename T>
 __global__ void __launch_bounds__(RS_VBS)
     k_gm_cgs_dots(int n, int i, const T *__restrict__ w, const T *__restrict__ V, size_t ldv,
-                  DevState *st, RedSlot slot)
+                  DevState *st, RedSlot slot, int second)
 {
-    if (stopped(st, true))
-        return;
     constexpr int N = V16<T>::N;
     constexpr int NW = RS_VBS / 64;
     __shared__ double sh[NW][RS_CGS_CHUNK];
     __shared__ int last;
-    const int cnt = i + 1, ld = st->m + 1;
+    // state reads before the stop test, so they share its round trip
+    double *Hb = st->H, *h2b = st->h2;
+    const int ld = st->m + 1;
+    if (stopped(st, true))
+        return;
+    const int cnt = i + 1;
     const int G = (int)gridDim.x;
     const int nv = n / N;
     const int stride = G * RS_VBS;
@@ -1074,8 +1101,6 @@ __global__ void __launch_bounds__(RS_VBS)
         break;
             RS_CGS_CASE(1) RS_CGS_CASE(2) RS_CGS_CASE(3) RS_CGS_CASE(4)
             RS_CGS_CASE(5) RS_CGS_CASE(6) RS_CGS_CASE(7) RS_CGS_CASE(8)
-            RS_CGS_CASE(9) RS_CGS_CASE(10) RS_CGS_CASE(11) RS_CGS_CASE(12)
-            RS_CGS_CASE(13) RS_CGS_CASE(14) RS_CGS_CASE(15) RS_CGS_CASE(16)
 #undef RS_CGS_CASE
         default:
             break;
@@ -1159,59 +1184,98 @@ __global__ void __launch_bounds__(RS_VBS)
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
             const double r = wave_sum(acc[c]);
-            if (lane == 0 && k0 + c * NW < cnt)
-                st->H[k0 + c * NW + i * ld] = r;
+            if (lane == 0 && k0 + c * NW < cnt) {
+                if (second)
+                    h2b[k0 + c * NW] = r;
+                else
+                    Hb[k0 + c * NW + i * ld] = r;
+            }
         }
     }
 }
 
-// w -= sum_k H(k,i) v_k (each element in k order, add's alpha == 0 skip),
-// then H(i+1,i) = ||w||, Givens, monitor
-template <typename T>
+// sixteen basis vectors of element j from column k0 (indices past i clamp
+// to v_i, so every load is valid and none sits under a condition)
+template <typename T, typename VT>
+__device__ __forceinline__ void cgs_load16(const T *__restrict__ V, size_t ldv, int i, int k0,
+                                           int j, VT (&kv)[16])
+{
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+        kv[q] = reinterpret_cast<const VT *>(V + (size_t)min(k0 + q, i) * ldv)[j];
+}
+
+// w_j -= h_k v_kj for k = k0 .. min(k0 + 15, i), in k order, skipped where
+// h_k == 0 (add's alpha == 0 rule): the loads are pinned before any update,
+// so they overlap and none sinks under the skip rule; skipped updates are
+// selected away
+template <typename T, typename VT, int N>
+__device__ __forceinline__ void cgs_apply16(VT &wv, VT (&kv)[16], const double *Hc, int i,
+                                            int k0)
+{
+    T *we = reinterpret_cast<T *>(&wv);
+    T hq[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+        hq[q] = (T)(-Hc[min(k0 + q, i)]);
+#pragma unroll
+    for (int q = 0; q < 16; ++q)
+        pin_vec<T, N>(reinterpret_cast<T *>(&kv[q]));
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const bool on = k0 + q <= i && hq[q] != T(0);
+        const T *ke = reinterpret_cast<const T *>(&kv[q]);
+#pragma unroll
+        for (int c = 0; c < N; ++c) {
+            const T u = we[c] + hq[q] * ke[c];
+            we[c] = on ? u : we[c];
+        }
+    }
+}
+
+// w -= sum_k h_k v_k (each element in k order, add's alpha == 0 skip).
+// PASS 0 (classical Gram-Schmidt) and PASS 2 (CGS2's second pass): then
+// H(i+1,i) = ||w||, Givens, monitor; PASS 2 first adds its coefficients
+// (st->h2) into column i of H.  PASS 1 (CGS2's first pass): the update
+// alone.  h is column i of H for PASS 0 and 1, st->h2 for PASS 2.
+template <typename T, int PASS>
 __global__ void __launch_bounds__(RS_VBS)
     k_gm_cgs_last(int n, int i, T *__restrict__ w, const T *__restrict__ V, size_t ldv,
                   DevState *st, RedSlot slot)
 {
-    if (stopped(st, true))
-        return;
     using VT = typename V16<T>::type;
     constexpr int N = V16<T>::N;
-    const int ld = st->m + 1;
-    const double *Hc = st->H + i * ld;
-    double a = 0;
     const int nv = n / N;
     const int stride = gridDim.x * RS_VBS;
-    for (int j = blockIdx.x * RS_VBS + threadIdx.x; j < nv; j += stride) {
-        VT wv = reinterpret_cast<const VT *>(w)[j];
-        T *we = reinterpret_cast<T *>(&wv);
-        // sixteen basis vectors and coefficients per step, loaded
-        // unconditionally (indices past i clamp to v_i) and pinned before any
-        // update, so all loads overlap and none sinks under the skip rule;
-        // updates stay in k order, skipped ones selected away
-        for (int k0 = 0; k0 <= i; k0 += 16) {
-            VT kv[16];
-            T hq[16];
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const int k = min(k0 + q, i);
-                kv[q] = reinterpret_cast<const VT *>(V + (size_t)k * ldv)[j];
-                hq[q] = (T)(-Hc[k]);
-            }
-#pragma unroll
-            for (int q = 0; q < 16; ++q)
-                pin_vec<T, N>(reinterpret_cast<T *>(&kv[q]));
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const bool on = k0 + q <= i && hq[q] != T(0);
-                const T *ke = reinterpret_cast<const T *>(&kv[q]);
-#pragma unroll
-                for (int c = 0; c < N; ++c) {
-                    const T u = we[c] + hq[q] * ke[c];
-                    we[c] = on ? u : we[c];
-                }
-            }
+    const int j0 = blockIdx.x * RS_VBS + threadIdx.x;
+    // The first element's vector and first sixteen basis vectors are loaded
+    // before the stop test, together with the state reads: the kernel then
+    // waits for one memory round trip before its first update, not two.
+    const int jf = nv > 0 ? min(j0, nv - 1) : 0;
+    VT wv{}, kv[16];
+    if (nv > 0) {
+        wv = reinterpret_cast<const VT *>(w)[jf];
+        cgs_load16<T, VT>(V, ldv, i, 0, jf, kv);
+    }
+    double *Hb = st->H;
+    const double *h2b = st->h2;
+    const int ld = st->m + 1;
+    if (stopped(st, true))
+        return;
+    const double *Hc = PASS == 2 ? h2b : Hb + i * ld;
+    double a = 0;
+    for (int j = j0; j < nv; j += stride) {
+        if (j != j0) {
+            wv = reinterpret_cast<const VT *>(w)[j];
+            cgs_load16<T, VT>(V, ldv, i, 0, j, kv);
+        }
+        cgs_apply16<T, VT, N>(wv, kv, Hc, i, 0);
+        for (int k0 = 16; k0 <= i; k0 += 16) {
+            cgs_load16<T, VT>(V, ldv, i, k0, j, kv);
+            cgs_apply16<T, VT, N>(wv, kv, Hc, i, k0);
         }
         reinterpret_cast<VT *>(w)[j] = wv;
+        const T *we = reinterpret_cast<const T *>(&wv);
 #pragma unroll
         for (int c = 0; c < N; ++c)
             a += (double)(we[c] * we[c]);
@@ -1226,9 +1290,17 @@ __global__ void __launch_bounds__(RS_VBS)
         w[j] = wj;
         a += (double)(wj * wj);
     }
+    if (PASS == 1)
+        return;
     double v[1] = {a}, tot[1];
-    if (block_reduce_finish<RS_VBS, 1>(slot, v, tot) && threadIdx.x == 0)
+    if (block_reduce_finish<RS_VBS, 1>(slot, v, tot) && threadIdx.x == 0) {
+        if (PASS == 2) {
+            double *Hi = Hb + i * ld;
+            for (int k = 0; k <= i; ++k)
+                Hi[k] += h2b[k];
+        }
         gm_finish_column(st, i, tot[0]);
+    }
 }
 
 // v_{i+1} = (1/H(i+1,i)) w
@@ -1382,6 +1454,7 @@ void state_init(rsgpu_solver *S, DevState *dst, DevBuf<double> &hist,
         h.s = small + (size_t)(m + 1) * m;
         h.gc = h.s + (m + 1);
         h.gs = h.gc + (m + 1);
+        h.h2 = h.gs + (m + 1);
     }
     RS_HIP(hipMemcpyAsync(dst, &h, sizeof(h), hipMemcpyHostToDevice,
                           S->ctx->stream));
@@ -1836,12 +1909,20 @@ void run_gmres(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const T *b, T *x,
             }
             papply(S, M, u, w, st);
             if (S->o.cgs && sizeof(T) == 4) {
-                // a quarter of the vector grid: 16 elements per thread
-                // amortise the in-group reductions
                 hipLaunchKernelGGL(k_gm_cgs_dots<T>, dim3(std::max(1, G / 4)), dim3(RS_VBS), 0,
-                                   s, n, i, w, V, L, st, sl);
-                hipLaunchKernelGGL(k_gm_cgs_last<T>, dim3(G), dim3(RS_VBS), 0, s, n, i, w, V, L,
-                                   st, sl);
+                                   s, n, i, w, V, L, st, sl, 0);
+                hipLaunchKernelGGL((k_gm_cgs_last<T, 0>), dim3(G), dim3(RS_VBS), 0, s, n, i, w,
+                                   V, L, st, sl);
+            } else if (S->o.cgs) {
+                // CGS2: two classical passes, H(:,i) = h1 + h2
+                hipLaunchKernelGGL(k_gm_cgs_dots<T>, dim3(std::max(1, G / 4)), dim3(RS_VBS), 0,
+                                   s, n, i, w, V, L, st, sl, 0);
+                hipLaunchKernelGGL((k_gm_cgs_last<T, 1>), dim3(G), dim3(RS_VBS), 0, s, n, i, w,
+                                   V, L, st, sl);
+                hipLaunchKernelGGL(k_gm_cgs_dots<T>, dim3(std::max(1, G / 4)), dim3(RS_VBS), 0,
+                                   s, n, i, w, V, L, st, sl, 1);
+                hipLaunchKernelGGL((k_gm_cgs_last<T, 2>), dim3(G), dim3(RS_VBS), 0, s, n, i, w,
+                                   V, L, st, sl);
             } else {
                 for (int k = 0; k <= i; ++k)
                     hipLaunchKernelGGL(k_gm_mgs<T>, dim3(G), dim3(RS_VBS), 0, s, n, i, k, w,
@@ -1932,7 +2013,7 @@ extern "C" int rsgpu_solver_create(rsgpu_ctx *ctx, const rsgpu_solver_opts *opts
         S->ticket.alloc(2 * RS_TICKET_WORDS);
         S->ticket.zero(ctx->stream);
         const int m = std::max(1, opts->restart);
-        S->gmres_small.alloc((size_t)2 * ((m + 1) * m + 3 * (m + 1)));
+        S->gmres_small.alloc((size_t)2 * ((m + 1) * m + 4 * (m + 1)));
         size_t hcap = std::min<size_t>((size_t)opts->max_iter * 2 + 8, 1u << 22);
         S->hist.alloc(hcap);
         S->hist_in.alloc(std::min<size_t>((size_t)opts->inner_max_iter * 2 + 8,
@@ -2061,7 +2142,7 @@ void run_mixed(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const double *b,
         throw RsError{RSGPU_BAD_ARGUMENT, "mixed solve needs an FP32 preconditioner"};
     int inner_total = 0, outer = 0;
     DevState host;
-    double *small_in = S->gmres_small.p + ((size_t)(m + 1) * m + 3 * (m + 1));
+    double *small_in = S->gmres_small.p + ((size_t)(m + 1) * m + 4 * (m + 1));
     launch_resid<double>(S, A, b, x, r, nullptr, r32, st, FIN_INIT_NORM);
     for (;;) {
         RS_HIP(hipMemcpyAsync(&host, st, sizeof(host), hipMemcpyDeviceToHost, s));

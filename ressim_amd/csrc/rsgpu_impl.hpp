@@ -325,6 +325,7 @@ struct DevState {
     double normr;
     double hsub;        // unrotated H(i+1,i) for the basis scaling
     double *H, *s, *gc, *gs;
+    double *h2;         // CGS2: the second pass's projections of column gi
     int m;
     int pad1;
     // QMR (QMR.java:136-243); c1, c2: the p / q scale factors, val: the d / s

@@ -71,11 +71,12 @@ def test_c2_assembly_bitexact(gpu, c2):
     assert np.array_equal(A.getData(), s["data"])
 
 
-def _gmres64(gpu, A, b, rtol, nblocks=256):
+def _gmres64(gpu, A, b, rtol, nblocks=256, cgs=False):
     import torch
     M = gpu.BlockILU(nblocks)
     M.setMatrix(A)
     g = gpu.GMRES(b, 30)
+    g.cgs = cgs
     g.setIterationMonitor(gpu.DefaultIterationMonitor(20000, rtol, 1e-50, 1e5))
     g.setPreconditioner(M)
     x = torch.zeros(A.n, dtype=torch.float64, device="cuda")
@@ -113,6 +114,15 @@ def test_c2_fp64_and_mixed_solves_vs_oracle(gpu, c2, c2_oracle_solution):
     assert abs(st.iterations - so.iterations) <= 3
     assert rel <= 1e-10
     assert err <= 1e-8
+    # CGS2 orthogonalisation (the bench's FP64 form): same iterates up to
+    # round-off
+    x2, st2 = _gmres64(gpu, A, b, 1e-12, cgs=True)
+    err2 = _relerr(x2, xo)
+    print(f"C2 FP64 CGS2 rtol 1e-12: {st2.iterations} iterations, true rel residual "
+          f"{st2.true_residual / st2.b_norm:.2e}, rel err vs oracle {err2:.2e}", flush=True)
+    assert abs(st2.iterations - so.iterations) <= 3
+    assert st2.true_residual <= 1e-10 * st2.b_norm
+    assert err2 <= 1e-8
     # mixed form refined to a true residual of 1e-11
     xm, sm = _mixed(gpu, A, b, 1e-11)
     errm = _relerr(xm, xo)

@@ -73,6 +73,26 @@ def test_gmres(gpu, oracle, c1, prec, nb):
     np.testing.assert_allclose(it.history[:3], ho[:3], rtol=1e-10)
 
 
+@pytest.mark.parametrize("sys", ["c1", "s3"])
+def test_gmres_cgs2_fp64_vs_oracle_mgs(gpu, oracle, c1, s3, sys):
+    """FP64 GMRES with CGS2 orthogonalisation (rsgpu_solver_opts.cgs) against
+    the oracle's MGS GMRES (GMRES.java:150-153): the same Krylov iterates up
+    to round-off, so the same iteration count within 3 and the solution to
+    1e-8."""
+    s = {"c1": c1, "s3": s3}[sys]
+    xo, so, ho = oracle.solve("gmres", s["rowptr"], s["colind"], s["data"], s["b"],
+                              rtol=1e-12, prec="block_ilu", nblocks=4)
+    assert so.status == 0
+    solver = gpu.GMRES(s["b"], 30)
+    solver.cgs = True
+    solver.setIterationMonitor(gpu.DefaultIterationMonitor(100000, 1e-12, 1e-50, 1e5))
+    x, _ = _run(gpu, s, solver, gpu.BlockILU(4))
+    _check(s, x, xo)
+    it = solver.getIterationMonitor()
+    assert abs(it.iterations() - so.iterations) <= 3
+    np.testing.assert_allclose(it.history[:3], ho[:3], rtol=1e-10)
+
+
 def test_ir_diag_dominant(gpu, oracle):
     rng = np.random.default_rng(11)
     n = 400
