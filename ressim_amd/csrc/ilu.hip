@@ -700,16 +700,25 @@ static bool band_block(int b0, int b1, const std::vector<int> &rp, const std::ve
     return true;
 }
 
+// slots per line in the full blocks' LDS layout: the sweep's step count
+// Lx + 2 Ly - 2 rounded up to whole 16-step rounds
+__host__ __device__ inline int band_slot_pitch(int Lx, int Ly)
+{
+    return (Lx + 2 * Ly - 2 + 15) & ~15;
+}
+
 // RSGPU_ILU_BAND=0 keeps the level schedules for every block
 static void band_setup(rsgpu_prec *M, const std::vector<int> &rp, const std::vector<int> &ci,
                        const std::vector<int> &dg)
 {
     M->band = false;
+    M->band_slots = 0;
     M->band_dims.release();
     M->band_pos.release();
     M->band_mask.release();
     M->band64.release();
     M->band32.release();
+    M->band_rd.release();
     const char *e = getenv("RSGPU_ILU_BAND");
     if ((e && e[0] == '0') || M->cell || ilu_kernel_choice() == 3 || M->n == 0 ||
         M->kind == RSGPU_PREC_ICC)
@@ -749,6 +758,8 @@ static void band_setup(rsgpu_prec *M, const std::vector<int> &rp, const std::vec
         }
         if (!full)
             dims[b].y = -Ly;
+        else
+            M->band_slots = std::max(M->band_slots, Ly * band_slot_pitch(Lx, Ly));
     }
     hipStream_t s = M->ctx->stream;
     M->band_dims.upload(dims, s);
@@ -764,6 +775,8 @@ static void band_setup(rsgpu_prec *M, const std::vector<int> &rp, const std::vec
     } else {
         M->band32.alloc(9 * np);
         M->band32.zero(s);
+        M->band_rd.alloc(np);
+        M->band_rd.zero(s);
     }
     RS_HIP(hipStreamSynchronize(s));
     M->band = true;
@@ -1727,9 +1740,15 @@ static void launch_rowdiag(hipStream_t s, const SweepData &D, const int *diag, c
 // per-row line-wavefront records from the factor: L0..L3, U0..U3 (zero
 // where absent) of row r at 8 (r + PAD), diagonals at 8 (n + 2 PAD) + r + PAD
 // (the padding rows stay zero)
+// rd (FP32 factors only, may be null): 1 / U_ii rounded to FP64 at
+// rd[r + PAD]; (float)((double)y * rd) equals y / U_ii rounded to FP32 for
+// every y (the product's error is below 2^-51 relative, while a quotient of
+// two floats that is not exactly representable lies at least 2^-49 relative
+// from every rounding midpoint of FP32), at a third of the division's
+// dependent instructions
 template <typename T>
 __global__ void k_band_gather(int n, const int *__restrict__ pos, const T *__restrict__ lu,
-                              T *__restrict__ out)
+                              T *__restrict__ out, double *__restrict__ rd)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n)
@@ -1741,7 +1760,10 @@ __global__ void k_band_gather(int n, const int *__restrict__ pos, const T *__res
         o[k] = p[k] >= 0 ? lu[p[k]] : T(0);
         o[4 + k] = p[5 + k] >= 0 ? lu[p[5 + k]] : T(0);
     }
-    out[(size_t)8 * ((size_t)n + 2 * RS_BAND_PAD) + r + RS_BAND_PAD] = lu[p[4]];
+    const T dg = lu[p[4]];
+    out[(size_t)8 * ((size_t)n + 2 * RS_BAND_PAD) + r + RS_BAND_PAD] = dg;
+    if (rd)
+        rd[r + RS_BAND_PAD] = 1.0 / (double)dg;
 }
 
 template <typename T>
@@ -1750,7 +1772,8 @@ static void launch_band_gather(rsgpu_prec *M, const T *lu, T *out)
     if (!M->band)
         return;
     hipLaunchKernelGGL(k_band_gather<T>, dim3((M->n + 255) / 256), dim3(256), 0,
-                       M->ctx->stream, M->n, M->band_pos.p, lu, out);
+                       M->ctx->stream, M->n, M->band_pos.p, lu, out,
+                       sizeof(T) == 4 ? M->band_rd.p : nullptr);
     RS_CHECK_LAUNCH();
 }
 
@@ -2034,12 +2057,161 @@ __device__ __forceinline__ void band_sweep(int Lx, int Ly, int g0, const T *rec,
     } while (s0 < S);
 }
 
+// 16-byte vector of T (LDS accesses of the full-block sweeps)
+template <typename T>
+struct BandVec;
+template <>
+struct BandVec<float> {
+    using type = float4;
+    static constexpr int N = 4;
+};
+template <>
+struct BandVec<double> {
+    using type = double2;
+    static constexpr int N = 2;
+};
+
+// Full blocks (every row holds all of its in-block 9-point neighbours): the
+// same wavefront, with the block's vector in LDS by line and step instead of
+// by row.  Line j owns SP = band_slot_pitch(Lx, Ly) slots from j SP, and slot
+// s holds row i = s - 2 j (the row lane j treats at L step s; slots outside
+// 0 <= i < Lx are padding), so the L sweep walks its lane's slots upwards and
+// the U sweep downwards (U step t: slot SP - 1 - t) and a round of 16 steps
+// covers 16 aligned slots: its right-hand sides come in with 16-byte LDS
+// reads issued one round ahead, and its results go out with 16-byte writes at
+// the round's end (lanes past Ly write nothing).
+// NOPRED: the couplings are summed without the boundary tests.  An absent
+// coupling's record value is zero, and the sum, which starts at +0, is never
+// -0 before its last term, so adding a zero product changes no bit as long
+// as the neighbour value is finite: with finite data the result equals the
+// tested form bit for bit (non-finite values may then spread further).
+// FP32 U sweeps divide by multiplying with the FP64 reciprocal rd (exact,
+// see k_band_gather); FP64 sweeps divide.
+template <typename T, bool UPPER>
+struct FullStage {
+    using DT = typename std::conditional<sizeof(T) == 4, double, T>::type;
+    BandRec<T> v;
+    DT d;
+};
+
+template <typename T, bool UPPER, bool NOPRED>
+__device__ __forceinline__ void band_sweep_full(int Lx, int Ly, int SP, int g0, const T *rec,
+                                                const T *dgl, const double *rdl, T *xs)
+{
+    constexpr int D = 16;
+    static_assert(band_depth<T, UPPER, true>() == D, "full-block rounds are 16 steps");
+    constexpr bool RCP = sizeof(T) == 4;
+    constexpr int dir = UPPER ? -1 : 1;
+    using VT = typename BandVec<T>::type;
+    constexpr int NV = BandVec<T>::N;      // elements per 16-byte LDS access
+    constexpr int NB = D / NV;             // accesses per round
+    const int j = threadIdx.x, jj = min(j, Ly - 1);
+    const bool live = j < Ly;
+    const bool js = jj > 0, jn = jj < Ly - 1;
+    // line position of step 0 (i = slot - 2 j; U step t uses slot SP - 1 - t)
+    const int i0 = UPPER ? SP - 1 - 2 * jj : -2 * jj;
+    const int rb = jj * Lx + i0;
+    const T *rp = rec + (size_t)8 * (g0 + rb) + (UPPER ? 4 : 0);
+    using DT = typename FullStage<T, UPPER>::DT;
+    const DT *dp = RCP ? (const DT *)(rdl + (g0 + rb)) : (const DT *)(dgl + (g0 + rb));
+    VT *xl = reinterpret_cast<VT *>(xs + (size_t)jj * SP);
+    FullStage<T, UPPER> ring[D];
+    auto load = [&](const T *rps, const DT *dps, int q, FullStage<T, UPPER> &st) {
+        st.v = *reinterpret_cast<const BandRec<T> *>(rps + 8 * q);
+        if (UPPER)
+            st.d = dps[q];
+    };
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        load(rp, dp, dir * k, ring[k]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    const int rounds = SP / D;
+    // round r's slots: L [16 r, 16 r + 16), U [SP - 16 (r + 1), SP - 16 r)
+    auto slot0 = [&](int r) { return UPPER ? SP - D * (r + 1) : D * r; };
+    VT rin[NB], rnext[NB];
+#pragma unroll
+    for (int u = 0; u < NB; ++u)
+        rin[u] = xl[slot0(0) / NV + u];
+    T cur = T(0), h1 = T(0), h2 = T(0), h3 = T(0);
+    int r = 0;
+    do {
+        const int rn = min(r + 1, rounds - 1);
+#pragma unroll
+        for (int u = 0; u < NB; ++u)
+            rnext[u] = xl[slot0(rn) / NV + u];
+        const T *rps = rp + (ptrdiff_t)8 * dir * (D * r);
+        const DT *dps = dp + dir * (D * r);
+        const int is0 = i0 + dir * (D * r);
+        T outv[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            FullStage<T, UPPER> &st = ring[k];
+            const int i = is0 + dir * k;
+            const bool act = live && (unsigned)i < (unsigned)Lx;
+            // L: slot 16 r + k is element k of the round; U: slot
+            // SP - 1 - 16 r - k is element 15 - k
+            const int e = UPPER ? D - 1 - k : k;
+            const T rhs = reinterpret_cast<const T *>(&rin[e / NV])[e % NV];
+            h3 = h2;
+            h2 = h1;
+            h1 = UPPER ? dpp_mov<RS_DPP_WAVE_SHL1>(cur) : dpp_mov<RS_DPP_WAVE_SHR1>(cur);
+            const T o0 = UPPER ? cur : h3, o1 = UPPER ? h1 : h2;
+            const T o2 = UPPER ? h2 : h1, o3 = UPPER ? h3 : cur;
+            T sum = T(0);
+            if (NOPRED) {
+                sum = sum + st.v.v[0] * o0;
+                sum = sum + st.v.v[1] * o1;
+                sum = sum + st.v.v[2] * o2;
+                sum = sum + st.v.v[3] * o3;
+            } else {
+                const bool w = i > 0, ea = i < Lx - 1;
+                // L: (i-1,j-1), (i,j-1), (i+1,j-1), (i-1,j); U: (i+1,j),
+                // (i-1,j+1), (i,j+1), (i+1,j+1)
+                const bool p0 = UPPER ? ea : (w && js);
+                const bool p1 = UPPER ? (w && jn) : js;
+                const bool p2 = UPPER ? jn : (ea && js);
+                const bool p3 = UPPER ? (ea && jn) : w;
+                T q = sum + st.v.v[0] * o0;
+                sum = p0 ? q : sum;
+                q = sum + st.v.v[1] * o1;
+                sum = p1 ? q : sum;
+                q = sum + st.v.v[2] * o2;
+                sum = p2 ? q : sum;
+                q = sum + st.v.v[3] * o3;
+                sum = p3 ? q : sum;
+            }
+            T val = rhs - sum;
+            if (UPPER) {
+                if (RCP)
+                    val = (T)((double)val * (double)st.d);
+                else
+                    val = val / (T)st.d;
+            }
+            outv[e] = val;
+            cur = act ? val : cur;
+            load(rps, dps, dir * (k + D), st);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (live) {
+#pragma unroll
+            for (int u = 0; u < NB; ++u)
+                xl[slot0(r) / NV + u] = *reinterpret_cast<const VT *>(&outv[u * NV]);
+        }
+#pragma unroll
+        for (int u = 0; u < NB; ++u)
+            rin[u] = rnext[u];
+        ++r;
+    } while (r < rounds);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(64)
     k_ilu_apply_band(const int *__restrict__ bstart, const int2 *__restrict__ dims,
                      const unsigned char *__restrict__ mask, const T *__restrict__ rec,
-                     const T *__restrict__ dgl, const T *__restrict__ b, T *__restrict__ x,
-                     const int *skip1, const int *skip2, BlkSkip bskip)
+                     const T *__restrict__ dgl, const double *__restrict__ rdl,
+                     const T *__restrict__ b, T *__restrict__ x, const int *skip1,
+                     const int *skip2, BlkSkip bskip)
 {
     if ((skip1 && *skip1) || (skip2 && *skip2))
         return;
@@ -2047,9 +2219,55 @@ __global__ void __launch_bounds__(64)
         return;
     extern __shared__ unsigned char rs_smem[];
     const int b0 = bstart[blockIdx.x], nb = bstart[blockIdx.x + 1] - b0;
-    T *xv = (T *)rs_smem + RS_BAND_PAD;
     const int2 d = dims[blockIdx.x];   // Lx, Ly; Ly < 0: the block's pattern is not full
     const int Lx = d.x, Ly = d.y < 0 ? -d.y : d.y;
+    if (d.y > 0 && sizeof(T) == 4) {
+        // full FP32 block: slot layout (band_sweep_full; in FP64 its larger
+        // register set made it slower than the per-step form below)
+        T *xs = (T *)rs_smem;
+        const int SP = band_slot_pitch(Lx, Ly), ns = Ly * SP;
+        // slot s of line j holds row s - 2 j of the line: the padding
+        // slots [0, 2 j) and [2 j + Lx, SP) are zeroed, then the rows are
+        // copied line by line, sixteen lines per batch, loads in flight
+        // together (line indices past Ly clamp, only the stores are
+        // predicated)
+        for (int j = 0; j < Ly; ++j)
+            for (int p = threadIdx.x; p < SP - Lx; p += 64)
+                xs[j * SP + (p < 2 * j ? p : p + Lx)] = T(0);
+        for (int i = threadIdx.x; i < Lx; i += 64)
+            for (int j0 = 0; j0 < Ly; j0 += 16) {
+                T v[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u)
+                    v[u] = b[b0 + min(j0 + u, Ly - 1) * Lx + i];
+#pragma unroll
+                for (int u = 0; u < 16; ++u)
+                    if (j0 + u < Ly)
+                        xs[(j0 + u) * SP + i + 2 * (j0 + u)] = v[u];
+            }
+        __syncthreads();
+        band_sweep_full<T, false, true>(Lx, Ly, SP, b0, rec, dgl, rdl, xs);
+        __syncthreads();
+        band_sweep_full<T, true, true>(Lx, Ly, SP, b0, rec, dgl, rdl, xs);
+        __syncthreads();
+        {
+            int jl = 0, il = (int)threadIdx.x;
+            while (il >= Lx) {
+                il -= Lx;
+                ++jl;
+            }
+            for (int q = threadIdx.x; q < nb; q += 64) {
+                x[b0 + q] = xs[jl * SP + il + 2 * jl];
+                il += 64;
+                while (il >= Lx) {
+                    il -= Lx;
+                    ++jl;
+                }
+            }
+        }
+        return;
+    }
+    T *xv = (T *)rs_smem + RS_BAND_PAD;
     if (nb > 0)
         wg_copy(xv, b + b0, nb);
     for (int r = threadIdx.x; r < RS_BAND_PAD; r += 64) {
@@ -2078,7 +2296,9 @@ template <typename T>
 static void launch_band(rsgpu_prec *M, const T *rec, const T *b, T *x, const int *sk1,
                         const int *sk2)
 {
-    const size_t shm = (size_t)(M->max_block_rows + 2 * RS_BAND_PAD) * sizeof(T);
+    const size_t shm =
+        std::max((size_t)(M->max_block_rows + 2 * RS_BAND_PAD), (size_t)M->band_slots) *
+        sizeof(T);
     static bool attr = false;
     if (!attr && shm > 65536) {
         RS_HIP(hipFuncSetAttribute((const void *)k_ilu_apply_band<T>,
@@ -2088,10 +2308,11 @@ static void launch_band(rsgpu_prec *M, const T *rec, const T *b, T *x, const int
     }
     // padded layout: rows -PAD .. n+PAD-1 of 8 values, then the diagonals
     const size_t np = (size_t)M->n + 2 * RS_BAND_PAD;
+    const double *rdl = (sizeof(T) == 4 && M->band_rd.p) ? M->band_rd.p + RS_BAND_PAD : nullptr;
     hipLaunchKernelGGL(k_ilu_apply_band<T>, dim3(M->nblocks), dim3(64), shm, M->ctx->stream,
                        M->bstart.p, M->band_dims.p, M->band_mask.p + RS_BAND_PAD,
-                       rec + (size_t)8 * RS_BAND_PAD, rec + 8 * np + RS_BAND_PAD, b, x, sk1,
-                       sk2, M->skip);
+                       rec + (size_t)8 * RS_BAND_PAD, rec + 8 * np + RS_BAND_PAD, rdl, b, x,
+                       sk1, sk2, M->skip);
     RS_CHECK_LAUNCH();
 }
 
@@ -2104,7 +2325,8 @@ static void launch_apply(rsgpu_prec *M, const SweepData &A_, const T *va, const 
     if (M->n == 0)
         return;
     if (TRANS == RS_MODE_ILU && M->band && &A_ == &M->L &&
-        (size_t)(M->max_block_rows + 2 * RS_BAND_PAD) * sizeof(T) <=
+        std::max((size_t)(M->max_block_rows + 2 * RS_BAND_PAD), (size_t)M->band_slots) *
+                sizeof(T) <=
             (size_t)lds_limit(M->ctx->device)) {
         launch_band<T>(M, sizeof(T) == 8 ? (const T *)M->band64.p : (const T *)M->band32.p, b,
                        x, sk1, sk2);

@@ -288,6 +288,12 @@ struct rsgpu_prec {
     DevBuf<unsigned char> band_mask;
     DevBuf<double> band64;       // per row: L[4], U[4] (8 values), diagonals after
     DevBuf<float> band32;
+    // FP32: per row 1 / U_ii in FP64 (padded like the records), for the full
+    // blocks' U sweep
+    DevBuf<double> band_rd;
+    // elements of the largest full block's per-line slot layout (Ly lines of
+    // round_up(Lx + 2 Ly - 2, 16) slots)
+    int band_slots = 0;
     ~rsgpu_prec()
     {
         amg_destroy(amg);
