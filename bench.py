@@ -1116,6 +1116,7 @@ def probe(a, A, n, nnz, lib, mtj, _lib, ctx, stream, dev):
     else:
         M = mtj.BlockILU(a.nblocks, precision=_lib.FP64, ctx=ctx)
         g = mtj.GMRES(xin, 30)
+        g.cgs = not a.fp64_mgs
         g.setIterationMonitor(mtj.DefaultIterationMonitor(60, 1e-30, 1e-50, 1e5))
         g.setPreconditioner(M)
         M.setMatrix(A)
