@@ -162,7 +162,8 @@ enum rsgpu_prec_kind {
     RSGPU_PREC_DIAGONAL = 3,   /* mtj/sparse/DiagonalPreconditioner.java         */
     RSGPU_PREC_AMG = 4,        /* mtj/sparse/AMG.java (smoothed aggregation)     */
     RSGPU_PREC_SSOR = 5,       /* mtj/sparse/SSOR.java                           */
-    RSGPU_PREC_ICC = 6         /* mtj/sparse/ICC.java (FP64)                     */
+    RSGPU_PREC_ICC = 6,        /* mtj/sparse/ICC.java (FP64)                     */
+    RSGPU_PREC_ILUT = 7        /* mtj/sparse/ILUT.java (FP64)                    */
 };
 enum rsgpu_precision { RSGPU_FP64 = 0, RSGPU_FP32 = 1 };
 
@@ -237,6 +238,15 @@ int rsgpu_prec_set_amg_params(rsgpu_prec *M, const rsgpu_amg_params *p);
  * Takes effect at the next set_matrix. */
 int rsgpu_prec_set_ssor(rsgpu_prec *M, double omega_f, double omega_r,
                         int reverse, int block_rows, int ordering);
+/* ILUT(LU, tau, p) (ILUT.java:91-118; defaults tau = 1e-6, p = 25): drop
+ * tolerance tau (relative to each row's 2-norm) and the entries kept per
+ * side beyond A's own.  set_matrix factors on the device
+ * (ILUT.setMatrix/factor, ILUT.java:136-208; errors: 11 "Missing diagonal
+ * entry on row k", 10 "Zero diagonal entry on row k during ILU process");
+ * apply / trans_apply are ILUT.apply / transApply (:120-134).  Every
+ * factorisation starts from A's nonzero entries.  Takes effect at the next
+ * set_matrix. */
+int rsgpu_prec_set_ilut(rsgpu_prec *M, double tau, int p);
 /* Hierarchy introspection (tests): level sizes / nonzeros; returns the
  * level count m (the last level is solved by a dense LU). */
 int rsgpu_amg_levels(const rsgpu_prec *M, int *sizes, int *nnzs, int cap);

@@ -11,7 +11,8 @@
  * Reference files restated (paths relative to the reference root):
  *   mtj/src/no/uib/cipr/matrix/sparse/{CompRowMatrix,BiCGstab,GMRES,ILU,
  *     UnitLowerCompRowMatrix,UpperCompRowMatrix,DefaultIterationMonitor,
- *     AbstractIterationMonitor,IR,Arrays,AMG,SSOR,CompColMatrix}.java,
+ *     AbstractIterationMonitor,IR,Arrays,AMG,SSOR,CompColMatrix,ILUT,
+ *     SparseVector}.java,
  *   mtj/src/no/uib/cipr/matrix/DenseLU.java (amg.c)
  *   mtj/src/no/uib/cipr/matrix/{DenseVector,GivensRotation,Matrices,
  *     AbstractTriangDenseMatrix,DenseMatrix}.java
@@ -183,6 +184,12 @@ int or_qmr(int n, const int *rowptr, const int *colind, const double *data,
            const double *b, double *x, const or_opts *opts, or_stats *stats,
            double *hist, int hist_cap);
 int or_block_starts(int n, int nblocks, int *bstart);
+/* ILUT (ilut.c): the factor in CSR with diagonal positions; apply and
+ * transApply are or_ilu_apply / or_ilu_transapply on it (the same sweeps as
+ * ILUT.java:320-443).  or_ilut_bound: capacity for ocolind / oval. */
+int or_ilut_factor(int n, const int *rowptr, const int *colind, const double *val, double tau,
+                   int p, int *orowptr, int *ocolind, double *oval, int *odiag, int *bad_row);
+long long or_ilut_bound(int n, const int *rowptr, const int *colind, int p);
 /* All-host-cores GMRES(m) + block-Jacobi ILU(0) over opts->nblocks equal
  * row ranges (mtj_omp.c; the CPU baseline's multi-threaded leg): nthreads
  * OpenMP threads (0 = OpenMP's default). */

@@ -126,6 +126,10 @@ def lib():
         L.or_icc_apply.argtypes = [C.c_int, _i32p, _i32p, _f64p, _i32p, _f64p, _f64p, _f64p]
         L.or_icc_apply.restype = None
         L.or_dgesv.argtypes = [C.c_int, C.c_int, _f64p, C.c_int, _i32p, _f64p, C.c_int]
+        L.or_ilut_factor.argtypes = [C.c_int, _i32p, _i32p, _f64p, C.c_double, C.c_int, _i32p,
+                                     _i32p, _f64p, _i32p, C.c_void_p]
+        L.or_ilut_bound.argtypes = [C.c_int, _i32p, _i32p, C.c_int]
+        L.or_ilut_bound.restype = C.c_longlong
         L.or_amg_params_default.argtypes = [C.c_void_p]
         L.or_amg_setup.argtypes = [C.c_int, _i32p, _i32p, _f64p, C.c_void_p, C.c_void_p,
                                    C.c_void_p]
@@ -329,6 +333,23 @@ def icc_apply(rowptr, colind, R, diag, b):
     y = np.zeros(n)
     lib().or_icc_apply(n, _i32(rowptr), _i32(colind), _f64(R), _i32(diag), _f64(b), x, y)
     return x
+
+
+def ilut_factor(rowptr, colind, data, tau=1e-6, p=25):
+    """ILUT.setMatrix (ILUT.java:136-208): (status, rowptr, colind, values,
+    diag positions, bad_row 1-based) of the factor."""
+    n = len(rowptr) - 1
+    rp, ci = _i32(rowptr), _i32(colind)
+    cap = int(lib().or_ilut_bound(n, rp, ci, int(p)))
+    orp = np.zeros(n + 1, np.int32)
+    oci = np.zeros(max(cap, 1), np.int32)
+    ova = np.zeros(max(cap, 1))
+    dg = np.zeros(max(n, 1), np.int32)
+    bad = C.c_int(-1)
+    st = lib().or_ilut_factor(n, rp, ci, _f64(data), float(tau), int(p), orp, oci, ova, dg,
+                              C.byref(bad))
+    nnz = int(orp[n]) if st == 0 else 0
+    return st, orp, oci[:nnz].copy(), ova[:nnz].copy(), dg[:n].copy(), bad.value
 
 
 def block_starts(n, nblocks):

@@ -17,8 +17,8 @@ BREAKDOWN, DIVERGENCE, ITERATIONS = 1, 2, 3
 ZERO_PIVOT, MISSING_DIAGONAL, NOT_IN_PATTERN, BAD_ARGUMENT, SINGULAR = 10, 11, 12, 13, 14
 HIP_ERROR, OUT_OF_MEMORY, NOT_BUILT = -1, -2, -3
 
-PREC_IDENTITY, PREC_ILU0, PREC_BLOCK_ILU0, PREC_DIAGONAL, PREC_AMG, PREC_SSOR, PREC_ICC = (
-    0, 1, 2, 3, 4, 5, 6)
+(PREC_IDENTITY, PREC_ILU0, PREC_BLOCK_ILU0, PREC_DIAGONAL, PREC_AMG, PREC_SSOR, PREC_ICC,
+ PREC_ILUT) = (0, 1, 2, 3, 4, 5, 6, 7)
 FP64, FP32 = 0, 1
 BICGSTAB, GMRES, IR, CG, BICG, CGS, QMR = 0, 1, 2, 3, 4, 5, 6
 BATCH_NO_GRAPH, BATCH_NO_AUTOTUNE = 1, 2
@@ -126,6 +126,7 @@ SIGNATURES = {
     "rsgpu_amg_params_default": (None, [C.POINTER(AmgParams)]),
     "rsgpu_prec_set_amg_params": (C.c_int, [_h, C.POINTER(AmgParams)]),
     "rsgpu_prec_set_ssor": (C.c_int, [_h, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]),
+    "rsgpu_prec_set_ilut": (C.c_int, [_h, C.c_double, C.c_int]),
     "rsgpu_amg_levels": (C.c_int, [_h, _ip, _ip, C.c_int]),
     "rsgpu_amg_get_level": (C.c_int, [_h, C.c_int, _ip, _ip, _dp, _ip, _ip, _ip, _dp,
                                       C.POINTER(C.c_int)]),

@@ -644,15 +644,17 @@ extern "C" int rsgpu_prec_create(rsgpu_ctx *ctx, int kind, int nblocks,
         return RSGPU_BAD_ARGUMENT;
     if (kind != RSGPU_PREC_IDENTITY && kind != RSGPU_PREC_ILU0 &&
         kind != RSGPU_PREC_BLOCK_ILU0 && kind != RSGPU_PREC_AMG &&
-        kind != RSGPU_PREC_DIAGONAL && kind != RSGPU_PREC_SSOR && kind != RSGPU_PREC_ICC) {
+        kind != RSGPU_PREC_DIAGONAL && kind != RSGPU_PREC_SSOR && kind != RSGPU_PREC_ICC &&
+        kind != RSGPU_PREC_ILUT) {
         ctx->err = "preconditioner kind not available";
         return RSGPU_BAD_ARGUMENT;
     }
     if (precision != RSGPU_FP64 && precision != RSGPU_FP32)
         return RSGPU_BAD_ARGUMENT;
-    if ((kind == RSGPU_PREC_AMG || kind == RSGPU_PREC_SSOR || kind == RSGPU_PREC_ICC) &&
+    if ((kind == RSGPU_PREC_AMG || kind == RSGPU_PREC_SSOR || kind == RSGPU_PREC_ICC ||
+         kind == RSGPU_PREC_ILUT) &&
         precision != RSGPU_FP64) {
-        ctx->err = "AMG, SSOR and ICC run in FP64";
+        ctx->err = "AMG, SSOR, ICC and ILUT run in FP64";
         return RSGPU_BAD_ARGUMENT;
     }
     if (kind == RSGPU_PREC_BLOCK_ILU0 && nblocks < 1)
@@ -692,6 +694,16 @@ extern "C" int rsgpu_prec_set_ssor(rsgpu_prec *M, double omega_f, double omega_r
     M->amg_params.reverse = reverse ? 1 : 0;
     M->amg_params.block_rows = block_rows;
     M->amg_params.ordering = ordering;
+    M->ready = false;
+    return RSGPU_OK;
+}
+
+extern "C" int rsgpu_prec_set_ilut(rsgpu_prec *M, double tau, int p)
+{
+    if (!M || M->kind != RSGPU_PREC_ILUT)
+        return RSGPU_BAD_ARGUMENT;
+    M->ilut_tau = tau;
+    M->ilut_p = p;
     M->ready = false;
     return RSGPU_OK;
 }
@@ -789,6 +801,14 @@ extern "C" int rsgpu_prec_set_matrix(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
         M->ready = st == RSGPU_OK;
         M->nnz = A->nnz;
         return st;
+    }
+    if (M->kind == RSGPU_PREC_ILUT) {
+        try {
+            return ilut_set_matrix(M, A, bad_row);
+        } catch (const RsError &e) {
+            M->ready = false;
+            throw;
+        }
     }
     if (M->kind == RSGPU_PREC_ICC) {
         int st = RSGPU_OK;

@@ -478,6 +478,101 @@ static bool ilu_quirk_active(int n, const std::vector<int> &rp,
     return false;
 }
 
+// Sweeps, factor storage and scratch for the factor pattern (rp, ci) with
+// diagonal positions dg, over the blocks in M->h_bstart: everything of the
+// setup that follows from the factor's pattern (ILU(0): A's pattern or its
+// block-diagonal part; ILUT: the pattern its factorisation produced).
+void prec_setup_pattern(rsgpu_prec *M, const std::vector<int> &rp,
+                        const std::vector<int> &ci, const std::vector<int> &dg, int bs)
+{
+    hipStream_t s = M->ctx->stream;
+    const int n = M->n, nb = M->nblocks;
+    const bool icc = M->kind == RSGPU_PREC_ICC;
+    M->has_t = false;
+    // L (unit lower, forward), U (upper, backward) in operation order: the
+    // factor row's entries left and right of the diagonal
+    const int es = M->precision == RSGPU_FP64 ? 8 : 4;
+    // 3x3 block matrices with cell-aligned blocks: cell schedules, when the
+    // pipelined cell sweep applies (at most 64 cells per level, the block in
+    // LDS)
+    M->cell = false;
+    bool aligned = bs == 3 && !icc && getenv("RSGPU_ILU_CELLS") == nullptr &&
+                   ilu_kernel_choice() != 3;
+    for (int b = 0; aligned && b <= nb; ++b)
+        aligned = M->h_bstart[b] % 3 == 0;
+    int wb = 0;
+    if (aligned && cell_pattern(n, rp, ci, dg, wb) && wb <= 8) {
+        const int WB = wb <= 2 ? 2 : wb <= 4 ? wb : 8;   // the kernel's instances
+        HostSweep HL, HU;
+        build_sweep_cells(nb, M->h_bstart, rp, ci, dg, true, WB, es, HL);
+        build_sweep_cells(nb, M->h_bstart, rp, ci, dg, false, WB, es, HU);
+        int maxlev = 0;
+        for (int b = 0; b < nb; ++b)
+            maxlev = std::max(maxlev, (HL.blk_lev[b + 1] - HL.blk_lev[b]) +
+                                          (HU.blk_lev[b + 1] - HU.blk_lev[b]));
+        const size_t shm =
+            cell_xv_bytes(M->max_block_rows, es) + (size_t)maxlev * sizeof(LevelMeta);
+        // 32-bit byte offsets into the record arrays
+        const size_t lim = (size_t)INT32_MAX;
+        const bool addressable = HL.col.size() * 4 <= lim && HU.col.size() * 4 <= lim &&
+                                 HL.src.size() * es <= lim && HU.src.size() * es <= lim;
+        if (HL.max_count <= 64 && HU.max_count <= 64 && addressable &&
+            shm <= (size_t)lds_limit(M->ctx->device)) {
+            upload_sweep(HL, M->L, s, es);
+            upload_sweep(HU, M->U, s, es);
+            RS_HIP(hipStreamSynchronize(s));
+            M->cell = true;
+        }
+    }
+    M->cell_factor = false;
+    if (M->cell && getenv("RSGPU_ILU_CELL_FACTOR") == nullptr) {
+        std::vector<int> rec;
+        if (build_cell_records(n, rp, ci, dg, rec)) {
+            M->cell_rec.upload(rec, s);
+            RS_HIP(hipStreamSynchronize(s));
+            M->cell_factor = true;
+        }
+    }
+    if (!M->cell_factor)
+        M->cell_rec.release();
+    if (!M->cell) {
+        const RangeEntries eL{rp.data(), dg.data(), 0, 0, ci.data()};
+        const RangeEntries eU{dg.data(), rp.data() + 1, 1, 0, ci.data()};
+        const int W = pad_width(std::max(max_entries(eL, n), max_entries(eU, n)));
+        HostSweep H;
+        build_sweep(nb, M->h_bstart, eL, true, W, H);
+        upload_sweep(H, M->L, s, es);
+        build_sweep(nb, M->h_bstart, eU, false, W, H);
+        upload_sweep(H, M->U, s, es);
+        RS_HIP(hipStreamSynchronize(s));   // H's host arrays are released next
+    }
+    M->status.alloc(2);
+    if (M->precision == RSGPU_FP64) {
+        M->lu64.alloc(std::max(M->nnz, 1));
+        M->ell64_L.alloc(M->L.ell_size);
+        M->ell64_U.alloc(M->U.ell_size);
+        if (!M->cell)
+            M->dgU64.alloc(M->U.nrows_lev);
+    } else {
+        M->lu32.alloc(std::max(M->nnz, 1));
+        M->ell32_L.alloc(M->L.ell_size);
+        M->ell32_U.alloc(M->U.ell_size);
+        if (!M->cell)
+            M->dgU32.alloc(M->U.nrows_lev);
+    }
+    band_setup(M, rp, ci, dg);
+    const int elem = M->precision == RSGPU_FP64 ? 8 : 4;
+    M->lds_rows = RS_LDS_BYTES / elem - 1;
+    if (M->max_block_rows > M->lds_rows) {
+        if (M->precision == RSGPU_FP64)
+            M->scratch64.alloc((size_t)n + nb);
+        else
+            M->scratch32.alloc((size_t)n + nb);
+    }
+    RS_HIP(hipStreamSynchronize(s));
+    M->ready = true;
+}
+
 void prec_setup(rsgpu_prec *M, rsgpu_csr *A)
 {
     hipStream_t s = M->ctx->stream;
@@ -559,88 +654,7 @@ void prec_setup(rsgpu_prec *M, rsgpu_csr *A)
     if (!ident)
         M->src.upload(src, s);
 
-    // L (unit lower, forward), U (upper, backward) in operation order: the
-    // factor row's entries left and right of the diagonal
-    const int es = M->precision == RSGPU_FP64 ? 8 : 4;
-    // 3x3 block matrices with cell-aligned blocks: cell schedules, when the
-    // pipelined cell sweep applies (at most 64 cells per level, the block in
-    // LDS)
-    M->cell = false;
-    bool aligned = A->bs == 3 && !icc && getenv("RSGPU_ILU_CELLS") == nullptr &&
-                   ilu_kernel_choice() != 3;
-    for (int b = 0; aligned && b <= nb; ++b)
-        aligned = M->h_bstart[b] % 3 == 0;
-    int wb = 0;
-    if (aligned && cell_pattern(n, rp, ci, dg, wb) && wb <= 8) {
-        const int WB = wb <= 2 ? 2 : wb <= 4 ? wb : 8;   // the kernel's instances
-        HostSweep HL, HU;
-        build_sweep_cells(nb, M->h_bstart, rp, ci, dg, true, WB, es, HL);
-        build_sweep_cells(nb, M->h_bstart, rp, ci, dg, false, WB, es, HU);
-        int maxlev = 0;
-        for (int b = 0; b < nb; ++b)
-            maxlev = std::max(maxlev, (HL.blk_lev[b + 1] - HL.blk_lev[b]) +
-                                          (HU.blk_lev[b + 1] - HU.blk_lev[b]));
-        const size_t shm =
-            cell_xv_bytes(M->max_block_rows, es) + (size_t)maxlev * sizeof(LevelMeta);
-        // 32-bit byte offsets into the record arrays
-        const size_t lim = (size_t)INT32_MAX;
-        const bool addressable = HL.col.size() * 4 <= lim && HU.col.size() * 4 <= lim &&
-                                 HL.src.size() * es <= lim && HU.src.size() * es <= lim;
-        if (HL.max_count <= 64 && HU.max_count <= 64 && addressable &&
-            shm <= (size_t)lds_limit(M->ctx->device)) {
-            upload_sweep(HL, M->L, s, es);
-            upload_sweep(HU, M->U, s, es);
-            RS_HIP(hipStreamSynchronize(s));
-            M->cell = true;
-        }
-    }
-    M->cell_factor = false;
-    if (M->cell && getenv("RSGPU_ILU_CELL_FACTOR") == nullptr) {
-        std::vector<int> rec;
-        if (build_cell_records(n, rp, ci, dg, rec)) {
-            M->cell_rec.upload(rec, s);
-            RS_HIP(hipStreamSynchronize(s));
-            M->cell_factor = true;
-        }
-    }
-    if (!M->cell_factor)
-        M->cell_rec.release();
-    if (!M->cell) {
-        const RangeEntries eL{rp.data(), dg.data(), 0, 0, ci.data()};
-        const RangeEntries eU{dg.data(), rp.data() + 1, 1, 0, ci.data()};
-        const int W = pad_width(std::max(max_entries(eL, n), max_entries(eU, n)));
-        HostSweep H;
-        build_sweep(nb, M->h_bstart, eL, true, W, H);
-        upload_sweep(H, M->L, s, es);
-        build_sweep(nb, M->h_bstart, eU, false, W, H);
-        upload_sweep(H, M->U, s, es);
-        RS_HIP(hipStreamSynchronize(s));   // H's host arrays are released next
-    }
-    M->status.alloc(2);
-    if (M->precision == RSGPU_FP64) {
-        M->lu64.alloc(std::max(M->nnz, 1));
-        M->ell64_L.alloc(M->L.ell_size);
-        M->ell64_U.alloc(M->U.ell_size);
-        if (!M->cell)
-            M->dgU64.alloc(M->U.nrows_lev);
-    } else {
-        M->lu32.alloc(std::max(M->nnz, 1));
-        M->ell32_L.alloc(M->L.ell_size);
-        M->ell32_U.alloc(M->U.ell_size);
-        if (!M->cell)
-            M->dgU32.alloc(M->U.nrows_lev);
-    }
-    band_setup(M, rp, ci, dg);
-    const int elem = M->precision == RSGPU_FP64 ? 8 : 4;
-    M->lds_rows = RS_LDS_BYTES / elem - 1;
-    if (M->max_block_rows > M->lds_rows) {
-        if (M->precision == RSGPU_FP64)
-            M->scratch64.alloc((size_t)n + nb);
-        else
-            M->scratch32.alloc((size_t)n + nb);
-    }
-    RS_HIP(hipStreamSynchronize(s));
-    M->ready = true;
+    prec_setup_pattern(M, rp, ci, dg, A->bs);
 }
 
 // Line-wavefront form of a block [b0, b1): Ly lines of Lx rows in natural
@@ -1809,6 +1823,23 @@ int prec_factor(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
         return st[0];
     }
     return RSGPU_OK;
+}
+
+// FP64 sweep arrays from the factor values in M->lu64 (the gathers that
+// follow a factorisation), for a factor computed outside prec_factor.
+void prec_refresh_fp64(rsgpu_prec *M)
+{
+    hipStream_t s = M->ctx->stream;
+    launch_ell<double>(s, M->L, M->lu64.p, M->ell64_L.p, true);
+    launch_ell<double>(s, M->U, M->lu64.p, M->ell64_U.p, true);
+    if (!M->cell)
+        launch_rowdiag<double>(s, M->U, M->diag.p, M->lu64.p, M->dgU64.p);
+    launch_band_gather<double>(M, M->lu64.p, M->band64.p);
+    if (M->has_t) {
+        launch_ell<double>(s, M->Ut, M->lu64.p, M->ell64_Ut.p, false);
+        launch_ell<double>(s, M->Lt, M->lu64.p, M->ell64_Lt.p, false);
+        launch_rowdiag<double>(s, M->Ut, M->diag.p, M->lu64.p, M->dgUt64.p);
+    }
 }
 
 template <typename T>

@@ -214,6 +214,8 @@ struct AmgData;
 void amg_destroy(AmgData *d);
 struct IccData;
 void icc_destroy(IccData *d);
+struct IlutData;
+void ilut_destroy(IlutData *d);
 
 // Block b of a block-Jacobi sweep is skipped while
 // done[(blk_sys ? blk_sys[b] : b) * stride] != 0 (the done flag of the
@@ -277,6 +279,11 @@ struct rsgpu_prec {
     // batched solve enqueues its applies
     BlkSkip skip{};
     IccData *icc = nullptr;     // ICC: fill-in reach lists and factor levels
+    // ILUT: drop tolerance and extra entries per side (ILUT.java:91-118
+    // defaults), the device factorisation's buffers
+    double ilut_tau = 1e-6;
+    int ilut_p = 25;
+    IlutData *ilut = nullptr;
     // line-wavefront sweeps (every block a grid of Ly <= 64 lines of Lx rows
     // coupled only among 9-point neighbours): per block (Lx, Ly), per row the
     // factor positions of the 4 L slots, the diagonal and the 4 U slots (-1
@@ -298,6 +305,7 @@ struct rsgpu_prec {
     {
         amg_destroy(amg);
         icc_destroy(icc);
+        ilut_destroy(ilut);
     }
 };
 
@@ -566,6 +574,10 @@ void convert_f32_to_f64(hipStream_t s, const float *a, double *b, size_t n);
 
 // ilu.hip
 void prec_setup(rsgpu_prec *M, rsgpu_csr *A);
+void prec_setup_pattern(rsgpu_prec *M, const std::vector<int> &rp, const std::vector<int> &ci,
+                        const std::vector<int> &dg, int bs);
+void prec_refresh_fp64(rsgpu_prec *M);
+int ilut_set_matrix(rsgpu_prec *M, rsgpu_csr *A, int *bad_row);
 int prec_factor(rsgpu_prec *M, rsgpu_csr *A, int *bad_row);
 int icc_set_matrix(rsgpu_prec *M, rsgpu_csr *A, int *bad_row);
 void prec_apply(rsgpu_prec *M, const double *b, double *x, const DevState *st);

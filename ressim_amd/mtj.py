@@ -514,6 +514,23 @@ class ICC(Preconditioner):
         super().__init__(1, L.FP64, ctx or (R.ctx if R is not None else None))
 
 
+class ILUT(Preconditioner):
+    """mtj/.../sparse/ILUT.java: ILU with the dual-threshold dropping of
+    Saad, ``ILUT(LU, tau, p)`` (defaults tau = 1e-6, p = 25, ILUT.java:116-118),
+    factored on the device at setMatrix; FP64.  ``ILUT(LU)`` accepts the
+    matrix the reference passes as its work storage; only its shape is
+    checked."""
+    kind = L.PREC_ILUT
+
+    def __init__(self, LU: Optional[CompRowMatrix] = None, tau: float = 1e-6, p: int = 25,
+                 ctx: Optional[Context] = None):
+        if LU is not None and not LU.isSquare():
+            raise ValueError("ILU only applies to square matrices")
+        super().__init__(1, L.FP64, ctx or (LU.ctx if LU is not None else None))
+        self.tau, self.p = float(tau), int(p)
+        _raise(L.load().rsgpu_prec_set_ilut(self.h, self.tau, self.p), self.ctx.h)
+
+
 class SSOR(Preconditioner):
     """mtj/.../sparse/SSOR.java: SSOR(F, reverse, omegaF, omegaR); ``SSOR()``
     is SSOR(F) = reverse sweep, omega 1.  apply(b, x) starts from x's old
