@@ -2821,6 +2821,8 @@ void prec_apply(rsgpu_prec *M, const double *b, double *x, const DevState *st)
             icc_apply(M, b, x, sk1, sk2);
         return;
     }
+    if (M->kind == RSGPU_PREC_ILUT && ilut_apply(M, b, x, false, sk1, sk2))
+        return;
     if (M->precision == RSGPU_FP64) {
         if (M->cell)
             launch_apply_cells<double>(M, M->ell64_L.p, M->ell64_U.p, b, x, sk1, sk2);
@@ -2893,6 +2895,8 @@ void prec_trans_apply(rsgpu_prec *M, const double *b, double *x, const DevState 
     const int *sk2 = st ? &st->cycle_end : nullptr;
     if (M->precision != RSGPU_FP64)
         throw RsError{RSGPU_BAD_ARGUMENT, "transApply needs an FP64 preconditioner"};
+    if (M->kind == RSGPU_PREC_ILUT && ilut_apply(M, b, x, true, sk1, sk2))
+        return;
     if (!M->has_t) {
         prec_setup_transpose(M);
         launch_ell<double>(s, M->Ut, M->lu64.p, M->ell64_Ut.p, false);

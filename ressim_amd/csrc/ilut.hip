@@ -51,6 +51,15 @@
 
 enum { ILUT_ST_MISSING = 0, ILUT_ST_ZERO = 1, ILUT_ST_OVERFLOW = 2, ILUT_ST_TIMEOUT = 3 };
 
+// Largest dynamic LDS a workgroup may request on this device (opt-in).
+static size_t ilut_lds_limit(int device)
+{
+    int a = 0, b = 0;
+    (void)hipDeviceGetAttribute(&a, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
+    (void)hipDeviceGetAttribute(&b, hipDeviceAttributeSharedMemPerBlockOptin, device);
+    return (size_t)std::max(a, b);
+}
+
 namespace {
 
 struct IlutArgs {
@@ -420,9 +429,11 @@ __global__ void __launch_bounds__(64) k_ilut_factor(IlutArgs a)
         atomicOr(&a.status[ILUT_ST_OVERFLOW], 1);
 }
 
-// Factor rows from their slots into CSR (rowptr given); diagonal positions.
+// Factor rows from their slots into CSR (rowptr given); diagonal positions
+// and the sweeps' per-row ranges (L: [rp, dg), U: [dg + 1, rp + 1)).
 __global__ void k_ilut_compact(int n, const int *off, const int2 *info, const int *ocol,
-                               const double *oval, const int *rp, int *ci, double *val, int *dg)
+                               const double *oval, const int *rp, int *ci, double *val, int *dg,
+                               int2 *bl, int2 *bu)
 {
     const int r = blockIdx.x;
     if (r >= n)
@@ -432,8 +443,175 @@ __global__ void k_ilut_compact(int n, const int *off, const int2 *info, const in
         ci[dst + e] = ocol[src + e];
         val[dst + e] = oval[src + e];
     }
-    if (threadIdx.x == 0)
+    if (threadIdx.x == 0) {
         dg[r] = dst + info[r].x;
+        bl[r] = make_int2(dst, dst + info[r].x);
+        bu[r] = make_int2(dst + info[r].x + 1, dst + len);
+    }
+}
+
+// ---------------------------------------------------------------- sweeps
+// ILUT's fill makes long dependency chains (level counts near n / 2 on grid
+// matrices), so its solves run as one wave walking the rows in order with
+// the vector in LDS.  Gather rows (L.solve, U.solve): lane t forms the
+// product of entry t, and the products are added to sum = 0 in entry
+// (column) order by readlane -- the statement order of
+// UnitLowerFlexCompRowMatrix.solve / UpperFlexCompRowMatrix.solve
+// (ILUT.java:321-344, 390-413); lanes past the row's end add -0.0, an exact
+// identity.  Scatter rows (transSolve, :347-370, 417-443): x[i] is final
+// when row i is reached; lanes subtract v * x[i] from their distinct
+// columns.  A ring of SEQ_D rows of operands (the first 64 entries of each
+// row and its right-hand side or diagonal) is in flight ahead of the row
+// being solved, and the row ranges SEQ_D rows further ahead.
+#define SEQ_D 8
+
+__device__ __forceinline__ double readlane_f64(double v, int l)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+
+// s + p(lane 0) + p(lane 1) + ... + p(lane m - 1), left to right
+__device__ __forceinline__ double lane_serial_sum(double s, double p, int m)
+{
+    for (int u0 = 0; u0 < m; u0 += 8) {
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            s = s + readlane_f64(p, u0 + k);
+    }
+    return s;
+}
+
+struct SeqSlot {
+    int2 b;        // the row's entry range
+    int c;         // this lane's first-chunk column
+    double v;      // and value
+    double aux;    // L: b[i]; U, U': the diagonal
+};
+
+// MODE 0: L solve ascending (x[i] = rhs[i] - sum), 1: U solve descending
+// (x[i] = (x[i] - sum) / U(i,i)), 2: U' ascending (x[i] /= U(i,i), then
+// scatter), 3: L' descending (scatter).  X: the LDS vector, in place, with
+// SEQ_D junk rows past n: the steps past the last row run on empty ranges
+// (1, 1) and write there, so every step is the same straight-line code and
+// the compiler's in-order load counting keeps the ring in flight.  Every row
+// holds at most 64 entries on the swept side (checked by the caller).
+template <int MODE>
+__device__ void seq_sweep(int n, const int2 *__restrict__ bnd, const int *__restrict__ ci,
+                          const double *__restrict__ lu, const double *__restrict__ rhs,
+                          double *X)
+{
+    constexpr bool fwd = MODE == 0 || MODE == 2;
+    constexpr bool gather = MODE == 0 || MODE == 1;
+    const int lane = threadIdx.x;
+    auto row_of = [&](int s) { return s < n ? (fwd ? s : n - 1 - s) : s; };
+    // per-row data of the SEQ_D steps from base, lane l holding step base + l
+    // (vector loads: uniform addresses would become scalar loads, whose
+    // counter the LDS waits share).  bnd is padded by 3 SEQ_D empty ranges
+    // (1, 1) on both sides, so steps past the last row read padding.
+    auto block_bounds = [&](int base) {
+        const int st = base + (lane & (SEQ_D - 1));
+        return bnd[fwd ? st : n - 1 - st];
+    };
+    auto block_aux = [&](int base, int2 b) {
+        if (MODE == 0)
+            return rhs[row_of(min(base + (lane & (SEQ_D - 1)), n - 1))];
+        if (MODE == 3)
+            return 0.0;
+        return lu[b.x - 1];
+    };
+    auto load_slot = [&](SeqSlot &q, int2 bv, double av, int k) {
+        q.b.x = __builtin_amdgcn_readlane(bv.x, k);
+        q.b.y = __builtin_amdgcn_readlane(bv.y, k);
+        q.aux = readlane_f64(av, k);
+        const int t = min(q.b.x + lane, q.b.y - 1);
+        const bool in = q.b.x + lane < q.b.y;
+        const int c = ci[max(t, 0)];
+        const double v = lu[max(t, 0)];
+        q.c = in ? c : 0;
+        q.v = in ? v : 0.0;
+    };
+    // loop entry as after an iteration: (B1, A1) and B2 in flight, then the
+    // ring's loads last
+    SeqSlot ring[SEQ_D];
+    const int2 B0 = block_bounds(0);
+    const double A0 = block_aux(0, B0);
+    int2 B1 = block_bounds(SEQ_D);
+    double A1 = block_aux(SEQ_D, B1);
+    int2 B2 = block_bounds(2 * SEQ_D);
+#pragma unroll
+    for (int k = 0; k < SEQ_D; ++k)
+        load_slot(ring[k], B0, A0, k);
+    int s0 = 0;
+    do {
+        // ring: steps s0.. ; (B1, A1): steps s0 + D..; B2: steps s0 + 2D..
+        const double A2 = block_aux(s0 + 2 * SEQ_D, B2);
+        const int2 B3 = block_bounds(s0 + 3 * SEQ_D);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int k = 0; k < SEQ_D; ++k) {
+            const int s = s0 + k;
+            const int i = row_of(s);
+            const SeqSlot q = ring[k];
+            const int j0 = q.b.x, j1 = q.b.y;
+            if (gather) {
+                const double p = lane < j1 - j0 ? q.v * X[q.c] : -0.0;
+                const double sum = lane_serial_sum(0.0, p, j1 - j0);
+                const double xi = MODE == 0 ? q.aux - sum : (X[i] - sum) / q.aux;
+                if (lane == 0)
+                    X[i] = xi;
+            } else {
+                double xi = X[i];
+                if (MODE == 2) {
+                    xi = xi / q.aux;
+                    if (lane == 0)
+                        X[i] = xi;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                if (j0 + lane < j1)
+                    X[q.c] = X[q.c] - q.v * xi;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            // refill with step s + D
+            load_slot(ring[k], B1, A1, k);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        B1 = B2;
+        A1 = A2;
+        B2 = B3;
+        s0 += SEQ_D;
+    } while (s0 < n);
+}
+
+__device__ __forceinline__ bool seq_skip(const int *sk1, const int *sk2)
+{
+    return (sk1 && *sk1) || (sk2 && *sk2);
+}
+
+// apply: L then U; transApply: U' then L'.  One wave, x in LDS.
+template <bool TRANS>
+__global__ void __launch_bounds__(64) k_ilut_seq(int n, const int2 *bl, const int2 *bu,
+                                                 const int *ci, const double *lu,
+                                                 const double *b, double *x, const int *sk1,
+                                                 const int *sk2)
+{
+    if (seq_skip(sk1, sk2))
+        return;
+    extern __shared__ double X[];
+    if (TRANS) {
+        for (int r = threadIdx.x; r < n; r += 64)
+            X[r] = b[r];
+        __syncthreads();
+        seq_sweep<2>(n, bu, ci, lu, nullptr, X);
+        seq_sweep<3>(n, bl, ci, lu, nullptr, X);
+    } else {
+        seq_sweep<0>(n, bl, ci, lu, b, X);
+        seq_sweep<1>(n, bu, ci, lu, nullptr, X);
+    }
+    __syncthreads();
+    for (int r = threadIdx.x; r < n; r += 64)
+        x[r] = X[r];
 }
 
 } // namespace
@@ -443,6 +621,8 @@ struct IlutData {
     int p = INT_MIN;
     std::vector<int> h_off;
     DevBuf<int> off, ocol, flag, ctl, rp, ci, dg;
+    DevBuf<int2> bl, bu;     // per row: L and U entry ranges of the factor
+    bool seq = false;        // solves by the one-wave sequential sweeps
     DevBuf<double> oval, val;
     DevBuf<int2> info;
     DevBuf<double> zb;
@@ -578,9 +758,17 @@ int ilut_set_matrix(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
     D.ci.alloc(std::max(1, nnz));
     D.val.alloc(std::max(1, nnz));
     D.dg.alloc(std::max(1, n));
+    // sweep ranges padded by 3 SEQ_D empty ranges (1, 1) on both sides
+    const size_t pad = 3 * SEQ_D;
+    if (D.bl.n != (size_t)n + 2 * pad) {
+        std::vector<int2> e((size_t)n + 2 * pad, make_int2(1, 1));
+        D.bl.upload(e, s);
+        D.bu.upload(e, s);
+    }
     if (n)
         hipLaunchKernelGGL(k_ilut_compact, dim3(n), dim3(64), 0, s, n, D.off.p, D.info.p,
-                           D.ocol.p, D.oval.p, D.rp.p, D.ci.p, D.val.p, D.dg.p);
+                           D.ocol.p, D.oval.p, D.rp.p, D.ci.p, D.val.p, D.dg.p, D.bl.p + pad,
+                           D.bu.p + pad);
     RS_CHECK_LAUNCH();
     std::vector<int> ci(nnz);
     if (nnz)
@@ -612,7 +800,37 @@ int ilut_set_matrix(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
                               hipMemcpyDeviceToDevice, s));
     prec_refresh_fp64(M);
     RS_HIP(hipStreamSynchronize(s));
+    // the sequential sweeps: the vector in LDS, at most 64 entries per row
+    // and side (one lane per entry)
+    int side = 0;
+    for (int r = 0; r < n; ++r)
+        side = std::max({side, info[r].x, info[r].y - info[r].x - 1});
+    D.seq = n > 0 && side <= 64 &&
+            ((size_t)n + SEQ_D) * sizeof(double) <= ilut_lds_limit(M->ctx->device) &&
+            getenv("RSGPU_ILUT_LEVELS") == nullptr;
     M->pattern_id = A->pattern_id;
     M->ready = true;
     return RSGPU_OK;
+}
+
+// ILUT.apply / transApply by the sequential sweeps; false when they do not
+// apply (the vector exceeds LDS): the level-scheduled ILU sweeps run.
+bool ilut_apply(rsgpu_prec *M, const double *b, double *x, bool trans, const int *sk1,
+                const int *sk2)
+{
+    IlutData *D = M->ilut;
+    if (!D || !D->seq)
+        return false;
+    const size_t shm = ((size_t)M->n + SEQ_D) * sizeof(double);
+    auto k = trans ? k_ilut_seq<true> : k_ilut_seq<false>;
+    static size_t granted[2] = {65536, 65536};
+    if (shm > granted[trans]) {
+        RS_HIP(hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)shm));
+        granted[trans] = shm;
+    }
+    hipLaunchKernelGGL(k, dim3(1), dim3(64), shm, M->ctx->stream, M->n, D->bl.p + 3 * SEQ_D,
+                       D->bu.p + 3 * SEQ_D, M->colind.p, M->lu64.p, b, x, sk1, sk2);
+    RS_CHECK_LAUNCH();
+    return true;
 }

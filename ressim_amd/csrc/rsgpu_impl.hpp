@@ -578,6 +578,8 @@ void prec_setup_pattern(rsgpu_prec *M, const std::vector<int> &rp, const std::ve
                         const std::vector<int> &dg, int bs);
 void prec_refresh_fp64(rsgpu_prec *M);
 int ilut_set_matrix(rsgpu_prec *M, rsgpu_csr *A, int *bad_row);
+bool ilut_apply(rsgpu_prec *M, const double *b, double *x, bool trans, const int *sk1,
+                const int *sk2);
 int prec_factor(rsgpu_prec *M, rsgpu_csr *A, int *bad_row);
 int icc_set_matrix(rsgpu_prec *M, rsgpu_csr *A, int *bad_row);
 void prec_apply(rsgpu_prec *M, const double *b, double *x, const DevState *st);

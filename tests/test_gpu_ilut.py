@@ -175,3 +175,12 @@ def test_larger_default_parameters(gpu, oracle):
     print(f"ILUT lap2d 128x128 n={len(rp) - 1}", flush=True)
     A, M, fac = _check_factor(gpu, oracle, rp, ci, d, 1e-6, 25)
     _check_apply(M, oracle, fac)
+
+
+def test_level_scheduled_sweeps(gpu, oracle, monkeypatch):
+    """Past the LDS capacity of the sequential sweeps the factor is applied
+    by the level-scheduled ILU sweeps (forced here): bit-exact too."""
+    monkeypatch.setenv("RSGPU_ILUT_LEVELS", "1")
+    rp, ci, d = _lap2d(24, 3)
+    A, M, fac = _check_factor(gpu, oracle, rp, ci, d, 1e-4, 8)
+    _check_apply(M, oracle, fac, seed=8)
