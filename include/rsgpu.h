@@ -217,7 +217,12 @@ int rsgpu_prec_sweep_form(const rsgpu_prec *M, int *form);
  * one parallel step per colour), zero_guess (0: the cycle starts from x's
  * old content as AMG.apply does, AMG.java:208-216; 1: from zero, a fixed
  * linear preconditioner), max_coarse (largest dense coarsest level
- * accepted).  block_rows = 0 with ordering = 0 is SSOR.java exactly. */
+ * accepted), coarsen_ratio (below).  block_rows = 0 with ordering = 0 is
+ * SSOR.java exactly.  An FP32 AMG (rsgpu_prec_create precision FP32) builds
+ * the hierarchy in FP64 and runs the cycle -- smoothers, residuals and
+ * transfers -- in FP32 with the coarsest solve in FP64; coarsest levels
+ * above 256 rows are solved with their explicit inverse (from the LU at
+ * setup). */
 typedef struct {
     double omega_pre_f, omega_pre_r, omega_post_f, omega_post_r;
     int nu1, nu2, gamma, min;
@@ -227,6 +232,10 @@ typedef struct {
     int zero_guess;
     int max_coarse;
     int ordering;
+    /* no MTJ counterpart: coarsening stops at a level whose aggregates keep
+     * more than this fraction of its rows (0: never, AMG.java's rule of
+     * coarsening down to `min` rows) */
+    double coarsen_ratio;
 } rsgpu_amg_params;
 void rsgpu_amg_params_default(rsgpu_amg_params *p);
 /* Takes effect at the next rsgpu_prec_set_matrix (AMG.setMatrix,

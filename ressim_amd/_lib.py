@@ -37,7 +37,8 @@ class AmgParams(C.Structure):
                 ("omega_post_f", C.c_double), ("omega_post_r", C.c_double),
                 ("nu1", C.c_int), ("nu2", C.c_int), ("gamma", C.c_int), ("min", C.c_int),
                 ("omega", C.c_double), ("reverse", C.c_int), ("block_rows", C.c_int),
-                ("zero_guess", C.c_int), ("max_coarse", C.c_int), ("ordering", C.c_int)]
+                ("zero_guess", C.c_int), ("max_coarse", C.c_int), ("ordering", C.c_int),
+                ("coarsen_ratio", C.c_double)]
 
 
 class Stats(C.Structure):

@@ -651,10 +651,9 @@ extern "C" int rsgpu_prec_create(rsgpu_ctx *ctx, int kind, int nblocks,
     }
     if (precision != RSGPU_FP64 && precision != RSGPU_FP32)
         return RSGPU_BAD_ARGUMENT;
-    if ((kind == RSGPU_PREC_AMG || kind == RSGPU_PREC_SSOR || kind == RSGPU_PREC_ICC ||
-         kind == RSGPU_PREC_ILUT) &&
+    if ((kind == RSGPU_PREC_SSOR || kind == RSGPU_PREC_ICC || kind == RSGPU_PREC_ILUT) &&
         precision != RSGPU_FP64) {
-        ctx->err = "AMG, SSOR, ICC and ILUT run in FP64";
+        ctx->err = "SSOR, ICC and ILUT run in FP64";
         return RSGPU_BAD_ARGUMENT;
     }
     if (kind == RSGPU_PREC_BLOCK_ILU0 && nblocks < 1)
@@ -735,7 +734,8 @@ extern "C" int rsgpu_prec_set_amg_params(rsgpu_prec *M, const rsgpu_amg_params *
     if (p->omega_pre_f < 0 || p->omega_pre_f > 2 || p->omega_pre_r < 0 || p->omega_pre_r > 2 ||
         p->omega_post_f < 0 || p->omega_post_f > 2 || p->omega_post_r < 0 ||
         p->omega_post_r > 2 || p->nu1 < 0 || p->nu2 < 0 || p->gamma < 1 || p->min < 0 ||
-        p->block_rows < 0 || p->max_coarse < 1 || p->ordering < 0 || p->ordering > 1) {
+        p->block_rows < 0 || p->max_coarse < 1 || p->ordering < 0 || p->ordering > 1 ||
+        !(p->coarsen_ratio >= 0 && p->coarsen_ratio <= 1)) {
         M->ctx->err = "AMG parameter out of range";
         return RSGPU_BAD_ARGUMENT;
     }

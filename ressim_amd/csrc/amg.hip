@@ -45,6 +45,7 @@
 #include <algorithm>
 #include <cmath>
 #include <memory>
+#include <type_traits>
 
 #include "spmv_core.hpp"
 
@@ -55,8 +56,10 @@ struct Skip {
     __device__ __forceinline__ bool on() const { return (a && *a) || (b && *b); }
 };
 
+// One level (colour) of a smoother sweep: its rows, their first group and
+// their first row slot.
 struct AmgLevelMeta {
-    int count, off, roff, pad;
+    int count, gbase, roff, pad;
 };
 
 // one level of the hierarchy
@@ -75,9 +78,11 @@ struct AmgLevel {
     DevBuf<int> Prp, Pci, Trp, Tci;
     DevBuf<double> Pv, Tv;
     SpmvPlan Pplan, Tplan;
-    // smoother
-    int nb = 1, W = 4, max_block_rows = 0, nlev_max = 0;
-    DevBuf<int> bstart, blk_lev, meta, rows, row_glob, ell_col, ell_src;
+    // smoother, SELL-64: per level (colour) the rows sorted by descending
+    // off-diagonal count in groups of 64; entry s of lane t of group g at
+    // gofs[g] + 64 s + t, in the row's CSR order
+    int nb = 1, max_block_rows = 0, nlev_max = 0;
+    DevBuf<int> bstart, blk_lev, meta, rows, rlen, row_glob, gofs, ell_col, ell_src;
     DevBuf<double> ell_val, dgl;
     size_t ell_size = 0;
     int nrows_lev = 0;
@@ -87,6 +92,8 @@ struct AmgLevel {
     std::vector<AmgLevelMeta> h_meta;   // host copy (launch sizes)
     // vectors
     DevBuf<double> u, ualt, f, r;
+    // FP32 cycle: the level's matrix, transfer and smoother values and vectors
+    DevBuf<float> v32, Pv32, Tv32, ell_val32, dgl32, u32, ualt32, f32, r32;
 };
 
 struct AmgData {
@@ -96,6 +103,10 @@ struct AmgData {
     int nc = 0;
     DevBuf<double> lu;
     DevBuf<int> ipiv, flag;
+    DevBuf<double> cinv;       // coarsest inverse (column-major), above 256 rows
+    DevBuf<double> cpart;      // its column slices' partial products
+    DevBuf<double> cf, cu;     // FP64 coarsest vectors of the FP32 cycle
+    bool fp32 = false;         // the cycle runs in FP32 (coarsest solve in FP64)
     DevBuf<char> tmp_bytes;    // hipcub temporary storage
     uint64_t pattern0 = 0;
 };
@@ -121,6 +132,7 @@ extern "C" void rsgpu_amg_params_default(rsgpu_amg_params *p)
     p->zero_guess = 0;
     p->max_coarse = 2048;
     p->ordering = 1;
+    p->coarsen_ratio = 0;
 }
 
 // ---------------------------------------------------------------- helpers
@@ -723,16 +735,85 @@ __global__ void __launch_bounds__(256) k_dense_getf2(int n, double *a, int *ipiv
 }
 
 // u = f, then dgetrs ('N' or 'T') with one right-hand side
-__global__ void __launch_bounds__(256)
-    k_dense_getrs(int n, const double *__restrict__ a, const int *__restrict__ ipiv,
-                  const double *__restrict__ f, double *b, int trans, Skip sk)
+// dgetf2 column j for large matrices, as launches: the pivot (first row of
+// largest |a(i,j)|, i >= j, as idamax) and row swap, the column scaling, and
+// the trailing rank-1 update over many workgroups; same operations as the
+// one-workgroup kernel above
+__global__ void __launch_bounds__(1024) k_getf2_pivot(int n, int j, double *a, int *ipiv,
+                                                     int *info)
 {
-    if (sk.on())
-        return;
-    __shared__ int flag;
-    for (int i = threadIdx.x; i < n; i += blockDim.x)
-        b[i] = f[i];
+    __shared__ double sv[1024];
+    __shared__ int si[1024];
+    const int t = threadIdx.x;
+    double bv = -1.0;
+    int bi = n;
+    for (int i = j + 1 + t; i < n; i += blockDim.x) {
+        const double v = fabs(a[i + (size_t)j * n]);
+        if (v > bv) {   // first occurrence of the largest in this thread's rows
+            bv = v;
+            bi = i;
+        }
+    }
+    sv[t] = bv;
+    si[t] = bi;
     __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            const double v2 = sv[t + w];
+            const int i2 = si[t + w];
+            if (v2 > sv[t] || (v2 == sv[t] && i2 < si[t])) {
+                sv[t] = v2;
+                si[t] = i2;
+            }
+        }
+        __syncthreads();
+    }
+    // the sequential scan starts from row j and moves only on a larger value
+    const double vj = fabs(a[j + (size_t)j * n]);
+    const int jp = si[0] < n && sv[0] > vj ? si[0] : j;
+    __syncthreads();
+    if (t == 0) {
+        ipiv[j] = jp;
+        if (a[jp + (size_t)j * n] == 0 && *info == 0)
+            *info = j + 1;
+    }
+    const bool nz = a[jp + (size_t)j * n] != 0;
+    __syncthreads();
+    if (!nz)
+        return;
+    if (jp != j)
+        for (int c = t; c < n; c += blockDim.x) {
+            const double x = a[j + (size_t)c * n];
+            a[j + (size_t)c * n] = a[jp + (size_t)c * n];
+            a[jp + (size_t)c * n] = x;
+        }
+    __syncthreads();
+    if (j < n - 1) {
+        const double r = 1.0 / a[j + (size_t)j * n];
+        for (int i = j + 1 + t; i < n; i += blockDim.x)
+            a[i + (size_t)j * n] *= r;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_getf2_update(int n, int j, double *a)
+{
+    const int w = n - 1 - j;
+    const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (long long)w * w)
+        return;
+    const int i = j + 1 + (int)(q % w), c = j + 1 + (int)(q / w);
+    const double yj = a[j + (size_t)c * n];
+    if (yj != 0) {
+        const double temp = -1.0 * yj;
+        a[i + (size_t)c * n] = a[i + (size_t)c * n] + a[i + (size_t)j * n] * temp;
+    }
+}
+
+// dgetrs with one right-hand side b (in place) in one workgroup
+__device__ void dense_getrs_body(int n, const double *__restrict__ a,
+                                 const int *__restrict__ ipiv, double *b, int trans)
+{
+    __shared__ int flag;
     if (!trans) {
         if (threadIdx.x == 0)
             for (int j = 0; j < n; ++j)
@@ -786,6 +867,29 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+__global__ void __launch_bounds__(256)
+    k_dense_getrs(int n, const double *__restrict__ a, const int *__restrict__ ipiv,
+                  const double *__restrict__ f, double *b, int trans, Skip sk)
+{
+    if (sk.on())
+        return;
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        b[i] = f[i];
+    __syncthreads();
+    dense_getrs_body(n, a, ipiv, b, trans);
+}
+
+// column j = blockIdx.x of the inverse: the LU solve of e_j
+__global__ void __launch_bounds__(256)
+    k_dense_inv(int n, const double *__restrict__ a, const int *__restrict__ ipiv, double *inv)
+{
+    double *b = inv + (size_t)blockIdx.x * n;
+    for (int i = threadIdx.x; i < n; i += blockDim.x)
+        b[i] = i == (int)blockIdx.x ? 1.0 : 0.0;
+    __syncthreads();
+    dense_getrs_body(n, a, ipiv, b, 0);
+}
+
 // dgetrs 'N' for n <= 64 in one wavefront: lane i holds b[i]; the row swaps,
 // the unit-lower and the upper column sweeps run in dgetrs's order with the
 // pivot values broadcast across lanes.
@@ -827,67 +931,112 @@ __global__ void __launch_bounds__(64)
 }
 
 // ---------------------------------------------------------------- smoother
+template <typename T>
 __global__ void k_amg_gather(const int *__restrict__ src, const double *__restrict__ v,
-                             double *__restrict__ ell, size_t n)
+                             T *__restrict__ ell, size_t n)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         const int s = src[i];
-        ell[i] = s >= 0 ? v[s] : -0.0;
+        ell[i] = s >= 0 ? (T)v[s] : (T)-0.0;
     }
 }
 
+template <typename T>
 __global__ void k_amg_rowdiag(const int *__restrict__ row_glob, const int *__restrict__ diag,
-                              const double *__restrict__ v, double *__restrict__ dgl, int n)
+                              const double *__restrict__ v, T *__restrict__ dgl, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n)
-        dgl[i] = v[diag[row_glob[i]]];
+        dgl[i] = (T)v[diag[row_glob[i]]];
 }
 
-// One SSOR application (SSOR.java:143-194) per block.  ell_col >= 0 is a
-// local row of the block (LDS), < 0 encodes the global column -1-c, read
-// from xold (the vector at the start of the application).
-__device__ __forceinline__ void amg_ssor_sweep(int lb, int le, int step, int W,
+// One group of one level of an SSOR sweep (SSOR.java:143-194): lane t owns
+// row t of group g and forms its sum over the row's off-diagonal entries in
+// CSR order, then x_r += omega ((f_r - sum) / a_rr - x_r).  BLOCK: the
+// vector of the block is in LDS; an entry column < 0 encodes the global
+// column -1-c, read from xold (the vector at the start of the application).
+template <typename T, bool BLOCK>
+__device__ __forceinline__ void amg_ssor_group(const AmgLevelMeta &m, int g,
+                                               const int *__restrict__ rows,
+                                               const int *__restrict__ rlen,
+                                               const int *__restrict__ gofs,
+                                               const int *__restrict__ ecol,
+                                               const T *__restrict__ eval,
+                                               const T *__restrict__ dgl,
+                                               const T *__restrict__ f, int b0,
+                                               const T *__restrict__ xold, T omega, T *xv)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = (g << 6) + lane;
+    if (i >= m.count)
+        return;
+    const int r = rows[m.roff + i], len = rlen[m.roff + i];
+    const int e0 = gofs[m.gbase + g] + lane;
+    // eight entries' loads in flight at a time; the sum stays in entry order
+    T sigma = 0;
+    int q = 0;
+    for (; q + 8 <= len; q += 8) {
+        int c[8];
+        T v[8], xc[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            c[u] = ecol[e0 + 64 * (q + u)];
+            v[u] = eval[e0 + 64 * (q + u)];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            xc[u] = (!BLOCK || c[u] >= 0) ? xv[c[u]] : xold[-1 - c[u]];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            sigma += v[u] * xc[u];
+    }
+    for (; q < len; ++q) {
+        const int c = ecol[e0 + 64 * q];
+        const T xc = (!BLOCK || c >= 0) ? xv[c] : xold[-1 - c];
+        sigma += eval[e0 + 64 * q] * xc;
+    }
+    sigma = (f[b0 + r] - sigma) / dgl[m.roff + i];
+    const T xr = xv[r];
+    xv[r] = xr + omega * (sigma - xr);
+}
+
+template <typename T>
+__device__ __forceinline__ void amg_ssor_sweep(int lb, int le, int step,
                                                const AmgLevelMeta *__restrict__ meta,
                                                const int *__restrict__ rows,
+                                               const int *__restrict__ rlen,
+                                               const int *__restrict__ gofs,
                                                const int *__restrict__ ecol,
-                                               const double *__restrict__ eval,
-                                               const double *__restrict__ dgl,
-                                               const double *__restrict__ f, int b0,
-                                               const double *__restrict__ xold, double omega,
-                                               double *xv)
+                                               const T *__restrict__ eval,
+                                               const T *__restrict__ dgl,
+                                               const T *__restrict__ f, int b0,
+                                               const T *__restrict__ xold, T omega, T *xv)
 {
+    const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     for (int l = lb; l != le; l += step) {
         const AmgLevelMeta m = meta[l];
-        for (int t = threadIdx.x; t < m.count; t += blockDim.x) {
-            const int r = rows[m.roff + t];
-            const size_t q = (size_t)m.off + (size_t)t * W;
-            double sigma = 0;
-            for (int s = 0; s < W; ++s) {
-                const int c = ecol[q + s];
-                const double xc = c >= 0 ? xv[c] : xold[-1 - c];
-                sigma += eval[q + s] * xc;
-            }
-            sigma = (f[b0 + r] - sigma) / dgl[m.roff + t];
-            const double xr = xv[r];
-            xv[r] = xr + omega * (sigma - xr);
-        }
+        const int ng = (m.count + 63) >> 6;
+        for (int g = wave; g < ng; g += nw)
+            amg_ssor_group<T, true>(m, g, rows, rlen, gofs, ecol, eval, dgl, f, b0, xold, omega,
+                                    xv);
         __syncthreads();
     }
 }
 
+// One SSOR application per block, the block's vector in LDS.
+template <typename T>
 __global__ void __launch_bounds__(256)
     k_amg_ssor(const int *__restrict__ bstart, const int *__restrict__ blk_lev,
                const AmgLevelMeta *__restrict__ meta, const int *__restrict__ rows,
-               const int *__restrict__ ecol, const double *__restrict__ eval,
-               const double *__restrict__ dgl, int W, const double *__restrict__ f,
-               const double *__restrict__ xold, double *__restrict__ xnew, double omega_f,
-               double omega_r, int reverse, Skip sk)
+               const int *__restrict__ rlen, const int *__restrict__ gofs,
+               const int *__restrict__ ecol, const T *__restrict__ eval,
+               const T *__restrict__ dgl, const T *__restrict__ f, const T *__restrict__ xold,
+               T *__restrict__ xnew, T omega_f, T omega_r, int reverse, Skip sk)
 {
     if (sk.on())
         return;
-    __shared__ double xv[RS_AMG_LDS_ROWS + 1];
+    __shared__ T xv[RS_AMG_LDS_ROWS + 1];
     const int blk = blockIdx.x;
     const int b0 = bstart[blk], nb = bstart[blk + 1] - b0;
     for (int r = threadIdx.x; r < nb; r += blockDim.x)
@@ -896,63 +1045,127 @@ __global__ void __launch_bounds__(256)
         xv[nb] = 0;
     __syncthreads();
     const int l0 = blk_lev[blk], l1 = blk_lev[blk + 1];
-    amg_ssor_sweep(l0, l1, 1, W, meta, rows, ecol, eval, dgl, f, b0, xold, omega_f, xv);
+    amg_ssor_sweep<T>(l0, l1, 1, meta, rows, rlen, gofs, ecol, eval, dgl, f, b0, xold, omega_f,
+                      xv);
     if (reverse && l1 > l0)
-        amg_ssor_sweep(l1 - 1, l0 - 1, -1, W, meta, rows, ecol, eval, dgl, f, b0, xold,
-                       omega_r, xv);
+        amg_ssor_sweep<T>(l1 - 1, l0 - 1, -1, meta, rows, rlen, gofs, ecol, eval, dgl, f, b0,
+                          xold, omega_r, xv);
     for (int r = threadIdx.x; r < nb; r += blockDim.x)
         xnew[b0 + r] = xv[r];
 }
 
 // One level (colour) of an in-place SSOR sweep over a whole matrix: the
-// level's rows do not couple, so every row reads the current vector.
+// level's rows do not couple, so every row reads the current vector.  Four
+// groups per workgroup.
+template <typename T>
 __global__ void __launch_bounds__(256)
-    k_amg_ssor_level(AmgLevelMeta m, const int *__restrict__ rows,
-                     const int *__restrict__ ecol, const double *__restrict__ eval,
-                     const double *__restrict__ dgl, int W, const double *__restrict__ f,
-                     double *x, double omega, Skip sk)
+    k_amg_ssor_level(AmgLevelMeta m, const int *__restrict__ rows, const int *__restrict__ rlen,
+                     const int *__restrict__ gofs, const int *__restrict__ ecol,
+                     const T *__restrict__ eval, const T *__restrict__ dgl,
+                     const T *__restrict__ f, T *x, T omega, Skip sk)
 {
     if (sk.on())
         return;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= m.count)
+    const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if ((g << 6) >= m.count)
         return;
-    const int r = rows[m.roff + t];
-    const size_t q = (size_t)m.off + (size_t)t * W;
-    double sigma = 0;
-    for (int s = 0; s < W; ++s)
-        sigma += eval[q + s] * x[ecol[q + s]];
-    sigma = (f[r] - sigma) / dgl[m.roff + t];
-    const double xr = x[r];
-    x[r] = xr + omega * (sigma - xr);
+    amg_ssor_group<T, false>(m, g, rows, rlen, gofs, ecol, eval, dgl, f, 0, x, omega, x);
 }
 
 // ---------------------------------------------------------------- cycle kernels
-template <class Epi>
+template <typename T, class Epi>
 __global__ void __launch_bounds__(RS_SPMV_BS)
     k_amg_spmv(const int *__restrict__ wg_rows, const int *__restrict__ rowptr,
-               const int *__restrict__ colind, const double *__restrict__ val,
-               const double *__restrict__ x, Epi epi, Skip sk)
+               const int *__restrict__ colind, const T *__restrict__ val,
+               const T *__restrict__ x, Epi epi, Skip sk)
 {
     if (sk.on())
         return;
-    spmv_core<double, RS_SPMV_BS, RS_SPMV_NNZ>(wg_rows, rowptr, colind, val, x, epi);
+    spmv_core<T, RS_SPMV_BS, RS_SPMV_NNZ>(wg_rows, rowptr, colind, val, x, epi);
 }
 
 // prolongation u += P uc with CompColMatrix.multAdd's per-row order
+template <typename T>
 struct EpiSeqAdd {
-    double *y;
-    __device__ double init(int r) const { return y[r]; }
-    __device__ void row(int r, double dot) { y[r] = dot; }
+    T *y;
+    __device__ T init(int r) const { return y[r]; }
+    __device__ void row(int r, T dot) { y[r] = dot; }
 };
 
-__global__ void k_amg_copy(const double *__restrict__ a, double *__restrict__ b, int n, Skip sk)
+template <typename T>
+__global__ void k_amg_copy(const T *__restrict__ a, T *__restrict__ b, int n, Skip sk)
 {
     if (sk.on())
         return;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n)
-        b[i] = a ? a[i] : 0.0;
+        b[i] = a ? a[i] : T(0);
+}
+
+template <typename S, typename D>
+__global__ void k_amg_cvt(const S *__restrict__ a, D *__restrict__ b, size_t n, Skip sk)
+{
+    if (sk.on())
+        return;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        b[i] = (D)a[i];
+}
+
+// coarsest solve by the explicit inverse (column-major): u = C^-1 f as
+// RS_CINV_SLICES column slices (partial sums per slice, coalesced column
+// reads), then the slices added in order; C^-T f (transposed cycle): one
+// wave per row of C^-T, i.e. per column of the array, lanes over its
+// entries, a fixed-order lane reduction
+#define RS_CINV_SLICES 64
+__global__ void __launch_bounds__(256)
+    k_amg_cinv_part(int n, const double *__restrict__ inv, const double *__restrict__ f,
+                    double *__restrict__ part, Skip sk)
+{
+    if (sk.on())
+        return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const int sl = blockIdx.y;
+    const int j0 = (int)(((long long)n * sl) / RS_CINV_SLICES);
+    const int j1 = (int)(((long long)n * (sl + 1)) / RS_CINV_SLICES);
+    double acc = 0;
+    for (int j = j0; j < j1; ++j)
+        acc += inv[i + (size_t)j * n] * f[j];
+    part[(size_t)sl * n + i] = acc;
+}
+
+__global__ void __launch_bounds__(256)
+    k_amg_cinv_sum(int n, const double *__restrict__ part, double *__restrict__ u, Skip sk)
+{
+    if (sk.on())
+        return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    double acc = 0;
+    for (int sl = 0; sl < RS_CINV_SLICES; ++sl)
+        acc += part[(size_t)sl * n + i];
+    u[i] = acc;
+}
+
+__global__ void __launch_bounds__(256)
+    k_amg_cinv_t(int n, const double *__restrict__ inv, const double *__restrict__ f,
+                 double *__restrict__ u, Skip sk)
+{
+    if (sk.on())
+        return;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (i >= n)
+        return;
+    double acc = 0;
+    for (int j = lane; j < n; j += 64)
+        acc += inv[j + (size_t)i * n] * f[j];
+    for (int o = 32; o > 0; o >>= 1)
+        acc += __shfl_down(acc, o, 64);
+    if (lane == 0)
+        u[i] = acc;
 }
 
 // ---------------------------------------------------------------- host: level setup
@@ -1014,22 +1227,25 @@ static void build_ssor_schedule(AmgLevel &L, int block_rows, int ordering, hipSt
     std::vector<int> bstart(nb + 1);
     for (int b = 0; b <= nb; ++b)
         bstart[b] = (int)(((long long)n * b) / nb);
-    int maxrows = 0, W = 1;
+    int maxrows = 0;
     for (int b = 0; b < nb; ++b)
         maxrows = std::max(maxrows, bstart[b + 1] - bstart[b]);
     const bool global = maxrows > RS_AMG_LDS_ROWS;
     if (global && nb > 1)
         throw RsError{RSGPU_BAD_ARGUMENT, "AMG smoother block larger than the LDS vector"};
-    for (int i = 0; i < n; ++i)
-        W = std::max(W, rp[i + 1] - rp[i] - 1);
-    W = (W + 3) & ~3;
+    auto offd = [&](int r) {
+        int c = 0;
+        for (int j = rp[r]; j < rp[r + 1]; ++j)
+            c += ci[j] != r;
+        return c;
+    };
     std::vector<int> level(n, 0), pushed(n, 0), blk(n, 0), colour;
     for (int b = 0; b < nb; ++b)
         for (int r = bstart[b]; r < bstart[b + 1]; ++r)
             blk[r] = b;
     if (ordering == 1)
         greedy_colour(n, rp, ci, blk, colour);
-    std::vector<int> blk_lev(nb + 1), rows, row_glob, col, src;
+    std::vector<int> blk_lev(nb + 1), rows, rlen, row_glob, gofs, col, src;
     std::vector<AmgLevelMeta> meta;
     int nlev_max = 0;
     for (int b = 0; b < nb; ++b) {
@@ -1064,38 +1280,58 @@ static void build_ssor_schedule(AmgLevel &L, int block_rows, int ordering, hipSt
             byl[level[r]].push_back(r);
         blk_lev[b] = (int)meta.size();
         for (int l = 0; l < nlev; ++l) {
+            // rows of one level do not couple: any order within it; longest first
+            std::vector<int> &R = byl[l];
+            std::vector<int> len(R.size());
+            for (size_t t = 0; t < R.size(); ++t)
+                len[t] = offd(R[t]);
+            std::vector<int> ord(R.size());
+            for (size_t t = 0; t < R.size(); ++t)
+                ord[t] = (int)t;
+            std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return len[x] > len[y]; });
             AmgLevelMeta m;
-            m.count = (int)byl[l].size();
-            m.off = (int)col.size();
+            m.count = (int)R.size();
+            m.gbase = (int)gofs.size();
             m.roff = (int)rows.size();
             m.pad = 0;
-            col.resize(col.size() + (size_t)W * m.count, b1 - b0);
-            src.resize(src.size() + (size_t)W * m.count, -1);
-            for (int t = 0; t < m.count; ++t) {
-                const int r = byl[l][t];
-                rows.push_back(r - b0);
-                row_glob.push_back(r);
-                int q = 0;
-                for (int j = rp[r]; j < rp[r + 1]; ++j) {
-                    const int c = ci[j];
-                    if (c == r)
-                        continue;
-                    const size_t o = (size_t)m.off + (size_t)t * W + q;
-                    col[o] = (c >= b0 && c < b1) ? c - b0 : -1 - c;
-                    src[o] = j;
-                    ++q;
+            for (int t0 = 0; t0 < m.count; t0 += 64) {
+                const int tn = std::min(64, m.count - t0);
+                const int w = len[ord[t0]];
+                const size_t base = col.size();
+                if (base + (size_t)64 * w > (size_t)INT32_MAX)
+                    throw RsError{RSGPU_BAD_ARGUMENT, "AMG smoother arrays exceed 2^31 entries"};
+                gofs.push_back((int)base);
+                col.resize(base + (size_t)64 * w, b1 - b0);
+                src.resize(base + (size_t)64 * w, -1);
+                for (int t = 0; t < tn; ++t) {
+                    const int r = R[ord[t0 + t]];
+                    rows.push_back(r - b0);
+                    rlen.push_back(len[ord[t0 + t]]);
+                    row_glob.push_back(r);
+                    int q = 0;
+                    for (int j = rp[r]; j < rp[r + 1]; ++j) {
+                        const int c = ci[j];
+                        if (c == r)
+                            continue;
+                        const size_t o = base + (size_t)64 * q + t;
+                        col[o] = (c >= b0 && c < b1) ? c - b0 : -1 - c;
+                        src[o] = j;
+                        ++q;
+                    }
                 }
             }
             meta.push_back(m);
         }
     }
     blk_lev[nb] = (int)meta.size();
-    col.resize(col.size() + W, 0);
-    src.resize(src.size() + W, -1);
+    // sentinels keep every array non-empty
+    col.resize(col.size() + 64, 0);
+    src.resize(src.size() + 64, -1);
     row_glob.push_back(0);
     rows.push_back(0);
+    rlen.push_back(0);
+    gofs.push_back(0);
     L.nb = nb;
-    L.W = W;
     L.max_block_rows = maxrows;
     L.nlev_max = nlev_max;
     L.global_sweep = global;
@@ -1104,21 +1340,32 @@ static void build_ssor_schedule(AmgLevel &L, int block_rows, int ordering, hipSt
     L.blk_lev.upload(blk_lev, s);
     L.meta.upload((const int *)meta.data(), meta.size() * 4, s);
     L.rows.upload(rows, s);
+    L.rlen.upload(rlen, s);
     L.row_glob.upload(row_glob, s);
+    L.gofs.upload(gofs, s);
     L.ell_col.upload(col, s);
     L.ell_src.upload(src, s);
     L.ell_size = col.size();
     L.nrows_lev = (int)row_glob.size();
     L.ell_val.alloc(L.ell_size);
     L.dgl.alloc(L.nrows_lev);
+    RS_HIP(hipStreamSynchronize(s));   // the host vectors are released next
 }
 
-static void ssor_values(AmgLevel &L, hipStream_t s)
+static void ssor_values(AmgLevel &L, hipStream_t s, bool fp32 = false)
 {
-    hipLaunchKernelGGL(k_amg_gather, dim3(gridn((long long)L.ell_size)), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_amg_gather<double>, dim3(gridn((long long)L.ell_size)), dim3(256), 0, s,
                        L.ell_src.p, L.v.p, L.ell_val.p, L.ell_size);
-    hipLaunchKernelGGL(k_amg_rowdiag, dim3(gridn(L.nrows_lev)), dim3(256), 0, s,
+    hipLaunchKernelGGL(k_amg_rowdiag<double>, dim3(gridn(L.nrows_lev)), dim3(256), 0, s,
                        L.row_glob.p, L.diag.p, L.v.p, L.dgl.p, L.nrows_lev);
+    if (fp32) {
+        L.ell_val32.alloc(L.ell_size);
+        L.dgl32.alloc(L.nrows_lev);
+        hipLaunchKernelGGL(k_amg_gather<float>, dim3(gridn((long long)L.ell_size)), dim3(256), 0,
+                           s, L.ell_src.p, L.v.p, L.ell_val32.p, L.ell_size);
+        hipLaunchKernelGGL(k_amg_rowdiag<float>, dim3(gridn(L.nrows_lev)), dim3(256), 0, s,
+                           L.row_glob.p, L.diag.p, L.v.p, L.dgl32.p, L.nrows_lev);
+    }
     RS_CHECK_LAUNCH();
 }
 
@@ -1327,7 +1574,7 @@ static void amg_transpose(AmgData *D, AmgLevel &L, hipStream_t s)
 
 // u and ualt carry one extra zero entry: the padding column of an in-place
 // (global) sweep
-static void level_vectors(AmgLevel &L, hipStream_t s)
+static void level_vectors(AmgLevel &L, hipStream_t s, bool fp32 = false)
 {
     L.u.alloc((size_t)L.n + 1);
     L.ualt.alloc((size_t)L.n + 1);
@@ -1335,6 +1582,14 @@ static void level_vectors(AmgLevel &L, hipStream_t s)
     L.ualt.zero(s);
     L.f.alloc(std::max(L.n, 1));
     L.r.alloc(std::max(L.n, 1));
+    if (fp32) {
+        L.u32.alloc((size_t)L.n + 1);
+        L.ualt32.alloc((size_t)L.n + 1);
+        L.u32.zero(s);
+        L.ualt32.zero(s);
+        L.f32.alloc(std::max(L.n, 1));
+        L.r32.alloc(std::max(L.n, 1));
+    }
 }
 
 // AMG.setMatrix (AMG.java:228-294).  Level 0 holds its own copy of A
@@ -1349,6 +1604,7 @@ int amg_setup(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
         M->amg = new AmgData();
     AmgData *D = M->amg;
     D->p = M->amg_params;
+    D->fp32 = M->precision == RSGPU_FP32;
     const rsgpu_amg_params &p = D->p;
     const bool same = !D->L.empty() && D->pattern0 == A->pattern_id;
     if (!same) {
@@ -1401,6 +1657,11 @@ int amg_setup(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
         const int c = amg_aggregate(D, L, k, s);
         if (c == 0)
             break;
+        // optional stop: a level whose aggregates keep more than the given
+        // fraction of its rows (aggregates degenerating towards singletons)
+        // becomes the coarsest
+        if (p.coarsen_ratio > 0 && (double)c > p.coarsen_ratio * n)
+            break;
         L.c = c;
         amg_transpose(D, L, s);
         // AP = A P (n x c), Ac = P' AP (c x c)
@@ -1419,7 +1680,7 @@ int amg_setup(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
     // smoothers, transfer plans and vectors
     for (int k = 0; k < D->m; ++k) {
         AmgLevel &L = *D->L[k];
-        level_vectors(L, s);
+        level_vectors(L, s, D->fp32);
         if (k == D->m - 1)
             break;
         if (k > 0) {
@@ -1442,7 +1703,20 @@ int amg_setup(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
             L.nb_rows_param = p.block_rows;
             L.ordering_param = p.ordering;
         }
-        ssor_values(L, s);
+        ssor_values(L, s, D->fp32);
+        if (D->fp32) {
+            // FP32 copies of the level's matrix and transfer operators
+            L.v32.alloc(std::max(L.nnz, 1));
+            L.Pv32.alloc(std::max(L.pnnz, 1));
+            L.Tv32.alloc(std::max(L.pnnz, 1));
+            hipLaunchKernelGGL((k_amg_cvt<double, float>), dim3(gridn(L.nnz)), dim3(256), 0, s,
+                               L.v.p, L.v32.p, (size_t)L.nnz, Skip{nullptr, nullptr});
+            hipLaunchKernelGGL((k_amg_cvt<double, float>), dim3(gridn(L.pnnz)), dim3(256), 0, s,
+                               L.Pv.p, L.Pv32.p, (size_t)L.pnnz, Skip{nullptr, nullptr});
+            hipLaunchKernelGGL((k_amg_cvt<double, float>), dim3(gridn(L.pnnz)), dim3(256), 0, s,
+                               L.Tv.p, L.Tv32.p, (size_t)L.pnnz, Skip{nullptr, nullptr});
+            RS_CHECK_LAUNCH();
+        }
     }
     // coarsest level: dense LU (AMG.java:268-271)
     AmgLevel &Lc = *D->L[D->m - 1];
@@ -1459,98 +1733,226 @@ int amg_setup(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
         const int *cci = Lc.ci;
         hipLaunchKernelGGL(k_dense_fill, dim3(gridn(D->nc)), dim3(256), 0, s, D->nc, crp, cci,
                            Lc.v.p, D->lu.p);
-        hipLaunchKernelGGL(k_dense_getf2, dim3(1), dim3(256), 0, s, D->nc, D->lu.p, D->ipiv.p,
-                           D->flag.p);
+        if (D->nc <= 256) {
+            hipLaunchKernelGGL(k_dense_getf2, dim3(1), dim3(256), 0, s, D->nc, D->lu.p,
+                               D->ipiv.p, D->flag.p);
+        } else {
+            set_int(s, D->flag.p, 0);
+            for (int j = 0; j < D->nc; ++j) {
+                hipLaunchKernelGGL(k_getf2_pivot, dim3(1), dim3(1024), 0, s, D->nc, j, D->lu.p,
+                                   D->ipiv.p, D->flag.p);
+                const long long w = D->nc - 1 - j;
+                if (w > 0)
+                    hipLaunchKernelGGL(k_getf2_update, dim3(gridn(w * w)), dim3(256), 0, s,
+                                       D->nc, j, D->lu.p);
+            }
+        }
         RS_CHECK_LAUNCH();
         if (read_int(s, D->flag.p)) {
             M->ctx->err = "AMG coarsest matrix is singular";
             return RSGPU_SINGULAR;
         }
+        // above 256 rows the cycle's coarsest solve multiplies by the
+        // explicit inverse (its columns solved from the LU at setup): the
+        // one-workgroup LU solve costs two barriers per row
+        if (D->nc > 256) {
+            D->cinv.alloc((size_t)D->nc * D->nc);
+            D->cpart.alloc((size_t)D->nc * RS_CINV_SLICES);
+            hipLaunchKernelGGL(k_dense_inv, dim3(D->nc), dim3(256), 0, s, D->nc, D->lu.p,
+                               D->ipiv.p, D->cinv.p);
+            RS_CHECK_LAUNCH();
+        } else {
+            D->cinv.release();
+        }
+    }
+    if (D->fp32) {
+        D->cf.alloc(std::max(D->nc, 1));
+        D->cu.alloc(std::max(D->nc, 1));
     }
     RS_HIP(hipStreamSynchronize(s));
     return RSGPU_OK;
 }
 
 // ---------------------------------------------------------------- cycle
-static void amg_cycle(AmgData *D, int k, const double *f, bool trans, hipStream_t s, Skip sk)
+// the level's arrays of the cycle's precision
+template <typename T>
+struct AmgArr;
+template <>
+struct AmgArr<double> {
+    static double *u(AmgLevel &L) { return L.u.p; }
+    static double *f(AmgLevel &L) { return L.f.p; }
+    static double *r(AmgLevel &L) { return L.r.p; }
+    static const double *v(AmgLevel &L) { return L.v.p; }
+    static const double *Pv(AmgLevel &L) { return L.Pv.p; }
+    static const double *Tv(AmgLevel &L) { return L.Tv.p; }
+    static const double *ev(AmgLevel &L) { return L.ell_val.p; }
+    static const double *dg(AmgLevel &L) { return L.dgl.p; }
+    static double *ualt(AmgLevel &L) { return L.ualt.p; }
+    static void swap(AmgLevel &L) { swap_buf(L.u, L.ualt); }
+};
+template <>
+struct AmgArr<float> {
+    static float *u(AmgLevel &L) { return L.u32.p; }
+    static float *f(AmgLevel &L) { return L.f32.p; }
+    static float *r(AmgLevel &L) { return L.r32.p; }
+    static const float *v(AmgLevel &L) { return L.v32.p; }
+    static const float *Pv(AmgLevel &L) { return L.Pv32.p; }
+    static const float *Tv(AmgLevel &L) { return L.Tv32.p; }
+    static const float *ev(AmgLevel &L) { return L.ell_val32.p; }
+    static const float *dg(AmgLevel &L) { return L.dgl32.p; }
+    static float *ualt(AmgLevel &L) { return L.ualt32.p; }
+    static void swap(AmgLevel &L) { swap_buf(L.u32, L.ualt32); }
+};
+
+// the coarsest solve in FP64: u = C^-1 f (DenseLU.solve, or the inverse)
+static void amg_coarse64(AmgData *D, const double *f, double *u, bool trans, hipStream_t s,
+                         Skip sk)
 {
-    AmgLevel &L = *D->L[k];
-    const rsgpu_amg_params &p = D->p;
-    if (k == D->m - 1) {
-        if (D->nc > 0 && D->nc <= 64 && !trans)
-            hipLaunchKernelGGL(k_dense_getrs64, dim3(1), dim3(64), 0, s, D->nc, D->lu.p,
-                               D->ipiv.p, f, L.u.p, sk);
-        else if (D->nc > 0)
-            hipLaunchKernelGGL(k_dense_getrs, dim3(1), dim3(256), 0, s, D->nc, D->lu.p,
-                               D->ipiv.p, f, L.u.p, trans ? 1 : 0, sk);
+    if (D->nc == 0)
+        return;
+    if (D->cinv.p && !trans) {
+        hipLaunchKernelGGL(k_amg_cinv_part, dim3(gridn(D->nc), RS_CINV_SLICES), dim3(256), 0, s,
+                           D->nc, D->cinv.p, f, D->cpart.p, sk);
+        hipLaunchKernelGGL(k_amg_cinv_sum, dim3(gridn(D->nc)), dim3(256), 0, s, D->nc,
+                           D->cpart.p, u, sk);
+    } else if (D->cinv.p)
+        hipLaunchKernelGGL(k_amg_cinv_t, dim3((D->nc + 3) / 4), dim3(256), 0, s, D->nc,
+                           D->cinv.p, f, u, sk);
+    else if (D->nc <= 64 && !trans)
+        hipLaunchKernelGGL(k_dense_getrs64, dim3(1), dim3(64), 0, s, D->nc, D->lu.p, D->ipiv.p,
+                           f, u, sk);
+    else
+        hipLaunchKernelGGL(k_dense_getrs, dim3(1), dim3(256), 0, s, D->nc, D->lu.p, D->ipiv.p,
+                           f, u, trans ? 1 : 0, sk);
+    RS_CHECK_LAUNCH();
+}
+
+// one SSOR application on level L (the block kernel or, for one block past
+// the LDS vector, one launch per level / colour)
+template <typename T>
+static void amg_smooth(AmgLevel &L, const T *f, double wf, double wr, int reverse,
+                       hipStream_t s, Skip sk)
+{
+    using A = AmgArr<T>;
+    if (L.global_sweep) {
+        const int nl = (int)L.h_meta.size();
+        auto launch = [&](int l, double w) {
+            const AmgLevelMeta &m = L.h_meta[l];
+            const int ng = (m.count + 63) >> 6;
+            hipLaunchKernelGGL(k_amg_ssor_level<T>, dim3(std::max(1, (ng + 3) / 4)), dim3(256),
+                               0, s, m, L.rows.p, L.rlen.p, L.gofs.p, L.ell_col.p, A::ev(L),
+                               A::dg(L), f, A::u(L), (T)w, sk);
+        };
+        for (int l = 0; l < nl; ++l)
+            launch(l, wf);
+        if (reverse)
+            for (int l = nl - 1; l >= 0; --l)
+                launch(l, wr);
         RS_CHECK_LAUNCH();
         return;
     }
-    auto smooth = [&](double wf, double wr) {
-        if (L.global_sweep) {
-            const int nl = (int)L.h_meta.size();
-            for (int l = 0; l < nl; ++l)
-                hipLaunchKernelGGL(k_amg_ssor_level, dim3(gridn(L.h_meta[l].count)), dim3(256),
-                                   0, s, L.h_meta[l], L.rows.p, L.ell_col.p, L.ell_val.p,
-                                   L.dgl.p, L.W, f, L.u.p, wf, sk);
-            if (p.reverse)
-                for (int l = nl - 1; l >= 0; --l)
-                    hipLaunchKernelGGL(k_amg_ssor_level, dim3(gridn(L.h_meta[l].count)),
-                                       dim3(256), 0, s, L.h_meta[l], L.rows.p, L.ell_col.p,
-                                       L.ell_val.p, L.dgl.p, L.W, f, L.u.p, wr, sk);
-            RS_CHECK_LAUNCH();
-            return;
-        }
-        hipLaunchKernelGGL(k_amg_ssor, dim3(L.nb), dim3(256), 0, s, L.bstart.p, L.blk_lev.p,
-                           (const AmgLevelMeta *)L.meta.p, L.rows.p, L.ell_col.p, L.ell_val.p,
-                           L.dgl.p, L.W, f, L.u.p, L.ualt.p, wf, wr, p.reverse, sk);
-        RS_CHECK_LAUNCH();
-        swap_buf(L.u, L.ualt);
-    };
-    // SSOR.transApply is apply (the matrix is assumed symmetric)
-    for (int i = 0; i < p.nu1; ++i)
-        smooth(p.omega_pre_f, p.omega_pre_r);
-    AmgLevel &C = *D->L[k + 1];
-    hipLaunchKernelGGL(k_amg_copy, dim3(gridn(C.n)), dim3(256), 0, s, (const double *)nullptr,
-                       C.u.p, C.n, sk);
-    // r = f - A u (r.set(f); A.multAdd(-1, u, r))
-    hipLaunchKernelGGL((k_amg_spmv<EpiResid<double>>), dim3(L.plan.nwg), dim3(RS_SPMV_BS), 0,
-                       s, L.plan.wg_rows.p, L.rp, L.ci, L.v.p, L.u.p,
-                       EpiResid<double>{L.r.p, f, -1.0}, sk);
-    // f_c = I' r (CompColMatrix.transMult)
-    hipLaunchKernelGGL((k_amg_spmv<EpiPlain<double>>), dim3(L.Tplan.nwg), dim3(RS_SPMV_BS), 0,
-                       s, L.Tplan.wg_rows.p, L.Trp.p, L.Tci.p, L.Tv.p, L.r.p,
-                       EpiPlain<double>{C.f.p}, sk);
+    hipLaunchKernelGGL(k_amg_ssor<T>, dim3(L.nb), dim3(256), 0, s, L.bstart.p, L.blk_lev.p,
+                       (const AmgLevelMeta *)L.meta.p, L.rows.p, L.rlen.p, L.gofs.p,
+                       L.ell_col.p, A::ev(L), A::dg(L), f, A::u(L), A::ualt(L), (T)wf, (T)wr,
+                       reverse, sk);
     RS_CHECK_LAUNCH();
-    for (int g = 0; g < p.gamma; ++g)
-        amg_cycle(D, k + 1, C.f.p, trans, s, sk);
-    // u += I u_c (CompColMatrix.multAdd)
-    hipLaunchKernelGGL((k_amg_spmv<EpiSeqAdd>), dim3(L.Pplan.nwg), dim3(RS_SPMV_BS), 0, s,
-                       L.Pplan.wg_rows.p, L.Prp.p, L.Pci.p, L.Pv.p, C.u.p, EpiSeqAdd{L.u.p},
-                       sk);
-    RS_CHECK_LAUNCH();
-    for (int i = 0; i < p.nu2; ++i)
-        smooth(p.omega_post_f, p.omega_post_r);
+    A::swap(L);
 }
 
-// AMG.apply / transApply (AMG.java:208-226)
+// AMG.cycle (AMG.java:302-342) in precision T; the coarsest solve in FP64
+template <typename T>
+static void amg_cycle(AmgData *D, int k, const T *f, bool trans, hipStream_t s, Skip sk)
+{
+    using A = AmgArr<T>;
+    AmgLevel &L = *D->L[k];
+    const rsgpu_amg_params &p = D->p;
+    if (k == D->m - 1) {
+        if (std::is_same<T, double>::value) {
+            amg_coarse64(D, (const double *)f, (double *)A::u(L), trans, s, sk);
+        } else if (D->nc > 0) {
+            hipLaunchKernelGGL((k_amg_cvt<T, double>), dim3(gridn(D->nc)), dim3(256), 0, s, f,
+                               D->cf.p, (size_t)D->nc, sk);
+            amg_coarse64(D, D->cf.p, D->cu.p, trans, s, sk);
+            hipLaunchKernelGGL((k_amg_cvt<double, T>), dim3(gridn(D->nc)), dim3(256), 0, s,
+                               D->cu.p, A::u(L), (size_t)D->nc, sk);
+            RS_CHECK_LAUNCH();
+        }
+        return;
+    }
+    // SSOR.transApply is apply (the matrix is assumed symmetric)
+    for (int i = 0; i < p.nu1; ++i)
+        amg_smooth<T>(L, f, p.omega_pre_f, p.omega_pre_r, p.reverse, s, sk);
+    AmgLevel &C = *D->L[k + 1];
+    hipLaunchKernelGGL(k_amg_copy<T>, dim3(gridn(C.n)), dim3(256), 0, s, (const T *)nullptr,
+                       A::u(C), C.n, sk);
+    // r = f - A u (r.set(f); A.multAdd(-1, u, r))
+    hipLaunchKernelGGL((k_amg_spmv<T, EpiResid<T>>), dim3(L.plan.nwg), dim3(RS_SPMV_BS), 0, s,
+                       L.plan.wg_rows.p, L.rp, L.ci, A::v(L), A::u(L),
+                       EpiResid<T>{A::r(L), f, T(-1)}, sk);
+    // f_c = I' r (CompColMatrix.transMult)
+    hipLaunchKernelGGL((k_amg_spmv<T, EpiPlain<T>>), dim3(L.Tplan.nwg), dim3(RS_SPMV_BS), 0, s,
+                       L.Tplan.wg_rows.p, L.Trp.p, L.Tci.p, A::Tv(L), A::r(L),
+                       EpiPlain<T>{A::f(C)}, sk);
+    RS_CHECK_LAUNCH();
+    for (int g = 0; g < p.gamma; ++g)
+        amg_cycle<T>(D, k + 1, A::f(C), trans, s, sk);
+    // u += I u_c (CompColMatrix.multAdd)
+    hipLaunchKernelGGL((k_amg_spmv<T, EpiSeqAdd<T>>), dim3(L.Pplan.nwg), dim3(RS_SPMV_BS), 0, s,
+                       L.Pplan.wg_rows.p, L.Prp.p, L.Pci.p, A::Pv(L), A::u(C),
+                       EpiSeqAdd<T>{A::u(L)}, sk);
+    RS_CHECK_LAUNCH();
+    for (int i = 0; i < p.nu2; ++i)
+        amg_smooth<T>(L, f, p.omega_post_f, p.omega_post_r, p.reverse, s, sk);
+}
+
+template <typename T>
+static void amg_apply_t(AmgData *D, const T *b, T *x, hipStream_t s, Skip sk, bool trans)
+{
+    using A = AmgArr<T>;
+    AmgLevel &L0 = *D->L[0];
+    const int n = L0.n;
+    if (n == 0)
+        return;
+    hipLaunchKernelGGL(k_amg_copy<T>, dim3(gridn(n)), dim3(256), 0, s,
+                       D->p.zero_guess ? (const T *)nullptr : (const T *)x, A::u(L0), n, sk);
+    RS_CHECK_LAUNCH();
+    amg_cycle<T>(D, 0, b, trans, s, sk);
+    hipLaunchKernelGGL(k_amg_copy<T>, dim3(gridn(n)), dim3(256), 0, s, (const T *)A::u(L0), x, n,
+                       sk);
+    RS_CHECK_LAUNCH();
+}
+
+// AMG.apply / transApply (AMG.java:208-226); an FP32 hierarchy on FP64
+// vectors converts at the boundary
 void amg_apply(rsgpu_prec *M, const double *b, double *x, const DevState *st, bool trans)
 {
     AmgData *D = M->amg;
     hipStream_t s = M->ctx->stream;
     Skip sk{st ? &st->done : nullptr, st ? &st->cycle_end : nullptr};
-    AmgLevel &L0 = *D->L[0];
-    const int n = L0.n;
-    if (n == 0)
+    const int n = D->L[0]->n;
+    if (!D->fp32) {
+        amg_apply_t<double>(D, b, x, s, sk, trans);
         return;
-    hipLaunchKernelGGL(k_amg_copy, dim3(gridn(n)), dim3(256), 0, s,
-                       D->p.zero_guess ? (const double *)nullptr : (const double *)x, L0.u.p, n,
-                       sk);
+    }
+    M->tmpa32.alloc(std::max(n, 1));
+    M->tmpb32.alloc(std::max(n, 1));
+    hipLaunchKernelGGL((k_amg_cvt<double, float>), dim3(gridn(n)), dim3(256), 0, s, b,
+                       M->tmpa32.p, (size_t)n, sk);
+    hipLaunchKernelGGL((k_amg_cvt<double, float>), dim3(gridn(n)), dim3(256), 0, s,
+                       (const double *)x, M->tmpb32.p, (size_t)n, sk);
+    amg_apply_t<float>(D, M->tmpa32.p, M->tmpb32.p, s, sk, trans);
+    hipLaunchKernelGGL((k_amg_cvt<float, double>), dim3(gridn(n)), dim3(256), 0, s,
+                       (const float *)M->tmpb32.p, x, (size_t)n, sk);
     RS_CHECK_LAUNCH();
-    amg_cycle(D, 0, b, trans, s, sk);
-    hipLaunchKernelGGL(k_amg_copy, dim3(gridn(n)), dim3(256), 0, s, (const double *)L0.u.p, x,
-                       n, sk);
-    RS_CHECK_LAUNCH();
+}
+
+void amg_apply32(rsgpu_prec *M, const float *b, float *x, const DevState *st)
+{
+    AmgData *D = M->amg;
+    Skip sk{st ? &st->done : nullptr, st ? &st->cycle_end : nullptr};
+    if (!D->fp32)
+        throw RsError{RSGPU_BAD_ARGUMENT, "FP32 apply needs an FP32 AMG"};
+    amg_apply_t<float>(D, b, x, M->ctx->stream, sk, false);
 }
 
 int amg_levels(const rsgpu_prec *M, int *sizes, int *nnzs, int cap)
@@ -1673,27 +2075,10 @@ void ssor_apply(rsgpu_prec *M, const double *b, double *x, const DevState *st)
     if (n == 0)
         return;
     const rsgpu_amg_params &p = D->p;
-    hipLaunchKernelGGL(k_amg_copy, dim3(gridn(n)), dim3(256), 0, s, (const double *)x, L.u.p, n,
-                       sk);
-    if (L.global_sweep) {
-        const int nl = (int)L.h_meta.size();
-        for (int l = 0; l < nl; ++l)
-            hipLaunchKernelGGL(k_amg_ssor_level, dim3(gridn(L.h_meta[l].count)), dim3(256), 0, s,
-                               L.h_meta[l], L.rows.p, L.ell_col.p, L.ell_val.p, L.dgl.p, L.W, b,
-                               L.u.p, p.omega_pre_f, sk);
-        if (p.reverse)
-            for (int l = nl - 1; l >= 0; --l)
-                hipLaunchKernelGGL(k_amg_ssor_level, dim3(gridn(L.h_meta[l].count)), dim3(256),
-                                   0, s, L.h_meta[l], L.rows.p, L.ell_col.p, L.ell_val.p,
-                                   L.dgl.p, L.W, b, L.u.p, p.omega_pre_r, sk);
-    } else {
-        hipLaunchKernelGGL(k_amg_ssor, dim3(L.nb), dim3(256), 0, s, L.bstart.p, L.blk_lev.p,
-                           (const AmgLevelMeta *)L.meta.p, L.rows.p, L.ell_col.p, L.ell_val.p,
-                           L.dgl.p, L.W, b, L.u.p, L.ualt.p, p.omega_pre_f, p.omega_pre_r,
-                           p.reverse, sk);
-        swap_buf(L.u, L.ualt);
-    }
-    hipLaunchKernelGGL(k_amg_copy, dim3(gridn(n)), dim3(256), 0, s, (const double *)L.u.p, x, n,
-                       sk);
+    hipLaunchKernelGGL(k_amg_copy<double>, dim3(gridn(n)), dim3(256), 0, s, (const double *)x,
+                       L.u.p, n, sk);
+    amg_smooth<double>(L, b, p.omega_pre_f, p.omega_pre_r, p.reverse, s, sk);
+    hipLaunchKernelGGL(k_amg_copy<double>, dim3(gridn(n)), dim3(256), 0, s,
+                       (const double *)L.u.p, x, n, sk);
     RS_CHECK_LAUNCH();
 }

@@ -2864,9 +2864,14 @@ void prec_apply(rsgpu_prec *M, const float *b, float *x, const DevState *st)
         RS_CHECK_LAUNCH();
         return;
     }
+    if (M->kind == RSGPU_PREC_AMG && M->precision == RSGPU_FP32) {
+        amg_apply32(M, b, x, st);
+        return;
+    }
     if (M->precision != RSGPU_FP32 || M->kind == RSGPU_PREC_AMG ||
         M->kind == RSGPU_PREC_SSOR)
-        throw RsError{RSGPU_BAD_ARGUMENT, "FP32 apply needs an FP32 ILU or diagonal preconditioner"};
+        throw RsError{RSGPU_BAD_ARGUMENT,
+                      "FP32 apply needs an FP32 ILU, AMG or diagonal preconditioner"};
     if (M->cell)
         launch_apply_cells<float>(M, M->ell32_L.p, M->ell32_U.p, b, x, sk1, sk2);
     else

@@ -578,6 +578,7 @@ void prec_setup_pattern(rsgpu_prec *M, const std::vector<int> &rp, const std::ve
                         const std::vector<int> &dg, int bs);
 void prec_refresh_fp64(rsgpu_prec *M);
 int ilut_set_matrix(rsgpu_prec *M, rsgpu_csr *A, int *bad_row);
+void amg_apply32(rsgpu_prec *M, const float *b, float *x, const DevState *st);
 bool ilut_apply(rsgpu_prec *M, const double *b, double *x, bool trans, const int *sk1,
                 const int *sk2);
 int prec_factor(rsgpu_prec *M, rsgpu_csr *A, int *bad_row);

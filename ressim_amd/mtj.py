@@ -562,8 +562,8 @@ class AMG(Preconditioner):
     _ORDER = ("omega_pre_f", "omega_pre_r", "omega_post_f", "omega_post_r", "nu1", "nu2",
               "gamma", "min", "omega")
 
-    def __init__(self, *args, ctx: Optional[Context] = None, **kw):
-        super().__init__(1, L.FP64, ctx)
+    def __init__(self, *args, ctx: Optional[Context] = None, precision: int = L.FP64, **kw):
+        super().__init__(1, precision, ctx)
         p = L.AmgParams()
         L.load().rsgpu_amg_params_default(C.byref(p))
         if len(args) == 7:
