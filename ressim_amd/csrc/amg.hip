@@ -1001,6 +1001,73 @@ __device__ __forceinline__ void amg_ssor_group(const AmgLevelMeta &m, int g,
     xv[r] = xr + omega * (sigma - xr);
 }
 
+// The same update with eight lanes per row (rows with many entries): lane k
+// of the row's unit forms the products of entries k, k + 8, ... of a chunk
+// of RS_ROW8_CHUNK into LDS, and the unit's lane 0 adds them in entry order.
+// i: the row's slot in the level; P: the unit's RS_ROW8_CHUNK LDS slots.
+#define RS_ROW8_CHUNK 128
+template <typename T, bool BLOCK>
+__device__ __forceinline__ void amg_ssor_row8(const AmgLevelMeta &m, int i, bool live,
+                                              const int *__restrict__ rows,
+                                              const int *__restrict__ rlen,
+                                              const int *__restrict__ gofs,
+                                              const int *__restrict__ ecol,
+                                              const T *__restrict__ eval,
+                                              const T *__restrict__ dgl,
+                                              const T *__restrict__ f, int b0,
+                                              const T *__restrict__ xold, T omega, T *xv, T *P)
+{
+    const int k = threadIdx.x & 7;
+    const int r = live ? rows[m.roff + i] : 0;
+    const int len = live ? rlen[m.roff + i] : 0;
+    const int e0 = live ? gofs[m.gbase + (i >> 6)] + (i & 63) : 0;
+    // the longest row among the wave's eight units bounds the chunk loop
+    int lmax = len;
+    for (int o = 8; o < 64; o <<= 1)
+        lmax = max(lmax, __shfl_xor(lmax, o, 64));
+    T sigma = 0;
+    constexpr int PU = RS_ROW8_CHUNK / 8;
+    for (int s0 = 0; s0 < lmax; s0 += RS_ROW8_CHUNK) {
+        T p[PU];
+#pragma unroll
+        for (int u = 0; u < PU; ++u) {
+            const int sq = s0 + k + 8 * u;
+            int c = 0;
+            T v = T(-0.0);
+            if (sq < len) {
+                c = ecol[e0 + 64 * sq];
+                v = eval[e0 + 64 * sq];
+            }
+            const T xc = sq >= len ? T(0) : (!BLOCK || c >= 0) ? xv[c] : xold[-1 - c];
+            p[u] = v * xc;
+        }
+#pragma unroll
+        for (int u = 0; u < PU; ++u)
+            P[k + 8 * u] = p[u];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (k == 0) {
+            // slots past the row's end hold -0.0 * 0 = -0.0, an exact
+            // identity: add in blocks of eight with the reads issued first
+            const int nq = (min(RS_ROW8_CHUNK, len - s0) + 7) & ~7;
+            for (int q = 0; q < nq; q += 8) {
+                T t[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    t[u] = P[q + u];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    sigma += t[u];
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
+    if (k == 0 && live) {
+        sigma = (f[b0 + r] - sigma) / dgl[m.roff + i];
+        const T xr = xv[r];
+        xv[r] = xr + omega * (sigma - xr);
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ void amg_ssor_sweep(int lb, int le, int step,
                                                const AmgLevelMeta *__restrict__ meta,
@@ -1013,13 +1080,23 @@ __device__ __forceinline__ void amg_ssor_sweep(int lb, int le, int step,
                                                const T *__restrict__ f, int b0,
                                                const T *__restrict__ xold, T omega, T *xv)
 {
+    __shared__ T P8[32 * RS_ROW8_CHUNK];
     const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     for (int l = lb; l != le; l += step) {
         const AmgLevelMeta m = meta[l];
-        const int ng = (m.count + 63) >> 6;
-        for (int g = wave; g < ng; g += nw)
-            amg_ssor_group<T, true>(m, g, rows, rlen, gofs, ecol, eval, dgl, f, b0, xold, omega,
-                                    xv);
+        if (m.pad) {
+            // eight lanes per row: blockDim / 8 rows at a time
+            const int unit = threadIdx.x >> 3, nu = blockDim.x >> 3;
+            const int nr = (m.count + nu - 1) / nu * nu;
+            for (int i = unit; i < nr; i += nu)
+                amg_ssor_row8<T, true>(m, i, i < m.count, rows, rlen, gofs, ecol, eval, dgl, f,
+                                       b0, xold, omega, xv, P8 + RS_ROW8_CHUNK * unit);
+        } else {
+            const int ng = (m.count + 63) >> 6;
+            for (int g = wave; g < ng; g += nw)
+                amg_ssor_group<T, true>(m, g, rows, rlen, gofs, ecol, eval, dgl, f, b0, xold,
+                                        omega, xv);
+        }
         __syncthreads();
     }
 }
@@ -1066,6 +1143,17 @@ __global__ void __launch_bounds__(256)
 {
     if (sk.on())
         return;
+    if (m.pad) {
+        __shared__ T P8[32 * RS_ROW8_CHUNK];
+        const int unit = threadIdx.x >> 3;
+        const int i = blockIdx.x * 32 + unit;
+        // a wave leaves together: its eight units share the chunk loop
+        if (__ballot(i < m.count) == 0)
+            return;
+        amg_ssor_row8<T, false>(m, i, i < m.count, rows, rlen, gofs, ecol, eval, dgl, f, 0, x,
+                                omega, x, P8 + RS_ROW8_CHUNK * unit);
+        return;
+    }
     const int g = blockIdx.x * 4 + (threadIdx.x >> 6);
     if ((g << 6) >= m.count)
         return;
@@ -1217,6 +1305,13 @@ static void greedy_colour(int n, const std::vector<int> &rp, const std::vector<i
     }
 }
 
+// RSGPU_AMG_ROW8=0 keeps one lane per row on every level
+static bool split_rows()
+{
+    const char *e = getenv("RSGPU_AMG_ROW8");
+    return !(e && e[0] == '0');
+}
+
 static void build_ssor_schedule(AmgLevel &L, int block_rows, int ordering, hipStream_t s)
 {
     const int n = L.n;
@@ -1293,7 +1388,11 @@ static void build_ssor_schedule(AmgLevel &L, int block_rows, int ordering, hipSt
             m.count = (int)R.size();
             m.gbase = (int)gofs.size();
             m.roff = (int)rows.size();
-            m.pad = 0;
+            // pad: 1 = eight lanes per row (long rows), 0 = one lane per row
+            long long tot = 0;
+            for (size_t t = 0; t < R.size(); ++t)
+                tot += len[t];
+            m.pad = (m.count > 0 && tot > 32LL * m.count && split_rows()) ? 1 : 0;
             for (int t0 = 0; t0 < m.count; t0 += 64) {
                 const int tn = std::min(64, m.count - t0);
                 const int w = len[ord[t0]];
@@ -1839,7 +1938,8 @@ static void amg_smooth(AmgLevel &L, const T *f, double wf, double wr, int revers
         auto launch = [&](int l, double w) {
             const AmgLevelMeta &m = L.h_meta[l];
             const int ng = (m.count + 63) >> 6;
-            hipLaunchKernelGGL(k_amg_ssor_level<T>, dim3(std::max(1, (ng + 3) / 4)), dim3(256),
+            const int nblk = m.pad ? (m.count + 31) / 32 : (ng + 3) / 4;
+            hipLaunchKernelGGL(k_amg_ssor_level<T>, dim3(std::max(1, nblk)), dim3(256),
                                0, s, m, L.rows.p, L.rlen.p, L.gofs.p, L.ell_col.p, A::ev(L),
                                A::dg(L), f, A::u(L), (T)w, sk);
         };
