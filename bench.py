@@ -109,6 +109,11 @@ def main():
     ap.add_argument("--inner-rtol", type=float, default=1e-3)
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-amg", action="store_true",
+                    help="skip the GMRES(30) + AMG legs")
+    ap.add_argument("--amg-ratio", type=float, default=0.5,
+                    help="AMG coarsening stops at a level whose aggregates keep more than this "
+                         "fraction of its rows (0: MTJ's rule)")
     ap.add_argument("--cpu-iters", type=int, default=300)
     ap.add_argument("--adjoint", action="store_true",
                     help="measure the adjoint gradient workload (64x64x32 well-rate mismatch) "
@@ -146,7 +151,8 @@ def main():
                     help="FP64 GMRES with MTJ's modified Gram-Schmidt (default: CGS2)")
     ap.add_argument("--mgs", action="store_true",
                     help="modified Gram-Schmidt in the FP32 inner GMRES (default: classical)")
-    ap.add_argument("--probe", choices=["spmv", "ilu", "ilu32", "solve", "mixed"], default=None,
+    ap.add_argument("--probe", choices=["spmv", "ilu", "ilu32", "solve", "mixed", "amg", "amg32"],
+                    default=None,
                     help="profiling probe: run only the named kernel(s) on the workload "
                          "(plus a 512 MiB device copy as the traffic reference) and exit")
     a = ap.parse_args()
@@ -351,13 +357,19 @@ def main():
         kt["sliced_format"] = {"chunks": nch, "slots": nslots, "tables": ntab}
     kt["cold_copies"] = NCOPY
 
+    # ---------------------------------------------------------- AMG
+    amg, xs_amg = (None, {}) if a.no_amg else bench_amg(a, A, b, n, dev)
+
     # ---------------------------------------------------------- CPU baseline
     cpu = None
     if not a.no_cpu:
         try:
-            cpu = cpu_baseline(A, pr, a, mon_rtol, st64.iterations, x64, xmx)
+            cpu = cpu_baseline(A, pr, a, mon_rtol, st64.iterations, x64, xmx, xs_amg)
         except Exception as e:  # the baseline must not hide the GPU result
             log(f"cpu baseline failed: {e!r}")
+        if amg and cpu:
+            for k in xs_amg:
+                amg[k]["rel_err_vs_fp64_oracle"] = cpu.pop(f"rel_err_amg_{k}", None)
 
     spmv_key = "spmv_sliced_fp64" if nch else "spmv_csr_fp64"
     il = kt["ilu_apply_fp32"]
@@ -391,6 +403,7 @@ def main():
                   "true_rel_residual": stmx.true_residual / stmx.b_norm,
                   "rel_err_vs_fp64_oracle": cpu and cpu.get("rel_err_mixed"),
                   "rel_err_vs_gpu_fp64": err_mx},
+        "amg": amg,
         "kernels": kt,
         "spmv": {"kernel": spmv_key, "cold_GBps": sp["cold_GBps"], "cold_us": sp["cold_us"],
                  "warm_us": sp["warm_us"], "frac": sp["cold_frac"],
@@ -1105,6 +1118,21 @@ def probe(a, A, n, nnz, lib, mtj, _lib, ctx, stream, dev):
         M.setMatrix(A)
         for _ in range(max(5, reps // 5)):
             lib.rsgpu_prec_apply(M.h, xin.data_ptr(), y.data_ptr())
+    elif a.probe in ("amg", "amg32"):
+        prec = _lib.FP32 if a.probe == "amg32" else _lib.FP64
+        M = mtj.AMG(precision=prec, zero_guess=1, coarsen_ratio=a.amg_ratio, max_coarse=8192,
+                    ctx=ctx)
+        M.setMatrix(A)
+        torch.cuda.synchronize()
+        if prec == _lib.FP32:
+            x32 = xin.float()
+            y32 = torch.zeros(n, dtype=torch.float32, device=dev)
+            for _ in range(5):
+                lib.rsgpu_prec_apply_f32(M.h, x32.data_ptr(), y32.data_ptr())
+        else:
+            for _ in range(5):
+                lib.rsgpu_prec_apply(M.h, xin.data_ptr(), y.data_ptr())
+        log(f"probe {a.probe}: levels {M.levels()}")
     elif a.probe == "ilu32":
         M = mtj.BlockILU(a.nblocks, precision=_lib.FP32, ctx=ctx)
         M.setMatrix(A)
@@ -1136,7 +1164,80 @@ def probe(a, A, n, nnz, lib, mtj, _lib, ctx, stream, dev):
     log(f"probe {a.probe}: n={n} nnz={nnz} reps={reps}; copy 512 MiB x3")
 
 
-def cpu_baseline(A, pr, a, mon_rtol, gpu_iters, x64, xmx):
+def bench_amg(a, A, b, n, dev):
+    """GMRES(30) + smoothed-aggregation AMG (AMG.java with the multicolour
+    SSOR smoother, zero initial guess per cycle) on the same system, FP64
+    and mixed (FP32 cycle with the FP64 coarsest solve inside FP64
+    refinement), with coarsening stopped where aggregates keep more than
+    --amg-ratio of a level's rows; and the cycle time of MTJ's own full
+    hierarchy.  Returns (results, GPU solutions by leg)."""
+    import torch
+    from ressim_amd import _lib, mtj
+    res, xs = {}, {}
+    p = dict(zero_guess=1, coarsen_ratio=a.amg_ratio, max_coarse=8192)
+
+    def cycle_ms(M, reps=5):
+        xa = torch.zeros(n, dtype=torch.float64, device=dev)
+        M.apply(b, xa)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            M.apply(b, xa)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t) / reps * 1e3
+
+    for leg, prec in (("fp64", _lib.FP64), ("mixed", _lib.FP32)):
+        M = mtj.AMG(precision=prec, **p)
+        M.setMatrix(A)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        M.setMatrix(A)
+        torch.cuda.synchronize()
+        setup = (time.perf_counter() - t) * 1e3
+        cyc = cycle_ms(M)
+        if prec == _lib.FP64:
+            S = mtj.GMRES(b, 30)
+            S.cgs = True
+            S.setIterationMonitor(mtj.DefaultIterationMonitor(2000, a.rtol, 1e-50, 1e5))
+        else:
+            S = mtj.MixedPrecisionSolver(b, inner="gmres", restart=30, inner_rtol=a.inner_rtol,
+                                         inner_max_iter=600, cgs=True)
+            S.setIterationMonitor(mtj.DefaultIterationMonitor(200, a.rtol, 1e-50, 1e5))
+        S.setPreconditioner(M)
+        xa = torch.zeros(n, dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        S.solve(A, b, xa)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t
+        ax = torch.empty_like(xa)
+        A.mult(xa, ax)
+        rel = float(torch.linalg.norm(b - ax) / torch.linalg.norm(b))
+        st = getattr(S, "last_stats", None)
+        lv = M.levels()
+        res[leg] = {"ms_per_solve": dt * 1e3, "setup_ms": setup, "cycle_ms": cyc,
+                    "levels": len(lv), "level_rows": [l[0] for l in lv],
+                    "iterations": S.getIterationMonitor().iterations(),
+                    "true_rel_residual": rel}
+        if st is not None and leg == "mixed":
+            res[leg]["inner_iterations"] = st.inner_iterations
+        xs[leg] = xa
+        log(f"AMG {leg}: {len(lv)} levels {[l[0] for l in lv]}, setup {setup:.1f} ms, cycle "
+            f"{cyc:.2f} ms, GMRES(30)+AMG {res[leg]['iterations']} iterations "
+            f"{dt * 1e3:.1f} ms")
+        del S, M
+    M = mtj.AMG(zero_guess=1)
+    M.setMatrix(A)
+    lv = M.levels()
+    res["mtj_hierarchy"] = {"levels": len(lv), "cycle_ms": cycle_ms(M, 3),
+                            "level_rows": [l[0] for l in lv]}
+    log(f"AMG with MTJ's full hierarchy: {len(lv)} levels, cycle "
+        f"{res['mtj_hierarchy']['cycle_ms']:.2f} ms")
+    res["params"] = dict(p, smoother="multicolour SSOR (omegas of AMG())")
+    return res, xs
+
+
+def cpu_baseline(A, pr, a, mon_rtol, gpu_iters, x64, xmx, extra=None):
     """The oracle's FP64 GMRES(30) + block-Jacobi ILU(0) on the same matrix
     and right-hand side, with the GPU FP64 solve's monitor tolerance, solved
     completely twice on the host cores: one thread (MTJ's concurrency; the
@@ -1166,7 +1267,9 @@ def cpu_baseline(A, pr, a, mon_rtol, gpu_iters, x64, xmx):
     log(f"cpu oracle 1 thread: {t1:.2f}s, {s1.iterations} iterations, true rel residual "
         f"{r1:.2e}; {threads} threads: {tn:.2f}s, {sn.iterations} iterations, {rn:.2e}")
     log(f"GPU vs FP64 oracle: FP64 {e64:.2e}, mixed {emx:.2e}")
-    return {"value": 1.0 / tn, "unit": "solves/s", "cores": threads, "kind": "port",
+    errs = {f"rel_err_amg_{k}": float(np.linalg.norm(v.cpu().numpy() - x1) / nx1)
+            for k, v in (extra or {}).items()}
+    return {**errs,"value": 1.0 / tn, "unit": "solves/s", "cores": threads, "kind": "port",
             "cpu_model": model, "nproc": os.cpu_count(),
             "sample": f"one complete FP64 GMRES(30) + block ILU(0) x{a.nblocks} solve of the "
                       f"same system (monitor rtol {mon_rtol:.3e}, as the GPU FP64 solve) by the "
