@@ -726,6 +726,7 @@ static void band_setup(rsgpu_prec *M, const std::vector<int> &rp, const std::vec
                        const std::vector<int> &dg)
 {
     M->band = false;
+    M->band_factor = false;
     M->band_slots = 0;
     M->band_dims.release();
     M->band_pos.release();
@@ -794,6 +795,13 @@ static void band_setup(rsgpu_prec *M, const std::vector<int> &rp, const std::vec
     }
     RS_HIP(hipStreamSynchronize(s));
     M->band = true;
+    // the line-wavefront factorisation: every block full (RSGPU_ILU_BAND_FACTOR=0
+    // keeps the level kernel)
+    const char *ef = getenv("RSGPU_ILU_BAND_FACTOR");
+    bool all_full = !(ef && ef[0] == '0') && M->kind != RSGPU_PREC_ILUT;
+    for (int b = 0; all_full && b < nb; ++b)
+        all_full = dims[b].y > 0;
+    M->band_factor = all_full;
 }
 
 // Transposed sweeps: U' (forward, subtract in ascending source row, divide)
@@ -1705,7 +1713,14 @@ __global__ void __launch_bounds__(64)
 
 // ---------------------------------------------------------------- host ops
 template <typename T>
-static void launch_factor(rsgpu_prec *M, T *lu, const double *aval)
+static void launch_factor_band(rsgpu_prec *M, T *lu);
+// status code of the line-wavefront factorisation on a zero pivot
+#define RS_BAND_FACTOR_PIVOT (-99)
+
+// band: the line-wavefront factorisation where it applies (full 9-point
+// blocks); its zero-pivot marker makes the caller run the level kernel
+template <typename T>
+static void launch_factor(rsgpu_prec *M, T *lu, const double *aval, bool band = true)
 {
     hipStream_t s = M->ctx->stream;
     const int nnz = M->nnz;
@@ -1716,6 +1731,10 @@ static void launch_factor(rsgpu_prec *M, T *lu, const double *aval)
     }
     int init[2] = {0, 0x7fffffff};
     RS_HIP(hipMemcpyAsync(M->status.p, init, sizeof(init), hipMemcpyHostToDevice, s));
+    if (band && M->band_factor) {
+        launch_factor_band<T>(M, lu);
+        return;
+    }
     if (M->cell_factor) {
         hipLaunchKernelGGL(k_ilu_factor_cells<T>, dim3(M->nblocks), dim3(64), 0, s,
                            M->bstart.p, M->L.blk_lev.p, (const LevelMeta *)M->L.lev_meta.p,
@@ -1795,6 +1814,38 @@ int prec_factor(rsgpu_prec *M, rsgpu_csr *A, int *bad_row)
 {
     hipStream_t s = M->ctx->stream;
     int st[2];
+    if (M->band_factor) {
+        // a zero pivot in the line-wavefront factorisation: the level kernel
+        // factors again and reports it as ILU.factor does
+        if (M->precision == RSGPU_FP64)
+            launch_factor<double>(M, M->lu64.p, A->val.p);
+        else
+            launch_factor<float>(M, M->lu32.p, A->val.p);
+        RS_HIP(hipMemcpyAsync(st, M->status.p, sizeof(st), hipMemcpyDeviceToHost, s));
+        RS_HIP(hipStreamSynchronize(s));
+        const bool again = st[0] == RS_BAND_FACTOR_PIVOT;
+        if (M->precision == RSGPU_FP64) {
+            if (again)
+                launch_factor<double>(M, M->lu64.p, A->val.p, false);
+            prec_refresh_fp64(M);
+        } else {
+            if (again)
+                launch_factor<float>(M, M->lu32.p, A->val.p, false);
+            launch_ell<float>(s, M->L, M->lu32.p, M->ell32_L.p, true);
+            launch_ell<float>(s, M->U, M->lu32.p, M->ell32_U.p, true);
+            if (!M->cell)
+                launch_rowdiag<float>(s, M->U, M->diag.p, M->lu32.p, M->dgU32.p);
+            launch_band_gather<float>(M, M->lu32.p, M->band32.p);
+        }
+        RS_HIP(hipMemcpyAsync(st, M->status.p, sizeof(st), hipMemcpyDeviceToHost, s));
+        RS_HIP(hipStreamSynchronize(s));
+        if (st[0] != 0) {
+            if (bad_row)
+                *bad_row = st[1];
+            return st[0];
+        }
+        return RSGPU_OK;
+    }
     if (M->precision == RSGPU_FP64) {
         launch_factor<double>(M, M->lu64.p, A->val.p);
         launch_ell<double>(s, M->L, M->lu64.p, M->ell64_L.p, true);
@@ -1951,6 +2002,199 @@ __device__ __forceinline__ double dpp_mov(double v)
 }
 #define RS_DPP_WAVE_SHR1 0x138   // lane k gets lane k-1 (lane 0: 0)
 #define RS_DPP_WAVE_SHL1 0x130   // lane k gets lane k+1 (lane 63: 0)
+
+// ---------------------------------------------------------------- line-wavefront factorisation
+// ILU.factor (ILU.java:86-125) on a full 9-point block of Ly lines of Lx
+// rows (every row holds all of its in-block neighbours), one wave per
+// block, lane j owning line j: row (x, j) at step s = x + 2j, as the L
+// sweep.  The rows its lower entries point at, (x-1, j-1), (x, j-1),
+// (x+1, j-1) and (x-1, j), were finished by lane j-1 three, two and one
+// steps earlier and by itself one step earlier; each step publishes its
+// row's diagonal and four upper entries, shifted to lane j+1 with a DPP
+// wave shift.  Per row the statement order of ILU.factor: for each lower
+// entry in ascending column, divide by that row's pivot, then subtract its
+// multiple of that row's upper entries from the entries of this row in the
+// same columns (ascending), so factors are bit-identical to the level
+// kernel's.  Records: 9 values per row (L0..L3, diagonal, U0..U3 in
+// ascending column; zero where absent), padded by RS_BAND_PAD rows.
+#define RS_BF_D 8
+
+template <typename T>
+struct BandIn {
+    T v[9];
+};
+
+template <typename T>
+__global__ void k_band_in(int n, const int *__restrict__ pos, const T *__restrict__ lu,
+                          T *__restrict__ rec)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n)
+        return;
+    const int *p = pos + (size_t)9 * r;
+    T *o = rec + (size_t)9 * (r + RS_BAND_PAD);
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+        o[k] = p[k] >= 0 ? lu[p[k]] : T(0);
+}
+
+template <typename T>
+__global__ void k_band_out(int n, const int *__restrict__ pos, const T *__restrict__ rec,
+                           T *__restrict__ lu)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n)
+        return;
+    const int *p = pos + (size_t)9 * r;
+    const T *o = rec + (size_t)9 * (r + RS_BAND_PAD);
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+        if (p[k] >= 0)
+            lu[p[k]] = o[k];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64)
+    k_ilu_factor_band(const int *__restrict__ bstart, const int2 *__restrict__ dims,
+                      const T *__restrict__ rin, T *__restrict__ rout, int *status)
+{
+    const int blk = blockIdx.x;
+    const int b0 = bstart[blk], Lx = dims[blk].x, Ly = dims[blk].y;
+    const int j = threadIdx.x;
+    const bool line = j < Ly;
+    const int jj = min(j, Ly - 1);
+    const int S = Lx + 2 * Ly - 2;
+    // lane j's row at step s is b0 + jj Lx + s - 2 jj (records are padded,
+    // so steps before and after the line read other rows or padding)
+    const ptrdiff_t rb = (ptrdiff_t)b0 + (ptrdiff_t)jj * Lx - 2 * jj;
+    const T *rp = rin + 9 * (rb + RS_BAND_PAD);
+    T *wp = rout + 9 * (rb + RS_BAND_PAD);
+    T *junk = rout;                       // the first padding row
+    const bool js = jj > 0, jn = jj < Ly - 1;
+    BandIn<T> ring[RS_BF_D];
+#pragma unroll
+    for (int k = 0; k < RS_BF_D; ++k) {
+        ring[k] = *reinterpret_cast<const BandIn<T> *>(rp + 9 * k);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    // published (diagonal, U0..U3) of lane j-1's last three rows and of this
+    // lane's previous row
+    T h1[5], h2[5], h3[5], pv[5];
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+        h1[q] = h2[q] = h3[q] = pv[q] = T(0);
+    bool bad = false;
+    int s0 = 0;
+    do {
+#pragma unroll
+        for (int k = 0; k < RS_BF_D; ++k) {
+            const int s = s0 + k;
+            const int x = s - 2 * jj;
+            const bool act = line && x >= 0 && x < Lx;
+            const bool w = x > 0, e = x < Lx - 1;
+            T a[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q)
+                a[q] = ring[k].v[q];
+            // lower entry L0 = (x-1, j-1): row h3
+            const bool p0 = w && js, p1 = js, p2 = e && js, p3 = w;
+            T m;
+            m = a[0] / h3[0];
+            bad |= act && p0 && h3[0] == T(0);
+            a[0] = p0 ? m : a[0];
+            {
+                const T t1 = a[1] - a[0] * h3[1];   // (x, j-1)   <- U0 of L0
+                const T t3 = a[3] - a[0] * h3[3];   // (x-1, j)   <- U2 of L0
+                const T t4 = a[4] - a[0] * h3[4];   // (x, j)     <- U3 of L0
+                a[1] = p0 && p1 ? t1 : a[1];
+                a[3] = p0 && p3 ? t3 : a[3];
+                a[4] = p0 ? t4 : a[4];
+            }
+            // L1 = (x, j-1): row h2
+            m = a[1] / h2[0];
+            bad |= act && p1 && h2[0] == T(0);
+            a[1] = p1 ? m : a[1];
+            {
+                const T t2 = a[2] - a[1] * h2[1];   // (x+1, j-1) <- U0 of L1
+                const T t3 = a[3] - a[1] * h2[2];   // (x-1, j)   <- U1 of L1
+                const T t4 = a[4] - a[1] * h2[3];   // (x, j)     <- U2 of L1
+                const T t5 = a[5] - a[1] * h2[4];   // (x+1, j)   <- U3 of L1
+                a[2] = p1 && p2 ? t2 : a[2];
+                a[3] = p1 && p3 ? t3 : a[3];
+                a[4] = p1 ? t4 : a[4];
+                a[5] = p1 && e ? t5 : a[5];
+            }
+            // L2 = (x+1, j-1): row h1
+            m = a[2] / h1[0];
+            bad |= act && p2 && h1[0] == T(0);
+            a[2] = p2 ? m : a[2];
+            {
+                const T t4 = a[4] - a[2] * h1[2];   // (x, j)     <- U1 of L2
+                const T t5 = a[5] - a[2] * h1[3];   // (x+1, j)   <- U2 of L2
+                a[4] = p2 ? t4 : a[4];
+                a[5] = p2 && e ? t5 : a[5];
+            }
+            // L3 = (x-1, j): this lane's previous row
+            m = a[3] / pv[0];
+            bad |= act && p3 && pv[0] == T(0);
+            a[3] = p3 ? m : a[3];
+            {
+                const T t4 = a[4] - a[3] * pv[1];   // (x, j)     <- U0 of L3
+                const T t6 = a[6] - a[3] * pv[3];   // (x-1, j+1) <- U2 of L3
+                const T t7 = a[7] - a[3] * pv[4];   // (x, j+1)   <- U3 of L3
+                a[4] = p3 ? t4 : a[4];
+                a[6] = p3 && jn ? t6 : a[6];
+                a[7] = p3 && jn ? t7 : a[7];
+            }
+            T *o = act ? wp + 9 * s : junk;
+#pragma unroll
+            for (int q = 0; q < 9; ++q)
+                o[q] = a[q];
+            T cur[5] = {a[4], a[5], a[6], a[7], a[8]};
+#pragma unroll
+            for (int q = 0; q < 5; ++q) {
+                cur[q] = act ? cur[q] : T(0);
+                h3[q] = h2[q];
+                h2[q] = h1[q];
+                h1[q] = dpp_mov<RS_DPP_WAVE_SHR1>(cur[q]);
+                pv[q] = cur[q];
+            }
+            ring[k] = *reinterpret_cast<const BandIn<T> *>(rp + 9 * (s + RS_BF_D));
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        s0 += RS_BF_D;
+    } while (s0 < S);
+    if (__ballot(bad) && threadIdx.x == 0)
+        atomicExch(status, RS_BAND_FACTOR_PIVOT);
+}
+
+template <typename T>
+static void launch_factor_band(rsgpu_prec *M, T *lu)
+{
+    hipStream_t s = M->ctx->stream;
+    const size_t np = (size_t)M->n + 2 * RS_BAND_PAD;
+    DevBuf<T> *inb, *outb;
+    if constexpr (sizeof(T) == 8) {
+        inb = &M->bf_in64;
+        outb = &M->bf_out64;
+    } else {
+        inb = &M->bf_in32;
+        outb = &M->bf_out32;
+    }
+    DevBuf<T> &in = *inb, &out = *outb;
+    if (in.n != 9 * np) {
+        in.alloc(9 * np);
+        in.zero(s);
+        out.alloc(9 * np);
+        out.zero(s);
+    }
+    const int g = (M->n + 255) / 256;
+    hipLaunchKernelGGL(k_band_in<T>, dim3(g), dim3(256), 0, s, M->n, M->band_pos.p, lu, in.p);
+    hipLaunchKernelGGL(k_ilu_factor_band<T>, dim3(M->nblocks), dim3(64), 0, s, M->bstart.p,
+                       M->band_dims.p, in.p, out.p, M->status.p);
+    hipLaunchKernelGGL(k_band_out<T>, dim3(g), dim3(256), 0, s, M->n, M->band_pos.p, out.p, lu);
+    RS_CHECK_LAUNCH();
+}
 
 template <typename T>
 struct alignas(16) BandRec {

@@ -45,6 +45,7 @@ struct rsgpu_solver {
     std::vector<double> h_hist;
     hipEvent_t ev[2] = {nullptr, nullptr};
     int *flags_host = nullptr;    // pinned [2][4]
+    int *flags_dev = nullptr;     // the same memory as the device addresses it
     // the matrix whose FP64 sliced copy this solve filled (its SpMVs then
     // stream the sliced copy)
     const rsgpu_csr *sl64 = nullptr;
@@ -1462,6 +1463,18 @@ void state_init(rsgpu_solver *S, DevState *dst, DevBuf<double> &hist,
 
 RedSlot slot_of(rsgpu_solver *S) { return RedSlot{S->partials.p, S->ticket.p}; }
 
+// The solver's two stop flags into the host's pinned buffer: one launch in
+// place of two 4-byte copies (each a copy kernel of its own); the host reads
+// them after the event recorded behind this kernel.
+__global__ void k_post_flags(const DevState *__restrict__ st, int *dst)
+{
+    if (threadIdx.x == 0) {
+        const int d = st->done, c = st->cycle_end;
+        __hip_atomic_store(dst, d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(dst + 1, c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 // host view of the flags, read one step behind
 struct Poller {
     rsgpu_solver *S;
@@ -1471,11 +1484,9 @@ struct Poller {
     void post()
     {
         hipStream_t s = S->ctx->stream;
-        int *dst = S->flags_host + 4 * (k & 1);
-        RS_HIP(hipMemcpyAsync(dst, &st->done, sizeof(int) * 1,
-                              hipMemcpyDeviceToHost, s));
-        RS_HIP(hipMemcpyAsync(dst + 1, &st->cycle_end, sizeof(int),
-                              hipMemcpyDeviceToHost, s));
+        hipLaunchKernelGGL(k_post_flags, dim3(1), dim3(64), 0, s, st,
+                           S->flags_dev + 4 * (k & 1));
+        RS_CHECK_LAUNCH();
         RS_HIP(hipEventRecord(S->ev[k & 1], s));
         k++;
     }
@@ -2046,6 +2057,7 @@ extern "C" int rsgpu_solver_create(rsgpu_ctx *ctx, const rsgpu_solver_opts *opts
         RS_HIP(hipEventCreateWithFlags(&S->ev[1], hipEventDisableTiming));
         RS_HIP(hipHostMalloc((void **)&S->flags_host, sizeof(int) * 8,
                              hipHostMallocDefault));
+        RS_HIP(hipHostGetDevicePointer((void **)&S->flags_dev, S->flags_host, 0));
         RS_HIP(hipStreamSynchronize(ctx->stream));
     } catch (const RsError &e) {
         rsgpu_solver_destroy(S);

@@ -301,6 +301,11 @@ struct rsgpu_prec {
     // elements of the largest full block's per-line slot layout (Ly lines of
     // round_up(Lx + 2 Ly - 2, 16) slots)
     int band_slots = 0;
+    // the line-wavefront factorisation (every block full): its 9-value
+    // input and output records per row, padded like the sweep records
+    bool band_factor = false;
+    DevBuf<double> bf_in64, bf_out64;
+    DevBuf<float> bf_in32, bf_out32;
     ~rsgpu_prec()
     {
         amg_destroy(amg);

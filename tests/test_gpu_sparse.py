@@ -366,6 +366,77 @@ def test_block_ilu_line_wavefront(gpu, oracle, monkeypatch, precision, shape):
     np.testing.assert_array_equal(out[1], want)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("shape", [(64, 64, 1), (64, 64, 16), (37, 50, 5), (128, 64, 2),
+                                   (9, 7, 1), (3, 2, 1)])
+def test_block_ilu_line_wavefront_factor(gpu, oracle, monkeypatch, precision, shape):
+    """ILU(0) factorisation of full 9-point blocks by the line wavefront:
+    the factor is the oracle's (ILU.java:86-125), bit for bit, FP64 and
+    FP32, and the same bits as the level-scheduled factorisation
+    (RSGPU_ILU_BAND_FACTOR=0)."""
+    from ressim_amd import _lib
+    nx, ny, nb = shape
+    s = oracle_system(oracle, P.config1(nx=nx, ny=ny))
+    rp, ci, d = s["rowptr"], s["colind"], s["data"]
+    n = len(rp) - 1
+    brp, bci, bd, _ = oracle.block_filter(rp, ci, d, nb)
+    if precision == "fp64":
+        st, want, _, _ = oracle.ilu_factor(brp, bci, bd)
+    else:
+        st, want, _, _ = oracle.ilu_factor_f32(brp, bci, bd.astype(np.float32))
+        want = want.astype(np.float64)
+    assert st == 0
+    print(f"band factor {precision} {nx}x{ny} blocks={nb}", flush=True)
+    for flag in ("1", "0"):
+        monkeypatch.setenv("RSGPU_ILU_BAND_FACTOR", flag)
+        A = _mat(gpu, rp, ci, d, n, n)
+        M = gpu.BlockILU(nb, precision=_lib.FP64 if precision == "fp64" else _lib.FP32)
+        M.setMatrix(A)
+        assert M.info()["sweep_form"] == "lines"
+        rp_g, ci_g, lu_g = M.factors()
+        np.testing.assert_array_equal(rp_g, brp)
+        np.testing.assert_array_equal(ci_g, bci)
+        assert np.array_equal(lu_g.view(np.int64), want.view(np.int64)), flag
+        # a second factorisation on new values (same pattern) reuses the plan
+        d2 = d * (1.0 + 0.25 * np.random.default_rng(nx + nb).standard_normal(d.size))
+        A2 = _mat(gpu, rp, ci, d2, n, n)
+        M.setMatrix(A2)
+        _, _, bd2, _ = oracle.block_filter(rp, ci, d2, nb)
+        if precision == "fp64":
+            _, want2, _, _ = oracle.ilu_factor(brp, bci, bd2)
+        else:
+            _, want2, _, _ = oracle.ilu_factor_f32(brp, bci, bd2.astype(np.float32))
+            want2 = want2.astype(np.float64)
+        assert np.array_equal(M.factors()[2].view(np.int64), want2.view(np.int64)), flag
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_block_ilu_line_wavefront_zero_pivot(gpu, oracle, monkeypatch, precision):
+    """A zero pivot inside a full 9-point block: the line-wavefront
+    factorisation hands the block set back to the level kernel, which
+    reports the reference's error on the reference's row."""
+    from ressim_amd import _lib
+    s = oracle_system(oracle, P.config1(nx=16, ny=12))
+    rp, ci, d = s["rowptr"], s["colind"], s["data"].copy()
+    n = len(rp) - 1
+    r = 16 * 5 + 7
+    # row r's lower entries and diagonal set to zero: its elimination
+    # subtracts exact zeros, so U(r,r) = 0 and row r + 1 meets a zero pivot
+    lo = rp[r] + np.nonzero(ci[rp[r]:rp[r + 1]] <= r)[0]
+    d[lo] = 0.0
+    st, _, _, _ = oracle.ilu_factor(rp, ci, d)
+    assert st != 0
+    msgs = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("RSGPU_ILU_BAND_FACTOR", flag)
+        A = _mat(gpu, rp, ci, d, n, n)
+        M = gpu.BlockILU(1, precision=_lib.FP64 if precision == "fp64" else _lib.FP32)
+        with pytest.raises(RuntimeError, match="Zero pivot") as ei:
+            M.setMatrix(A)
+        msgs.append(str(ei.value))
+    assert msgs[0] == msgs[1]
+
+
 @pytest.mark.parametrize("kernel", ["pf", "generic"])
 @pytest.mark.parametrize("nblocks", [1, 3, 16])
 def test_block_ilu_sweep_kernels(gpu, oracle, systems, monkeypatch, kernel, nblocks):
