@@ -333,6 +333,11 @@ int rsgpu_solver_solve_host(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M,
                             rsgpu_stats *st);
 /* Residuals the monitor saw, in order (IterationReporter.monitor replay). */
 int rsgpu_solver_history(const rsgpu_solver *S, double *out_host, int cap);
+/* GMRES inner steps of the last solve enqueued as one fused launch (sliced
+ * SpMV + line-wavefront block ILU(0) apply + first Gram-Schmidt pass; used
+ * with the cgs option on full line blocks, RSGPU_GMRES_FUSED=0 in the
+ * environment turns it off).  No reference counterpart. */
+int rsgpu_solver_fused_steps(const rsgpu_solver *S);
 
 /* ---- Fully-implicit black-oil Jacobian (no reference counterpart) -------- */
 /* rs is IMPES (numerics/Discretisation.java:208-225 solves pressure only);

@@ -32,7 +32,7 @@
 #include <cstdlib>
 #include <cstring>
 
-#include "rsgpu_impl.hpp"
+#include "spmv_core.hpp"
 
 #define RS_LDS_BYTES 65536
 
@@ -728,6 +728,7 @@ static void band_setup(rsgpu_prec *M, const std::vector<int> &rp, const std::vec
     M->band = false;
     M->band_factor = false;
     M->band_slots = 0;
+    M->h_band_dims.clear();
     M->band_dims.release();
     M->band_pos.release();
     M->band_mask.release();
@@ -777,6 +778,7 @@ static void band_setup(rsgpu_prec *M, const std::vector<int> &rp, const std::vec
             M->band_slots = std::max(M->band_slots, Ly * band_slot_pitch(Lx, Ly));
     }
     hipStream_t s = M->ctx->stream;
+    M->h_band_dims = dims;
     M->band_dims.upload(dims, s);
     M->band_pos.upload(pos, s);
     // masks and records padded by RS_BAND_PAD zero rows on both sides
@@ -1981,9 +1983,11 @@ static int ilu_kernel_choice()
 // statement order of UnitLowerCompRowMatrix.solve and
 // UpperCompRowMatrix.solve, bit for bit.  The block's vector lives in LDS
 // (right-hand side, then y, then x, each written only by the lane that owns
-// the row); each lane's rows are consecutive, so the row records (8 values
-// plus diagonal and mask) are prefetched RS_BAND_D steps ahead in a register
-// ring.
+// the row); each lane's rows are consecutive, so the row records (the
+// sweep's 4 values of the row's 8, the diagonal and mask) are prefetched
+// RS_BAND_D steps ahead in a register ring.  The L and U values of a row
+// share one 32-byte record: the L sweep's fetch of a line brings the U
+// halves into the last-level cache for the U sweep.
 #define RS_BAND_D 8
 
 template <int CTRL>
@@ -2202,6 +2206,16 @@ struct alignas(16) BandRec {
 };
 
 
+// Progress hooks of the wavefront sweeps.  ready(b): every row (i, j) of
+// the block with i + 2 j < b holds its right-hand side (called by L sweeps
+// before they read those rows); done(t): the first t steps of a U sweep
+// are written (called after each round).  The plain apply needs neither;
+// the fused GMRES step has other waves produce and consume the vector.
+struct NoHook {
+    __device__ __forceinline__ void ready(int) {}
+    __device__ __forceinline__ void done(int) {}
+};
+
 // Global loads in flight: the ring depth D (steps ahead), within the 63
 // loads the wave's counter tracks and about 200 registers of ring.
 template <typename T, bool UPPER, bool FULL>
@@ -2235,9 +2249,10 @@ __device__ __forceinline__ void band_load(const T *rp, const T *dp, const unsign
         st.m = mp[q];
 }
 
-template <typename T, bool UPPER, bool FULL>
+template <typename T, bool UPPER, bool FULL, class Hook = NoHook>
 __device__ __forceinline__ void band_sweep(int Lx, int Ly, int g0, const T *rec, const T *dgl,
-                                           const unsigned char *mask, T *xv, int junk)
+                                           const unsigned char *mask, T *xv, int junk,
+                                           Hook hk = Hook())
 {
     constexpr int D = band_depth<T, UPPER, FULL>();
     constexpr int dir = UPPER ? -1 : 1;
@@ -2248,7 +2263,7 @@ __device__ __forceinline__ void band_sweep(int Lx, int Ly, int g0, const T *rec,
     const int i0 = UPPER ? Lx - 1 + 2 * (Ly - 1 - jj) : -2 * jj;
     const int rb = jj * Lx + i0;
     const bool js = jj > 0, jn = jj < Ly - 1;
-    const T *rp = rec + (size_t)8 * (g0 + rb) + (UPPER ? 4 : 0);
+    const T *rp = rec + (size_t)8 * (g0 + rb);
     const T *dp = dgl + (g0 + rb);
     const unsigned char *mp = mask + (g0 + rb);
     const T *xr = xv + rb;
@@ -2266,9 +2281,14 @@ __device__ __forceinline__ void band_sweep(int Lx, int Ly, int g0, const T *rec,
         __builtin_amdgcn_sched_barrier(0);
     }
     T cur = T(0), h1 = T(0), h2 = T(0), h3 = T(0);
+    if (!UPPER)
+        hk.ready(1);
     T rhs = xr[0];
     int s0 = 0;
     do {
+        // step s reads the right-hand side of step s + 1
+        if (!UPPER)
+            hk.ready(s0 + D + 1);
         // base pointers of this round (every access below has an immediate
         // offset from them)
         const T *rps = rp + (ptrdiff_t)8 * dir * s0;
@@ -2329,6 +2349,8 @@ __device__ __forceinline__ void band_sweep(int Lx, int Ly, int g0, const T *rec,
             __builtin_amdgcn_sched_barrier(0);
         }
         s0 += D;
+        if (UPPER)
+            hk.done(s0);
     } while (s0 < S);
 }
 
@@ -2369,9 +2391,10 @@ struct FullStage {
     DT d;
 };
 
-template <typename T, bool UPPER, bool NOPRED>
+template <typename T, bool UPPER, bool NOPRED, class Hook = NoHook>
 __device__ __forceinline__ void band_sweep_full(int Lx, int Ly, int SP, int g0, const T *rec,
-                                                const T *dgl, const double *rdl, T *xs)
+                                                const T *dgl, const double *rdl, T *xs,
+                                                Hook hk = Hook())
 {
     constexpr int D = 16;
     static_assert(band_depth<T, UPPER, true>() == D, "full-block rounds are 16 steps");
@@ -2386,7 +2409,7 @@ __device__ __forceinline__ void band_sweep_full(int Lx, int Ly, int SP, int g0, 
     // line position of step 0 (i = slot - 2 j; U step t uses slot SP - 1 - t)
     const int i0 = UPPER ? SP - 1 - 2 * jj : -2 * jj;
     const int rb = jj * Lx + i0;
-    const T *rp = rec + (size_t)8 * (g0 + rb) + (UPPER ? 4 : 0);
+    const T *rp = rec + (size_t)8 * (g0 + rb);
     using DT = typename FullStage<T, UPPER>::DT;
     const DT *dp = RCP ? (const DT *)(rdl + (g0 + rb)) : (const DT *)(dgl + (g0 + rb));
     VT *xl = reinterpret_cast<VT *>(xs + (size_t)jj * SP);
@@ -2405,6 +2428,9 @@ __device__ __forceinline__ void band_sweep_full(int Lx, int Ly, int SP, int g0, 
     // round r's slots: L [16 r, 16 r + 16), U [SP - 16 (r + 1), SP - 16 r)
     auto slot0 = [&](int r) { return UPPER ? SP - D * (r + 1) : D * r; };
     VT rin[NB], rnext[NB];
+    // L round r reads slots [16 r, 16 r + 16) (rows i + 2 j < 16 (r + 1))
+    if (!UPPER)
+        hk.ready(D);
 #pragma unroll
     for (int u = 0; u < NB; ++u)
         rin[u] = xl[slot0(0) / NV + u];
@@ -2412,6 +2438,8 @@ __device__ __forceinline__ void band_sweep_full(int Lx, int Ly, int SP, int g0, 
     int r = 0;
     do {
         const int rn = min(r + 1, rounds - 1);
+        if (!UPPER)
+            hk.ready(D * (rn + 1));
 #pragma unroll
         for (int u = 0; u < NB; ++u)
             rnext[u] = xl[slot0(rn) / NV + u];
@@ -2473,6 +2501,8 @@ __device__ __forceinline__ void band_sweep_full(int Lx, int Ly, int SP, int g0, 
             for (int u = 0; u < NB; ++u)
                 xl[slot0(r) / NV + u] = *reinterpret_cast<const VT *>(&outv[u * NV]);
         }
+        if (UPPER)
+            hk.done(D * (r + 1));
 #pragma unroll
         for (int u = 0; u < NB; ++u)
             rin[u] = rnext[u];
@@ -2483,7 +2513,8 @@ __device__ __forceinline__ void band_sweep_full(int Lx, int Ly, int SP, int g0, 
 template <typename T>
 __global__ void __launch_bounds__(64)
     k_ilu_apply_band(const int *__restrict__ bstart, const int2 *__restrict__ dims,
-                     const unsigned char *__restrict__ mask, const T *__restrict__ rec,
+                     const unsigned char *__restrict__ mask, const T *__restrict__ recL,
+                     const T *__restrict__ recU,
                      const T *__restrict__ dgl, const double *__restrict__ rdl,
                      const T *__restrict__ b, T *__restrict__ x, const int *skip1,
                      const int *skip2, BlkSkip bskip)
@@ -2521,9 +2552,9 @@ __global__ void __launch_bounds__(64)
                         xs[(j0 + u) * SP + i + 2 * (j0 + u)] = v[u];
             }
         __syncthreads();
-        band_sweep_full<T, false, true>(Lx, Ly, SP, b0, rec, dgl, rdl, xs);
+        band_sweep_full<T, false, true>(Lx, Ly, SP, b0, recL, dgl, rdl, xs);
         __syncthreads();
-        band_sweep_full<T, true, true>(Lx, Ly, SP, b0, rec, dgl, rdl, xs);
+        band_sweep_full<T, true, true>(Lx, Ly, SP, b0, recU, dgl, rdl, xs);
         __syncthreads();
         {
             int jl = 0, il = (int)threadIdx.x;
@@ -2554,13 +2585,13 @@ __global__ void __launch_bounds__(64)
     // outside the block, whose values are never used)
     const int junk = nb + RS_BAND_PAD - 1;
     if (d.y > 0) {
-        band_sweep<T, false, true>(Lx, Ly, b0, rec, dgl, mask, xv, junk);
+        band_sweep<T, false, true>(Lx, Ly, b0, recL, dgl, mask, xv, junk);
         __syncthreads();
-        band_sweep<T, true, true>(Lx, Ly, b0, rec, dgl, mask, xv, junk);
+        band_sweep<T, true, true>(Lx, Ly, b0, recU, dgl, mask, xv, junk);
     } else {
-        band_sweep<T, false, false>(Lx, Ly, b0, rec, dgl, mask, xv, junk);
+        band_sweep<T, false, false>(Lx, Ly, b0, recL, dgl, mask, xv, junk);
         __syncthreads();
-        band_sweep<T, true, false>(Lx, Ly, b0, rec, dgl, mask, xv, junk);
+        band_sweep<T, true, false>(Lx, Ly, b0, recU, dgl, mask, xv, junk);
     }
     __syncthreads();
     for (int r = threadIdx.x; r < nb; r += 64)
@@ -2581,15 +2612,449 @@ static void launch_band(rsgpu_prec *M, const T *rec, const T *b, T *x, const int
                                    lds_limit(M->ctx->device)));
         attr = true;
     }
-    // padded layout: rows -PAD .. n+PAD-1 of 8 values, then the diagonals
+    // padded layout: rows -PAD .. n+PAD-1 of 8 values (4 L, then 4 U), then
+    // the diagonals
     const size_t np = (size_t)M->n + 2 * RS_BAND_PAD;
     const double *rdl = (sizeof(T) == 4 && M->band_rd.p) ? M->band_rd.p + RS_BAND_PAD : nullptr;
     hipLaunchKernelGGL(k_ilu_apply_band<T>, dim3(M->nblocks), dim3(64), shm, M->ctx->stream,
                        M->bstart.p, M->band_dims.p, M->band_mask.p + RS_BAND_PAD,
-                       rec + (size_t)8 * RS_BAND_PAD, rec + 8 * np + RS_BAND_PAD, rdl, b, x,
+                       rec + (size_t)8 * RS_BAND_PAD, rec + (size_t)8 * RS_BAND_PAD + 4,
+                       rec + 8 * np + RS_BAND_PAD, rdl, b, x,
                        sk1, sk2, M->skip);
     RS_CHECK_LAUNCH();
 }
+
+// ---------------------------------------------------------------- fused GMRES step
+// One inner step of GMRES with block-Jacobi ILU(0) on full line blocks
+// (every block Ly lines of Lx rows, Lx a multiple of 64, block starts on
+// 64-row boundaries) and a sliced matrix copy, in one launch:
+//   u = A v_i (for i > 0, v_i = (1 / H(i,i-1)) w is formed on the fly and
+//       stored, the rule of k_spmv_scale_sl),
+//   w' = M^-1 u (the line-wavefront L and U sweeps),
+//   H(0..i, i) = V' w' (the first projection pass of classical
+//       Gram-Schmidt; column i taken from v_i as formed above).
+// One workgroup per block.  Wave 0 runs the two sweeps at raised issue
+// priority; the producer waves (fuse_producers) compute the block's SpMV rows, 64 rows
+// of one line at a time, directly into the sweep's LDS vector, in the order
+// the L sweep reaches them (the sweep waits on per-task flags only when it
+// catches up), and while the U sweep runs they store the finished rows of
+// w' and accumulate their projections on the basis.  Each SpMV row and each
+// sweep result is that of the separate kernels bit for bit (same per-row
+// operation order); the projections are summed per lane over its rows, over
+// lanes and producer waves in a fixed order, then over blocks in block
+// order, so they are reproducible run to run (their rounding differs from
+// the unfused kernel's grid-stride order).
+// producer waves: 11 in FP32 (168 registers per wave, three waves per SIMD);
+// 7 in FP64, whose larger sweep state needs more than 168
+template <typename T>
+constexpr int fuse_producers()
+{
+    return sizeof(T) == 4 ? 11 : 7;
+}
+template <typename T>
+constexpr int fuse_threads()
+{
+    return 64 * (1 + fuse_producers<T>());
+}
+#define RS_FUSE_COLS 32
+#define RS_FUSE_SPIN (1 << 24)
+
+// LDS progress between the waves of a fused step.  Flags are written with
+// workgroup-scope release and read with acquire; the U sweep's step count is
+// published with a plain store after its round's LDS writes (one wave's LDS
+// operations complete in issue order) so the sweep never waits for them.
+struct FuseHook {
+    int *flags;    // phase-A task t = h Ly + j (rows 64 h .. 64 h + 63 of line j) done
+    int *uprog;    // U steps completed
+    int *err;      // a bounded wait ran out
+    int ntA, Ly;
+    __device__ __forceinline__ void ready(int bound)
+    {
+        const int lane = threadIdx.x & 63;
+        for (int spin = 0;
+             !__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP); ++spin) {
+            bool miss = false;
+            for (int t = lane; t < ntA; t += 64) {
+                const int h = t / Ly, j = t - h * Ly;
+                const int f = __hip_atomic_load(&flags[t], __ATOMIC_ACQUIRE,
+                                                __HIP_MEMORY_SCOPE_WORKGROUP);
+                miss |= (64 * h + 2 * j < bound) && f == 0;
+            }
+            if (!__any(miss))
+                return;
+            if (spin >= RS_FUSE_SPIN) {
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                return;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __device__ __forceinline__ void done(int steps)
+    {
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        if ((threadIdx.x & 63) == 0)
+            __hip_atomic_store(uprog, steps, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    }
+};
+
+// keeps a loaded 16-byte vector live at this point (its load is issued
+// before any code after it)
+__device__ __forceinline__ void fuse_pin(const float4 &v)
+{
+    asm volatile("" ::"v"(v.x), "v"(v.y), "v"(v.z), "v"(v.w));
+}
+__device__ __forceinline__ void fuse_pin(const double2 &v)
+{
+    asm volatile("" ::"v"(v.x), "v"(v.y));
+}
+
+// Sums of a[0..31] over the wave's 64 lanes: lane l returns column l >> 1.
+// Each exchange round halves the columns a lane keeps (it sends the half
+// its partner keeps), so the 32 sums take 32 shuffles instead of 32 trees
+// of 6; the summation order is fixed.
+__device__ __forceinline__ double wave_sum_cols32(const double (&a)[32])
+{
+    const int lane = threadIdx.x & 63;
+    double b[16];
+    {
+        const bool hi = lane & 32;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const double keep = hi ? a[q + 16] : a[q], send = hi ? a[q] : a[q + 16];
+            b[q] = keep + __shfl_xor(send, 32, 64);
+        }
+    }
+#pragma unroll
+    for (int w = 8, o = 16; w >= 1; w >>= 1, o >>= 1) {
+        const bool hi = lane & o;
+#pragma unroll
+        for (int q = 0; q < w; ++q) {
+            const double keep = hi ? b[q + w] : b[q], send = hi ? b[q] : b[q + w];
+            b[q] = keep + __shfl_xor(send, o, 64);
+        }
+    }
+    return b[0] + __shfl_xor(b[0], 1, 64);
+}
+
+// SpMV rows into the sweep's LDS vector (CompRowMatrix.mult's zero rule):
+// row r of the matrix at xs[r + base]
+template <typename T>
+struct EpiLds {
+    T *xs;
+    int base;
+    __device__ T init(int) const { return T(0); }
+    __device__ void row(int r, T dot) { xs[r + base] = (dot != T(0)) ? dot : T(0); }
+};
+
+template <typename T>
+__global__ void __launch_bounds__(fuse_threads<T>())
+    k_gm_fused(SlView sv, const T *__restrict__ val, const T *__restrict__ win,
+               T *__restrict__ vout, int scale, const int *__restrict__ bstart,
+               const int2 *__restrict__ dims, const T *__restrict__ recL,
+               const T *__restrict__ recU,
+               const T *__restrict__ dgl, const double *__restrict__ rdl, T *__restrict__ wout,
+               const T *__restrict__ V, size_t ldv, int i, DevState *st, RedSlot slot,
+               int vec_elems)
+{
+    using VT = typename BandVec<T>::type;
+    constexpr int N = BandVec<T>::N;
+    constexpr bool SLOTS = sizeof(T) == 4;   // FP32: slot layout; FP64: row layout
+    constexpr int P = fuse_producers<T>(), BS = fuse_threads<T>();
+    extern __shared__ unsigned char rs_smem[];
+    __shared__ double sh[P][RS_FUSE_COLS];
+    __shared__ int uprog, err, last;
+    double *Hb = st->H;
+    const int ld = st->m + 1;
+    const double hsub = st->hsub;
+    {
+        const int d_ = st->done, c_ = st->cycle_end;
+        if (d_ | c_)
+            return;
+    }
+    const int G = (int)gridDim.x;
+    const int b = xcd_remap((int)blockIdx.x, G);
+    const int b0 = bstart[b], nb = bstart[b + 1] - b0;
+    const int Lx = dims[b].x, Ly = dims[b].y;
+    const int SP = band_slot_pitch(Lx, Ly);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int ntA = nb / 64;
+    T *xv = (T *)rs_smem + (SLOTS ? 0 : RS_BAND_PAD);
+    int *flags = (int *)((T *)rs_smem + vec_elems);
+    const T sc = scale ? (T)(1. / hsub) : T(1);
+    // LDS set-up: padding zeroed, flags cleared
+    if (SLOTS) {
+        for (int j = 0; j < Ly; ++j)
+            for (int p = threadIdx.x; p < SP - Lx; p += BS)
+                xv[j * SP + (p < 2 * j ? p : p + Lx)] = T(0);
+    } else {
+        for (int r = threadIdx.x; r < RS_BAND_PAD; r += BS) {
+            xv[-1 - r] = T(0);
+            xv[nb + r] = T(0);
+        }
+    }
+    for (int t = threadIdx.x; t < ntA; t += BS)
+        flags[t] = 0;
+    if (threadIdx.x == 0) {
+        uprog = 0;
+        err = 0;
+    }
+    __syncthreads();
+    FuseHook hk{flags, &uprog, &err, ntA, Ly};
+    double a[RS_FUSE_COLS];
+#pragma unroll
+    for (int q = 0; q < RS_FUSE_COLS; ++q)
+        a[q] = 0;
+    const int cnt = i + 1;
+    if (wid == 0) {
+        // ---- the sweeps
+        __builtin_amdgcn_s_setprio(3);
+        if (SLOTS) {
+            band_sweep_full<T, false, true>(Lx, Ly, SP, b0, recL, dgl, rdl, xv, hk);
+            band_sweep_full<T, true, true>(Lx, Ly, SP, b0, recU, dgl, rdl, xv, hk);
+        } else {
+            const int junk = nb + RS_BAND_PAD - 1;
+            band_sweep<T, false, true>(Lx, Ly, b0, recL, dgl, nullptr, xv, junk, hk);
+            band_sweep<T, true, true>(Lx, Ly, b0, recU, dgl, nullptr, xv, junk, hk);
+        }
+        __builtin_amdgcn_s_setprio(0);
+    } else {
+        const int p = wid - 1;
+        // ---- phase A: SpMV rows, line segments in the L sweep's order
+        for (int t = p; t < ntA; t += P) {
+            const int h = t / Ly, j = t - h * Ly;
+            const int row0 = b0 + j * Lx + 64 * h;
+            // row (x, j) lives at slot x + 2 j of line j (FP32) or at its
+            // block row (FP64)
+            EpiLds<T> epi{xv, SLOTS ? j * SP + 2 * j - (b0 + j * Lx) : -b0};
+            // FP32: all 32 slots of a row in flight at once (the producers
+            // are latency-bound: one task per wave at a time)
+            constexpr int B = SLOTS ? 32 : RS_SL_BATCH;
+            if (scale) {
+                const int r = row0 + lane;
+                vout[r] = (sc == T(0)) ? T(0) : sc * win[r];
+                sliced_rows<T, EpiLds<T>, true, B>(row0, sv, val, win, epi, sc);
+            } else {
+                sliced_rows<T, EpiLds<T>, false, B>(row0, sv, val, win, epi);
+            }
+            if (lane == 0)
+                __hip_atomic_store(&flags[t], 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        // ---- phase B: finished rows of w' stored and projected, from the
+        // block's end (the U sweep finishes it first); a lane owns N
+        // consecutive rows of one line
+        const int PT = SLOTS ? SP : Lx + 2 * Ly - 2;   // U steps of a full pass
+        const int rowsB = 64 * N;
+        const int ntB = (nb + rowsB - 1) / rowsB;
+        const int kmax = scale ? i - 1 : i;            // columns read from V
+        for (int kk = p; kk < ntB; kk += P) {
+            const int q = (ntB - 1 - kk) * rowsB + N * lane;
+            const bool live = q < nb;
+            const int qc = live ? q : nb - N;
+            const int j = qc / Lx, x0 = qc - j * Lx;
+            // the lane's rows are final once the U sweep has passed its
+            // first row (the lowest slot x0 + 2 j of them)
+            const int need = PT - (x0 + 2 * j);
+            for (int spin = 0;
+                 !__hip_atomic_load(&err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                 ++spin) {
+                const int u = __hip_atomic_load(&uprog, __ATOMIC_ACQUIRE,
+                                                __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (!__any(u < need))
+                    break;
+                if (spin >= RS_FUSE_SPIN) {
+                    __hip_atomic_store(&err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            const int li = SLOTS ? j * SP + x0 + 2 * j : qc;
+            VT wv;
+            T *we = reinterpret_cast<T *>(&wv);
+#pragma unroll
+            for (int e = 0; e < N; ++e)
+                we[e] = xv[li + e];
+            const size_t gr = (size_t)b0 + qc;
+            if (live)
+                *reinterpret_cast<VT *>(wout + gr) = wv;
+            // column i: v_i as the SpMV formed it (win's rows, scaled)
+            VT vi = *reinterpret_cast<const VT *>(win + gr);
+            {
+                T *ve = reinterpret_cast<T *>(&vi);
+#pragma unroll
+                for (int e = 0; e < N; ++e)
+                    ve[e] = (sc == T(0)) ? T(0) : sc * ve[e];
+            }
+            // basis columns in chunks of FC: every load of a chunk is issued
+            // (and pinned) before the first product, and every product is
+            // formed and then selected, so no load sinks under a condition
+            constexpr int FC = 8;
+#pragma unroll
+            for (int c0 = 0; c0 < RS_FUSE_COLS; c0 += FC) {
+                if (c0 < cnt) {
+                    VT kv[FC];
+#pragma unroll
+                    for (int u = 0; u < FC; ++u)
+                        kv[u] = *reinterpret_cast<const VT *>(
+                            V + (size_t)max(min(c0 + u, kmax), 0) * ldv + gr);
+#pragma unroll
+                    for (int u = 0; u < FC; ++u)
+                        fuse_pin(kv[u]);
+#pragma unroll
+                    for (int u = 0; u < FC; ++u) {
+                        const int k = c0 + u;
+                        const VT kk = (scale && k == i) ? vi : kv[u];
+                        const T *ke = reinterpret_cast<const T *>(&kk);
+                        double acc = a[k];
+#pragma unroll
+                        for (int e = 0; e < N; ++e)
+                            acc = acc + (double)(we[e] * ke[e]);
+                        a[k] = (live && k < cnt) ? acc : a[k];
+                    }
+                }
+            }
+        }
+    }
+    // ---- projections: producer waves' sums, then the block's partials
+    static_assert(RS_FUSE_COLS == 32, "wave_sum_cols32 sums 32 columns");
+    if (wid > 0) {
+        const double r = wave_sum_cols32(a);
+        if ((lane & 1) == 0 && (lane >> 1) < cnt)
+            sh[wid - 1][lane >> 1] = r;
+    }
+    __syncthreads();
+    // a wait that ran out (never expected: every producer finishes its
+    // tasks unconditionally) stops the solve with an error
+    if (err && threadIdx.x == 0) {
+        st->status = RSGPU_HIP_ERROR;
+        st->done = 1;
+    }
+    if (threadIdx.x < cnt) {
+        double r = 0;
+#pragma unroll
+        for (int u = 0; u < P; ++u)
+            r += sh[u][threadIdx.x];
+        __hip_atomic_store(&slot.partials[(size_t)threadIdx.x * G + b], r, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // arrival ticket (as block_reduce_finish); thread 0 is in wave 0, which
+    // made every partial store of this group
+    if (threadIdx.x == 0) {
+        wait_vm_stores();
+        const int ng = G < RS_NGROUP ? G : RS_NGROUP;
+        const int g = (int)blockIdx.x % ng;
+        const int gsize = (G - g + ng - 1) / ng;
+        unsigned *gt = slot.ticket + (1 + g) * RS_TPAD;
+        int fin = 0;
+        const unsigned t =
+            __hip_atomic_fetch_add(gt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((int)t == gsize - 1) {
+            __hip_atomic_store(gt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned u = __hip_atomic_fetch_add(slot.ticket, 1u, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT);
+            if ((int)u == ng - 1) {
+                __hip_atomic_store(slot.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                fin = 1;
+            }
+        }
+        last = fin;
+    }
+    __syncthreads();
+    if (!last)
+        return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    // wave wid sums columns wid, wid + 8, ... over the blocks in order; all
+    // loads of a batch (4 partials per lane) in flight before the adds
+    constexpr int NW = BS / 64;
+    for (int k = wid; k < cnt; k += NW) {
+        double acc = 0;
+        for (int b1 = lane; b1 < G; b1 += 256) {
+            double t4[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                t4[u] = __hip_atomic_load(&slot.partials[(size_t)k * G + min(b1 + 64 * u, G - 1)],
+                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            acc += t4[0];
+#pragma unroll
+            for (int u = 1; u < 4; ++u) {
+                const double s_ = acc + t4[u];
+                acc = b1 + 64 * u < G ? s_ : acc;
+            }
+        }
+        const double r = wave_sum(acc);
+        if (lane == 0)
+            Hb[k + i * ld] = r;
+    }
+}
+
+// Whether the fused step applies: block ILU(0) with line-wavefront records
+// on full blocks of Lx % 64 == 0 starting on 64-row boundaries, the sliced
+// copy, at most RS_FUSE_COLS basis vectors, the LDS vector and flags within
+// the device limit.  RSGPU_GMRES_FUSED=0 turns it off.
+bool gmres_fused_ok(const rsgpu_csr *A, const rsgpu_prec *M, int elem_bytes, int m)
+{
+    const char *e = getenv("RSGPU_GMRES_FUSED");
+    const bool off = e && e[0] == '0';
+    if (off || !M || !A || !A->sl.ok() || !M->band || M->n != A->n || M->n == 0 ||
+        m + 1 > RS_FUSE_COLS || M->kind == RSGPU_PREC_ILUT || M->kind == RSGPU_PREC_ICC ||
+        M->kind == RSGPU_PREC_AMG || M->kind == RSGPU_PREC_SSOR ||
+        M->kind == RSGPU_PREC_DIAGONAL || M->kind == RSGPU_PREC_IDENTITY || M->skip.done ||
+        (int)M->h_band_dims.size() != M->nblocks)
+        return false;
+    if ((elem_bytes == 4) != (M->precision == RSGPU_FP32))
+        return false;
+    size_t vec = 0;
+    for (int b = 0; b < M->nblocks; ++b) {
+        const int2 d = M->h_band_dims[b];
+        const int b0 = M->h_bstart[b], nb = M->h_bstart[b + 1] - b0;
+        if (d.y <= 0 || d.x % 64 != 0 || b0 % 64 != 0 || nb != d.x * d.y)
+            return false;
+        vec = std::max(vec, elem_bytes == 4 ? (size_t)d.y * band_slot_pitch(d.x, d.y)
+                                            : (size_t)nb + 2 * RS_BAND_PAD);
+    }
+    const size_t shm = vec * elem_bytes + (size_t)(M->max_block_rows / 64 + 1) * 4;
+    return shm <= (size_t)lds_limit(M->ctx->device);
+}
+
+template <typename T>
+void launch_gmres_fused(hipStream_t s, const rsgpu_csr *A, const T *slv, rsgpu_prec *M,
+                        const T *win, T *vout, bool scale, T *wout, const T *V, size_t ldv,
+                        int i, DevState *st, RedSlot slot)
+{
+    const bool f32 = sizeof(T) == 4;
+    size_t vec = 0;
+    for (int b = 0; b < M->nblocks; ++b) {
+        const int2 d = M->h_band_dims[b];
+        vec = std::max(vec, f32 ? (size_t)d.y * band_slot_pitch(d.x, d.y)
+                                : (size_t)(M->h_bstart[b + 1] - M->h_bstart[b]) +
+                                      2 * RS_BAND_PAD);
+    }
+    const size_t shm = vec * sizeof(T) + (size_t)(M->max_block_rows / 64 + 1) * 4;
+    static bool attr = false;
+    if (!attr && shm > 65536) {
+        RS_HIP(hipFuncSetAttribute((const void *)k_gm_fused<T>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   lds_limit(M->ctx->device)));
+        attr = true;
+    }
+    const size_t np = (size_t)M->n + 2 * RS_BAND_PAD;
+    const T *rec = f32 ? (const T *)M->band32.p : (const T *)M->band64.p;
+    const double *rdl = (f32 && M->band_rd.p) ? M->band_rd.p + RS_BAND_PAD : nullptr;
+    hipLaunchKernelGGL(k_gm_fused<T>, dim3(M->nblocks), dim3(fuse_threads<T>()), shm, s,
+                       A->slview(),
+                       slv, win, vout, scale ? 1 : 0, M->bstart.p, M->band_dims.p,
+                       rec + (size_t)8 * RS_BAND_PAD, rec + (size_t)8 * RS_BAND_PAD + 4,
+                       rec + 8 * np + RS_BAND_PAD, rdl, wout, V,
+                       ldv, i, st, slot, (int)vec);
+    RS_CHECK_LAUNCH();
+}
+
+template void launch_gmres_fused<float>(hipStream_t, const rsgpu_csr *, const float *,
+                                        rsgpu_prec *, const float *, float *, bool, float *,
+                                        const float *, size_t, int, DevState *, RedSlot);
+template void launch_gmres_fused<double>(hipStream_t, const rsgpu_csr *, const double *,
+                                         rsgpu_prec *, const double *, double *, bool, double *,
+                                         const double *, size_t, int, DevState *, RedSlot);
 
 template <typename T, int TRANS>
 static void launch_apply(rsgpu_prec *M, const SweepData &A_, const T *va, const T *da,

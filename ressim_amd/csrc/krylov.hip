@@ -49,6 +49,7 @@ struct rsgpu_solver {
     // the matrix whose FP64 sliced copy this solve filled (its SpMVs then
     // stream the sliced copy)
     const rsgpu_csr *sl64 = nullptr;
+    int fused_steps = 0;          // fused GMRES steps enqueued by the last solve
 };
 
 // vector kernels: at least 4 elements per lane, at most 1024 groups
@@ -921,6 +922,78 @@ __device__ void gm_finish_column(DevState *st, int i, double wnorm2)
     }
 }
 
+// gm_finish_column by the first wave of the finishing group (all 64 lanes
+// active; tot valid in lane 0).  The column's entries, the earlier
+// rotations and s(i), s(i+1) are fetched with one load per lane into LDS, lane
+// 0 applies the rotations in order from LDS (the operations and their order
+// of gm_finish_column, so the results are bit-identical), and the rotated
+// entries are stored back one per lane: a few memory round trips instead of
+// one per dependent access of the sequential form.  i + 1 < 64.
+__device__ void gm_finish_column_wave(DevState *st, int i, double wnorm2, double *lh)
+{
+    const int lane = threadIdx.x & 63;
+    const int ld = st->m + 1;
+    double *Hc = st->H + i * ld;
+    double *gcp = st->gc, *gsp = st->gs, *sv = st->s;
+    double *lc = lh + 64, *ls = lh + 128;
+    if (lane <= i)
+        lh[lane] = Hc[lane];
+    if (lane < i) {
+        lc[lane] = gcp[lane];
+        ls[lane] = gsp[lane];
+    }
+    const double s0 = sv[i], s1 = sv[i + 1];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) {
+        const double hn = sqrt(wnorm2);
+        st->hsub = hn;
+        // previous rotations on column i; cur is the running Hc[q]
+        double cur = lh[0];
+        for (int q = 0; q < i; ++q) {
+            const double c = lc[q], s = ls[q], nx = lh[q + 1];
+            const double temp = c * cur + s * nx;
+            cur = -s * cur + c * nx;
+            lh[q] = temp;
+        }
+        // new rotation (GivensRotation(x, y))
+        const double x = cur, y = hn;
+        const double roe = fabs(x) > fabs(y) ? x : y;
+        const double scale = fabs(x) + fabs(y);
+        double c, s;
+        if (scale != 0) {
+            const double xs = x / scale, ys = y / scale;
+            double rr = scale * sqrt(xs * xs + ys * ys);
+            if (roe < 0)
+                rr *= -1;
+            c = x / rr;
+            s = y / rr;
+        } else {
+            c = 1;
+            s = 0;
+        }
+        gcp[i] = c;
+        gsp[i] = s;
+        Hc[i] = c * x + s * y;
+        Hc[i + 1] = -s * x + c * y;
+        const double t0 = c * s0 + s * s1;
+        const double t1 = -s * s0 + c * s1;
+        sv[i] = t0;
+        sv[i + 1] = t1;
+        st->iter++;
+        st->gi = i + 1;
+        if (!(i + 1 < st->m) || dev_monitor(st, fabs(t1))) {
+            st->cycle_end = 1;
+            if (st->status)
+                st->done = 1;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (lane < i)
+        Hc[lane] = lh[lane];
+}
+
 // MGS step k of column i: w -= H(k-1,i) v_{k-1} (k > 0), H(k,i) = w . v_k
 template <typename T>
 __global__ void __launch_bounds__(RS_VBS)
@@ -1007,8 +1080,13 @@ __global__ void __launch_bounds__(RS_VBS)
         a += (double)(wj * wj);
     }
     double v[1] = {a}, tot[1];
-    if (block_reduce_finish<RS_VBS, 1>(slot, v, tot) && threadIdx.x == 0)
-        gm_finish_column(st, i, tot[0]);
+    __shared__ double lh[3 * 64];
+    if (block_reduce_finish<RS_VBS, 1>(slot, v, tot) && threadIdx.x < 64) {
+        if (i + 1 < 64)
+            gm_finish_column_wave(st, i, __shfl(tot[0], 0, 64), lh);
+        else if (threadIdx.x == 0)
+            gm_finish_column(st, i, tot[0]);
+    }
 }
 
 // Classical Gram-Schmidt (no reference counterpart; the FP32 inner GMRES of
@@ -1294,13 +1372,143 @@ __global__ void __launch_bounds__(RS_VBS)
     if (PASS == 1)
         return;
     double v[1] = {a}, tot[1];
-    if (block_reduce_finish<RS_VBS, 1>(slot, v, tot) && threadIdx.x == 0) {
-        if (PASS == 2) {
-            double *Hi = Hb + i * ld;
-            for (int k = 0; k <= i; ++k)
-                Hi[k] += h2b[k];
+    __shared__ double lh[3 * 64];
+    if (block_reduce_finish<RS_VBS, 1>(slot, v, tot) && threadIdx.x < 64) {
+        if (i + 1 < 64) {
+            // H(:, i) = h1 + h2, one entry per lane (each lane reads back
+            // its own store in gm_finish_column_wave)
+            if (PASS == 2 && (int)threadIdx.x <= i)
+                Hb[threadIdx.x + i * ld] += h2b[threadIdx.x];
+            gm_finish_column_wave(st, i, __shfl(tot[0], 0, 64), lh);
+        } else if (threadIdx.x == 0) {
+            if (PASS == 2) {
+                double *Hi = Hb + i * ld;
+                for (int k = 0; k <= i; ++k)
+                    Hi[k] += h2b[k];
+            }
+            gm_finish_column(st, i, tot[0]);
         }
-        gm_finish_column(st, i, tot[0]);
+    }
+}
+
+// FP64 CGS2's first update and second projection pass in one sweep over
+// the basis: w -= V h1 (k_gm_cgs_last<double, 1>'s update: each element in
+// k order, add's alpha == 0 skip), then st->h2 = V' w over the updated
+// elements (the second pass of k_gm_cgs_dots, summed per thread in element
+// order, then over the group's waves, then over the groups in order).  Each
+// basis element is loaded once and used by both, so the step reads the
+// basis three times instead of four.  One element per lane: the i + 1 basis
+// values of an element stay in registers between the update and the
+// products.
+#define RS_UD_COLS 32
+__global__ void __launch_bounds__(RS_VBS)
+    k_gm_cgs_upd_dots(int n, int i, double *__restrict__ w, const double *__restrict__ V,
+                      size_t ldv, DevState *st, RedSlot slot)
+{
+    constexpr int NW = RS_VBS / 64;
+    __shared__ double sh[NW][RS_UD_COLS];
+    __shared__ double hs[RS_UD_COLS];
+    __shared__ int last;
+    double *Hb = st->H, *h2b = st->h2;
+    const int ld = st->m + 1;
+    if (stopped(st, true))
+        return;
+    const int cnt = i + 1;
+    const int G = (int)gridDim.x;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (threadIdx.x < RS_UD_COLS)
+        hs[threadIdx.x] = -Hb[min((int)threadIdx.x, i) + i * ld];
+    __syncthreads();
+    double a[RS_UD_COLS];
+#pragma unroll
+    for (int q = 0; q < RS_UD_COLS; ++q)
+        a[q] = 0;
+    for (int j = blockIdx.x * RS_VBS + threadIdx.x; j < n; j += G * RS_VBS) {
+        double wj = w[j];
+        double kv[RS_UD_COLS];
+#pragma unroll
+        for (int c0 = 0; c0 < RS_UD_COLS; c0 += 8) {
+            if (c0 < cnt) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    kv[c0 + u] = V[(size_t)min(c0 + u, i) * ldv + j];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < RS_UD_COLS; ++q) {
+            if (q < cnt) {
+                const double h = hs[q];
+                const double u = wj + h * kv[q];
+                wj = h != 0.0 ? u : wj;
+            }
+        }
+        w[j] = wj;
+#pragma unroll
+        for (int q = 0; q < RS_UD_COLS; ++q)
+            if (q < cnt)
+                a[q] += wj * kv[q];
+    }
+#pragma unroll
+    for (int q = 0; q < RS_UD_COLS; ++q)
+        if (q < cnt) {
+            const double r = wave_sum(a[q]);
+            if (lane == 0)
+                sh[wid][q] = r;
+        }
+    __syncthreads();
+    if (threadIdx.x < cnt) {
+        double r = 0;
+#pragma unroll
+        for (int u = 0; u < NW; ++u)
+            r += sh[u][threadIdx.x];
+        __hip_atomic_store(&slot.partials[(size_t)threadIdx.x * G + blockIdx.x], r,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    // arrival ticket (as block_reduce_finish); thread 0 is in wave 0, which
+    // made every partial store of this group
+    if (threadIdx.x == 0) {
+        wait_vm_stores();
+        const int ng = G < RS_NGROUP ? G : RS_NGROUP;
+        const int g = (int)blockIdx.x % ng;
+        const int gsize = (G - g + ng - 1) / ng;
+        unsigned *gt = slot.ticket + (1 + g) * RS_TPAD;
+        int fin = 0;
+        const unsigned t =
+            __hip_atomic_fetch_add(gt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((int)t == gsize - 1) {
+            __hip_atomic_store(gt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned u = __hip_atomic_fetch_add(slot.ticket, 1u, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT);
+            if ((int)u == ng - 1) {
+                __hip_atomic_store(slot.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                fin = 1;
+            }
+        }
+        last = fin;
+    }
+    __syncthreads();
+    if (!last)
+        return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    // wave wid sums columns wid, wid + NW, ... over the groups in order
+    for (int k = wid; k < cnt; k += NW) {
+        double acc = 0;
+        for (int b0 = lane; b0 < G; b0 += 256) {
+            double t4[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                t4[u] = __hip_atomic_load(&slot.partials[(size_t)k * G + min(b0 + 64 * u, G - 1)],
+                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            acc += t4[0];
+#pragma unroll
+            for (int u = 1; u < 4; ++u) {
+                const double s_ = acc + t4[u];
+                acc = b0 + 64 * u < G ? s_ : acc;
+            }
+        }
+        const double r = wave_sum(acc);
+        if (lane == 0)
+            h2b[k] = r;
     }
 }
 
@@ -1320,25 +1528,63 @@ __global__ void __launch_bounds__(RS_VBS)
 }
 
 // dtrtrs on the leading gi x gi triangle of H (singular -> status 14)
+// One wave.  Up to 64 columns the triangle is staged in LDS with one load
+// per lane and column, and lane i carries s[i]: each column's division is
+// formed by every lane and the updates of s[0..k-1] run one per lane, each
+// s[i] still updated in descending k (the sequential loop's operations and
+// order, bit for bit).
 __global__ void k_gm_backsolve(DevState *st)
 {
-    if (st->done || threadIdx.x != 0)
+    __shared__ double Hs[64 * 64];
+    if (st->done)
         return;
     const int ld = st->m + 1, m = st->gi;
     const double *H = st->H;
     double *s = st->s;
-    for (int k = 0; k < m; ++k)
-        if (H[k + k * ld] == 0) {
+    const int lane = threadIdx.x;
+    if (m > 64) {
+        if (lane != 0)
+            return;
+        for (int k = 0; k < m; ++k)
+            if (H[k + k * ld] == 0) {
+                st->status = RSGPU_SINGULAR;
+                st->done = 1;
+                return;
+            }
+        for (int k = m - 1; k >= 0; --k)
+            if (s[k] != 0) {
+                s[k] = s[k] / H[k + k * ld];
+                for (int i = 0; i < k; ++i)
+                    s[i] = s[i] - s[k] * H[i + k * ld];
+            }
+        return;
+    }
+    const bool zero = lane < m && H[lane + lane * ld] == 0;
+    if (__any(zero)) {
+        if (lane == 0) {
             st->status = RSGPU_SINGULAR;
             st->done = 1;
-            return;
         }
-    for (int k = m - 1; k >= 0; --k)
-        if (s[k] != 0) {
-            s[k] = s[k] / H[k + k * ld];
-            for (int i = 0; i < k; ++i)
-                s[i] = s[i] - s[k] * H[i + k * ld];
+        return;
+    }
+    for (int k = 0; k < m; ++k)
+        if (lane <= k)
+            Hs[k * 64 + lane] = H[lane + k * ld];
+    double si = lane < m ? s[lane] : 0.0;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (int k = m - 1; k >= 0; --k) {
+        const double sk = __shfl(si, k, 64);
+        if (sk != 0) {
+            const double q = sk / Hs[k * 64 + k];
+            if (lane == k)
+                si = q;
+            else if (lane < k)
+                si = si - q * Hs[k * 64 + lane];
         }
+    }
+    if (lane < m)
+        s[lane] = si;
 }
 
 // x += s_j v_j, j = 0..gi-1, in that order per element
@@ -1908,8 +2154,49 @@ void run_gmres(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const T *b, T *x,
         hipLaunchKernelGGL(k_gm_start<T>, dim3(G), dim3(RS_VBS), 0, s, n, r, V,
                            st, sl);
         const T *slv = sliced_vals<T>(S, A);
+        // fused step: SpMV, block ILU apply and the first projection pass in
+        // one launch; its input and output vectors alternate between w and u
+        const bool fused = slv && S->o.cgs && !S->o.prec_transpose && M &&
+                           (size_t)(m + 1) * M->nblocks <= S->partials.n &&
+                           gmres_fused_ok(A, M, (int)sizeof(T), m);
+        T *wcur = w, *wnext = u;
+        // FP64 CGS2: the first update and the second projection share a pass
+        // (RSGPU_CGS_UD=0: separate kernels)
+        const char *eud = getenv("RSGPU_CGS_UD");
+        const bool cgs_ud = sizeof(T) == 8 && m + 1 <= RS_UD_COLS && !(eud && eud[0] == '0');
         for (int i = 0; i < m; ++i) {
             T *vi = V + (size_t)i * L;
+            if (fused) {
+                launch_gmres_fused<T>(s, A, slv, M, i > 0 ? wcur : vi, vi, i > 0, wnext, V, L,
+                                      i, st, sl);
+                ++S->fused_steps;
+                if (sizeof(T) == 4) {
+                    hipLaunchKernelGGL((k_gm_cgs_last<T, 0>), dim3(G), dim3(RS_VBS), 0, s, n, i,
+                                       wnext, V, L, st, sl);
+                } else {
+                    // CGS2: w -= V h1 and h2 = V' w in one pass, then w -= V h2
+                    if (cgs_ud)
+                        hipLaunchKernelGGL(k_gm_cgs_upd_dots, dim3(G), dim3(RS_VBS), 0, s, n,
+                                           i, (double *)wnext, (const double *)V, L, st, sl);
+                    else {
+                        hipLaunchKernelGGL((k_gm_cgs_last<T, 1>), dim3(G), dim3(RS_VBS), 0, s,
+                                           n, i, wnext, V, L, st, sl);
+                        hipLaunchKernelGGL(k_gm_cgs_dots<T>, dim3(std::max(1, G / 4)),
+                                           dim3(RS_VBS), 0, s, n, i, wnext, V, L, st, sl, 1);
+                    }
+                    hipLaunchKernelGGL((k_gm_cgs_last<T, 2>), dim3(G), dim3(RS_VBS), 0, s, n, i,
+                                       wnext, V, L, st, sl);
+                }
+                RS_CHECK_LAUNCH();
+                std::swap(wcur, wnext);
+                if ((i + 1) % every == 0 && i + 1 < m) {
+                    P.post();
+                    P.wait_prev(done, cend);
+                    if (done || cend)
+                        break;
+                }
+                continue;
+            }
             if (i > 0 && slv) {
                 // v_i = w / H(i,i-1) stored by the SpMV that reads it
                 hipLaunchKernelGGL(k_spmv_scale_sl<T>, dim3(sliced_tiles(n)),
@@ -1928,10 +2215,15 @@ void run_gmres(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const T *b, T *x,
                 // CGS2: two classical passes, H(:,i) = h1 + h2
                 hipLaunchKernelGGL(k_gm_cgs_dots<T>, dim3(std::max(1, G / 4)), dim3(RS_VBS), 0,
                                    s, n, i, w, V, L, st, sl, 0);
-                hipLaunchKernelGGL((k_gm_cgs_last<T, 1>), dim3(G), dim3(RS_VBS), 0, s, n, i, w,
-                                   V, L, st, sl);
-                hipLaunchKernelGGL(k_gm_cgs_dots<T>, dim3(std::max(1, G / 4)), dim3(RS_VBS), 0,
-                                   s, n, i, w, V, L, st, sl, 1);
+                if (cgs_ud) {
+                    hipLaunchKernelGGL(k_gm_cgs_upd_dots, dim3(G), dim3(RS_VBS), 0, s, n, i,
+                                       (double *)w, (const double *)V, L, st, sl);
+                } else {
+                    hipLaunchKernelGGL((k_gm_cgs_last<T, 1>), dim3(G), dim3(RS_VBS), 0, s, n,
+                                       i, w, V, L, st, sl);
+                    hipLaunchKernelGGL(k_gm_cgs_dots<T>, dim3(std::max(1, G / 4)),
+                                       dim3(RS_VBS), 0, s, n, i, w, V, L, st, sl, 1);
+                }
                 hipLaunchKernelGGL((k_gm_cgs_last<T, 2>), dim3(G), dim3(RS_VBS), 0, s, n, i, w,
                                    V, L, st, sl);
             } else {
@@ -2261,6 +2553,7 @@ extern "C" int rsgpu_solver_solve(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M,
             S->partials.alloc(std::max(need, need_cgs) + 4096);
         state_init(S, S->st, S->hist, S->o.max_iter, S->o.rtol, S->o.atol,
                    S->o.dtol, m, S->gmres_small.p);
+        S->fused_steps = 0;
         if (stats)
             std::memset(stats, 0, sizeof(*stats));
         // FP64 solves stream the sliced copy, refreshed from A's values here
@@ -2296,6 +2589,11 @@ extern "C" int rsgpu_solver_solve(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M,
     } catch (const RsError &e) {
         return solver_catch(ctx, e);
     }
+}
+
+extern "C" int rsgpu_solver_fused_steps(const rsgpu_solver *S)
+{
+    return S ? S->fused_steps : RSGPU_BAD_ARGUMENT;
 }
 
 extern "C" int rsgpu_solver_history(const rsgpu_solver *S, double *out, int cap)

@@ -290,6 +290,7 @@ struct rsgpu_prec {
     // absent), the slot masks, and the per-row records gathered after each
     // factorisation
     bool band = false;
+    std::vector<int2> h_band_dims;   // host copy of band_dims
     DevBuf<int2> band_dims;
     DevBuf<int> band_pos;
     DevBuf<unsigned char> band_mask;
@@ -592,6 +593,15 @@ void prec_apply(rsgpu_prec *M, const double *b, double *x, const DevState *st);
 void prec_apply(rsgpu_prec *M, const float *b, float *x, const DevState *st);
 void prec_trans_apply(rsgpu_prec *M, const double *b, double *x,
                       const DevState *st = nullptr);
+// GMRES inner step fused into one launch (sliced SpMV with the basis scale,
+// line-wavefront block ILU(0) apply, first projection pass onto V): whether
+// it applies to (A, M) with elements of elem_bytes and restart m, and its
+// launch (wout must not alias win)
+bool gmres_fused_ok(const rsgpu_csr *A, const rsgpu_prec *M, int elem_bytes, int m);
+template <typename T>
+void launch_gmres_fused(hipStream_t s, const rsgpu_csr *A, const T *slv, rsgpu_prec *M,
+                        const T *win, T *vout, bool scale, T *wout, const T *V, size_t ldv,
+                        int i, DevState *st, RedSlot slot);
 
 // amg.hip
 int amg_setup(rsgpu_prec *M, rsgpu_csr *A, int *bad_row);

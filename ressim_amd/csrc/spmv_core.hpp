@@ -280,14 +280,14 @@ __device__ __forceinline__ void spmv_core_b3(const int *__restrict__ wg_rows,
 // nontemporal hint keeps them out of the last-level cache, which the FP32
 // inner solve's working set largely fits.
 #define RS_SL_BATCH 16
-template <typename T, class Epi, bool SCALE = false>
+template <typename T, class Epi, bool SCALE = false, int B = RS_SL_BATCH>
 __device__ __forceinline__ void sliced_batch(int s0, int W, int r, int cmax, unsigned m,
                                              const int (&offs)[32], const T *vc,
                                              const T *__restrict__ x, T &dot, T sc = T(1))
 {
-    T v[RS_SL_BATCH], xg[RS_SL_BATCH];
+    T v[B], xg[B];
 #pragma unroll
-    for (int k = 0; k < RS_SL_BATCH; ++k) {
+    for (int k = 0; k < B; ++k) {
         const int sl = min(s0 + k, W - 1);
         const int col = min(max(r + offs[s0 + k], 0), cmax);
         v[k] = vc[(size_t)sl * RS_SL_ROWS];
@@ -297,21 +297,22 @@ __device__ __forceinline__ void sliced_batch(int s0, int W, int r, int cmax, uns
     // separate scaling pass would store it
     if (SCALE) {
 #pragma unroll
-        for (int k = 0; k < RS_SL_BATCH; ++k)
+        for (int k = 0; k < B; ++k)
             xg[k] = (sc == T(0)) ? T(0) : sc * xg[k];
     }
     // every product and sum is formed and then selected: a conditional add
     // lets the compiler sink the slot's loads under the condition, where
     // each waits for memory on its own
 #pragma unroll
-    for (int k = 0; k < RS_SL_BATCH; ++k) {
+    for (int k = 0; k < B; ++k) {
         const T q = dot + v[k] * xg[k];
         dot = ((m >> (s0 + k)) & 1u) ? q : dot;
     }
 }
 
-// the 64 rows from row0 (wave-uniform, row0 < n)
-template <typename T, class Epi, bool SCALE = false>
+// the 64 rows from row0 (wave-uniform, row0 < n); B: slots per batch of
+// loads in flight (32: every slot of a row in one round trip)
+template <typename T, class Epi, bool SCALE = false, int B = RS_SL_BATCH>
 __device__ __forceinline__ void sliced_rows(int row0, const SlView &S, const T *__restrict__ val,
                                             const T *__restrict__ x, Epi &epi, T sc = T(1))
 {
@@ -342,9 +343,9 @@ __device__ __forceinline__ void sliced_rows(int row0, const SlView &S, const T *
 #pragma unroll
         for (int k = 0; k < 32; ++k)
             offs[k] = k < W ? offs[k] : lastoff;
-        sliced_batch<T, Epi, SCALE>(0, W, rr, cmax, m, offs, vc, x, dot, sc);
-        if (W > RS_SL_BATCH)
-            sliced_batch<T, Epi, SCALE>(RS_SL_BATCH, W, rr, cmax, m, offs, vc, x, dot, sc);
+        sliced_batch<T, Epi, SCALE, B>(0, W, rr, cmax, m, offs, vc, x, dot, sc);
+        if (B < 32 && W > B)
+            sliced_batch<T, Epi, SCALE, B>(B, W, rr, cmax, m, offs, vc, x, dot, sc);
     }
     if (live)
         epi.row(r, dot);

@@ -661,6 +661,7 @@ class AbstractIterativeSolver:
         self._h = None
         self._key = None
         self.last_stats = None
+        self.fused_steps = 0
 
     def setPreconditioner(self, M: Preconditioner):
         self.M = M
@@ -727,6 +728,8 @@ class AbstractIterativeSolver:
             bb = np.ascontiguousarray(b, np.float64)
             code = lib.rsgpu_solver_solve_host(h, A.h, Mh, _hptr(bb), _hptr(x), C.byref(st))
         self.last_stats = st
+        # GMRES steps that ran as one fused launch (rsgpu_solver_fused_steps)
+        self.fused_steps = lib.rsgpu_solver_fused_steps(h)
         self.iter._iterations = st.iterations
         self.iter._residual = st.residual
         hl = max(0, min(st.hist_len, 1 << 22))

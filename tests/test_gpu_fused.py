@@ -1,0 +1,134 @@
+"""The fused GMRES inner step against the separate kernels and the oracle.
+
+With the classical Gram-Schmidt option on full line blocks (every block Ly
+lines of Lx rows, Lx a multiple of 64) one launch runs the sliced SpMV with
+the basis scale, the line-wavefront block ILU(0) apply and the first
+projection pass (GMRES.java:127-176 over BlockDiagonalPreconditioner.java:
+46-75).  Its SpMV rows and sweep results are those of the separate kernels
+bit for bit; only the projections' summation order differs (per block, then
+over blocks in block order), so the fused and the separate-kernel solves
+agree to round-off: the same iteration count within 2% (3 at least), FP64
+solutions within 1e-10 of each other, and both within 1e-8 of the oracle's
+FP64 solve.
+RSGPU_GMRES_FUSED=0 selects the separate kernels.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from ressim_amd import problems as P
+from tests.helpers import oracle_system, scipy_csr
+
+pytestmark = pytest.mark.gpu
+
+# (nx, ny, nz, blocks): blocks of Ly lines of Lx = nx rows inside one z-plane
+CASES = [
+    (64, 64, 16, 32),    # Lx 64, Ly 32
+    (64, 64, 16, 16),    # Lx 64, Ly 64 (the most lines a block may have)
+    (128, 32, 8, 8),     # Lx 128, Ly 32 (the 128x128x64 system's blocks)
+    (192, 16, 6, 12),    # Lx 192, Ly 8 (three 64-row segments per line)
+]
+
+
+@pytest.fixture(scope="module", params=CASES, ids=lambda c: "%dx%dx%d-b%d" % c)
+def case(request, oracle):
+    nx, ny, nz, nb = request.param
+    pr = P.config2(nx=nx, ny=ny, nz=nz)
+    s = oracle_system(oracle, pr)
+    xo, so, _ = oracle.solve("gmres", s["rowptr"], s["colind"], s["data"], s["b"],
+                             rtol=1e-12, restart=30, prec="block_ilu", nblocks=nb,
+                             max_iter=20000)
+    assert so.status == 0
+    return pr, s, nb, xo, so
+
+
+def _fused_env(on):
+    if on:
+        os.environ.pop("RSGPU_GMRES_FUSED", None)
+    else:
+        os.environ["RSGPU_GMRES_FUSED"] = "0"
+
+
+def _relerr(x, y):
+    return float(np.linalg.norm(x - y) / np.linalg.norm(y))
+
+
+def _fp64(gpu, s, nb, on):
+    import torch
+    _fused_env(on)
+    try:
+        n = len(s["rowptr"]) - 1
+        A = gpu.CompRowMatrix.from_csr(n, n, s["rowptr"], s["colind"], s["data"])
+        M = gpu.BlockILU(nb)
+        M.setMatrix(A)
+        g = gpu.GMRES(s["b"], 30)
+        g.cgs = True
+        g.setIterationMonitor(gpu.DefaultIterationMonitor(20000, 1e-12, 1e-50, 1e5))
+        g.setPreconditioner(M)
+        b = torch.tensor(s["b"], device="cuda")
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        g.solve(A, b, x)
+        return x.cpu().numpy(), g.last_stats, g.fused_steps
+    finally:
+        _fused_env(True)
+
+
+def _mixed(gpu, s, nb, on, rtol=1e-10):
+    import torch
+    from ressim_amd import _lib
+    _fused_env(on)
+    try:
+        n = len(s["rowptr"]) - 1
+        A = gpu.CompRowMatrix.from_csr(n, n, s["rowptr"], s["colind"], s["data"])
+        A.sync_f32()
+        M = gpu.BlockILU(nb, precision=_lib.FP32)
+        M.setMatrix(A)
+        g = gpu.MixedPrecisionSolver(s["b"], inner="gmres", restart=30, inner_rtol=1e-3,
+                                     inner_max_iter=600, cgs=True)
+        g.setIterationMonitor(gpu.DefaultIterationMonitor(200, rtol, 1e-50, 1e5))
+        g.setPreconditioner(M)
+        b = torch.tensor(s["b"], device="cuda")
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        g.solve(A, b, x)
+        return x.cpu().numpy(), g.last_stats, g.fused_steps
+    finally:
+        _fused_env(True)
+
+
+def test_fp64_cgs2_fused_vs_separate_and_oracle(gpu, case):
+    pr, s, nb, xo, so = case
+    print(f"{pr.name} blocks {nb}: FP64 GMRES(30) CGS2, fused and separate", flush=True)
+    xf, sf, kf = _fp64(gpu, s, nb, True)
+    xs, ss, ks = _fp64(gpu, s, nb, False)
+    A = scipy_csr(s["rowptr"], s["colind"], s["data"])
+    rf = np.linalg.norm(s["b"] - A @ xf) / np.linalg.norm(s["b"])
+    print(f"  iterations fused {sf.iterations} separate {ss.iterations} oracle "
+          f"{so.iterations}; fused steps {kf}; true rel residual {rf:.2e}; "
+          f"fused vs separate {_relerr(xf, xs):.2e}, vs oracle {_relerr(xf, xo):.2e}",
+          flush=True)
+    assert kf > 0 and ks == 0
+    assert sf.status == 0 and ss.status == 0
+    tol = max(3, so.iterations // 50)
+    assert abs(sf.iterations - ss.iterations) <= tol
+    assert abs(sf.iterations - so.iterations) <= tol
+    assert rf <= 1e-10
+    assert _relerr(xf, xs) <= 1e-10
+    assert _relerr(xf, xo) <= 1e-8 and _relerr(xs, xo) <= 1e-8
+
+
+def test_mixed_fused_vs_separate_and_oracle(gpu, case):
+    pr, s, nb, xo, so = case
+    print(f"{pr.name} blocks {nb}: mixed (FP32 GMRES(30) CGS inside FP64 refinement), "
+          f"fused and separate", flush=True)
+    xf, sf, kf = _mixed(gpu, s, nb, True)
+    xs, ss, ks = _mixed(gpu, s, nb, False)
+    print(f"  refinements fused {sf.outer_iterations} separate {ss.outer_iterations}; inner "
+          f"fused {sf.inner_iterations} separate {ss.inner_iterations}; fused steps {kf}; "
+          f"true rel residual {sf.true_residual / sf.b_norm:.2e}; fused vs oracle "
+          f"{_relerr(xf, xo):.2e}, separate vs oracle {_relerr(xs, xo):.2e}", flush=True)
+    assert kf > 0 and ks == 0
+    assert sf.status == 0 and ss.status == 0
+    assert sf.true_residual <= 1e-10 * sf.b_norm
+    assert abs(sf.inner_iterations - ss.inner_iterations) <= max(3, ss.inner_iterations // 20)
+    assert _relerr(xf, xo) <= 1e-8 and _relerr(xs, xo) <= 1e-8
