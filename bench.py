@@ -41,6 +41,19 @@ def spmv_bytes(n, nnz, s_val, s_vec):
     return nnz * (s_val + 4) + (n + 1) * 4 + n * s_vec + n * s_vec
 
 
+# column of the fused GMRES step timed by the bench: 15 basis vectors, the
+# mean of a GMRES(30) cycle's 1..30
+FUSED_I = 14
+
+
+def fused_bytes(n, nnz, nnz_lu, s, i):
+    """Algorithmic bytes of the fused GMRES step at column i: the CSR SpMV
+    and the ILU apply (the per-unit figures above) without the round trip
+    of their intermediate u = A v (never stored), plus the i + 1 basis
+    vectors read for the projections."""
+    return spmv_bytes(n, nnz, s, s) + ilu_bytes(n, nnz_lu, s) - 2 * n * s + (i + 1) * n * s
+
+
 def sliced_bytes(n, nchunks, nslots, s_val):
     """Bytes the sliced SpMV must move: every value slot (zeros where a row
     lacks the slot), the 4-byte row masks, two ints per chunk, x read once
@@ -62,15 +75,23 @@ def ilu_bytes(n, nnz_lu, s):
 PMC_PROFILE = os.path.join(ROOT, "profiles", "k_spmv_mpfa3d_128x128x64_pmc.md")
 ILU_PMC_PROFILE = os.path.join(ROOT, "profiles", "k_ilu_apply_mpfa3d_128x128x64_pmc.md")
 ILU32_KERNEL = "k_ilu_apply_band<float>"
+FUSED_PMC_PROFILE = os.path.join(ROOT, "profiles", "k_gm_fused_mpfa3d_128x128x64_pmc.md")
+FUSED_KERNEL = "k_gm_fused<float>"
 
 
 def measured_traffic(pr, kernel, profile=PMC_PROFILE):
+    """The kernel's `traffic` column (scaled FETCH_SIZE + WRITE_SIZE bytes
+    per dispatch) from a committed counter profile's table."""
     if (pr.nx, pr.ny, pr.nz) != (128, 128, 64) or not os.path.exists(profile):
         return None
+    col = None
     for line in open(profile):
         cells = [c.strip() for c in line.split("|")]
-        if len(cells) > 4 and cells[1] == f"`{kernel}`" and cells[4].endswith(" B"):
-            return int(cells[4][:-2].replace(",", ""))
+        if len(cells) > 2 and cells[1] == "kernel":
+            col = cells.index("traffic") if "traffic" in cells else None
+        elif col is not None and len(cells) > col and cells[1] == f"`{kernel}`" \
+                and cells[col].endswith(" B"):
+            return int(cells[col][:-2].replace(",", ""))
     return None
 
 
@@ -151,7 +172,8 @@ def main():
                     help="FP64 GMRES with MTJ's modified Gram-Schmidt (default: CGS2)")
     ap.add_argument("--mgs", action="store_true",
                     help="modified Gram-Schmidt in the FP32 inner GMRES (default: classical)")
-    ap.add_argument("--probe", choices=["spmv", "ilu", "ilu32", "solve", "mixed", "amg", "amg32"],
+    ap.add_argument("--probe", choices=["spmv", "ilu", "ilu32", "solve", "mixed", "amg", "amg32",
+                                        "fused"],
                     default=None,
                     help="profiling probe: run only the named kernel(s) on the workload "
                          "(plus a 512 MiB device copy as the traffic reference) and exit")
@@ -306,8 +328,9 @@ def main():
                      for _ in range(NCOPY - 1)]
     pre32 = [M32] + [mtj.BlockILU(a.nblocks, precision=_lib.FP32, ctx=ctx)
                      for _ in range(NCOPY - 1)]
-    for Pm in pre64[1:] + pre32[1:]:
-        Pm.setMatrix(A)
+    for c in range(1, NCOPY):
+        pre64[c].setMatrix(mats[c])
+        pre32[c].setMatrix(mats[c])
     torch.cuda.synchronize()
     R = a.spmv_reps
 
@@ -336,12 +359,36 @@ def main():
         pre64[c].h, xs64[c].data_ptr(), ys64[c].data_ptr()), max(10, R // 2))
     kt["ilu_apply_fp32"] = both(lambda c: lambda: lib.rsgpu_prec_apply_f32(
         pre32[c].h, xs32[c].data_ptr(), ys32[c].data_ptr()), max(10, R // 2))
+    # the fused GMRES step (the mixed solve's dominant kernel, k_gm_fused):
+    # one launch at column FUSED_I on copy c's matrix, preconditioner and
+    # basis (rsgpu_gmres_fused_step_f32)
+    gf = mtj.GMRES(xs64[0], a.restart)
+    gf.cgs = True
+    hf = gf._handle(ctx)
+    ldv = (n + 3) // 4 * 4
+    Vs = [torch.randn(FUSED_I + 1, ldv, dtype=torch.float32, device=dev) for _ in range(NCOPY)]
+    wsf = [torch.empty(n, dtype=torch.float32, device=dev) for _ in range(NCOPY)]
+    prj = [torch.empty(FUSED_I + 1, dtype=torch.float64, device=dev) for _ in range(NCOPY)]
+    fused_ok = nch and lib.rsgpu_gmres_fused_step_f32(
+        hf, A.h, M32.h, Vs[0].data_ptr(), ldv, FUSED_I, wsf[0].data_ptr(),
+        prj[0].data_ptr()) == 0
+    for c in range(1, NCOPY if fused_ok else 1):
+        fused_ok = lib.rsgpu_gmres_fused_step_f32(
+            hf, mats[c].h, pre32[c].h, Vs[c].data_ptr(), ldv, FUSED_I, wsf[c].data_ptr(),
+            prj[c].data_ptr()) == 0
+    if fused_ok:
+        kt["gmres_fused_fp32"] = both(lambda c: lambda: lib.rsgpu_gmres_fused_step_f32(
+            hf, mats[c].h, pre32[c].h, Vs[c].data_ptr(), ldv, FUSED_I, wsf[c].data_ptr(),
+            prj[c].data_ptr()), max(10, R // 2))
+    else:
+        log("fused GMRES step not timed: " + _lib.load().rsgpu_last_error(ctx.h).decode())
     ctx.set_stream(None)
-    del mats[1:], pre64[1:], pre32[1:]
+    del mats[1:], pre64[1:], pre32[1:], Vs, wsf, prj
     ilu64_bytes, ilu32_bytes = ilu_bytes(n, nnz_lu, 8), ilu_bytes(n, nnz_lu, 4)
+    fz_bytes = fused_bytes(n, nnz, nnz_lu, 4, FUSED_I)
     for k, byt in (("spmv_csr_fp64", b64), ("spmv_csr_fp32", b32), ("spmv_sliced_fp64", b64),
                    ("spmv_sliced_fp32", b32), ("ilu_apply_fp64", ilu64_bytes),
-                   ("ilu_apply_fp32", ilu32_bytes)):
+                   ("ilu_apply_fp32", ilu32_bytes), ("gmres_fused_fp32", fz_bytes)):
         if k in kt:
             v = kt[k]
             v["algorithmic_bytes"] = byt
@@ -374,6 +421,26 @@ def main():
     spmv_key = "spmv_sliced_fp64" if nch else "spmv_csr_fp64"
     il = kt["ilu_apply_fp32"]
     sp = kt[spmv_key]
+    if "gmres_fused_fp32" in kt:
+        fz = kt["gmres_fused_fp32"]
+        fz["column"] = FUSED_I
+        roof = {"bound": "hbm", "achieved": fz["cold_GBps"], "peak": PEAK_HBM_GBS,
+                "unit": "GB/s", "frac": fz["cold_frac"],
+                "traffic": measured_traffic(pr, FUSED_KERNEL, FUSED_PMC_PROFILE),
+                "kernel": f"{FUSED_KERNEL} (the FP32 inner GMRES step: sliced SpMV, "
+                          "block-Jacobi ILU(0) apply and the first projection pass in one "
+                          f"launch, at column {FUSED_I}; the mixed solve's largest kernel "
+                          "time)",
+                "algorithmic_bytes": fz_bytes, "cold_us": fz["cold_us"],
+                "warm_us": fz["warm_us"]}
+    else:
+        roof = {"bound": "hbm", "achieved": il["cold_GBps"], "peak": PEAK_HBM_GBS,
+                "unit": "GB/s", "frac": il["cold_frac"],
+                "traffic": measured_traffic(pr, ILU32_KERNEL, ILU_PMC_PROFILE),
+                "kernel": f"{ILU32_KERNEL} (block-Jacobi ILU(0) apply of the FP32 inner "
+                          "solve)",
+                "algorithmic_bytes": ilu32_bytes, "cold_us": il["cold_us"],
+                "warm_us": il["warm_us"]}
     out = {
         "metric": "solves/s + SpMV HBM GB/s, 1M-cell MPFA, mixed vs FP64 at 1e-8 residual",
         "value": 1.0 / dtmx,
@@ -408,13 +475,7 @@ def main():
         "spmv": {"kernel": spmv_key, "cold_GBps": sp["cold_GBps"], "cold_us": sp["cold_us"],
                  "warm_us": sp["warm_us"], "frac": sp["cold_frac"],
                  "algorithmic_bytes": b64},
-        "roofline": {"bound": "hbm", "achieved": il["cold_GBps"], "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": il["cold_frac"],
-                     "traffic": measured_traffic(pr, ILU32_KERNEL, ILU_PMC_PROFILE),
-                     "kernel": f"{ILU32_KERNEL} (block-Jacobi ILU(0) apply of the FP32 inner "
-                               "solve, the mixed solve's largest kernel time)",
-                     "algorithmic_bytes": ilu32_bytes, "cold_us": il["cold_us"],
-                     "warm_us": il["warm_us"]},
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -1133,6 +1194,26 @@ def probe(a, A, n, nnz, lib, mtj, _lib, ctx, stream, dev):
             for _ in range(5):
                 lib.rsgpu_prec_apply(M.h, xin.data_ptr(), y.data_ptr())
         log(f"probe {a.probe}: levels {M.levels()}")
+    elif a.probe == "fused":
+        # the fused GMRES step at the bench's column, FP32 (the mixed inner
+        # solve's) and FP64
+        A.sync_f32()
+        ldv = (n + 3) // 4 * 4
+        for prec, dt, f in ((_lib.FP32, torch.float32, lib.rsgpu_gmres_fused_step_f32),
+                            (_lib.FP64, torch.float64, lib.rsgpu_gmres_fused_step)):
+            M = mtj.BlockILU(a.nblocks, precision=prec, ctx=ctx)
+            M.setMatrix(A)
+            g = mtj.GMRES(xin, a.restart)
+            g.cgs = True
+            h = g._handle(ctx)
+            V = torch.randn(FUSED_I + 1, ldv, dtype=dt, device=dev)
+            w = torch.empty(n, dtype=dt, device=dev)
+            pj = torch.empty(FUSED_I + 1, dtype=torch.float64, device=dev)
+            torch.cuda.synchronize()
+            for _ in range(max(5, reps // 5)):
+                assert f(h, A.h, M.h, V.data_ptr(), ldv, FUSED_I, w.data_ptr(),
+                         pj.data_ptr()) == 0
+            torch.cuda.synchronize()
     elif a.probe == "ilu32":
         M = mtj.BlockILU(a.nblocks, precision=_lib.FP32, ctx=ctx)
         M.setMatrix(A)

@@ -338,6 +338,19 @@ int rsgpu_solver_history(const rsgpu_solver *S, double *out_host, int cap);
  * with the cgs option on full line blocks, RSGPU_GMRES_FUSED=0 in the
  * environment turns it off).  No reference counterpart. */
 int rsgpu_solver_fused_steps(const rsgpu_solver *S);
+/* One fused GMRES inner step on caller vectors, for timing and testing the
+ * fused kernel (no reference counterpart): w = M^-1 (A v) with v = column i
+ * of V (no scaling), and proj[k] = V(:,k)' w for k = 0..i (device, FP64
+ * sums).  V is n x (i+1), column-major, leading dimension ldv (>= n, a
+ * multiple of 4; V and w 16-byte aligned, w not aliasing V); i below the
+ * solver's restart; A and M must meet the fused step's conditions.  The
+ * FP32 form needs rsgpu_csr_sync_f32 and an FP32 preconditioner. */
+int rsgpu_gmres_fused_step(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M,
+                           const double *V_dev, size_t ldv, int i, double *w_dev,
+                           double *proj_dev);
+int rsgpu_gmres_fused_step_f32(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M,
+                               const float *V_dev, size_t ldv, int i, float *w_dev,
+                               double *proj_dev);
 
 /* ---- Fully-implicit black-oil Jacobian (no reference counterpart) -------- */
 /* rs is IMPES (numerics/Discretisation.java:208-225 solves pressure only);

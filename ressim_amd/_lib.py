@@ -138,6 +138,10 @@ SIGNATURES = {
     "rsgpu_solver_solve_host": (C.c_int, [_h, _h, _h, _dp, _dp, C.POINTER(Stats)]),
     "rsgpu_solver_history": (C.c_int, [_h, _dp, C.c_int]),
     "rsgpu_solver_fused_steps": (C.c_int, [_h]),
+    "rsgpu_gmres_fused_step": (C.c_int, [_h, _h, _h, C.c_void_p, C.c_size_t, C.c_int,
+                                         C.c_void_p, C.c_void_p]),
+    "rsgpu_gmres_fused_step_f32": (C.c_int, [_h, _h, _h, C.c_void_p, C.c_size_t, C.c_int,
+                                             C.c_void_p, C.c_void_p]),
     "rsgpu_prec_set_blocks": (C.c_int, [_h, C.c_int, _ip]),
     "rsgpu_csr_create_bsr3": (C.c_int, [_h, C.c_int, C.c_int, _ip, _ip, _hp]),
     "rsgpu_csr_block_info": (C.c_int, [_h, C.POINTER(C.c_int), C.POINTER(C.c_int),

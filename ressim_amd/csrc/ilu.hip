@@ -2249,12 +2249,13 @@ __device__ __forceinline__ void band_load(const T *rp, const T *dp, const unsign
         st.m = mp[q];
 }
 
-template <typename T, bool UPPER, bool FULL, class Hook = NoHook>
+template <typename T, bool UPPER, bool FULL, class Hook = NoHook, int DMAX = 16>
 __device__ __forceinline__ void band_sweep(int Lx, int Ly, int g0, const T *rec, const T *dgl,
                                            const unsigned char *mask, T *xv, int junk,
                                            Hook hk = Hook())
 {
-    constexpr int D = band_depth<T, UPPER, FULL>();
+    constexpr int D0 = band_depth<T, UPPER, FULL>();
+    constexpr int D = D0 < DMAX ? D0 : DMAX;
     constexpr int dir = UPPER ? -1 : 1;
     const int j = threadIdx.x, jj = min(j, Ly - 1);
     const bool live = j < Ly;
@@ -2634,11 +2635,14 @@ static void launch_band(rsgpu_prec *M, const T *rec, const T *b, T *x, const int
 //   H(0..i, i) = V' w' (the first projection pass of classical
 //       Gram-Schmidt; column i taken from v_i as formed above).
 // One workgroup per block.  Wave 0 runs the two sweeps at raised issue
-// priority; the producer waves (fuse_producers) compute the block's SpMV rows, 64 rows
-// of one line at a time, directly into the sweep's LDS vector, in the order
-// the L sweep reaches them (the sweep waits on per-task flags only when it
-// catches up), and while the U sweep runs they store the finished rows of
-// w' and accumulate their projections on the basis.  Each SpMV row and each
+// priority; the producer waves (fuse_producers) compute the block's SpMV
+// rows, 64 rows of one line at a time, directly into the sweep's LDS vector,
+// in the order the L sweep reaches them (the sweep waits on per-task flags
+// only when it catches up), and while the U sweep runs they store the
+// finished rows of w' and accumulate their projections on the basis.  The
+// kernel is bound by memory bandwidth (SpMV values and factor records, then
+// factor records and basis), so the overlap hides the sweeps' dependency
+// chains rather than any traffic.  Each SpMV row and each
 // sweep result is that of the separate kernels bit for bit (same per-row
 // operation order); the projections are summed per lane over its rows, over
 // lanes and producer waves in a fixed order, then over blocks in block
@@ -2650,6 +2654,14 @@ template <typename T>
 constexpr int fuse_producers()
 {
     return sizeof(T) == 4 ? 11 : 7;
+}
+// record prefetch depth of the FP64 sweeps in the fused step (their own
+// depth: 11 FP64 producers with an 8-deep ring fit 168 registers but were
+// no faster, the SpMV part being bound by memory bandwidth)
+template <typename T>
+constexpr int fuse_ring()
+{
+    return 16;
 }
 template <typename T>
 constexpr int fuse_threads()
@@ -2755,7 +2767,7 @@ __global__ void __launch_bounds__(fuse_threads<T>())
                const T *__restrict__ recU,
                const T *__restrict__ dgl, const double *__restrict__ rdl, T *__restrict__ wout,
                const T *__restrict__ V, size_t ldv, int i, DevState *st, RedSlot slot,
-               int vec_elems)
+               int vec_elems, double *hout)
 {
     using VT = typename BandVec<T>::type;
     constexpr int N = BandVec<T>::N;
@@ -2814,8 +2826,10 @@ __global__ void __launch_bounds__(fuse_threads<T>())
             band_sweep_full<T, true, true>(Lx, Ly, SP, b0, recU, dgl, rdl, xv, hk);
         } else {
             const int junk = nb + RS_BAND_PAD - 1;
-            band_sweep<T, false, true>(Lx, Ly, b0, recL, dgl, nullptr, xv, junk, hk);
-            band_sweep<T, true, true>(Lx, Ly, b0, recU, dgl, nullptr, xv, junk, hk);
+            band_sweep<T, false, true, FuseHook, fuse_ring<T>()>(Lx, Ly, b0, recL, dgl, nullptr,
+                                                                 xv, junk, hk);
+            band_sweep<T, true, true, FuseHook, fuse_ring<T>()>(Lx, Ly, b0, recU, dgl, nullptr,
+                                                                xv, junk, hk);
         }
         __builtin_amdgcn_s_setprio(0);
     } else {
@@ -2982,8 +2996,12 @@ __global__ void __launch_bounds__(fuse_threads<T>())
             }
         }
         const double r = wave_sum(acc);
-        if (lane == 0)
-            Hb[k + i * ld] = r;
+        if (lane == 0) {
+            if (hout)
+                hout[k] = r;
+            else
+                Hb[k + i * ld] = r;
+        }
     }
 }
 
@@ -3019,7 +3037,7 @@ bool gmres_fused_ok(const rsgpu_csr *A, const rsgpu_prec *M, int elem_bytes, int
 template <typename T>
 void launch_gmres_fused(hipStream_t s, const rsgpu_csr *A, const T *slv, rsgpu_prec *M,
                         const T *win, T *vout, bool scale, T *wout, const T *V, size_t ldv,
-                        int i, DevState *st, RedSlot slot)
+                        int i, DevState *st, RedSlot slot, double *hout)
 {
     const bool f32 = sizeof(T) == 4;
     size_t vec = 0;
@@ -3045,16 +3063,18 @@ void launch_gmres_fused(hipStream_t s, const rsgpu_csr *A, const T *slv, rsgpu_p
                        slv, win, vout, scale ? 1 : 0, M->bstart.p, M->band_dims.p,
                        rec + (size_t)8 * RS_BAND_PAD, rec + (size_t)8 * RS_BAND_PAD + 4,
                        rec + 8 * np + RS_BAND_PAD, rdl, wout, V,
-                       ldv, i, st, slot, (int)vec);
+                       ldv, i, st, slot, (int)vec, hout);
     RS_CHECK_LAUNCH();
 }
 
 template void launch_gmres_fused<float>(hipStream_t, const rsgpu_csr *, const float *,
                                         rsgpu_prec *, const float *, float *, bool, float *,
-                                        const float *, size_t, int, DevState *, RedSlot);
+                                        const float *, size_t, int, DevState *, RedSlot,
+                                        double *);
 template void launch_gmres_fused<double>(hipStream_t, const rsgpu_csr *, const double *,
                                          rsgpu_prec *, const double *, double *, bool, double *,
-                                         const double *, size_t, int, DevState *, RedSlot);
+                                         const double *, size_t, int, DevState *, RedSlot,
+                                         double *);
 
 template <typename T, int TRANS>
 static void launch_apply(rsgpu_prec *M, const SweepData &A_, const T *va, const T *da,

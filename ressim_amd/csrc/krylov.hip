@@ -50,6 +50,7 @@ struct rsgpu_solver {
     // stream the sliced copy)
     const rsgpu_csr *sl64 = nullptr;
     int fused_steps = 0;          // fused GMRES steps enqueued by the last solve
+    DevBuf<DevState> st_probe;    // state of rsgpu_gmres_fused_step calls
 };
 
 // vector kernels: at least 4 elements per lane, at most 1024 groups
@@ -2589,6 +2590,87 @@ extern "C" int rsgpu_solver_solve(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M,
     } catch (const RsError &e) {
         return solver_catch(ctx, e);
     }
+}
+
+// One fused GMRES step on caller vectors (rsgpu_gmres_fused_step): the
+// solver's inner state holds the projections, the sliced FP64 copy is
+// refreshed when A's values changed since it was filled.
+template <typename T>
+static int fused_step_call(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const T *V, size_t ldv,
+                           int i, T *w, double *proj)
+{
+    if (!S || !A || !M || !V || !w || !proj)
+        return RSGPU_BAD_ARGUMENT;
+    rsgpu_ctx *ctx = S->ctx;
+    const int m = std::max(1, S->o.restart);
+    const bool f32 = sizeof(T) == 4;
+    const char *why = nullptr;
+    if (A->n != S->n || M->n != A->n)
+        why = "fused step: sizes differ";
+    else if (i < 0 || i + 1 > m)
+        why = "fused step: column i must be below the solver's restart";
+    else if (ldv < (size_t)A->n || ldv % 4 != 0 || ((uintptr_t)V & 15) || ((uintptr_t)w & 15))
+        why = "fused step: ldv must be >= n and a multiple of 4, V and w 16-byte aligned";
+    else if (!M->ready || M->pattern_id != A->pattern_id)
+        why = "Preconditioner.setMatrix must be called before use";
+    else if (f32 && !A->f32_current())
+        why = "fused step: FP32 form needs rsgpu_csr_sync_f32 after the last value change";
+    else if (!gmres_fused_ok(A, M, (int)sizeof(T), m))
+        why = "fused step: the matrix and preconditioner do not meet its conditions";
+    if (why) {
+        ctx->err = why;
+        return RSGPU_BAD_ARGUMENT;
+    }
+    try {
+        RS_HIP(hipSetDevice(ctx->device));
+        hipStream_t s = ctx->stream;
+        const size_t need = (size_t)(i + 1) * M->nblocks + 4096;
+        if (S->partials.n < need)
+            S->partials.alloc(need);
+        const T *slv;
+        if (f32) {
+            slv = (const T *)A->sl.v32.p;
+        } else {
+            if (A->sl.gen64 != A->val_gen)
+                sliced_fill<double>(A, s);
+            slv = (const T *)A->sl.v64.p;
+        }
+        // a state of its own, set once: the kernel reads its stop flags and
+        // writes the projections straight to proj
+        if (!S->st_probe.p) {
+            S->st_probe.alloc(1);
+            state_init(S, S->st_probe.p, S->hist_in, 1, 0.0, 0.0, 1e300, m, nullptr);
+        }
+        launch_gmres_fused<T>(s, A, slv, M, V + (size_t)i * ldv, nullptr, false, w, V, ldv, i,
+                              S->st_probe.p, slot_of(S), proj);
+        return RSGPU_OK;
+    } catch (const RsError &e) {
+        return solver_catch(ctx, e);
+    }
+}
+
+extern "C" int rsgpu_gmres_fused_step(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M,
+                                      const double *V, size_t ldv, int i, double *w,
+                                      double *proj)
+{
+    if (M && M->precision != RSGPU_FP64) {
+        if (S)
+            S->ctx->err = "fused step: FP64 form needs an FP64 preconditioner";
+        return RSGPU_BAD_ARGUMENT;
+    }
+    return fused_step_call<double>(S, A, M, V, ldv, i, w, proj);
+}
+
+extern "C" int rsgpu_gmres_fused_step_f32(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M,
+                                          const float *V, size_t ldv, int i, float *w,
+                                          double *proj)
+{
+    if (M && M->precision != RSGPU_FP32) {
+        if (S)
+            S->ctx->err = "fused step: FP32 form needs an FP32 preconditioner";
+        return RSGPU_BAD_ARGUMENT;
+    }
+    return fused_step_call<float>(S, A, M, V, ldv, i, w, proj);
 }
 
 extern "C" int rsgpu_solver_fused_steps(const rsgpu_solver *S)

@@ -596,12 +596,13 @@ void prec_trans_apply(rsgpu_prec *M, const double *b, double *x,
 // GMRES inner step fused into one launch (sliced SpMV with the basis scale,
 // line-wavefront block ILU(0) apply, first projection pass onto V): whether
 // it applies to (A, M) with elements of elem_bytes and restart m, and its
-// launch (wout must not alias win)
+// launch (wout must not alias win; the projections go to column i of the
+// state's H, or to hout when that is given)
 bool gmres_fused_ok(const rsgpu_csr *A, const rsgpu_prec *M, int elem_bytes, int m);
 template <typename T>
 void launch_gmres_fused(hipStream_t s, const rsgpu_csr *A, const T *slv, rsgpu_prec *M,
                         const T *win, T *vout, bool scale, T *wout, const T *V, size_t ldv,
-                        int i, DevState *st, RedSlot slot);
+                        int i, DevState *st, RedSlot slot, double *hout = nullptr);
 
 // amg.hip
 int amg_setup(rsgpu_prec *M, rsgpu_csr *A, int *bad_row);

@@ -783,6 +783,20 @@ class GMRES(AbstractIterativeSolver):
             raise ValueError("restart must be a positive integer")
         self.restart = restart
 
+    def fused_step(self, A: "CompRowMatrix", M: "Preconditioner", V, i: int, w, proj):
+        """One fused inner step on device tensors (rsgpu_gmres_fused_step):
+        w = M^-1 (A V[:, i]) and proj[k] = V[:, k]' w for k <= i.  V is a
+        column-major (ldv x cols) view: a torch tensor of shape (cols, ldv)
+        whose rows are the basis vectors (float64, or float32 for the FP32
+        form)."""
+        lib = L.load()
+        h = self._handle(A.ctx)
+        _torch_sync(V, w)
+        f = lib.rsgpu_gmres_fused_step_f32 if V.dtype.is_floating_point and \
+            V.element_size() == 4 else lib.rsgpu_gmres_fused_step
+        code = f(h, A.h, M.h, V.data_ptr(), V.stride(0), i, w.data_ptr(), proj.data_ptr())
+        _raise(code, A.ctx.h)
+
 
 class IR(AbstractIterativeSolver):
     """mtj/.../sparse/IR.java."""

@@ -132,3 +132,55 @@ def test_mixed_fused_vs_separate_and_oracle(gpu, case):
     assert sf.true_residual <= 1e-10 * sf.b_norm
     assert abs(sf.inner_iterations - ss.inner_iterations) <= max(3, ss.inner_iterations // 20)
     assert _relerr(xf, xo) <= 1e-8 and _relerr(xs, xo) <= 1e-8
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_fused_step_operator_bitexact_and_projections(gpu, oracle, prec):
+    """rsgpu_gmres_fused_step on caller vectors: w is bit-identical to
+    CompRowMatrix.mult followed by the block ILU(0) apply (the separate
+    kernels), and proj = V' w (products in the working precision, sums in
+    FP64) to 1e-12 of the sum of |terms|; out-of-range calls are refused."""
+    import torch
+    from ressim_amd import _lib
+    pr = P.config2(nx=128, ny=32, nz=8)
+    s = oracle_system(oracle, pr)
+    n, nb, i = pr.n, 8, 7
+    f32 = prec == "fp32"
+    dt = torch.float32 if f32 else torch.float64
+    lib = _lib.load()
+    A = gpu.CompRowMatrix.from_csr(n, n, s["rowptr"], s["colind"], s["data"])
+    if f32:
+        A.sync_f32()
+    M = gpu.BlockILU(nb, precision=_lib.FP32 if f32 else _lib.FP64)
+    M.setMatrix(A)
+    g = gpu.GMRES(s["b"], 30)
+    g.cgs = True
+    ldv = (n + 3) // 4 * 4 + 4
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    V = torch.randn(i + 1, ldv, dtype=dt, device="cuda", generator=gen)
+    w = torch.empty(n, dtype=dt, device="cuda")
+    proj = torch.empty(i + 1, dtype=torch.float64, device="cuda")
+    g.fused_step(A, M, V, i, w, proj)
+    torch.cuda.synchronize()
+    u = torch.empty(n, dtype=dt, device="cuda")
+    z = torch.empty(n, dtype=dt, device="cuda")
+    x = V[i, :n].contiguous()
+    if f32:
+        assert lib.rsgpu_csr_mult_f32(A.h, x.data_ptr(), u.data_ptr()) == 0
+        assert lib.rsgpu_prec_apply_f32(M.h, u.data_ptr(), z.data_ptr()) == 0
+    else:
+        assert lib.rsgpu_csr_mult(A.h, x.data_ptr(), u.data_ptr()) == 0
+        assert lib.rsgpu_prec_apply(M.h, u.data_ptr(), z.data_ptr()) == 0
+    torch.cuda.synchronize()
+    assert torch.equal(w, z)
+    terms = V[:, :n] * w                      # products in the working precision
+    ref = terms.double().sum(dim=1)
+    scale = terms.double().abs().sum(dim=1)
+    assert torch.all((proj - ref).abs() <= 1e-12 * scale)
+    # refused: a column at the restart, a leading dimension not a multiple of 4
+    h = g._handle(A.ctx)
+    f = lib.rsgpu_gmres_fused_step_f32 if f32 else lib.rsgpu_gmres_fused_step
+    assert f(h, A.h, M.h, V.data_ptr(), ldv, 30, w.data_ptr(), proj.data_ptr()) == \
+        _lib.BAD_ARGUMENT
+    assert f(h, A.h, M.h, V.data_ptr(), ldv - 2, i, w.data_ptr(), proj.data_ptr()) == \
+        _lib.BAD_ARGUMENT
