@@ -2721,34 +2721,6 @@ __device__ __forceinline__ void fuse_pin(const double2 &v)
     asm volatile("" ::"v"(v.x), "v"(v.y));
 }
 
-// Sums of a[0..31] over the wave's 64 lanes: lane l returns column l >> 1.
-// Each exchange round halves the columns a lane keeps (it sends the half
-// its partner keeps), so the 32 sums take 32 shuffles instead of 32 trees
-// of 6; the summation order is fixed.
-__device__ __forceinline__ double wave_sum_cols32(const double (&a)[32])
-{
-    const int lane = threadIdx.x & 63;
-    double b[16];
-    {
-        const bool hi = lane & 32;
-#pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const double keep = hi ? a[q + 16] : a[q], send = hi ? a[q] : a[q + 16];
-            b[q] = keep + __shfl_xor(send, 32, 64);
-        }
-    }
-#pragma unroll
-    for (int w = 8, o = 16; w >= 1; w >>= 1, o >>= 1) {
-        const bool hi = lane & o;
-#pragma unroll
-        for (int q = 0; q < w; ++q) {
-            const double keep = hi ? b[q + w] : b[q], send = hi ? b[q] : b[q + w];
-            b[q] = keep + __shfl_xor(send, o, 64);
-        }
-    }
-    return b[0] + __shfl_xor(b[0], 1, 64);
-}
-
 // SpMV rows into the sweep's LDS vector (CompRowMatrix.mult's zero rule):
 // row r of the matrix at xs[r + base]
 template <typename T>
@@ -2759,7 +2731,64 @@ struct EpiLds {
     __device__ void row(int r, T dot) { xs[r + base] = (dot != T(0)) ? dot : T(0); }
 };
 
-template <typename T>
+// The delayed reorthogonalisation step's column work (one wave of the
+// finishing group of step i).  colsum holds z = V(:,0..i-1)' w, beta =
+// v~_i' w, then (i > 0) s = V(:,0..i-1)' v~_i and alpha = v~_i' v~_i.  With
+// rho = sqrt(alpha - s's) and eta = st->hsub (the norm v~_i was scaled by),
+// the reorthogonalised v_i = (v~_i - V s) / rho has M^-1 A v_i =
+// (w - V Hr s) / rho, so column i of the unrotated Hessenberg is
+// Hr(r, i) = ([z; c]_r - (Hr s)_r) / rho with c = (beta - s'z) / rho, the
+// product taken over column i - 1 corrected as in dcgs_finish_column_wave
+// (which then finishes column i - 1).  The update kernel forms v_i and
+// (w - V [z; c]) / rho from ds, rho and c.
+__device__ void dcgs_step_wave(DevState *st, int i, bool reo, const double *colsum, double *lh)
+{
+    const int lane = threadIdx.x & 63;
+    const int ld = st->m + 1, cnt = i + 1;
+    double *Hr = st->Hr;
+    if (!reo) {
+        if (lane == 0) {
+            Hr[i * ld] = colsum[0];
+            st->dc = colsum[0];
+            st->drho = 1.0;
+        }
+        return;
+    }
+    const double *sl = colsum + cnt;
+    const double sk = lane < i ? sl[lane] : 0.0;
+    const double zk = lane < i ? colsum[lane] : 0.0;
+    const double alpha = sl[i], beta = colsum[i];
+    const double ss = wave_sum(sk * sk);
+    const double rho = alpha - ss > 0 ? sqrt(alpha - ss) : 0.0;
+    const double sz = wave_sum(sk * zk);
+    const double c = (beta - sz) / rho;
+    const double eta = st->hsub;
+    if (lane <= i) {
+        double hs = 0;
+        for (int k = max(lane - 1, 0); k < i; ++k) {
+            const double hk = k < i - 1 ? Hr[lane + k * ld]
+                              : lane < i ? Hr[lane + k * ld] + eta * sl[lane]
+                                         : eta * rho;
+            hs = hs + hk * sl[k];
+        }
+        const double zr = lane < i ? zk : c;
+        Hr[lane + i * ld] = (zr - hs) / rho;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    dcgs_finish_column_wave(st, i - 1, lh, sl);
+    if (lane == 0) {
+        st->dc = c;
+        st->drho = rho;
+    }
+}
+
+// DCGS (FP64 only): the delayed reorthogonalisation form.  Phase B also
+// projects the step's input vector v~_i (column i as the SpMV formed it) on
+// the basis (the reorthogonalisation coefficients of v~_i and its squared
+// norm), and the finishing wave finishes column i - 1 with them and forms
+// column i of the unrotated Hessenberg (dcgs_step_wave).
+template <typename T, bool DCGS>
 __global__ void __launch_bounds__(fuse_threads<T>())
     k_gm_fused(SlView sv, const T *__restrict__ val, const T *__restrict__ win,
                T *__restrict__ vout, int scale, const int *__restrict__ bstart,
@@ -2775,6 +2804,9 @@ __global__ void __launch_bounds__(fuse_threads<T>())
     constexpr int P = fuse_producers<T>(), BS = fuse_threads<T>();
     extern __shared__ unsigned char rs_smem[];
     __shared__ double sh[P][RS_FUSE_COLS];
+    __shared__ double sh2[DCGS ? P : 1][RS_FUSE_COLS];
+    __shared__ double colsum[DCGS ? 2 * RS_FUSE_COLS : 1];
+    __shared__ double lh[DCGS ? 3 * 64 : 1];
     __shared__ int uprog, err, last;
     double *Hb = st->H;
     const int ld = st->m + 1;
@@ -2813,11 +2845,10 @@ __global__ void __launch_bounds__(fuse_threads<T>())
     }
     __syncthreads();
     FuseHook hk{flags, &uprog, &err, ntA, Ly};
-    double a[RS_FUSE_COLS];
-#pragma unroll
-    for (int q = 0; q < RS_FUSE_COLS; ++q)
-        a[q] = 0;
     const int cnt = i + 1;
+    // projection columns: V' w, then (DCGS, i > 0) V' v~_i
+    const bool reo = DCGS && scale;
+    const int ncol = reo ? 2 * cnt : cnt;
     if (wid == 0) {
         // ---- the sweeps
         __builtin_amdgcn_s_setprio(3);
@@ -2834,6 +2865,12 @@ __global__ void __launch_bounds__(fuse_threads<T>())
         __builtin_amdgcn_s_setprio(0);
     } else {
         const int p = wid - 1;
+        double a[RS_FUSE_COLS], a2[RS_FUSE_COLS];
+#pragma unroll
+        for (int q = 0; q < RS_FUSE_COLS; ++q) {
+            a[q] = 0;
+            a2[q] = 0;
+        }
         // ---- phase A: SpMV rows, line segments in the L sweep's order
         for (int t = p; t < ntA; t += P) {
             const int h = t / Ly, j = t - h * Ly;
@@ -2924,17 +2961,32 @@ __global__ void __launch_bounds__(fuse_threads<T>())
                         for (int e = 0; e < N; ++e)
                             acc = acc + (double)(we[e] * ke[e]);
                         a[k] = (live && k < cnt) ? acc : a[k];
+                        if constexpr (DCGS) {
+                            const T *ve = reinterpret_cast<const T *>(&vi);
+                            double acc2 = a2[k];
+#pragma unroll
+                            for (int e = 0; e < N; ++e)
+                                acc2 = acc2 + (double)(ve[e] * ke[e]);
+                            a2[k] = (reo && live && k < cnt) ? acc2 : a2[k];
+                        }
                     }
                 }
             }
         }
-    }
-    // ---- projections: producer waves' sums, then the block's partials
-    static_assert(RS_FUSE_COLS == 32, "wave_sum_cols32 sums 32 columns");
-    if (wid > 0) {
-        const double r = wave_sum_cols32(a);
-        if ((lane & 1) == 0 && (lane >> 1) < cnt)
-            sh[wid - 1][lane >> 1] = r;
+        // ---- projections: the wave's sums per column
+        static_assert(RS_FUSE_COLS == 32, "wave_sum_cols32 sums 32 columns");
+        {
+            const double r = wave_sum_cols32(a);
+            if ((lane & 1) == 0 && (lane >> 1) < cnt)
+                sh[p][lane >> 1] = r;
+        }
+        if constexpr (DCGS) {
+            if (reo) {
+                const double r = wave_sum_cols32(a2);
+                if ((lane & 1) == 0 && (lane >> 1) < cnt)
+                    sh2[p][lane >> 1] = r;
+            }
+        }
     }
     __syncthreads();
     // a wait that ran out (never expected: every producer finishes its
@@ -2943,12 +2995,13 @@ __global__ void __launch_bounds__(fuse_threads<T>())
         st->status = RSGPU_HIP_ERROR;
         st->done = 1;
     }
-    if (threadIdx.x < cnt) {
+    if ((int)threadIdx.x < ncol) {
+        const int k = (int)threadIdx.x;
         double r = 0;
 #pragma unroll
         for (int u = 0; u < P; ++u)
-            r += sh[u][threadIdx.x];
-        __hip_atomic_store(&slot.partials[(size_t)threadIdx.x * G + b], r, __ATOMIC_RELAXED,
+            r += k < cnt ? sh[u][k] : sh2[DCGS ? u : 0][k - cnt];
+        __hip_atomic_store(&slot.partials[(size_t)k * G + b], r, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     }
     // arrival ticket (as block_reduce_finish); thread 0 is in wave 0, which
@@ -2980,7 +3033,7 @@ __global__ void __launch_bounds__(fuse_threads<T>())
     // wave wid sums columns wid, wid + 8, ... over the blocks in order; all
     // loads of a batch (4 partials per lane) in flight before the adds
     constexpr int NW = BS / 64;
-    for (int k = wid; k < cnt; k += NW) {
+    for (int k = wid; k < ncol; k += NW) {
         double acc = 0;
         for (int b1 = lane; b1 < G; b1 += 256) {
             double t4[4];
@@ -2997,11 +3050,23 @@ __global__ void __launch_bounds__(fuse_threads<T>())
         }
         const double r = wave_sum(acc);
         if (lane == 0) {
-            if (hout)
+            if constexpr (DCGS) {
+                colsum[k] = r;
+                if (k < cnt)
+                    Hb[k + i * ld] = r;
+                else
+                    st->ds[k - cnt] = r;
+            } else if (hout) {
                 hout[k] = r;
-            else
+            } else {
                 Hb[k + i * ld] = r;
+            }
         }
+    }
+    if constexpr (DCGS) {
+        __syncthreads();
+        if (wid == 0)
+            dcgs_step_wave(st, i, reo, colsum, lh);
     }
 }
 
@@ -3037,7 +3102,7 @@ bool gmres_fused_ok(const rsgpu_csr *A, const rsgpu_prec *M, int elem_bytes, int
 template <typename T>
 void launch_gmres_fused(hipStream_t s, const rsgpu_csr *A, const T *slv, rsgpu_prec *M,
                         const T *win, T *vout, bool scale, T *wout, const T *V, size_t ldv,
-                        int i, DevState *st, RedSlot slot, double *hout)
+                        int i, DevState *st, RedSlot slot, double *hout, bool dcgs)
 {
     const bool f32 = sizeof(T) == 4;
     size_t vec = 0;
@@ -3048,33 +3113,34 @@ void launch_gmres_fused(hipStream_t s, const rsgpu_csr *A, const T *slv, rsgpu_p
                                       2 * RS_BAND_PAD);
     }
     const size_t shm = vec * sizeof(T) + (size_t)(M->max_block_rows / 64 + 1) * 4;
-    static bool attr = false;
-    if (!attr && shm > 65536) {
-        RS_HIP(hipFuncSetAttribute((const void *)k_gm_fused<T>,
-                                   hipFuncAttributeMaxDynamicSharedMemorySize,
+    // the delayed reorthogonalisation form exists in FP64 only
+    const bool dc = dcgs && !f32;
+    auto kern = dc ? k_gm_fused<T, !f32> : k_gm_fused<T, false>;
+    static bool attr[2] = {false, false};
+    if (!attr[dc] && shm > 65536) {
+        RS_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    lds_limit(M->ctx->device)));
-        attr = true;
+        attr[dc] = true;
     }
     const size_t np = (size_t)M->n + 2 * RS_BAND_PAD;
     const T *rec = f32 ? (const T *)M->band32.p : (const T *)M->band64.p;
     const double *rdl = (f32 && M->band_rd.p) ? M->band_rd.p + RS_BAND_PAD : nullptr;
-    hipLaunchKernelGGL(k_gm_fused<T>, dim3(M->nblocks), dim3(fuse_threads<T>()), shm, s,
-                       A->slview(),
+    hipLaunchKernelGGL(kern, dim3(M->nblocks), dim3(fuse_threads<T>()), shm, s, A->slview(),
                        slv, win, vout, scale ? 1 : 0, M->bstart.p, M->band_dims.p,
                        rec + (size_t)8 * RS_BAND_PAD, rec + (size_t)8 * RS_BAND_PAD + 4,
-                       rec + 8 * np + RS_BAND_PAD, rdl, wout, V,
-                       ldv, i, st, slot, (int)vec, hout);
+                       rec + 8 * np + RS_BAND_PAD, rdl, wout, V, ldv, i, st, slot, (int)vec,
+                       hout);
     RS_CHECK_LAUNCH();
 }
 
 template void launch_gmres_fused<float>(hipStream_t, const rsgpu_csr *, const float *,
                                         rsgpu_prec *, const float *, float *, bool, float *,
                                         const float *, size_t, int, DevState *, RedSlot,
-                                        double *);
+                                        double *, bool);
 template void launch_gmres_fused<double>(hipStream_t, const rsgpu_csr *, const double *,
                                          rsgpu_prec *, const double *, double *, bool, double *,
                                          const double *, size_t, int, DevState *, RedSlot,
-                                         double *);
+                                         double *, bool);
 
 template <typename T, int TRANS>
 static void launch_apply(rsgpu_prec *M, const SweepData &A_, const T *va, const T *da,

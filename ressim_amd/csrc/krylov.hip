@@ -51,6 +51,7 @@ struct rsgpu_solver {
     const rsgpu_csr *sl64 = nullptr;
     int fused_steps = 0;          // fused GMRES steps enqueued by the last solve
     DevBuf<DevState> st_probe;    // state of rsgpu_gmres_fused_step calls
+    DevBuf<double> dcgs;          // FP64 DCGS2: Hr ((m+1) x m) and ds (m+2)
 };
 
 // vector kernels: at least 4 elements per lane, at most 1024 groups
@@ -923,78 +924,6 @@ __device__ void gm_finish_column(DevState *st, int i, double wnorm2)
     }
 }
 
-// gm_finish_column by the first wave of the finishing group (all 64 lanes
-// active; tot valid in lane 0).  The column's entries, the earlier
-// rotations and s(i), s(i+1) are fetched with one load per lane into LDS, lane
-// 0 applies the rotations in order from LDS (the operations and their order
-// of gm_finish_column, so the results are bit-identical), and the rotated
-// entries are stored back one per lane: a few memory round trips instead of
-// one per dependent access of the sequential form.  i + 1 < 64.
-__device__ void gm_finish_column_wave(DevState *st, int i, double wnorm2, double *lh)
-{
-    const int lane = threadIdx.x & 63;
-    const int ld = st->m + 1;
-    double *Hc = st->H + i * ld;
-    double *gcp = st->gc, *gsp = st->gs, *sv = st->s;
-    double *lc = lh + 64, *ls = lh + 128;
-    if (lane <= i)
-        lh[lane] = Hc[lane];
-    if (lane < i) {
-        lc[lane] = gcp[lane];
-        ls[lane] = gsp[lane];
-    }
-    const double s0 = sv[i], s1 = sv[i + 1];
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    if (lane == 0) {
-        const double hn = sqrt(wnorm2);
-        st->hsub = hn;
-        // previous rotations on column i; cur is the running Hc[q]
-        double cur = lh[0];
-        for (int q = 0; q < i; ++q) {
-            const double c = lc[q], s = ls[q], nx = lh[q + 1];
-            const double temp = c * cur + s * nx;
-            cur = -s * cur + c * nx;
-            lh[q] = temp;
-        }
-        // new rotation (GivensRotation(x, y))
-        const double x = cur, y = hn;
-        const double roe = fabs(x) > fabs(y) ? x : y;
-        const double scale = fabs(x) + fabs(y);
-        double c, s;
-        if (scale != 0) {
-            const double xs = x / scale, ys = y / scale;
-            double rr = scale * sqrt(xs * xs + ys * ys);
-            if (roe < 0)
-                rr *= -1;
-            c = x / rr;
-            s = y / rr;
-        } else {
-            c = 1;
-            s = 0;
-        }
-        gcp[i] = c;
-        gsp[i] = s;
-        Hc[i] = c * x + s * y;
-        Hc[i + 1] = -s * x + c * y;
-        const double t0 = c * s0 + s * s1;
-        const double t1 = -s * s0 + c * s1;
-        sv[i] = t0;
-        sv[i + 1] = t1;
-        st->iter++;
-        st->gi = i + 1;
-        if (!(i + 1 < st->m) || dev_monitor(st, fabs(t1))) {
-            st->cycle_end = 1;
-            if (st->status)
-                st->done = 1;
-        }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    if (lane < i)
-        Hc[lane] = lh[lane];
-}
-
 // MGS step k of column i: w -= H(k-1,i) v_{k-1} (k > 0), H(k,i) = w . v_k
 template <typename T>
 __global__ void __launch_bounds__(RS_VBS)
@@ -1084,7 +1013,7 @@ __global__ void __launch_bounds__(RS_VBS)
     __shared__ double lh[3 * 64];
     if (block_reduce_finish<RS_VBS, 1>(slot, v, tot) && threadIdx.x < 64) {
         if (i + 1 < 64)
-            gm_finish_column_wave(st, i, __shfl(tot[0], 0, 64), lh);
+            gm_finish_column_wave(st, i, sqrt(__shfl(tot[0], 0, 64)), lh);
         else if (threadIdx.x == 0)
             gm_finish_column(st, i, tot[0]);
     }
@@ -1380,7 +1309,7 @@ __global__ void __launch_bounds__(RS_VBS)
             // its own store in gm_finish_column_wave)
             if (PASS == 2 && (int)threadIdx.x <= i)
                 Hb[threadIdx.x + i * ld] += h2b[threadIdx.x];
-            gm_finish_column_wave(st, i, __shfl(tot[0], 0, 64), lh);
+            gm_finish_column_wave(st, i, sqrt(__shfl(tot[0], 0, 64)), lh);
         } else if (threadIdx.x == 0) {
             if (PASS == 2) {
                 double *Hi = Hb + i * ld;
@@ -1511,6 +1440,174 @@ __global__ void __launch_bounds__(RS_VBS)
         if (lane == 0)
             h2b[k] = r;
     }
+}
+
+// FP64 GMRES with delayed reorthogonalisation (DCGS2; no reference
+// counterpart, the orthogonalisation of the FP64 drop-in's fused step): the
+// update of step i.  With s = st->ds (the reorthogonalisation coefficients
+// of v~_i = V(:, i) on V(:, 0..i-1)), z = H(0..i-1, i) (the projections of
+// w on them), rho = st->drho and c = st->dc from the fused step's finishing
+// wave:
+//   v_i = (v~_i - sum_k s_k v_k) / rho      (stored over v~_i; i > 0)
+//   u   = (w - sum_k z_k v_k - c v_i) / rho  (stored over w)
+// each element's sums in k order, one pass over the basis; then
+// eta = ||u||, the subdiagonal Hr(i+1, i) and the next step's scale.
+__global__ void __launch_bounds__(RS_VBS)
+    k_gm_dcgs2_update(int n, int i, double *__restrict__ w, double *__restrict__ V, size_t ldv,
+                      DevState *st, RedSlot slot)
+{
+    __shared__ double zs[RS_UD_COLS], ss[RS_UD_COLS];
+    if (stopped(st, true))
+        return;
+    const int ld = st->m + 1;
+    const double *Hc = st->H + i * ld;
+    const double rho = st->drho, c = st->dc;
+    if ((int)threadIdx.x < i) {
+        zs[threadIdx.x] = Hc[threadIdx.x];
+        ss[threadIdx.x] = st->ds[threadIdx.x];
+    }
+    __syncthreads();
+    double *vi = V + (size_t)i * ldv;
+    double a = 0;
+    for (int j = blockIdx.x * RS_VBS + threadIdx.x; j < n; j += gridDim.x * RS_VBS) {
+        double vt = vi[j], wj = w[j];
+        for (int k0 = 0; k0 < i; k0 += 8) {
+            double kv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                kv[u] = V[(size_t)min(k0 + u, i - 1) * ldv + j];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                asm volatile("" ::"v"(kv[u]));
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const bool on = k0 + u < i;
+                const double v1 = vt - ss[min(k0 + u, i - 1)] * kv[u];
+                const double w1 = wj - zs[min(k0 + u, i - 1)] * kv[u];
+                vt = on ? v1 : vt;
+                wj = on ? w1 : wj;
+            }
+        }
+        const double vn = i > 0 ? vt / rho : vt;
+        const double u = (wj - c * vn) / rho;
+        if (i > 0)
+            vi[j] = vn;
+        w[j] = u;
+        a += u * u;
+    }
+    double v[1] = {a}, tot[1];
+    if (block_reduce_finish<RS_VBS, 1>(slot, v, tot) && threadIdx.x == 0) {
+        const double eta = sqrt(tot[0]);
+        st->Hr[i + 1 + i * ld] = eta;
+        st->hsub = eta;
+    }
+}
+
+// DCGS2: the end of a full cycle finishes column m - 1 (the fused step of a
+// following column would have): s = V(:, 0..m-1)' v~_m and alpha = v~_m'
+// v~_m with v~_m = w / st->hsub formed on the fly as the fused step forms
+// its input, then dcgs_finish_column_wave.  Skipped when the cycle ended
+// earlier.
+__global__ void __launch_bounds__(RS_VBS)
+    k_gm_dcgs2_close(int n, int m, const double *__restrict__ w, const double *__restrict__ V,
+                     size_t ldv, DevState *st, RedSlot slot)
+{
+    constexpr int NW = RS_VBS / 64;
+    __shared__ double sh[NW][32];
+    __shared__ double colsum[32];
+    __shared__ double lh[3 * 64];
+    __shared__ int last;
+    if (stopped(st, true))
+        return;
+    const int G = (int)gridDim.x;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const double sc = 1. / st->hsub;
+    double a[32];
+#pragma unroll
+    for (int q = 0; q < 32; ++q)
+        a[q] = 0;
+    for (int j = blockIdx.x * RS_VBS + threadIdx.x; j < n; j += G * RS_VBS) {
+        const double vt = sc == 0.0 ? 0.0 : sc * w[j];
+#pragma unroll
+        for (int k0 = 0; k0 < 32; k0 += 8) {
+            if (k0 <= m) {
+                double kv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    kv[u] = V[(size_t)min(k0 + u, m - 1) * ldv + j];
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    asm volatile("" ::"v"(kv[u]));
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int k = k0 + u;
+                    const double f = k < m ? kv[u] : vt;
+                    const double p = a[k] + vt * f;
+                    a[k] = k <= m ? p : a[k];
+                }
+            }
+        }
+    }
+    {
+        const double r = wave_sum_cols32(a);
+        if ((lane & 1) == 0 && (lane >> 1) <= m)
+            sh[wid][lane >> 1] = r;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x <= m) {
+        double r = 0;
+#pragma unroll
+        for (int u = 0; u < NW; ++u)
+            r += sh[u][threadIdx.x];
+        __hip_atomic_store(&slot.partials[(size_t)threadIdx.x * G + blockIdx.x], r,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (threadIdx.x == 0) {
+        wait_vm_stores();
+        const int ng = G < RS_NGROUP ? G : RS_NGROUP;
+        const int g = (int)blockIdx.x % ng;
+        const int gsize = (G - g + ng - 1) / ng;
+        unsigned *gt = slot.ticket + (1 + g) * RS_TPAD;
+        int fin = 0;
+        const unsigned t =
+            __hip_atomic_fetch_add(gt, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((int)t == gsize - 1) {
+            __hip_atomic_store(gt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned u = __hip_atomic_fetch_add(slot.ticket, 1u, __ATOMIC_RELAXED,
+                                                      __HIP_MEMORY_SCOPE_AGENT);
+            if ((int)u == ng - 1) {
+                __hip_atomic_store(slot.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                fin = 1;
+            }
+        }
+        last = fin;
+    }
+    __syncthreads();
+    if (!last)
+        return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    for (int k = wid; k <= m; k += NW) {
+        double acc = 0;
+        for (int b1 = lane; b1 < G; b1 += 256) {
+            double t4[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                t4[u] = __hip_atomic_load(&slot.partials[(size_t)k * G + min(b1 + 64 * u, G - 1)],
+                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            acc += t4[0];
+#pragma unroll
+            for (int u = 1; u < 4; ++u) {
+                const double s_ = acc + t4[u];
+                acc = b1 + 64 * u < G ? s_ : acc;
+            }
+        }
+        const double r = wave_sum(acc);
+        if (lane == 0)
+            colsum[k] = r;
+    }
+    __syncthreads();
+    if (wid == 0)
+        dcgs_finish_column_wave(st, m - 1, lh, colsum);
 }
 
 // v_{i+1} = (1/H(i+1,i)) w
@@ -1682,7 +1779,7 @@ const float *csr_values<float>(rsgpu_csr *A) { return A->val32.p; }
 
 void state_init(rsgpu_solver *S, DevState *dst, DevBuf<double> &hist,
                 int max_iter, double rtol, double atol, double dtol, int m,
-                double *small)
+                double *small, double *dcgs = nullptr)
 {
     DevState h;
     std::memset(&h, 0, sizeof(h));
@@ -1703,6 +1800,10 @@ void state_init(rsgpu_solver *S, DevState *dst, DevBuf<double> &hist,
         h.gc = h.s + (m + 1);
         h.gs = h.gc + (m + 1);
         h.h2 = h.gs + (m + 1);
+    }
+    if (dcgs) {
+        h.Hr = dcgs;
+        h.ds = dcgs + (size_t)(m + 1) * m;
     }
     RS_HIP(hipMemcpyAsync(dst, &h, sizeof(h), hipMemcpyHostToDevice,
                           S->ctx->stream));
@@ -2165,13 +2266,21 @@ void run_gmres(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const T *b, T *x,
         // (RSGPU_CGS_UD=0: separate kernels)
         const char *eud = getenv("RSGPU_CGS_UD");
         const bool cgs_ud = sizeof(T) == 8 && m + 1 <= RS_UD_COLS && !(eud && eud[0] == '0');
+        // FP64 fused steps: delayed reorthogonalisation (DCGS2) in place of
+        // CGS2 (RSGPU_DCGS2=0: CGS2)
+        const char *edc = getenv("RSGPU_DCGS2");
+        const bool dcgs = fused && sizeof(T) == 8 && S->dcgs.p && !(edc && edc[0] == '0') &&
+                          (size_t)2 * (m + 1) * M->nblocks <= S->partials.n;
         for (int i = 0; i < m; ++i) {
             T *vi = V + (size_t)i * L;
             if (fused) {
                 launch_gmres_fused<T>(s, A, slv, M, i > 0 ? wcur : vi, vi, i > 0, wnext, V, L,
-                                      i, st, sl);
+                                      i, st, sl, nullptr, dcgs);
                 ++S->fused_steps;
-                if (sizeof(T) == 4) {
+                if (dcgs) {
+                    hipLaunchKernelGGL(k_gm_dcgs2_update, dim3(G), dim3(RS_VBS), 0, s, n, i,
+                                       (double *)wnext, (double *)V, L, st, sl);
+                } else if (sizeof(T) == 4) {
                     hipLaunchKernelGGL((k_gm_cgs_last<T, 0>), dim3(G), dim3(RS_VBS), 0, s, n, i,
                                        wnext, V, L, st, sl);
                 } else {
@@ -2246,6 +2355,10 @@ void run_gmres(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const T *b, T *x,
                     break;
             }
         }
+        // DCGS2: a full cycle's last column is finished here
+        if (dcgs)
+            hipLaunchKernelGGL(k_gm_dcgs2_close, dim3(G), dim3(RS_VBS), 0, s, n, m,
+                               (const double *)wcur, (const double *)V, L, st, sl);
         hipLaunchKernelGGL(k_gm_backsolve, dim3(1), dim3(64), 0, s, st);
         hipLaunchKernelGGL(k_gm_xupdate<T>, dim3(E), dim3(RS_VBS), 0, s, n, V, L, x, st);
         launch_resid<T>(S, A, b, x, u, nullptr, nullptr, st, FIN_NONE);
@@ -2318,6 +2431,8 @@ extern "C" int rsgpu_solver_create(rsgpu_ctx *ctx, const rsgpu_solver_opts *opts
         S->ticket.zero(ctx->stream);
         const int m = std::max(1, opts->restart);
         S->gmres_small.alloc((size_t)2 * ((m + 1) * m + 4 * (m + 1)));
+        if (opts->kind == RSGPU_GMRES && !opts->mixed)
+            S->dcgs.alloc((size_t)(m + 1) * m + m + 2);
         size_t hcap = std::min<size_t>((size_t)opts->max_iter * 2 + 8, 1u << 22);
         S->hist.alloc(hcap);
         S->hist_in.alloc(std::min<size_t>((size_t)opts->inner_max_iter * 2 + 8,
@@ -2553,7 +2668,7 @@ extern "C" int rsgpu_solver_solve(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M,
         if (S->partials.n < std::max(need, need_cgs) + 4096)
             S->partials.alloc(std::max(need, need_cgs) + 4096);
         state_init(S, S->st, S->hist, S->o.max_iter, S->o.rtol, S->o.atol,
-                   S->o.dtol, m, S->gmres_small.p);
+                   S->o.dtol, m, S->gmres_small.p, S->dcgs.p);
         S->fused_steps = 0;
         if (stats)
             std::memset(stats, 0, sizeof(*stats));

@@ -348,6 +348,13 @@ struct DevState {
     double *h2;         // CGS2: the second pass's projections of column gi
     int m;
     int pad1;
+    // FP64 GMRES with delayed reorthogonalisation (DCGS2): the Hessenberg
+    // before rotations ((m+1) x m, column-major), the reorthogonalisation
+    // projections of the newest basis vector (and its squared norm last),
+    // and the step's scalars: rho (norm of the reorthogonalised vector) and
+    // c (its projection coefficient)
+    double *Hr, *ds;
+    double drho, dc;
     // QMR (QMR.java:136-243); c1, c2: the p / q scale factors, val: the d / s
     // scale factor of the coming update
     double q_rho, q_rho1, q_xi, q_gamma, q_gamma1, q_theta, q_theta1, q_eta, q_delta, q_ep,
@@ -370,6 +377,34 @@ __device__ __forceinline__ double wave_sum(double v)
     for (int o = 32; o > 0; o >>= 1)
         v += __shfl_xor(v, o, 64);
     return v;
+}
+
+// Sums of a[0..31] over the wave's 64 lanes: lane l returns column l >> 1.
+// Each exchange round halves the columns a lane keeps (it sends the half
+// its partner keeps), so the 32 sums take 32 shuffles instead of 32 trees
+// of 6; the summation order is fixed.
+__device__ __forceinline__ double wave_sum_cols32(const double (&a)[32])
+{
+    const int lane = threadIdx.x & 63;
+    double b[16];
+    {
+        const bool hi = lane & 32;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const double keep = hi ? a[q + 16] : a[q], send = hi ? a[q] : a[q + 16];
+            b[q] = keep + __shfl_xor(send, 32, 64);
+        }
+    }
+#pragma unroll
+    for (int w = 8, o = 16; w >= 1; w >>= 1, o >>= 1) {
+        const bool hi = lane & o;
+#pragma unroll
+        for (int q = 0; q < w; ++q) {
+            const double keep = hi ? b[q + w] : b[q], send = hi ? b[q] : b[q + w];
+            b[q] = keep + __shfl_xor(send, o, 64);
+        }
+    }
+    return b[0] + __shfl_xor(b[0], 1, 64);
 }
 
 template <int BS>
@@ -544,6 +579,112 @@ __device__ __forceinline__ bool dev_monitor(DevState *st, double r)
     return false;
 }
 
+// ---------------------------------------------------------------- GMRES columns
+// GMRES.solve's per-column work on the least-squares system (GMRES.java:
+// 155-163, GivensRotation.java:42-93) by one wave of the finishing group
+// (all 64 lanes active): column i's entries 0..i come from src (H's own
+// column when src is null) and its subdiagonal is hn.  The entries, the
+// earlier rotations and s(i), s(i+1) are fetched with one load per lane into
+// LDS (lh: 3 x 64 doubles), lane 0 applies the rotations in order, and the
+// rotated entries are stored back one per lane: the sequential loop's
+// operations and order, bit for bit.  i + 1 < 64.
+__device__ inline void gm_finish_column_wave(DevState *st, int i, double hn, double *lh,
+                                             const double *src = nullptr)
+{
+    const int lane = threadIdx.x & 63;
+    const int ld = st->m + 1;
+    double *Hc = st->H + i * ld;
+    const double *Sc = src ? src : Hc;
+    double *gcp = st->gc, *gsp = st->gs, *sv = st->s;
+    double *lc = lh + 64, *ls = lh + 128;
+    if (lane <= i)
+        lh[lane] = Sc[lane];
+    if (lane < i) {
+        lc[lane] = gcp[lane];
+        ls[lane] = gsp[lane];
+    }
+    const double s0 = sv[i], s1 = sv[i + 1];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) {
+        st->hsub = hn;
+        // previous rotations on column i; cur is the running Hc[q]
+        double cur = lh[0];
+        for (int q = 0; q < i; ++q) {
+            const double c = lc[q], s = ls[q], nx = lh[q + 1];
+            const double temp = c * cur + s * nx;
+            cur = -s * cur + c * nx;
+            lh[q] = temp;
+        }
+        // new rotation (GivensRotation(x, y))
+        const double x = cur, y = hn;
+        const double roe = fabs(x) > fabs(y) ? x : y;
+        const double scale = fabs(x) + fabs(y);
+        double c, s;
+        if (scale != 0) {
+            const double xs = x / scale, ys = y / scale;
+            double rr = scale * sqrt(xs * xs + ys * ys);
+            if (roe < 0)
+                rr *= -1;
+            c = x / rr;
+            s = y / rr;
+        } else {
+            c = 1;
+            s = 0;
+        }
+        gcp[i] = c;
+        gsp[i] = s;
+        Hc[i] = c * x + s * y;
+        Hc[i + 1] = -s * x + c * y;
+        const double t0 = c * s0 + s * s1;
+        const double t1 = -s * s0 + c * s1;
+        sv[i] = t0;
+        sv[i + 1] = t1;
+        st->iter++;
+        st->gi = i + 1;
+        if (!(i + 1 < st->m) || dev_monitor(st, fabs(t1))) {
+            st->cycle_end = 1;
+            if (st->status)
+                st->done = 1;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    if (lane < i)
+        Hc[lane] = lh[lane];
+}
+
+// Delayed reorthogonalisation, the step that finishes column c (one wave of
+// the finishing group).  The basis vector v~ = u / eta that followed column
+// c (eta = st->hsub, its first projection's norm) has projections
+// ds[0..c] = V' v~ and squared norm ds[c+1] (ds in LDS or global memory
+// written before this wave's group started); its reorthogonalised form is
+// v = (v~ - V ds) / rho with rho = sqrt(ds[c+1] - sum ds^2), so the Arnoldi
+// relation of column c becomes Hr(0..c, c) += eta ds, Hr(c+1, c) = eta rho.
+// Column c is then rotated and monitored (gm_finish_column_wave).  Returns
+// rho in every lane.
+__device__ inline double dcgs_finish_column_wave(DevState *st, int c, double *lh,
+                                                 const double *ds)
+{
+    const int lane = threadIdx.x & 63;
+    const int ld = st->m + 1;
+    double *Hr = st->Hr;
+    const double sk = lane <= c ? ds[lane] : 0.0;
+    const double alpha = ds[c + 1];
+    const double ss = wave_sum(sk * sk);
+    const double r2 = alpha - ss;
+    const double rho = r2 > 0 ? sqrt(r2) : 0.0;
+    const double eta = st->hsub;
+    if (lane <= c)
+        Hr[lane + c * ld] += eta * sk;
+    if (lane == 0)
+        Hr[c + 1 + c * ld] = eta * rho;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    gm_finish_column_wave(st, c, eta * rho, lh, Hr + c * ld);
+    return rho;
+}
+
 // ---------------------------------------------------------------- host API
 // spmv.hip
 void spmv_build_plan(const std::vector<int> &rowptr, int n, SpmvPlan &plan,
@@ -602,7 +743,8 @@ bool gmres_fused_ok(const rsgpu_csr *A, const rsgpu_prec *M, int elem_bytes, int
 template <typename T>
 void launch_gmres_fused(hipStream_t s, const rsgpu_csr *A, const T *slv, rsgpu_prec *M,
                         const T *win, T *vout, bool scale, T *wout, const T *V, size_t ldv,
-                        int i, DevState *st, RedSlot slot, double *hout = nullptr);
+                        int i, DevState *st, RedSlot slot, double *hout = nullptr,
+                        bool dcgs = false);
 
 // amg.hip
 int amg_setup(rsgpu_prec *M, rsgpu_csr *A, int *bad_row);
