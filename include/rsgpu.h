@@ -203,7 +203,8 @@ int rsgpu_prec_info(const rsgpu_prec *M, int *nnz, int *nlevels_lower,
 /* Which sweep kernel ILU.apply runs (no MTJ counterpart; all three are
  * bit-identical to the sequential solves): 0 level schedules, 1 line
  * wavefront (every block Ly <= 64 lines of Lx rows coupled among 9-point
- * neighbours; RSGPU_ILU_BAND=0 in the environment turns it off), 2 3x3 cell
+ * neighbours, at most 4 blocks per CU; RSGPU_ILU_BAND=0 in the environment
+ * turns it off, RSGPU_ILU_BAND_APPLY=0/1 forces the choice), 2 3x3 cell
  * schedules. */
 int rsgpu_prec_sweep_form(const rsgpu_prec *M, int *form);
 

@@ -2518,7 +2518,7 @@ __global__ void __launch_bounds__(64)
                      const T *__restrict__ recU,
                      const T *__restrict__ dgl, const double *__restrict__ rdl,
                      const T *__restrict__ b, T *__restrict__ x, const int *skip1,
-                     const int *skip2, BlkSkip bskip)
+                     const int *skip2, BlkSkip bskip, int slots)
 {
     if ((skip1 && *skip1) || (skip2 && *skip2))
         return;
@@ -2528,7 +2528,7 @@ __global__ void __launch_bounds__(64)
     const int b0 = bstart[blockIdx.x], nb = bstart[blockIdx.x + 1] - b0;
     const int2 d = dims[blockIdx.x];   // Lx, Ly; Ly < 0: the block's pattern is not full
     const int Lx = d.x, Ly = d.y < 0 ? -d.y : d.y;
-    if (d.y > 0 && sizeof(T) == 4) {
+    if (slots && d.y > 0 && sizeof(T) == 4) {
         // full FP32 block: slot layout (band_sweep_full; in FP64 its larger
         // register set made it slower than the per-step form below)
         T *xs = (T *)rs_smem;
@@ -2599,12 +2599,52 @@ __global__ void __launch_bounds__(64)
         x[b0 + r] = xv[r];
 }
 
+// The line wavefront gives each block one wave with a short dependency
+// chain: the faster apply while the blocks are about one per CU (the
+// 128x128x64 system's 256).  With many blocks per CU the level-scheduled
+// kernel's throughput wins (the 64-system ensemble's 7,060 blocks: 352
+// against 508 us per FP32 apply, 592 against 1,193 us per FP64 apply), so
+// above RS_BAND_MAX_PER_CU blocks per CU the apply takes the level
+// schedules.  RSGPU_ILU_BAND_APPLY=1 forces the wavefront, =0 the levels.
+#define RS_BAND_MAX_PER_CU 4
+static bool band_apply_preferred(const rsgpu_prec *M)
+{
+    const char *e = getenv("RSGPU_ILU_BAND_APPLY");
+    if (e && (e[0] == '0' || e[0] == '1'))
+        return e[0] == '1';
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                hipSuccess || cus <= 0)
+            cus = 256;
+    }
+    return M->nblocks <= RS_BAND_MAX_PER_CU * cus;
+}
+
+bool prec_band_apply(const rsgpu_prec *M) { return band_apply_preferred(M); }
+
+// Layout of full FP32 blocks: the per-line slot layout; RSGPU_BAND_SLOTS=0
+// selects the row layout of the per-step sweep (bit-identical), whose
+// smaller LDS vector (Lx Ly + 2 RS_BAND_PAD elements against
+// Ly round_up(Lx + 2 Ly - 2, 16)) lets more blocks share a CU but was
+// slower on the ensemble's 7,060 blocks too (1.20 against 1.11 s per mixed
+// ensemble: three blocks per CU already keep its SIMDs busy).
+static bool band_use_slots(const rsgpu_prec *)
+{
+    const char *e = getenv("RSGPU_BAND_SLOTS");
+    return !(e && e[0] == '0');
+}
+
 template <typename T>
 static void launch_band(rsgpu_prec *M, const T *rec, const T *b, T *x, const int *sk1,
                         const int *sk2)
 {
+    const bool slots = sizeof(T) == 4 && band_use_slots(M);
     const size_t shm =
-        std::max((size_t)(M->max_block_rows + 2 * RS_BAND_PAD), (size_t)M->band_slots) *
+        std::max((size_t)(M->max_block_rows + 2 * RS_BAND_PAD),
+                 slots ? (size_t)M->band_slots : (size_t)0) *
         sizeof(T);
     static bool attr = false;
     if (!attr && shm > 65536) {
@@ -2621,7 +2661,7 @@ static void launch_band(rsgpu_prec *M, const T *rec, const T *b, T *x, const int
                        M->bstart.p, M->band_dims.p, M->band_mask.p + RS_BAND_PAD,
                        rec + (size_t)8 * RS_BAND_PAD, rec + (size_t)8 * RS_BAND_PAD + 4,
                        rec + 8 * np + RS_BAND_PAD, rdl, b, x,
-                       sk1, sk2, M->skip);
+                       sk1, sk2, M->skip, slots ? 1 : 0);
     RS_CHECK_LAUNCH();
 }
 
@@ -3082,7 +3122,7 @@ bool gmres_fused_ok(const rsgpu_csr *A, const rsgpu_prec *M, int elem_bytes, int
         m + 1 > RS_FUSE_COLS || M->kind == RSGPU_PREC_ILUT || M->kind == RSGPU_PREC_ICC ||
         M->kind == RSGPU_PREC_AMG || M->kind == RSGPU_PREC_SSOR ||
         M->kind == RSGPU_PREC_DIAGONAL || M->kind == RSGPU_PREC_IDENTITY || M->skip.done ||
-        (int)M->h_band_dims.size() != M->nblocks)
+        (int)M->h_band_dims.size() != M->nblocks || !band_apply_preferred(M))
         return false;
     if ((elem_bytes == 4) != (M->precision == RSGPU_FP32))
         return false;
@@ -3150,7 +3190,7 @@ static void launch_apply(rsgpu_prec *M, const SweepData &A_, const T *va, const 
     hipStream_t s = M->ctx->stream;
     if (M->n == 0)
         return;
-    if (TRANS == RS_MODE_ILU && M->band && &A_ == &M->L &&
+    if (TRANS == RS_MODE_ILU && M->band && &A_ == &M->L && band_apply_preferred(M) &&
         std::max((size_t)(M->max_block_rows + 2 * RS_BAND_PAD), (size_t)M->band_slots) *
                 sizeof(T) <=
             (size_t)lds_limit(M->ctx->device)) {

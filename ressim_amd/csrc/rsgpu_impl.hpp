@@ -740,6 +740,9 @@ void prec_trans_apply(rsgpu_prec *M, const double *b, double *x,
 // launch (wout must not alias win; the projections go to column i of the
 // state's H, or to hout when that is given)
 bool gmres_fused_ok(const rsgpu_csr *A, const rsgpu_prec *M, int elem_bytes, int m);
+// whether ILU.apply takes the line wavefront when M has its records (few
+// blocks per CU; many take the level schedules)
+bool prec_band_apply(const rsgpu_prec *M);
 template <typename T>
 void launch_gmres_fused(hipStream_t s, const rsgpu_csr *A, const T *slv, rsgpu_prec *M,
                         const T *win, T *vout, bool scale, T *wout, const T *V, size_t ldv,

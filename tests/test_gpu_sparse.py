@@ -323,7 +323,8 @@ def test_block_ilu_line_wavefront(gpu, oracle, monkeypatch, precision, shape):
     """Block ILU(0) apply on 2-D 9-point MPFA-O matrices: blocks of whole
     lines (at most 64 of them) take the line-wavefront kernel, others the
     level schedules; either way the result is the oracle's, bit for bit,
-    FP64 and FP32, and the same bits as with the wavefront switched off."""
+    FP64 and FP32 (FP32 full blocks in both LDS layouts), and the same bits
+    as with the wavefront switched off."""
     import torch
     from ressim_amd import _lib
     nx, ny, nb, form = shape
@@ -339,8 +340,13 @@ def test_block_ilu_line_wavefront(gpu, oracle, monkeypatch, precision, shape):
     b = np.random.default_rng(nx * ny + nb).standard_normal(n)
     b[n // 2] = np.inf          # propagates as in the sequential sweeps
     out = []
-    for band in ("1", "0"):
+    # FP32 full blocks: both LDS layouts of the wavefront (per-line slots,
+    # rows), then the level schedules
+    runs = (("1", "1"), ("1", "0"), ("0", "1")) if precision == "fp32" else (("1", "1"),
+                                                                          ("0", "1"))
+    for band, slots in runs:
         monkeypatch.setenv("RSGPU_ILU_BAND", band)
+        monkeypatch.setenv("RSGPU_BAND_SLOTS", slots)
         A = _mat(gpu, rp, ci, d, n, n)
         M = gpu.BlockILU(nb, precision=_lib.FP64 if precision == "fp64" else _lib.FP32)
         M.setMatrix(A)
@@ -362,8 +368,8 @@ def test_block_ilu_line_wavefront(gpu, oracle, monkeypatch, precision, shape):
     else:
         _, lu, dg, _ = oracle.ilu_factor_f32(brp, bci, bd.astype(np.float32))
         want = oracle.ilu_apply_f32(brp, bci, lu, dg, b.astype(np.float32))
-    np.testing.assert_array_equal(out[0], want)
-    np.testing.assert_array_equal(out[1], want)
+    for x in out:
+        np.testing.assert_array_equal(x, want)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
