@@ -782,10 +782,12 @@ def bench_ensemble(a):
     M32, Smx = make(_lib.FP32, bs, True)
     dtmx, stmx = timed(M32, Smx, True, a.steps, a.warmup, "mixed FP32-BiCGstab/FP64-IR")
     xmx = x.clone()
+    sweep32 = M32.info()["sweep_form"]        # the autotuner's ILU apply kernel
     del M32
     M64, S64 = make(_lib.FP64, bs, False)
     dt64, st64 = timed(M64, S64, False, max(1, a.steps // 2), a.warmup, "fp64 BiCGstab")
     x64 = x.clone()
+    sweep64 = M64.info()["sweep_form"]
     del M64
     ss = sys_start.astype(np.int64)
     errs = [float(torch.linalg.norm(xmx[ss[k]:ss[k + 1]] - x64[ss[k]:ss[k + 1]]) /
@@ -817,8 +819,10 @@ def bench_ensemble(a):
     b64 = spmv_bytes(n, nnz, 8, 8)
     gbs = b64 / t_spmv / 1e9
     info = Smx.info()
+    info["ilu_sweep_fp32"], info["ilu_sweep_fp64"] = sweep32, sweep64
     log(f"batched SpMV fp64 {t_spmv * 1e6:.0f} us {gbs:.0f} GB/s ({gbs / PEAK_HBM_GBS:.1%}); "
-        f"autotuned stages fp64 {info['spmv_stage_fp64']} fp32 {info['spmv_stage_fp32']}")
+        f"autotuned stages fp64 {info['spmv_stage_fp64']} fp32 {info['spmv_stage_fp32']}, "
+        f"ILU apply kernels fp32 {sweep32} fp64 {sweep64}")
     ctx.set_stream(None)
 
     cpu = None

@@ -574,6 +574,7 @@ struct rsgpu_batch {
     uint64_t plan_pattern = 0;
     BPlan p64, p32;
     float tune_us64 = 0, tune_us32 = 0;
+    float sweep_us[2] = {0, 0};      // ILU apply: level schedules, line wavefront
     // state
     DevBuf<BatchSys> st;
     std::vector<BatchSys> h_st;
@@ -738,6 +739,47 @@ void batch_papply(rsgpu_prec *M, const double *b, double *x)
     prec_apply(M, b, x, nullptr);
 }
 void batch_papply(rsgpu_prec *M, const float *b, float *x) { prec_apply(M, b, x, nullptr); }
+
+// ILU sweep kernel: when the preconditioner has line-wavefront records, its
+// apply is timed with the wavefront and with the level schedules (median of
+// 5 after a warm-up, on the batch's own blocks) and the faster is kept for
+// this pattern (rsgpu_prec_sweep_form then reports it).  The two are
+// bit-identical, so the choice changes time only.
+template <typename T>
+void autotune_sweeps(rsgpu_batch *B, rsgpu_prec *M)
+{
+    if (!M->band || M->band_pick >= 0 || (B->flags & RSGPU_BATCH_NO_AUTOTUNE) ||
+        getenv("RSGPU_ILU_BAND_APPLY"))
+        return;
+    hipStream_t s = B->ctx->stream;
+    T *x = (T *)(sizeof(T) == 8 ? (void *)B->w64.p : (void *)B->w32.p);
+    T *y = x + B->ldv;
+    hipEvent_t e0, e1;
+    RS_HIP(hipEventCreate(&e0));
+    RS_HIP(hipEventCreate(&e1));
+    float us[2];
+    for (int pick = 0; pick < 2; ++pick) {
+        M->band_pick = pick;
+        std::vector<float> t;
+        for (int rep = 0; rep < 6; ++rep) {
+            RS_HIP(hipEventRecord(e0, s));
+            batch_papply(M, x, y);
+            RS_HIP(hipEventRecord(e1, s));
+            RS_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            RS_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (rep)
+                t.push_back(ms * 1000.f);
+        }
+        std::sort(t.begin(), t.end());
+        us[pick] = t[t.size() / 2];
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    M->band_pick = us[1] < us[0] ? 1 : 0;
+    B->sweep_us[0] = us[0];
+    B->sweep_us[1] = us[1];
+}
 
 // One BiCGstab iteration of every system that is still running (T = double:
 // the plain FP64 solve on x; T = float: the inner solve on z), preceded in
@@ -1015,6 +1057,12 @@ extern "C" int rsgpu_batch_solve(rsgpu_batch *B, rsgpu_csr *A, rsgpu_prec *M,
         RS_HIP(hipSetDevice(ctx->device));
         hipStream_t s = ctx->stream;
         prepare_plans(B, A);
+        if (M->kind == RSGPU_PREC_ILU0 || M->kind == RSGPU_PREC_BLOCK_ILU0) {
+            if (M->precision == RSGPU_FP32)
+                autotune_sweeps<float>(B, M);
+            else
+                autotune_sweeps<double>(B, M);
+        }
         prepare_skip(B, M);
         // per-system states (DefaultIterationMonitor of each solve)
         for (int k = 0; k < B->nsys; ++k) {

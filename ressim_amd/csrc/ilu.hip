@@ -727,6 +727,7 @@ static void band_setup(rsgpu_prec *M, const std::vector<int> &rp, const std::vec
 {
     M->band = false;
     M->band_factor = false;
+    M->band_pick = -1;
     M->band_slots = 0;
     M->h_band_dims.clear();
     M->band_dims.release();
@@ -2605,13 +2606,16 @@ __global__ void __launch_bounds__(64)
 // kernel's throughput wins (the 64-system ensemble's 7,060 blocks: 352
 // against 508 us per FP32 apply, 592 against 1,193 us per FP64 apply), so
 // above RS_BAND_MAX_PER_CU blocks per CU the apply takes the level
-// schedules.  RSGPU_ILU_BAND_APPLY=1 forces the wavefront, =0 the levels.
+// schedules, unless a batched solve timed both on the preconditioner
+// (band_pick).  RSGPU_ILU_BAND_APPLY=1 forces the wavefront, =0 the levels.
 #define RS_BAND_MAX_PER_CU 4
 static bool band_apply_preferred(const rsgpu_prec *M)
 {
     const char *e = getenv("RSGPU_ILU_BAND_APPLY");
     if (e && (e[0] == '0' || e[0] == '1'))
         return e[0] == '1';
+    if (M->band_pick >= 0)
+        return M->band_pick == 1;
     static int cus = 0;
     if (!cus) {
         int dev = 0;

@@ -290,6 +290,9 @@ struct rsgpu_prec {
     // absent), the slot masks, and the per-row records gathered after each
     // factorisation
     bool band = false;
+    // which apply kernel a batched solve's timing picked for this pattern
+    // (-1: none, the block-count rule; 0 level schedules; 1 line wavefront)
+    int band_pick = -1;
     std::vector<int2> h_band_dims;   // host copy of band_dims
     DevBuf<int2> band_dims;
     DevBuf<int> band_pos;
