@@ -2714,6 +2714,9 @@ constexpr int fuse_threads()
 }
 #define RS_FUSE_COLS 32
 #define RS_FUSE_SPIN (1 << 24)
+// bytes of the fused kernel's static LDS arrays (projection sums of 11
+// producers, flags), kept free beside its dynamic LDS
+#define RS_FUSE_STATIC 6144
 
 // LDS progress between the waves of a fused step.  Flags are written with
 // workgroup-scope release and read with acquire; the U sweep's step count is
@@ -2840,7 +2843,7 @@ __global__ void __launch_bounds__(fuse_threads<T>())
                const T *__restrict__ recU,
                const T *__restrict__ dgl, const double *__restrict__ rdl, T *__restrict__ wout,
                const T *__restrict__ V, size_t ldv, int i, DevState *st, RedSlot slot,
-               int vec_elems, double *hout)
+               int vec_elems, double *hout, int npre, int pre_off)
 {
     using VT = typename BandVec<T>::type;
     constexpr int N = BandVec<T>::N;
@@ -2851,7 +2854,7 @@ __global__ void __launch_bounds__(fuse_threads<T>())
     __shared__ double sh2[DCGS ? P : 1][RS_FUSE_COLS];
     __shared__ double colsum[DCGS ? 2 * RS_FUSE_COLS : 1];
     __shared__ double lh[DCGS ? 3 * 64 : 1];
-    __shared__ int uprog, err, last;
+    __shared__ int uprog, err, last, precnt;
     double *Hb = st->H;
     const int ld = st->m + 1;
     const double hsub = st->hsub;
@@ -2886,9 +2889,13 @@ __global__ void __launch_bounds__(fuse_threads<T>())
     if (threadIdx.x == 0) {
         uprog = 0;
         err = 0;
+        precnt = 0;
     }
     __syncthreads();
     FuseHook hk{flags, &uprog, &err, ntA, Ly};
+    // basis columns 0..npre-1 of the block's rows, copied to LDS by the
+    // producers between their SpMV rows and the U sweep's first rows
+    T *vpre = (T *)(rs_smem + pre_off);
     const int cnt = i + 1;
     // projection columns: V' w, then (DCGS, i > 0) V' v~_i
     const bool reo = DCGS && scale;
@@ -2942,6 +2949,34 @@ __global__ void __launch_bounds__(fuse_threads<T>())
         const int rowsB = 64 * N;
         const int ntB = (nb + rowsB - 1) / rowsB;
         const int kmax = scale ? i - 1 : i;            // columns read from V
+        // While the L sweep finishes its last rows the producers would
+        // wait: they copy the first npre basis columns of the block to LDS
+        // (every load of a lane's share in flight), so the projections
+        // read only the rest from memory while the U sweep streams its
+        // records.  The readers wait for all producers' copies.
+        if (npre > 0) {
+            constexpr int NL = 8;
+            const int nvec = nb / N;
+            for (int v0 = p * 64 + lane; v0 < npre * nvec; v0 += P * 64 * NL) {
+                VT t[NL];
+#pragma unroll
+                for (int u = 0; u < NL; ++u) {
+                    const int id = min(v0 + u * P * 64, npre * nvec - 1);
+                    const int c = id / nvec, v = id - c * nvec;
+                    t[u] = *reinterpret_cast<const VT *>(V + (size_t)min(c, kmax) * ldv + b0 +
+                                                         (size_t)v * N);
+                }
+#pragma unroll
+                for (int u = 0; u < NL; ++u) {
+                    const int id = v0 + u * P * 64;
+                    if (id < npre * nvec)
+                        reinterpret_cast<VT *>(vpre)[id] = t[u];
+                }
+            }
+            if (lane == 0)
+                __hip_atomic_fetch_add(&precnt, 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
+        bool pre_ready = npre == 0;
         for (int kk = p; kk < ntB; kk += P) {
             const int q = (ntB - 1 - kk) * rowsB + N * lane;
             const bool live = q < nb;
@@ -2980,18 +3015,41 @@ __global__ void __launch_bounds__(fuse_threads<T>())
                 for (int e = 0; e < N; ++e)
                     ve[e] = (sc == T(0)) ? T(0) : sc * ve[e];
             }
+            if (!pre_ready) {
+                for (int spin = 0;
+                     !__hip_atomic_load(&err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                     ++spin) {
+                    if (__hip_atomic_load(&precnt, __ATOMIC_ACQUIRE,
+                                          __HIP_MEMORY_SCOPE_WORKGROUP) >= P)
+                        break;
+                    if (spin >= RS_FUSE_SPIN) {
+                        __hip_atomic_store(&err, 1, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                }
+                pre_ready = true;
+            }
             // basis columns in chunks of FC: every load of a chunk is issued
             // (and pinned) before the first product, and every product is
-            // formed and then selected, so no load sinks under a condition
+            // formed and then selected, so no load sinks under a condition;
+            // the first chunk comes from LDS when it was copied there
             constexpr int FC = 8;
 #pragma unroll
             for (int c0 = 0; c0 < RS_FUSE_COLS; c0 += FC) {
                 if (c0 < cnt) {
                     VT kv[FC];
+                    if (c0 == 0 && npre > 0) {
 #pragma unroll
-                    for (int u = 0; u < FC; ++u)
-                        kv[u] = *reinterpret_cast<const VT *>(
-                            V + (size_t)max(min(c0 + u, kmax), 0) * ldv + gr);
+                        for (int u = 0; u < FC; ++u)
+                            kv[u] = reinterpret_cast<const VT *>(vpre + (size_t)u * nb)[qc / N];
+                    } else {
+#pragma unroll
+                        for (int u = 0; u < FC; ++u)
+                            kv[u] = *reinterpret_cast<const VT *>(
+                                V + (size_t)max(min(c0 + u, kmax), 0) * ldv + gr);
+                    }
 #pragma unroll
                     for (int u = 0; u < FC; ++u)
                         fuse_pin(kv[u]);
@@ -3156,15 +3214,29 @@ void launch_gmres_fused(hipStream_t s, const rsgpu_csr *A, const T *slv, rsgpu_p
                                 : (size_t)(M->h_bstart[b + 1] - M->h_bstart[b]) +
                                       2 * RS_BAND_PAD);
     }
-    const size_t shm = vec * sizeof(T) + (size_t)(M->max_block_rows / 64 + 1) * 4;
+    size_t shm = vec * sizeof(T) + (size_t)(M->max_block_rows / 64 + 1) * 4;
+    // the first 8 basis columns in LDS when every one is a column of V
+    // (FP32; FP64 blocks do not fit) and the device's LDS holds them beside
+    // the sweep vector and the kernel's own arrays
+    const size_t pre_off = (shm + 15) & ~(size_t)15;
+    const size_t pre_bytes = (size_t)8 * M->max_block_rows * sizeof(T);
+    const int kmax = scale ? i - 1 : i;
+    const int npre = (f32 && kmax >= 7 &&
+                      pre_off + pre_bytes + RS_FUSE_STATIC <= (size_t)lds_limit(M->ctx->device))
+                         ? 8
+                         : 0;
+    if (npre)
+        shm = pre_off + pre_bytes;
     // the delayed reorthogonalisation form exists in FP64 only
     const bool dc = dcgs && !f32;
     auto kern = dc ? k_gm_fused<T, !f32> : k_gm_fused<T, false>;
-    static bool attr[2] = {false, false};
-    if (!attr[dc] && shm > 65536) {
+    // the dynamic LDS limit is raised to what the launch needs (the
+    // kernel's static arrays take the rest of the device's opt-in maximum)
+    static size_t attr[2] = {65536, 65536};
+    if (shm > attr[dc]) {
         RS_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                   lds_limit(M->ctx->device)));
-        attr[dc] = true;
+                                   (int)shm));
+        attr[dc] = shm;
     }
     const size_t np = (size_t)M->n + 2 * RS_BAND_PAD;
     const T *rec = f32 ? (const T *)M->band32.p : (const T *)M->band64.p;
@@ -3173,7 +3245,7 @@ void launch_gmres_fused(hipStream_t s, const rsgpu_csr *A, const T *slv, rsgpu_p
                        slv, win, vout, scale ? 1 : 0, M->bstart.p, M->band_dims.p,
                        rec + (size_t)8 * RS_BAND_PAD, rec + (size_t)8 * RS_BAND_PAD + 4,
                        rec + 8 * np + RS_BAND_PAD, rdl, wout, V, ldv, i, st, slot, (int)vec,
-                       hout);
+                       hout, npre, (int)pre_off);
     RS_CHECK_LAUNCH();
 }
 
