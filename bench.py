@@ -912,6 +912,11 @@ def bench_blackoil(a):
             lib.rsgpu_csr_mult(J.h, xin.data_ptr(), y.data_ptr())
         for _ in range(a.spmv_reps):
             lib.rsgpu_csr_mult_f32(J.h, x32.data_ptr(), y32.data_ptr())
+        J.sync_sliced()
+        for _ in range(a.spmv_reps):
+            lib.rsgpu_csr_mult_sliced(J.h, xin.data_ptr(), y.data_ptr())
+        for _ in range(a.spmv_reps):
+            lib.rsgpu_csr_mult_sliced_f32(J.h, x32.data_ptr(), y32.data_ptr())
         torch.cuda.synchronize()
         src = torch.ones(64 << 20, dtype=torch.float64, device=dev)     # 512 MiB
         dst = torch.empty_like(src)
@@ -919,7 +924,8 @@ def bench_blackoil(a):
             dst.copy_(src)
         torch.cuda.synchronize()
         ctx.set_stream(None)
-        log(f"probe spmv: block rows {pr.n}, block nnz {nnzb}, reps {a.spmv_reps}; copy 512 MiB x3")
+        log(f"probe spmv: block rows {pr.n}, block nnz {nnzb}, reps {a.spmv_reps} each of the "
+            f"block-row core and the sliced block form, FP64 then FP32; copy 512 MiB x3")
         return
     # assembly kernel time
     e0 = torch.cuda.Event(enable_timing=True)
@@ -978,52 +984,79 @@ def bench_blackoil(a):
     err = float(torch.linalg.norm(x - x64) / torch.linalg.norm(x64))
     log(f"mixed vs fp64 relative solution difference {err:.3e}")
 
-    # block SpMV against the CSR kernel on the same expanded matrix
-    xin = torch.randn(n, dtype=torch.float64, device=dev)
-    yb = torch.empty(n, dtype=torch.float64, device=dev)
-    yc = torch.empty(n, dtype=torch.float64, device=dev)
-    Jc = mtj.CompRowMatrix.from_csr(n, n, J.getRowPointers(), J.getColumnIndices(),
-                                    J.getData(), ctx=ctx)
-
-    def time_mult(Am, y, reps=50):
-        for _ in range(3):
-            lib.rsgpu_csr_mult(Am.h, xin.data_ptr(), y.data_ptr())
-        e0.record(stream)
-        for _ in range(reps):
-            lib.rsgpu_csr_mult(Am.h, xin.data_ptr(), y.data_ptr())
-        e1.record(stream)
-        e1.synchronize()
-        return e0.elapsed_time(e1) / reps * 1e-3
-
-    t_b = time_mult(J, yb)
-    t_c = time_mult(Jc, yc)
-    same = bool(torch.equal(yb, yc))
-    # FP32 values and vectors (the mixed form's inner SpMV)
-    J.sync_f32()
-    Jc.sync_f32()
-    x32 = xin.float()
-    y32 = torch.empty(n, dtype=torch.float32, device=dev)
+    # Block SpMV kernels on the same matrix, each timed warm (the same
+    # operands every launch) and cold (launches cycle over J and two more
+    # copies with their own vectors: 3 x 152 MB in FP32 and 3 x 289 MB in
+    # FP64 exceed the 256 MB last-level cache): the block-row core
+    # (rsgpu_csr_mult), the block form of the sliced copy
+    # (rsgpu_csr_mult_sliced, what the solves stream) and the CSR kernel on
+    # the expanded matrix.
+    NC = 3
+    Js = [J]
+    for _ in range(NC - 1):
+        boc = BlackOilJacobian(pr, ctx)
+        Jn = boc.matrix()
+        boc.assemble(u, u_old, Jn, torch.empty_like(R))
+        Js.append(Jn)
+    Jcs = [mtj.CompRowMatrix.from_csr(n, n, J.getRowPointers(), J.getColumnIndices(),
+                                      J.getData(), ctx=ctx) for _ in range(NC)]
+    for Am in Js + Jcs:
+        Am.sync_f32()
+    for Am in Js:
+        Am.sync_sliced()
+    xs = [torch.randn(n, dtype=torch.float64, device=dev) for _ in range(NC)]
+    ys = [torch.empty(n, dtype=torch.float64, device=dev) for _ in range(NC)]
+    xs32 = [v.float() for v in xs]
+    ys32 = [torch.empty(n, dtype=torch.float32, device=dev) for _ in range(NC)]
     torch.cuda.synchronize()
 
-    def time_mult32(Am, reps=50):
-        for _ in range(3):
-            lib.rsgpu_csr_mult_f32(Am.h, x32.data_ptr(), y32.data_ptr())
+    def launches(fs, reps=60):
+        for f in fs:
+            f()
         e0.record(stream)
-        for _ in range(reps):
-            lib.rsgpu_csr_mult_f32(Am.h, x32.data_ptr(), y32.data_ptr())
+        for r in range(reps):
+            fs[r % len(fs)]()
         e1.record(stream)
         e1.synchronize()
         return e0.elapsed_time(e1) / reps * 1e-3
 
-    t_b32 = time_mult32(J)
-    t_c32 = time_mult32(Jc)
-    log(f"fp32: block SpMV {t_b32 * 1e6:.1f} us, CSR kernel {t_c32 * 1e6:.1f} us")
+    kern = {"block": (lib.rsgpu_csr_mult, lib.rsgpu_csr_mult_f32, Js),
+            "sliced": (lib.rsgpu_csr_mult_sliced, lib.rsgpu_csr_mult_sliced_f32, Js),
+            "csr": (lib.rsgpu_csr_mult, lib.rsgpu_csr_mult_f32, Jcs)}
+    tk = {}
+    for k, (f64, f32, mats) in kern.items():
+        for prec, fn, xv, yv in (("fp64", f64, xs, ys), ("fp32", f32, xs32, ys32)):
+            fs = [(lambda Am, xx, yy: lambda: fn(Am.h, xx.data_ptr(), yy.data_ptr()))(
+                mats[c], xv[c], yv[c]) for c in range(NC)]
+            tk[f"{k}_{prec}"] = {"warm_us": launches(fs[:1]) * 1e6,
+                                 "cold_us": launches(fs) * 1e6}
+    # results of the three kernels on the same input
+    outs = []
+    for fn, Am in ((lib.rsgpu_csr_mult, J), (lib.rsgpu_csr_mult_sliced, J),
+                   (lib.rsgpu_csr_mult, Jcs[0])):
+        fn(Am.h, xs[0].data_ptr(), ys[0].data_ptr())
+        torch.cuda.synchronize()
+        outs.append(ys[0].clone())
+    same = bool(torch.equal(outs[0], outs[1]) and torch.equal(outs[0], outs[2]))
     bsr_bytes = nnzb * (72 + 4) + (pr.n + 1) * 4 + 2 * n * 8
+    bsr32_bytes = nnzb * (36 + 4) + (pr.n + 1) * 4 + 2 * n * 4
+    nch, nslots, _ = J.sliced_info()
+    sl_bytes = nslots * 8 + 2 * n + 2 * n * 8
     csr_bytes = spmv_bytes(n, 9 * nnzb, 8, 8)
-    gbs = bsr_bytes / t_b / 1e9
-    log(f"block SpMV {t_b * 1e6:.1f} us {gbs:.0f} GB/s ({gbs / PEAK_HBM_GBS:.1%}); CSR kernel on "
-        f"the expanded matrix {t_c * 1e6:.1f} us ({csr_bytes / t_c / 1e9:.0f} GB/s of its "
-        f"bytes); results identical: {same}; assembly {t_asm * 1e3:.2f} ms")
+    for k, v in tk.items():
+        byt = bsr_bytes if k.endswith("fp64") else bsr32_bytes
+        v["cold_GBps"] = byt / v["cold_us"] / 1e3
+        v["cold_frac"] = v["cold_GBps"] / PEAK_HBM_GBS
+        log(f"{k}: warm {v['warm_us']:.1f} us, cold {v['cold_us']:.1f} us = "
+            f"{v['cold_GBps']:.0f} GB/s ({v['cold_frac']:.1%}) of the BSR bytes")
+    best = min(("block_fp64", "sliced_fp64"), key=lambda k: tk[k]["cold_us"])
+    t_b = tk[best]["cold_us"] * 1e-6
+    gbs = tk[best]["cold_GBps"]
+    log(f"block SpMV ({best}) cold {t_b * 1e6:.1f} us {gbs:.0f} GB/s "
+        f"({gbs / PEAK_HBM_GBS:.1%}) of {bsr_bytes} BSR bytes (sliced copy's own bytes "
+        f"{sl_bytes}); CSR kernel cold {tk['csr_fp64']['cold_us']:.1f} us "
+        f"({csr_bytes / tk['csr_fp64']['cold_us'] / 1e3:.0f} GB/s of its bytes); results "
+        f"identical: {same}; assembly {t_asm * 1e3:.2f} ms")
     ctx.set_stream(None)
     cpu = None
     if not a.no_cpu:
@@ -1052,13 +1085,14 @@ def bench_blackoil(a):
                   "true_rel_residual": stmx.true_residual / stmx.b_norm,
                   "rel_err_vs_fp64": err},
         "assembly_ms": t_asm * 1e3,
-        "spmv": {"block_us": t_b * 1e6, "block_GBps": gbs, "csr_us": t_c * 1e6,
-                 "bitwise_equal": same, "block_bytes": bsr_bytes, "block_fp32_us": t_b32 * 1e6,
-                 "csr_fp32_us": t_c32 * 1e6},
+        "spmv": {"kernels": tk, "bitwise_equal": same, "bsr_bytes": bsr_bytes,
+                 "bsr32_bytes": bsr32_bytes, "sliced_bytes": sl_bytes, "csr_bytes": csr_bytes},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": gbs / PEAK_HBM_GBS, "traffic": None,
-                     "kernel": "k_spmv<double,EpiMult,B3> (3x3 block rows)",
-                     "algorithmic_bytes": bsr_bytes},
+                     "kernel": ("k_spmv_sliced<double,true> (block form of the sliced copy)"
+                                if best.startswith("sliced")
+                                else "k_spmv<double,EpiMult,B3> (3x3 block rows)"),
+                     "algorithmic_bytes": bsr_bytes, "cold_us": t_b * 1e6},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -119,7 +119,11 @@ int rsgpu_csr_mult_f32(rsgpu_csr *A, const float *x_dev, float *y_dev);
  * union of its rows' column offsets (column - row) as a table and per row a
  * mask of the slots it holds; values slot-major per chunk.  Built with the
  * pattern when every chunk's union has at most 32 offsets (nchunks = 0
- * otherwise; RSGPU_SLICED=0 in the environment leaves it out).  nslots =
+ * otherwise; RSGPU_SLICED=0 in the environment leaves it out).  A 3x3 block
+ * matrix (rsgpu_csr_create_bsr3) gets the block form instead: a chunk's
+ * table lists block offsets (block column - block row, at most 10), row
+ * 3I+a holds slots 3k+e for column 3(I+d_k)+e, masks per row over the
+ * table's blocks.  nslots =
  * values per copy.  y = A x over it is bit-identical to rsgpu_csr_mult
  * (mult_sliced_f32: to rsgpu_csr_mult_f32).  The FP64 copy is refreshed by
  * rsgpu_csr_sync_sliced (and by every FP64 solve at its start), the FP32

@@ -209,6 +209,7 @@ extern "C" int rsgpu_csr_create_bsr3(rsgpu_ctx *ctx, int nb, int nnzb, const int
     spmv_build_plans(A->h_rowptr, n, A, s, false);
     spmv_build_plan_b3(hb, nb, A->bplan, s, RS_SPMV_NNZ);
     spmv_build_plan_b3(hb, nb, A->bplan32, s, RS_SPMV_NNZ32);
+    sliced_build3(A, hb, hc, s);
     A->pattern_id = next_pattern_id();
     RS_HIP(hipStreamSynchronize(s));
     *out = A;

@@ -3180,7 +3180,7 @@ bool gmres_fused_ok(const rsgpu_csr *A, const rsgpu_prec *M, int elem_bytes, int
 {
     const char *e = getenv("RSGPU_GMRES_FUSED");
     const bool off = e && e[0] == '0';
-    if (off || !M || !A || !A->sl.ok() || !M->band || M->n != A->n || M->n == 0 ||
+    if (off || !M || !A || !A->sl.ok() || A->sl.b3 || !M->band || M->n != A->n || M->n == 0 ||
         m + 1 > RS_FUSE_COLS || M->kind == RSGPU_PREC_ILUT || M->kind == RSGPU_PREC_ICC ||
         M->kind == RSGPU_PREC_AMG || M->kind == RSGPU_PREC_SSOR ||
         M->kind == RSGPU_PREC_DIAGONAL || M->kind == RSGPU_PREC_IDENTITY || M->skip.done ||

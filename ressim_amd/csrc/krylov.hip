@@ -277,7 +277,7 @@ __global__ void __launch_bounds__(RS_SPMV_BS)
 
 // k_spmv_red over the sliced copy: each workgroup computes a contiguous run
 // of 4-chunk tiles, then the same deterministic reduction
-template <typename T, class Epi, int NR>
+template <typename T, class Epi, int NR, bool B3>
 __global__ void __launch_bounds__(RS_SPMV_BS)
     k_spmv_red_sl(SlView sv, int ntiles, const T *__restrict__ val, const T *__restrict__ x,
                   Epi epi, DevState *st, RedSlot slot, int fin, int inner_step)
@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(RS_SPMV_BS)
     for (int g = g0; g < g1; ++g) {
         const int row0 = sliced_wave_row(g);
         if (row0 < sv.n)
-            sliced_rows<T>(row0, sv, val, x, epi);
+            sliced_rows_any<B3, T>(row0, sv, val, x, epi);
     }
     double v[NR], tot[NR];
     v[0] = epi.a1;
@@ -300,7 +300,7 @@ __global__ void __launch_bounds__(RS_SPMV_BS)
         finish_scalar(st, fin, tot[0], NR > 1 ? tot[NR - 1] : 0.0);
 }
 
-template <typename T>
+template <typename T, bool B3>
 __global__ void __launch_bounds__(RS_SPMV_BS)
     k_spmv_skip_sl(SlView sv, const T *__restrict__ val, const T *__restrict__ x,
                    T *__restrict__ y, const DevState *st, int inner_step)
@@ -311,14 +311,14 @@ __global__ void __launch_bounds__(RS_SPMV_BS)
     if (row0 >= sv.n)
         return;
     EpiMult<T> epi{y};
-    sliced_rows<T>(row0, sv, val, x, epi);
+    sliced_rows_any<B3, T>(row0, sv, val, x, epi);
 }
 
 // GMRES's scale step fused into the next step's SpMV: v = (1/H(i,i-1)) w is
 // formed on the fly for every gathered entry (k_gm_scale's rule, so the
 // products are bit-identical to scaling first), and each wave stores its own
 // 64 rows of v (the basis vector v_i).  Skipped when the step is.
-template <typename T>
+template <typename T, bool B3>
 __global__ void __launch_bounds__(RS_SPMV_BS)
     k_spmv_scale_sl(SlView sv, const T *__restrict__ val, const T *__restrict__ w,
                     T *__restrict__ vout, T *__restrict__ y, const DevState *st)
@@ -333,7 +333,7 @@ __global__ void __launch_bounds__(RS_SPMV_BS)
     if (r < sv.n)
         vout[r] = (sc == T(0)) ? T(0) : sc * w[r];
     EpiMult<T> epi{y};
-    sliced_rows<T, EpiMult<T>, true>(row0, sv, val, w, epi, sc);
+    sliced_rows_any<B3, T, EpiMult<T>, true>(row0, sv, val, w, epi, sc);
 }
 
 // y = A x without a reduction, skipped when the solve has stopped
@@ -1882,7 +1882,7 @@ static int red_grid(int ngroups)
     return std::max(1, std::min(ngroups, cap));
 }
 
-template <typename T, class Epi, int NR>
+template <typename T, class Epi, int NR, bool B3>
 static int red_grid_sl(int ntiles)
 {
     static int cap = 0;
@@ -1890,7 +1890,7 @@ static int red_grid_sl(int ntiles)
         int per_cu = 0, cus = 0, dev = 0;
         RS_HIP(hipGetDevice(&dev));
         RS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, (const void *)k_spmv_red_sl<T, Epi, NR>, RS_SPMV_BS, 0));
+            &per_cu, (const void *)k_spmv_red_sl<T, Epi, NR, B3>, RS_SPMV_BS, 0));
         RS_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
         cap = std::max(1, per_cu) * std::max(1, cus);
     }
@@ -1917,9 +1917,14 @@ void launch_red(rsgpu_solver *S, rsgpu_csr *A, const T *x, Epi epi, DevState *st
     hipStream_t s = S->ctx->stream;
     if (const T *sv = sliced_vals<T>(S, A)) {
         const int ntiles = sliced_tiles(A->n);
-        hipLaunchKernelGGL((k_spmv_red_sl<T, Epi, NR>), dim3(red_grid_sl<T, Epi, NR>(ntiles)),
-                           dim3(RS_SPMV_BS), 0, s, A->slview(), ntiles, sv, x, epi, st,
-                           slot_of(S), fin, inner);
+        if (A->sl.b3)
+            hipLaunchKernelGGL((k_spmv_red_sl<T, Epi, NR, true>),
+                               dim3(red_grid_sl<T, Epi, NR, true>(ntiles)), dim3(RS_SPMV_BS), 0,
+                               s, A->slview(), ntiles, sv, x, epi, st, slot_of(S), fin, inner);
+        else
+            hipLaunchKernelGGL((k_spmv_red_sl<T, Epi, NR, false>),
+                               dim3(red_grid_sl<T, Epi, NR, false>(ntiles)), dim3(RS_SPMV_BS), 0,
+                               s, A->slview(), ntiles, sv, x, epi, st, slot_of(S), fin, inner);
         RS_CHECK_LAUNCH();
         return;
     }
@@ -1960,8 +1965,9 @@ void launch_spmv_skip(rsgpu_solver *S, rsgpu_csr *A, const T *x, T *y, DevState 
 {
     hipStream_t s = S->ctx->stream;
     if (const T *sv = sliced_vals<T>(S, A)) {
-        hipLaunchKernelGGL(k_spmv_skip_sl<T>, dim3(sliced_tiles(A->n)), dim3(RS_SPMV_BS), 0,
-                           s, A->slview(), sv, x, y, st, 1);
+        hipLaunchKernelGGL((A->sl.b3 ? k_spmv_skip_sl<T, true> : k_spmv_skip_sl<T, false>),
+                           dim3(sliced_tiles(A->n)), dim3(RS_SPMV_BS), 0, s, A->slview(), sv, x,
+                           y, st, 1);
         RS_CHECK_LAUNCH();
         return;
     }
@@ -2309,8 +2315,10 @@ void run_gmres(rsgpu_solver *S, rsgpu_csr *A, rsgpu_prec *M, const T *b, T *x,
             }
             if (i > 0 && slv) {
                 // v_i = w / H(i,i-1) stored by the SpMV that reads it
-                hipLaunchKernelGGL(k_spmv_scale_sl<T>, dim3(sliced_tiles(n)),
-                                   dim3(RS_SPMV_BS), 0, s, A->slview(), slv, w, vi, u, st);
+                hipLaunchKernelGGL((A->sl.b3 ? k_spmv_scale_sl<T, true>
+                                             : k_spmv_scale_sl<T, false>),
+                                   dim3(sliced_tiles(n)), dim3(RS_SPMV_BS), 0, s, A->slview(),
+                                   slv, w, vi, u, st);
                 RS_CHECK_LAUNCH();
             } else {
                 launch_spmv_skip<T>(S, A, vi, u, st);

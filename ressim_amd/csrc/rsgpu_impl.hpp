@@ -107,12 +107,22 @@ struct SpmvPlan {
 // one coalesced run and gathers x at row + offset, contiguous across the
 // wave.  Built when every chunk's union has at most 32 offsets; absent
 // otherwise (nchunks == 0).
+//
+// Block form (b3, 3x3 block matrices): the same chunks of scalar rows, but a
+// table lists block offsets (block column minus block row, ascending), a
+// row's slots are (entry k, block column e) at 3k + e, and mask16 holds per
+// scalar row the bits of the entries its block row has.  Row 3I+a reads
+// slot 3k+e against x[3 (I + d_k) + e], which is its expanded CSR order.
+// Built when every chunk's union has at most RS_SL3_MAXB block offsets.
 #define RS_SL_ROWS 256
+#define RS_SL3_MAXB 10
 struct SlicedFmt {
     int nchunks = 0, ntables = 0;
+    bool b3 = false;
     long long nslots = 0;              // values per copy (64 per slot row)
     DevBuf<int> ctab, cbase, tptr, toff;
     DevBuf<unsigned> mask;
+    DevBuf<unsigned short> mask16;     // block form
     DevBuf<double> v64;                // filled by sliced_fill (per solve)
     DevBuf<float> v32;                 // filled with the FP32 CSR copy
     uint64_t gen64 = 0;                // value generation v64 was filled from
@@ -127,6 +137,7 @@ struct SlView {
     int n, ncols, nchunks;
     const int *ctab, *cbase, *tptr, *toff;
     const unsigned *mask;
+    const unsigned short *mask16;      // block form only
 };
 
 struct rsgpu_csr {
@@ -165,7 +176,7 @@ struct rsgpu_csr {
     SlView slview() const
     {
         return SlView{n, ncols, sl.nchunks, sl.ctab.p, sl.cbase.p, sl.tptr.p, sl.toff.p,
-                      sl.mask.p};
+                      sl.mask.p, sl.mask16.p};
     }
 };
 
@@ -705,6 +716,9 @@ void sliced_fill(rsgpu_csr *A, hipStream_t s);
 // y = A x over the sliced copy of precision T (mult's zero rule)
 template <typename T>
 void sliced_mult(hipStream_t s, const rsgpu_csr *A, const T *x, T *y);
+// block form of the sliced copy of a 3x3 block matrix (block pattern)
+void sliced_build3(rsgpu_csr *A, const std::vector<int> &brp, const std::vector<int> &bcol,
+                   hipStream_t s);
 // block-row plans of a 3x3 block matrix (scalar row bounds at block rows)
 void spmv_build_plan_b3(const std::vector<int> &brp, int nb, SpmvPlan &plan, hipStream_t s,
                         int nnz_wg);

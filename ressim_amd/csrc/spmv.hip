@@ -70,14 +70,20 @@ static bool sliced_enabled()
 // Chunk tables, shared tables and row masks of SlicedFmt from the host
 // pattern.  The format is left out when a chunk's union exceeds 32 offsets,
 // a row repeats a column, or the slot count passes 2^31 slot rows.
-static void sliced_build(rsgpu_csr *A, hipStream_t s)
+static void sliced_reset(SlicedFmt &F)
 {
-    SlicedFmt &F = A->sl;
     F.nchunks = F.ntables = 0;
     F.nslots = 0;
     F.gen64 = 0;
+    F.b3 = false;
     F.v64.release();
     F.v32.release();
+}
+
+static void sliced_build(rsgpu_csr *A, hipStream_t s)
+{
+    SlicedFmt &F = A->sl;
+    sliced_reset(F);
     const int n = A->n;
     if (n == 0 || A->bs != 1 || !sliced_enabled())
         return;
@@ -153,6 +159,73 @@ static void sliced_build(rsgpu_csr *A, hipStream_t s)
     F.nchunks = nch;
 }
 
+// Block form of SlicedFmt for a 3x3 block matrix from its block pattern
+// (rows sorted, no repeats): per chunk of scalar rows, the ascending union
+// of the block offsets of every block row with a scalar row in the chunk
+// (a block row can straddle two chunks; each of its scalar rows takes the
+// mask of its own chunk's table).  Left out when a union exceeds
+// RS_SL3_MAXB offsets or the slot count passes 2^31 slot rows.
+void sliced_build3(rsgpu_csr *A, const std::vector<int> &brp, const std::vector<int> &bcol,
+                   hipStream_t s)
+{
+    SlicedFmt &F = A->sl;
+    sliced_reset(F);
+    const int n = A->n;
+    if (n == 0 || A->bs != 3 || !sliced_enabled())
+        return;
+    const int nch = (n + RS_SL_ROWS - 1) / RS_SL_ROWS;
+    std::vector<int> ctab(nch), cbase(nch), tptr{0}, toff;
+    std::vector<unsigned short> mask(n, 0);
+    std::map<std::vector<int>, int> tables;
+    std::vector<int> u;
+    long long rows = 0;
+    for (int c = 0; c < nch; ++c) {
+        const int r0 = RS_SL_ROWS * c, r1 = std::min(n, r0 + RS_SL_ROWS);
+        const int I0 = r0 / 3, I1 = (r1 - 1) / 3;
+        u.clear();
+        for (int I = I0; I <= I1; ++I)
+            for (int b = brp[I]; b < brp[I + 1]; ++b)
+                u.push_back(bcol[b] - I);
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        if (u.size() > RS_SL3_MAXB)
+            return;
+        for (int r = r0; r < r1; ++r) {
+            const int I = r / 3;
+            unsigned mk = 0;
+            for (int b = brp[I]; b < brp[I + 1]; ++b)
+                mk |= 1u << (std::lower_bound(u.begin(), u.end(), bcol[b] - I) - u.begin());
+            mask[r] = (unsigned short)mk;
+        }
+        auto it = tables.find(u);
+        int t;
+        if (it == tables.end()) {
+            t = (int)tables.size();
+            tables.emplace(u, t);
+            toff.insert(toff.end(), u.begin(), u.end());
+            tptr.push_back((int)toff.size());
+        } else {
+            t = it->second;
+        }
+        ctab[c] = t;
+        cbase[c] = (int)rows;
+        rows += 3LL * (long long)u.size();
+        if (rows >= (1LL << 31))
+            return;
+    }
+    toff.insert(toff.end(), 32, 0);
+    F.ctab.upload(ctab, s);
+    F.cbase.upload(cbase, s);
+    F.tptr.upload(tptr, s);
+    F.toff.upload(toff, s);
+    F.mask16.upload(mask, s);
+    RS_HIP(hipStreamSynchronize(s));
+    F.ntables = (int)tables.size();
+    F.nslots = rows * RS_SL_ROWS;
+    F.b3 = true;
+    F.nchunks = nch;
+}
+
 void spmv_build_plans(const std::vector<int> &rowptr, int n, rsgpu_csr *A, hipStream_t s,
                       bool sliced)
 {
@@ -187,6 +260,34 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// the block form's fill: row r's entries are its block row's blocks in
+// ascending block column, three scalars each
+template <typename T>
+__global__ void __launch_bounds__(256)
+    k_sliced_fill3(SlView S, const int *__restrict__ rowptr, const double *__restrict__ val,
+                   T *__restrict__ out)
+{
+    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
+    const int c = (int)(g / RS_SL_ROWS), i = (int)(g % RS_SL_ROWS);
+    if (c >= S.nchunks)
+        return;
+    const int r = c * RS_SL_ROWS + i;
+    const int t = S.ctab[c];
+    const int W = S.tptr[t + 1] - S.tptr[t];
+    const unsigned m = r < S.n ? S.mask16[r] : 0u;
+    int j = r < S.n ? rowptr[r] : 0;
+    T *o = out + (size_t)S.cbase[c] * RS_SL_ROWS + i;
+    for (int k = 0; k < W; ++k) {
+        const bool has = (m >> k) & 1u;
+        for (int e = 0; e < 3; ++e) {
+            T v = T(0);
+            if (has)
+                v = (T)val[j++];
+            o[(size_t)(3 * k + e) * RS_SL_ROWS] = v;
+        }
+    }
+}
+
 template <typename T>
 static DevBuf<T> &sliced_buf(rsgpu_csr *A);
 template <>
@@ -203,8 +304,9 @@ void sliced_fill(rsgpu_csr *A, hipStream_t s)
     DevBuf<T> &b = sliced_buf<T>(A);
     b.alloc((size_t)F.nslots);
     const long long threads = (long long)F.nchunks * RS_SL_ROWS;
-    hipLaunchKernelGGL(k_sliced_fill<T>, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s,
-                       A->slview(), A->rowptr.p, A->val.p, b.p);
+    hipLaunchKernelGGL((F.b3 ? k_sliced_fill3<T> : k_sliced_fill<T>),
+                       dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, A->slview(),
+                       A->rowptr.p, A->val.p, b.p);
     RS_CHECK_LAUNCH();
     if (sizeof(T) == 8)
         F.gen64 = A->val_gen;
@@ -212,7 +314,7 @@ void sliced_fill(rsgpu_csr *A, hipStream_t s)
 template void sliced_fill<double>(rsgpu_csr *, hipStream_t);
 template void sliced_fill<float>(rsgpu_csr *, hipStream_t);
 
-template <typename T>
+template <typename T, bool B3>
 __global__ void __launch_bounds__(256)
     k_spmv_sliced(SlView S, const T *__restrict__ val, const T *__restrict__ x,
                   T *__restrict__ y)
@@ -221,15 +323,15 @@ __global__ void __launch_bounds__(256)
     if (row0 >= S.n)
         return;
     EpiMult<T> epi{y};
-    sliced_rows<T>(row0, S, val, x, epi);
+    sliced_rows_any<B3, T>(row0, S, val, x, epi);
 }
 
 template <typename T>
 void sliced_mult(hipStream_t s, const rsgpu_csr *A, const T *x, T *y)
 {
     const T *val = sizeof(T) == 8 ? (const T *)A->sl.v64.p : (const T *)A->sl.v32.p;
-    hipLaunchKernelGGL(k_spmv_sliced<T>, dim3(sliced_tiles(A->n)), dim3(256), 0, s,
-                       A->slview(), val, x, y);
+    hipLaunchKernelGGL((A->sl.b3 ? k_spmv_sliced<T, true> : k_spmv_sliced<T, false>),
+                       dim3(sliced_tiles(A->n)), dim3(256), 0, s, A->slview(), val, x, y);
     RS_CHECK_LAUNCH();
 }
 template void sliced_mult<double>(hipStream_t, const rsgpu_csr *, const double *, double *);

@@ -351,6 +351,89 @@ __device__ __forceinline__ void sliced_rows(int row0, const SlView &S, const T *
         epi.row(r, dot);
 }
 
+// Block form (SlicedFmt::b3): row r = 3I + a of the 64 rows from row0.
+// NB table entries are read (NB >= the chunk's entry count W; entries past
+// W repeat the last one and their slots repeat the last slot), and the 3 NB
+// value loads and gathers are all issued before the first product.  Slots
+// are summed in the order 3k + e, each product rounded to T before it is
+// added, and absent entries are skipped, as in the scalar form.
+template <int NB, typename T, class Epi, bool SCALE>
+__device__ __forceinline__ void sliced_rows3_nb(int r, int I, unsigned m, int W, const int *off,
+                                                const T *vc, const T *__restrict__ x, int cmax,
+                                                T &dot, T sc)
+{
+    const int lastoff = off[W - 1];
+    int offs[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k)
+        offs[k] = off[k];
+#pragma unroll
+    for (int k = 0; k < NB; ++k)
+        offs[k] = k < W ? offs[k] : lastoff;
+    const int ns = 3 * W;
+    T v[3 * NB], xg[3 * NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k)
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            const int sl = min(3 * k + e, ns - 1);
+            const int col = min(max(3 * (I + offs[k]) + e, 0), cmax);
+            v[3 * k + e] = vc[(size_t)sl * RS_SL_ROWS];
+            xg[3 * k + e] = x[col];
+        }
+    if (SCALE) {
+#pragma unroll
+        for (int k = 0; k < 3 * NB; ++k)
+            xg[k] = (sc == T(0)) ? T(0) : sc * xg[k];
+    }
+#pragma unroll
+    for (int k = 0; k < NB; ++k)
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+            const T q = dot + v[3 * k + e] * xg[3 * k + e];
+            dot = ((m >> k) & 1u) ? q : dot;
+        }
+}
+
+template <typename T, class Epi, bool SCALE = false>
+__device__ __forceinline__ void sliced_rows3(int row0, const SlView &S, const T *__restrict__ val,
+                                             const T *__restrict__ x, Epi &epi, T sc = T(1))
+{
+    const int lane = threadIdx.x & 63;
+    const int c = row0 / RS_SL_ROWS, sub = row0 - c * RS_SL_ROWS;
+    const int r = row0 + lane;
+    const int t = S.ctab[c];
+    const int o0 = S.tptr[t], W = S.tptr[t + 1] - o0;
+    const T *vc = val + (size_t)S.cbase[c] * RS_SL_ROWS + sub + lane;
+    const int *off = S.toff + o0;
+    const bool live = r < S.n;
+    const int rr = min(r, S.n - 1);
+    const int I = rr / 3;
+    const unsigned m = S.mask16[rr];
+    const int cmax = S.ncols - 1;
+    T dot = epi.init(rr);
+    asm volatile("" : "+v"(dot));
+    // seven entries: the 3-D seven-point block stencil in one round trip
+    if (W > 0 && W <= 7)
+        sliced_rows3_nb<7, T, Epi, SCALE>(rr, I, m, W, off, vc, x, cmax, dot, sc);
+    else if (W > 7)
+        sliced_rows3_nb<RS_SL3_MAXB, T, Epi, SCALE>(rr, I, m, W, off, vc, x, cmax, dot, sc);
+    if (live)
+        epi.row(r, dot);
+}
+
+// the scalar or the block form
+template <bool B3, typename T, class Epi, bool SCALE = false>
+__device__ __forceinline__ void sliced_rows_any(int row0, const SlView &S,
+                                                const T *__restrict__ val,
+                                                const T *__restrict__ x, Epi &epi, T sc = T(1))
+{
+    if constexpr (B3)
+        sliced_rows3<T, Epi, SCALE>(row0, S, val, x, epi, sc);
+    else
+        sliced_rows<T, Epi, SCALE>(row0, S, val, x, epi, sc);
+}
+
 // first row of the calling wave in 256-row tile `tile` (wave-uniform)
 __device__ __forceinline__ int sliced_wave_row(int tile)
 {

@@ -70,6 +70,95 @@ def test_block_spmv_bitexact(gpu, oracle):
                                                              x32))
 
 
+def _block_sliced_check(gpu, oracle, A, rp, ci, data, rng):
+    import torch
+    n = A.n
+    nch, nslots, ntab = A.sliced_info()
+    assert nch == (n + 255) // 256 and nslots % 64 == 0 and nslots >= len(ci) and ntab >= 1
+    A.sync_sliced()
+    A.sync_f32()
+    x = rng.standard_normal(n)
+
+    def run(xh):
+        xd = torch.tensor(xh, device="cuda")
+        yd = torch.empty_like(xd)
+        A.mult_sliced(xd, yd)
+        return yd.cpu().numpy()
+    np.testing.assert_array_equal(run(x), oracle.mult(rp, ci, data, x))
+    x32 = x.astype(np.float32)
+    np.testing.assert_array_equal(run(x32),
+                                  oracle.mult_f32(rp, ci, data.astype(np.float32), x32))
+    # the block-row core on the same matrix
+    y = torch.zeros(n, dtype=torch.float64, device="cuda")
+    A.mult(torch.tensor(x, device="cuda"), y)
+    np.testing.assert_array_equal(y.cpu().numpy(), run(x))
+    # inf and NaN reach exactly the rows whose pattern holds them
+    xb = x.copy()
+    xb[n // 3] = np.inf
+    xb[n // 2 + 1] = np.nan
+    yo = oracle.mult(rp, ci, data, xb)
+    assert np.isfinite(yo).sum() > n // 2
+    np.testing.assert_array_equal(run(xb), yo)
+    # new values need a new sync; the copy follows them
+    d2 = rng.standard_normal(len(ci))
+    A.set_data(d2)
+    A.sync_sliced()
+    np.testing.assert_array_equal(run(x), oracle.mult(rp, ci, d2, x))
+
+
+@pytest.mark.parametrize("shape", [(12, 9, 5), (23, 7, 1), (50, 1, 1)])
+def test_block_sliced_spmv_bitexact(gpu, oracle, shape):
+    """The block form of the sliced copy on black-oil Jacobians (seven-,
+    five- and three-entry block stencils; 256-row chunks split block rows,
+    a partial last chunk): FP64 and FP32 products bit-identical to the
+    oracle's row-sequential sums of the expanded CSR and to the block-row
+    core, non-finite x entries reaching only the rows that hold them."""
+    nx, ny, nz = shape
+    pr = P.config3(nx=nx, ny=ny, nz=nz)
+    bo, J, R = _device_jacobian(gpu, pr)
+    rp, ci = J.getRowPointers(), J.getColumnIndices()
+    ntab_expected = {(12, 9, 5): 7, (23, 7, 1): 5, (50, 1, 1): 3}[shape]
+    _block_sliced_check(gpu, oracle, J, rp, ci, J.getData(),
+                        np.random.default_rng(8 + nx))
+    # no chunk's table is wider than the stencil
+    nch, nslots, _ = J.sliced_info()
+    assert nslots <= nch * 256 * 3 * ntab_expected
+
+
+def test_block_sliced_generic_patterns(gpu, oracle):
+    """Random block patterns within a band: eight to ten block offsets per
+    chunk (the wider load batch), an empty block row, and a pattern whose
+    chunks span more than ten block offsets (no sliced copy, refused)."""
+    import torch
+    rng = np.random.default_rng(9)
+    nb = 700
+    rows = []
+    for I in range(nb):
+        if I in (5, 86, 699):
+            rows.append(np.zeros(0, np.int32))
+            continue
+        cand = np.arange(max(0, I - 4), min(nb, I + 5))
+        k = int(rng.integers(1, len(cand) + 1))
+        rows.append(np.sort(rng.choice(cand, size=k, replace=False)).astype(np.int32))
+    brp = np.zeros(nb + 1, np.int32)
+    brp[1:] = np.cumsum([len(r) for r in rows])
+    bci = np.concatenate(rows)
+    A = gpu.CompRowMatrix.from_bsr3(nb, brp, bci)
+    rp, ci = oracle.bo_expand(brp, bci)
+    data = rng.standard_normal(len(ci))
+    A.set_data(data)
+    assert A.sliced_info()[0] > 0
+    _block_sliced_check(gpu, oracle, A, rp, ci, data, rng)
+    # offsets spread over more than ten block columns: block-row core only
+    rows = [np.sort(rng.choice(nb, size=4, replace=False)).astype(np.int32) for _ in range(nb)]
+    brp[1:] = np.cumsum([len(r) for r in rows])
+    B = gpu.CompRowMatrix.from_bsr3(nb, brp, np.concatenate(rows))
+    assert B.sliced_info()[0] == 0
+    x = torch.zeros(3 * nb, dtype=torch.float64, device="cuda")
+    with pytest.raises(ValueError):
+        B.mult_sliced(x, torch.empty_like(x))
+
+
 def test_generic_bsr3_with_long_and_empty_block_rows(gpu, oracle):
     """A block pattern with an empty block row and one block row longer
     than the SpMV stage (its three scalar rows are summed one by one)."""
