@@ -209,7 +209,9 @@ int rsgpu_prec_info(const rsgpu_prec *M, int *nnz, int *nlevels_lower,
  * wavefront (every block Ly <= 64 lines of Lx rows coupled among 9-point
  * neighbours, at most 4 blocks per CU; RSGPU_ILU_BAND=0 in the environment
  * turns it off, RSGPU_ILU_BAND_APPLY=0/1 forces the choice), 2 3x3 cell
- * schedules. */
+ * schedules, 3 3x3 cell schedules with the level records streamed through
+ * LDS (2 or 3 neighbour blocks per triangle and a level's records within
+ * 4 KiB; RSGPU_ILU_CELL_STREAM=0 turns it off). */
 int rsgpu_prec_sweep_form(const rsgpu_prec *M, int *form);
 
 /* AMG parameters: the constructor arguments of AMG(omegaPreF, omegaPreR,

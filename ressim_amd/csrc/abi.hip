@@ -936,7 +936,7 @@ extern "C" int rsgpu_prec_sweep_form(const rsgpu_prec *M, int *form)
 {
     if (!M || !form)
         return RSGPU_BAD_ARGUMENT;
-    *form = M->cell ? 2 : (M->band && prec_band_apply(M)) ? 1 : 0;
+    *form = M->cell ? (prec_cell_stream(M) ? 3 : 2) : (M->band && prec_band_apply(M)) ? 1 : 0;
     return RSGPU_OK;
 }
 

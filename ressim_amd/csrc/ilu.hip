@@ -1714,6 +1714,222 @@ __global__ void __launch_bounds__(64)
         x[b0 + r] = xv[r];
 }
 
+// ---------------------------------------------------------------- streamed cell sweeps
+// A block's level records are one contiguous run of each record array
+// (levels in order, a level's cells in order).  The wave streams whole
+// levels into an LDS ring of st3_depth level slots with direct-to-LDS
+// buffer loads (per instruction 64 lanes x 16 consecutive bytes, no
+// registers held), st3_depth - 1 levels ahead of the level it computes, and
+// its lanes read their records from the ring one level ahead; the gathers
+// and the rows' arithmetic are the cell pipeline's (cell_gather,
+// cell_finish), so the results are the same bits.  Every level issues the
+// same NV + 1 copies (levels past the sweep's end copy its last level again
+// into a slot that is never read), so the wait for the next level's slot is
+// a fixed count of younger copies.  The copies are issued from inline
+// assembly: the compiler's own tracking of direct-to-LDS loads waits for
+// every one of them before each LDS read, which leaves one level in flight.
+// Buffer range checking turns reads past a record array's end into zeros.
+#define RS_ST3_D2_64 10
+#define RS_ST3_D2_32 16
+#define RS_ST3_MAXNV 4
+
+template <typename T, int NV>
+constexpr int st3_depth()
+{
+    constexpr int d = sizeof(T) == 8 ? RS_ST3_D2_64 : RS_ST3_D2_32;
+    constexpr int dv = 63 / (NV + 1) + 1;   // copies in flight within the 6-bit vmcnt
+    return d < dv ? d : dv;
+}
+
+template <int K>
+__device__ __forceinline__ void wait_vm_count()
+{
+    static_assert(K >= 0 && K <= 63, "6-bit vmcnt");
+    asm volatile("s_waitcnt vmcnt(%0)" ::"n"(K) : "memory");
+}
+
+// one direct-to-LDS copy: lane i's NB bytes at byte voff of the buffer go
+// to LDS byte lds + NB i
+template <int NB>
+__device__ __forceinline__ void lds_dma(rs_v4i rsrc, unsigned lds, int voff)
+{
+    static_assert(NB == 16 || NB == 4, "dword or dwordx4 lanes");
+    if constexpr (NB == 16)
+        asm volatile("s_mov_b32 m0, %0\n\t"
+                     "s_nop 0\n\t"
+                     "buffer_load_dwordx4 %1, %2, 0 offen lds" ::"s"(lds),
+                     "v"(voff), "s"(rsrc)
+                     : "memory", "m0");
+    else
+        asm volatile("s_mov_b32 m0, %0\n\t"
+                     "s_nop 0\n\t"
+                     "buffer_load_dword %1, %2, 0 offen lds" ::"s"(lds),
+                     "v"(voff), "s"(rsrc)
+                     : "memory", "m0");
+}
+
+__device__ __forceinline__ rs_v4i raw_rsrc(const void *p, int nbytes)
+{
+    const unsigned long long a = (unsigned long long)p;
+    rs_v4i r;
+    r.x = (int)(unsigned)a;
+    r.y = (int)(unsigned)((a >> 32) & 0xffffu);   // stride 0
+    r.z = nbytes;
+    r.w = RS_BUF_WORD3;
+    return r;
+}
+
+__device__ __forceinline__ unsigned lds_addr(const unsigned char *p)
+{
+    return (unsigned)(size_t)(const __attribute__((address_space(3))) unsigned char *)p;
+}
+
+struct St3Ring {
+    rs_v4i vals, cols;          // record arrays
+    unsigned char *rv_lds, *rc_lds;   // the rings
+    int rv, rc;                 // record bytes (values, indices)
+    int sv, sc;                 // slot bytes
+    bool c16;                   // index copies of 16 (else 4) bytes per lane
+};
+
+// copy the level whose first record is `off` into the slot of level m
+template <int NV, int D2>
+__device__ __forceinline__ void st3_copy(const St3Ring &S, int off, int m)
+{
+    off = __builtin_amdgcn_readfirstlane(off);
+    const int slot = m % D2;
+    const int lane = (int)threadIdx.x;
+    const unsigned lv = lds_addr(S.rv_lds + slot * S.sv);
+    const int vb = off * S.rv + lane * 16;
+#pragma unroll
+    for (int q = 0; q < NV; ++q)
+        lds_dma<16>(S.vals, lv + q * 1024, vb + q * 1024);
+    const unsigned lc = lds_addr(S.rc_lds + slot * S.sc);
+    if (S.c16)
+        lds_dma<16>(S.cols, lc, off * S.rc + lane * 16);
+    else
+        lds_dma<4>(S.cols, lc, off * S.rc + lane * 4);
+}
+
+template <typename T, int WB, bool DIV>
+__device__ __forceinline__ void cell_load_ring(const LevelMeta &m, bool valid, const St3Ring &S,
+                                               int slot, CellRegs<T, WB, DIV> &R)
+{
+    constexpr int IR = cell_irec<WB>(), VR = cell_vrec<WB, DIV>();
+    constexpr int es = (int)sizeof(T);
+    const int count = __builtin_amdgcn_readfirstlane(m.count);
+    const int t = threadIdx.x;
+    const int tt = min(t, count - 1);
+    R.act = valid && t < count;
+    const unsigned char *pc = S.rc_lds + slot * S.sc + tt * (IR * 4);
+    const unsigned char *pv = S.rv_lds + slot * S.sv + tt * (VR * es);
+#pragma unroll
+    for (int k = 0; k < IR / 4; ++k) {
+        const rs_v4i q = *(const rs_v4i *)(pc + 16 * k);
+        __builtin_memcpy((char *)R.idx + 16 * k, &q, 16);
+    }
+#pragma unroll
+    for (int k = 0; k < VR * es / 16; ++k) {
+        const rs_v4i q = *(const rs_v4i *)(pv + 16 * k);
+        __builtin_memcpy((char *)R.v + 16 * k, &q, 16);
+    }
+}
+
+template <typename T, int WB, bool DIV, int NV>
+__device__ __forceinline__ void sweep_st3(int nl, const LevelMeta *sm, const St3Ring &S,
+                                          unsigned char *xv, int junkb)
+{
+    constexpr int D2 = st3_depth<T, NV>();
+    // copies younger than the next level's: those of the D2 - 2 levels after it
+    constexpr int WAIT = (D2 - 2) * (NV + 1);
+    if (nl <= 0)
+        return;
+    const int last = nl - 1;
+    for (int k = 0; k < D2 - 1; ++k)
+        st3_copy<NV, D2>(S, sm[min(k, last)].off, k);
+    // metadata one iteration ahead: the next copy's first record, the next
+    // ring read's level
+    int offc = sm[min(D2 - 1, last)].off;
+    LevelMeta mr = sm[min(1, last)];
+    wait_vm_count<WAIT>();
+    CellRegs<T, WB, DIV> r[2];
+    cell_load_ring<T, WB, DIV>(sm[0], true, S, 0, r[0]);
+    for (int l = 0; l < nl; l += 2) {
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const int lev = l + k;
+            st3_copy<NV, D2>(S, offc, lev + D2 - 1);
+            offc = sm[min(lev + D2, last)].off;
+            wait_vm_count<WAIT>();
+            CellGathered<T, WB> g;
+            cell_gather<T, WB, DIV>(r[k], xv, g);
+            __builtin_amdgcn_sched_barrier(0);
+            cell_load_ring<T, WB, DIV>(mr, lev + 1 <= last, S, (lev + 1) % D2, r[k ^ 1]);
+            mr = sm[min(lev + 2, last)];
+            asm volatile("" ::: "memory");
+            __builtin_amdgcn_sched_barrier(0);
+            cell_finish<T, WB, DIV>(r[k], g, xv, junkb);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+}
+
+// One wave per block, L then U cell sweeps streamed through the rings.
+// LDS: the block's vector (+ zero and junk slots), level metadata, then the
+// value ring (D2 slots of sv bytes) and the index ring (D2 slots of sc).
+template <typename T, int WB, int NV>
+__global__ void __launch_bounds__(64)
+    k_ilu_apply_st3(const int *__restrict__ bstart, PfSweep A, PfSweep B,
+                    const T *__restrict__ b, T *__restrict__ x, int xv_bytes, int ring_off,
+                    int sv, int sc, int c16, const int *skip1, const int *skip2, BlkSkip bskip)
+{
+    if ((skip1 && *skip1) || (skip2 && *skip2))
+        return;
+    if (blk_skipped(bskip, blockIdx.x))
+        return;
+    constexpr int D2 = st3_depth<T, NV>();
+    extern __shared__ unsigned char rs_smem[];
+    const int blk = blockIdx.x;
+    const int b0 = bstart[blk], b1 = bstart[blk + 1], nb = b1 - b0;
+    T *xv = (T *)rs_smem;
+    LevelMeta *sm = (LevelMeta *)(rs_smem + xv_bytes);
+    const int la0 = A.blk[blk], nla = A.blk[blk + 1] - la0;
+    const int lb0 = B.blk[blk], nlb = B.blk[blk + 1] - lb0;
+    if (nla > 0)
+        wg_copy(sm, A.meta + la0, nla);
+    if (nlb > 0)
+        wg_copy(sm + nla, B.meta + lb0, nlb);
+    if (nb > 0)
+        wg_copy(xv, b + b0, nb);
+    if (threadIdx.x < 3)
+        xv[nb + threadIdx.x] = T(0);
+    __syncthreads();
+    const int junkb = (nb + 3) * (int)sizeof(T);
+    constexpr int IR = cell_irec<WB>();
+    St3Ring S;
+    S.rv_lds = rs_smem + ring_off;
+    S.rc_lds = rs_smem + ring_off + D2 * sv;
+    S.rc = IR * 4;
+    S.sv = sv;
+    S.sc = sc;
+    S.c16 = __builtin_amdgcn_readfirstlane(c16) != 0;
+    S.vals = raw_rsrc(A.val, A.val_bytes);
+    S.cols = raw_rsrc(A.colb, A.col_bytes);
+    S.rv = cell_vrec<WB, false>() * (int)sizeof(T);
+    sweep_st3<T, WB, false, NV>(nla, sm, S, rs_smem, junkb);
+    S.vals = raw_rsrc(B.val, B.val_bytes);
+    S.cols = raw_rsrc(B.colb, B.col_bytes);
+    S.rv = cell_vrec<WB, true>() * (int)sizeof(T);
+    sweep_st3<T, WB, true, NV>(nlb, sm + nla, S, rs_smem, junkb);
+    // no copy may still be writing the rings when the workgroup's LDS is
+    // released
+    wait_vm_count<0>();
+    __syncthreads();
+    for (int r = threadIdx.x; r < nb; r += blockDim.x)
+        x[b0 + r] = xv[r];
+}
+
 // ---------------------------------------------------------------- host ops
 template <typename T>
 static void launch_factor_band(rsgpu_prec *M, T *lu);
@@ -3577,6 +3793,93 @@ static size_t cell_xv_bytes(int max_block_rows, int es)
     return ((size_t)(max_block_rows + 4) * es + 15) & ~(size_t)15;
 }
 
+template <typename T, int WB, int NV>
+static void launch_st3(rsgpu_prec *M, const PfSweep &A, const PfSweep &B, const T *b, T *x,
+                       size_t xv_bytes, size_t ring_off, int sv, int sc, bool c16,
+                       const int *sk1, const int *sk2)
+{
+    constexpr int D2 = st3_depth<T, NV>();
+    const size_t shm = ring_off + (size_t)D2 * (sv + sc);
+    static size_t granted = 65536;
+    if (shm > granted) {
+        RS_HIP(hipFuncSetAttribute((const void *)k_ilu_apply_st3<T, WB, NV>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+        granted = shm;
+    }
+    hipLaunchKernelGGL((k_ilu_apply_st3<T, WB, NV>), dim3(M->nblocks), dim3(64), shm,
+                       M->ctx->stream, M->bstart.p, A, B, b, x, (int)xv_bytes, (int)ring_off,
+                       sv, sc, (int)c16, sk1, sk2, M->skip);
+}
+
+// The streamed cell sweeps' layout for M in elements of es bytes, or false
+// where they do not apply: WB of 2 or 3 neighbour blocks, a level's records
+// within RS_ST3_MAXNV KiB (values) and 1 KiB (indices), the rings within the
+// LDS limit.  RSGPU_ILU_CELL_STREAM=0 keeps the register pipeline
+// (k_ilu_apply_pf3).
+struct St3Plan {
+    int nv, sv, sc;
+    bool c16;
+    size_t xv_bytes, meta_bytes, ring_off;
+};
+
+static bool st3_plan(const rsgpu_prec *M, int es, St3Plan &P)
+{
+    const char *e = getenv("RSGPU_ILU_CELL_STREAM");
+    if ((e && e[0] == '0') || !M->cell || M->n == 0)
+        return false;
+    const int WB = M->L.W;
+    if (M->U.W != WB || (WB != 2 && WB != 3))
+        return false;
+    const int maxc = std::max(M->L.max_count, M->U.max_count);
+    const int rv = ((9 * WB + 6 + 3) & ~3) * es;   // the U record, the larger
+    const int rc = ((1 + WB + 3) & ~3) * 4;
+    P.nv = (maxc * rv + 1023) / 1024;
+    if (maxc <= 0 || maxc > 64 || P.nv > RS_ST3_MAXNV || maxc * rc > 1024)
+        return false;
+    P.c16 = maxc * rc > 256;
+    P.sc = P.c16 ? 1024 : 256;
+    P.sv = P.nv * 1024;
+    P.xv_bytes = cell_xv_bytes(M->max_block_rows, es);
+    P.meta_bytes = (size_t)(M->L.max_levels + M->U.max_levels) * sizeof(LevelMeta);
+    P.ring_off = (P.xv_bytes + P.meta_bytes + 15) & ~(size_t)15;
+    const int d = es == 8 ? RS_ST3_D2_64 : RS_ST3_D2_32, dv = 63 / (P.nv + 1) + 1;
+    const int d2 = std::min(d, dv);
+    return P.ring_off + (size_t)d2 * (P.sv + P.sc) <= (size_t)lds_limit(M->ctx->device);
+}
+
+bool prec_cell_stream(const rsgpu_prec *M)
+{
+    St3Plan P;
+    return st3_plan(M, M->precision == RSGPU_FP32 ? 4 : 8, P);
+}
+
+template <typename T, int WB>
+static bool try_st3(rsgpu_prec *M, const PfSweep &A, const PfSweep &B, const T *b, T *x,
+                    const int *sk1, const int *sk2)
+{
+    St3Plan P;
+    if (!st3_plan(M, (int)sizeof(T), P))
+        return false;
+    const size_t xv_bytes = P.xv_bytes, ring_off = P.ring_off;
+    const int sv = P.sv, sc = P.sc, nv = P.nv;
+    const bool c16 = P.c16;
+    switch (nv) {
+    case 1:
+        launch_st3<T, WB, 1>(M, A, B, b, x, xv_bytes, ring_off, sv, sc, c16, sk1, sk2);
+        break;
+    case 2:
+        launch_st3<T, WB, 2>(M, A, B, b, x, xv_bytes, ring_off, sv, sc, c16, sk1, sk2);
+        break;
+    case 3:
+        launch_st3<T, WB, 3>(M, A, B, b, x, xv_bytes, ring_off, sv, sc, c16, sk1, sk2);
+        break;
+    default:
+        launch_st3<T, WB, 4>(M, A, B, b, x, xv_bytes, ring_off, sv, sc, c16, sk1, sk2);
+        break;
+    }
+    return true;
+}
+
 template <typename T>
 static void launch_apply_cells(rsgpu_prec *M, const T *vl, const T *vu, const T *b, T *x,
                                const int *sk1, const int *sk2)
@@ -3587,6 +3890,14 @@ static void launch_apply_cells(rsgpu_prec *M, const T *vl, const T *vu, const T 
     const size_t shm =
         xv_bytes + (size_t)(M->L.max_levels + M->U.max_levels) * sizeof(LevelMeta);
     const PfSweep A = pf_sweep<T>(M->L, vl, nullptr), B = pf_sweep<T>(M->U, vu, nullptr);
+    if (M->L.W == 2 && try_st3<T, 2>(M, A, B, b, x, sk1, sk2)) {
+        RS_CHECK_LAUNCH();
+        return;
+    }
+    if (M->L.W == 3 && try_st3<T, 3>(M, A, B, b, x, sk1, sk2)) {
+        RS_CHECK_LAUNCH();
+        return;
+    }
     switch (M->L.W) {
     case 2:
         launch_pf3<T, 2>(M, A, B, b, x, xv_bytes, shm, sk1, sk2);

@@ -760,6 +760,8 @@ bool gmres_fused_ok(const rsgpu_csr *A, const rsgpu_prec *M, int elem_bytes, int
 // whether ILU.apply takes the line wavefront when M has its records (few
 // blocks per CU; many take the level schedules)
 bool prec_band_apply(const rsgpu_prec *M);
+// whether ILU.apply on 3x3 cells streams its records through LDS rings
+bool prec_cell_stream(const rsgpu_prec *M);
 template <typename T>
 void launch_gmres_fused(hipStream_t s, const rsgpu_csr *A, const T *slv, rsgpu_prec *M,
                         const T *win, T *vout, bool scale, T *wout, const T *V, size_t ldv,

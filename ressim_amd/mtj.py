@@ -450,7 +450,7 @@ class Preconditioner:
         f = C.c_int(0)
         L.load().rsgpu_prec_sweep_form(self.h, C.byref(f))
         return dict(nnz=a.value, levels_lower=b.value, levels_upper=c.value,
-                    sweep_form=("levels", "lines", "cells")[f.value])
+                    sweep_form=("levels", "lines", "cells", "cells_streamed")[f.value])
 
 
 class IdentityPreconditioner(Preconditioner):

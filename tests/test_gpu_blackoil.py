@@ -278,17 +278,27 @@ def _filter_blocks(rp, ci, val, bstart):
     return frp, ci[keep].astype(np.int32), val[keep]
 
 
-@pytest.mark.parametrize("blocks", ["whole", "lines2", "lines6", "planes"])
+@pytest.mark.parametrize("stream", ["streamed", "registers"])
+@pytest.mark.parametrize("blocks", ["whole", "lines2", "lines6", "lines16", "planes"])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
-def test_cell_ilu_bitexact(gpu, oracle, monkeypatch, blocks, prec):
+def test_cell_ilu_bitexact(gpu, oracle, monkeypatch, blocks, prec, stream):
     """ILU(0) on a 3x3 block matrix with cell-aligned blocks runs cell
     schedules (three rows per lane, one level per cell wavefront): factors
     and apply bit-exact against the oracle's scalar ILU(0) of the
-    block-filtered matrix, in FP64 and FP32; and with fewer levels than the
-    row schedule of the same blocks."""
+    block-filtered matrix, in FP64 and FP32, with the level records
+    streamed through LDS rings and with the register pipeline; and with
+    fewer levels than the row schedule of the same blocks.  Line blocks (two
+    neighbour blocks per triangle, levels of up to 16 cells) and the 2-plane
+    blocks (three, up to 14 cells: 4,032 B of FP64 U records) stream; the
+    whole 11x7x4 block has levels of 28 cells, 8,064 B in FP64 (past the
+    rings' 4 KiB: registers) and 4,032 B in FP32 (streamed, with 16-byte
+    index copies)."""
     import torch
     from ressim_amd import _lib
-    pr = P.config3(nx=11, ny=7, nz=4)
+    if stream == "registers":
+        monkeypatch.setenv("RSGPU_ILU_CELL_STREAM", "0")
+    pr = P.config3(nx=11, ny=7, nz=4) if blocks != "lines16" else P.config3(nx=21, ny=16,
+                                                                            nz=3)
     rng = np.random.default_rng(12)
     u = pr.u.copy()
     u[0::3] += rng.uniform(-300, 300, pr.n)
@@ -333,6 +343,10 @@ def test_cell_ilu_bitexact(gpu, oracle, monkeypatch, blocks, prec):
         assert _lib.load().rsgpu_prec_apply_f32(M.h, bd.data_ptr(), z.data_ptr()) == 0
         M.ctx.synchronize()
         assert np.array_equal(z.cpu().numpy(), oracle.ilu_apply_f32(frp, fci, lu, dg, b32))
+    # which sweep kernel ran
+    form = M.info()["sweep_form"]
+    streams = stream == "streamed" and not (blocks == "whole" and prec == "f64")
+    assert form == ("cells_streamed" if streams else "cells"), form
     # the cell schedule has about a third of the row schedule's levels
     cells = M.info()
     monkeypatch.setenv("RSGPU_ILU_CELLS", "0")
