@@ -163,6 +163,10 @@ def main():
     ap.add_argument("--bo-lines", type=int, default=16,
                     help="x-lines per block-Jacobi ILU(0) block of the black-oil Jacobian")
     ap.add_argument("--bo-nz", type=int, default=50)
+    ap.add_argument("--c2-locked", action="store_true",
+                    help="the 128x128x64 system in its incompressible form (no accumulation, "
+                         "centre cell locked by zeroRows): FP64 and mixed solve times")
+    ap.add_argument("--locked-refinements", type=int, default=60)
     ap.add_argument("--impes", action="store_true",
                     help="measure the IMPES pressure-Jacobian assembly (control surfaces, "
                          "flux and diagonal parts, sources) on the 128x128x64 MPFA-O mesh "
@@ -198,6 +202,8 @@ def main():
         return bench_blackoil(a)
     if a.impes:
         return bench_impes(a)
+    if a.c2_locked:
+        return bench_c2_locked(a)
     lib = _lib.load()
     dev = torch.device("cuda:0")
     torch.cuda.set_device(0)
@@ -860,6 +866,103 @@ def bench_ensemble(a):
                      "kernel": "k_spmv<double,EpiMult> over the block-diagonal batch",
                      "algorithmic_bytes": b64},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_c2_locked(a):
+    """The 128x128x64 pressure system in its incompressible form: no
+    accumulation diagonal, the centre cell locked by Matrices.zeroRows
+    (LinearSolver.java:256-260).  It is near-singular (the oracle needs
+    18,228 GMRES(30) iterations to 1e-10).  One FP64 GMRES(30) + block
+    ILU(0) x256 solve to true relative residual --rtol (monitor calibrated
+    as in the main bench), and one mixed solve (FP32 GMRES(30) inner to
+    --inner-rtol, FP64 refinement on the true residual, at most
+    --locked-refinements refinements).  Prints one JSON line."""
+    import torch
+    from ressim_amd import _lib, mtj, problems as P
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(0)
+    pr = P.config2(nx=a.nx, ny=a.ny, nz=a.nz, accumulation=0.0)
+    pr.pinned = [pr.centre]
+    n = pr.n
+    ctx = mtj.Context.default()
+    stream = torch.cuda.Stream()          # default priority
+    ctx.set_stream(stream.cuda_stream)
+    A, fails = mtj.mpfa_o_assemble(pr.dim, pr.nx, pr.ny, pr.nz, pr.dx, pr.dy, pr.dz, pr.K6,
+                                   ctx=ctx)
+    mtj.Matrices.zeroRows(A, 1.0, pr.pinned)
+    b = torch.tensor(pr.rhs(), device=dev)
+    x = torch.zeros(n, dtype=torch.float64, device=dev)
+    log(f"locked-cell {pr.name}: n={n} nnz={A.nnz}, centre cell {pr.centre} locked")
+    M64 = mtj.BlockILU(a.nblocks, precision=_lib.FP64, ctx=ctx)
+    g64 = mtj.GMRES(b, 30)
+    g64.cgs = True
+    g64.setPreconditioner(M64)
+
+    def fp64(rtol):
+        g64.setIterationMonitor(mtj.DefaultIterationMonitor(200000, rtol, 1e-50, 1e5))
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        M64.setMatrix(A)
+        x.zero_()
+        g64.solve(A, b, x)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, g64.last_stats
+
+    mon = a.rtol
+    for _ in range(4):
+        dt64, st64 = fp64(mon)
+        ratio = (st64.true_residual / st64.b_norm) / a.rtol
+        log(f"fp64 GMRES(30)+BJ-ILU0 (monitor rtol {mon:.2e}): {dt64:.2f} s, "
+            f"{st64.iterations} iterations, true rel res "
+            f"{st64.true_residual / st64.b_norm:.3e}")
+        if ratio <= 1.0:
+            break
+        mon /= 1.25 * ratio
+    x64 = x.clone()
+    M32 = mtj.BlockILU(a.nblocks, precision=_lib.FP32, ctx=ctx)
+    gmx = mtj.MixedPrecisionSolver(b, inner="gmres", restart=30, inner_rtol=a.inner_rtol,
+                                   inner_max_iter=600, cgs=True)
+    gmx.setIterationMonitor(mtj.DefaultIterationMonitor(a.locked_refinements, a.rtol, 1e-50,
+                                                        1e5))
+    gmx.setPreconditioner(M32)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    A.sync_f32()
+    M32.setMatrix(A)
+    x.zero_()
+    converged = True
+    try:
+        gmx.solve(A, b, x)
+    except mtj.IterativeSolverNotConvergedException as e:
+        converged = False
+        log(f"mixed: not converged ({e})")
+    torch.cuda.synchronize()
+    dtmx = time.perf_counter() - t
+    stmx = gmx.last_stats
+    err = float(torch.linalg.norm(x - x64) / torch.linalg.norm(x64))
+    log(f"mixed FP32-GMRES/FP64-IR: {dtmx:.2f} s, {stmx.outer_iterations} refinements, "
+        f"{stmx.inner_iterations} inner iterations, true rel res "
+        f"{stmx.true_residual / stmx.b_norm:.3e}, vs fp64 {err:.2e}")
+    out = {
+        "metric": "128x128x64 incompressible (locked centre cell) pressure solves/s, "
+                  "GMRES(30) + block ILU(0) x256 to 1e-8",
+        "value": 1.0 / dt64, "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": dt64 * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded log-normal permeability, assembled on the GPU)",
+        "config": {"workload": f"C2 {pr.nx}x{pr.ny}x{pr.nz} MPFA-O, no accumulation, centre "
+                               f"cell locked, GMRES(30) + block ILU(0) x{a.nblocks}",
+                   "n": n, "nnz": A.nnz},
+        "fp64": {"s_per_solve": dt64, "iterations": st64.iterations,
+                 "true_rel_residual": st64.true_residual / st64.b_norm,
+                 "monitor_rtol": mon},
+        "mixed": {"s_per_solve": dtmx, "converged": converged,
+                  "refinements": stmx.outer_iterations,
+                  "inner_iterations": stmx.inner_iterations,
+                  "true_rel_residual": stmx.true_residual / stmx.b_norm,
+                  "rel_err_vs_fp64": err},
     }
     print(json.dumps(out), flush=True)
 
