@@ -88,15 +88,25 @@ struct EpiMultDot {
     const T *z2;    // nullptr -> second dot uses y itself
     int nd;
     double a1 = 0, a2 = 0;
-    __device__ T init(int) const { return T(0); }
+    T z1v = T(0), z2v = T(0);
+    // the dot operands of row r are read here, before the row's own loads,
+    // so their latency overlaps the row's instead of following it
+    __device__ T init(int r)
+    {
+        if (nd >= 1)
+            z1v = z1[r];
+        if (nd >= 2 && z2)
+            z2v = z2[r];
+        return T(0);
+    }
     __device__ void row(int r, T dot)
     {
         const T v = (dot != T(0)) ? dot : T(0);
         y[r] = v;
         if (nd >= 1)
-            a1 += (double)(z1[r] * v);
+            a1 += (double)(z1v * v);
         if (nd >= 2)
-            a2 += (double)(z2 ? z2[r] * v : v * v);
+            a2 += (double)(z2 ? z2v * v : v * v);
     }
 };
 
