@@ -81,10 +81,15 @@ FUSED_KERNEL = "k_gm_fused<float>"
 ORTHO64 = "DCGS2 (classical Gram-Schmidt, reorthogonalisation delayed by one step)"
 
 
-def measured_traffic(pr, kernel, profile=PMC_PROFILE):
+BO_PMC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                              "k_spmv_b3_blackoil_100x100x50_pmc.md")
+
+
+def measured_traffic(pr, kernel, profile=PMC_PROFILE, shape=(128, 128, 64)):
     """The kernel's `traffic` column (scaled FETCH_SIZE + WRITE_SIZE bytes
-    per dispatch) from a committed counter profile's table."""
-    if (pr.nx, pr.ny, pr.nz) != (128, 128, 64) or not os.path.exists(profile):
+    per dispatch) from a committed counter profile's table, for the grid
+    `shape` that profile was taken on."""
+    if (pr.nx, pr.ny, pr.nz) != shape or not os.path.exists(profile):
         return None
     col = None
     for line in open(profile):
@@ -1191,7 +1196,11 @@ def bench_blackoil(a):
         "spmv": {"kernels": tk, "bitwise_equal": same, "bsr_bytes": bsr_bytes,
                  "bsr32_bytes": bsr32_bytes, "sliced_bytes": sl_bytes, "csr_bytes": csr_bytes},
         "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": gbs / PEAK_HBM_GBS, "traffic": None,
+                     "frac": gbs / PEAK_HBM_GBS,
+                     "traffic": measured_traffic(
+                         pr, ("k_spmv_sliced<double, true>" if best.startswith("sliced")
+                              else "k_spmv<double, EpiMult, B3=true>"),
+                         BO_PMC_PROFILE, shape=(100, 100, 50)),
                      "kernel": ("k_spmv_sliced<double,true> (block form of the sliced copy)"
                                 if best.startswith("sliced")
                                 else "k_spmv<double,EpiMult,B3> (3x3 block rows)"),
