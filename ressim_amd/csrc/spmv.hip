@@ -235,55 +235,67 @@ void spmv_build_plans(const std::vector<int> &rowptr, int n, rsgpu_csr *A, hipSt
         sliced_build(A, s);
 }
 
-// sliced values of chunk c, slot s from the CSR values (zero where the row
-// lacks the slot); one row per thread
-template <typename T>
+// Sliced values from the CSR values (zero where a row lacks the slot), one
+// workgroup per chunk, one wave per 64 rows.  A wave's 64 rows are one
+// contiguous run of the CSR values (at most 32 entries per row, 16 KiB):
+// the wave stages the run in LDS with coalesced loads, then each lane walks
+// its row's slots and stores slot by slot, 64 consecutive values per store
+// across the wave.  Scalar form: slot s holds the row's next entry when
+// mask bit s is set.  Block form: slots 3k..3k+2 hold the next three
+// entries when block bit k is set.
+#define RS_FILL_ROW_MAX 32
+template <typename T, bool B3>
 __global__ void __launch_bounds__(256)
     k_sliced_fill(SlView S, const int *__restrict__ rowptr, const double *__restrict__ val,
                   T *__restrict__ out)
 {
-    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
-    const int c = (int)(g / RS_SL_ROWS), i = (int)(g % RS_SL_ROWS);
-    if (c >= S.nchunks)
-        return;
+    __shared__ double stage[4][64 * RS_FILL_ROW_MAX];
+    const int c = (int)blockIdx.x, wave = (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int i = wave * 64 + lane;                   // row within the chunk
+    const int r0 = c * RS_SL_ROWS + wave * 64;        // first row of the wave
     const int r = c * RS_SL_ROWS + i;
-    const int t = S.ctab[c];
-    const int W = S.tptr[t + 1] - S.tptr[t];
-    const unsigned m = r < S.n ? S.mask[r] : 0u;
-    int j = r < S.n ? rowptr[r] : 0;
-    T *o = out + (size_t)S.cbase[c] * RS_SL_ROWS + i;
-    for (int sl = 0; sl < W; ++sl) {
-        T v = T(0);
-        if ((m >> sl) & 1u)
-            v = (T)val[j++];
-        o[(size_t)sl * RS_SL_ROWS] = v;
+    double *st = stage[wave];
+    if (r0 < S.n) {
+        // eight loads per lane in flight per round (indices past the run
+        // clamp to its last entry; only the LDS store is predicated)
+        const int base = rowptr[r0], len = rowptr[min(r0 + 64, S.n)] - base;
+        for (int j0 = 0; j0 < len; j0 += 8 * 64) {
+            double v[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                v[q] = val[base + min(j0 + q * 64 + lane, len - 1)];
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                if (j0 + q * 64 + lane < len)
+                    st[j0 + q * 64 + lane] = v[q];
+        }
     }
-}
-
-// the block form's fill: row r's entries are its block row's blocks in
-// ascending block column, three scalars each
-template <typename T>
-__global__ void __launch_bounds__(256)
-    k_sliced_fill3(SlView S, const int *__restrict__ rowptr, const double *__restrict__ val,
-                   T *__restrict__ out)
-{
-    const long long g = (long long)blockIdx.x * 256 + threadIdx.x;
-    const int c = (int)(g / RS_SL_ROWS), i = (int)(g % RS_SL_ROWS);
-    if (c >= S.nchunks)
-        return;
-    const int r = c * RS_SL_ROWS + i;
+    __syncthreads();
     const int t = S.ctab[c];
     const int W = S.tptr[t + 1] - S.tptr[t];
-    const unsigned m = r < S.n ? S.mask16[r] : 0u;
-    int j = r < S.n ? rowptr[r] : 0;
     T *o = out + (size_t)S.cbase[c] * RS_SL_ROWS + i;
-    for (int k = 0; k < W; ++k) {
-        const bool has = (m >> k) & 1u;
-        for (int e = 0; e < 3; ++e) {
+    int j = 0;
+    unsigned m = 0;
+    if (r < S.n) {
+        j = rowptr[r] - rowptr[r0];
+        m = B3 ? (unsigned)S.mask16[r] : S.mask[r];
+    }
+    if (B3) {
+        for (int k = 0; k < W; ++k) {
+            const bool has = (m >> k) & 1u;
+            for (int e = 0; e < 3; ++e) {
+                T v = T(0);
+                if (has)
+                    v = (T)st[j++];
+                o[(size_t)(3 * k + e) * RS_SL_ROWS] = v;
+            }
+        }
+    } else {
+        for (int sl = 0; sl < W; ++sl) {
             T v = T(0);
-            if (has)
-                v = (T)val[j++];
-            o[(size_t)(3 * k + e) * RS_SL_ROWS] = v;
+            if ((m >> sl) & 1u)
+                v = (T)st[j++];
+            o[(size_t)sl * RS_SL_ROWS] = v;
         }
     }
 }
@@ -303,10 +315,9 @@ void sliced_fill(rsgpu_csr *A, hipStream_t s)
         return;
     DevBuf<T> &b = sliced_buf<T>(A);
     b.alloc((size_t)F.nslots);
-    const long long threads = (long long)F.nchunks * RS_SL_ROWS;
-    hipLaunchKernelGGL((F.b3 ? k_sliced_fill3<T> : k_sliced_fill<T>),
-                       dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, s, A->slview(),
-                       A->rowptr.p, A->val.p, b.p);
+    hipLaunchKernelGGL((F.b3 ? k_sliced_fill<T, true> : k_sliced_fill<T, false>),
+                       dim3((unsigned)F.nchunks), dim3(256), 0, s, A->slview(), A->rowptr.p,
+                       A->val.p, b.p);
     RS_CHECK_LAUNCH();
     if (sizeof(T) == 8)
         F.gen64 = A->val_gen;
