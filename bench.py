@@ -1035,6 +1035,35 @@ def bench_blackoil(a):
         log(f"probe spmv: block rows {pr.n}, block nnz {nnzb}, reps {a.spmv_reps} each of the "
             f"block-row core and the sliced block form, FP64 then FP32; copy 512 MiB x3")
         return
+    if a.probe == "ilu":
+        # kernel-only run for rocprofv3: the block-Jacobi ILU(0) apply (16
+        # x-lines of one plane per block) in FP64 and FP32, then the 512 MiB
+        # copy that calibrates FETCH_SIZE / WRITE_SIZE
+        bs = 3 * grid_blocks([pr], a.bo_lines * pr.nx)
+        J.sync_f32()
+        zin = torch.randn(n, dtype=torch.float64, device=dev)
+        z = torch.empty_like(zin)
+        z32in, z32 = zin.float(), torch.empty(n, dtype=torch.float32, device=dev)
+        for prec in (_lib.FP64, _lib.FP32):
+            M = mtj.BlockILU(1, precision=prec, ctx=ctx)
+            M.setBlocks(bs)
+            M.setMatrix(J)
+            torch.cuda.synchronize()
+            for _ in range(a.spmv_reps):
+                if prec == _lib.FP64:
+                    lib.rsgpu_prec_apply(M.h, zin.data_ptr(), z.data_ptr())
+                else:
+                    lib.rsgpu_prec_apply_f32(M.h, z32in.data_ptr(), z32.data_ptr())
+            torch.cuda.synchronize()
+            log(f"probe ilu: {'fp64' if prec == _lib.FP64 else 'fp32'} sweep form "
+                f"{M.info()['sweep_form']}, {len(bs) - 1} blocks, reps {a.spmv_reps}")
+        src = torch.ones(64 << 20, dtype=torch.float64, device=dev)     # 512 MiB
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        ctx.set_stream(None)
+        return
     # assembly kernel time
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
