@@ -962,12 +962,20 @@ __global__ void __launch_bounds__(64)
             const int I = (b0 + rows[m.roff + 3 * t]) / 3;
             const int *rec = crec + (size_t)RS_CREC * I;
             const int rp3 = rec[0], nb = rec[1], dI = rec[2], n3 = 3 * nb;
+            // a row's loads are issued together (indices past the row clamp
+            // to its last entry; only the LDS stores are predicated): a
+            // guarded load per slot waits for memory on its own
 #pragma unroll
-            for (int a = 0; a < 3; ++a)
+            for (int a = 0; a < 3; ++a) {
+                T row[RS_CQ];
+#pragma unroll
+                for (int q = 0; q < RS_CQ; ++q)
+                    row[q] = lu[rp3 + a * n3 + min(q, n3 - 1)];
 #pragma unroll
                 for (int q = 0; q < RS_CQ; ++q)
                     if (q < n3)
-                        RS_OWN(a, q) = lu[rp3 + a * n3 + q];
+                        RS_OWN(a, q) = row[q];
+            }
             bool bad[3] = {false, false, false};
             for (int s = 0; s < dI; ++s) {
                 const int rpJ = rec[4 + 4 * s], w = rec[5 + 4 * s];
