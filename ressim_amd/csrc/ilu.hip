@@ -2015,12 +2015,22 @@ __global__ void k_band_gather(int n, const int *__restrict__ pos, const T *__res
         return;
     const int *p = pos + (size_t)9 * r;
     T *o = out + (size_t)8 * (r + RS_BAND_PAD);
+    // every load issued, absent positions read entry 0 and are then
+    // replaced by zero (a guarded load per slot waits for memory on its own)
+    int q[9];
+    T v[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+        q[k] = p[k];
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+        v[k] = lu[max(q[k], 0)];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        o[k] = p[k] >= 0 ? lu[p[k]] : T(0);
-        o[4 + k] = p[5 + k] >= 0 ? lu[p[5 + k]] : T(0);
+        o[k] = q[k] >= 0 ? v[k] : T(0);
+        o[4 + k] = q[5 + k] >= 0 ? v[5 + k] : T(0);
     }
-    const T dg = lu[p[4]];
+    const T dg = v[4];
     out[(size_t)8 * ((size_t)n + 2 * RS_BAND_PAD) + r + RS_BAND_PAD] = dg;
     if (rd)
         rd[r + RS_BAND_PAD] = 1.0 / (double)dg;
@@ -2262,9 +2272,18 @@ __global__ void k_band_in(int n, const int *__restrict__ pos, const T *__restric
         return;
     const int *p = pos + (size_t)9 * r;
     T *o = rec + (size_t)9 * (r + RS_BAND_PAD);
+    // every load issued (absent positions read entry 0), then selected
+    int q[9];
+    T v[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k)
-        o[k] = p[k] >= 0 ? lu[p[k]] : T(0);
+        q[k] = p[k];
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+        v[k] = lu[max(q[k], 0)];
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+        o[k] = q[k] >= 0 ? v[k] : T(0);
 }
 
 template <typename T>
