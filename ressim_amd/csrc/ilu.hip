@@ -2286,19 +2286,44 @@ __global__ void k_band_in(int n, const int *__restrict__ pos, const T *__restric
         o[k] = q[k] >= 0 ? v[k] : T(0);
 }
 
+// The factor rows of a workgroup's 256 rows are one contiguous run of lu:
+// it is staged in LDS (coalesced reads), the rows' band values are written
+// into the stage, and the run is written back coalesced.  A run longer than
+// the stage (rows with more than 9 entries) is written in place.
 template <typename T>
-__global__ void k_band_out(int n, const int *__restrict__ pos, const T *__restrict__ rec,
-                           T *__restrict__ lu)
+__global__ void __launch_bounds__(256)
+    k_band_out(int n, const int *__restrict__ rowptr, const int *__restrict__ pos,
+               const T *__restrict__ rec, T *__restrict__ lu)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n)
-        return;
-    const int *p = pos + (size_t)9 * r;
-    const T *o = rec + (size_t)9 * (r + RS_BAND_PAD);
+    __shared__ T seg[256 * 9];
+    const int r0 = (int)blockIdx.x * 256, r1 = min(r0 + 256, n);
+    const int base = rowptr[r0], len = rowptr[r1] - base;
+    const bool staged = len <= 256 * 9;
+    const int r = r0 + (int)threadIdx.x;
+    if (staged) {
+        for (int j = threadIdx.x; j < len; j += 256)
+            seg[j] = lu[base + j];
+        __syncthreads();
+    }
+    if (r < n) {
+        const int *p = pos + (size_t)9 * r;
+        const T *o = rec + (size_t)9 * (r + RS_BAND_PAD);
 #pragma unroll
-    for (int k = 0; k < 9; ++k)
-        if (p[k] >= 0)
-            lu[p[k]] = o[k];
+        for (int k = 0; k < 9; ++k) {
+            const int q = p[k];
+            if (q >= 0) {
+                if (staged)
+                    seg[q - base] = o[k];
+                else
+                    lu[q] = o[k];
+            }
+        }
+    }
+    if (staged) {
+        __syncthreads();
+        for (int j = threadIdx.x; j < len; j += 256)
+            lu[base + j] = seg[j];
+    }
 }
 
 template <typename T>
@@ -2440,7 +2465,8 @@ static void launch_factor_band(rsgpu_prec *M, T *lu)
     hipLaunchKernelGGL(k_band_in<T>, dim3(g), dim3(256), 0, s, M->n, M->band_pos.p, lu, in.p);
     hipLaunchKernelGGL(k_ilu_factor_band<T>, dim3(M->nblocks), dim3(64), 0, s, M->bstart.p,
                        M->band_dims.p, in.p, out.p, M->status.p);
-    hipLaunchKernelGGL(k_band_out<T>, dim3(g), dim3(256), 0, s, M->n, M->band_pos.p, out.p, lu);
+    hipLaunchKernelGGL(k_band_out<T>, dim3(g), dim3(256), 0, s, M->n, M->rowptr.p,
+                       M->band_pos.p, out.p, lu);
     RS_CHECK_LAUNCH();
 }
 
